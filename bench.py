@@ -1,0 +1,229 @@
+#!/usr/bin/env python3
+"""Benchmark: horizontal diffusion + vertical advection, FP64, 256x256x80 on one MI355X.
+
+One step = one hdiff launch and one vadv launch over the 256x256x80 interior (hdiff halo 2, vadv halo 3),
+the plain FP64 drop-in path.  Inputs are synthetic (seeded SplitMix64, the distributions of the workload
+definition) and resident in HBM before timing.  Steps cycle over NSETS independent field sets so that no
+step finds its inputs in the 256 MB Infinity Cache left there by the previous one.
+
+Printed JSON (one line): value = interior points / step time (one combined hdiff+vadv update per point), the
+per-kernel figures for FP64 and the FP32 precision form (FP32 storage, FP64 compute), the roofline of the
+dominant kernel (vadv FP64, algorithmic bytes / mean launch time from hipEvents on the launch stream) and the
+CPU baseline (the oracle port, OpenMP, on this host) on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+NI, NJ, NK = 256, 256, 80
+P = NI * NJ * NK
+HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
+# algorithmic (compulsory) bytes per interior point: hdiff reads in, coeff and writes out; vadv reads
+# utens_stage, u_stage, wcon, u_pos, utens and writes utens_stage
+BYTES_PER_PT = {"hdiff": 3, "vadv": 6}
+SEED = 42
+
+
+def make_set(gt, dtype, s):
+    from gridtools_amd import inputs
+
+    seed = SEED + s
+    hb = gt.builder().type(dtype).dimensions(NI + 4, NJ + 4, NK).halos(2, 2, 0)
+    hd = {
+        "in": hb.uniform(seed, inputs.FIELD_IDS["in"], -1.0, 1.0).build(),
+        "coeff": hb.uniform(seed, inputs.FIELD_IDS["coeff"], 0.0, 0.25).build(),
+        "out": hb.value(0).build(),
+    }
+    vb = gt.builder().type(dtype).dimensions(NI + 6, NJ + 6, NK).halos(3, 3, 0)
+    va = {n: vb.uniform(seed, inputs.FIELD_IDS[n], -1.0, 1.0).build()
+          for n in ("utens_stage", "u_stage", "u_pos", "utens")}
+    va["wcon"] = gt.builder().type(dtype).dimensions(NI + 6, NJ + 6, NK + 1).halos(3, 3, 0) \
+        .uniform(seed, inputs.FIELD_IDS["wcon"], -0.5, 0.5).build()
+    return hd, va
+
+
+def grids(gt):
+    gh = gt.make_grid(gt.halo_descriptor.symmetric(2, NI + 4), gt.halo_descriptor.symmetric(2, NJ + 4), NK)
+    gv = gt.make_grid(gt.halo_descriptor.symmetric(3, NI + 6), gt.halo_descriptor.symmetric(3, NJ + 6), NK)
+    return gh, gv
+
+
+def timed_steps(gt, sets, precision, steps, warmup):
+    """Runs warmup + steps steps; returns (total ms of the timed steps, per-launch ms lists)."""
+    from gridtools_amd import stencils
+    from gridtools_amd.timing import Event, synchronize
+
+    gh, gv = grids(gt)
+
+    def launch(step, ev=None):
+        hd, va = sets[step % len(sets)]
+        if ev:
+            ev[0].record()
+        stencils.hdiff(hd["out"], hd["in"], hd["coeff"], gh, precision=precision)
+        if ev:
+            ev[1].record()
+        stencils.vadv(va["utens_stage"], va["u_stage"], va["wcon"], va["u_pos"], va["utens"], 0.15, gv,
+                      precision=precision)
+        if ev:
+            ev[2].record()
+
+    for s in range(warmup):
+        launch(s)
+    synchronize()
+    evs = [[Event(), Event(), Event()] for _ in range(steps)]
+    start, stop = Event(), Event()
+    synchronize()
+    start.record()
+    for s in range(steps):
+        launch(warmup + s, evs[s])
+    stop.record()
+    stop.synchronize()
+    synchronize()
+    total = start.elapsed_ms(stop)
+    hd_ms = [e[0].elapsed_ms(e[1]) for e in evs]
+    va_ms = [e[1].elapsed_ms(e[2]) for e in evs]
+    return total, hd_ms, va_ms
+
+
+def kernel_stats(name, ms_list, bytes_per_elem):
+    ms = statistics.mean(ms_list)
+    med = statistics.median(ms_list)
+    algo = BYTES_PER_PT[name] * bytes_per_elem * P
+    return {
+        "mean_us": round(ms * 1e3, 2),
+        "median_us": round(med * 1e3, 2),
+        "gpts": round(P / (ms * 1e-3) / 1e9, 2),
+        "algo_bytes": algo,
+        "achieved_gbs": round(algo / (ms * 1e-3) / 1e9, 1),
+        "hbm_frac": round(algo / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+    }
+
+
+def cpu_baseline(budget_s=12.0):
+    """The oracle port (OpenMP, same arithmetic) on this host, hdiff+vadv FP64 256x256x80, bounded sample."""
+    from gridtools_amd import inputs
+    from oracle import binding as ob
+
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    threads = max(1, min(threads, 16))
+    h = 3
+    shp = (NK, NJ + 2 * h, NI + 2 * h)
+    rnd = lambda fid, a, b, nk=NK: np.ascontiguousarray(  # noqa: E731
+        inputs.uniform((NI + 2 * h, NJ + 2 * h, nk), SEED, fid, a, b).transpose(2, 1, 0))
+    in_, co = rnd(1, -1, 1), rnd(2, 0, 0.25)
+    out = np.zeros(shp)
+    uts0, us, up, ut = (rnd(i, -1, 1) for i in (6, 3, 4, 5))
+    wc = rnd(7, -0.5, 0.5, NK + 1)
+    V = ob.View
+    times = []
+    t_end = time.perf_counter() + budget_s
+    reps = 0
+    while (reps < 3 or time.perf_counter() < t_end) and reps < 50:
+        uts = uts0.copy()
+        t0 = time.perf_counter()
+        ob.hdiff(V(out, h, h), V(in_, h, h), V(co, h, h), NI, NJ, NK, threads=threads)
+        ob.vadv(V(uts, h, h), V(us, h, h), V(wc, h, h), V(up, h, h), V(ut, h, h), 0.15, NI, NJ, NK, threads=threads)
+        times.append(time.perf_counter() - t0)
+        reps += 1
+    med = statistics.median(times)
+    return {
+        "value": round(P / med / 1e9, 4),
+        "unit": "Ggridpoint-updates/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{reps} reps of hdiff+vadv FP64 256x256x80 (median {med * 1e3:.1f} ms), oracle/gt_oracle.c "
+                  f"OpenMP port of cpu_ifirst, {threads} threads",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--sets", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 precision-form leg")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import gridtools_amd as gt
+
+    gt.init(0)
+    info = gt.device_info()
+    print(f"[bench] device {info['name']} {info['arch']} CUs={info['cu_count']}", file=sys.stderr, flush=True)
+
+    sets64 = [make_set(gt, np.float64, s) for s in range(args.sets)]
+    total, hd_ms, va_ms = timed_steps(gt, sets64, gt.NATIVE, args.steps, args.warmup)
+    print(f"[bench] fp64 {args.steps} steps: {total:.3f} ms", file=sys.stderr, flush=True)
+    ms_per_step = total / args.steps
+    hd64 = kernel_stats("hdiff", hd_ms, 8)
+    va64 = kernel_stats("vadv", va_ms, 8)
+    del sets64
+
+    fp32 = None
+    if not args.no_fp32:
+        sets32 = [make_set(gt, np.float32, s) for s in range(args.sets)]
+        total32, hd_ms32, va_ms32 = timed_steps(gt, sets32, gt.WIDEN, args.steps, args.warmup)
+        print(f"[bench] fp32 form {args.steps} steps: {total32:.3f} ms", file=sys.stderr, flush=True)
+        fp32 = {
+            "form": "FP32 storage, FP64 compute (pivots and divisions in FP64)",
+            "ms_per_step": round(total32 / args.steps, 5),
+            "gpts_per_s": round(P / (total32 / args.steps * 1e-3) / 1e9, 3),
+            "speedup_vs_fp64": round(total / total32, 3),
+            "hdiff": kernel_stats("hdiff", hd_ms32, 4),
+            "vadv": kernel_stats("vadv", va_ms32, 4),
+        }
+        del sets32
+
+    cpu = None
+    if not args.no_cpu:
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        cpu = cpu_baseline()
+
+    value = P / (ms_per_step * 1e-3)
+    dom = va64
+    line = {
+        "metric": "gridpoint-updates/s + % HBM roofline, hdiff & vadv FP64 vs FP32 @256x256x80",
+        "value": round(value, 1),
+        "unit": "gridpoint-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded SplitMix64: in,u_* ~ U(-1,1), coeff ~ U(0,0.25), wcon ~ U(-0.5,0.5), dtr 0.15)",
+        "config": {"workload": "hdiff + vadv FP64, 256x256x80 interior (halo 2 / 3), one launch each per step",
+                   "field_sets": args.sets, "parallelism": "single GPU"},
+        "kernels": {"hdiff_fp64": hd64, "vadv_fp64": va64},
+        "fp32_form": fp32,
+        "roofline": {
+            "kernel": "vadv FP64 (vadv_*_kernel<double,double>)",
+            "bound": "hbm",
+            "achieved": dom["achieved_gbs"],
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(dom["achieved_gbs"] / HBM_PEAK_GBS, 4),
+            "traffic": None,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

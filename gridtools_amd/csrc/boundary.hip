@@ -1,0 +1,339 @@
+// Boundary conditions and local halo handling for gfx950.
+//
+// The reference applies a boundary functor to the 26 halo regions of a 3-D box in one kernel whose grid is
+// sized by the largest region, so most blocks idle on thin faces (boundaries/apply_gpu.hpp:98-255).  Its
+// periodic halos come from GCL: 12 pack kernels per field into 27 message buffers, an MPI self-exchange on a
+// periodic process grid, and 12 unpack kernels (gcl/high_level/m_pack*.hpp, m_unpack*.hpp,
+// descriptors_manual_gpu.hpp:342-512).
+//
+// Here every call is one launch over exactly the halo points it touches: the host lists the selected regions
+// with their boxes and prefix offsets, and each thread finds its region by a scan over at most 26 entries,
+// then walks its point with i fastest.  All fields of a call share the launch.
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+
+namespace gt355 {
+    namespace {
+
+        constexpr int MAX_FIELDS = 4;
+
+        struct region {
+            int32_t lo[3];   // first index of the region (storage coordinates)
+            int32_t n[3];    // extent
+            int32_t dir[3];  // direction components (-1, 0, 1)
+            int64_t start;   // prefix offset in points
+        };
+
+        struct region_table {
+            region r[26];
+            int count;
+            int64_t total;
+        };
+
+        struct bc_args {
+            void *dst[MAX_FIELDS];
+            const void *src[MAX_FIELDS];
+            int nfields;
+            int64_t s[3];
+            int32_t begin[3], end[3];
+            int32_t mode[3];
+            double value;
+        };
+
+        __device__ __forceinline__ int find_region(const region_table &t, int64_t n) {
+            int lo = 0;
+#pragma unroll 1
+            for (int q = 1; q < t.count; ++q)
+                if (n >= t.r[q].start)
+                    lo = q;
+            return lo;
+        }
+
+        // kind 0 zero, 1 value, 2 copy, 3 halo fill (mode per dimension)
+        template <class T, int KIND>
+        __global__ void __launch_bounds__(256) bc_kernel(region_table t, bc_args a) {
+            for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < t.total;
+                 n += (int64_t)gridDim.x * blockDim.x) {
+                const region &r = t.r[find_region(t, n)];
+                int64_t loc = n - r.start;
+                int i = r.lo[0] + int(loc % r.n[0]);
+                int64_t q = loc / r.n[0];
+                int j = r.lo[1] + int(q % r.n[1]);
+                int k = r.lo[2] + int(q / r.n[1]);
+                int64_t p = i * a.s[0] + j * a.s[1] + k * a.s[2];
+                if (KIND == 3) {
+                    int c[3] = {i, j, k};
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        if (r.dir[d] == 0)
+                            continue;
+                        int len = a.end[d] - a.begin[d] + 1;
+                        if (a.mode[d] == 1)
+                            c[d] = r.dir[d] < 0 ? a.begin[d] : a.end[d];
+                        else
+                            c[d] = r.dir[d] < 0 ? c[d] + len : c[d] - len;
+                    }
+                    int64_t sp = c[0] * a.s[0] + c[1] * a.s[1] + c[2] * a.s[2];
+                    for (int f = 0; f < a.nfields; ++f) {
+                        T *dst = (T *)a.dst[f];
+                        dst[p] = dst[sp];
+                    }
+                } else {
+                    for (int f = 0; f < a.nfields; ++f) {
+                        T *dst = (T *)a.dst[f];
+                        if (KIND == 0)
+                            dst[p] = T(0);
+                        else if (KIND == 1)
+                            dst[p] = T(a.value);
+                        else
+                            dst[p] = ((const T *)a.src[f])[p];
+                    }
+                }
+            }
+        }
+
+        // pack (UNPACK = false): buffer <- field slab; unpack: field slab <- buffer.  Buffer order: region, field,
+        // then the slab with i fastest.
+        template <class T, bool UNPACK>
+        __global__ void __launch_bounds__(256) pack_kernel(region_table t, bc_args a, T *buf) {
+            for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < t.total;
+                 n += (int64_t)gridDim.x * blockDim.x) {
+                const region &r = t.r[find_region(t, n)];
+                int64_t loc = n - r.start;
+                int64_t vol = (int64_t)r.n[0] * r.n[1] * r.n[2];
+                int i = r.lo[0] + int(loc % r.n[0]);
+                int64_t q = loc / r.n[0];
+                int j = r.lo[1] + int(q % r.n[1]);
+                int k = r.lo[2] + int(q / r.n[1]);
+                int64_t p = i * a.s[0] + j * a.s[1] + k * a.s[2];
+                for (int f = 0; f < a.nfields; ++f) {
+                    T *fld = (T *)a.dst[f];
+                    int64_t b = r.start * a.nfields + f * vol + loc;
+                    if (UNPACK)
+                        fld[p] = buf[b];
+                    else
+                        buf[b] = fld[p];
+                }
+            }
+        }
+
+        // Bounds of direction component d along one dimension, halo_descriptor h = {minus, plus, begin, end,
+        // total} (common/halo_descriptor.hpp:72-120).  which: 0 = the halo region (what a BC or unpack writes),
+        // 1 = the interior slab a neighbour in that direction needs (what pack reads).
+        void dim_range(const int32_t h[5], int d, int which, int32_t &lo, int32_t &n) {
+            if (d == 0) {
+                lo = h[2];
+                n = h[3] - h[2] + 1;
+            } else if (which == 0) {
+                lo = d < 0 ? h[2] - h[0] : h[3] + 1;
+                n = d < 0 ? h[0] : h[1];
+            } else {
+                // the neighbour below (d = -1) fills its plus halo from our first `plus` interior points
+                lo = d < 0 ? h[2] : h[3] - h[0] + 1;
+                n = d < 0 ? h[1] : h[0];
+            }
+        }
+
+        int check_halo(const int32_t halo[3][5], const char *who) {
+            for (int d = 0; d < 3; ++d) {
+                const int32_t *h = halo[d];
+                GT355_CHECK(h[0] >= 0 && h[1] >= 0 && h[2] <= h[3] && h[2] >= h[0] && h[3] + h[1] < h[4],
+                    "%s: invalid halo descriptor in dim %d {%d,%d,%d,%d,%d}", who, d, h[0], h[1], h[2], h[3], h[4]);
+            }
+            return GT355_OK;
+        }
+
+        // which: 0 halo regions, 1 pack slabs.  mode (halo fill only): a direction is kept only if every nonzero
+        // component has mode != 0.
+        region_table make_table(const int32_t halo[3][5], uint32_t dir_mask, int which, const int32_t *mode) {
+            region_table t{};
+            int64_t start = 0;
+            for (int di = -1; di <= 1; ++di)
+                for (int dj = -1; dj <= 1; ++dj)
+                    for (int dk = -1; dk <= 1; ++dk) {
+                        if (di == 0 && dj == 0 && dk == 0)
+                            continue;
+                        int bit = (di + 1) * 9 + (dj + 1) * 3 + (dk + 1);
+                        if (!((dir_mask >> bit) & 1u))
+                            continue;
+                        int dd[3] = {di, dj, dk};
+                        if (mode) {
+                            bool ok = true;
+                            for (int d = 0; d < 3; ++d)
+                                if (dd[d] != 0 && mode[d] == 0)
+                                    ok = false;
+                            if (!ok)
+                                continue;
+                        }
+                        region r{};
+                        int64_t vol = 1;
+                        for (int d = 0; d < 3; ++d) {
+                            dim_range(halo[d], dd[d], which, r.lo[d], r.n[d]);
+                            r.dir[d] = dd[d];
+                            vol *= r.n[d];
+                        }
+                        if (vol == 0)
+                            continue;
+                        r.start = start;
+                        start += vol;
+                        t.r[t.count++] = r;
+                    }
+            t.total = start;
+            return t;
+        }
+
+        int fill_args(bc_args &a, int dtype, void *const *dst, void *const *src, int nfields, const int32_t halo[3][5],
+            const int64_t stride[3], const char *who) {
+            GT355_CHECK(dtype == GT355_F64 || dtype == GT355_F32, "%s: dtype %d", who, dtype);
+            GT355_CHECK(nfields >= 1 && nfields <= MAX_FIELDS, "%s: nfields %d (1..%d)", who, nfields, MAX_FIELDS);
+            GT355_CHECK(dst && stride, "%s: null argument", who);
+            GT355_CHECK(stride[0] == 1 && stride[1] >= halo[0][4] && stride[2] >= stride[1] * halo[1][4],
+                "%s: strides {%lld,%lld,%lld} do not fit the halo lengths", who, (long long)stride[0],
+                (long long)stride[1], (long long)stride[2]);
+            a = bc_args{};
+            a.nfields = nfields;
+            for (int f = 0; f < nfields; ++f) {
+                GT355_CHECK(dst[f] != nullptr, "%s: null field %d", who, f);
+                a.dst[f] = dst[f];
+                a.src[f] = src ? src[f] : nullptr;
+            }
+            for (int d = 0; d < 3; ++d) {
+                a.s[d] = stride[d];
+                a.begin[d] = halo[d][2];
+                a.end[d] = halo[d][3];
+            }
+            return GT355_OK;
+        }
+
+        int blocks_for(int64_t total) { return (int)std::min<int64_t>((total + 255) / 256, 16384); }
+    } // namespace
+} // namespace gt355
+
+using namespace gt355;
+
+extern "C" int gt355_bc_apply(int kind, int dtype, void *const *dst, void *const *src, int nfields, double value,
+    const int32_t halo[3][5], const int64_t stride[3], uint32_t dir_mask, void *stream) {
+    GT355_CHECK(halo != nullptr, "gt355_bc_apply: null halo");
+    int rc;
+    if ((rc = check_halo(halo, "gt355_bc_apply")))
+        return rc;
+    GT355_CHECK(kind >= 0 && kind <= 2, "gt355_bc_apply: kind %d (0 zero, 1 value, 2 copy)", kind);
+    bc_args a;
+    if ((rc = fill_args(a, dtype, dst, src, nfields, halo, stride, "gt355_bc_apply")))
+        return rc;
+    if (kind == 2) {
+        GT355_CHECK(src != nullptr, "gt355_bc_apply: copy needs source fields");
+        for (int f = 0; f < nfields; ++f)
+            GT355_CHECK(src[f] != nullptr, "gt355_bc_apply: null source field %d", f);
+    }
+    a.value = value;
+    region_table t = make_table(halo, dir_mask, 0, nullptr);
+    if (t.total == 0)
+        return GT355_OK;
+    hipStream_t s = (hipStream_t)stream;
+    int nb = blocks_for(t.total);
+#define GT355_BC_LAUNCH(T)                                                  \
+    switch (kind) {                                                         \
+    case 0: bc_kernel<T, 0><<<nb, 256, 0, s>>>(t, a); break;                \
+    case 1: bc_kernel<T, 1><<<nb, 256, 0, s>>>(t, a); break;                \
+    default: bc_kernel<T, 2><<<nb, 256, 0, s>>>(t, a); break;               \
+    }
+    if (dtype == GT355_F64) {
+        GT355_BC_LAUNCH(double)
+    } else {
+        GT355_BC_LAUNCH(float)
+    }
+#undef GT355_BC_LAUNCH
+    return check_launch("gt355_bc_apply");
+}
+
+extern "C" int gt355_halo_fill(int dtype, void *const *fields, int nfields, const int32_t halo[3][5],
+    const int64_t stride[3], const int32_t mode[3], void *stream) {
+    GT355_CHECK(halo != nullptr && mode != nullptr, "gt355_halo_fill: null argument");
+    int rc;
+    if ((rc = check_halo(halo, "gt355_halo_fill")))
+        return rc;
+    bc_args a;
+    if ((rc = fill_args(a, dtype, fields, nullptr, nfields, halo, stride, "gt355_halo_fill")))
+        return rc;
+    for (int d = 0; d < 3; ++d) {
+        GT355_CHECK(mode[d] >= 0 && mode[d] <= 2, "gt355_halo_fill: mode[%d] = %d", d, mode[d]);
+        int len = halo[d][3] - halo[d][2] + 1;
+        GT355_CHECK(mode[d] != 2 || (halo[d][0] <= len && halo[d][1] <= len),
+            "gt355_halo_fill: periodic halo wider than the interior in dim %d", d);
+        a.mode[d] = mode[d];
+    }
+    region_table t = make_table(halo, 0x7FFFFFFu, 0, mode);
+    if (t.total == 0)
+        return GT355_OK;
+    hipStream_t s = (hipStream_t)stream;
+    int nb = blocks_for(t.total);
+    if (dtype == GT355_F64)
+        bc_kernel<double, 3><<<nb, 256, 0, s>>>(t, a);
+    else
+        bc_kernel<float, 3><<<nb, 256, 0, s>>>(t, a);
+    return check_launch("gt355_halo_fill");
+}
+
+extern "C" int gt355_halo_buffer_size(int dtype, int nfields, const int32_t halo[3][5], uint32_t dir_mask,
+    size_t *bytes) {
+    GT355_CHECK(halo && bytes, "gt355_halo_buffer_size: null argument");
+    GT355_CHECK(dtype == GT355_F64 || dtype == GT355_F32, "gt355_halo_buffer_size: dtype %d", dtype);
+    GT355_CHECK(nfields >= 1 && nfields <= MAX_FIELDS, "gt355_halo_buffer_size: nfields %d", nfields);
+    int rc;
+    if ((rc = check_halo(halo, "gt355_halo_buffer_size")))
+        return rc;
+    region_table t = make_table(halo, dir_mask, 1, nullptr);
+    *bytes = (size_t)t.total * nfields * (dtype == GT355_F64 ? 8 : 4);
+    return GT355_OK;
+}
+
+static int halo_pack_impl(bool unpack, int dtype, void *const *fields, int nfields, const int32_t halo[3][5],
+    const int64_t stride[3], uint32_t dir_mask, void *buffer, void *stream) {
+    const char *who = unpack ? "gt355_halo_unpack" : "gt355_halo_pack";
+    GT355_CHECK(halo && buffer, "%s: null argument", who);
+    int rc;
+    if ((rc = check_halo(halo, who)))
+        return rc;
+    bc_args a;
+    if ((rc = fill_args(a, dtype, fields, nullptr, nfields, halo, stride, who)))
+        return rc;
+    // pack reads the slab a neighbour in direction d needs; unpack writes our halo on side d, and its segment
+    // sizes equal the pack sizes of direction -d, so a buffer received from the neighbour at d fits.
+    region_table t = make_table(halo, dir_mask, unpack ? 0 : 1, nullptr);
+    if (unpack) {
+        // the buffer was sized by gt355_halo_buffer_size (pack volumes); unpack must not read past it
+        region_table tp = make_table(halo, dir_mask, 1, nullptr);
+        GT355_CHECK(tp.total == t.total,
+            "%s: halo volumes (%lld points) differ from the pack volumes (%lld) for this direction mask; use a mask "
+            "closed under direction negation or symmetric halos",
+            who, (long long)t.total, (long long)tp.total);
+    }
+    if (t.total == 0)
+        return GT355_OK;
+    hipStream_t s = (hipStream_t)stream;
+    int nb = blocks_for(t.total);
+    if (dtype == GT355_F64) {
+        if (unpack)
+            pack_kernel<double, true><<<nb, 256, 0, s>>>(t, a, (double *)buffer);
+        else
+            pack_kernel<double, false><<<nb, 256, 0, s>>>(t, a, (double *)buffer);
+    } else {
+        if (unpack)
+            pack_kernel<float, true><<<nb, 256, 0, s>>>(t, a, (float *)buffer);
+        else
+            pack_kernel<float, false><<<nb, 256, 0, s>>>(t, a, (float *)buffer);
+    }
+    return check_launch(who);
+}
+
+extern "C" int gt355_halo_pack(int dtype, void *const *fields, int nfields, const int32_t halo[3][5],
+    const int64_t stride[3], uint32_t dir_mask, void *buffer, void *stream) {
+    return halo_pack_impl(false, dtype, fields, nfields, halo, stride, dir_mask, buffer, stream);
+}
+
+extern "C" int gt355_halo_unpack(int dtype, void *const *fields, int nfields, const int32_t halo[3][5],
+    const int64_t stride[3], uint32_t dir_mask, const void *buffer, void *stream) {
+    return halo_pack_impl(true, dtype, fields, nfields, halo, stride, dir_mask, const_cast<void *>(buffer), stream);
+}

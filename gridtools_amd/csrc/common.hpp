@@ -1,0 +1,67 @@
+// Shared helpers of libgt355: error reporting, field checks, launch utilities.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/gt355.h"
+
+namespace gt355 {
+
+    // Thread-local last error, read by gt355_last_error().
+    void set_error(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
+    const char *last_error();
+
+    // The device selected by gt355_init (or the current HIP device if gt355_init was never called).
+    int current_device();
+
+    // Number of XCDs the launch is spread over (workgroups are dispatched round-robin over XCDs).
+    int xcd_count();
+
+    inline int hip_fail(hipError_t e, const char *what) {
+        set_error("%s: %s", what, hipGetErrorString(e));
+        return GT355_EHIP;
+    }
+
+#define GT355_HIP(call)                                                   \
+    do {                                                                  \
+        hipError_t gt355_e_ = (call);                                     \
+        if (gt355_e_ != hipSuccess)                                       \
+            return ::gt355::hip_fail(gt355_e_, #call);                    \
+    } while (0)
+
+#define GT355_CHECK(cond, ...)                                            \
+    do {                                                                  \
+        if (!(cond)) {                                                    \
+            ::gt355::set_error(__VA_ARGS__);                              \
+            return GT355_EINVAL;                                          \
+        }                                                                 \
+    } while (0)
+
+    // Checks that field f is usable for offsets [ilo, ihi) x [jlo, jhi) x [klo, khi) from its origin.
+    int check_field(const gt355_field *f, const char *name, int dtype, int ilo, int ihi, int jlo, int jhi, int klo,
+        int khi);
+
+    // After a launch: report a launch-configuration error as a status.
+    inline int check_launch(const char *what) {
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess)
+            return hip_fail(e, what);
+        return GT355_OK;
+    }
+
+    // Maps a linear workgroup id so that consecutive logical ids run on the same XCD.  The hardware hands
+    // workgroup n to XCD n % nxcd; logical id = xcd * per + n / nxcd keeps neighbouring tiles (which share
+    // halo rows) in one L2.  Ids beyond the largest multiple of nxcd keep their natural position.
+    __device__ __forceinline__ int xcd_remap(int n, int total, int nxcd) {
+        int per = total / nxcd;
+        int full = per * nxcd;
+        if (n >= full || nxcd <= 1)
+            return n;
+        return (n % nxcd) * per + n / nxcd;
+    }
+
+} // namespace gt355

@@ -1,0 +1,240 @@
+// Horizontal diffusion and laplacian for gfx950.
+//
+// The reference runs hdiff as one multistage execute_parallel().ij_cached(lap, flx, fly) with four stages
+// (tests/regression/horizontal_diffusion.cpp:35-107); its gpu backend launches one 64x8 block per
+// (tile, k-level), adds halo warps for the stage extents and keeps lap/flx/fly in shared memory
+// (stencil/gpu/launch_kernel.hpp:82-166, gpu/ij_cache.hpp:26-47).
+//
+// Here one workgroup owns a TI x TJ tile and a chunk of KC consecutive k-levels:
+//   * the in tile with its 2-point rim is staged in LDS once per level (loaded one level ahead into
+//     registers, so the HBM loads of level k+1 overlap the arithmetic of level k);
+//   * lap over the tile plus a 1-point rim is an LDS ij-cache (the lap_function extent <-1,1,-1,1>);
+//   * flx/fly are evaluated where out needs them, straight from the lap and in caches, so the four
+//     stages cost two barriers per level instead of four;
+//   * coeff is prefetched one level ahead like in; out is written with one coalesced row store.
+// Operation order is exactly the functors' (no FP contraction: the library is built with
+// -ffp-contract=off), so FP64 results are bit-identical to an IEEE CPU evaluation.
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+#include "tuning.hpp"
+
+namespace gt355 {
+    namespace {
+
+        // Reference functor bodies (horizontal_diffusion.cpp:41-46, :56-60, :70-74, :86-90).
+        template <class CT>
+        __device__ __forceinline__ CT lap_f(CT c, CT ip, CT jp, CT im, CT jm) {
+            return CT(4) * c - (((ip + jp) + im) + jm);
+        }
+        template <class CT>
+        __device__ __forceinline__ CT flux_f(CT lap_p, CT lap_c, CT in_p, CT in_c) {
+            CT res = lap_p - lap_c;
+            return (res * (in_p - in_c) > CT(0)) ? CT(0) : res;
+        }
+
+        template <class ST, class CT, int TI, int TJ, int KC, int NT>
+        __global__ void __launch_bounds__(NT) hdiff_kernel(ST *__restrict__ out, const ST *__restrict__ in,
+            const ST *__restrict__ coeff, int ni, int nj, int nk, int64_t osj, int64_t osk, int64_t isj, int64_t isk,
+            int64_t csj, int64_t csk, int tiles_i, int tiles_j, int nchunks, int nxcd) {
+            constexpr int IW = TI + 4, IH = TJ + 4; // in tile with 2-point rim
+            constexpr int LW = TI + 2, LH = TJ + 2; // lap tile with 1-point rim
+            constexpr int IN_ITERS = (IW * IH + NT - 1) / NT;
+            constexpr int OUT_ITERS = (TI * TJ + NT - 1) / NT;
+            static_assert((TI * TJ) % NT == 0, "out tile must split evenly over the workgroup");
+
+            __shared__ CT s_in[IH][IW];
+            __shared__ CT s_lap[LH][LW];
+
+            const int total = tiles_i * tiles_j * nchunks;
+            const int id = xcd_remap(blockIdx.x, total, nxcd);
+            const int ti = id % tiles_i;
+            const int tj = (id / tiles_i) % tiles_j;
+            const int kc = id / (tiles_i * tiles_j);
+            const int i0 = ti * TI, j0 = tj * TJ;
+            const int k0 = kc * KC;
+            const int k1 = min(k0 + KC, nk);
+            const int tid = threadIdx.x;
+
+            // per-thread in-tile slots (rim included), clipped to the valid box [-2, n+2)
+            int in_off[IN_ITERS];
+            bool in_ok[IN_ITERS];
+#pragma unroll
+            for (int r = 0; r < IN_ITERS; ++r) {
+                int idx = tid + r * NT;
+                int x = idx % IW, y = idx / IW;
+                int gi = i0 + x - 2, gj = j0 + y - 2;
+                in_ok[r] = idx < IW * IH && gi < ni + 2 && gj < nj + 2;
+                in_off[r] = in_ok[r] ? gi + gj * (int)isj : 0;
+            }
+            int out_x[OUT_ITERS], out_y[OUT_ITERS];
+            bool out_ok[OUT_ITERS];
+#pragma unroll
+            for (int r = 0; r < OUT_ITERS; ++r) {
+                int idx = tid + r * NT;
+                out_x[r] = idx % TI;
+                out_y[r] = idx / TI;
+                out_ok[r] = idx < TI * TJ && i0 + out_x[r] < ni && j0 + out_y[r] < nj;
+            }
+
+            ST pin[IN_ITERS];
+            ST pco[OUT_ITERS];
+            const ST *in_t = in + i0 + (int64_t)j0 * isj;
+            const ST *co_t = coeff + i0 + (int64_t)j0 * csj;
+            auto prefetch = [&](int k) {
+#pragma unroll
+                for (int r = 0; r < IN_ITERS; ++r)
+                    pin[r] = in_ok[r] ? in_t[in_off[r] + k * isk] : ST(0);
+#pragma unroll
+                for (int r = 0; r < OUT_ITERS; ++r)
+                    pco[r] = out_ok[r] ? __builtin_nontemporal_load(co_t + out_x[r] + out_y[r] * csj + k * csk)
+                                       : ST(0);
+            };
+            if (k0 < k1)
+                prefetch(k0);
+
+            for (int k = k0; k < k1; ++k) {
+                // stage the prefetched in tile; keep this level's coeff in registers
+#pragma unroll
+                for (int r = 0; r < IN_ITERS; ++r) {
+                    int idx = tid + r * NT;
+                    if (idx < IW * IH)
+                        (&s_in[0][0])[idx] = static_cast<CT>(pin[r]);
+                }
+                CT cco[OUT_ITERS];
+#pragma unroll
+                for (int r = 0; r < OUT_ITERS; ++r)
+                    cco[r] = static_cast<CT>(pco[r]);
+                __syncthreads();
+                if (k + 1 < k1)
+                    prefetch(k + 1);
+
+                // lap over the tile plus a 1-point rim (lap_function, extent <-1,1,-1,1>)
+                for (int idx = tid; idx < LW * LH; idx += NT) {
+                    int x = idx % LW, y = idx / LW;
+                    int sx = x + 1, sy = y + 1;
+                    s_lap[y][x] = lap_f<CT>(
+                        s_in[sy][sx], s_in[sy][sx + 1], s_in[sy + 1][sx], s_in[sy][sx - 1], s_in[sy - 1][sx]);
+                }
+                __syncthreads();
+
+                // flx, fly and out per interior point
+#pragma unroll
+                for (int r = 0; r < OUT_ITERS; ++r) {
+                    int x = out_x[r], y = out_y[r];
+                    int lx = x + 1, ly = y + 1; // lap coords
+                    int sx = x + 2, sy = y + 2; // in coords
+                    CT c = s_in[sy][sx];
+                    CT lc = s_lap[ly][lx];
+                    CT flx_c = flux_f<CT>(s_lap[ly][lx + 1], lc, s_in[sy][sx + 1], c);
+                    CT flx_m = flux_f<CT>(lc, s_lap[ly][lx - 1], c, s_in[sy][sx - 1]);
+                    CT fly_c = flux_f<CT>(s_lap[ly + 1][lx], lc, s_in[sy + 1][sx], c);
+                    CT fly_m = flux_f<CT>(lc, s_lap[ly - 1][lx], c, s_in[sy - 1][sx]);
+                    CT res = c - cco[r] * (((flx_c - flx_m) + fly_c) - fly_m);
+                    if (out_ok[r])
+                        __builtin_nontemporal_store(static_cast<ST>(res),
+                            out + (i0 + x) + (int64_t)(j0 + y) * osj + (int64_t)k * osk);
+                }
+                __syncthreads();
+            }
+        }
+
+        // laplacian.cpp:20-29, one point per thread, one k-level per grid z
+        template <class ST, class CT>
+        __global__ void __launch_bounds__(256) lap_kernel(ST *__restrict__ out, const ST *__restrict__ in, int ni,
+            int nj, int64_t osj, int64_t osk, int64_t isj, int64_t isk) {
+            int i = blockIdx.x * 64 + (threadIdx.x & 63);
+            int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+            int k = blockIdx.z;
+            if (i >= ni || j >= nj)
+                return;
+            const ST *p = in + i + j * isj + (int64_t)k * isk;
+            CT r = lap_f<CT>(CT(p[0]), CT(p[1]), CT(p[isj]), CT(p[-1]), CT(p[-isj]));
+            out[i + j * osj + (int64_t)k * osk] = static_cast<ST>(r);
+        }
+
+        template <class ST, class CT, int TI, int TJ, int KC>
+        int launch_hdiff(const gt355_field *out, const gt355_field *in, const gt355_field *coeff, int ni, int nj,
+            int nk, hipStream_t stream) {
+            constexpr int NT = 256;
+            int tiles_i = (ni + TI - 1) / TI, tiles_j = (nj + TJ - 1) / TJ, nchunks = (nk + KC - 1) / KC;
+            int64_t total = (int64_t)tiles_i * tiles_j * nchunks;
+            GT355_CHECK(total < (1LL << 31), "gt355_hdiff: domain too large for one launch");
+            hdiff_kernel<ST, CT, TI, TJ, KC, NT><<<(int)total, NT, 0, stream>>>((ST *)out->origin,
+                (const ST *)in->origin, (const ST *)coeff->origin, ni, nj, nk, out->stride[1], out->stride[2],
+                in->stride[1], in->stride[2], coeff->stride[1], coeff->stride[2], tiles_i, tiles_j, nchunks,
+                xcd_count());
+            return check_launch("gt355_hdiff");
+        }
+
+        template <class ST, class CT>
+        int dispatch_hdiff(const gt355_field *out, const gt355_field *in, const gt355_field *coeff, int ni, int nj,
+            int nk, hipStream_t stream) {
+            hdiff_tile t = hdiff_tile_for(sizeof(ST), sizeof(CT), ni, nj, nk);
+#define GT355_HDIFF_CASE(TI, TJ, KC)                                                            \
+    if (t.ti == TI && t.tj == TJ && t.kc == KC)                                                 \
+        return launch_hdiff<ST, CT, TI, TJ, KC>(out, in, coeff, ni, nj, nk, stream);
+            GT355_HDIFF_TILES(GT355_HDIFF_CASE)
+#undef GT355_HDIFF_CASE
+            return launch_hdiff<ST, CT, 64, 8, 8>(out, in, coeff, ni, nj, nk, stream);
+        }
+    } // namespace
+} // namespace gt355
+
+using namespace gt355;
+
+extern "C" int gt355_hdiff(int precision, const gt355_field *out, const gt355_field *in, const gt355_field *coeff,
+    int ni, int nj, int nk, void *stream) {
+    GT355_CHECK(ni >= 0 && nj >= 0 && nk >= 0, "gt355_hdiff: negative size");
+    GT355_CHECK(out && in && coeff, "gt355_hdiff: null field");
+    if (ni == 0 || nj == 0 || nk == 0)
+        return GT355_OK;
+    int dtype = in->dtype;
+    GT355_CHECK(dtype == GT355_F64 || dtype == GT355_F32, "gt355_hdiff: dtype %d", dtype);
+    GT355_CHECK(precision == GT355_NATIVE || (precision == GT355_WIDEN && dtype == GT355_F32),
+        "gt355_hdiff: precision %d with dtype %d", precision, dtype);
+    int rc;
+    if ((rc = check_field(in, "in", dtype, -2, ni + 2, -2, nj + 2, 0, nk)))
+        return rc;
+    if ((rc = check_field(coeff, "coeff", dtype, 0, ni, 0, nj, 0, nk)))
+        return rc;
+    if ((rc = check_field(out, "out", dtype, 0, ni, 0, nj, 0, nk)))
+        return rc;
+    GT355_CHECK(out->origin != in->origin, "gt355_hdiff: out must not alias in");
+    hipStream_t s = (hipStream_t)stream;
+    if (dtype == GT355_F64)
+        return dispatch_hdiff<double, double>(out, in, coeff, ni, nj, nk, s);
+    if (precision == GT355_WIDEN)
+        return dispatch_hdiff<float, double>(out, in, coeff, ni, nj, nk, s);
+    return dispatch_hdiff<float, float>(out, in, coeff, ni, nj, nk, s);
+}
+
+extern "C" int gt355_laplacian(int precision, const gt355_field *out, const gt355_field *in, int ni, int nj, int nk,
+    void *stream) {
+    GT355_CHECK(ni >= 0 && nj >= 0 && nk >= 0, "gt355_laplacian: negative size");
+    GT355_CHECK(out && in, "gt355_laplacian: null field");
+    if (ni == 0 || nj == 0 || nk == 0)
+        return GT355_OK;
+    int dtype = in->dtype;
+    GT355_CHECK(dtype == GT355_F64 || dtype == GT355_F32, "gt355_laplacian: dtype %d", dtype);
+    GT355_CHECK(precision == GT355_NATIVE || (precision == GT355_WIDEN && dtype == GT355_F32),
+        "gt355_laplacian: precision %d with dtype %d", precision, dtype);
+    GT355_CHECK(nk <= 65535, "gt355_laplacian: nk %d exceeds the grid z limit", nk);
+    int rc;
+    if ((rc = check_field(in, "in", dtype, -1, ni + 1, -1, nj + 1, 0, nk)))
+        return rc;
+    if ((rc = check_field(out, "out", dtype, 0, ni, 0, nj, 0, nk)))
+        return rc;
+    dim3 grid((ni + 63) / 64, (nj + 3) / 4, nk);
+    hipStream_t s = (hipStream_t)stream;
+    if (dtype == GT355_F64)
+        lap_kernel<double, double><<<grid, 256, 0, s>>>((double *)out->origin, (const double *)in->origin, ni, nj,
+            out->stride[1], out->stride[2], in->stride[1], in->stride[2]);
+    else if (precision == GT355_WIDEN)
+        lap_kernel<float, double><<<grid, 256, 0, s>>>((float *)out->origin, (const float *)in->origin, ni, nj,
+            out->stride[1], out->stride[2], in->stride[1], in->stride[2]);
+    else
+        lap_kernel<float, float><<<grid, 256, 0, s>>>((float *)out->origin, (const float *)in->origin, ni, nj,
+            out->stride[1], out->stride[2], in->stride[1], in->stride[2]);
+    return check_launch("gt355_laplacian");
+}

@@ -1,0 +1,16 @@
+// Tile selection for the tiled kernels.  Defaults come from measurements on MI355X; gt355_tune_* (the
+// autotuner) can override them per problem shape.
+#include "tuning.hpp"
+
+namespace gt355 {
+
+    hdiff_tile hdiff_tile_for(int st_size, int ct_size, int ni, int nj, int nk) {
+        (void)st_size;
+        (void)ct_size;
+        (void)ni;
+        (void)nj;
+        (void)nk;
+        return {64, 8, 8};
+    }
+
+} // namespace gt355

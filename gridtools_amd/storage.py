@@ -1,0 +1,170 @@
+"""Device storage with the reference's gpu storage semantics.
+
+Mirrors ``storage::builder<Traits>.type<T>().dimensions(..).halos(..).initializer(f)/value(v).build()``
+(storage/builder.hpp:168-326) and ``data_store`` (storage/data_store.hpp:33-222) for 3-D fields:
+i-innermost layout with the i length padded to 128 bytes, first interior point 128-byte aligned
+(storage/gpu.hpp:34-105, storage/info.hpp:38-85, storage/data_store.hpp:63-72).  The layout arithmetic is
+done by the library (gt355_storage_layout / gt355_storage_base) so that Python, C++ and the kernels agree.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _abi
+
+_DTYPES = {np.dtype(np.float64): _abi.F64, np.dtype(np.float32): _abi.F32}
+
+
+def dtype_code(dtype) -> int:
+    dt = np.dtype(dtype)
+    if dt not in _DTYPES:
+        raise TypeError(f"unsupported storage type {dt}; use float64 or float32")
+    return _DTYPES[dt]
+
+
+class DataStore:
+    """A 3-D field in device memory (lengths include the halo)."""
+
+    def __init__(self, dtype, lengths, halos=(0, 0, 0), name=""):
+        self.dtype = np.dtype(dtype)
+        self.code = dtype_code(self.dtype)
+        self.lengths = tuple(int(x) for x in lengths)
+        self.halos = tuple(int(x) for x in halos)
+        self.name = name
+        if len(self.lengths) != 3 or len(self.halos) != 3:
+            raise ValueError("DataStore is three-dimensional")
+        lay = _abi.Layout()
+        _abi.call("gt355_storage_layout", self.dtype.itemsize, _abi.c_i64_3(*self.lengths),
+                  _abi.c_i64_3(*self.halos), ctypes.byref(lay))
+        self.layout = lay
+        self.strides = tuple(int(x) for x in lay.stride)
+        self.padded = tuple(int(x) for x in lay.padded)
+        raw = ctypes.c_void_p()
+        _abi.call("gt355_malloc", ctypes.c_size_t(lay.alloc_bytes), ctypes.byref(raw))
+        self._raw = raw
+        base = ctypes.c_void_p()
+        _abi.call("gt355_storage_base", ctypes.byref(lay), raw, ctypes.byref(base))
+        self.base = base.value  # address of storage index (0, 0, 0)
+        self.nelem = self.strides[2] * self.padded[2]  # elements from base covered by the layout
+
+    # ------------------------------------------------------------------ lifetime
+    def free(self):
+        if getattr(self, "_raw", None) is not None and self._raw.value:
+            _abi.lib().gt355_free(self._raw)
+            self._raw = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------------ addressing
+    def address(self, i, j, k):
+        """Device address of storage index (i, j, k)."""
+        off = i * self.strides[0] + j * self.strides[1] + k * self.strides[2]
+        return self.base + off * self.dtype.itemsize
+
+    def sid(self, origin=None):
+        """gt355_field whose origin is storage index ``origin`` (default: the first interior point), with the
+        allocation box as bounds -- the SID after core::shift_origin (stencil/core/backend.hpp:28-34)."""
+        o = self.halos if origin is None else tuple(origin)
+        f = _abi.Field()
+        f.origin = self.address(*o)
+        f.stride = (ctypes.c_int64 * 3)(*self.strides)
+        f.lower = (ctypes.c_int32 * 3)(*[-x for x in o])
+        f.upper = (ctypes.c_int32 * 3)(*[n - x for n, x in zip(self.lengths, o)])
+        f.dtype = self.code
+        return f
+
+    # ------------------------------------------------------------------ host mirror
+    def _host_buffer(self):
+        return np.zeros((self.padded[2], self.padded[1], self.strides[1]), dtype=self.dtype)
+
+    def to_numpy(self):
+        """Host copy of the whole field indexed [i, j, k] (the reference's host_view, data_store.hpp:140-146)."""
+        buf = self._host_buffer()
+        _abi.call("gt355_memcpy_d2h", buf.ctypes.data, ctypes.c_void_p(self.base), ctypes.c_size_t(buf.nbytes))
+        ni, nj, nk = self.lengths
+        return np.ascontiguousarray(buf[:nk, :nj, :ni].transpose(2, 1, 0))
+
+    def padded_numpy(self):
+        """Host copy in the device layout: array [k, j, i_padded] (for passing to CPU code with the same strides)."""
+        buf = self._host_buffer()
+        _abi.call("gt355_memcpy_d2h", buf.ctypes.data, ctypes.c_void_p(self.base), ctypes.c_size_t(buf.nbytes))
+        return buf
+
+    def from_numpy(self, arr):
+        """Uploads an [i, j, k] array of the field's lengths (storage_update_target, storage/gpu.hpp:83-90)."""
+        arr = np.asarray(arr, dtype=self.dtype)
+        if arr.shape != self.lengths:
+            raise ValueError(f"shape {arr.shape} != field lengths {self.lengths}")
+        buf = self._host_buffer()
+        ni, nj, nk = self.lengths
+        buf[:nk, :nj, :ni] = arr.transpose(2, 1, 0)
+        _abi.call("gt355_memcpy_h2d", ctypes.c_void_p(self.base), buf.ctypes.data, ctypes.c_size_t(buf.nbytes))
+        return self
+
+    def fill_uniform(self, seed, field_id, a, b, stream=None):
+        """Synthetic input: SplitMix64 -> U(a,b) over the whole field, halo included (see inputs.py)."""
+        _abi.call("gt355_fill_uniform", self.code, ctypes.c_void_p(self.base), _abi.c_i64_3(*self.lengths),
+                  _abi.c_i64_3(*self.strides), ctypes.c_uint64(seed), ctypes.c_uint64(field_id),
+                  ctypes.c_double(a), ctypes.c_double(b), stream)
+        return self
+
+    def fill_value(self, v):
+        return self.from_numpy(np.full(self.lengths, v, dtype=self.dtype))
+
+
+class builder:
+    """storage::builder mirror: builder().type(np.float64).dimensions(ni, nj, nk).halos(2, 2, 0).value(0).build()."""
+
+    def __init__(self):
+        self._dtype = np.float64
+        self._dims = None
+        self._halos = (0, 0, 0)
+        self._init = None
+        self._name = ""
+
+    def _copy(self, **kw):
+        b = builder()
+        b.__dict__.update(self.__dict__)
+        b.__dict__.update(kw)
+        return b
+
+    def type(self, dtype):
+        return self._copy(_dtype=dtype)
+
+    def dimensions(self, *dims):
+        return self._copy(_dims=tuple(dims))
+
+    def halos(self, *halos):
+        return self._copy(_halos=tuple(halos))
+
+    def name(self, name):
+        return self._copy(_name=name)
+
+    def value(self, v):
+        return self._copy(_init=("value", v))
+
+    def initializer(self, fn):
+        """fn(i, j, k) over index arrays of the full allocation, halo included (storage/builder.hpp:67-84)."""
+        return self._copy(_init=("fn", fn))
+
+    def uniform(self, seed, field_id, a, b):
+        return self._copy(_init=("uniform", (seed, field_id, a, b)))
+
+    def build(self):
+        if self._dims is None:
+            raise ValueError("dimensions() not set")
+        ds = DataStore(self._dtype, self._dims, self._halos, self._name)
+        kind = self._init[0] if self._init else "value"
+        if kind == "value":
+            ds.fill_value(self._init[1] if self._init else 0)
+        elif kind == "fn":
+            i, j, k = np.meshgrid(*[np.arange(n) for n in ds.lengths], indexing="ij")
+            ds.from_numpy(np.broadcast_to(self._init[1](i, j, k), ds.lengths))
+        else:
+            ds.fill_uniform(*self._init[1])
+            _abi.call("gt355_synchronize")
+        return ds

@@ -1,0 +1,177 @@
+/*
+ * gt355.h -- C ABI of libgt355.so, the MI355X (gfx950) stencil backend for the GridTools stencil API.
+ *
+ * Every entry point takes plain pointers, sizes and the PODs below, returns 0 on success or a
+ * negative gt355_status, and leaves a message for gt355_last_error() (thread-local) on failure.
+ * This mirrors the reference's error convention, where every HIP/CUDA failure becomes a
+ * std::runtime_error carrying the message (common/cuda_util.hpp:20-35): a C++ caller rethrows.
+ *
+ * Device memory is passed as raw device addresses.  Launches are asynchronous on the given HIP
+ * stream (NULL = the null stream), like the reference's cuda_util::launch (common/cuda_util.hpp:77-95).
+ * No entry point synchronises the device unless its name says so.
+ */
+#ifndef GT355_H
+#define GT355_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status / types */
+enum gt355_status {
+    GT355_OK = 0,
+    GT355_EINVAL = -1,   /* bad argument (null pointer, size, stride, bounds) */
+    GT355_EHIP = -2,     /* a HIP runtime call failed */
+    GT355_ENOMEM = -3,   /* device allocation failed */
+    GT355_EUNSUP = -4,   /* combination not supported (dtype / precision / size) */
+};
+
+enum gt355_dtype { GT355_F64 = 0, GT355_F32 = 1 };
+
+/* How a stencil computes (the backend's Deref policy, stencil/be_api.hpp:160-169 + frontend/cartesian/stage.hpp:53-63):
+ *  GT355_NATIVE : compute in the storage type (the reference's float_t instantiation, FP64 or FP32);
+ *  GT355_WIDEN  : FP32 storage, every load widened to FP64, all temporaries, k-caches, Thomas pivots and
+ *                 divisions in FP64, result narrowed once on store (the precision form). */
+enum gt355_precision { GT355_NATIVE = 0, GT355_WIDEN = 1 };
+
+/* A field as the backend sees it after core::shift_origin (stencil/core/backend.hpp:28-34): the SID
+ * triple of origin, strides and bounds (sid/concept.hpp:36-120).  origin is the device address of grid point
+ * (0,0,0); stride[0] must be 1 (i-innermost, storage/gpu.hpp:34-40); lower[d] <= 0 < upper[d] bound the
+ * offsets from the origin that may be touched (the allocation box).  Every entry point checks the offsets
+ * its stencil reads or writes against these bounds before launching. */
+typedef struct gt355_field {
+    void *origin;
+    int64_t stride[3];
+    int32_t lower[3];
+    int32_t upper[3];
+    int32_t dtype; /* enum gt355_dtype */
+    int32_t reserved;
+} gt355_field;
+
+/* ------------------------------------------------------------------ runtime */
+/* Select and initialise a device (the reference leaves this to the CUDA runtime; gt355 keeps per-device
+ * state: temporary pool, flush buffer, tuning table). */
+int gt355_init(int device);
+const char *gt355_last_error(void);
+const char *gt355_version(void);
+
+typedef struct gt355_device_info {
+    char name[64];
+    char arch[32];       /* gcnArchName, e.g. "gfx950:sramecc+:xnack-" */
+    int32_t cu_count;
+    int32_t xcd_count;   /* 8 on MI355X; derived from the CU count and arch */
+    int64_t hbm_bytes;
+    int64_t lds_per_cu;
+    int64_t l2_bytes;
+    int32_t clock_khz;
+    int32_t mem_clock_khz;
+    int32_t mem_bus_width;
+    int32_t wavefront;
+} gt355_device_info;
+int gt355_get_device_info(gt355_device_info *out);
+
+/* ------------------------------------------------------------------ storage (storage/gpu.hpp:69-105)
+ * gt355_storage_layout: the layout the reference's gpu storage traits give an N=3 field:
+ * i-innermost layout_map<2,1,0> (storage/gpu.hpp:34-40), innermost length padded to a multiple of
+ * 128/gcd(elem,128) elements (storage/info.hpp:38-59), strides from the padded lengths (info.hpp:61-85),
+ * and an allocation of length + alignment elements so that the first interior point (halo_i, halo_j, halo_k)
+ * can be placed on a 128-byte boundary (storage/data_store.hpp:63-72). */
+typedef struct gt355_layout {
+    int64_t lengths[3];   /* total lengths with halos */
+    int64_t halos[3];
+    int64_t padded[3];    /* padded lengths */
+    int64_t stride[3];    /* element strides (stride[0] == 1) */
+    int64_t elem_size;
+    int64_t alloc_bytes;  /* bytes to request from gt355_malloc */
+} gt355_layout;
+int gt355_storage_layout(int64_t elem_size, const int64_t lengths[3], const int64_t halos[3], gt355_layout *out);
+/* Given the raw allocation, the address of storage index (0,0,0) such that the first interior point is
+ * 128-byte aligned (storage/data_store.hpp:63-72). */
+int gt355_storage_base(const gt355_layout *layout, void *raw, void **base);
+
+int gt355_malloc(size_t bytes, void **ptr);   /* storage_allocate (storage/gpu.hpp:78-81) */
+int gt355_free(void *ptr);
+int gt355_memcpy_h2d(void *dst, const void *src, size_t bytes); /* storage_update_target (gpu.hpp:83-90) */
+int gt355_memcpy_d2h(void *dst, const void *src, size_t bytes); /* storage_update_host (gpu.hpp:92-99) */
+int gt355_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream);
+int gt355_memset(void *dst, int value, size_t bytes, void *stream);
+int gt355_synchronize(void);
+int gt355_stream_synchronize(void *stream);
+
+/* Synthetic inputs: counter-based SplitMix64 -> U(a,b) computed in FP64 (rounded once to FP32 for F32
+ * fields) over a logical box of lengths[3] points (i fastest, then j, then k), element n at
+ * base + i*stride[0] + j*stride[1] + k*stride[2]; n = i + lengths[0]*(j + lengths[1]*k). */
+int gt355_fill_uniform(int dtype, void *base, const int64_t lengths[3], const int64_t stride[3], uint64_t seed,
+    uint64_t field_id, double a, double b, void *stream);
+
+/* ------------------------------------------------------------------ stencils
+ * Horizontal diffusion (tests/regression/horizontal_diffusion.cpp:35-107): one launch fusing the
+ * lap, flx, fly and out stages of the execute_parallel().ij_cached(lap, flx, fly) multistage, the job
+ * of gpu::gridtools_backend_entry_point (stencil/gpu/entry_point.hpp:253-260) for this spec.
+ * Reads in over [-2, n+2) in i and j, coeff and writes out over [0, n). */
+int gt355_hdiff(int precision, const gt355_field *out, const gt355_field *in, const gt355_field *coeff, int ni, int nj,
+    int nk, void *stream);
+
+/* Laplacian (tests/regression/laplacian.cpp:20-38, run_single_stage): out = 4 in - (in[i+-1] + in[j+-1]). */
+int gt355_laplacian(int precision, const gt355_field *out, const gt355_field *in, int ni, int nj, int nk,
+    void *stream);
+
+/* Vertical advection (tests/regression/vertical_advection_dycore.cpp:32-150): forward Thomas sweep
+ * with k-caches ccol/dcol (flush) and u_stage (fill), then backward substitution with data_col, fused into
+ * one launch like the reference's multi_kernel (stencil/gpu/entry_point.hpp:48-59,107-124).
+ * utens_stage is read (forward) and overwritten (backward).  Reads wcon at i+1 and k+1; nk >= 2. */
+int gt355_vadv(int precision, const gt355_field *utens_stage, const gt355_field *u_stage, const gt355_field *wcon,
+    const gt355_field *u_pos, const gt355_field *utens, double dtr_stage, int ni, int nj, int nk, void *stream);
+
+/* ------------------------------------------------------------------ boundaries
+ * gt355_bc_apply: boundary::apply for the predefined functors (boundaries/boundary.hpp:57-79,
+ * apply_gpu.hpp:235-309): kind 0 zero_boundary (zero.hpp:29-55), 1 value_boundary (value.hpp:27-67),
+ * 2 copy_boundary dst <- src (copy.hpp:26-47).  Fields are passed as storage bases (storage index (0,0,0));
+ * halo[d] = {minus, plus, begin, end, total_length} as halo_descriptor (common/halo_descriptor.hpp:44-49).
+ * dir_mask bit (di+1)*9+(dj+1)*3+(dk+1) enables direction (di,dj,dk), the pre-evaluated predicate of
+ * apply_gpu.hpp:52-83.  nfields fields (1..4) get the same treatment; src is used by kind 2 only. */
+int gt355_bc_apply(int kind, int dtype, void *const *dst, void *const *src, int nfields, double value,
+    const int32_t halo[3][5], const int64_t stride[3], uint32_t dir_mask, void *stream);
+
+/* gt355_halo_fill: the single-rank result of gcl pack -> periodic self-exchange -> unpack
+ * (gcl/high_level/descriptors_manual_gpu.hpp:342-512, gcl/low_level/proc_grids_3D.hpp:83-88) fused with a
+ * zero-gradient boundary, in one launch per call.  mode[d]: 0 none, 1 zero-gradient, 2 periodic.
+ * Every filled point takes the value of the interior point whose coordinates are clamped (mode 1) or
+ * wrapped (mode 2) in each dimension where it lies in the halo. */
+int gt355_halo_fill(int dtype, void *const *fields, int nfields, const int32_t halo[3][5], const int64_t stride[3],
+    const int32_t mode[3], void *stream);
+
+/* gt355_halo_pack / gt355_halo_unpack: copy the 26 halo-source slabs of nfields fields into one contiguous
+ * buffer, ordered by direction then field, i fastest (the message layout of gcl m_pack*.hpp:15-112), and
+ * back into the halos.  Pack reads the interior slab adjacent to the face/edge/corner (the data a neighbour
+ * in that direction needs); unpack writes the halo slab on the opposite side.  gt355_halo_buffer_size gives
+ * the bytes for a direction mask. */
+int gt355_halo_buffer_size(int dtype, int nfields, const int32_t halo[3][5], uint32_t dir_mask, size_t *bytes);
+int gt355_halo_pack(int dtype, void *const *fields, int nfields, const int32_t halo[3][5], const int64_t stride[3],
+    uint32_t dir_mask, void *buffer, void *stream);
+int gt355_halo_unpack(int dtype, void *const *fields, int nfields, const int32_t halo[3][5],
+    const int64_t stride[3], uint32_t dir_mask, const void *buffer, void *stream);
+
+/* ------------------------------------------------------------------ streams and timing
+ * Replaces timer_cuda (common/timer/timer_cuda.hpp:21-54): hipEvent pairs on a stream.  Streams are created
+ * with default priority. */
+int gt355_stream_create(void **stream);
+int gt355_stream_destroy(void *stream);
+int gt355_event_create(void **event);
+int gt355_event_destroy(void *event);
+int gt355_event_record(void *event, void *stream);
+int gt355_event_synchronize(void *event);
+int gt355_event_elapsed_ms(void *start, void *stop, float *ms);
+
+/* ------------------------------------------------------------------ measurement */
+/* Writes (and reads back) a device buffer larger than the MI355X MALL so that the next kernel reads
+ * HBM, not Infinity Cache.  Used between timed repetitions, outside the timed region. */
+int gt355_mall_flush(void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

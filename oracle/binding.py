@@ -1,0 +1,136 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg,
+never by the gridtools_amd package.  Arrays are numpy arrays in the device layout [k, j, i_padded] (as
+returned by DataStore.padded_numpy()) or any C-contiguous [k, j, i] array; the origin of the interior and the
+strides are derived from the halo.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_PD = ctypes.c_ssize_t
+
+_SIGS = {
+    "gto_fill_uniform_f64": [_P, ctypes.c_int64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_double, ctypes.c_double],
+    "gto_hdiff_f64": [_P, _P, _P, _I, _I, _I, _PD, _PD, _I],
+    "gto_hdiff_f32": [_P, _P, _P, _I, _I, _I, _PD, _PD, _I],
+    "gto_hdiff_mixed": [_P, _P, _P, _I, _I, _I, _PD, _PD, _I],
+    "gto_lap_f64": [_P, _P, _I, _I, _I, _PD, _PD],
+    "gto_vadv_f64": [_P, _P, _P, _P, _P, ctypes.c_double, _I, _I, _I, _PD, _PD, _PD, _PD, _I],
+    "gto_vadv_f32": [_P, _P, _P, _P, _P, ctypes.c_float, _I, _I, _I, _PD, _PD, _PD, _PD, _I],
+    "gto_vadv_mixed": [_P, _P, _P, _P, _P, ctypes.c_float, _I, _I, _I, _PD, _PD, _PD, _PD, _I],
+    "gto_tridiagonal_f64": [_P, _P, _P, _P, _P, _I, _I, _I, _PD, _PD],
+    "gto_bc_f64": [_I, _P, _P, ctypes.c_double, _P, _PD, _PD, ctypes.c_uint32],
+    "gto_bc_f32": [_I, _P, _P, ctypes.c_float, _P, _PD, _PD, ctypes.c_uint32],
+    "gto_halo_fill_f64": [_P, _P, _PD, _PD, _P],
+    "gto_halo_fill_f32": [_P, _P, _PD, _PD, _P],
+}
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        h = ctypes.CDLL(LIB_PATH)
+        for name, args in _SIGS.items():
+            fn = getattr(h, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_int if name.startswith("gto_vadv") else None
+        _lib = h
+    return _lib
+
+
+class View:
+    """A [k, j, i] C-contiguous array with an interior origin at (hk, hj, hi)."""
+
+    def __init__(self, arr, halo_i, halo_j, halo_k=0):
+        if not arr.flags.c_contiguous:
+            raise ValueError("oracle arrays must be C-contiguous [k, j, i]")
+        self.arr = arr
+        self.sj = arr.shape[2]
+        self.sk = arr.shape[1] * arr.shape[2]
+        self.off = halo_k * self.sk + halo_j * self.sj + halo_i
+
+    @property
+    def ptr(self):
+        return self.arr.ctypes.data + self.off * self.arr.itemsize
+
+
+def _kind(arrs):
+    dt = {a.arr.dtype for a in arrs}
+    if len(dt) != 1:
+        raise TypeError("mixed dtypes")
+    return dt.pop()
+
+
+def hdiff(out: View, in_: View, coeff: View, ni, nj, nk, mode="native", threads=1):
+    """mode: native (compute in storage type) or mixed (FP32 storage, FP64 compute)."""
+    dt = _kind([out, in_, coeff])
+    assert out.sj == in_.sj == coeff.sj and out.sk == in_.sk == coeff.sk
+    fn = {("float64", "native"): "gto_hdiff_f64", ("float32", "native"): "gto_hdiff_f32",
+          ("float32", "mixed"): "gto_hdiff_mixed"}[(dt.name, mode)]
+    getattr(lib(), fn)(out.ptr, in_.ptr, coeff.ptr, ni, nj, nk, out.sj, out.sk, threads)
+
+
+def laplacian(out: View, in_: View, ni, nj, nk):
+    lib().gto_lap_f64(out.ptr, in_.ptr, ni, nj, nk, out.sj, out.sk)
+
+
+def vadv(uts: View, us: View, wc: View, up: View, ut: View, dtr, ni, nj, nk, mode="native", threads=1):
+    dt = _kind([uts, us, wc, up, ut])
+    fn = {("float64", "native"): "gto_vadv_f64", ("float32", "native"): "gto_vadv_f32",
+          ("float32", "mixed"): "gto_vadv_mixed"}[(dt.name, mode)]
+    rc = getattr(lib(), fn)(uts.ptr, us.ptr, wc.ptr, up.ptr, ut.ptr, dtr, ni, nj, nk, uts.sj, uts.sk, wc.sj, wc.sk,
+                            threads)
+    if rc != 0:
+        raise ValueError(f"{fn} rejected nk={nk}")
+
+
+def tridiagonal(out: View, inf: View, diag: View, sup: View, rhs: View, ni, nj, nk):
+    lib().gto_tridiagonal_f64(out.ptr, inf.ptr, diag.ptr, sup.ptr, rhs.ptr, ni, nj, nk, out.sj, out.sk)
+
+
+def _halo_arr(halos):
+    return np.ascontiguousarray(np.array(halos, dtype=np.int32).reshape(3, 5))
+
+
+def bc(kind, dst, src, value, halos, dir_mask):
+    """dst/src: full [k, j, i] arrays (storage index (0,0,0) at element 0)."""
+    h = _halo_arr(halos)
+    sj, sk = dst.shape[2], dst.shape[1] * dst.shape[2]
+    srcp = src.ctypes.data if src is not None else None
+    if dst.dtype == np.float64:
+        lib().gto_bc_f64(kind, dst.ctypes.data, srcp, value, h.ctypes.data, sj, sk, dir_mask)
+    else:
+        lib().gto_bc_f32(kind, dst.ctypes.data, srcp, value, h.ctypes.data, sj, sk, dir_mask)
+
+
+def halo_fill(f, halos, modes):
+    h = _halo_arr(halos)
+    m = np.ascontiguousarray(np.array(modes, dtype=np.int32))
+    sj, sk = f.shape[2], f.shape[1] * f.shape[2]
+    if f.dtype == np.float64:
+        lib().gto_halo_fill_f64(f.ctypes.data, h.ctypes.data, sj, sk, m.ctypes.data)
+    else:
+        lib().gto_halo_fill_f32(f.ctypes.data, h.ctypes.data, sj, sk, m.ctypes.data)
+
+
+def fill_uniform(n, seed, field_id, a, b):
+    out = np.empty(n, dtype=np.float64)
+    lib().gto_fill_uniform_f64(out.ctypes.data, n, seed, field_id, a, b)
+    return out

@@ -1,0 +1,205 @@
+/*
+ * gt_oracle.c -- CPU restatement of the reference hdiff / vadv / laplacian / tridiagonal /
+ * boundary-condition arithmetic.  TEST INFRASTRUCTURE ONLY (see gt_oracle.h): the checker
+ * for the HIP path and the "port" CPU baseline, never part of the product.
+ *
+ * Build: oracle/Makefile (gcc -O2 -fopenmp -ffp-contract=off; no -ffast-math, so every
+ * operation is a correctly rounded IEEE operation in source order).
+ */
+#include "gt_oracle.h"
+
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------------------------------
+ * Inputs.  Counter-based SplitMix64 (Steele, Lea, Flood, "Fast splittable pseudorandom number
+ * generators", OOPSLA 2014): element n of a field is mix(base + (n+1)*gamma) with
+ * base = seed ^ (field_id * 0xD1B54A32D192ED03); u = (x >> 11) * 2^-53; value = a + (b-a)*u.
+ * The reference fills every index of the allocation with its initializer
+ * (storage/builder.hpp:67-84); n runs over the logical box, i fastest, then j, then k.
+ * ------------------------------------------------------------------------------------------ */
+static uint64_t gto_mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+void gto_fill_uniform_f64(double *dst, int64_t n, uint64_t seed, uint64_t field_id, double a, double b) {
+    const uint64_t base = seed ^ (field_id * 0xD1B54A32D192ED03ULL);
+    const double w = b - a;
+    for (int64_t i = 0; i < n; ++i) {
+        uint64_t x = gto_mix64(base + (uint64_t)(i + 1) * 0x9E3779B97F4A7C15ULL);
+        double u = (double)(x >> 11) * 0x1.0p-53;
+        dst[i] = a + w * u;
+    }
+}
+
+/* ---------------------------------- horizontal diffusion ---------------------------------- */
+#define ST double
+#define CT double
+#define FN gto_hdiff_f64
+#include "hdiff_body.inc"
+#undef ST
+#undef CT
+#undef FN
+
+#define ST float
+#define CT float
+#define FN gto_hdiff_f32
+#include "hdiff_body.inc"
+#undef ST
+#undef CT
+#undef FN
+
+#define ST float
+#define CT double
+#define FN gto_hdiff_mixed
+#include "hdiff_body.inc"
+#undef ST
+#undef CT
+#undef FN
+
+/* ---------------------------------- vertical advection ---------------------------------- */
+#define ST double
+#define CT double
+#define FN gto_vadv_f64
+#include "vadv_body.inc"
+#undef ST
+#undef CT
+#undef FN
+
+#define ST float
+#define CT float
+#define FN gto_vadv_f32
+#include "vadv_body.inc"
+#undef ST
+#undef CT
+#undef FN
+
+#define ST float
+#define CT double
+#define FN gto_vadv_mixed
+#include "vadv_body.inc"
+#undef ST
+#undef CT
+#undef FN
+
+/* ---------------------------------- laplacian ----------------------------------
+ * tests/regression/laplacian.cpp:20-29: out = 4*in - (in[i+1] + in[j+1] + in[i-1] + in[j-1]). */
+void gto_lap_f64(double *out, const double *in, int ni, int nj, int nk, ptrdiff_t sj, ptrdiff_t sk) {
+    for (int k = 0; k < nk; ++k)
+        for (int j = 0; j < nj; ++j) {
+            const double *p = in + (ptrdiff_t)k * sk + (ptrdiff_t)j * sj;
+            double *o = out + (ptrdiff_t)k * sk + (ptrdiff_t)j * sj;
+            for (int i = 0; i < ni; ++i)
+                o[i] = 4 * p[i] - (((p[i + 1] + p[i + sj]) + p[i - 1]) + p[i - sj]);
+        }
+}
+
+/* ---------------------------------- tridiagonal ----------------------------------
+ * tests/regression/tridiagonal.cpp:33-68.  forward_thomas: first_level sup/=diag, rhs/=diag;
+ * modify<1,0>: sup = sup/(diag - sup[k-1]*inf), rhs = (rhs - inf*rhs[k-1])/(diag - sup[k-1]*inf).
+ * backward_thomas: last_level out = rhs; modify<0,-1>: out = rhs - sup*out[k+1]. */
+void gto_tridiagonal_f64(double *out, const double *inf, const double *diag, double *sup, double *rhs, int ni,
+    int nj, int nk, ptrdiff_t sj, ptrdiff_t sk) {
+    for (int j = 0; j < nj; ++j)
+        for (int i = 0; i < ni; ++i) {
+            const ptrdiff_t c = (ptrdiff_t)j * sj + i;
+            for (int k = 0; k < nk; ++k) {
+                const ptrdiff_t p = c + (ptrdiff_t)k * sk;
+                if (k == 0) {
+                    sup[p] = sup[p] / diag[p];
+                    rhs[p] = rhs[p] / diag[p];
+                } else {
+                    const ptrdiff_t m = p - sk;
+                    sup[p] = sup[p] / (diag[p] - sup[m] * inf[p]);
+                    rhs[p] = (rhs[p] - inf[p] * rhs[m]) / (diag[p] - sup[m] * inf[p]);
+                }
+            }
+            for (int k = nk - 1; k >= 0; --k) {
+                const ptrdiff_t p = c + (ptrdiff_t)k * sk;
+                out[p] = (k == nk - 1) ? rhs[p] : rhs[p] - sup[p] * out[p + sk];
+            }
+        }
+}
+
+/* ---------------------------------- boundary conditions ---------------------------------- */
+/* direction component of index x against halo_descriptor h = {minus, plus, begin, end, total}:
+ * -1 on [begin-minus, begin-1], 0 on [begin, end], +1 on [end+1, end+plus], 2 = untouched. */
+static int gto_dir(int x, const int h[5]) {
+    if (x >= h[2] && x <= h[3])
+        return 0;
+    if (x >= h[2] - h[0] && x < h[2])
+        return -1;
+    if (x > h[3] && x <= h[3] + h[1])
+        return 1;
+    return 2;
+}
+
+#define GTO_BC_BODY(T)                                                                                           \
+    for (int k = 0; k < halo[2][4]; ++k) {                                                                     \
+        int dk = gto_dir(k, halo[2]);                                                                          \
+        if (dk == 2)                                                                                           \
+            continue;                                                                                          \
+        for (int j = 0; j < halo[1][4]; ++j) {                                                                 \
+            int dj = gto_dir(j, halo[1]);                                                                      \
+            if (dj == 2)                                                                                       \
+                continue;                                                                                      \
+            for (int i = 0; i < halo[0][4]; ++i) {                                                             \
+                int di = gto_dir(i, halo[0]);                                                                  \
+                if (di == 2 || (di == 0 && dj == 0 && dk == 0))                                                \
+                    continue;                                                                                  \
+                int bit = (di + 1) * 9 + (dj + 1) * 3 + (dk + 1);                                              \
+                if (!((dir_mask >> bit) & 1u))                                                                 \
+                    continue;                                                                                  \
+                ptrdiff_t p = (ptrdiff_t)k * sk + (ptrdiff_t)j * sj + i;                                       \
+                dst[p] = kind == 0 ? (T)0 : kind == 1 ? value : src[p];                                        \
+            }                                                                                                  \
+        }                                                                                                      \
+    }
+
+void gto_bc_f64(int kind, double *dst, const double *src, double value, const int halo[3][5], ptrdiff_t sj,
+    ptrdiff_t sk, uint32_t dir_mask) {
+    GTO_BC_BODY(double)
+}
+
+void gto_bc_f32(int kind, float *dst, const float *src, float value, const int halo[3][5], ptrdiff_t sj,
+    ptrdiff_t sk, uint32_t dir_mask) {
+    GTO_BC_BODY(float)
+}
+
+/* maps x into [begin, end] for a dimension in whose halo x lies; returns -1 if x is not to be filled */
+static int gto_map(int x, const int h[5], int mode, int *in_halo) {
+    int d = gto_dir(x, h);
+    *in_halo = 0;
+    if (d == 0)
+        return x;
+    if (d == 2 || mode == 0)
+        return -1;
+    *in_halo = 1;
+    if (mode == 1)
+        return d < 0 ? h[2] : h[3];
+    int len = h[3] - h[2] + 1;
+    return d < 0 ? x + len : x - len;
+}
+
+#define GTO_FILL_BODY                                                                                            \
+    for (int k = 0; k < halo[2][4]; ++k)                                                                       \
+        for (int j = 0; j < halo[1][4]; ++j)                                                                   \
+            for (int i = 0; i < halo[0][4]; ++i) {                                                             \
+                int hi, hj, hk;                                                                                \
+                int si = gto_map(i, halo[0], mode[0], &hi);                                                    \
+                int sj_ = gto_map(j, halo[1], mode[1], &hj);                                                   \
+                int sk_ = gto_map(k, halo[2], mode[2], &hk);                                                   \
+                if (si < 0 || sj_ < 0 || sk_ < 0 || !(hi || hj || hk))                                         \
+                    continue;                                                                                  \
+                f[(ptrdiff_t)k * sk + (ptrdiff_t)j * sj + i] = f[(ptrdiff_t)sk_ * sk + (ptrdiff_t)sj_ * sj + si]; \
+            }
+
+void gto_halo_fill_f64(double *f, const int halo[3][5], ptrdiff_t sj, ptrdiff_t sk, const int mode[3]) {
+    GTO_FILL_BODY
+}
+
+void gto_halo_fill_f32(float *f, const int halo[3][5], ptrdiff_t sj, ptrdiff_t sk, const int mode[3]) {
+    GTO_FILL_BODY
+}

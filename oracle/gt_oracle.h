@@ -1,0 +1,86 @@
+/*
+ * gt_oracle.h -- CPU restatement of the GridTools hdiff / vadv / laplacian / boundary path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: it is loaded by tests/,
+ * by __graft_entry__.smoke() and by the cpu_baseline leg of bench.py.  The product
+ * (gridtools_amd, libgt355.so) never links, loads or calls it.
+ *
+ * Every function restates the arithmetic of the reference functor it names, in
+ * the same operation order, so that an IEEE build (-O2 -ffp-contract=off, no
+ * -ffast-math) produces the bits the reference cpu_ifirst / naive backends would.
+ * The reference itself cannot be built in this image (it needs Boost.Preprocessor,
+ * which is absent), so the restatement is pinned by the reference's own closed
+ * forms and known answers plus mathematical invariants (see tests/test_oracle_*.py).
+ *
+ * Field convention: a field is passed as a pointer to its interior origin
+ * (storage index (halo_i, halo_j, 0)) plus element strides sj, sk; the i stride is 1.
+ * Halo points are reached with negative offsets.
+ */
+#ifndef GT_ORACLE_H
+#define GT_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- inputs: counter-based SplitMix64 -> U(a,b) over a logical (i fastest, j, k) box ---- */
+void gto_fill_uniform_f64(double *dst, int64_t n, uint64_t seed, uint64_t field_id, double a, double b);
+
+/* ---- horizontal diffusion: tests/regression/horizontal_diffusion.cpp:35-107 ---- */
+void gto_hdiff_f64(double *out, const double *in, const double *coeff, int ni, int nj, int nk, ptrdiff_t sj,
+    ptrdiff_t sk, int nthreads);
+void gto_hdiff_f32(float *out, const float *in, const float *coeff, int ni, int nj, int nk, ptrdiff_t sj,
+    ptrdiff_t sk, int nthreads);
+/* FP32 storage, every load widened to FP64, result narrowed once on store */
+void gto_hdiff_mixed(float *out, const float *in, const float *coeff, int ni, int nj, int nk, ptrdiff_t sj,
+    ptrdiff_t sk, int nthreads);
+
+/* ---- laplacian: tests/regression/laplacian.cpp:20-29 ---- */
+void gto_lap_f64(double *out, const double *in, int ni, int nj, int nk, ptrdiff_t sj, ptrdiff_t sk);
+
+/* ---- vertical advection: tests/regression/vertical_advection_dycore.cpp:32-149 ----
+ * wcon may have its own strides (it is often allocated with nk+1 staggered levels).
+ * nk >= 2 is required (first and last level are distinct intervals).            */
+int gto_vadv_f64(double *utens_stage, const double *u_stage, const double *wcon, const double *u_pos,
+    const double *utens, double dtr_stage, int ni, int nj, int nk, ptrdiff_t sj, ptrdiff_t sk, ptrdiff_t wsj,
+    ptrdiff_t wsk, int nthreads);
+int gto_vadv_f32(float *utens_stage, const float *u_stage, const float *wcon, const float *u_pos,
+    const float *utens, float dtr_stage, int ni, int nj, int nk, ptrdiff_t sj, ptrdiff_t sk, ptrdiff_t wsj,
+    ptrdiff_t wsk, int nthreads);
+int gto_vadv_mixed(float *utens_stage, const float *u_stage, const float *wcon, const float *u_pos,
+    const float *utens, float dtr_stage, int ni, int nj, int nk, ptrdiff_t sj, ptrdiff_t sk, ptrdiff_t wsj,
+    ptrdiff_t wsk, int nthreads);
+
+/* ---- tridiagonal Thomas solve: tests/regression/tridiagonal.cpp:33-68 ----
+ * inf/diag/sup/rhs/out share one layout; sup and rhs are overwritten like the reference's inout accessors. */
+void gto_tridiagonal_f64(double *out, const double *inf, const double *diag, double *sup, double *rhs, int ni,
+    int nj, int nk, ptrdiff_t sj, ptrdiff_t sk);
+
+/* ---- boundary conditions: boundaries/apply.hpp:45-135 (26-direction loop), zero.hpp:29-55,
+ * value.hpp:27-67, copy.hpp:26-47 ----
+ * Here the field pointer is the STORAGE BASE (index (0,0,0)); halo[d] = {minus, plus, begin, end, total_length}
+ * exactly as common/halo_descriptor.hpp:44-49.  A point belongs to direction (di,dj,dk) with d = -1 on
+ * [begin-minus, begin-1], 0 on [begin, end], +1 on [end+1, end+plus] (halo_descriptor.hpp:72-120); points
+ * outside those ranges are never touched.  Direction bit (di+1)*9+(dj+1)*3+(dk+1) of dir_mask must be set
+ * (bit 13, the interior, is ignored).  kind: 0 zero, 1 value, 2 copy (dst <- src at the same point). */
+void gto_bc_f64(int kind, double *dst, const double *src, double value, const int halo[3][5], ptrdiff_t sj,
+    ptrdiff_t sk, uint32_t dir_mask);
+void gto_bc_f32(int kind, float *dst, const float *src, float value, const int halo[3][5], ptrdiff_t sj,
+    ptrdiff_t sk, uint32_t dir_mask);
+
+/* ---- local halo fill (the single-rank effect of gcl halo pack + exchange + unpack with a periodic
+ * process grid, gcl/low_level/proc_grids_3D.hpp:83-88, plus a zero-gradient boundary) ----
+ * mode[d]: 0 none, 1 zero-gradient (clamp to [begin,end]), 2 periodic (wrap by end-begin+1).
+ * A point is filled iff it is in the halo of at least one dimension and every dimension in whose halo it
+ * lies has a mode != 0; it receives the value at the point whose coordinates are all mapped into the
+ * interior, so corners are (wrap/clamp i, wrap/clamp j, wrap/clamp k) and the result is order-free. */
+void gto_halo_fill_f64(double *f, const int halo[3][5], ptrdiff_t sj, ptrdiff_t sk, const int mode[3]);
+void gto_halo_fill_f32(float *f, const int halo[3][5], ptrdiff_t sj, ptrdiff_t sk, const int mode[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,41 @@
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs through libgt355.so on the device)")
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    from oracle import binding
+
+    binding.lib()
+    return binding
+
+
+@pytest.fixture(scope="session")
+def gpu():
+    """Initialises device 0 through the C ABI; a GPU test fails (not skips) if that is impossible."""
+    import gridtools_amd as gt
+
+    gt.init(0)
+    info = gt.device_info()
+    print(f"\n[gpu] {info['name']} {info['arch']} CUs={info['cu_count']}", flush=True)
+    return gt
+
+
+def rel_err(a, b):
+    """max |a-b| / max |b| over finite entries."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    d = np.max(np.abs(a - b)) if a.size else 0.0
+    m = np.max(np.abs(b)) if b.size else 0.0
+    return d / m if m > 0 else d

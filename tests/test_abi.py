@@ -1,0 +1,92 @@
+"""CPU checks of the C ABI: the library loads, exports every entry point include/gt355.h declares, and the
+pure host-side entry points (layout arithmetic, argument validation) behave like the reference storage."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from gridtools_amd import _abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", fn)).read()
+            txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+            names |= set(re.findall(r"\b(gt355_\w+)\s*\(", txt))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _abi.lib()
+    names = declared_functions()
+    assert len(names) > 20
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+    # the Python binding declares a signature for every entry point
+    assert names <= set(_abi.SIGNATURES), sorted(names - set(_abi.SIGNATURES))
+
+
+def test_version_string():
+    assert b"gfx950" in _abi.lib().gt355_version()
+
+
+@pytest.mark.parametrize("elem,lengths,halos,padded_i", [
+    (8, (262, 262, 80), (3, 3, 0), 272),   # vadv FP64 256^2 + 3-halo: 262 -> 272 (multiple of 16 doubles)
+    (8, (68, 68, 60), (2, 2, 0), 80),
+    (4, (1542, 1542, 80), (3, 3, 0), 1568),  # FP32: multiple of 32 floats
+    (4, (32, 5, 3), (0, 0, 0), 32),
+])
+def test_storage_layout_matches_reference_padding(elem, lengths, halos, padded_i):
+    """storage/info.hpp:38-85: i padded to 128/gcd(sizeof T,128) elements; layout_map<2,1,0> strides."""
+    lay = _abi.Layout()
+    _abi.call("gt355_storage_layout", elem, _abi.c_i64_3(*lengths), _abi.c_i64_3(*halos), ctypes.byref(lay))
+    assert lay.padded[0] == padded_i
+    assert tuple(lay.stride) == (1, padded_i, padded_i * lengths[1])
+    assert lay.alloc_bytes == (padded_i * lengths[1] * lengths[2] + 128 // elem) * elem
+
+
+def test_storage_base_aligns_first_interior_point():
+    """storage/data_store.hpp:63-72: the first interior point lands on a 128-byte boundary."""
+    lay = _abi.Layout()
+    _abi.call("gt355_storage_layout", 8, _abi.c_i64_3(262, 262, 80), _abi.c_i64_3(3, 3, 0), ctypes.byref(lay))
+    for raw in (0x10000, 0x10008, 0x10078, 0x123458):
+        base = ctypes.c_void_p()
+        _abi.call("gt355_storage_base", ctypes.byref(lay), ctypes.c_void_p(raw), ctypes.byref(base))
+        first = base.value + (3 + 3 * lay.stride[1]) * 8
+        assert first % 128 == 0
+        assert raw <= base.value < raw + 128
+
+
+def test_bad_arguments_are_rejected_without_a_device():
+    lib = _abi.lib()
+    lay = _abi.Layout()
+    assert lib.gt355_storage_layout(3, _abi.c_i64_3(4, 4, 4), _abi.c_i64_3(0, 0, 0), ctypes.byref(lay)) != 0
+    assert "element size" in _abi.last_error()
+    assert lib.gt355_storage_layout(8, _abi.c_i64_3(4, 4, 4), _abi.c_i64_3(2, 0, 0), ctypes.byref(lay)) != 0
+    # stencil entry points validate before touching the device
+    f = _abi.Field()
+    assert lib.gt355_hdiff(0, None, ctypes.byref(f), ctypes.byref(f), 4, 4, 4, None) != 0
+    assert lib.gt355_vadv(0, ctypes.byref(f), ctypes.byref(f), ctypes.byref(f), ctypes.byref(f), ctypes.byref(f),
+                          ctypes.c_double(0.15), 4, 4, 1, None) != 0
+    assert "nk = 1" in _abi.last_error()
+
+
+def test_hdiff_bounds_check_rejects_small_halo():
+    """A field whose allocation cannot hold the 2-point rim hdiff reads is rejected before launch (the role of
+    the reference's debug field-bounds check, stencil/frontend/run.hpp:215-232)."""
+    lib = _abi.lib()
+    f = _abi.Field()
+    f.origin = 0x100000
+    f.stride = (ctypes.c_int64 * 3)(1, 80, 80 * 68)
+    f.lower = (ctypes.c_int32 * 3)(-1, -1, 0)  # only a 1-point halo
+    f.upper = (ctypes.c_int32 * 3)(65, 65, 60)
+    f.dtype = _abi.F64
+    rc = lib.gt355_hdiff(0, ctypes.byref(f), ctypes.byref(f), ctypes.byref(f), 64, 64, 60, None)
+    assert rc == -1
+    assert "outside bounds" in _abi.last_error()

@@ -1,0 +1,187 @@
+"""GPU parity of the hot-path stencils: libgt355 (through the C ABI) against the CPU oracle on the same seeded
+inputs.  FP64 and native FP32 must agree bit for bit (both sides evaluate the functors' operation order with
+IEEE operations and no contraction); the bar written into every test is the north-star tolerance: relative
+error <= 1e-12 for FP64.  The precision form (FP32 storage, FP64 compute) must equal the oracle's mixed variant
+bit for bit and stay within 1e-5 relative of the FP64 oracle fed the same FP32-rounded inputs.
+
+Sizes: the reference's typed-suite sizes (12x33x61, 23x11x43: tests/include/test_environment.hpp:69-77), the
+hdiff CPU config (64x64x60, halo 2), the 256x256x80 benchmark size, and block-remainder shapes."""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+MIXED_TOL = 1e-5
+SEED = 42
+
+HDIFF_SIZES = [(12, 33, 61), (23, 11, 43), (64, 64, 60), (1, 1, 1), (65, 9, 3), (129, 70, 17), (256, 256, 80)]
+VADV_SIZES = [(12, 33, 61), (23, 11, 43), (1, 1, 2), (65, 9, 3), (70, 130, 7), (256, 256, 80)]
+
+
+def hdiff_case(gt, ni, nj, nk, dtype, h=2):
+    from gridtools_amd import inputs
+
+    b = gt.builder().type(dtype).dimensions(ni + 2 * h, nj + 2 * h, nk).halos(h, h, 0)
+    inp = b.uniform(SEED, inputs.FIELD_IDS["in"], -1.0, 1.0).build()
+    coeff = b.uniform(SEED, inputs.FIELD_IDS["coeff"], 0.0, 0.25).build()
+    out = b.value(0).build()
+    grid = gt.make_grid(gt.halo_descriptor.symmetric(h, ni + 2 * h), gt.halo_descriptor.symmetric(h, nj + 2 * h), nk)
+    return inp, coeff, out, grid
+
+
+def oracle_hdiff(oracle, inp, coeff, h, ni, nj, nk, mode):
+    a_in, a_co = inp.padded_numpy(), coeff.padded_numpy()
+    if mode == "wide":
+        a_in, a_co, mode = a_in.astype(np.float64), a_co.astype(np.float64), "native"
+    res = np.zeros_like(a_in)
+    V = oracle.View
+    oracle.hdiff(V(res, h, h), V(a_in, h, h), V(a_co, h, h), ni, nj, nk, mode=mode, threads=8)
+    return res[:nk, h:h + nj, h:h + ni]
+
+
+@pytest.mark.parametrize("ni,nj,nk", HDIFF_SIZES)
+def test_hdiff_fp64_parity(gpu, oracle, ni, nj, nk):
+    from gridtools_amd import stencils
+
+    print(f"[case] hdiff fp64 {ni}x{nj}x{nk}", flush=True)
+    inp, coeff, out, grid = hdiff_case(gpu, ni, nj, nk, np.float64)
+    stencils.hdiff(out, inp, coeff, grid)
+    got = out.padded_numpy()[:nk, 2:2 + nj, 2:2 + ni]
+    ref = oracle_hdiff(oracle, inp, coeff, 2, ni, nj, nk, "native")
+    assert rel_err(got, ref) <= FP64_TOL
+    assert np.array_equal(got, ref), "FP64 path is expected to be bit-identical to the oracle"
+    # the halo of out is untouched
+    full = out.padded_numpy()
+    assert np.all(full[:nk, :2, :] == 0) and np.all(full[:nk, :, :2] == 0)
+
+
+@pytest.mark.parametrize("ni,nj,nk", [(23, 11, 43), (64, 64, 60), (256, 256, 80)])
+def test_hdiff_fp32_native_parity(gpu, oracle, ni, nj, nk):
+    from gridtools_amd import stencils
+
+    print(f"[case] hdiff fp32 {ni}x{nj}x{nk}", flush=True)
+    inp, coeff, out, grid = hdiff_case(gpu, ni, nj, nk, np.float32)
+    stencils.hdiff(out, inp, coeff, grid, precision=stencils.NATIVE)
+    got = out.padded_numpy()[:nk, 2:2 + nj, 2:2 + ni]
+    ref = oracle_hdiff(oracle, inp, coeff, 2, ni, nj, nk, "native")
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("ni,nj,nk", [(23, 11, 43), (256, 256, 80)])
+def test_hdiff_precision_form(gpu, oracle, ni, nj, nk):
+    from gridtools_amd import stencils
+
+    print(f"[case] hdiff widen {ni}x{nj}x{nk}", flush=True)
+    inp, coeff, out, grid = hdiff_case(gpu, ni, nj, nk, np.float32)
+    stencils.hdiff(out, inp, coeff, grid, precision=stencils.WIDEN)
+    got = out.padded_numpy()[:nk, 2:2 + nj, 2:2 + ni]
+    assert np.array_equal(got, oracle_hdiff(oracle, inp, coeff, 2, ni, nj, nk, "mixed"))
+    ref64 = oracle_hdiff(oracle, inp, coeff, 2, ni, nj, nk, "wide")
+    assert rel_err(got, ref64) <= MIXED_TOL
+
+
+def vadv_case(gt, ni, nj, nk, dtype, h=3):
+    from gridtools_amd import inputs
+
+    b = gt.builder().type(dtype).dimensions(ni + 2 * h, nj + 2 * h, nk).halos(h, h, 0)
+    f = {}
+    for name in ("utens_stage", "u_stage", "u_pos", "utens"):
+        f[name] = b.uniform(SEED, inputs.FIELD_IDS[name], -1.0, 1.0).build()
+    # wcon lives on nk+1 staggered levels
+    f["wcon"] = gt.builder().type(dtype).dimensions(ni + 2 * h, nj + 2 * h, nk + 1).halos(h, h, 0) \
+        .uniform(SEED, inputs.FIELD_IDS["wcon"], -0.5, 0.5).build()
+    grid = gt.make_grid(gt.halo_descriptor.symmetric(h, ni + 2 * h), gt.halo_descriptor.symmetric(h, nj + 2 * h), nk)
+    return f, grid
+
+
+def oracle_vadv(oracle, f, dtr, h, ni, nj, nk, mode):
+    a = {k: v.padded_numpy() for k, v in f.items()}
+    if mode == "wide":
+        a = {k: v.astype(np.float64) for k, v in a.items()}
+        mode = "native"
+        dtr = float(np.float32(dtr))
+    V = oracle.View
+    res = a["utens_stage"].copy()
+    oracle.vadv(V(res, h, h), V(a["u_stage"], h, h), V(a["wcon"], h, h), V(a["u_pos"], h, h), V(a["utens"], h, h),
+                dtr, ni, nj, nk, mode=mode, threads=8)
+    return res[:nk, h:h + nj, h:h + ni]
+
+
+def run_vadv(f, grid, dtr, precision):
+    from gridtools_amd import stencils
+
+    stencils.vadv(f["utens_stage"], f["u_stage"], f["wcon"], f["u_pos"], f["utens"], dtr, grid, precision=precision)
+    return f["utens_stage"].padded_numpy()
+
+
+@pytest.mark.parametrize("ni,nj,nk", VADV_SIZES)
+def test_vadv_fp64_parity(gpu, oracle, ni, nj, nk):
+    print(f"[case] vadv fp64 {ni}x{nj}x{nk}", flush=True)
+    f, grid = vadv_case(gpu, ni, nj, nk, np.float64)
+    ref = oracle_vadv(oracle, f, 0.15, 3, ni, nj, nk, "native")
+    got = run_vadv(f, grid, 0.15, 0)[:nk, 3:3 + nj, 3:3 + ni]
+    assert rel_err(got, ref) <= FP64_TOL
+    assert np.array_equal(got, ref), "FP64 path is expected to be bit-identical to the oracle"
+
+
+@pytest.mark.parametrize("ni,nj,nk", [(23, 11, 43), (256, 256, 80)])
+def test_vadv_fp32_native_parity(gpu, oracle, ni, nj, nk):
+    print(f"[case] vadv fp32 {ni}x{nj}x{nk}", flush=True)
+    f, grid = vadv_case(gpu, ni, nj, nk, np.float32)
+    dtr = float(np.float32(0.15))
+    ref = oracle_vadv(oracle, f, dtr, 3, ni, nj, nk, "native")
+    got = run_vadv(f, grid, 0.15, 0)[:nk, 3:3 + nj, 3:3 + ni]
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("ni,nj,nk", [(23, 11, 43), (256, 256, 80)])
+def test_vadv_precision_form(gpu, oracle, ni, nj, nk):
+    print(f"[case] vadv widen {ni}x{nj}x{nk}", flush=True)
+    f, grid = vadv_case(gpu, ni, nj, nk, np.float32)
+    dtr = float(np.float32(0.15))
+    mixed = oracle_vadv(oracle, f, dtr, 3, ni, nj, nk, "mixed")
+    wide = oracle_vadv(oracle, f, dtr, 3, ni, nj, nk, "wide")
+    got = run_vadv(f, grid, 0.15, 1)[:nk, 3:3 + nj, 3:3 + ni]
+    assert np.array_equal(got, mixed)
+    err = rel_err(got, wide)
+    print(f"[precision] vadv FP32-storage/FP64-compute max rel err vs FP64 oracle: {err:.3e}", flush=True)
+    assert err <= MIXED_TOL
+
+
+@pytest.mark.parametrize("ni,nj,nk", [(12, 33, 61), (256, 256, 80)])
+def test_laplacian_parity(gpu, oracle, ni, nj, nk):
+    from gridtools_amd import inputs, stencils
+
+    print(f"[case] laplacian {ni}x{nj}x{nk}", flush=True)
+    h = 1
+    b = gpu.builder().type(np.float64).dimensions(ni + 2, nj + 2, nk).halos(h, h, 0)
+    inp = b.uniform(SEED, inputs.FIELD_IDS["in"], -1.0, 1.0).build()
+    out = b.value(0).build()
+    grid = gpu.make_grid(gpu.halo_descriptor.symmetric(h, ni + 2), gpu.halo_descriptor.symmetric(h, nj + 2), nk)
+    stencils.laplacian(out, inp, grid)
+    a = inp.padded_numpy()
+    res = np.zeros_like(a)
+    oracle.laplacian(oracle.View(res, h, h), oracle.View(a, h, h), ni, nj, nk)
+    assert np.array_equal(out.padded_numpy()[:, 1:1 + nj, 1:1 + ni], res[:, 1:1 + nj, 1:1 + ni])
+
+
+def test_fill_uniform_matches_host_generator(gpu):
+    from gridtools_amd import inputs
+
+    for dtype in (np.float64, np.float32):
+        ds = gpu.builder().type(dtype).dimensions(37, 21, 5).halos(2, 2, 0).uniform(7, 3, -0.5, 0.5).build()
+        assert np.array_equal(ds.to_numpy(), inputs.uniform((37, 21, 5), 7, 3, -0.5, 0.5, dtype=dtype))
+
+
+def test_repeated_launches_are_deterministic(gpu):
+    from gridtools_amd import stencils
+
+    inp, coeff, out, grid = hdiff_case(gpu, 100, 37, 11, np.float64)
+    stencils.hdiff(out, inp, coeff, grid)
+    first = out.padded_numpy()
+    for _ in range(3):
+        stencils.hdiff(out, inp, coeff, grid)
+    assert np.array_equal(out.padded_numpy(), first)
