@@ -65,7 +65,7 @@ namespace gt355 {
                 int x = idx % IW, y = idx / IW;
                 int gi = i0 + x - 2, gj = j0 + y - 2;
                 in_ok[r] = idx < IW * IH && gi < ni + 2 && gj < nj + 2;
-                in_off[r] = in_ok[r] ? gi + gj * (int)isj : 0;
+                in_off[r] = in_ok[r] ? (x - 2) + (y - 2) * (int)isj : 0; // relative to the tile origin in_t
             }
             int out_x[OUT_ITERS], out_y[OUT_ITERS];
             bool out_ok[OUT_ITERS];

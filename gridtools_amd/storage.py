@@ -42,6 +42,8 @@ class DataStore:
         raw = ctypes.c_void_p()
         _abi.call("gt355_malloc", ctypes.c_size_t(lay.alloc_bytes), ctypes.byref(raw))
         self._raw = raw
+        # padding and alignment slack are zeroed once so that no copy of the allocation ever carries garbage
+        _abi.call("gt355_memset", raw, 0, ctypes.c_size_t(lay.alloc_bytes), None)
         base = ctypes.c_void_p()
         _abi.call("gt355_storage_base", ctypes.byref(lay), raw, ctypes.byref(base))
         self.base = base.value  # address of storage index (0, 0, 0)

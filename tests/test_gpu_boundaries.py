@@ -20,6 +20,12 @@ def field(gt, dtype, lengths, seed, fid):
     return gt.builder().type(dtype).dimensions(*lengths).halos(0, 0, 0).uniform(seed, fid, -1.0, 1.0).build()
 
 
+def box(ds, arr):
+    """The logical box of a padded [k, j, i_padded] copy (padding columns excluded)."""
+    ni, nj, nk = ds.lengths
+    return arr[:nk, :nj, :ni]
+
+
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("kind", [0, 1, 2])
 def test_bc_apply_matches_oracle(gpu, oracle, dtype, kind):
@@ -37,7 +43,7 @@ def test_bc_apply_matches_oracle(gpu, oracle, dtype, kind):
         oracle.bc(kind, e, s.padded_numpy() if kind == 2 else None, 0.75, hal, bc.dir_mask)
     bc.apply(*dst, src=src if kind == 2 else None)
     for d, e in zip(dst, exp):
-        assert np.array_equal(d.padded_numpy(), e)
+        assert np.array_equal(box(d, d.padded_numpy()), box(d, e))
 
 
 @pytest.mark.parametrize("modes", [(2, 1, 0), (1, 1, 0), (2, 2, 0), (2, 0, 0), (1, 2, 2)])
@@ -54,7 +60,7 @@ def test_halo_fill_matches_oracle(gpu, oracle, modes):
         oracle.halo_fill(e, hal, modes)
     boundaries.halo_fill(fs, hal, modes)
     for f, e in zip(fs, exp):
-        assert np.array_equal(f.padded_numpy(), e)
+        assert np.array_equal(box(f, f.padded_numpy()), box(f, e))
 
 
 def test_pack_self_exchange_unpack_is_periodic_fill(gpu, oracle):
@@ -84,8 +90,7 @@ def test_pack_self_exchange_unpack_is_periodic_fill(gpu, oracle):
         dirs.sort(key=lambda d: boundaries.direction_bit(*d))
         size = {}
         for d in dirs:
-            n = [ni if c == 0 else 3 for c in d[:2]]
-            size[d] = 2 * n[0] * n[1] * nk
+            size[d] = 2 * (ni if d[0] == 0 else 3) * (nj if d[1] == 0 else 3) * nk
         off, o = {}, 0
         for d in dirs:
             off[d] = o
@@ -97,7 +102,7 @@ def test_pack_self_exchange_unpack_is_periodic_fill(gpu, oracle):
         _abi.call("gt355_memcpy_h2d", recv, recv_host.ctypes.data, ctypes.c_size_t(nbytes))
         boundaries.halo_unpack(fs, hal, recv.value, mask)
         for f, e in zip(fs, exp):
-            assert np.array_equal(f.padded_numpy(), e)
+            assert np.array_equal(box(f, f.padded_numpy()), box(f, e))
     finally:
         _abi.lib().gt355_free(send)
         _abi.lib().gt355_free(recv)
