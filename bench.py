@@ -107,6 +107,30 @@ def kernel_stats(name, ms_list, bytes_per_elem):
     }
 
 
+def copy_leg(gt, steps, warmup, nsets):
+    """copy_stencil FP64 over the same 256x256x80 interior (16 B/point): the bandwidth a pure stream reaches."""
+    from gridtools_amd import stencils
+    from gridtools_amd.timing import Event, synchronize
+
+    b = gt.builder().type(np.float64).dimensions(NI, NJ, NK).halos(0, 0, 0)
+    pairs = [(b.uniform(SEED + s, 1, -1.0, 1.0).build(), b.value(0).build()) for s in range(max(nsets, 4) * 2)]
+    g = gt.make_grid(NI, NJ, NK)
+    for s in range(warmup):
+        src, dst = pairs[s % len(pairs)]
+        stencils.copy(dst, src, g)
+    synchronize()
+    evs = [(Event(), Event()) for _ in range(steps)]
+    for s in range(steps):
+        src, dst = pairs[(warmup + s) % len(pairs)]
+        evs[s][0].record()
+        stencils.copy(dst, src, g)
+        evs[s][1].record()
+    synchronize()
+    ms = statistics.mean(a.elapsed_ms(z) for a, z in evs)
+    gbs = 16 * P / (ms * 1e-3) / 1e9
+    return {"mean_us": round(ms * 1e3, 2), "gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+
+
 def cpu_baseline(budget_s=12.0):
     """The oracle port (OpenMP, same arithmetic) on this host, hdiff+vadv FP64 256x256x80, bounded sample."""
     from gridtools_amd import inputs
@@ -150,9 +174,13 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--sets", type=int, default=4)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 precision-form leg")
+    ap.add_argument("--legs", default="fp64,fp32,copy,cpu",
+                    help="comma list of legs: fp64 (always run), fp32 (precision form), copy (bandwidth yardstick), "
+                         "cpu (CPU baseline)")
     args = ap.parse_args()
+    legs = set(args.legs.split(","))
+    args.no_fp32 = "fp32" not in legs
+    args.no_cpu = "cpu" not in legs
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
               file=sys.stderr)
@@ -187,6 +215,8 @@ def main():
         }
         del sets32
 
+    copy = copy_leg(gt, args.steps, args.warmup, args.sets) if "copy" in legs else None
+
     cpu = None
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
@@ -211,6 +241,7 @@ def main():
                    "field_sets": args.sets, "parallelism": "single GPU"},
         "kernels": {"hdiff_fp64": hd64, "vadv_fp64": va64},
         "fp32_form": fp32,
+        "copy_fp64": copy,
         "roofline": {
             "kernel": "vadv FP64 (vadv_*_kernel<double,double>)",
             "bound": "hbm",

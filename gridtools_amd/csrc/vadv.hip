@@ -140,6 +140,122 @@ namespace gt355 {
             }
         }
 
+        // One wavefront per workgroup, one column per lane, k-depth bucket NK >= nk known at compile time.
+        // ccol/dcol of the whole column stay in registers between the sweeps (the compiler places them in the
+        // unified VGPR/AGPR file: at one wave per SIMD a lane owns 512 registers), so the kernel moves only the
+        // compulsory bytes.  With one wave per SIMD nothing else hides memory latency, so every input is
+        // software-pipelined D levels ahead through register rings:
+        //   ring A (u_stage, wcon at i and i+1) of level L is consumed at forward step L-1,
+        //   ring B (utens_stage, u_pos, utens) of level L at forward step L,
+        //   ring C (u_pos again) of level L at backward step L.
+        // Loops are fully unrolled over NK with uniform guards on the runtime nk, so every ring slot and every
+        // ccol/dcol element has a static register.  Lanes past ni read the last valid column and store nothing.
+        template <class ST, class CT, int NK, int D>
+        __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
+        vadv_onchip_kernel(ST *__restrict__ uts_f, const ST *__restrict__ us_f, const ST *__restrict__ wc_f,
+            const ST *__restrict__ up_f, const ST *__restrict__ ut_f, CT dtr, int ni, int nk, int64_t sj, int64_t sk,
+            int64_t wsj, int64_t wsk) {
+            static_assert(D >= 1 && D <= NK, "prefetch depth");
+            const int i_raw = blockIdx.x * 64 + threadIdx.x;
+            const bool store_ok = i_raw < ni;
+            const int i = store_ok ? i_raw : ni - 1;
+            const int j = blockIdx.y;
+            const int64_t col = (int64_t)j * sj + i;
+            const int64_t wcol = (int64_t)j * wsj + i;
+            const ST *us = us_f + col;
+            const ST *up = up_f + col;
+            const ST *ut = ut_f + col;
+            ST *uts = uts_f + col;
+            const ST *wc = wc_f + wcol;
+
+            ST a_us[D], a_wi[D], a_wip[D]; // ring A
+            ST b_uts[D], b_up[D], b_ut[D]; // ring B
+            auto load_a = [&](int L) {
+                a_us[L % D] = us[L * sk];
+                a_wi[L % D] = wc[L * wsk];
+                a_wip[L % D] = wc[L * wsk + 1];
+            };
+            auto load_b = [&](int L) {
+                b_uts[L % D] = uts[L * sk];
+                b_up[L % D] = up[L * sk];
+                b_ut[L % D] = ut[L * sk];
+            };
+#pragma unroll
+            for (int L = 0; L < D; ++L) {
+                if (L < nk)
+                    load_b(L);
+                if (L + 1 < nk)
+                    load_a(L + 1);
+            }
+
+            CT c[NK], d[NK];
+            CT us_m = CT(0), us_c = CT(us[0]), w_c_i = CT(0), w_c_ip = CT(0);
+#pragma unroll
+            for (int k = 0; k < NK; ++k) {
+                if (k < nk) {
+                    const CT v_uts = CT(b_uts[k % D]), v_up = CT(b_up[k % D]), v_ut = CT(b_ut[k % D]);
+                    CT us_p = CT(0), w_p_i = CT(0), w_p_ip = CT(0);
+                    if (k + 1 < nk) {
+                        us_p = CT(a_us[(k + 1) % D]);
+                        w_p_i = CT(a_wi[(k + 1) % D]);
+                        w_p_ip = CT(a_wip[(k + 1) % D]);
+                    }
+                    // refill the slots just consumed
+                    if (k + D < NK && k + D < nk)
+                        load_b(k + D);
+                    if (k + 1 + D < NK && k + 1 + D < nk)
+                        load_a(k + 1 + D);
+                    if (k == 0) {
+                        fwd_out<CT> r = forward_first<CT>(dtr, w_p_i, w_p_ip, us_c, us_p, v_up, v_ut, v_uts);
+                        c[0] = r.c;
+                        d[0] = r.d;
+                    } else if (k < nk - 1) {
+                        fwd_out<CT> r = forward_body<CT>(dtr, w_c_i, w_c_ip, w_p_i, w_p_ip, us_m, us_c, us_p, v_up,
+                            v_ut, v_uts, c[k > 0 ? k - 1 : 0], d[k > 0 ? k - 1 : 0]);
+                        c[k] = r.c;
+                        d[k] = r.d;
+                    } else {
+                        d[k] = forward_last<CT>(dtr, w_c_i, w_c_ip, us_m, us_c, v_up, v_ut, v_uts,
+                            c[k > 0 ? k - 1 : 0], d[k > 0 ? k - 1 : 0]);
+                    }
+                    us_m = us_c;
+                    us_c = us_p;
+                    w_c_i = w_p_i;
+                    w_c_ip = w_p_ip;
+                }
+            }
+
+            // backward sweep: u_pos of level L is prefetched at backward step L + D (or up front for the top D
+            // levels, which no active step precedes)
+            ST r_up[D];
+#pragma unroll
+            for (int L = 0; L < NK; ++L)
+                if (L < nk && L + D >= nk)
+                    r_up[L % D] = up[L * sk];
+            CT data = CT(0);
+#pragma unroll
+            for (int k = NK - 1; k >= 0; --k) {
+                if (k < nk) {
+                    const CT v_up = CT(r_up[k % D]);
+                    if (k - D >= 0)
+                        r_up[k % D] = up[(k - D) * sk];
+                    CT out;
+                    if (k == nk - 1) {
+                        // u_backward_function::apply(last_level) (vertical_advection_dycore.cpp:117-121)
+                        out = dtr * (d[k] - v_up);
+                        data = d[k];
+                    } else {
+                        // u_backward_function::apply(modify<0,-1>) (vertical_advection_dycore.cpp:110-115)
+                        CT dt = d[k] - c[k] * data;
+                        out = dtr * (dt - v_up);
+                        data = dt;
+                    }
+                    if (store_ok)
+                        uts[k * sk] = ST(out);
+                }
+            }
+        }
+
         // Scratch pool (the role of the reference's cached_allocator, sid/allocator.hpp:46-56): grows, never
         // shrinks, one per device; calls on one device are serialised by the caller's stream order.
         struct scratch_pool {
@@ -174,6 +290,13 @@ namespace gt355 {
             return GT355_OK;
         }
 
+        // Largest k-depth whose ccol/dcol fit the register file next to the prefetch rings without private-memory
+        // spills (FP64 NK=80: 255 VGPRs + 160 AGPRs; deeper buckets spill).
+        template <class CT>
+        constexpr int onchip_max_nk() {
+            return 80;
+        }
+
         template <class ST, class CT>
         int launch_vadv(const gt355_field *uts, const gt355_field *us, const gt355_field *wc, const gt355_field *up,
             const gt355_field *ut, double dtr_stage, int ni, int nj, int nk, hipStream_t stream) {
@@ -181,14 +304,32 @@ namespace gt355 {
             GT355_CHECK(us->stride[1] == sj && us->stride[2] == sk && up->stride[1] == sj && up->stride[2] == sk &&
                             ut->stride[1] == sj && ut->stride[2] == sk,
                 "gt355_vadv: utens_stage, u_stage, u_pos and utens must share one layout");
+            // the reference keeps dtr_stage as a global_parameter of the storage type (vertical_advection_dycore.cpp:139)
+            const CT dtr_c = CT(ST(dtr_stage));
+            if (nk <= onchip_max_nk<CT>()) {
+                GT355_CHECK(nj <= 65535, "gt355_vadv: nj too large");
+                dim3 grid((ni + 63) / 64, nj);
+#define GT355_VADV_ONCHIP(NKB)                                                                                   \
+    if (nk <= NKB) {                                                                                             \
+        vadv_onchip_kernel<ST, CT, NKB, (NKB < 8 ? NKB : 8)><<<grid, 64, 0, stream>>>((ST *)uts->origin,         \
+            (const ST *)us->origin, (const ST *)wc->origin, (const ST *)up->origin, (const ST *)ut->origin, dtr_c, \
+            ni, nk, sj, sk, wc->stride[1], wc->stride[2]);                                                       \
+        return check_launch("gt355_vadv");                                                                       \
+    }
+                GT355_VADV_ONCHIP(16)
+                GT355_VADV_ONCHIP(32)
+                GT355_VADV_ONCHIP(48)
+                GT355_VADV_ONCHIP(64)
+                GT355_VADV_ONCHIP(80)
+#undef GT355_VADV_ONCHIP
+            }
             int64_t plane = (int64_t)ni * nj;
             size_t bytes = sizeof(CT) * (size_t)plane * (size_t)(2 * nk - 1);
             void *scratch = nullptr;
             int rc = get_scratch(bytes, &scratch);
             if (rc)
                 return rc;
-            // the reference keeps dtr_stage as a global_parameter of the storage type (vertical_advection_dycore.cpp:139)
-            CT dtr = CT(ST(dtr_stage));
+            CT dtr = dtr_c;
             dim3 grid((ni + 63) / 64, (nj + 3) / 4);
             GT355_CHECK(grid.y <= 65535, "gt355_vadv: nj too large");
             vadv_scratch_kernel<ST, CT><<<grid, 256, 0, stream>>>((ST *)uts->origin, (const ST *)us->origin,

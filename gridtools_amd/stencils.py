@@ -25,6 +25,12 @@ def hdiff(out, in_, coeff, grid: Grid, precision=NATIVE, stream=None):
               grid.j_size, grid.k_size, stream)
 
 
+def copy(out, in_, grid: Grid, stream=None):
+    """copy_stencil (tests/regression/copy_stencil.cpp:23-44): out = in over the interior."""
+    o, i = _sid(out, grid), _sid(in_, grid)
+    _abi.call("gt355_copy", ctypes.byref(o), ctypes.byref(i), grid.i_size, grid.j_size, grid.k_size, stream)
+
+
 def laplacian(out, in_, grid: Grid, precision=NATIVE, stream=None):
     """Laplacian (tests/regression/laplacian.cpp:20-38)."""
     o, i = _sid(out, grid), _sid(in_, grid)

@@ -115,6 +115,10 @@ int gt355_fill_uniform(int dtype, void *base, const int64_t lengths[3], const in
 int gt355_hdiff(int precision, const gt355_field *out, const gt355_field *in, const gt355_field *coeff, int ni, int nj,
     int nk, void *stream);
 
+/* copy_stencil (tests/regression/copy_stencil.cpp:23-44, run_single_stage(copy_functor)): out = in over the
+ * interior.  Also the library's bandwidth yardstick (one read + one write per point). */
+int gt355_copy(const gt355_field *out, const gt355_field *in, int ni, int nj, int nk, void *stream);
+
 /* Laplacian (tests/regression/laplacian.cpp:20-38, run_single_stage): out = 4 in - (in[i+-1] + in[j+-1]). */
 int gt355_laplacian(int precision, const gt355_field *out, const gt355_field *in, int ni, int nj, int nk,
     void *stream);
