@@ -57,54 +57,69 @@ def grids(gt):
 
 
 def timed_steps(gt, sets, precision, steps, warmup):
-    """Runs warmup + steps steps; returns (total ms of the timed steps, per-launch ms lists)."""
+    """Runs warmup + steps steps; returns (total ms of the timed steps, mean ms per hdiff launch, mean ms per vadv
+    launch).  The step loop is bracketed by one event pair (synchronised on both sides).  Per-kernel times come
+    from separate runs of `steps` back-to-back launches of one kernel, also bracketed by one event pair on the
+    launch stream, so that they measure launch-to-launch kernel time without the dispatch gap an event pair
+    around every single launch would add."""
     from gridtools_amd import stencils
     from gridtools_amd.timing import Event, synchronize
 
     gh, gv = grids(gt)
 
-    def launch(step, ev=None):
-        hd, va = sets[step % len(sets)]
-        if ev:
-            ev[0].record()
+    def hdiff(step):
+        hd = sets[step % len(sets)][0]
         stencils.hdiff(hd["out"], hd["in"], hd["coeff"], gh, precision=precision)
-        if ev:
-            ev[1].record()
+
+    def vadv(step):
+        va = sets[step % len(sets)][1]
         stencils.vadv(va["utens_stage"], va["u_stage"], va["wcon"], va["u_pos"], va["utens"], 0.15, gv,
                       precision=precision)
-        if ev:
-            ev[2].record()
+
+    def timed(fn_list, n, offset):
+        start, stop = Event(), Event()
+        synchronize()
+        start.record()
+        for s in range(n):
+            for fn in fn_list:
+                fn(offset + s)
+        stop.record()
+        stop.synchronize()
+        synchronize()
+        return start.elapsed_ms(stop)
 
     for s in range(warmup):
-        launch(s)
-    synchronize()
-    evs = [[Event(), Event(), Event()] for _ in range(steps)]
-    start, stop = Event(), Event()
-    synchronize()
-    start.record()
-    for s in range(steps):
-        launch(warmup + s, evs[s])
-    stop.record()
-    stop.synchronize()
-    synchronize()
-    total = start.elapsed_ms(stop)
-    hd_ms = [e[0].elapsed_ms(e[1]) for e in evs]
-    va_ms = [e[1].elapsed_ms(e[2]) for e in evs]
+        hdiff(s)
+        vadv(s)
+    total = timed([hdiff, vadv], steps, warmup)
+    hd_ms = timed([hdiff], steps, warmup + steps) / steps
+    va_ms = timed([vadv], steps, warmup + 2 * steps) / steps
     return total, hd_ms, va_ms
 
 
-def kernel_stats(name, ms_list, bytes_per_elem):
-    ms = statistics.mean(ms_list)
-    med = statistics.median(ms_list)
+def kernel_stats(name, ms, bytes_per_elem):
     algo = BYTES_PER_PT[name] * bytes_per_elem * P
     return {
         "mean_us": round(ms * 1e3, 2),
-        "median_us": round(med * 1e3, 2),
         "gpts": round(P / (ms * 1e-3) / 1e9, 2),
         "algo_bytes": algo,
         "achieved_gbs": round(algo / (ms * 1e-3) / 1e9, 1),
         "hbm_frac": round(algo / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
     }
+
+
+def measured_traffic(kernel_prefix):
+    """HBM bytes per launch of a kernel as measured by the calibrated rocprofv3 FETCH_SIZE/WRITE_SIZE passes
+    committed under profiles/ (line 'TRAFFIC <kernel> <workload> = <bytes>'); None if no such line exists."""
+    import glob
+    import re
+
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        for line in open(path):
+            m = re.match(r"TRAFFIC (\S+) (\S+) = (\d+)", line.strip())
+            if m and m.group(1).startswith(kernel_prefix) and m.group(2) == f"{NI}x{NJ}x{NK}":
+                return {"bytes": int(m.group(3)), "source": os.path.relpath(path, ROOT), "kernel": m.group(1)}
+    return None
 
 
 def copy_leg(gt, steps, warmup, nsets):
@@ -119,14 +134,14 @@ def copy_leg(gt, steps, warmup, nsets):
         src, dst = pairs[s % len(pairs)]
         stencils.copy(dst, src, g)
     synchronize()
-    evs = [(Event(), Event()) for _ in range(steps)]
+    start, stop = Event(), Event()
+    start.record()
     for s in range(steps):
         src, dst = pairs[(warmup + s) % len(pairs)]
-        evs[s][0].record()
         stencils.copy(dst, src, g)
-        evs[s][1].record()
-    synchronize()
-    ms = statistics.mean(a.elapsed_ms(z) for a, z in evs)
+    stop.record()
+    stop.synchronize()
+    ms = start.elapsed_ms(stop) / steps
     gbs = 16 * P / (ms * 1e-3) / 1e9
     return {"mean_us": round(ms * 1e3, 2), "gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
 
@@ -150,7 +165,7 @@ def cpu_baseline(budget_s=12.0):
     times = []
     t_end = time.perf_counter() + budget_s
     reps = 0
-    while (reps < 3 or time.perf_counter() < t_end) and reps < 50:
+    while (reps < 3 or time.perf_counter() < t_end) and reps < 5000:
         uts = uts0.copy()
         t0 = time.perf_counter()
         ob.hdiff(V(out, h, h), V(in_, h, h), V(co, h, h), NI, NJ, NK, threads=threads)
@@ -224,6 +239,7 @@ def main():
 
     value = P / (ms_per_step * 1e-3)
     dom = va64
+    traffic = measured_traffic("vadv_onchip_kernel<double,double,80")
     line = {
         "metric": "gridpoint-updates/s + % HBM roofline, hdiff & vadv FP64 vs FP32 @256x256x80",
         "value": round(value, 1),
@@ -249,7 +265,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(dom["achieved_gbs"] / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic["bytes"] if traffic else None,
+            "traffic_source": traffic,
         },
         "cpu_baseline": cpu,
     }

@@ -2,6 +2,9 @@
 // autotuner) can override them per problem shape.
 #include "tuning.hpp"
 
+#include <cstdlib>
+#include <string>
+
 namespace gt355 {
 
     hdiff_tile hdiff_tile_for(int st_size, int ct_size, int ni, int nj, int nk) {
@@ -11,6 +14,12 @@ namespace gt355 {
         (void)nj;
         (void)nk;
         return {64, 8, 8};
+    }
+
+    int tune_param(const char *name, int dflt) {
+        std::string key = std::string("GT355_TUNE_") + name;
+        const char *v = std::getenv(key.c_str());
+        return v ? std::atoi(v) : dflt;
     }
 
 } // namespace gt355

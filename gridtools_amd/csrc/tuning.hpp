@@ -24,4 +24,7 @@ namespace gt355 {
     // recorded one, otherwise a heuristic default.
     hdiff_tile hdiff_tile_for(int st_size, int ct_size, int ni, int nj, int nk);
 
+    // A named integer tuning knob: the environment variable GT355_TUNE_<name> if set, otherwise dflt.
+    int tune_param(const char *name, int dflt);
+
 } // namespace gt355

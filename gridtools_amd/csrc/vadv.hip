@@ -17,6 +17,7 @@
 #include <mutex>
 
 #include "common.hpp"
+#include "tuning.hpp"
 
 namespace gt355 {
     namespace {
@@ -142,24 +143,27 @@ namespace gt355 {
 
         // One wavefront per workgroup, one column per lane, k-depth bucket NK >= nk known at compile time.
         // ccol/dcol of the whole column stay in registers between the sweeps (the compiler places them in the
-        // unified VGPR/AGPR file: at one wave per SIMD a lane owns 512 registers), so the kernel moves only the
-        // compulsory bytes.  With one wave per SIMD nothing else hides memory latency, so every input is
+        // unified VGPR/AGPR file: at one wave per SIMD a lane owns 512 registers), so neither sweep touches a
+        // temporary in memory.  With one wave per SIMD nothing else hides memory latency, so every input is
         // software-pipelined D levels ahead through register rings:
         //   ring A (u_stage, wcon at i and i+1) of level L is consumed at forward step L-1,
-        //   ring B (utens_stage, u_pos, utens) of level L at forward step L,
-        //   ring C (u_pos again) of level L at backward step L.
-        // Loops are fully unrolled over NK with uniform guards on the runtime nk, so every ring slot and every
-        // ccol/dcol element has a static register.  Lanes past ni read the last valid column and store nothing.
-        template <class ST, class CT, int NK, int D>
+        //   ring B (utens_stage, u_pos, utens) of level L at forward step L.
+        // The sweeps are straight-line code: loops are fully unrolled over NK, every load is unconditional (levels
+        // past the runtime top nk-1 are clamped to it, so they re-read a cached line), and the first / interior /
+        // top-level formulas are chosen by value, not by branch.  That keeps the hardware's in-order load counter
+        // exact across the unrolled steps, so the wait before step k covers only level k's loads.
+        // u_pos is needed again by the backward sweep.  UPOS_LDS parks the column's u_pos in LDS (NK*64*sizeof(ST)
+        // bytes per wavefront; nothing is re-read from memory); otherwise the backward sweep re-loads u_pos through
+        // its own D-deep ring.  Lanes past ni read the last valid column and store nothing.
+        template <class ST, class CT, int NK, int D, bool UPOS_LDS>
         __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
         vadv_onchip_kernel(ST *__restrict__ uts_f, const ST *__restrict__ us_f, const ST *__restrict__ wc_f,
             const ST *__restrict__ up_f, const ST *__restrict__ ut_f, CT dtr, int ni, int nk, int64_t sj, int64_t sk,
             int64_t wsj, int64_t wsk) {
             static_assert(D >= 1 && D <= NK, "prefetch depth");
-            const int i_raw = blockIdx.x * 64 + threadIdx.x;
-            const bool store_ok = i_raw < ni;
-            const int i = store_ok ? i_raw : ni - 1;
+            const int i = min(int(blockIdx.x * 64 + threadIdx.x), ni - 1);
             const int j = blockIdx.y;
+            const int top = nk - 1;
             const int64_t col = (int64_t)j * sj + i;
             const int64_t wcol = (int64_t)j * wsj + i;
             const ST *us = us_f + col;
@@ -167,92 +171,106 @@ namespace gt355 {
             const ST *ut = ut_f + col;
             ST *uts = uts_f + col;
             const ST *wc = wc_f + wcol;
+            auto lvl = [&](int L) { return (int64_t)min(L, top); };
+            // The loads read memory nothing in the forward sweep writes, so the compiler would treat them as
+            // invariant and hoist all of them to the kernel entry (one register per in-flight value: ~800 VGPRs).
+            // Routing the strides through an empty volatile asm at every step gives each step's loads an opaque
+            // dependency that only exists from that step on.
+            int64_t sk_o = sk, wsk_o = wsk;
+            auto pin = [&]() {
+                asm volatile("" : "+s"(sk_o));
+                asm volatile("" : "+s"(wsk_o));
+            };
 
             ST a_us[D], a_wi[D], a_wip[D]; // ring A
             ST b_uts[D], b_up[D], b_ut[D]; // ring B
             auto load_a = [&](int L) {
-                a_us[L % D] = us[L * sk];
-                a_wi[L % D] = wc[L * wsk];
-                a_wip[L % D] = wc[L * wsk + 1];
+                const int64_t l = lvl(L);
+                a_us[L % D] = us[l * sk_o];
+                a_wi[L % D] = wc[l * wsk_o];
+                a_wip[L % D] = wc[l * wsk_o + 1];
             };
             auto load_b = [&](int L) {
-                b_uts[L % D] = uts[L * sk];
-                b_up[L % D] = up[L * sk];
-                b_ut[L % D] = ut[L * sk];
+                const int64_t l = lvl(L);
+                b_uts[L % D] = uts[l * sk_o];
+                b_up[L % D] = up[l * sk_o];
+                b_ut[L % D] = ut[l * sk_o];
             };
-#pragma unroll
+#pragma clang loop unroll(full)
             for (int L = 0; L < D; ++L) {
-                if (L < nk)
-                    load_b(L);
-                if (L + 1 < nk)
-                    load_a(L + 1);
+                load_b(L);
+                load_a(L + 1);
             }
 
+            __shared__ ST s_up[UPOS_LDS ? NK : 1][64];
             CT c[NK], d[NK];
             CT us_m = CT(0), us_c = CT(us[0]), w_c_i = CT(0), w_c_ip = CT(0);
-#pragma unroll
+#pragma clang loop unroll(full)
             for (int k = 0; k < NK; ++k) {
-                if (k < nk) {
-                    const CT v_uts = CT(b_uts[k % D]), v_up = CT(b_up[k % D]), v_ut = CT(b_ut[k % D]);
-                    CT us_p = CT(0), w_p_i = CT(0), w_p_ip = CT(0);
-                    if (k + 1 < nk) {
-                        us_p = CT(a_us[(k + 1) % D]);
-                        w_p_i = CT(a_wi[(k + 1) % D]);
-                        w_p_ip = CT(a_wip[(k + 1) % D]);
-                    }
-                    // refill the slots just consumed
-                    if (k + D < NK && k + D < nk)
-                        load_b(k + D);
-                    if (k + 1 + D < NK && k + 1 + D < nk)
-                        load_a(k + 1 + D);
-                    if (k == 0) {
-                        fwd_out<CT> r = forward_first<CT>(dtr, w_p_i, w_p_ip, us_c, us_p, v_up, v_ut, v_uts);
-                        c[0] = r.c;
-                        d[0] = r.d;
-                    } else if (k < nk - 1) {
-                        fwd_out<CT> r = forward_body<CT>(dtr, w_c_i, w_c_ip, w_p_i, w_p_ip, us_m, us_c, us_p, v_up,
-                            v_ut, v_uts, c[k > 0 ? k - 1 : 0], d[k > 0 ? k - 1 : 0]);
-                        c[k] = r.c;
-                        d[k] = r.d;
-                    } else {
-                        d[k] = forward_last<CT>(dtr, w_c_i, w_c_ip, us_m, us_c, v_up, v_ut, v_uts,
-                            c[k > 0 ? k - 1 : 0], d[k > 0 ? k - 1 : 0]);
-                    }
-                    us_m = us_c;
-                    us_c = us_p;
-                    w_c_i = w_p_i;
-                    w_c_ip = w_p_ip;
+                if constexpr (UPOS_LDS)
+                    s_up[k][threadIdx.x] = b_up[k % D];
+                const CT v_uts = CT(b_uts[k % D]), v_up = CT(b_up[k % D]), v_ut = CT(b_ut[k % D]);
+                // at the top level the k+1 wind is taken as zero: with gcv = cs = c = 0 the interior formula computes
+                // b = (dtr - a) - 0 and correction - 0*(us_p - us_c), i.e. exactly the last_level functor's values
+                // for finite inputs (us_p is set to us_c, so the dropped term is 0*(+0) = +0 and x - (+0) == x)
+                const bool at_top = k == top;
+                const CT us_p = at_top ? us_c : CT(a_us[(k + 1) % D]);
+                const CT w_p_i = at_top ? CT(0) : CT(a_wi[(k + 1) % D]);
+                const CT w_p_ip = at_top ? CT(0) : CT(a_wip[(k + 1) % D]);
+                // refill the slots just consumed
+                pin();
+                if (k + D < NK)
+                    load_b(k + D);
+                if (k + 1 + D < NK)
+                    load_a(k + 1 + D);
+                if (k == 0) {
+                    fwd_out<CT> r = forward_first<CT>(dtr, w_p_i, w_p_ip, us_c, us_p, v_up, v_ut, v_uts);
+                    c[0] = r.c;
+                    d[0] = r.d;
+                } else {
+                    const CT cm = c[k > 0 ? k - 1 : 0], dm = d[k > 0 ? k - 1 : 0];
+                    fwd_out<CT> r = forward_body<CT>(
+                        dtr, w_c_i, w_c_ip, w_p_i, w_p_ip, us_m, us_c, us_p, v_up, v_ut, v_uts, cm, dm);
+                    c[k] = r.c; // never read at the top level
+                    d[k] = r.d;
                 }
+                us_m = us_c;
+                us_c = us_p;
+                w_c_i = w_p_i;
+                w_c_ip = w_p_ip;
+                // one scheduling region per step: the loads stay where the ring puts them (D levels ahead) instead
+                // of being hoisted wholesale, which would need a register per in-flight value
+                __builtin_amdgcn_sched_barrier(0);
             }
 
-            // backward sweep: u_pos of level L is prefetched at backward step L + D (or up front for the top D
-            // levels, which no active step precedes)
-            ST r_up[D];
-#pragma unroll
-            for (int L = 0; L < NK; ++L)
-                if (L < nk && L + D >= nk)
-                    r_up[L % D] = up[L * sk];
+            ST r_up[D]; // backward u_pos ring (!UPOS_LDS): level L consumed at backward step L, loaded at step L + D
+            if constexpr (!UPOS_LDS) {
+#pragma clang loop unroll(full)
+                for (int s = 0; s < D; ++s)
+                    r_up[(NK - 1 - s) % D] = up[lvl(NK - 1 - s) * sk];
+            }
             CT data = CT(0);
-#pragma unroll
+#pragma clang loop unroll(full)
             for (int k = NK - 1; k >= 0; --k) {
-                if (k < nk) {
-                    const CT v_up = CT(r_up[k % D]);
+                CT v_up;
+                if constexpr (UPOS_LDS) {
+                    v_up = CT(s_up[k][threadIdx.x]);
+                } else {
+                    v_up = CT(r_up[k % D]);
+                    pin();
                     if (k - D >= 0)
-                        r_up[k % D] = up[(k - D) * sk];
-                    CT out;
-                    if (k == nk - 1) {
-                        // u_backward_function::apply(last_level) (vertical_advection_dycore.cpp:117-121)
-                        out = dtr * (d[k] - v_up);
-                        data = d[k];
-                    } else {
-                        // u_backward_function::apply(modify<0,-1>) (vertical_advection_dycore.cpp:110-115)
-                        CT dt = d[k] - c[k] * data;
-                        out = dtr * (dt - v_up);
-                        data = dt;
-                    }
-                    if (store_ok)
-                        uts[k * sk] = ST(out);
+                        r_up[k % D] = up[lvl(k - D) * sk_o];
                 }
+                // u_backward_function (vertical_advection_dycore.cpp:110-121): the top level starts the recurrence
+                const CT dt = d[k] - c[k] * data;
+                const CT nd = k == top ? d[k] : dt;
+                const CT out = dtr * (nd - v_up);
+                data = nd;
+                // Unconditional store to the clamped level: a step above the top writes a value that the step at
+                // the top overwrites later in program order, and lanes past ni compute column ni-1 and write its
+                // bits again.  No branch, so the load counter stays exact.
+                uts[lvl(k) * sk_o] = ST(out);
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
 
@@ -311,11 +329,20 @@ namespace gt355 {
                 dim3 grid((ni + 63) / 64, nj);
 #define GT355_VADV_ONCHIP(NKB)                                                                                   \
     if (nk <= NKB) {                                                                                             \
-        vadv_onchip_kernel<ST, CT, NKB, (NKB < 8 ? NKB : 8)><<<grid, 64, 0, stream>>>((ST *)uts->origin,         \
-            (const ST *)us->origin, (const ST *)wc->origin, (const ST *)up->origin, (const ST *)ut->origin, dtr_c, \
-            ni, nk, sj, sk, wc->stride[1], wc->stride[2]);                                                       \
+        if (upos_lds)                                                                                            \
+            vadv_onchip_kernel<ST, CT, NKB, (NKB < 8 ? NKB : 8), true><<<grid, 64, 0, stream>>>(                  \
+                (ST *)uts->origin, (const ST *)us->origin, (const ST *)wc->origin, (const ST *)up->origin,       \
+                (const ST *)ut->origin, dtr_c, ni, nk, sj, sk, wc->stride[1], wc->stride[2]);                    \
+        else                                                                                                     \
+            vadv_onchip_kernel<ST, CT, NKB, (NKB < 8 ? NKB : 8), false><<<grid, 64, 0, stream>>>(                 \
+                (ST *)uts->origin, (const ST *)us->origin, (const ST *)wc->origin, (const ST *)up->origin,       \
+                (const ST *)ut->origin, dtr_c, ni, nk, sj, sk, wc->stride[1], wc->stride[2]);                    \
         return check_launch("gt355_vadv");                                                                       \
     }
+                // u_pos parked in LDS costs NK*64*sizeof(ST) bytes per wavefront (40 KB for FP64 at NK=80: four
+                // wavefronts still fill a CU's 160 KB) and saves the backward re-read; measured faster for FP64 and
+                // FP32 storage at 256x256x80 (54.5 vs 56.3 us FP64).
+                const bool upos_lds = tune_param("vadv_upos_lds", 1) != 0;
                 GT355_VADV_ONCHIP(16)
                 GT355_VADV_ONCHIP(32)
                 GT355_VADV_ONCHIP(48)
