@@ -13,7 +13,12 @@ namespace gt355 {
         (void)ni;
         (void)nj;
         (void)nk;
-        return {64, 8, 8};
+        // 64x8 tiles with 5-level chunks: 2048 workgroups at 256x256x80, one full round at 8 waves per SIMD
+        hdiff_tile t{64, 8, 5};
+        t.ti = tune_param("hdiff_ti", t.ti);
+        t.tj = tune_param("hdiff_tj", t.tj);
+        t.kc = tune_param("hdiff_kc", t.kc);
+        return t;
     }
 
     int tune_param(const char *name, int dflt) {
