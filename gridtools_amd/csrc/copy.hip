@@ -30,10 +30,52 @@ namespace gt355 {
                     dst[v] = src[v];
             }
         }
+        // Strided 3-D copy with element conversion (one thread per point, i fastest).
+        template <class D, class S>
+        __global__ void __launch_bounds__(256) convert_kernel(D *__restrict__ dst, const S *__restrict__ src, int64_t n0,
+            int64_t n1, int64_t n2, int64_t d1, int64_t d2, int64_t s1, int64_t s2) {
+            const int64_t total = n0 * n1 * n2;
+            for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < total;
+                 n += (int64_t)gridDim.x * blockDim.x) {
+                const int64_t i = n % n0, r = n / n0, j = r % n1, k = r / n1;
+                dst[i + j * d1 + k * d2] = static_cast<D>(src[i + j * s1 + k * s2]);
+            }
+        }
     } // namespace
 } // namespace gt355
 
 using namespace gt355;
+
+extern "C" int gt355_convert(int dst_dtype, void *dst, const int64_t dst_stride[3], int src_dtype, const void *src,
+    const int64_t src_stride[3], const int64_t lengths[3], void *stream) {
+    GT355_CHECK(dst && src && dst_stride && src_stride && lengths, "gt355_convert: null argument");
+    GT355_CHECK((dst_dtype == GT355_F64 || dst_dtype == GT355_F32) && (src_dtype == GT355_F64 || src_dtype == GT355_F32),
+        "gt355_convert: dtypes %d <- %d", dst_dtype, src_dtype);
+    for (int d = 0; d < 3; ++d)
+        GT355_CHECK(lengths[d] >= 0, "gt355_convert: negative length");
+    GT355_CHECK(dst_stride[0] == 1 && src_stride[0] == 1, "gt355_convert: i strides must be 1");
+    GT355_CHECK(dst_stride[1] >= lengths[0] && dst_stride[2] >= dst_stride[1] * lengths[1] &&
+                    src_stride[1] >= lengths[0] && src_stride[2] >= src_stride[1] * lengths[1],
+        "gt355_convert: strides do not fit the lengths");
+    const int64_t total = lengths[0] * lengths[1] * lengths[2];
+    if (total == 0)
+        return GT355_OK;
+    const int blocks = (int)std::min<int64_t>((total + 255) / 256, 65536);
+    hipStream_t s = (hipStream_t)stream;
+#define GT355_CONVERT(D, S)                                                                                      \
+    convert_kernel<D, S><<<blocks, 256, 0, s>>>((D *)dst, (const S *)src, lengths[0], lengths[1], lengths[2],    \
+        dst_stride[1], dst_stride[2], src_stride[1], src_stride[2])
+    if (dst_dtype == GT355_F64 && src_dtype == GT355_F64)
+        GT355_CONVERT(double, double);
+    else if (dst_dtype == GT355_F64)
+        GT355_CONVERT(double, float);
+    else if (src_dtype == GT355_F64)
+        GT355_CONVERT(float, double);
+    else
+        GT355_CONVERT(float, float);
+#undef GT355_CONVERT
+    return check_launch("gt355_convert");
+}
 
 extern "C" int gt355_copy(const gt355_field *out, const gt355_field *in, int ni, int nj, int nk, void *stream) {
     GT355_CHECK(out && in, "gt355_copy: null field");

@@ -38,6 +38,22 @@ def laplacian(out, in_, grid: Grid, precision=NATIVE, stream=None):
               grid.k_size, stream)
 
 
+def hdiff_vadv(in_, coeff, u_stage, utens_stage, wcon, u_pos, utens, dtr_stage, grid: Grid, precision=NATIVE,
+               stream=None):
+    """The hdiff -> vadv composition: u_stage = hdiff(in, coeff), then vertical advection of u_stage."""
+    s = [_sid(f, grid) for f in (in_, coeff, u_stage, utens_stage, wcon, u_pos, utens)]
+    _abi.call("gt355_hdiff_vadv", precision, *[ctypes.byref(x) for x in s], ctypes.c_double(dtr_stage),
+              grid.i_size, grid.j_size, grid.k_size, stream)
+
+
+def convert(dst, src, stream=None):
+    """dst = src elementwise over the whole field (same lengths; the element type may differ)."""
+    if dst.lengths != src.lengths:
+        raise ValueError(f"lengths differ: {dst.lengths} vs {src.lengths}")
+    _abi.call("gt355_convert", dst.code, ctypes.c_void_p(dst.base), _abi.c_i64_3(*dst.strides), src.code,
+              ctypes.c_void_p(src.base), _abi.c_i64_3(*src.strides), _abi.c_i64_3(*src.lengths), stream)
+
+
 def vadv(utens_stage, u_stage, wcon, u_pos, utens, dtr_stage, grid: Grid, precision=NATIVE, stream=None):
     """Vertical advection dycore (tests/regression/vertical_advection_dycore.cpp:32-150).  utens_stage is
     read and overwritten."""

@@ -130,6 +130,19 @@ int gt355_laplacian(int precision, const gt355_field *out, const gt355_field *in
 int gt355_vadv(int precision, const gt355_field *utens_stage, const gt355_field *u_stage, const gt355_field *wcon,
     const gt355_field *u_pos, const gt355_field *utens, double dtr_stage, int ni, int nj, int nk, void *stream);
 
+/* hdiff -> vadv composition (one run() with multi_pass(hdiff MSS, forward MSS, backward MSS); the reference gpu
+ * backend splits it into the same two launches, stencil/gpu/entry_point.hpp:96-124): u_stage = hdiff(in, coeff),
+ * then vadv(utens_stage, u_stage, wcon, u_pos, utens).  Every field is validated before the first launch. */
+int gt355_hdiff_vadv(int precision, const gt355_field *in, const gt355_field *coeff, const gt355_field *u_stage,
+    const gt355_field *utens_stage, const gt355_field *wcon, const gt355_field *u_pos, const gt355_field *utens,
+    double dtr_stage, int ni, int nj, int nk, void *stream);
+
+/* Strided 3-D copy with element conversion between F64 and F32 (the role of layout_transformation,
+ * include/gridtools/layout_transformation/gpu.hpp:72-116, plus a dtype change): dst[i + j*ds1 + k*ds2] =
+ * (dst type) src[i + j*ss1 + k*ss2] over lengths[0] x lengths[1] x lengths[2]; i strides must be 1. */
+int gt355_convert(int dst_dtype, void *dst, const int64_t dst_stride[3], int src_dtype, const void *src,
+    const int64_t src_stride[3], const int64_t lengths[3], void *stream);
+
 /* ------------------------------------------------------------------ boundaries
  * gt355_bc_apply: boundary::apply for the predefined functors (boundaries/boundary.hpp:57-79,
  * apply_gpu.hpp:235-309): kind 0 zero_boundary (zero.hpp:29-55), 1 value_boundary (value.hpp:27-67),

@@ -83,6 +83,37 @@ def test_hdiff_precision_form(gpu, oracle, ni, nj, nk):
     assert rel_err(got, ref64) <= MIXED_TOL
 
 
+@pytest.mark.parametrize("impl", [0, 1])
+@pytest.mark.parametrize("dtype,precision", [(np.float64, 0), (np.float32, 0), (np.float32, 1)])
+@pytest.mark.parametrize("ni,nj,nk", [(12, 33, 5), (130, 70, 3), (256, 256, 4)])
+def test_hdiff_both_kernels_all_precisions(gpu, oracle, monkeypatch, impl, dtype, precision, ni, nj, nk):
+    """The LDS-tile kernel and the DPP register-sweep kernel are each the default for one storage type; both
+    must agree with the oracle for every precision (selected through the GT355_TUNE_hdiff_impl knob)."""
+    from gridtools_amd import stencils
+
+    monkeypatch.setenv("GT355_TUNE_hdiff_impl", str(impl))
+    print(f"[case] hdiff impl={impl} {np.dtype(dtype).name} prec={precision} {ni}x{nj}x{nk}", flush=True)
+    inp, coeff, out, grid = hdiff_case(gpu, ni, nj, nk, dtype)
+    stencils.hdiff(out, inp, coeff, grid, precision=precision)
+    got = out.padded_numpy()[:nk, 2:2 + nj, 2:2 + ni]
+    mode = "mixed" if precision == 1 else "native"
+    assert np.array_equal(got, oracle_hdiff(oracle, inp, coeff, 2, ni, nj, nk, mode))
+
+
+@pytest.mark.parametrize("jb,p", [(8, 4), (16, 8), (32, 4), (32, 8)])
+def test_hdiff_sweep_variants(gpu, oracle, monkeypatch, jb, p):
+    from gridtools_amd import stencils
+
+    monkeypatch.setenv("GT355_TUNE_hdiff_impl", "1")
+    monkeypatch.setenv("GT355_TUNE_hdiff_jb", str(jb))
+    monkeypatch.setenv("GT355_TUNE_hdiff_p", str(p))
+    ni, nj, nk = 97, 75, 3
+    inp, coeff, out, grid = hdiff_case(gpu, ni, nj, nk, np.float64)
+    stencils.hdiff(out, inp, coeff, grid)
+    got = out.padded_numpy()[:nk, 2:2 + nj, 2:2 + ni]
+    assert np.array_equal(got, oracle_hdiff(oracle, inp, coeff, 2, ni, nj, nk, "native"))
+
+
 def vadv_case(gt, ni, nj, nk, dtype, h=3):
     from gridtools_amd import inputs
 
@@ -125,6 +156,30 @@ def test_vadv_fp64_parity(gpu, oracle, ni, nj, nk):
     got = run_vadv(f, grid, 0.15, 0)[:nk, 3:3 + nj, 3:3 + ni]
     assert rel_err(got, ref) <= FP64_TOL
     assert np.array_equal(got, ref), "FP64 path is expected to be bit-identical to the oracle"
+
+
+@pytest.mark.parametrize("ni,nj,nk", [(70, 9, 81), (33, 20, 137)])
+def test_vadv_deep_columns_use_scratch_kernel(gpu, oracle, ni, nj, nk):
+    """nk above the largest register bucket (80) runs the kernel that keeps ccol/dcol in a pooled buffer."""
+    print(f"[case] vadv fp64 deep {ni}x{nj}x{nk}", flush=True)
+    f, grid = vadv_case(gpu, ni, nj, nk, np.float64)
+    ref = oracle_vadv(oracle, f, 0.15, 3, ni, nj, nk, "native")
+    got = run_vadv(f, grid, 0.15, 0)[:nk, 3:3 + nj, 3:3 + ni]
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("upos_lds", [0, 1])
+@pytest.mark.parametrize("dtype,precision", [(np.float64, 0), (np.float32, 1)])
+@pytest.mark.parametrize("nk", [2, 17, 43, 64, 80])
+def test_vadv_register_buckets_and_upos_placement(gpu, oracle, monkeypatch, upos_lds, dtype, precision, nk):
+    """Every k-depth bucket (16..80, including nk far below the bucket size) and both u_pos placements."""
+    monkeypatch.setenv("GT355_TUNE_vadv_upos_lds", str(upos_lds))
+    ni, nj = 67, 5
+    f, grid = vadv_case(gpu, ni, nj, nk, dtype)
+    dtr = 0.15 if dtype == np.float64 else float(np.float32(0.15))
+    ref = oracle_vadv(oracle, f, dtr, 3, ni, nj, nk, "native" if precision == 0 else "mixed")
+    got = run_vadv(f, grid, 0.15, precision)[:nk, 3:3 + nj, 3:3 + ni]
+    assert np.array_equal(got, ref)
 
 
 @pytest.mark.parametrize("ni,nj,nk", [(23, 11, 43), (256, 256, 80)])
