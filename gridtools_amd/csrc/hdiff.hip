@@ -17,21 +17,11 @@
 #include <hip/hip_runtime.h>
 
 #include "common.hpp"
+#include "functors.hpp"
 #include "tuning.hpp"
 
 namespace gt355 {
     namespace {
-
-        // Reference functor bodies (horizontal_diffusion.cpp:41-46, :56-60, :70-74, :86-90).
-        template <class CT>
-        __device__ __forceinline__ CT lap_f(CT c, CT ip, CT jp, CT im, CT jm) {
-            return CT(4) * c - (((ip + jp) + im) + jm);
-        }
-        template <class CT>
-        __device__ __forceinline__ CT flux_f(CT lap_p, CT lap_c, CT in_p, CT in_c) {
-            CT res = lap_p - lap_c;
-            return (res * (in_p - in_c) > CT(0)) ? CT(0) : res;
-        }
 
         template <class ST, class CT, int TI, int TJ, int KC, int NT>
         __global__ void __launch_bounds__(NT) hdiff_kernel(ST *__restrict__ out, const ST *__restrict__ in,

@@ -17,62 +17,11 @@
 #include <mutex>
 
 #include "common.hpp"
+#include "functors.hpp"
 #include "tuning.hpp"
 
 namespace gt355 {
     namespace {
-
-        constexpr double BET_M = 0.5, BET_P = 0.5; // vertical_advection_defs.hpp:14-16
-
-        template <class CT>
-        struct fwd_out {
-            CT c, d;
-        };
-
-        // u_forward_function::apply(first_level) (vertical_advection_dycore.cpp:83-96)
-        template <class CT>
-        __device__ __forceinline__ fwd_out<CT> forward_first(
-            CT dtr, CT w_p_i, CT w_p_ip, CT us_c, CT us_p, CT up, CT ut, CT uts) {
-            CT gcv = CT(.25) * (w_p_ip + w_p_i);
-            CT cs = gcv * CT(BET_M);
-            CT c = gcv * CT(BET_P);
-            CT b = dtr - c;
-            CT correction = -cs * (us_p - us_c);
-            CT d = ((dtr * up + ut) + uts) + correction;
-            CT divided = CT(1) / b;
-            return {c * divided, d * divided};
-        }
-
-        // u_forward_function::apply(modify<1,-1>) (vertical_advection_dycore.cpp:50-67)
-        template <class CT>
-        __device__ __forceinline__ fwd_out<CT> forward_body(CT dtr, CT w_c_i, CT w_c_ip, CT w_p_i, CT w_p_ip,
-            CT us_m, CT us_c, CT us_p, CT up, CT ut, CT uts, CT cc_m, CT dc_m) {
-            CT gav = -CT(.25) * (w_c_ip + w_c_i);
-            CT gcv = CT(.25) * (w_p_ip + w_p_i);
-            CT as = gav * CT(BET_M);
-            CT cs = gcv * CT(BET_M);
-            CT a = gav * CT(BET_P);
-            CT c = gcv * CT(BET_P);
-            CT b = (dtr - a) - c;
-            CT correction = -as * (us_m - us_c) - cs * (us_p - us_c);
-            CT d = ((dtr * up + ut) + uts) + correction;
-            CT divided = CT(1) / (b - cc_m * a);
-            return {c * divided, (d - dc_m * a) * divided};
-        }
-
-        // u_forward_function::apply(last_level) (vertical_advection_dycore.cpp:69-81); ccol is not written
-        template <class CT>
-        __device__ __forceinline__ CT forward_last(
-            CT dtr, CT w_c_i, CT w_c_ip, CT us_m, CT us_c, CT up, CT ut, CT uts, CT cc_m, CT dc_m) {
-            CT gav = -CT(.25) * (w_c_ip + w_c_i);
-            CT as = gav * CT(BET_M);
-            CT a = gav * CT(BET_P);
-            CT b = dtr - a;
-            CT correction = -as * (us_m - us_c);
-            CT d = ((dtr * up + ut) + uts) + correction;
-            CT divided = CT(1) / (b - cc_m * a);
-            return (d - dc_m * a) * divided;
-        }
 
         // One column per lane; ccol/dcol of levels 0..nk-2 go to scratch (k-major, column-minor).
         template <class ST, class CT>
