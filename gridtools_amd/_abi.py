@@ -103,6 +103,15 @@ SIGNATURES = {
     "gt355_event_synchronize": [_P],
     "gt355_event_elapsed_ms": [_P, _P, ctypes.POINTER(ctypes.c_float)],
     "gt355_mall_flush": [_P],
+    "gt355_tune_hdiff": [_I, _FP, _FP, _FP, _I, _I, _I, _I, _P, ctypes.POINTER(ctypes.c_int32),
+                         ctypes.POINTER(ctypes.c_double)],
+    "gt355_tune_save": [ctypes.c_char_p],
+    "gt355_tune_load": [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)],
+    "gt355_tune_clear": [],
+    "gt355_graph_begin": [_P],
+    "gt355_graph_end": [_P, ctypes.POINTER(_P)],
+    "gt355_graph_launch": [_P, _P],
+    "gt355_graph_destroy": [_P],
 }
 _RESTYPES = {"gt355_last_error": ctypes.c_char_p, "gt355_version": ctypes.c_char_p}
 

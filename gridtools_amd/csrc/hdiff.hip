@@ -16,6 +16,8 @@
 // -ffp-contract=off), so FP64 results are bit-identical to an IEEE CPU evaluation.
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "common.hpp"
 #include "functors.hpp"
 #include "tuning.hpp"
@@ -300,31 +302,65 @@ namespace gt355 {
             return check_launch("gt355_hdiff");
         }
 
+        // Launches one hdiff configuration; GT355_EUNSUP if that shape is not instantiated.
+        template <class ST, class CT>
+        int launch_cfg(const hdiff_cfg &c, const gt355_field *out, const gt355_field *in, const gt355_field *coeff,
+            int ni, int nj, int nk, hipStream_t stream) {
+            if (c.impl == 1) {
+#define GT355_HSWEEP(JB, P)                                                                     \
+    if (c.a == JB && c.b == P)                                                                  \
+        return launch_hdiff_sweep<ST, CT, JB, P>(out, in, coeff, ni, nj, nk, stream);
+                GT355_HDIFF_SWEEPS(GT355_HSWEEP)
+#undef GT355_HSWEEP
+            } else {
+#define GT355_HDIFF_CASE(TI, TJ, KC)                                                            \
+    if (c.a == TI && c.b == TJ && c.c == KC)                                                    \
+        return launch_hdiff<ST, CT, TI, TJ, KC>(out, in, coeff, ni, nj, nk, stream);
+                GT355_HDIFF_TILES(GT355_HDIFF_CASE)
+#undef GT355_HDIFF_CASE
+            }
+            set_error("gt355_hdiff: configuration impl=%d (%d,%d,%d) is not instantiated", c.impl, c.a, c.b, c.c);
+            return GT355_EUNSUP;
+        }
+
         template <class ST, class CT>
         int dispatch_hdiff(const gt355_field *out, const gt355_field *in, const gt355_field *coeff, int ni, int nj,
             int nk, hipStream_t stream) {
-            // measured at 256x256x80: FP64 storage is fastest with the LDS tile (24.9 us vs 27.1 us), FP32 storage
-            // with the register sweep (17.1 us vs 21.8 us)
-            if (tune_param("hdiff_impl", sizeof(ST) == 8 ? 0 : 1) == 1) {
-                const int jb = tune_param("hdiff_jb", 16), pd = tune_param("hdiff_p", 4);
-#define GT355_HSWEEP(JB, P)                                                                     \
-    if (jb == JB && pd == P)                                                                    \
-        return launch_hdiff_sweep<ST, CT, JB, P>(out, in, coeff, ni, nj, nk, stream);
-                GT355_HSWEEP(8, 4)
-                GT355_HSWEEP(16, 4)
-                GT355_HSWEEP(16, 8)
-                GT355_HSWEEP(32, 4)
-                GT355_HSWEEP(32, 8)
-#undef GT355_HSWEEP
-                return launch_hdiff_sweep<ST, CT, 16, 4>(out, in, coeff, ni, nj, nk, stream);
-            }
-            hdiff_tile t = hdiff_tile_for(sizeof(ST), sizeof(CT), ni, nj, nk);
-#define GT355_HDIFF_CASE(TI, TJ, KC)                                                            \
-    if (t.ti == TI && t.tj == TJ && t.kc == KC)                                                 \
-        return launch_hdiff<ST, CT, TI, TJ, KC>(out, in, coeff, ni, nj, nk, stream);
-            GT355_HDIFF_TILES(GT355_HDIFF_CASE)
-#undef GT355_HDIFF_CASE
-            return launch_hdiff<ST, CT, 64, 8, 8>(out, in, coeff, ni, nj, nk, stream);
+            return launch_cfg<ST, CT>(hdiff_cfg_for(sizeof(ST), sizeof(CT), ni, nj, nk), out, in, coeff, ni, nj, nk,
+                stream);
+        }
+
+        // Every instantiated configuration, in the order the tuner tries them.
+        std::vector<hdiff_cfg> hdiff_candidates() {
+            std::vector<hdiff_cfg> v;
+#define GT355_ADD_TILE(TI, TJ, KC) v.push_back({0, TI, TJ, KC});
+            GT355_HDIFF_TILES(GT355_ADD_TILE)
+#undef GT355_ADD_TILE
+#define GT355_ADD_SWEEP(JB, P) v.push_back({1, JB, P, 0});
+            GT355_HDIFF_SWEEPS(GT355_ADD_SWEEP)
+#undef GT355_ADD_SWEEP
+            return v;
+        }
+
+        int check_hdiff_args(int precision, const gt355_field *out, const gt355_field *in, const gt355_field *coeff,
+            int ni, int nj, int nk) {
+            GT355_CHECK(ni >= 0 && nj >= 0 && nk >= 0, "gt355_hdiff: negative size");
+            GT355_CHECK(out && in && coeff, "gt355_hdiff: null field");
+            int dtype = in->dtype;
+            GT355_CHECK(dtype == GT355_F64 || dtype == GT355_F32, "gt355_hdiff: dtype %d", dtype);
+            GT355_CHECK(precision == GT355_NATIVE || (precision == GT355_WIDEN && dtype == GT355_F32),
+                "gt355_hdiff: precision %d with dtype %d", precision, dtype);
+            if (ni == 0 || nj == 0 || nk == 0)
+                return GT355_OK;
+            int rc;
+            if ((rc = check_field(in, "in", dtype, -2, ni + 2, -2, nj + 2, 0, nk)))
+                return rc;
+            if ((rc = check_field(coeff, "coeff", dtype, 0, ni, 0, nj, 0, nk)))
+                return rc;
+            if ((rc = check_field(out, "out", dtype, 0, ni, 0, nj, 0, nk)))
+                return rc;
+            GT355_CHECK(out->origin != in->origin, "gt355_hdiff: out must not alias in");
+            return GT355_OK;
         }
     } // namespace
 } // namespace gt355
@@ -333,28 +369,75 @@ using namespace gt355;
 
 extern "C" int gt355_hdiff(int precision, const gt355_field *out, const gt355_field *in, const gt355_field *coeff,
     int ni, int nj, int nk, void *stream) {
-    GT355_CHECK(ni >= 0 && nj >= 0 && nk >= 0, "gt355_hdiff: negative size");
-    GT355_CHECK(out && in && coeff, "gt355_hdiff: null field");
-    if (ni == 0 || nj == 0 || nk == 0)
-        return GT355_OK;
-    int dtype = in->dtype;
-    GT355_CHECK(dtype == GT355_F64 || dtype == GT355_F32, "gt355_hdiff: dtype %d", dtype);
-    GT355_CHECK(precision == GT355_NATIVE || (precision == GT355_WIDEN && dtype == GT355_F32),
-        "gt355_hdiff: precision %d with dtype %d", precision, dtype);
-    int rc;
-    if ((rc = check_field(in, "in", dtype, -2, ni + 2, -2, nj + 2, 0, nk)))
+    int rc = check_hdiff_args(precision, out, in, coeff, ni, nj, nk);
+    if (rc || ni == 0 || nj == 0 || nk == 0)
         return rc;
-    if ((rc = check_field(coeff, "coeff", dtype, 0, ni, 0, nj, 0, nk)))
-        return rc;
-    if ((rc = check_field(out, "out", dtype, 0, ni, 0, nj, 0, nk)))
-        return rc;
-    GT355_CHECK(out->origin != in->origin, "gt355_hdiff: out must not alias in");
     hipStream_t s = (hipStream_t)stream;
-    if (dtype == GT355_F64)
+    if (in->dtype == GT355_F64)
         return dispatch_hdiff<double, double>(out, in, coeff, ni, nj, nk, s);
     if (precision == GT355_WIDEN)
         return dispatch_hdiff<float, double>(out, in, coeff, ni, nj, nk, s);
     return dispatch_hdiff<float, float>(out, in, coeff, ni, nj, nk, s);
+}
+
+extern "C" int gt355_tune_hdiff(int precision, const gt355_field *out, const gt355_field *in, const gt355_field *coeff,
+    int ni, int nj, int nk, int reps, void *stream, int32_t best[4], double *best_us) {
+    int rc = check_hdiff_args(precision, out, in, coeff, ni, nj, nk);
+    if (rc)
+        return rc;
+    GT355_CHECK(ni > 0 && nj > 0 && nk > 0, "gt355_tune_hdiff: empty domain");
+    GT355_CHECK(reps >= 1 && reps <= 10000, "gt355_tune_hdiff: reps %d", reps);
+    hipStream_t s = (hipStream_t)stream;
+    const int st = in->dtype == GT355_F64 ? 8 : 4, ct = (in->dtype == GT355_F64 || precision == GT355_WIDEN) ? 8 : 4;
+    auto launch = [&](const hdiff_cfg &c) {
+        if (st == 8)
+            return launch_cfg<double, double>(c, out, in, coeff, ni, nj, nk, s);
+        if (ct == 8)
+            return launch_cfg<float, double>(c, out, in, coeff, ni, nj, nk, s);
+        return launch_cfg<float, float>(c, out, in, coeff, ni, nj, nk, s);
+    };
+    hipEvent_t e0, e1;
+    GT355_HIP(hipEventCreate(&e0));
+    GT355_HIP(hipEventCreate(&e1));
+    hdiff_cfg win{};
+    double win_us = 1e30;
+    for (const hdiff_cfg &c : hdiff_candidates()) {
+        if ((rc = launch(c)))
+            break;
+        (void)hipEventRecord(e0, s);
+        for (int r = 0; r < reps && !rc; ++r)
+            rc = launch(c);
+        (void)hipEventRecord(e1, s);
+        if (rc)
+            break;
+        hipError_t err = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (err == hipSuccess)
+            err = hipEventElapsedTime(&ms, e0, e1);
+        if (err != hipSuccess) {
+            rc = hip_fail(err, "gt355_tune_hdiff: timing");
+            break;
+        }
+        const double us = 1e3 * ms / reps;
+        if (us < win_us) {
+            win_us = us;
+            win = c;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc)
+        return rc;
+    record_hdiff_cfg(st, ct, ni, nj, nk, win, win_us);
+    if (best) {
+        best[0] = win.impl;
+        best[1] = win.a;
+        best[2] = win.b;
+        best[3] = win.c;
+    }
+    if (best_us)
+        *best_us = win_us;
+    return GT355_OK;
 }
 
 extern "C" int gt355_laplacian(int precision, const gt355_field *out, const gt355_field *in, int ni, int nj, int nk,

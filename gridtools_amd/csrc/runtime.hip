@@ -298,6 +298,37 @@ int gt355_event_elapsed_ms(void *start, void *stop, float *ms) {
     return GT355_OK;
 }
 
+int gt355_graph_begin(void *stream) {
+    GT355_CHECK(stream != nullptr, "gt355_graph_begin: capture needs a non-null stream");
+    GT355_HIP(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal));
+    return GT355_OK;
+}
+
+int gt355_graph_end(void *stream, void **graph_exec) {
+    GT355_CHECK(stream != nullptr && graph_exec != nullptr, "gt355_graph_end: null argument");
+    hipGraph_t g = nullptr;
+    GT355_HIP(hipStreamEndCapture((hipStream_t)stream, &g));
+    hipGraphExec_t e = nullptr;
+    hipError_t err = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (err != hipSuccess)
+        return hip_fail(err, "gt355_graph_end: hipGraphInstantiate");
+    *graph_exec = e;
+    return GT355_OK;
+}
+
+int gt355_graph_launch(void *graph_exec, void *stream) {
+    GT355_CHECK(graph_exec != nullptr, "gt355_graph_launch: null graph");
+    GT355_HIP(hipGraphLaunch((hipGraphExec_t)graph_exec, (hipStream_t)stream));
+    return GT355_OK;
+}
+
+int gt355_graph_destroy(void *graph_exec) {
+    if (graph_exec)
+        GT355_HIP(hipGraphExecDestroy((hipGraphExec_t)graph_exec));
+    return GT355_OK;
+}
+
 int gt355_mall_flush(void *stream) {
     // 1 GiB: four times the 256 MB Infinity Cache of MI355X.
     const size_t bytes = size_t(1) << 30;

@@ -1,25 +1,29 @@
-// Tile selection for the tiled kernels.  Defaults come from measurements on MI355X; gt355_tune_* (the
-// autotuner) can override them per problem shape.
+// Launch-configuration selection and the autotuner's table.
+//
+// The defaults were measured on MI355X at 256x256x80.  gt355_tune_hdiff (hdiff.hip) times every instantiated
+// configuration for one problem shape and records the fastest here; gt355_tune_save / gt355_tune_load persist the
+// table as text, one entry per line:
+//   hdiff <storage bytes> <compute bytes> <ni> <nj> <nk> <impl> <a> <b> <c> <microseconds>
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+
+#include "common.hpp"
 #include "tuning.hpp"
 
-#include <cstdlib>
-#include <string>
-
 namespace gt355 {
-
-    hdiff_tile hdiff_tile_for(int st_size, int ct_size, int ni, int nj, int nk) {
-        (void)st_size;
-        (void)ct_size;
-        (void)ni;
-        (void)nj;
-        (void)nk;
-        // 64x8 tiles with 5-level chunks: 2048 workgroups at 256x256x80, one full round at 8 waves per SIMD
-        hdiff_tile t{64, 8, 5};
-        t.ti = tune_param("hdiff_ti", t.ti);
-        t.tj = tune_param("hdiff_tj", t.tj);
-        t.kc = tune_param("hdiff_kc", t.kc);
-        return t;
-    }
+    namespace {
+        using tune_key = std::tuple<int, int, int, int, int>;
+        struct entry {
+            hdiff_cfg cfg;
+            double us;
+        };
+        std::mutex g_tune_mutex;
+        std::map<tune_key, entry> g_hdiff;
+    } // namespace
 
     int tune_param(const char *name, int dflt) {
         std::string key = std::string("GT355_TUNE_") + name;
@@ -27,4 +31,78 @@ namespace gt355 {
         return v ? std::atoi(v) : dflt;
     }
 
+    hdiff_cfg hdiff_cfg_for(int st_size, int ct_size, int ni, int nj, int nk) {
+        // measured defaults at 256x256x80: FP64 storage is fastest with the LDS tile 64x8 with 5-level chunks (2048
+        // workgroups, one full round at 8 waves per SIMD; 24.9 us), FP32 storage with the register sweep (17.1 us)
+        hdiff_cfg c = st_size == 8 ? hdiff_cfg{0, 64, 8, 5} : hdiff_cfg{1, 16, 4, 0};
+        {
+            std::lock_guard<std::mutex> lock(g_tune_mutex);
+            auto it = g_hdiff.find(tune_key{st_size, ct_size, ni, nj, nk});
+            if (it != g_hdiff.end())
+                c = it->second.cfg;
+        }
+        const int impl = tune_param("hdiff_impl", c.impl);
+        if (impl != c.impl)
+            c = impl == 1 ? hdiff_cfg{1, 16, 4, 0} : hdiff_cfg{0, 64, 8, 5};
+        if (c.impl == 1) {
+            c.a = tune_param("hdiff_jb", c.a);
+            c.b = tune_param("hdiff_p", c.b);
+        } else {
+            c.a = tune_param("hdiff_ti", c.a);
+            c.b = tune_param("hdiff_tj", c.b);
+            c.c = tune_param("hdiff_kc", c.c);
+        }
+        return c;
+    }
+
+    void record_hdiff_cfg(int st_size, int ct_size, int ni, int nj, int nk, hdiff_cfg cfg, double us) {
+        std::lock_guard<std::mutex> lock(g_tune_mutex);
+        g_hdiff[tune_key{st_size, ct_size, ni, nj, nk}] = entry{cfg, us};
+    }
+
 } // namespace gt355
+
+using namespace gt355;
+
+extern "C" int gt355_tune_save(const char *path) {
+    GT355_CHECK(path != nullptr, "gt355_tune_save: null path");
+    FILE *f = std::fopen(path, "w");
+    GT355_CHECK(f != nullptr, "gt355_tune_save: cannot open %s", path);
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    for (const auto &kv : g_hdiff) {
+        const auto &k = kv.first;
+        const auto &e = kv.second;
+        std::fprintf(f, "hdiff %d %d %d %d %d %d %d %d %d %.3f\n", std::get<0>(k), std::get<1>(k), std::get<2>(k),
+            std::get<3>(k), std::get<4>(k), e.cfg.impl, e.cfg.a, e.cfg.b, e.cfg.c, e.us);
+    }
+    std::fclose(f);
+    return GT355_OK;
+}
+
+extern "C" int gt355_tune_load(const char *path, int *entries) {
+    GT355_CHECK(path != nullptr, "gt355_tune_load: null path");
+    FILE *f = std::fopen(path, "r");
+    GT355_CHECK(f != nullptr, "gt355_tune_load: cannot open %s", path);
+    char kind[32];
+    int st, ct, ni, nj, nk, impl, a, b, c;
+    double us;
+    int n = 0;
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    while (std::fscanf(f, "%31s %d %d %d %d %d %d %d %d %d %lf", kind, &st, &ct, &ni, &nj, &nk, &impl, &a, &b, &c,
+               &us) == 11) {
+        if (std::string(kind) == "hdiff") {
+            g_hdiff[tune_key{st, ct, ni, nj, nk}] = entry{hdiff_cfg{impl, a, b, c}, us};
+            ++n;
+        }
+    }
+    std::fclose(f);
+    if (entries)
+        *entries = n;
+    return GT355_OK;
+}
+
+extern "C" int gt355_tune_clear(void) {
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    g_hdiff.clear();
+    return GT355_OK;
+}

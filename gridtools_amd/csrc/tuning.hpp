@@ -5,10 +5,6 @@
 
 namespace gt355 {
 
-    struct hdiff_tile {
-        int ti, tj, kc;
-    };
-
     // Every tile shape instantiated for hdiff: X(TI, TJ, KC).  TI*TJ must be a multiple of 256.
 #define GT355_HDIFF_TILES(X)                                                                                     \
     X(64, 4, 4) X(64, 4, 5) X(64, 4, 8) X(64, 4, 10) X(64, 4, 16)                                                  \
@@ -18,9 +14,19 @@ namespace gt355 {
     X(256, 1, 8) X(256, 1, 16)                                                                                     \
     X(32, 8, 8) X(32, 16, 8)
 
-    // Tile for an hdiff launch: the tuned entry for (storage size, compute size, ni, nj, nk) if the autotuner
-    // recorded one, otherwise a heuristic default.
-    hdiff_tile hdiff_tile_for(int st_size, int ct_size, int ni, int nj, int nk);
+    // Every register-sweep shape instantiated for hdiff: X(JB, P).
+#define GT355_HDIFF_SWEEPS(X) X(8, 4) X(16, 4) X(16, 8) X(32, 4) X(32, 8)
+
+    // One hdiff launch configuration: impl 0 = LDS tile (a, b, c) = (TI, TJ, KC); impl 1 = register sweep
+    // (a, b) = (JB, P).
+    struct hdiff_cfg {
+        int impl, a, b, c;
+    };
+
+    // The configuration for an hdiff launch, in this order: environment overrides (GT355_TUNE_hdiff_*), the entry
+    // the autotuner recorded for (storage size, compute size, ni, nj, nk), the measured default for the storage type.
+    hdiff_cfg hdiff_cfg_for(int st_size, int ct_size, int ni, int nj, int nk);
+    void record_hdiff_cfg(int st_size, int ct_size, int ni, int nj, int nk, hdiff_cfg cfg, double us);
 
     // A named integer tuning knob: the environment variable GT355_TUNE_<name> if set, otherwise dflt.
     int tune_param(const char *name, int dflt);

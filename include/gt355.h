@@ -183,6 +183,30 @@ int gt355_event_record(void *event, void *stream);
 int gt355_event_synchronize(void *event);
 int gt355_event_elapsed_ms(void *start, void *stop, float *ms);
 
+/* ------------------------------------------------------------------ autotuner
+ * No counterpart in the reference, whose tile shape is a backend template parameter fixed at compile time
+ * (stencil/gpu/entry_point.hpp:147-150, gpu<64,8,1> by default).  gt355_tune_hdiff times every instantiated hdiff
+ * configuration (LDS tiles TI x TJ x KC and register sweeps JB/P) on the given fields, `reps` launches each on
+ * `stream`, records the fastest for (storage type, compute type, ni, nj, nk) so later gt355_hdiff calls with that
+ * shape use it, and returns it in best[4] = {impl, a, b, c} with its mean time.  The fields' contents are
+ * overwritten (out) and read (in, coeff) as by gt355_hdiff.  The table persists across processes through
+ * gt355_tune_save / gt355_tune_load (text, one entry per line). */
+int gt355_tune_hdiff(int precision, const gt355_field *out, const gt355_field *in, const gt355_field *coeff, int ni,
+    int nj, int nk, int reps, void *stream, int32_t best[4], double *best_us);
+int gt355_tune_save(const char *path);
+int gt355_tune_load(const char *path, int *entries);
+int gt355_tune_clear(void);
+
+/* ------------------------------------------------------------------ graphs
+ * Stream capture of a launch sequence into a hipGraph (the reference has no equivalent: it launches every
+ * multistage eagerly on stream 0, stencil/gpu/launch_kernel.hpp:161).  Capture needs a non-null stream.  No entry
+ * point records a memset into a captured graph; entry points that may allocate (the deep-column vadv scratch
+ * pool) must have run once, outside capture, with the largest shape before a capture that uses them. */
+int gt355_graph_begin(void *stream);
+int gt355_graph_end(void *stream, void **graph_exec);
+int gt355_graph_launch(void *graph_exec, void *stream);
+int gt355_graph_destroy(void *graph_exec);
+
 /* ------------------------------------------------------------------ measurement */
 /* Writes (and reads back) a device buffer larger than the MI355X MALL so that the next kernel reads
  * HBM, not Infinity Cache.  Used between timed repetitions, outside the timed region. */

@@ -90,3 +90,20 @@ def test_hdiff_bounds_check_rejects_small_halo():
     rc = lib.gt355_hdiff(0, ctypes.byref(f), ctypes.byref(f), ctypes.byref(f), 64, 64, 60, None)
     assert rc == -1
     assert "outside bounds" in _abi.last_error()
+
+
+def test_tune_table_save_load_roundtrip(tmp_path):
+    """The autotuner's table persists as text and reloads (no device needed)."""
+    from gridtools_amd import tune
+
+    tune.clear()
+    src = tmp_path / "in.txt"
+    src.write_text("hdiff 8 8 256 256 80 0 64 8 5 24.900\nhdiff 4 8 256 256 80 1 16 4 0 17.100\n")
+    assert tune.load(src) == 2
+    out = tmp_path / "out.txt"
+    tune.save(out)
+    lines = sorted(out.read_text().strip().splitlines())
+    assert lines == sorted(["hdiff 8 8 256 256 80 0 64 8 5 24.900", "hdiff 4 8 256 256 80 1 16 4 0 17.100"])
+    tune.clear()
+    tune.save(out)
+    assert out.read_text() == ""

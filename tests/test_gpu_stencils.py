@@ -240,3 +240,21 @@ def test_repeated_launches_are_deterministic(gpu):
     for _ in range(3):
         stencils.hdiff(out, inp, coeff, grid)
     assert np.array_equal(out.padded_numpy(), first)
+
+
+def test_autotuner_picks_a_config_and_dispatch_stays_exact(gpu, oracle, tmp_path):
+    from gridtools_amd import stencils, tune
+
+    tune.clear()
+    ni, nj, nk = 200, 120, 12
+    inp, coeff, out, grid = hdiff_case(gpu, ni, nj, nk, np.float64)
+    cfg, us = tune.hdiff(out, inp, coeff, grid, reps=3)
+    print(f"[tune] {ni}x{nj}x{nk} fp64 -> {cfg} {us:.2f} us", flush=True)
+    assert cfg["impl"] in ("lds-tile", "register-sweep") and us > 0
+    stencils.hdiff(out, inp, coeff, grid)  # now dispatched with the tuned configuration
+    got = out.padded_numpy()[:nk, 2:2 + nj, 2:2 + ni]
+    assert np.array_equal(got, oracle_hdiff(oracle, inp, coeff, 2, ni, nj, nk, "native"))
+    path = tmp_path / "tune.txt"
+    tune.save(path)
+    assert f"hdiff 8 8 {ni} {nj} {nk}" in path.read_text()
+    tune.clear()
