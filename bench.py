@@ -146,6 +146,160 @@ def copy_leg(gt, steps, warmup, nsets):
     return {"mean_us": round(ms * 1e3, 2), "gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
 
 
+def _event_time(fn, n):
+    """Mean ms of n calls of fn() bracketed by one event pair on the null stream (synchronised both sides)."""
+    from gridtools_amd.timing import Event, synchronize
+
+    synchronize()
+    start, stop = Event(), Event()
+    start.record()
+    for s in range(n):
+        fn(s)
+    stop.record()
+    stop.synchronize()
+    synchronize()
+    return start.elapsed_ms(stop) / n
+
+
+def compose_leg(gt, steps, warmup, n=512, nk=80, nsets=2):
+    """The precision form on the hdiff -> vadv composition at n x n x nk: one fused launch per step
+    (gt355_hdiff_vadv) with FP32 storage and FP64 compute, against the same composition in FP64.  Errors: each
+    output of the FP32 form against the FP64 path run on the same FP32-rounded inputs (the FP64 path equals the
+    FP64 oracle bit for bit), max and RMS, relative to the field's max |value| and RMS."""
+    from gridtools_amd import inputs, stencils
+
+    h = 3
+    grid = gt.make_grid(gt.halo_descriptor.symmetric(h, n + 2 * h), gt.halo_descriptor.symmetric(h, n + 2 * h), nk)
+
+    def make(dtype, seed):
+        b = gt.builder().type(dtype).dimensions(n + 2 * h, n + 2 * h, nk).halos(h, h, 0)
+        f = {k: b.uniform(seed, inputs.FIELD_IDS[k], -1.0, 1.0).build()
+             for k in ("in", "utens_stage", "u_pos", "utens")}
+        f["coeff"] = b.uniform(seed, inputs.FIELD_IDS["coeff"], 0.0, 0.25).build()
+        f["u_stage"] = b.value(0).build()
+        f["wcon"] = gt.builder().type(dtype).dimensions(n + 2 * h, n + 2 * h, nk + 1).halos(h, h, 0) \
+            .uniform(seed, inputs.FIELD_IDS["wcon"], -0.5, 0.5).build()
+        return f
+
+    def run(f, precision):
+        stencils.hdiff_vadv(f["in"], f["coeff"], f["u_stage"], f["utens_stage"], f["wcon"], f["u_pos"], f["utens"],
+                            0.15, grid, precision=precision)
+
+    res = {}
+    for name, dtype, prec in (("fp64", np.float64, gt.NATIVE), ("fp32_form", np.float32, gt.WIDEN)):
+        sets = [make(dtype, SEED + s) for s in range(nsets)]
+        for s in range(warmup):
+            run(sets[s % nsets], prec)
+        ms = _event_time(lambda s: run(sets[s % nsets], prec), steps)
+        res[name] = ms
+        del sets
+    pts = n * n * nk
+    # errors on one fresh input set: FP32 form vs the FP64 path on the widened FP32 inputs
+    f32 = make(np.float32, SEED)
+    f64 = {}
+    for k, v in f32.items():
+        f64[k] = gt.builder().type(np.float64).dimensions(*v.lengths).halos(*v.halos).build()
+        stencils.convert(f64[k], v)
+    run(f32, gt.WIDEN)
+    run(f64, gt.NATIVE)
+    gt_sync()
+    box = np.s_[:nk, h:h + n, h:h + n]
+    errors = {}
+    for k in ("u_stage", "utens_stage"):
+        a = f32[k].padded_numpy()[box].astype(np.float64)
+        r = f64[k].padded_numpy()[box]
+        d = a - r
+        errors[k] = {"max_rel": float(np.max(np.abs(d)) / np.max(np.abs(r))),
+                     "rms_rel": float(np.sqrt(np.mean(d * d)) / np.sqrt(np.mean(r * r)))}
+    # algorithmic bytes per point of the fused launch: in, coeff, utens_stage, wcon, u_pos, utens read, u_stage and
+    # utens_stage written (8 accesses)
+    out = {"workload": f"hdiff->vadv composition {n}x{n}x{nk}, one fused launch per step"}
+    for name, esize in (("fp64", 8), ("fp32_form", 4)):
+        ms = res[name]
+        gbs = 8 * esize * pts / (ms * 1e-3) / 1e9
+        out[name] = {"mean_us": round(ms * 1e3, 2), "gpts_per_s": round(pts / (ms * 1e-3) / 1e9, 3),
+                     "achieved_gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+    out["speedup_fp32_form_vs_fp64"] = round(res["fp64"] / res["fp32_form"], 3)
+    out["errors_vs_fp64"] = errors
+    return out
+
+
+def gt_sync():
+    from gridtools_amd.timing import synchronize
+
+    synchronize()
+
+
+def batch_leg(gt, nsteps=10, reps=3, seed=2024):
+    """The batching form: 64 domains, ni, nj ~ U{48..384}, nk ~ U{40..137} (seeded), laplacian + hdiff + vadv in
+    FP32 storage / FP64 compute, one persistent launch per timestep; the grid size from the batch tuner; a graph
+    of nsteps timesteps (u_stage -> in swap) replayed `reps` times."""
+    from gridtools_amd.batch import BatchedTimesteps, draw_sizes
+    from gridtools_amd.timing import Event, Stream
+
+    sizes = draw_sizes(seed)
+    bt = BatchedTimesteps(sizes, seed=seed)
+    w0, us0 = bt.batches[0].tune(reps=3)
+    w1, us1 = bt.batches[1].tune(reps=3)
+    st = Stream()
+    g = bt.capture(nsteps, st)
+    g.launch(st)  # warm-up replay
+    st.synchronize()
+    start, stop = Event(), Event()
+    start.record(st)
+    for _ in range(reps):
+        g.launch(st)
+    stop.record(st)
+    stop.synchronize()
+    ms = start.elapsed_ms(stop) / reps
+    pts = bt.points()
+    items, grid = bt.batches[0].info()
+    # algorithmic bytes per point and timestep (FP32 storage): in, coeff, utens_stage, wcon, u_pos, utens read;
+    # lap, u_stage, utens_stage written
+    gbs = 9 * 4 * pts * nsteps / (ms * 1e-3) / 1e9
+    return {
+        "workload": f"64 domains (seed {seed}), {pts / 1e6:.1f} M points, {nsteps}-timestep graph",
+        "ms_per_graph": round(ms, 4),
+        "ms_per_timestep": round(ms / nsteps, 4),
+        "gpts_per_s": round(pts * nsteps / (ms * 1e-3) / 1e9, 3),
+        "achieved_gbs": round(gbs, 1),
+        "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+        "items": items,
+        "tuned_wg_per_cu": [w0, w1],
+        "tune_us": [round(us0, 1), round(us1, 1)],
+        "workgroups": grid,
+    }
+
+
+def large_leg(gt, nsteps=20, n=1536, nk=80):
+    """Large single domain, FP32 storage / FP64 compute, n x n x nk, nsteps timesteps of composition + boundary
+    conditions (periodic i, zero-gradient j, 3-point halo) captured as one graph."""
+    from gridtools_amd.timestep import HdiffVadvTimestepper
+    from gridtools_amd.timing import Event, Stream
+
+    ts = HdiffVadvTimestepper(n, n, nk, seed=SEED)
+    st = Stream()
+    g = ts.capture(nsteps, st)
+    g.launch(st)
+    st.synchronize()
+    start, stop = Event(), Event()
+    start.record(st)
+    g.launch(st)
+    stop.record(st)
+    stop.synchronize()
+    ms = start.elapsed_ms(stop)
+    pts = ts.points()
+    gbs = 8 * 4 * pts * nsteps / (ms * 1e-3) / 1e9  # fused launch traffic only (halo fill excluded)
+    return {
+        "workload": f"{n}x{n}x{nk} FP32 storage / FP64 compute, {nsteps} timesteps (compose + halo BCs) in a graph",
+        "ms_total": round(ms, 3),
+        "ms_per_timestep": round(ms / nsteps, 4),
+        "gpts_per_s": round(pts * nsteps / (ms * 1e-3) / 1e9, 3),
+        "achieved_gbs_compose": round(gbs, 1),
+        "hbm_frac_compose": round(gbs / HBM_PEAK_GBS, 4),
+    }
+
+
 def cpu_baseline(budget_s=12.0):
     """The oracle port (OpenMP, same arithmetic) on this host, hdiff+vadv FP64 256x256x80, bounded sample."""
     from gridtools_amd import inputs
@@ -189,9 +343,11 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--sets", type=int, default=4)
-    ap.add_argument("--legs", default="fp64,fp32,copy,cpu",
-                    help="comma list of legs: fp64 (always run), fp32 (precision form), copy (bandwidth yardstick), "
-                         "cpu (CPU baseline)")
+    ap.add_argument("--legs", default="fp64,fp32,copy,compose,batch,large,cpu",
+                    help="comma list of legs: fp64 (always run), fp32 (precision form per kernel), copy (bandwidth "
+                         "yardstick), compose (precision form on the fused composition, 512x512x80), batch (64 "
+                         "uneven domains, persistent launch, graph of 10 timesteps), large (1536x1536x80, 20 "
+                         "timesteps with boundary conditions), cpu (CPU baseline)")
     args = ap.parse_args()
     legs = set(args.legs.split(","))
     args.no_fp32 = "fp32" not in legs
@@ -231,6 +387,18 @@ def main():
         del sets32
 
     copy = copy_leg(gt, args.steps, args.warmup, args.sets) if "copy" in legs else None
+    compose = None
+    if "compose" in legs:
+        print("[bench] compose leg", file=sys.stderr, flush=True)
+        compose = compose_leg(gt, max(10, args.steps // 4), min(args.warmup, 10))
+    batch = None
+    if "batch" in legs:
+        print("[bench] batch leg", file=sys.stderr, flush=True)
+        batch = batch_leg(gt)
+    large = None
+    if "large" in legs:
+        print("[bench] large-domain leg", file=sys.stderr, flush=True)
+        large = large_leg(gt)
 
     cpu = None
     if not args.no_cpu:
@@ -258,6 +426,9 @@ def main():
         "kernels": {"hdiff_fp64": hd64, "vadv_fp64": va64},
         "fp32_form": fp32,
         "copy_fp64": copy,
+        "precision_form_composition": compose,
+        "batching_form": batch,
+        "large_domain": large,
         "roofline": {
             "kernel": "vadv FP64 (vadv_*_kernel<double,double>)",
             "bound": "hbm",

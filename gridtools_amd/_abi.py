@@ -112,6 +112,11 @@ SIGNATURES = {
     "gt355_graph_end": [_P, ctypes.POINTER(_P)],
     "gt355_graph_launch": [_P, _P],
     "gt355_graph_destroy": [_P],
+    "gt355_batch_create": [_I, _P, _I, _I, ctypes.POINTER(_P)],
+    "gt355_batch_run": [_P, ctypes.c_double, _P],
+    "gt355_batch_info": [_P, ctypes.POINTER(_I), ctypes.POINTER(_I)],
+    "gt355_batch_destroy": [_P],
+    "gt355_batch_tune": [_P, _I, _P, ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_double)],
 }
 _RESTYPES = {"gt355_last_error": ctypes.c_char_p, "gt355_version": ctypes.c_char_p}
 

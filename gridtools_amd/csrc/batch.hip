@@ -1,0 +1,405 @@
+// Batching form: many independent domains of uneven ij x k sizes in one persistent launch.
+//
+// The reference batches only through expandable_run, which replicates a spec over a std::vector of fields that
+// all share one grid (stencil/frontend/expandable_run.hpp:80-183); domains of different sizes need separate runs.
+// Here the host lists every (domain, 64x4 column tile) as a work item, longest columns first, and a fixed grid of
+// workgroups strides through the list (item = blockIdx.x + n * gridDim.x; no counters, so the launch can be
+// captured in a graph and replayed).  Each item computes, for its 256 columns and every level of its domain:
+//   lap     = laplacian(in)                        (laplacian.cpp:20-29; the lap ij-cache of hdiff, stored)
+//   u_stage = hdiff(in, coeff)                     (horizontal_diffusion.cpp:35-107)
+//   vadv(utens_stage, u_stage, wcon, u_pos, utens) (vertical_advection_dycore.cpp:32-150), u_stage taken from the
+//                                                 diffusion in the compute type, not re-read
+// The k depth is a runtime value (up to 144 levels), so ccol/dcol go to the workgroup's own slot of a scratch
+// buffer (k-major, coalesced; written and read back by the same workgroup, so they stay in L2 / Infinity Cache).
+// Levels are processed in blocks of D with static ring slots inside a block; loads are unconditional (clamped to
+// the domain's top level) and stores past the top land on the top level before its own value does.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "common.hpp"
+#include "functors.hpp"
+#include "tuning.hpp"
+
+namespace gt355 {
+    namespace {
+        constexpr int BTI = 64, BTJ = 4, BMAXK = 144;
+        // forward steps run up to D-1 levels past a column's top; scratch rows = deepest nk + BPAST
+        constexpr int BPAST = 8;
+
+        struct bdom {
+            const void *in, *coeff, *wcon, *u_pos, *utens;
+            void *lap, *u_stage, *utens_stage;
+            int64_t sj, sk;   // shared layout of every field but wcon (origins at grid point (0,0,0))
+            int64_t wsj, wsk; // wcon layout
+            int32_t ni, nj, nk, tiles_i;
+        };
+
+        struct bitem {
+            int32_t dom, ti, tj, pad;
+        };
+
+        // field pointers read from the domain table are generic; marked global they load and store with global_*
+        // instructions (counted by vmcnt alone) instead of flat_* (which also hold lgkmcnt, as LDS does)
+        template <class T>
+        using gptr = __attribute__((address_space(1))) T *;
+
+        template <class ST, class CT, int D>
+        __global__ void __launch_bounds__(256) batch_kernel(const bdom *__restrict__ doms,
+            const bitem *__restrict__ items, int nitems, CT dtr, CT *__restrict__ scratch, int rows, int nxcd) {
+            static_assert(D % 2 == 0, "the LDS buffer parity of a block step is static only for even D");
+            constexpr int IW = BTI + 4, IH = BTJ + 4, LW = BTI + 2;
+            constexpr int NIN = IW * IH, IN_PER = (NIN + 255) / 256, NLAP = 512;
+            __shared__ ST s_in[2][IN_PER * 256];
+            __shared__ CT s_lap[NLAP];
+            const int tid = threadIdx.x, x = tid & 63, y = tid >> 6;
+            static_assert(D <= BPAST, "scratch rows cover the steps past the top");
+            CT *sc = scratch + (int64_t)blockIdx.x * (2 * rows * 256); // this workgroup's ccol/dcol slot
+            CT *sd = sc + (int64_t)rows * 256;
+
+            // consecutive items (neighbouring tiles, sharing rim rows) go to workgroups of one XCD
+            const int wid = xcd_remap(blockIdx.x, gridDim.x, nxcd);
+            for (int it = wid; it < nitems; it += gridDim.x) {
+                const bitem item = items[it];
+                const bdom dm = doms[item.dom];
+                const int ni = dm.ni, nj = dm.nj, nk = dm.nk, top = nk - 1;
+                const int i0 = item.ti * BTI, j0 = item.tj * BTJ;
+                // lanes outside the domain evaluate the nearest interior column (identical, harmless stores)
+                const int xe = min(x, ni - 1 - i0), ye = min(y, nj - 1 - j0);
+                const int i = i0 + xe, j = j0 + ye;
+                const gptr<const ST> in_f = (gptr<const ST>)dm.in, co_f = (gptr<const ST>)dm.coeff,
+                                     wc_f = (gptr<const ST>)dm.wcon, up_f = (gptr<const ST>)dm.u_pos,
+                                     ut_f = (gptr<const ST>)dm.utens;
+                const gptr<ST> lap_o = (gptr<ST>)dm.lap, ust_f = (gptr<ST>)dm.u_stage,
+                               uts_f = (gptr<ST>)dm.utens_stage;
+                int in_off[IN_PER];
+#pragma clang loop unroll(full)
+                for (int r = 0; r < IN_PER; ++r) {
+                    const int e = min(tid + r * 256, NIN - 1);
+                    const int gi = min(i0 - 2 + e % IW, ni + 1), gj = min(j0 - 2 + e / IW, nj + 1);
+                    in_off[r] = gi + gj * (int)dm.sj;
+                }
+                const int64_t col = (int64_t)j * dm.sj + i, wcol = (int64_t)j * dm.wsj + i;
+                auto lvl = [&](int L) { return (int64_t)min(L, top); };
+                // opaque per-step strides keep each load at its own step (not hoisted as loop invariant)
+                int64_t sk = dm.sk, wsk = dm.wsk;
+                auto pin = [&]() {
+                    asm volatile("" : "+s"(sk));
+                    asm volatile("" : "+s"(wsk));
+                };
+
+                ST h_in[IN_PER][D], h_co[D], b_uts[D], b_up[D], b_ut[D], a_wi[D], a_wip[D];
+                auto load_h = [&](int L, int slot) {
+                    const int64_t l = lvl(L);
+#pragma clang loop unroll(full)
+                    for (int r = 0; r < IN_PER; ++r)
+                        h_in[r][slot] = in_f[l * sk + in_off[r]];
+                    h_co[slot] = co_f[l * sk + col];
+                };
+                auto load_b = [&](int L, int slot) {
+                    const int64_t l = lvl(L);
+                    b_uts[slot] = uts_f[l * sk + col];
+                    b_up[slot] = up_f[l * sk + col];
+                    b_ut[slot] = ut_f[l * sk + col];
+                };
+                auto load_a = [&](int L, int slot) {
+                    const int64_t l = lvl(L);
+                    a_wi[slot] = wc_f[l * wsk + wcol];
+                    a_wip[slot] = wc_f[l * wsk + wcol + 1];
+                };
+                // laplacian and hdiff of level L from ring slot `slot` through LDS buffer `buf`; the lap
+                // ij-cache value of the column is the laplacian output (levels past the top recompute the top)
+                auto hdiff_level = [&](int L, int slot, int buf) -> CT {
+#pragma clang loop unroll(full)
+                    for (int r = 0; r < IN_PER; ++r)
+                        s_in[buf][tid + r * 256] = h_in[r][slot];
+                    const CT coeff = CT(h_co[slot]);
+                    __syncthreads();
+#pragma clang loop unroll(full)
+                    for (int q = 0; q < NLAP / 256; ++q) {
+                        const int idx = tid + q * 256;
+                        const int lx = idx % LW + 1, ly = idx / LW + 1;
+                        const ST *p = &s_in[buf][ly * IW + lx];
+                        s_lap[idx] = lap_f<CT>(CT(p[0]), CT(p[1]), CT(p[IW]), CT(p[-1]), CT(p[-IW]));
+                    }
+                    __syncthreads();
+                    const ST *ip = &s_in[buf][(ye + 2) * IW + xe + 2];
+                    const CT *lp = &s_lap[(ye + 1) * LW + xe + 1];
+                    const CT c = CT(ip[0]), lc = lp[0];
+                    const CT flx_c = flux_f<CT>(lp[1], lc, CT(ip[1]), c);
+                    const CT flx_m = flux_f<CT>(lc, lp[-1], c, CT(ip[-1]));
+                    const CT fly_c = flux_f<CT>(lp[LW], lc, CT(ip[IW]), c);
+                    const CT fly_m = flux_f<CT>(lc, lp[-LW], c, CT(ip[-IW]));
+                    const CT res = c - coeff * (((flx_c - flx_m) + fly_c) - fly_m);
+                    const int64_t o = lvl(L) * sk + col;
+                    lap_o[o] = ST(lc);
+                    ust_f[o] = ST(res);
+                    return res;
+                };
+
+#pragma clang loop unroll(full)
+                for (int s = 0; s < D; ++s) {
+                    load_h(s, s);
+                    load_b(s, s);
+                    load_a(s + 1, (s + 1) % D);
+                }
+                CT us_c = hdiff_level(0, 0, 0);
+                pin();
+                load_h(D, 0);
+                CT us_m = CT(0), w_c_i = CT(0), w_c_ip = CT(0), cm = CT(0), dmm = CT(0);
+                // forward step of level k: its inputs are in ring slot `slot`, those of level k+1 in `nslot`
+                auto fwd_step = [&](int k, int slot, int nslot, int buf, bool first) {
+                    const bool at_top = k == top;
+                    const CT us_next = hdiff_level(k + 1, nslot, buf);
+                    const CT v_uts = CT(b_uts[slot]), v_up = CT(b_up[slot]), v_ut = CT(b_ut[slot]);
+                    // top level: k+1 wind taken as zero and us_p = us_c, which reproduces last_level exactly
+                    const CT us_p = at_top ? us_c : us_next;
+                    const CT w_p_i = at_top ? CT(0) : CT(a_wi[nslot]);
+                    const CT w_p_ip = at_top ? CT(0) : CT(a_wip[nslot]);
+                    pin();
+                    load_h(k + 1 + D, nslot);
+                    load_b(k + D, slot);
+                    load_a(k + 1 + D, nslot);
+                    fwd_out<CT> r;
+                    if (first)
+                        r = forward_first<CT>(dtr, w_p_i, w_p_ip, us_c, us_p, v_up, v_ut, v_uts);
+                    else
+                        r = forward_body<CT>(
+                            dtr, w_c_i, w_c_ip, w_p_i, w_p_ip, us_m, us_c, us_p, v_up, v_ut, v_uts, cm, dmm);
+                    sc[k * 256 + tid] = r.c;
+                    sd[k * 256 + tid] = r.d;
+                    cm = r.c;
+                    dmm = r.d;
+                    us_m = us_c;
+                    us_c = us_p;
+                    w_c_i = w_p_i;
+                    w_c_ip = w_p_ip;
+                };
+                fwd_step(0, 0, 1 % D, 1, true);
+                // levels 1.. in blocks of D: level kb+s (kb = 1 + m*D) sits in slot (1+s)%D and evaluates the
+                // diffusion of level kb+s+1 through LDS buffer s&1; steps past the top compute values nobody reads
+                for (int kb = 1; kb < nk; kb += D) {
+#pragma clang loop unroll(full)
+                    for (int s = 0; s < D; ++s)
+                        fwd_step(kb + s, (1 + s) % D, (2 + s) % D, s & 1, false);
+                }
+                // backward sweep (vertical_advection_dycore.cpp:110-121) in blocks of D, top block first: a
+                // step above the top stores into the top level before the top level's own step stores there
+                const int nblk = (nk + D - 1) / D;
+                CT data = CT(0);
+                for (int bb = nblk - 1; bb >= 0; --bb) {
+                    CT r_c[D], r_d[D], r_p[D];
+                    pin();
+#pragma clang loop unroll(full)
+                    for (int s = 0; s < D; ++s) {
+                        const int k = bb * D + s;
+                        r_c[s] = sc[k * 256 + tid];
+                        r_d[s] = sd[k * 256 + tid];
+                        r_p[s] = CT(up_f[lvl(k) * sk + col]);
+                    }
+#pragma clang loop unroll(full)
+                    for (int s = D - 1; s >= 0; --s) {
+                        const int k = bb * D + s;
+                        const CT dt = r_d[s] - r_c[s] * data;
+                        const CT nd = k == top ? r_d[s] : dt;
+                        data = nd;
+                        uts_f[lvl(k) * sk + col] = ST(dtr * (nd - r_p[s]));
+                    }
+                }
+                __syncthreads(); // the next item reuses the LDS tiles
+            }
+        }
+
+    } // namespace
+} // namespace gt355
+
+using namespace gt355;
+
+struct gt355_batch {
+    bdom *doms;    // device table
+    bitem *items;  // device table
+    void *scratch; // ccol/dcol: gridmax workgroups x 2 x rows x 256 columns of the compute type
+    int ndoms, nitems, dtype, precision, grid, gridmax, cus, rows;
+};
+
+namespace gt355 {
+    namespace {
+        // workgroups per CU the persistent grid may use (the kernels fit 3-4 workgroups of 256 lanes per CU)
+        constexpr int BWG_MAX = 4;
+    } // namespace
+} // namespace gt355
+
+extern "C" int gt355_batch_create(int precision, const gt355_batch_domain *domains, int ndomains, int wg_per_cu,
+    gt355_batch **out) {
+    GT355_CHECK(out && domains && ndomains > 0, "gt355_batch_create: bad arguments");
+    *out = nullptr;
+    const int dt = domains[0].in.dtype;
+    GT355_CHECK(dt == GT355_F64 || dt == GT355_F32, "gt355_batch_create: dtype %d", dt);
+    GT355_CHECK(precision == GT355_NATIVE || (precision == GT355_WIDEN && dt == GT355_F32),
+        "gt355_batch_create: precision %d with dtype %d", precision, dt);
+    GT355_CHECK(wg_per_cu >= 1 && wg_per_cu <= BWG_MAX, "gt355_batch_create: wg_per_cu %d (1..%d)", wg_per_cu,
+        BWG_MAX);
+    std::vector<bdom> doms(ndomains);
+    std::vector<bitem> items;
+    for (int d = 0; d < ndomains; ++d) {
+        const gt355_batch_domain &b = domains[d];
+        const int ni = b.ni, nj = b.nj, nk = b.nk;
+        GT355_CHECK(ni > 0 && nj > 0 && nk >= 2 && nk <= BMAXK, "gt355_batch_create: domain %d size %dx%dx%d", d, ni,
+            nj, nk);
+        int rc;
+        if ((rc = check_field(&b.in, "in", dt, -2, ni + 2, -2, nj + 2, 0, nk)) ||
+            (rc = check_field(&b.coeff, "coeff", dt, 0, ni, 0, nj, 0, nk)) ||
+            (rc = check_field(&b.lap, "lap", dt, 0, ni, 0, nj, 0, nk)) ||
+            (rc = check_field(&b.u_stage, "u_stage", dt, 0, ni, 0, nj, 0, nk)) ||
+            (rc = check_field(&b.utens_stage, "utens_stage", dt, 0, ni, 0, nj, 0, nk)) ||
+            (rc = check_field(&b.wcon, "wcon", dt, 0, ni + 1, 0, nj, 0, nk)) ||
+            (rc = check_field(&b.u_pos, "u_pos", dt, 0, ni, 0, nj, 0, nk)) ||
+            (rc = check_field(&b.utens, "utens", dt, 0, ni, 0, nj, 0, nk)))
+            return rc;
+        const gt355_field *same[] = {&b.coeff, &b.lap, &b.u_stage, &b.utens_stage, &b.u_pos, &b.utens};
+        for (const gt355_field *f : same)
+            GT355_CHECK(f->stride[1] == b.in.stride[1] && f->stride[2] == b.in.stride[2],
+                "gt355_batch_create: domain %d: every field but wcon must share the layout of in", d);
+        GT355_CHECK(b.u_stage.origin != b.in.origin, "gt355_batch_create: domain %d: u_stage aliases in", d);
+        bdom &m = doms[d];
+        m.in = b.in.origin;
+        m.coeff = b.coeff.origin;
+        m.wcon = b.wcon.origin;
+        m.u_pos = b.u_pos.origin;
+        m.utens = b.utens.origin;
+        m.lap = b.lap.origin;
+        m.u_stage = b.u_stage.origin;
+        m.utens_stage = b.utens_stage.origin;
+        m.sj = b.in.stride[1];
+        m.sk = b.in.stride[2];
+        m.wsj = b.wcon.stride[1];
+        m.wsk = b.wcon.stride[2];
+        m.ni = ni;
+        m.nj = nj;
+        m.nk = nk;
+        m.tiles_i = (ni + BTI - 1) / BTI;
+        const int tj_n = (nj + BTJ - 1) / BTJ;
+        for (int tj = 0; tj < tj_n; ++tj)
+            for (int ti = 0; ti < m.tiles_i; ++ti)
+                items.push_back(bitem{d, ti, tj, 0});
+    }
+    // longest columns first, so the strided schedule ends with short items (a greedy balance)
+    std::stable_sort(items.begin(), items.end(),
+        [&](const bitem &a, const bitem &b) { return doms[a.dom].nk > doms[b.dom].nk; });
+    hipDeviceProp_t p;
+    GT355_HIP(hipGetDeviceProperties(&p, current_device()));
+    gt355_batch *bt = new gt355_batch{};
+    bt->ndoms = ndomains;
+    bt->nitems = (int)items.size();
+    bt->dtype = dt;
+    bt->precision = precision;
+    bt->cus = p.multiProcessorCount;
+    bt->grid = std::min<int>(bt->nitems, bt->cus * wg_per_cu);
+    bt->gridmax = std::min<int>(bt->nitems, bt->cus * BWG_MAX); // scratch for any grid the tuner may pick
+    hipError_t e = hipMalloc(&bt->doms, sizeof(bdom) * doms.size());
+    if (e == hipSuccess)
+        e = hipMalloc(&bt->items, sizeof(bitem) * items.size());
+    if (e == hipSuccess)
+        e = hipMemcpy(bt->doms, doms.data(), sizeof(bdom) * doms.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(bt->items, items.data(), sizeof(bitem) * items.size(), hipMemcpyHostToDevice);
+    // each batch owns its scratch, so batches on different streams never share it
+    int maxnk = 0;
+    for (const bdom &m : doms)
+        maxnk = std::max(maxnk, m.nk);
+    bt->rows = maxnk + BPAST;
+    const size_t ct = (dt == GT355_F64 || precision == GT355_WIDEN) ? 8 : 4;
+    if (e == hipSuccess)
+        e = hipMalloc(&bt->scratch, ct * 2 * (size_t)bt->rows * 256 * (size_t)bt->gridmax);
+    if (e != hipSuccess) {
+        (void)hipFree(bt->doms);
+        (void)hipFree(bt->items);
+        (void)hipFree(bt->scratch);
+        delete bt;
+        return hip_fail(e, "gt355_batch_create");
+    }
+    *out = bt;
+    return GT355_OK;
+}
+
+extern "C" int gt355_batch_run(gt355_batch *bt, double dtr_stage, void *stream) {
+    GT355_CHECK(bt != nullptr, "gt355_batch_run: null batch");
+    void *scr = bt->scratch;
+    hipStream_t s = (hipStream_t)stream;
+    const int D = 4;
+    if (bt->dtype == GT355_F64)
+        batch_kernel<double, double, D><<<bt->grid, 256, 0, s>>>(bt->doms, bt->items, bt->nitems, dtr_stage,
+            (double *)scr, bt->rows, xcd_count());
+    else if (bt->precision == GT355_WIDEN)
+        batch_kernel<float, double, D><<<bt->grid, 256, 0, s>>>(bt->doms, bt->items, bt->nitems,
+            double(float(dtr_stage)), (double *)scr, bt->rows, xcd_count());
+    else
+        batch_kernel<float, float, D><<<bt->grid, 256, 0, s>>>(bt->doms, bt->items, bt->nitems, float(dtr_stage),
+            (float *)scr, bt->rows, xcd_count());
+    return check_launch("gt355_batch_run");
+}
+
+extern "C" int gt355_batch_destroy(gt355_batch *bt) {
+    if (!bt)
+        return GT355_OK;
+    (void)hipFree(bt->doms);
+    (void)hipFree(bt->items);
+    (void)hipFree(bt->scratch);
+    delete bt;
+    return GT355_OK;
+}
+
+extern "C" int gt355_batch_info(const gt355_batch *bt, int *nitems, int *grid) {
+    GT355_CHECK(bt != nullptr, "gt355_batch_info: null batch");
+    if (nitems)
+        *nitems = bt->nitems;
+    if (grid)
+        *grid = bt->grid;
+    return GT355_OK;
+}
+
+extern "C" int gt355_batch_tune(gt355_batch *bt, int reps, void *stream, int *wg_per_cu, double *best_us) {
+    GT355_CHECK(bt != nullptr, "gt355_batch_tune: null batch");
+    GT355_CHECK(reps >= 1, "gt355_batch_tune: reps %d", reps);
+    hipStream_t s = (hipStream_t)stream;
+    hipEvent_t e0, e1;
+    GT355_HIP(hipEventCreate(&e0));
+    GT355_HIP(hipEventCreate(&e1));
+    int best = 0, rc = GT355_OK;
+    double best_t = 1e300;
+    for (int w = 1; w <= BWG_MAX && rc == GT355_OK; ++w) {
+        const int g = std::min<int>(bt->nitems, bt->cus * w);
+        if (w > 1 && g == std::min<int>(bt->nitems, bt->cus * (w - 1)))
+            break; // fewer items than workgroups: larger grids are the same grid
+        bt->grid = g;
+        if ((rc = gt355_batch_run(bt, 0.15, stream)) != GT355_OK) // warm-up
+            break;
+        hipError_t e = hipEventRecord(e0, s);
+        for (int r = 0; r < reps && rc == GT355_OK && e == hipSuccess; ++r)
+            rc = gt355_batch_run(bt, 0.15, stream);
+        if (e == hipSuccess)
+            e = hipEventRecord(e1, s);
+        if (e == hipSuccess)
+            e = hipEventSynchronize(e1);
+        float ms = 0;
+        if (e == hipSuccess)
+            e = hipEventElapsedTime(&ms, e0, e1);
+        if (e != hipSuccess && rc == GT355_OK)
+            rc = hip_fail(e, "gt355_batch_tune");
+        if (rc == GT355_OK && ms / reps < best_t) {
+            best_t = ms / reps;
+            best = w;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != GT355_OK)
+        return rc;
+    bt->grid = std::min<int>(bt->nitems, bt->cus * best);
+    if (wg_per_cu)
+        *wg_per_cu = best;
+    if (best_us)
+        *best_us = best_t * 1e3;
+    return GT355_OK;
+}

@@ -207,6 +207,35 @@ int gt355_graph_end(void *stream, void **graph_exec);
 int gt355_graph_launch(void *graph_exec, void *stream);
 int gt355_graph_destroy(void *graph_exec);
 
+/* ------------------------------------------------------------------ batching
+ * Many independent domains of uneven sizes in one persistent launch.  Replaces looping the reference's run()
+ * (stencil/frontend/run.hpp:34-60) over domains, and generalises expandable_run
+ * (stencil/frontend/expandable_run.hpp:80-183), which batches fields of ONE grid size only.  Per domain and level
+ * the batch computes lap = laplacian(in) (laplacian.cpp:20-29), u_stage = hdiff(in, coeff) and
+ * utens_stage = vadv(utens_stage, u_stage, wcon, u_pos, utens) with u_stage taken from the diffusion in the
+ * compute type; results equal gt355_laplacian + gt355_hdiff_vadv bit for bit.  Domains: ni, nj >= 1,
+ * 2 <= nk <= 144; `in` readable on [-2, n+2) in i and j, wcon on [0, ni+1) in i; every field but wcon shares
+ * the layout of `in`.  All domains have the dtype of domains[0].in.  gt355_batch_create builds the work table
+ * (one item per domain and 64x4 column tile, longest columns first) and the per-workgroup scratch;
+ * gt355_batch_run launches one persistent grid of cu_count * wg_per_cu workgroups (no counters or memsets, so
+ * it may be captured into a graph once the batch exists).  Each batch owns its ccol/dcol scratch (workgroups x
+ * 2 x (deepest nk + 8) x 256 values of the compute type): two runs of ONE batch must not overlap in time, runs of
+ * different batches may. */
+typedef struct gt355_batch_domain {
+    gt355_field in, coeff, lap, u_stage, utens_stage, wcon, u_pos, utens;
+    int32_t ni, nj, nk, reserved;
+} gt355_batch_domain;
+typedef struct gt355_batch gt355_batch;
+int gt355_batch_create(int precision, const gt355_batch_domain *domains, int ndomains, int wg_per_cu,
+    gt355_batch **batch);
+int gt355_batch_run(gt355_batch *batch, double dtr_stage, void *stream);
+int gt355_batch_info(const gt355_batch *batch, int *nitems, int *grid);
+/* The batch's part of the autotuner: times `reps` runs for each persistent grid size (1..4 workgroups per CU),
+ * keeps the fastest for later runs of this batch and returns it with its mean time.  The batch's outputs (and
+ * utens_stage, read and rewritten by every run) are overwritten as by that many gt355_batch_run calls. */
+int gt355_batch_tune(gt355_batch *batch, int reps, void *stream, int *wg_per_cu, double *best_us);
+int gt355_batch_destroy(gt355_batch *batch);
+
 /* ------------------------------------------------------------------ measurement */
 /* Writes (and reads back) a device buffer larger than the MI355X MALL so that the next kernel reads
  * HBM, not Infinity Cache.  Used between timed repetitions, outside the timed region. */

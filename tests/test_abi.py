@@ -107,3 +107,30 @@ def test_tune_table_save_load_roundtrip(tmp_path):
     tune.clear()
     tune.save(out)
     assert out.read_text() == ""
+
+
+def test_batch_create_validates_domains_without_a_device():
+    """gt355_batch_create rejects bad domains (k depth beyond the on-chip limit, undersized halos) before it
+    touches the device."""
+    from gridtools_amd.batch import BatchDomain
+
+    arr = (BatchDomain * 1)()
+    d = arr[0]
+    for n in ("in_", "coeff", "lap", "u_stage", "utens_stage", "wcon", "u_pos", "utens"):
+        f = getattr(d, n)
+        f.origin = 0x100000
+        f.stride = (ctypes.c_int64 * 3)(1, 96, 96 * 72)
+        f.lower = (ctypes.c_int32 * 3)(-3, -3, 0)
+        f.upper = (ctypes.c_int32 * 3)(67, 67, 150)
+        f.dtype = _abi.F32
+    d.ni, d.nj, d.nk = 64, 64, 145
+    h = ctypes.c_void_p()
+    lib = _abi.lib()
+    assert lib.gt355_batch_create(_abi.WIDEN, arr, 1, 2, ctypes.byref(h)) == -1
+    assert "145" in _abi.last_error()
+    d.nk = 60
+    d.in_.lower = (ctypes.c_int32 * 3)(-1, -1, 0)
+    assert lib.gt355_batch_create(_abi.WIDEN, arr, 1, 2, ctypes.byref(h)) == -1
+    assert "outside bounds" in _abi.last_error()
+    assert lib.gt355_batch_create(_abi.WIDEN, arr, 1, 0, ctypes.byref(h)) == -1
+    assert not h.value

@@ -1,0 +1,175 @@
+"""GPU parity of the batching form: uneven domains in one persistent launch (laplacian, hdiff and vadv per domain)
+against the oracle chain on each domain's seeded inputs, bit for bit, and its timestep loop eager vs replayed graph.
+
+The work table has more items than workgroups in the wide-domain cases (wg_per_cu = 1), so workgroups walk
+several items and reuse their LDS tiles and scratch slot; k depths cover the first/last-level special cases
+(nk = 2), partial ring blocks (nk not a multiple of the ring depth) and the deepest columns (137, 144).
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+H = 3
+SIZES = [(23, 11, 43), (70, 9, 81), (130, 37, 2), (64, 4, 137), (65, 5, 96), (48, 48, 144), (1, 1, 3)]
+WIDE = [(512, 256, 40), (97, 300, 61)]
+
+
+def oracle_domain(oracle, host, ni, nj, nk, dtr, mode):
+    """laplacian, hdiff and vadv of one domain (host arrays [k, j, i_padded]); returns interiors."""
+    V = oracle.View
+    lap = np.zeros_like(host["in_"])
+    ust = np.zeros_like(host["in_"])
+    oracle.laplacian(V(lap, H, H), V(host["in_"], H, H), ni, nj, nk)
+    oracle.hdiff(V(ust, H, H), V(host["in_"], H, H), V(host["coeff"], H, H), ni, nj, nk, mode=mode, threads=8)
+    uts = host["utens_stage"].copy()
+    oracle.vadv(V(uts, H, H), V(ust, H, H), V(host["wcon"], H, H), V(host["u_pos"], H, H), V(host["utens"], H, H),
+                dtr, ni, nj, nk, mode=mode, threads=8)
+    box = np.s_[:nk, H:H + nj, H:H + ni]
+    return lap[box], ust[box], uts[box]
+
+
+def gpu_domain(m):
+    g = m["grid"]
+    box = np.s_[:g.k_size, H:H + g.j_size, H:H + g.i_size]
+    return tuple(m[n].padded_numpy()[box] for n in ("lap", "u_stage", "utens_stage"))
+
+
+def make_members(gt, sizes, dtype, seed=11):
+    from gridtools_amd import inputs
+
+    members = []
+    for d, (ni, nj, nk) in enumerate(sizes):
+        s = seed + d
+        b = gt.builder().type(dtype).dimensions(ni + 2 * H, nj + 2 * H, nk).halos(H, H, 0)
+        members.append({
+            "in_": b.uniform(s, inputs.FIELD_IDS["in"], -1.0, 1.0).build(),
+            "coeff": b.uniform(s, inputs.FIELD_IDS["coeff"], 0.0, 0.25).build(),
+            "lap": b.value(0).build(),
+            "u_stage": b.value(0).build(),
+            "utens_stage": b.uniform(s, inputs.FIELD_IDS["utens_stage"], -1.0, 1.0).build(),
+            "u_pos": b.uniform(s, inputs.FIELD_IDS["u_pos"], -1.0, 1.0).build(),
+            "utens": b.uniform(s, inputs.FIELD_IDS["utens"], -1.0, 1.0).build(),
+            "wcon": gt.builder().type(dtype).dimensions(ni + 2 * H, nj + 2 * H, nk + 1).halos(H, H, 0)
+            .uniform(s, inputs.FIELD_IDS["wcon"], -0.5, 0.5).build(),
+            "grid": gt.make_grid(gt.halo_descriptor.symmetric(H, ni + 2 * H),
+                                 gt.halo_descriptor.symmetric(H, nj + 2 * H), nk),
+        })
+    return members
+
+
+def check_batch(gt, oracle, sizes, dtype, precision, wg_per_cu):
+    from gridtools_amd.batch import Batch
+
+    print(f"[case] batch {np.dtype(dtype).name} precision={precision} wg_per_cu={wg_per_cu} {sizes}", flush=True)
+    members = make_members(gt, sizes, dtype)
+    hosts = [{n: m[n].padded_numpy() for n in ("in_", "coeff", "utens_stage", "u_pos", "utens", "wcon")}
+             for m in members]
+    b = Batch(members, precision=precision, wg_per_cu=wg_per_cu)
+    nitems, grid = b.info()
+    print(f"[case] items {nitems} workgroups {grid}", flush=True)
+    b.run(0.15)
+    gt.synchronize()
+    dtr = 0.15 if dtype == np.float64 else float(np.float32(0.15))
+    for (ni, nj, nk), m, host in zip(sizes, members, hosts):
+        got = gpu_domain(m)
+        if precision == 1:
+            # FP64 chain on the FP32-rounded inputs, each output rounded once
+            ref = oracle_domain(oracle, {k: v.astype(np.float64) for k, v in host.items()}, ni, nj, nk, dtr,
+                                "native")
+            ref = tuple(r.astype(np.float32) for r in ref)
+        else:
+            ref = oracle_domain(oracle, host, ni, nj, nk, dtr, "native")
+        for name, g, r in zip(("lap", "u_stage", "utens_stage"), got, ref):
+            assert np.array_equal(g, r), (name, (ni, nj, nk), rel_err(g, r))
+    b.close()
+
+
+@pytest.mark.parametrize("dtype,precision", [(np.float64, 0), (np.float32, 1), (np.float32, 0)])
+def test_batch_uneven_domains_bit_exact(gpu, oracle, dtype, precision):
+    check_batch(gpu, oracle, SIZES, dtype, precision, wg_per_cu=2)
+
+
+@pytest.mark.parametrize("dtype,precision", [(np.float64, 0), (np.float32, 1)])
+def test_batch_more_items_than_workgroups(gpu, oracle, dtype, precision):
+    check_batch(gpu, oracle, WIDE + SIZES[:3], dtype, precision, wg_per_cu=1)
+
+
+def test_batch_matches_single_domain_entry_points(gpu):
+    """On the GPU the batch equals gt355_laplacian + gt355_hdiff_vadv run per domain (nk <= 80: the fused
+    composition, which also keeps u_stage in FP64 in the precision form)."""
+    from gridtools_amd import stencils
+    from gridtools_amd.batch import Batch
+
+    sizes = [(100, 60, 80), (256, 256, 80), (77, 130, 33)]
+    a = make_members(gpu, sizes, np.float32)
+    b = make_members(gpu, sizes, np.float32)
+    Batch(a, precision=1).run(0.15)
+    for m in b:
+        g = m["grid"]
+        stencils.laplacian(m["lap"], m["in_"], g, precision=1)
+        stencils.hdiff_vadv(m["in_"], m["coeff"], m["u_stage"], m["utens_stage"], m["wcon"], m["u_pos"], m["utens"],
+                            0.15, g, precision=1)
+    gpu.synchronize()
+    for ma, mb in zip(a, b):
+        for x, y in zip(gpu_domain(ma), gpu_domain(mb)):
+            assert np.array_equal(x, y)
+
+
+def test_batch_timesteps_graph_replays_match_eager(gpu, oracle):
+    """BatchedTimesteps: the first eager step equals the oracle chain; a captured 2-step graph replayed three
+    times matches an identical twin stepped eagerly, compared after every replay."""
+    from gridtools_amd.batch import BatchedTimesteps
+    from gridtools_amd.timing import Stream
+
+    sizes = [(40, 30, 50), (90, 17, 96), (64, 64, 41)]
+    eager = BatchedTimesteps(sizes, seed=5, halo=H)
+    # step 1 against the oracle (FP64 chain on the FP32 inputs, rounded once)
+    host = []
+    for f in eager.domains:
+        h = {n: f[n].padded_numpy().astype(np.float64) for n in ("coeff", "utens_stage", "u_pos", "utens", "wcon")}
+        h["in_"] = f["u"].padded_numpy().astype(np.float64)
+        host.append(h)
+    eager.step()
+    gpu.synchronize()
+    for d, ((ni, nj, nk), h) in enumerate(zip(sizes, host)):
+        _, ref_u, ref_t = oracle_domain(oracle, h, ni, nj, nk, float(np.float32(0.15)), "native")
+        box = np.s_[:nk, H:H + nj, H:H + ni]
+        assert np.array_equal(eager.current(d).padded_numpy()[box], ref_u.astype(np.float32))
+        assert np.array_equal(eager.domains[d]["utens_stage"].padded_numpy()[box], ref_t.astype(np.float32))
+    twin = BatchedTimesteps(sizes, seed=5, halo=H)
+    graphed = BatchedTimesteps(sizes, seed=5, halo=H)
+    st = Stream()
+    g = graphed.capture(2, st)
+    for r in range(3):
+        g.launch(st)
+        twin.step()
+        twin.step()
+        st.synchronize()
+        gpu.synchronize()
+        for d in range(len(sizes)):
+            assert np.array_equal(graphed.current(d).padded_numpy(), twin.current(d).padded_numpy()), (r, d)
+            assert np.array_equal(graphed.domains[d]["utens_stage"].padded_numpy(),
+                                  twin.domains[d]["utens_stage"].padded_numpy()), (r, d)
+
+
+def test_batch_tune_picks_a_grid_and_keeps_results(gpu):
+    """The grid-size tuner returns a valid choice, and the tuned batch computes the same bits (every item is
+    independent of which workgroup runs it)."""
+    from gridtools_amd.batch import Batch
+
+    sizes = [(300, 200, 60), (64, 300, 120), (100, 100, 90)]
+    a = make_members(gpu, sizes, np.float32)
+    b = make_members(gpu, sizes, np.float32)
+    tuner = Batch(make_members(gpu, sizes, np.float32), precision=1, wg_per_cu=1)
+    w, us = tuner.tune(reps=3)
+    print(f"[tune] batch wg_per_cu {w} ({us:.1f} us)", flush=True)
+    assert 1 <= w <= 4 and us > 0
+    Batch(a, precision=1, wg_per_cu=1).run(0.15)
+    Batch(b, precision=1, wg_per_cu=w).run(0.15)
+    gpu.synchronize()
+    for ma, mb in zip(a, b):
+        for x, y in zip(gpu_domain(ma), gpu_domain(mb)):
+            assert np.array_equal(x, y)
