@@ -3,6 +3,7 @@ interfaces, backed by the hand-written HIP kernels of libgt355.so (C ABI: includ
 from . import _abi
 from .grid import Grid, halo_descriptor, make_grid
 from .storage import DataStore, builder
+from .timing import synchronize
 
 NATIVE = _abi.NATIVE
 WIDEN = _abi.WIDEN
@@ -34,4 +35,4 @@ def device_info() -> dict:
 
 
 __all__ = ["Grid", "halo_descriptor", "make_grid", "DataStore", "builder", "init", "device_info", "NATIVE",
-           "WIDEN", "F64", "F32"]
+           "WIDEN", "F64", "F32", "synchronize"]

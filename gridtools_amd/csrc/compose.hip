@@ -25,22 +25,24 @@
 namespace gt355 {
     namespace {
 
-        template <class ST, class CT, int NK, int D, bool UPOS_LDS>
-        __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+        template <class ST, class CT, int NK, int D, bool UPOS_LDS, int TJ>
+        __global__ void __launch_bounds__(64 * TJ) __attribute__((amdgpu_waves_per_eu(1, 1)))
         hv_fused_kernel(const ST *__restrict__ in_f, const ST *__restrict__ co_f, ST *__restrict__ ust_f,
             ST *__restrict__ uts_f, const ST *__restrict__ wc_f, const ST *__restrict__ up_f,
             const ST *__restrict__ ut_f, CT dtr, int ni, int nj, int nk, int64_t isj, int64_t isk, int64_t csj,
             int64_t csk, int64_t osj, int64_t osk, int64_t sj, int64_t sk, int64_t wsj, int64_t wsk, int tiles_i,
             int tiles_j, int nxcd) {
-            constexpr int TI = 64, TJ = 4, IW = TI + 4, IH = TJ + 4, LW = TI + 2;
-            constexpr int NIN = IW * IH;                  // in tile with its 2-point rim: 68 x 8
-            constexpr int IN_PER = (NIN + 255) / 256;     // tile elements per thread: 3
-            constexpr int NLAP = 512;                     // lap slots evaluated (66 x 6 = 396 used)
-            static_assert(IN_PER * 256 >= (NLAP / LW + 2) * IW + LW + 1, "lap reads stay inside s_in");
-            __shared__ ST s_in[2][IN_PER * 256];
-            __shared__ CT s_lap[NLAP];
-            __shared__ ST s_up[UPOS_LDS ? NK : 1][256];
+            constexpr int NT = 64 * TJ, TI = 64, IW = TI + 4, IH = TJ + 4, LW = TI + 2;
+            constexpr int NIN = IW * IH;                  // in tile with its 2-point rim: 68 x (TJ + 4)
+            constexpr int IN_PER = (NIN + NT - 1) / NT;   // tile elements per thread
+            constexpr int NLU = LW * (TJ + 2);            // lap points of the tile and its 1-point rim
+            constexpr int LAP_PER = (NLU + NT - 1) / NT;  // lap points per thread (surplus threads redo the last)
+            __shared__ ST s_in[2][IN_PER * NT];
+            __shared__ CT s_lap[NLU];
+            __shared__ ST s_up[UPOS_LDS ? NK : 1][NT];
 
+            // one wavefront (TJ = 1) or four (TJ = 4) per workgroup; with one, the workgroup barriers below
+            // are wave barriers and every wavefront streams on its own
             const int tid = threadIdx.x;
             const int x = tid & 63, y = tid >> 6;
             const int id = xcd_remap(blockIdx.x, tiles_i * tiles_j, nxcd);
@@ -56,7 +58,7 @@ namespace gt355 {
             int in_off[IN_PER];
 #pragma clang loop unroll(full)
             for (int r = 0; r < IN_PER; ++r) {
-                const int e = min(tid + r * 256, NIN - 1);
+                const int e = min(tid + r * NT, NIN - 1);
                 const int gi = min(i0 - 2 + e % IW, ni + 1), gj = min(j0 - 2 + e / IW, nj + 1);
                 in_off[r] = gi + gj * (int)isj;
             }
@@ -101,12 +103,12 @@ namespace gt355 {
                 const int b = L & 1;
 #pragma clang loop unroll(full)
                 for (int r = 0; r < IN_PER; ++r)
-                    s_in[b][tid + r * 256] = h_in[r][L % D];
+                    s_in[b][tid + r * NT] = h_in[r][L % D];
                 const CT coeff = CT(h_co[L % D]);
                 __syncthreads();
 #pragma clang loop unroll(full)
-                for (int q = 0; q < NLAP / 256; ++q) {
-                    const int idx = tid + q * 256;
+                for (int q = 0; q < LAP_PER; ++q) {
+                    const int idx = min(tid + q * NT, NLU - 1);
                     const int lx = idx % LW + 1, ly = idx / LW + 1; // in-tile coordinates of the lap point
                     const ST *p = &s_in[b][ly * IW + lx];
                     s_lap[idx] = lap_f<CT>(CT(p[0]), CT(p[1]), CT(p[IW]), CT(p[-1]), CT(p[-IW]));
@@ -199,22 +201,23 @@ namespace gt355 {
             }
         }
 
-        template <class ST, class CT>
-        int launch_fused(const gt355_field *in, const gt355_field *coeff, const gt355_field *u_stage,
+        template <class ST, class CT, int TJ>
+        int launch_fused_tj(const gt355_field *in, const gt355_field *coeff, const gt355_field *u_stage,
             const gt355_field *uts, const gt355_field *wc, const gt355_field *up, const gt355_field *ut, CT dtr,
             int ni, int nj, int nk, hipStream_t stream) {
-            const int tiles_i = (ni + 63) / 64, tiles_j = (nj + 3) / 4;
+            const int tiles_i = (ni + 63) / 64, tiles_j = (nj + TJ - 1) / TJ;
             const int64_t blocks = (int64_t)tiles_i * tiles_j;
             GT355_CHECK(blocks < (1LL << 31), "gt355_hdiff_vadv: domain too large");
-            // u_pos parked in LDS: 80 levels x 256 columns is 80 KB in FP32 storage, too much in FP64
+            // u_pos parked in LDS in FP32 storage (80 levels x 64 TJ columns x 4 B); in FP64 it is re-read
             constexpr bool UPOS_LDS = sizeof(ST) == 4;
 #define GT355_FUSED(NKB, DD)                                                                                     \
     if (nk <= NKB) {                                                                                             \
-        hv_fused_kernel<ST, CT, NKB, DD, UPOS_LDS><<<(int)blocks, 256, 0, stream>>>((const ST *)in->origin,       \
-            (const ST *)coeff->origin, (ST *)u_stage->origin, (ST *)uts->origin, (const ST *)wc->origin,         \
-            (const ST *)up->origin, (const ST *)ut->origin, dtr, ni, nj, nk, in->stride[1], in->stride[2],       \
-            coeff->stride[1], coeff->stride[2], u_stage->stride[1], u_stage->stride[2], uts->stride[1],          \
-            uts->stride[2], wc->stride[1], wc->stride[2], tiles_i, tiles_j, xcd_count());                       \
+        hv_fused_kernel<ST, CT, NKB, DD, UPOS_LDS, TJ><<<(int)blocks, 64 * TJ, 0, stream>>>(                     \
+            (const ST *)in->origin, (const ST *)coeff->origin, (ST *)u_stage->origin, (ST *)uts->origin,         \
+            (const ST *)wc->origin, (const ST *)up->origin, (const ST *)ut->origin, dtr, ni, nj, nk,             \
+            in->stride[1], in->stride[2], coeff->stride[1], coeff->stride[2], u_stage->stride[1],                \
+            u_stage->stride[2], uts->stride[1], uts->stride[2], wc->stride[1], wc->stride[2], tiles_i, tiles_j,  \
+            xcd_count());                                                                                        \
         return check_launch("gt355_hdiff_vadv");                                                                 \
     }
             constexpr int DD = sizeof(ST) == 8 ? 4 : 8;
@@ -226,6 +229,18 @@ namespace gt355 {
 #undef GT355_FUSED
             set_error("gt355_hdiff_vadv: nk %d exceeds the fused kernel's deepest bucket", nk);
             return GT355_EUNSUP;
+        }
+
+        // tile height: 4 rows (default) or 1 row (one wavefront per workgroup, no cross-wave barriers; measured
+        // slower on MI355X: with one wavefront per SIMD the per-level LDS round trips of a lone wavefront are
+        // not hidden)
+        template <class ST, class CT>
+        int launch_fused(const gt355_field *in, const gt355_field *coeff, const gt355_field *u_stage,
+            const gt355_field *uts, const gt355_field *wc, const gt355_field *up, const gt355_field *ut, CT dtr,
+            int ni, int nj, int nk, hipStream_t stream) {
+            if (tune_param("hv_tj", 4) == 1)
+                return launch_fused_tj<ST, CT, 1>(in, coeff, u_stage, uts, wc, up, ut, dtr, ni, nj, nk, stream);
+            return launch_fused_tj<ST, CT, 4>(in, coeff, u_stage, uts, wc, up, ut, dtr, ni, nj, nk, stream);
         }
     } // namespace
 } // namespace gt355
@@ -259,8 +274,11 @@ extern "C" int gt355_hdiff_vadv(int precision, const gt355_field *in, const gt35
                     utens_stage->stride[1] == utens->stride[1] && utens_stage->stride[2] == utens->stride[2],
         "gt355_hdiff_vadv: utens_stage, u_pos and utens must share one layout");
     hipStream_t s = (hipStream_t)stream;
-    // fused single launch (default), or the two-launch sequence of the reference backend
-    if (nk <= 80 && tune_param("hv_fused", 1)) {
+    // the two-launch sequence of the reference backend (default: on MI355X at 512x512x80 it takes 300 us in FP64
+    // and 180 us in the precision form, the fused launch 337 us and 268 us, because the fused kernel holds
+    // ccol/dcol in registers at one wavefront per SIMD and its per-level hdiff barriers are then not hidden), or
+    // the fused single launch (GT355_TUNE_hv_fused=1), which keeps u_stage in the compute type
+    if (nk <= 80 && tune_param("hv_fused", 0)) {
         if (dt == GT355_F64)
             return launch_fused<double, double>(in, coeff, u_stage, utens_stage, wcon, u_pos, utens,
                 double(dtr_stage), ni, nj, nk, s);

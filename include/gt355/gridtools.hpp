@@ -1,0 +1,484 @@
+/*
+ * gt355/gridtools.hpp -- header-only C++17 front end of libgt355 with the GridTools user API for the stencils
+ * of the hot path: storage::builder / data_store (storage/builder.hpp:168-326, storage/data_store.hpp:33-147),
+ * halo_descriptor (common/halo_descriptor.hpp:44-227), make_grid (stencil/frontend/make_grid.hpp:19-42),
+ * global_parameter, stencil::run and expandable_run (stencil/frontend/run.hpp:201-258,
+ * stencil/frontend/expandable_run.hpp:150-183) with a backend tag, and boundaries::boundary /
+ * make_boundary (boundaries/boundary.hpp:31-79).
+ *
+ * The spec argument of run() names one of the path's stencils (specs::horizontal_diffusion, ...) in place of the
+ * reference's spec lambda; its fields follow the order of the reference regression specs.  Every call checks its
+ * status and throws std::runtime_error with the library's message (common/cuda_util.hpp:20-35).  Host code only:
+ * any C++17 compiler, link with -lgt355.
+ */
+#pragma once
+
+#include <algorithm>
+#include <array>
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <type_traits>
+#include <utility>
+#include <vector>
+
+#include "../gt355.h"
+
+namespace gt355 {
+
+    inline void check(int rc, const char *what) {
+        if (rc != GT355_OK)
+            throw std::runtime_error(std::string(what) + ": " + gt355_last_error());
+    }
+
+    // common/halo_descriptor.hpp:44-227: (minus, plus, begin, end, total_length), end inclusive
+    class halo_descriptor {
+        int m_minus = 0, m_plus = 0, m_begin = 0, m_end = 0, m_total_length = 1;
+
+      public:
+        halo_descriptor() = default;
+        // halo_descriptor(size): no halo (halo_descriptor.hpp:78-79)
+        explicit halo_descriptor(int size) : m_end(size - 1), m_total_length(size) {
+            if (size < 1)
+                throw std::invalid_argument("halo_descriptor: size < 1");
+        }
+        halo_descriptor(int minus, int plus, int begin, int end, int total_length)
+            : m_minus(minus), m_plus(plus), m_begin(begin), m_end(end), m_total_length(total_length) {
+            // the constructor's assertions (halo_descriptor.hpp:88-93), as exceptions
+            if (minus + plus + (end - begin + 1) > total_length)
+                throw std::invalid_argument("halo_descriptor: minus + plus + size exceeds total_length");
+            if (begin > end)
+                throw std::invalid_argument("halo_descriptor: begin > end");
+            if (plus > total_length - end - 1)
+                throw std::invalid_argument("halo_descriptor: plus halo exceeds total_length");
+            if (begin < minus)
+                throw std::invalid_argument("halo_descriptor: begin < minus");
+        }
+        int minus() const { return m_minus; }
+        int plus() const { return m_plus; }
+        int begin() const { return m_begin; }
+        int end() const { return m_end; }
+        int total_length() const { return m_total_length; }
+    };
+
+    namespace storage {
+        // Traits tag of the gpu storage (storage/gpu.hpp:69-105): i-innermost, i padded to 128 B, first interior
+        // point 128-byte aligned, device memory with a lazily synchronised host mirror.
+        struct gpu {};
+
+        template <class T>
+        constexpr int dtype_of() {
+            static_assert(std::is_same<T, double>::value || std::is_same<T, float>::value,
+                "gt355 storage holds double or float");
+            return std::is_same<T, double>::value ? GT355_F64 : GT355_F32;
+        }
+
+        template <class T>
+        struct host_view {
+            T *base;
+            int64_t stride[3];
+            std::array<int, 3> len;
+            T &operator()(int i, int j, int k) const { return base[i + j * stride[1] + k * stride[2]]; }
+            std::array<int, 3> const &lengths() const { return len; }
+        };
+
+        // storage/data_store.hpp:33-147 for the gpu traits: device allocation plus host mirror; host_view() makes
+        // the host copy current and the device copy stale, get_target_ptr() the converse; the const accessors
+        // leave both current.
+        template <class T>
+        class data_store {
+            gt355_layout m_layout{};
+            void *m_raw = nullptr;
+            T *m_target = nullptr;
+            std::vector<T> m_host;
+            enum class state { both, host_only, target_only } m_state = state::both;
+            std::string m_name;
+
+            size_t span() const { return (size_t)m_layout.stride[2] * (size_t)m_layout.lengths[2]; }
+            void to_host() {
+                if (m_state == state::target_only) {
+                    // launches may be pending on any stream: wait for the device before reading back
+                    check(gt355_synchronize(), "data_store: update host");
+                    check(gt355_memcpy_d2h(m_host.data(), m_target, span() * sizeof(T)), "data_store: update host");
+                    m_state = state::both;
+                }
+            }
+            void to_target() {
+                if (m_state == state::host_only) {
+                    check(gt355_memcpy_h2d(m_target, m_host.data(), span() * sizeof(T)), "data_store: update target");
+                    m_state = state::both;
+                }
+            }
+
+          public:
+            using data_t = T;
+            data_store(std::array<int, 3> lengths, std::array<int, 3> halos, std::string name)
+                : m_name(std::move(name)) {
+                const int64_t l[3] = {lengths[0], lengths[1], lengths[2]};
+                const int64_t h[3] = {halos[0], halos[1], halos[2]};
+                check(gt355_storage_layout(sizeof(T), l, h, &m_layout), "data_store: layout");
+                check(gt355_malloc((size_t)m_layout.alloc_bytes, &m_raw), "data_store: allocate");
+                void *base = nullptr;
+                check(gt355_storage_base(&m_layout, m_raw, &base), "data_store: align");
+                m_target = static_cast<T *>(base);
+                m_host.assign(span(), T(0));
+                check(gt355_memset(m_raw, 0, (size_t)m_layout.alloc_bytes, nullptr), "data_store: clear");
+                check(gt355_synchronize(), "data_store: clear");
+            }
+            data_store(data_store const &) = delete;
+            data_store &operator=(data_store const &) = delete;
+            ~data_store() {
+                if (m_raw)
+                    gt355_free(m_raw);
+            }
+
+            std::string const &name() const { return m_name; }
+            std::array<int, 3> lengths() const {
+                return {(int)m_layout.lengths[0], (int)m_layout.lengths[1], (int)m_layout.lengths[2]};
+            }
+            std::array<int, 3> native_lengths() const {
+                return {(int)m_layout.padded[0], (int)m_layout.padded[1], (int)m_layout.padded[2]};
+            }
+            std::array<int64_t, 3> strides() const {
+                return {m_layout.stride[0], m_layout.stride[1], m_layout.stride[2]};
+            }
+            std::array<int, 3> halos() const {
+                return {(int)m_layout.halos[0], (int)m_layout.halos[1], (int)m_layout.halos[2]};
+            }
+
+            T *get_target_ptr() {
+                to_target();
+                m_state = state::target_only;
+                return m_target;
+            }
+            T const *get_const_target_ptr() {
+                to_target();
+                return m_target;
+            }
+            storage::host_view<T> host_view() {
+                to_host();
+                m_state = state::host_only;
+                return {m_host.data(), {1, m_layout.stride[1], m_layout.stride[2]}, lengths()};
+            }
+            storage::host_view<T const> const_host_view() {
+                to_host();
+                return {m_host.data(), {1, m_layout.stride[1], m_layout.stride[2]}, lengths()};
+            }
+
+            // The SID of this store after core::shift_origin (stencil/core/backend.hpp:28-34): origin at storage
+            // index `origin`, bounds = the allocation box relative to it.  `write` marks the device copy as the
+            // only current one.
+            gt355_field sid(std::array<int, 3> origin, bool write) {
+                T *p = write ? get_target_ptr() : const_cast<T *>(get_const_target_ptr());
+                gt355_field f{};
+                f.origin = p + origin[0] + origin[1] * m_layout.stride[1] + origin[2] * m_layout.stride[2];
+                for (int d = 0; d < 3; ++d) {
+                    f.stride[d] = m_layout.stride[d];
+                    f.lower[d] = -origin[d];
+                    f.upper[d] = (int32_t)m_layout.lengths[d] - origin[d];
+                }
+                f.dtype = dtype_of<T>();
+                return f;
+            }
+        };
+
+        // storage/builder.hpp:168-326: builder<Traits>.type<T>().dimensions(...).halos(...)
+        //     [.initializer(f) | .value(v)].name(s).build() -> std::shared_ptr<data_store<T>>
+        template <class Traits, class T = void>
+        class builder_type {
+            std::array<int, 3> m_dims{{0, 0, 0}}, m_halos{{0, 0, 0}};
+            std::string m_name;
+            std::function<T(int, int, int)> m_init;
+
+            template <class, class>
+            friend class builder_type;
+
+          public:
+            template <class U>
+            builder_type<Traits, U> type() const {
+                builder_type<Traits, U> b;
+                b.m_dims = m_dims;
+                b.m_halos = m_halos;
+                b.m_name = m_name;
+                return b;
+            }
+            builder_type dimensions(int i, int j, int k) const {
+                builder_type b = *this;
+                b.m_dims = {i, j, k};
+                return b;
+            }
+            builder_type halos(int i, int j, int k) const {
+                builder_type b = *this;
+                b.m_halos = {i, j, k};
+                return b;
+            }
+            builder_type name(std::string n) const {
+                builder_type b = *this;
+                b.m_name = std::move(n);
+                return b;
+            }
+            template <class F>
+            builder_type initializer(F f) const {
+                static_assert(!std::is_void<T>::value, "builder: call type<T>() first");
+                builder_type b = *this;
+                b.m_init = [f](int i, int j, int k) { return T(f(i, j, k)); };
+                return b;
+            }
+            template <class U = T>
+            builder_type value(U v) const {
+                return initializer([v](int, int, int) { return v; });
+            }
+            std::shared_ptr<data_store<T>> build() const {
+                static_assert(!std::is_void<T>::value, "builder: call type<T>() first");
+                auto ds = std::make_shared<data_store<T>>(m_dims, m_halos, m_name);
+                if (m_init) {
+                    // the reference initializer fills every index of the allocation box (builder.hpp:67-84)
+                    auto v = ds->host_view();
+                    for (int k = 0; k < m_dims[2]; ++k)
+                        for (int j = 0; j < m_dims[1]; ++j)
+                            for (int i = 0; i < m_dims[0]; ++i)
+                                v(i, j, k) = m_init(i, j, k);
+                }
+                return ds;
+            }
+        };
+
+        // inside a template, a dependent element type needs the template keyword: builder<gpu>.template type<T>()
+        template <class Traits>
+        inline const builder_type<Traits> builder = {};
+    } // namespace storage
+
+    namespace stencil {
+        // core::grid (stencil/core/grid.hpp:30-128) for one k interval starting at 0
+        struct grid {
+            int i_start, i_size, j_start, j_size, k_size;
+            std::array<int, 3> origin() const { return {i_start, j_start, 0}; }
+        };
+
+        // make_grid(halo_descriptor, halo_descriptor, k_size) (stencil/frontend/make_grid.hpp:19-42)
+        inline grid make_grid(halo_descriptor const &hi, halo_descriptor const &hj, int k_size) {
+            return {hi.begin(), hi.end() - hi.begin() + 1, hj.begin(), hj.end() - hj.begin() + 1, k_size};
+        }
+        inline grid make_grid(int ni, int nj, int nk) { return {0, ni, 0, nj, nk}; }
+
+        template <class T>
+        struct global_parameter_t {
+            T value;
+        };
+        template <class T>
+        global_parameter_t<T> global_parameter(T v) {
+            return {v};
+        }
+
+        // The backend tag (the role of gpu<> at stencil/gpu/entry_point.hpp:253-260): a stream and the precision
+        // policy (GT355_NATIVE computes in the storage type; GT355_WIDEN: FP32 storage, FP64 compute).
+        template <int Precision = GT355_NATIVE>
+        struct gfx950 {
+            void *stream = nullptr;
+            static constexpr int precision = Precision;
+        };
+        using gfx950_widen = gfx950<GT355_WIDEN>;
+
+        // The path's stencil specs; fields in the order of the reference regression specs.
+        namespace specs {
+            struct horizontal_diffusion {}; // (in, coeff, out)                horizontal_diffusion.cpp:97-107
+            struct vertical_advection {};   // (utens_stage, u_stage, wcon, u_pos, utens, dtr_stage)
+                                            //                                 vertical_advection_dycore.cpp:135-148
+            struct laplacian {};            // (out, in)                       laplacian.cpp:20-29, 37
+            struct copy {};                 // (in, out)                       copy_stencil.cpp:23-41
+            struct hdiff_vadv {};           // (in, coeff, u_stage, utens_stage, wcon, u_pos, utens, dtr_stage):
+                                            // u_stage = hdiff(in, coeff), then vertical_advection of u_stage
+        } // namespace specs
+
+        template <class Be, class T>
+        void run(specs::horizontal_diffusion, Be be, grid const &g, std::shared_ptr<storage::data_store<T>> const &in,
+            std::shared_ptr<storage::data_store<T>> const &coeff, std::shared_ptr<storage::data_store<T>> const &out) {
+            auto o = g.origin();
+            gt355_field fi = in->sid(o, false), fc = coeff->sid(o, false), fo = out->sid(o, true);
+            check(gt355_hdiff(Be::precision, &fo, &fi, &fc, g.i_size, g.j_size, g.k_size, be.stream),
+                "run(horizontal_diffusion)");
+        }
+
+        template <class Be, class T>
+        void run(specs::vertical_advection, Be be, grid const &g,
+            std::shared_ptr<storage::data_store<T>> const &utens_stage,
+            std::shared_ptr<storage::data_store<T>> const &u_stage, std::shared_ptr<storage::data_store<T>> const &wcon,
+            std::shared_ptr<storage::data_store<T>> const &u_pos, std::shared_ptr<storage::data_store<T>> const &utens,
+            global_parameter_t<T> dtr_stage) {
+            auto o = g.origin();
+            gt355_field fs = utens_stage->sid(o, true), fu = u_stage->sid(o, false), fw = wcon->sid(o, false),
+                        fp = u_pos->sid(o, false), ft = utens->sid(o, false);
+            check(gt355_vadv(Be::precision, &fs, &fu, &fw, &fp, &ft, (double)dtr_stage.value, g.i_size, g.j_size,
+                      g.k_size, be.stream),
+                "run(vertical_advection)");
+        }
+
+        template <class Be, class T>
+        void run(specs::laplacian, Be be, grid const &g, std::shared_ptr<storage::data_store<T>> const &out,
+            std::shared_ptr<storage::data_store<T>> const &in) {
+            auto o = g.origin();
+            gt355_field fo = out->sid(o, true), fi = in->sid(o, false);
+            check(gt355_laplacian(Be::precision, &fo, &fi, g.i_size, g.j_size, g.k_size, be.stream), "run(laplacian)");
+        }
+
+        template <class Be, class T>
+        void run(specs::copy, Be be, grid const &g, std::shared_ptr<storage::data_store<T>> const &in,
+            std::shared_ptr<storage::data_store<T>> const &out) {
+            static_assert(Be::precision == GT355_NATIVE, "copy has no compute type");
+            auto o = g.origin();
+            gt355_field fo = out->sid(o, true), fi = in->sid(o, false);
+            check(gt355_copy(&fo, &fi, g.i_size, g.j_size, g.k_size, be.stream), "run(copy)");
+        }
+
+        template <class Be, class T>
+        void run(specs::hdiff_vadv, Be be, grid const &g, std::shared_ptr<storage::data_store<T>> const &in,
+            std::shared_ptr<storage::data_store<T>> const &coeff, std::shared_ptr<storage::data_store<T>> const &u_stage,
+            std::shared_ptr<storage::data_store<T>> const &utens_stage,
+            std::shared_ptr<storage::data_store<T>> const &wcon, std::shared_ptr<storage::data_store<T>> const &u_pos,
+            std::shared_ptr<storage::data_store<T>> const &utens, global_parameter_t<T> dtr_stage) {
+            auto o = g.origin();
+            gt355_field fi = in->sid(o, false), fc = coeff->sid(o, false), fu = u_stage->sid(o, true),
+                        fs = utens_stage->sid(o, true), fw = wcon->sid(o, false), fp = u_pos->sid(o, false),
+                        ft = utens->sid(o, false);
+            check(gt355_hdiff_vadv(Be::precision, &fi, &fc, &fu, &fs, &fw, &fp, &ft, (double)dtr_stage.value, g.i_size,
+                      g.j_size, g.k_size, be.stream),
+                "run(hdiff_vadv)");
+        }
+
+        namespace expandable_impl {
+            template <class T>
+            T const &member(T const &v, size_t) {
+                return v;
+            }
+            template <class T>
+            T const &member(std::vector<T> const &v, size_t m) {
+                return v.at(m);
+            }
+            template <class T>
+            size_t count(T const &) {
+                return 0;
+            }
+            template <class T>
+            size_t count(std::vector<T> const &v) {
+                return v.size();
+            }
+        } // namespace expandable_impl
+
+        // expandable_run<Factor>(spec, backend, grid, fields...) (stencil/frontend/expandable_run.hpp:150-183):
+        // std::vector arguments are expanded member by member, scalar arguments are shared; every vector must
+        // have the same size.  Factor is kept for source compatibility; the members are launched back to back
+        // on the backend's stream (the batching form, gt355_batch_*, is the one-launch path).
+        template <size_t Factor, class Spec, class Be, class... Args>
+        void expandable_run(Spec spec, Be be, grid const &g, Args const &... args) {
+            static_assert(Factor > 0, "expandable_run: Factor must be positive");
+            size_t n = 0;
+            for (size_t c : {expandable_impl::count(args)...}) {
+                if (c && n && c != n)
+                    throw std::invalid_argument("expandable_run: vector arguments differ in size");
+                n = std::max(n, c);
+            }
+            for (size_t m = 0; m < n; ++m)
+                run(spec, be, g, expandable_impl::member(args, m)...);
+        }
+    } // namespace stencil
+
+    namespace boundaries {
+        // boundary functors (boundaries/zero.hpp:29-55, value.hpp:27-67, copy.hpp:26-47)
+        struct zero_boundary {};
+        template <class T>
+        struct value_boundary {
+            T value;
+        };
+        struct copy_boundary {};
+
+        // precomputed_pred (boundaries/predicate.hpp:27-32) as a 27-bit direction mask: bit
+        // (di+1)*9 + (dj+1)*3 + (dk+1) selects direction (di, dj, dk)
+        constexpr uint32_t all_directions = 0x7FFFFFFu;
+        constexpr uint32_t direction_bit(int di, int dj, int dk) { return 1u << ((di + 1) * 9 + (dj + 1) * 3 + (dk + 1)); }
+
+        // boundary<BC, Arch> (boundaries/boundary.hpp:31-79): apply(fields...) fills the halo points of the 26
+        // regions the halo descriptors define, for the directions the predicate selects.
+        template <class BC, class Arch = storage::gpu>
+        class boundary {
+            std::array<halo_descriptor, 3> m_halos;
+            BC m_bc;
+            uint32_t m_mask;
+            void *m_stream;
+
+            template <class T>
+            void launch(int kind, std::vector<void *> dst, std::vector<void *> src, double value,
+                std::array<int64_t, 3> stride) const {
+                int32_t h[3][5];
+                for (int d = 0; d < 3; ++d) {
+                    h[d][0] = m_halos[d].minus();
+                    h[d][1] = m_halos[d].plus();
+                    h[d][2] = m_halos[d].begin();
+                    h[d][3] = m_halos[d].end();
+                    h[d][4] = m_halos[d].total_length();
+                }
+                check(gt355_bc_apply(kind, storage::dtype_of<T>(), dst.data(), src.empty() ? nullptr : src.data(),
+                          (int)dst.size(), value, h, stride.data(), m_mask, m_stream),
+                    "boundary::apply");
+            }
+
+          public:
+            boundary(std::array<halo_descriptor, 3> const &halos, BC const &bc, uint32_t predicate = all_directions,
+                void *stream = nullptr)
+                : m_halos(halos), m_bc(bc), m_mask(predicate), m_stream(stream) {}
+
+            template <class T, class... Rest>
+            void apply(std::shared_ptr<storage::data_store<T>> const &first, Rest const &... rest) const {
+                if constexpr (std::is_same<BC, copy_boundary>::value) {
+                    static_assert(sizeof...(Rest) == 1, "copy_boundary applies to (dst, src)");
+                    auto src = std::get<0>(std::tie(rest...));
+                    if (src->strides() != first->strides())
+                        throw std::invalid_argument("copy_boundary: dst and src layouts differ");
+                    launch<T>(2, {first->get_target_ptr()}, {(void *)src->get_const_target_ptr()}, 0.0,
+                        first->strides());
+                } else {
+                    if (!((rest->strides() == first->strides()) && ...))
+                        throw std::invalid_argument("boundary::apply: fields of different layouts");
+                    std::vector<void *> ptrs{first->get_target_ptr()};
+                    (ptrs.push_back(rest->get_target_ptr()), ...);
+                    if constexpr (std::is_same<BC, zero_boundary>::value)
+                        launch<T>(0, ptrs, {}, 0.0, first->strides());
+                    else
+                        launch<T>(1, ptrs, {}, (double)m_bc.value, first->strides());
+                }
+            }
+        };
+
+        template <class Arch = storage::gpu, class BC>
+        boundary<BC, Arch> make_boundary(std::array<halo_descriptor, 3> const &halos, BC const &bc,
+            uint32_t predicate = all_directions, void *stream = nullptr) {
+            return boundary<BC, Arch>(halos, bc, predicate, stream);
+        }
+
+        // The single-rank effect of a periodic / zero-gradient halo update (gt355_halo_fill): mode per dimension
+        enum halo_mode { none = 0, zero_gradient = 1, periodic = 2 };
+        template <class T, class... Rest>
+        void halo_fill(std::array<halo_descriptor, 3> const &halos, std::array<halo_mode, 3> modes, void *stream,
+            std::shared_ptr<storage::data_store<T>> const &first, Rest const &... rest) {
+            if (!((rest->strides() == first->strides()) && ...))
+                throw std::invalid_argument("halo_fill: fields of different layouts");
+            std::vector<void *> ptrs{first->get_target_ptr()};
+            (ptrs.push_back(rest->get_target_ptr()), ...);
+            int32_t h[3][5], m[3];
+            for (int d = 0; d < 3; ++d) {
+                h[d][0] = halos[d].minus();
+                h[d][1] = halos[d].plus();
+                h[d][2] = halos[d].begin();
+                h[d][3] = halos[d].end();
+                h[d][4] = halos[d].total_length();
+                m[d] = modes[d];
+            }
+            auto st = first->strides();
+            check(gt355_halo_fill(storage::dtype_of<T>(), ptrs.data(), (int)ptrs.size(), h, st.data(), m, stream),
+                "halo_fill");
+        }
+    } // namespace boundaries
+} // namespace gt355

@@ -24,6 +24,7 @@ _SIGS = {
     "gto_hdiff_f32": [_P, _P, _P, _I, _I, _I, _PD, _PD, _I],
     "gto_hdiff_mixed": [_P, _P, _P, _I, _I, _I, _PD, _PD, _I],
     "gto_lap_f64": [_P, _P, _I, _I, _I, _PD, _PD],
+    "gto_lap_f32": [_P, _P, _I, _I, _I, _PD, _PD],
     "gto_vadv_f64": [_P, _P, _P, _P, _P, ctypes.c_double, _I, _I, _I, _PD, _PD, _PD, _PD, _I],
     "gto_vadv_f32": [_P, _P, _P, _P, _P, ctypes.c_float, _I, _I, _I, _PD, _PD, _PD, _PD, _I],
     "gto_vadv_mixed": [_P, _P, _P, _P, _P, ctypes.c_float, _I, _I, _I, _PD, _PD, _PD, _PD, _I],
@@ -88,7 +89,11 @@ def hdiff(out: View, in_: View, coeff: View, ni, nj, nk, mode="native", threads=
 
 
 def laplacian(out: View, in_: View, ni, nj, nk):
-    lib().gto_lap_f64(out.ptr, in_.ptr, ni, nj, nk, out.sj, out.sk)
+    """laplacian.cpp:20-29 in the storage type (float64 or float32; nothing else is accepted)."""
+    dt = _kind([out, in_])
+    assert out.sj == in_.sj and out.sk == in_.sk
+    fn = {"float64": "gto_lap_f64", "float32": "gto_lap_f32"}[dt.name]
+    getattr(lib(), fn)(out.ptr, in_.ptr, ni, nj, nk, out.sj, out.sk)
 
 
 def vadv(uts: View, us: View, wc: View, up: View, ut: View, dtr, ni, nj, nk, mode="native", threads=1):
@@ -109,8 +114,20 @@ def _halo_arr(halos):
     return np.ascontiguousarray(np.array(halos, dtype=np.int32).reshape(3, 5))
 
 
+def _full(arr, like=None):
+    """A full [k, j, i] float64/float32 C-contiguous array (and the same dtype and shape as ``like``)."""
+    if arr.dtype not in (np.float64, np.float32) or not arr.flags.c_contiguous or arr.ndim != 3:
+        raise TypeError("oracle arrays are C-contiguous 3-D float64 or float32")
+    if like is not None and (arr.dtype != like.dtype or arr.shape != like.shape):
+        raise TypeError("src must match dst in dtype and shape")
+    return arr
+
+
 def bc(kind, dst, src, value, halos, dir_mask):
     """dst/src: full [k, j, i] arrays (storage index (0,0,0) at element 0)."""
+    _full(dst)
+    if src is not None:
+        _full(src, dst)
     h = _halo_arr(halos)
     sj, sk = dst.shape[2], dst.shape[1] * dst.shape[2]
     srcp = src.ctypes.data if src is not None else None
@@ -121,6 +138,7 @@ def bc(kind, dst, src, value, halos, dir_mask):
 
 
 def halo_fill(f, halos, modes):
+    _full(f)
     h = _halo_arr(halos)
     m = np.ascontiguousarray(np.array(modes, dtype=np.int32))
     sj, sk = f.shape[2], f.shape[1] * f.shape[2]

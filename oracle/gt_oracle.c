@@ -96,6 +96,17 @@ void gto_lap_f64(double *out, const double *in, int ni, int nj, int nk, ptrdiff_
         }
 }
 
+/* the float_t = float instantiation of the same functor: every operation in FP32 */
+void gto_lap_f32(float *out, const float *in, int ni, int nj, int nk, ptrdiff_t sj, ptrdiff_t sk) {
+    for (int k = 0; k < nk; ++k)
+        for (int j = 0; j < nj; ++j) {
+            const float *p = in + (ptrdiff_t)k * sk + (ptrdiff_t)j * sj;
+            float *o = out + (ptrdiff_t)k * sk + (ptrdiff_t)j * sj;
+            for (int i = 0; i < ni; ++i)
+                o[i] = 4 * p[i] - (((p[i + 1] + p[i + sj]) + p[i - 1]) + p[i - sj]);
+        }
+}
+
 /* ---------------------------------- tridiagonal ----------------------------------
  * tests/regression/tridiagonal.cpp:33-68.  forward_thomas: first_level sup/=diag, rhs/=diag;
  * modify<1,0>: sup = sup/(diag - sup[k-1]*inf), rhs = (rhs - inf*rhs[k-1])/(diag - sup[k-1]*inf).

@@ -40,6 +40,7 @@ void gto_hdiff_mixed(float *out, const float *in, const float *coeff, int ni, in
 
 /* ---- laplacian: tests/regression/laplacian.cpp:20-29 ---- */
 void gto_lap_f64(double *out, const double *in, int ni, int nj, int nk, ptrdiff_t sj, ptrdiff_t sk);
+void gto_lap_f32(float *out, const float *in, int ni, int nj, int nk, ptrdiff_t sj, ptrdiff_t sk);
 
 /* ---- vertical advection: tests/regression/vertical_advection_dycore.cpp:32-149 ----
  * wcon may have its own strides (it is often allocated with nk+1 staggered levels).

@@ -1,0 +1,297 @@
+// C++ regression tests of the gt355 front end (include/gt355/gridtools.hpp), written like the reference's
+// tests/regression/*.cpp: build storages with storage::builder, make a grid from halo descriptors, run the spec
+// on the backend tag, verify.  Expected values come from the oracle (oracle/gt_oracle.h, the CPU restatement of
+// the reference functors) on the same seeded inputs, or from the reference's known answers; FP64 and native FP32
+// must match bit for bit.
+//
+//   gt355_regression            all cases (needs a GPU)
+//   gt355_regression --host     only the host-side cases (no device calls)
+#include <gt355/gridtools.hpp>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../oracle/gt_oracle.h"
+
+using namespace gt355;
+using namespace gt355::stencil;
+
+namespace {
+    int failures = 0;
+
+    void report(const char *name, bool ok) {
+        std::printf("[%s] %s\n", ok ? "ok" : "FAIL", name);
+        std::fflush(stdout);
+        if (!ok)
+            ++failures;
+    }
+
+    // seeded host field over a logical box (i fastest), the library's fill_uniform sequence
+    struct host_field {
+        int ni, nj, nk;
+        std::vector<double> v;
+        host_field(int a, int b, int c, uint64_t seed, uint64_t fid, double lo, double hi)
+            : ni(a), nj(b), nk(c), v((size_t)a * b * c) {
+            gto_fill_uniform_f64(v.data(), (int64_t)v.size(), seed, fid, lo, hi);
+        }
+        double operator()(int i, int j, int k) const { return v[i + (size_t)ni * (j + (size_t)nj * k)]; }
+    };
+
+    // oracle-layout copy of a store's host mirror: [k][j][i padded], returns pointer to storage index (0,0,0)
+    template <class T>
+    std::vector<T> mirror(std::shared_ptr<storage::data_store<T>> const &ds) {
+        auto v = ds->const_host_view();
+        auto len = ds->lengths();
+        std::vector<T> out((size_t)v.stride[2] * len[2]);
+        for (int k = 0; k < len[2]; ++k)
+            for (int j = 0; j < len[1]; ++j)
+                for (int i = 0; i < len[0]; ++i)
+                    out[i + j * v.stride[1] + k * v.stride[2]] = v(i, j, k);
+        return out;
+    }
+
+    template <class T>
+    bool same_interior(std::shared_ptr<storage::data_store<T>> const &ds, std::vector<T> const &ref, grid const &g) {
+        auto v = ds->const_host_view();
+        for (int k = 0; k < g.k_size; ++k)
+            for (int j = g.j_start; j < g.j_start + g.j_size; ++j)
+                for (int i = g.i_start; i < g.i_start + g.i_size; ++i) {
+                    T a = v(i, j, k), b = ref[i + j * v.stride[1] + k * v.stride[2]];
+                    if (std::memcmp(&a, &b, sizeof(T)) != 0) {
+                        std::printf("  mismatch at (%d,%d,%d): %.17g vs %.17g\n", i, j, k, (double)a, (double)b);
+                        return false;
+                    }
+                }
+        return true;
+    }
+
+    template <class T>
+    auto seeded(int ni, int nj, int nk, int h, uint64_t seed, uint64_t fid, double lo, double hi, int extra_k = 0) {
+        host_field f(ni + 2 * h, nj + 2 * h, nk + extra_k, seed, fid, lo, hi);
+        return storage::builder<storage::gpu>
+            .template type<T>()
+            .dimensions(ni + 2 * h, nj + 2 * h, nk + extra_k)
+            .halos(h, h, 0)
+            .initializer([f](int i, int j, int k) { return f(i, j, k); })
+            .build();
+    }
+
+    template <class T>
+    void horizontal_diffusion(int ni, int nj, int nk) {
+        const int h = 2;
+        auto in = seeded<T>(ni, nj, nk, h, 42, 1, -1, 1);
+        auto coeff = seeded<T>(ni, nj, nk, h, 42, 2, 0, 0.25);
+        auto out = storage::builder<storage::gpu>.template type<T>().dimensions(ni + 4, nj + 4, nk).halos(h, h, 0).build();
+        auto g = make_grid(halo_descriptor(h, h, h, ni + h - 1, ni + 2 * h), halo_descriptor(h, h, h, nj + h - 1, nj + 2 * h), nk);
+        run(specs::horizontal_diffusion(), gfx950<>(), g, in, coeff, out);
+        auto hin = mirror(in), hco = mirror(coeff);
+        std::vector<T> ref(hin.size(), T(0));
+        auto st = in->strides();
+        const size_t o = h + h * st[1];
+        if constexpr (std::is_same<T, double>::value)
+            gto_hdiff_f64(ref.data() + o, hin.data() + o, hco.data() + o, ni, nj, nk, st[1], st[2], 4);
+        else
+            gto_hdiff_f32(ref.data() + o, hin.data() + o, hco.data() + o, ni, nj, nk, st[1], st[2], 4);
+        char name[96];
+        std::snprintf(name, sizeof name, "horizontal_diffusion %s %dx%dx%d", sizeof(T) == 8 ? "f64" : "f32", ni, nj,
+            nk);
+        report(name, same_interior(out, ref, g));
+    }
+
+    template <class T>
+    void vertical_advection(int ni, int nj, int nk) {
+        const int h = 3;
+        auto utens_stage = seeded<T>(ni, nj, nk, h, 42, 6, -1, 1);
+        auto u_stage = seeded<T>(ni, nj, nk, h, 42, 3, -1, 1);
+        auto u_pos = seeded<T>(ni, nj, nk, h, 42, 4, -1, 1);
+        auto utens = seeded<T>(ni, nj, nk, h, 42, 5, -1, 1);
+        auto wcon = seeded<T>(ni, nj, nk, h, 42, 7, -0.5, 0.5, 1);
+        auto g = make_grid(halo_descriptor(h, h, h, ni + h - 1, ni + 2 * h), halo_descriptor(h, h, h, nj + h - 1, nj + 2 * h), nk);
+        auto uts0 = mirror(utens_stage);
+        run(specs::vertical_advection(), gfx950<>(), g, utens_stage, u_stage, wcon, u_pos, utens,
+            global_parameter(T(0.15)));
+        auto hus = mirror(u_stage), hup = mirror(u_pos), hut = mirror(utens), hw = mirror(wcon);
+        auto st = u_stage->strides(), wst = wcon->strides();
+        const size_t o = h + h * st[1], wo = h + h * wst[1];
+        if constexpr (std::is_same<T, double>::value)
+            gto_vadv_f64(uts0.data() + o, hus.data() + o, hw.data() + wo, hup.data() + o, hut.data() + o, 0.15, ni, nj,
+                nk, st[1], st[2], wst[1], wst[2], 4);
+        else
+            gto_vadv_f32(uts0.data() + o, hus.data() + o, hw.data() + wo, hup.data() + o, hut.data() + o, 0.15f, ni,
+                nj, nk, st[1], st[2], wst[1], wst[2], 4);
+        char name[96];
+        std::snprintf(name, sizeof name, "vertical_advection %s %dx%dx%d", sizeof(T) == 8 ? "f64" : "f32", ni, nj, nk);
+        report(name, same_interior(utens_stage, uts0, g));
+    }
+
+    // laplacian.cpp:31-39: in = -1 everywhere -> out = 0 in the interior
+    void laplacian_known_answer() {
+        auto in = storage::builder<storage::gpu>.type<double>().dimensions(70, 70, 12).halos(1, 1, 0).value(-1).build();
+        auto out = storage::builder<storage::gpu>.type<double>().dimensions(70, 70, 12).halos(1, 1, 0).build();
+        auto g = make_grid(halo_descriptor(1, 1, 1, 68, 70), halo_descriptor(1, 1, 1, 68, 70), 12);
+        run(specs::laplacian(), gfx950<>(), g, out, in);
+        auto v = out->const_host_view();
+        bool ok = true;
+        for (int k = 0; k < 12; ++k)
+            for (int j = 1; j <= 68; ++j)
+                for (int i = 1; i <= 68; ++i)
+                    ok &= v(i, j, k) == 0.0;
+        report("laplacian known answer", ok);
+    }
+
+    // copy_stencil.cpp:35-41: in(i,j,k) = i + j + k
+    void copy_known_answer() {
+        auto in = storage::builder<storage::gpu>
+                      .type<float>()
+                      .dimensions(37, 21, 9)
+                      .initializer([](int i, int j, int k) { return i + j + k; })
+                      .build();
+        auto out = storage::builder<storage::gpu>.type<float>().dimensions(37, 21, 9).build();
+        run(specs::copy(), gfx950<>(), make_grid(37, 21, 9), in, out);
+        auto v = out->const_host_view();
+        bool ok = true;
+        for (int k = 0; k < 9; ++k)
+            for (int j = 0; j < 21; ++j)
+                for (int i = 0; i < 37; ++i)
+                    ok &= v(i, j, k) == float(i + j + k);
+        report("copy known answer", ok);
+    }
+
+    // the precision form of the composition: FP64 arithmetic on the FP32-rounded inputs; the default two-launch
+    // dispatch stores u_stage in FP32 between hdiff and vadv, each output is rounded once
+    void hdiff_vadv_precision_form(int ni, int nj, int nk) {
+        const int h = 3;
+        auto in = seeded<float>(ni, nj, nk, h, 9, 1, -1, 1);
+        auto coeff = seeded<float>(ni, nj, nk, h, 9, 2, 0, 0.25);
+        auto u_stage = storage::builder<storage::gpu>.type<float>().dimensions(ni + 6, nj + 6, nk).halos(h, h, 0).build();
+        auto utens_stage = seeded<float>(ni, nj, nk, h, 9, 6, -1, 1);
+        auto u_pos = seeded<float>(ni, nj, nk, h, 9, 4, -1, 1);
+        auto utens = seeded<float>(ni, nj, nk, h, 9, 5, -1, 1);
+        auto wcon = seeded<float>(ni, nj, nk, h, 9, 7, -0.5, 0.5, 1);
+        auto g = make_grid(halo_descriptor(h, h, h, ni + h - 1, ni + 2 * h), halo_descriptor(h, h, h, nj + h - 1, nj + 2 * h), nk);
+        auto widen = [](std::vector<float> const &v) { return std::vector<double>(v.begin(), v.end()); };
+        auto hin = widen(mirror(in)), hco = widen(mirror(coeff)), hts = widen(mirror(utens_stage)),
+             hup = widen(mirror(u_pos)), hut = widen(mirror(utens)), hw = widen(mirror(wcon));
+        run(specs::hdiff_vadv(), gfx950_widen(), g, in, coeff, u_stage, utens_stage, wcon, u_pos, utens,
+            global_parameter(0.15f));
+        auto st = in->strides(), wst = wcon->strides();
+        const size_t o = h + h * st[1], wo = h + h * wst[1];
+        std::vector<double> hus(hin.size(), 0.0);
+        gto_hdiff_f64(hus.data() + o, hin.data() + o, hco.data() + o, ni, nj, nk, st[1], st[2], 4);
+        for (double &v : hus)
+            v = (double)(float)v;
+        gto_vadv_f64(hts.data() + o, hus.data() + o, hw.data() + wo, hup.data() + o, hut.data() + o,
+            (double)0.15f, ni, nj, nk, st[1], st[2], wst[1], wst[2], 4);
+        std::vector<float> rus(hus.begin(), hus.end()), rts(hts.begin(), hts.end());
+        report("hdiff_vadv precision form u_stage", same_interior(u_stage, rus, g));
+        report("hdiff_vadv precision form utens_stage", same_interior(utens_stage, rts, g));
+    }
+
+    // expandable_run: vector members run as separate stencils, shared arguments once
+    void expandable() {
+        const int ni = 40, nj = 24, nk = 7, h = 2;
+        std::vector<std::shared_ptr<storage::data_store<double>>> ins, outs, refs;
+        auto coeff = seeded<double>(ni, nj, nk, h, 5, 2, 0, 0.25);
+        auto g = make_grid(halo_descriptor(h, h, h, ni + h - 1, ni + 2 * h), halo_descriptor(h, h, h, nj + h - 1, nj + 2 * h), nk);
+        for (int m = 0; m < 3; ++m) {
+            ins.push_back(seeded<double>(ni, nj, nk, h, 100 + m, 1, -1, 1));
+            outs.push_back(storage::builder<storage::gpu>.type<double>().dimensions(ni + 4, nj + 4, nk).halos(h, h, 0).build());
+            refs.push_back(storage::builder<storage::gpu>.type<double>().dimensions(ni + 4, nj + 4, nk).halos(h, h, 0).build());
+        }
+        expandable_run<2>(specs::horizontal_diffusion(), gfx950<>(), g, ins, coeff, outs);
+        bool ok = true;
+        for (int m = 0; m < 3; ++m) {
+            run(specs::horizontal_diffusion(), gfx950<>(), g, ins[m], coeff, refs[m]);
+            ok &= same_interior(outs[m], mirror(refs[m]), g);
+        }
+        report("expandable_run horizontal_diffusion x3", ok);
+    }
+
+    // boundaries: zero / value / copy on the 26 regions against the oracle's per-region loop
+    void boundary_conditions() {
+        const int n = 20, h = 3, nk = 6;
+        std::array<halo_descriptor, 3> halos{halo_descriptor(h, h, h, n + h - 1, n + 2 * h),
+            halo_descriptor(h, h, h, n + h - 1, n + 2 * h), halo_descriptor(nk)};
+        auto a = seeded<double>(n, n, nk, h, 3, 1, -1, 1);
+        auto b = seeded<double>(n, n, nk, h, 3, 2, -1, 1);
+        auto src = seeded<double>(n, n, nk, h, 3, 3, -1, 1);
+        auto ha = mirror(a), hb = mirror(b), hs = mirror(src);
+        int32_t hh[3][5];
+        for (int d = 0; d < 3; ++d) {
+            hh[d][0] = halos[d].minus();
+            hh[d][1] = halos[d].plus();
+            hh[d][2] = halos[d].begin();
+            hh[d][3] = halos[d].end();
+            hh[d][4] = halos[d].total_length();
+        }
+        auto st = a->strides();
+        const uint32_t pred = boundaries::all_directions & ~boundaries::direction_bit(1, 0, 0);
+        boundaries::make_boundary(halos, boundaries::value_boundary<double>{2.5}, pred).apply(a, b);
+        gto_bc_f64(1, ha.data(), nullptr, 2.5, hh, st[1], st[2], pred);
+        gto_bc_f64(1, hb.data(), nullptr, 2.5, hh, st[1], st[2], pred);
+        auto all = make_grid(n + 2 * h, n + 2 * h, nk);
+        report("boundary value_boundary (predicate)", same_interior(a, ha, all) && same_interior(b, hb, all));
+        boundaries::make_boundary(halos, boundaries::copy_boundary{}).apply(a, src);
+        gto_bc_f64(2, ha.data(), hs.data(), 0.0, hh, st[1], st[2], boundaries::all_directions);
+        report("boundary copy_boundary", same_interior(a, ha, all));
+        boundaries::make_boundary(halos, boundaries::zero_boundary{}).apply(b);
+        gto_bc_f64(0, hb.data(), nullptr, 0.0, hh, st[1], st[2], boundaries::all_directions);
+        report("boundary zero_boundary", same_interior(b, hb, all));
+        // periodic i, zero-gradient j
+        boundaries::halo_fill(halos, {boundaries::periodic, boundaries::zero_gradient, boundaries::none}, nullptr, a);
+        int32_t mode[3] = {2, 1, 0};
+        gto_halo_fill_f64(ha.data(), hh, st[1], st[2], mode);
+        report("halo_fill periodic i / zero-gradient j", same_interior(a, ha, all));
+    }
+
+    // host-only cases: descriptor validation and error propagation (no device calls)
+    void host_cases() {
+        bool threw = false;
+        try {
+            halo_descriptor(3, 3, 2, 10, 20); // begin < minus (halo_descriptor.hpp:93)
+        } catch (std::invalid_argument const &) {
+            threw = true;
+        }
+        report("halo_descriptor rejects begin < minus", threw);
+        auto g = make_grid(halo_descriptor(3, 3, 3, 258, 262), halo_descriptor(2, 2, 2, 65, 68), 80);
+        report("make_grid from halo descriptors",
+            g.i_start == 3 && g.i_size == 256 && g.j_start == 2 && g.j_size == 64 && g.k_size == 80);
+        threw = false;
+        try {
+            gt355_layout lay;
+            const int64_t l[3] = {4, 4, 4}, hh[3] = {0, 0, 0};
+            check(gt355_storage_layout(3, l, hh, &lay), "layout");
+        } catch (std::runtime_error const &e) {
+            threw = std::strstr(e.what(), "element size") != nullptr;
+        }
+        report("library errors become std::runtime_error", threw);
+    }
+} // namespace
+
+int main(int argc, char **argv) {
+    const bool host_only = argc > 1 && std::strcmp(argv[1], "--host") == 0;
+    try {
+        host_cases();
+        if (!host_only) {
+            check(gt355_init(0), "init");
+            horizontal_diffusion<double>(12, 33, 61);
+            horizontal_diffusion<double>(256, 256, 80);
+            horizontal_diffusion<float>(23, 11, 43);
+            vertical_advection<double>(23, 11, 43);
+            vertical_advection<double>(256, 256, 80);
+            vertical_advection<float>(70, 9, 80);
+            laplacian_known_answer();
+            copy_known_answer();
+            hdiff_vadv_precision_form(40, 30, 50);
+            expandable();
+            boundary_conditions();
+        }
+    } catch (std::exception const &e) {
+        std::printf("[FAIL] exception: %s\n", e.what());
+        return 1;
+    }
+    std::printf("%d failure(s)\n", failures);
+    return failures ? 1 : 0;
+}

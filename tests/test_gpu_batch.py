@@ -97,9 +97,10 @@ def test_batch_more_items_than_workgroups(gpu, oracle, dtype, precision):
     check_batch(gpu, oracle, WIDE + SIZES[:3], dtype, precision, wg_per_cu=1)
 
 
-def test_batch_matches_single_domain_entry_points(gpu):
-    """On the GPU the batch equals gt355_laplacian + gt355_hdiff_vadv run per domain (nk <= 80: the fused
-    composition, which also keeps u_stage in FP64 in the precision form)."""
+def test_batch_matches_single_domain_entry_points(gpu, monkeypatch):
+    """On the GPU the batch equals gt355_laplacian + gt355_hdiff_vadv run per domain with the fused composition
+    (nk <= 80), which also keeps u_stage in FP64 between the stages in the precision form."""
+    monkeypatch.setenv("GT355_TUNE_hv_fused", "1")
     from gridtools_amd import stencils
     from gridtools_amd.batch import Batch
 

@@ -1,7 +1,8 @@
 """Timestep sequences (hdiff -> vadv -> periodic-i / zero-gradient-j halo fill -> swap) on the GPU.
 
-* several steps, bit-identical to the same sequence replayed with the oracle (FP64; and the precision form, whose
-  fused composition keeps u_stage in FP64 between the stages);
+* several steps, bit-identical to the same sequence replayed with the oracle (FP64 and the precision form; with
+  the two-launch composition, which stores u_stage in FP32 in the precision form, and with the fused one, which
+  keeps it in FP64 between the stages);
 * a hipGraph of an even number of steps, replayed several times with eager work in between, equal after every
   replay to eager stepping of an identical second instance;
 * the 1536x1536x80 FP32 domain for 20 steps: the halo satisfies the boundary conditions exactly after every run.
@@ -13,8 +14,9 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def oracle_steps(oracle, ts, nsteps):
-    """Replays nsteps of the timestepper's operator on host copies of its current fields."""
+def oracle_steps(oracle, ts, nsteps, fused):
+    """Replays nsteps of the timestepper's operator on host copies of its current fields.  In the precision form
+    the two-launch composition stores u_stage in FP32 before the vertical sweep reads it; the fused one does not."""
     H = ts.halo
     ni, nj, nk = ts.ni, ts.nj, ts.nk
     f = {n: getattr(ts, n).padded_numpy() for n in ("u", "u_stage", "coeff", "utens_stage", "u_pos", "utens", "wcon")}
@@ -27,7 +29,8 @@ def oracle_steps(oracle, ts, nsteps):
         ust = np.zeros_like(cv(f["u"]))
         oracle.hdiff(V(ust, H, H), V(cv(f["u"]), H, H), V(cv(f["coeff"]), H, H), ni, nj, nk, threads=8)
         uts = cv(f["utens_stage"]).copy()
-        oracle.vadv(V(uts, H, H), V(ust, H, H), V(cv(f["wcon"]), H, H), V(cv(f["u_pos"]), H, H),
+        ust_v = ust.astype(np.float32).astype(np.float64) if wide and not fused else ust
+        oracle.vadv(V(uts, H, H), V(ust_v, H, H), V(cv(f["wcon"]), H, H), V(cv(f["u_pos"]), H, H),
                     V(cv(f["utens"]), H, H), dtr, ni, nj, nk, threads=8)
         # write back the interiors, keep the halos, then apply the boundary conditions
         new_ust = f["u_stage"].copy()
@@ -39,12 +42,14 @@ def oracle_steps(oracle, ts, nsteps):
     return f
 
 
+@pytest.mark.parametrize("fused", [0, 1])
 @pytest.mark.parametrize("dtype,precision", [(np.float64, 0), (np.float32, 1)])
-def test_steps_match_oracle(gpu, oracle, dtype, precision):
+def test_steps_match_oracle(gpu, oracle, monkeypatch, dtype, precision, fused):
     from gridtools_amd.timestep import HdiffVadvTimestepper
 
+    monkeypatch.setenv("GT355_TUNE_hv_fused", str(fused))
     ts = HdiffVadvTimestepper(70, 45, 24, dtype=dtype, precision=precision, seed=3)
-    ref = oracle_steps(oracle, ts, 4)
+    ref = oracle_steps(oracle, ts, 4, fused)
     for _ in range(4):
         ts.step()
     for name in ("u", "utens_stage"):

@@ -156,3 +156,31 @@ def test_laplacian_constant_known_answer(oracle):
     oracle.laplacian(oracle.View(out, h, h), oracle.View(inp, h, h), ni, nj, nk)
     assert np.array_equal(out[:, 1:-1, 1:-1], np.zeros((nk, nj, ni)))
     assert np.all(out[:, 0, :] == 99.0)  # halo untouched
+
+
+def test_laplacian_fp32_known_answers_and_rounding(oracle):
+    """laplacian.cpp:31-39 known answer in FP32, the harmonic-input answer (lap of a linear field is 0 in exact
+    arithmetic and in FP32 for small integers), and the FP32 result is the float rounding of each operation: it
+    differs from the FP64 laplacian rounded once on random inputs (a kernel computing in FP64 would fail this)."""
+    ni, nj, nk, h = 20, 9, 3, 1
+    inp = np.full((nk, nj + 2, ni + 2), -1.0, dtype=np.float32)
+    out = np.full_like(inp, 7.0)
+    oracle.laplacian(oracle.View(out, h, h), oracle.View(inp, h, h), ni, nj, nk)
+    assert np.array_equal(out[:, 1:-1, 1:-1], np.zeros((nk, nj, ni), np.float32))
+    assert np.all(out[:, 0, :] == 7.0)
+    k, j, i = np.meshgrid(np.arange(nk), np.arange(nj + 2), np.arange(ni + 2), indexing="ij")
+    lin = (3 * i - 2 * j + k).astype(np.float32)
+    oracle.laplacian(oracle.View(out, h, h), oracle.View(lin, h, h), ni, nj, nk)
+    assert np.array_equal(out[:, 1:-1, 1:-1], np.zeros((nk, nj, ni), np.float32))
+    rng = np.random.default_rng(3)
+    x = rng.uniform(-1, 1, size=(nk, nj + 2, ni + 2)).astype(np.float32)
+    o32 = np.zeros_like(x)
+    oracle.laplacian(oracle.View(o32, h, h), oracle.View(x, h, h), ni, nj, nk)
+    x64 = x.astype(np.float64)
+    o64 = np.zeros_like(x64)
+    oracle.laplacian(oracle.View(o64, h, h), oracle.View(x64, h, h), ni, nj, nk)
+    a, b = o32[:, 1:-1, 1:-1], o64[:, 1:-1, 1:-1]
+    assert np.max(np.abs(a - b)) < 1e-5
+    assert not np.array_equal(a, b.astype(np.float32))
+    with pytest.raises(TypeError):
+        oracle.laplacian(oracle.View(o32, h, h), oracle.View(x64, h, h), ni, nj, nk)
