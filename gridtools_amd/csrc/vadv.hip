@@ -104,7 +104,10 @@ namespace gt355 {
         // u_pos is needed again by the backward sweep.  UPOS_LDS parks the column's u_pos in LDS (NK*64*sizeof(ST)
         // bytes per wavefront; nothing is re-read from memory); otherwise the backward sweep re-loads u_pos through
         // its own D-deep ring.  Lanes past ni read the last valid column and store nothing.
-        template <class ST, class CT, int NK, int D, bool UPOS_LDS>
+        // SBM: instruction classes the per-step scheduling barrier lets through (0: none; 6: VALU and SALU, so
+        // the next step's independent arithmetic can fill the latency of this step's pivot chain while memory
+        // operations stay at their step)
+        template <class ST, class CT, int NK, int D, bool UPOS_LDS, int SBM = 0>
         __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
         vadv_onchip_kernel(ST *__restrict__ uts_f, const ST *__restrict__ us_f, const ST *__restrict__ wc_f,
             const ST *__restrict__ up_f, const ST *__restrict__ ut_f, CT dtr, int ni, int nk, int64_t sj, int64_t sk,
@@ -187,9 +190,10 @@ namespace gt355 {
                 us_c = us_p;
                 w_c_i = w_p_i;
                 w_c_ip = w_p_ip;
-                // one scheduling region per step: the loads stay where the ring puts them (D levels ahead) instead
-                // of being hoisted wholesale, which would need a register per in-flight value
-                __builtin_amdgcn_sched_barrier(0);
+                // one scheduling region per step for memory operations: the loads stay where the ring puts them
+                // (D levels ahead) instead of being hoisted wholesale, which would need a register per in-flight
+                // value
+                __builtin_amdgcn_sched_barrier(SBM);
             }
 
             ST r_up[D]; // backward u_pos ring (!UPOS_LDS): level L consumed at backward step L, loaded at step L + D
@@ -219,7 +223,7 @@ namespace gt355 {
                 // the top overwrites later in program order, and lanes past ni compute column ni-1 and write its
                 // bits again.  No branch, so the load counter stays exact.
                 uts[lvl(k) * sk_o] = ST(out);
-                __builtin_amdgcn_sched_barrier(0);
+                __builtin_amdgcn_sched_barrier(SBM);
             }
         }
 
@@ -292,6 +296,27 @@ namespace gt355 {
                 // wavefronts still fill a CU's 160 KB) and saves the backward re-read; measured faster for FP64 and
                 // FP32 storage at 256x256x80 (54.5 vs 56.3 us FP64).
                 const bool upos_lds = tune_param("vadv_upos_lds", 1) != 0;
+                // deepest bucket: the load-ahead depth D is selectable (GT355_TUNE_vadv_d)
+                if (nk > 64 && upos_lds) {
+                    const int d = tune_param("vadv_d", 8);
+#define GT355_VADV_D(DD)                                                                                         \
+    if (d == DD) {                                                                                               \
+        vadv_onchip_kernel<ST, CT, 80, DD, true><<<grid, 64, 0, stream>>>((ST *)uts->origin,                     \
+            (const ST *)us->origin, (const ST *)wc->origin, (const ST *)up->origin, (const ST *)ut->origin, dtr_c, \
+            ni, nk, sj, sk, wc->stride[1], wc->stride[2]);                                                       \
+        return check_launch("gt355_vadv");                                                                       \
+    }
+                    GT355_VADV_D(4)
+                    GT355_VADV_D(12)
+                    GT355_VADV_D(16)
+#undef GT355_VADV_D
+                    if (tune_param("vadv_sbm", 0) == 6) {
+                        vadv_onchip_kernel<ST, CT, 80, 8, true, 6><<<grid, 64, 0, stream>>>((ST *)uts->origin,
+                            (const ST *)us->origin, (const ST *)wc->origin, (const ST *)up->origin,
+                            (const ST *)ut->origin, dtr_c, ni, nk, sj, sk, wc->stride[1], wc->stride[2]);
+                        return check_launch("gt355_vadv");
+                    }
+                }
                 GT355_VADV_ONCHIP(16)
                 GT355_VADV_ONCHIP(32)
                 GT355_VADV_ONCHIP(48)
