@@ -97,6 +97,23 @@ def timed_steps(gt, sets, precision, steps, warmup):
     return total, hd_ms, va_ms
 
 
+def autotune(gt, dtype, precision, shapes):
+    """Runs the project's autotuner for hdiff on each (n, nk) shape (outside every timed region) on scratch fields
+    of that shape; later launches of that shape use the recorded winner.  Returns {shape: (config, us)}."""
+    from gridtools_amd import tune
+
+    out = {}
+    for n, nk in shapes:
+        hb = gt.builder().type(dtype).dimensions(n + 4, n + 4, nk).halos(2, 2, 0)
+        i_ = hb.uniform(SEED, 1, -1.0, 1.0).build()
+        c_ = hb.uniform(SEED, 2, 0.0, 0.25).build()
+        o_ = hb.value(0).build()
+        g = gt.make_grid(gt.halo_descriptor.symmetric(2, n + 4), gt.halo_descriptor.symmetric(2, n + 4), nk)
+        cfg, us = tune.hdiff(o_, i_, c_, g, precision=precision, reps=30)
+        out[f"{n}x{n}x{nk}"] = {"config": cfg, "us": round(us, 2)}
+    return out
+
+
 def kernel_stats(name, ms, bytes_per_elem):
     algo = BYTES_PER_PT[name] * bytes_per_elem * P
     return {
@@ -197,12 +214,14 @@ def compose_leg(gt, steps, warmup, n=512, nk=80, nsets=2):
             run(sets[s % nsets], prec)
         # default dispatch: hdiff launch, then vadv launch (u_stage through HBM)
         res[name] = _event_time(lambda s: run(sets[s % nsets], prec), steps)
-        # the fused single launch, for comparison
-        os.environ["GT355_TUNE_hv_fused"] = "1"
-        try:
-            two[name] = _event_time(lambda s: run(sets[s % nsets], prec), steps)
-        finally:
-            del os.environ["GT355_TUNE_hv_fused"]
+        # the fused single-launch kernels, for comparison: LDS tile (1) and DPP register sweep (2)
+        two[name] = {}
+        for variant, key in (("1", "lds_tile"), ("2", "dpp_sweep")):
+            os.environ["GT355_TUNE_hv_fused"] = variant
+            try:
+                two[name][key] = round(_event_time(lambda s: run(sets[s % nsets], prec), steps) * 1e3, 2)
+            finally:
+                del os.environ["GT355_TUNE_hv_fused"]
         del sets
     pts = n * n * nk
     # errors on one fresh input set: FP32 form vs the FP64 path on the widened FP32 inputs
@@ -230,7 +249,7 @@ def compose_leg(gt, steps, warmup, n=512, nk=80, nsets=2):
         gbs = 9 * esize * pts / (ms * 1e-3) / 1e9
         out[name] = {"mean_us": round(ms * 1e3, 2), "gpts_per_s": round(pts / (ms * 1e-3) / 1e9, 3),
                      "achieved_gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
-    out["fused_single_launch_us"] = {k: round(v * 1e3, 2) for k, v in two.items()}
+    out["fused_single_launch_us"] = two
     out["speedup_fp32_form_vs_fp64"] = round(res["fp64"] / res["fp32_form"], 3)
     out["errors_vs_fp64"] = errors
     return out
@@ -355,8 +374,9 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--sets", type=int, default=4)
-    ap.add_argument("--legs", default="fp64,fp32,copy,compose,batch,large,cpu",
-                    help="comma list of legs: fp64 (always run), fp32 (precision form per kernel), copy (bandwidth "
+    ap.add_argument("--legs", default="tune,fp64,fp32,copy,compose,batch,large,cpu",
+                    help="comma list of legs: tune (hdiff autotuner first, untimed), fp64 (always run), fp32 "
+                         "(precision form per kernel), copy (bandwidth "
                          "yardstick), compose (precision form on the fused composition, 512x512x80), batch (64 "
                          "uneven domains, persistent launch, graph of 10 timesteps), large (1536x1536x80, 20 "
                          "timesteps with boundary conditions), cpu (CPU baseline)")
@@ -375,6 +395,13 @@ def main():
     info = gt.device_info()
     print(f"[bench] device {info['name']} {info['arch']} CUs={info['cu_count']}", file=sys.stderr, flush=True)
 
+    tuned = None
+    if "tune" in legs:
+        # hdiff tiles chosen by the autotuner for every hdiff shape this run times (FP64 and the FP32 form)
+        print("[bench] autotune hdiff", file=sys.stderr, flush=True)
+        shapes = [(NI, NK), (512, NK)]
+        tuned = {"fp64": autotune(gt, np.float64, gt.NATIVE, shapes),
+                 "fp32_form": autotune(gt, np.float32, gt.WIDEN, shapes)}
     sets64 = [make_set(gt, np.float64, s) for s in range(args.sets)]
     total, hd_ms, va_ms = timed_steps(gt, sets64, gt.NATIVE, args.steps, args.warmup)
     print(f"[bench] fp64 {args.steps} steps: {total:.3f} ms", file=sys.stderr, flush=True)
@@ -436,6 +463,7 @@ def main():
         "config": {"workload": "hdiff + vadv FP64, 256x256x80 interior (halo 2 / 3), one launch each per step",
                    "field_sets": args.sets, "parallelism": "single GPU"},
         "kernels": {"hdiff_fp64": hd64, "vadv_fp64": va64},
+        "autotuned_hdiff": tuned,
         "fp32_form": fp32,
         "copy_fp64": copy,
         "precision_form_composition": compose,

@@ -20,6 +20,7 @@
 
 #include "common.hpp"
 #include "functors.hpp"
+#include "strip.hpp"
 #include "tuning.hpp"
 
 namespace gt355 {
@@ -211,6 +212,36 @@ namespace gt355 {
             }
         }
 
+
+        // Work item of the strip form: 64 columns (from i0) of row j of one domain.
+        struct sitem {
+            int32_t dom, i0, j, pad;
+        };
+        constexpr int SD = 4;                       // load-ahead depth of the strip sweep
+        constexpr int STAIL_ROWS = BMAXK - 80 + SD; // tail levels a wavefront's scratch holds
+
+        // The batching form on the strip sweep (strip.hpp): one wavefront per workgroup and per item, no LDS
+        // tile, no barrier, ccol/dcol in registers for the first 80 levels and in the wavefront's tail scratch
+        // beyond.  One sweep variant serves every depth (a column shorter than 80 levels runs the register part
+        // with its levels past the top clamped; separate shorter variants in the same kernel would share one
+        // register allocation and spill).  Items are taken with a static stride (no counters), longest columns
+        // first.
+        template <class ST, class CT>
+        __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
+        batch_strip_kernel(const bdom *__restrict__ doms, const sitem *__restrict__ items, int nitems, CT dtr,
+            CT *__restrict__ scratch, int nxcd) {
+            __shared__ ST s_up[80][64];
+            CT *tail = scratch + (int64_t)blockIdx.x * (STAIL_ROWS * 128);
+            const int wid = xcd_remap(blockIdx.x, gridDim.x, nxcd);
+            for (int it = wid; it < nitems; it += gridDim.x) {
+                const sitem item = items[it];
+                const bdom dm = doms[item.dom];
+                const strip_fields<ST> f{(const ST *)dm.in, (const ST *)dm.coeff, (const ST *)dm.wcon,
+                    (const ST *)dm.u_pos, (const ST *)dm.utens, (ST *)dm.u_stage, (ST *)dm.utens_stage,
+                    (ST *)dm.lap, dm.sj, dm.sk, dm.sj, dm.sk, dm.sj, dm.sk, dm.sj, dm.sk, dm.wsj, dm.wsk};
+                hv_strip<ST, CT, 80, SD, true, true, true>(f, dtr, dm.ni, dm.nj, dm.nk, item.i0, item.j, s_up, tail);
+            }
+        }
     } // namespace
 } // namespace gt355
 
@@ -218,15 +249,21 @@ using namespace gt355;
 
 struct gt355_batch {
     bdom *doms;    // device table
-    bitem *items;  // device table
-    void *scratch; // ccol/dcol: gridmax workgroups x 2 x rows x 256 columns of the compute type
+    void *items;   // device table: bitem (tile form) or sitem (strip form)
+    void *scratch; // ccol/dcol of every workgroup the grid may have, in the compute type
+    int impl;      // 1: 64x4 tiles, ccol/dcol in scratch; 2: 64-column strips, ccol/dcol in registers
     int ndoms, nitems, dtype, precision, grid, gridmax, cus, rows;
 };
 
 namespace gt355 {
     namespace {
-        // workgroups per CU the persistent grid may use (the kernels fit 3-4 workgroups of 256 lanes per CU)
+        // workgroups per CU the persistent grid may use, by tuner setting w = 1..BWG_MAX: the tile form fits 3-4
+        // workgroups of 256 lanes per CU (w per CU); the strip form one single-wavefront workgroup per SIMD
+        // (2w per CU: 2 fills half the SIMDs, 4 all of them, 6 and 8 queue a second round)
         constexpr int BWG_MAX = 4;
+        inline int batch_grid(const gt355_batch *bt, int w) {
+            return std::min<int>(bt->nitems, bt->cus * (bt->impl == 2 ? 2 * w : w));
+        }
     } // namespace
 } // namespace gt355
 
@@ -240,8 +277,11 @@ extern "C" int gt355_batch_create(int precision, const gt355_batch_domain *domai
         "gt355_batch_create: precision %d with dtype %d", precision, dt);
     GT355_CHECK(wg_per_cu >= 1 && wg_per_cu <= BWG_MAX, "gt355_batch_create: wg_per_cu %d (1..%d)", wg_per_cu,
         BWG_MAX);
+    // the strip form (default) or the tile form (GT355_TUNE_batch_impl=1)
+    const int impl = tune_param("batch_impl", 2) == 1 ? 1 : 2;
     std::vector<bdom> doms(ndomains);
     std::vector<bitem> items;
+    std::vector<sitem> sitems;
     for (int d = 0; d < ndomains; ++d) {
         const gt355_batch_domain &b = domains[d];
         const int ni = b.ni, nj = b.nj, nk = b.nk;
@@ -279,39 +319,56 @@ extern "C" int gt355_batch_create(int precision, const gt355_batch_domain *domai
         m.nj = nj;
         m.nk = nk;
         m.tiles_i = (ni + BTI - 1) / BTI;
-        const int tj_n = (nj + BTJ - 1) / BTJ;
-        for (int tj = 0; tj < tj_n; ++tj)
-            for (int ti = 0; ti < m.tiles_i; ++ti)
-                items.push_back(bitem{d, ti, tj, 0});
+        if (impl == 2) {
+            const int64_t lim = int64_t(1) << 32, es = dt == GT355_F64 ? 8 : 4;
+            GT355_CHECK(((int64_t)(nk - 1) * m.sk + (int64_t)(nj + 3) * m.sj + ni + 4) * es < lim &&
+                            ((int64_t)(nk - 1) * m.wsk + (int64_t)(nj + 1) * m.wsj + ni + 1) * es < lim,
+                "gt355_batch_create: domain %d spans 4 GiB or more", d);
+            const int strips = (ni + 63) / 64;
+            for (int j = 0; j < nj; ++j)
+                for (int si = 0; si < strips; ++si)
+                    sitems.push_back(sitem{d, si * 64, j, 0}); // disjoint strips: each column has one owner
+        } else {
+            const int tj_n = (nj + BTJ - 1) / BTJ;
+            for (int tj = 0; tj < tj_n; ++tj)
+                for (int ti = 0; ti < m.tiles_i; ++ti)
+                    items.push_back(bitem{d, ti, tj, 0});
+        }
     }
     // longest columns first, so the strided schedule ends with short items (a greedy balance)
     std::stable_sort(items.begin(), items.end(),
         [&](const bitem &a, const bitem &b) { return doms[a.dom].nk > doms[b.dom].nk; });
+    std::stable_sort(sitems.begin(), sitems.end(),
+        [&](const sitem &a, const sitem &b) { return doms[a.dom].nk > doms[b.dom].nk; });
     hipDeviceProp_t p;
     GT355_HIP(hipGetDeviceProperties(&p, current_device()));
     gt355_batch *bt = new gt355_batch{};
+    bt->impl = impl;
     bt->ndoms = ndomains;
-    bt->nitems = (int)items.size();
+    bt->nitems = impl == 2 ? (int)sitems.size() : (int)items.size();
     bt->dtype = dt;
     bt->precision = precision;
     bt->cus = p.multiProcessorCount;
-    bt->grid = std::min<int>(bt->nitems, bt->cus * wg_per_cu);
-    bt->gridmax = std::min<int>(bt->nitems, bt->cus * BWG_MAX); // scratch for any grid the tuner may pick
+    bt->grid = batch_grid(bt, wg_per_cu);
+    bt->gridmax = batch_grid(bt, BWG_MAX); // scratch for any grid the tuner may pick
+    const size_t item_bytes = impl == 2 ? sizeof(sitem) * sitems.size() : sizeof(bitem) * items.size();
+    const void *item_src = impl == 2 ? (const void *)sitems.data() : (const void *)items.data();
     hipError_t e = hipMalloc(&bt->doms, sizeof(bdom) * doms.size());
     if (e == hipSuccess)
-        e = hipMalloc(&bt->items, sizeof(bitem) * items.size());
+        e = hipMalloc(&bt->items, item_bytes);
     if (e == hipSuccess)
         e = hipMemcpy(bt->doms, doms.data(), sizeof(bdom) * doms.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess)
-        e = hipMemcpy(bt->items, items.data(), sizeof(bitem) * items.size(), hipMemcpyHostToDevice);
+        e = hipMemcpy(bt->items, item_src, item_bytes, hipMemcpyHostToDevice);
     // each batch owns its scratch, so batches on different streams never share it
     int maxnk = 0;
     for (const bdom &m : doms)
         maxnk = std::max(maxnk, m.nk);
     bt->rows = maxnk + BPAST;
     const size_t ct = (dt == GT355_F64 || precision == GT355_WIDEN) ? 8 : 4;
+    const size_t per_wg = impl == 2 ? (size_t)STAIL_ROWS * 128 : 2 * (size_t)bt->rows * 256;
     if (e == hipSuccess)
-        e = hipMalloc(&bt->scratch, ct * 2 * (size_t)bt->rows * 256 * (size_t)bt->gridmax);
+        e = hipMalloc(&bt->scratch, ct * per_wg * (size_t)bt->gridmax);
     if (e != hipSuccess) {
         (void)hipFree(bt->doms);
         (void)hipFree(bt->items);
@@ -327,15 +384,29 @@ extern "C" int gt355_batch_run(gt355_batch *bt, double dtr_stage, void *stream) 
     GT355_CHECK(bt != nullptr, "gt355_batch_run: null batch");
     void *scr = bt->scratch;
     hipStream_t s = (hipStream_t)stream;
+    if (bt->impl == 2) {
+        const sitem *it = (const sitem *)bt->items;
+        if (bt->dtype == GT355_F64)
+            batch_strip_kernel<double, double><<<bt->grid, 64, 0, s>>>(bt->doms, it, bt->nitems, dtr_stage,
+                (double *)scr, xcd_count());
+        else if (bt->precision == GT355_WIDEN)
+            batch_strip_kernel<float, double><<<bt->grid, 64, 0, s>>>(bt->doms, it, bt->nitems,
+                double(float(dtr_stage)), (double *)scr, xcd_count());
+        else
+            batch_strip_kernel<float, float><<<bt->grid, 64, 0, s>>>(bt->doms, it, bt->nitems, float(dtr_stage),
+                (float *)scr, xcd_count());
+        return check_launch("gt355_batch_run");
+    }
+    const bitem *it = (const bitem *)bt->items;
     const int D = 4;
     if (bt->dtype == GT355_F64)
-        batch_kernel<double, double, D><<<bt->grid, 256, 0, s>>>(bt->doms, bt->items, bt->nitems, dtr_stage,
+        batch_kernel<double, double, D><<<bt->grid, 256, 0, s>>>(bt->doms, it, bt->nitems, dtr_stage,
             (double *)scr, bt->rows, xcd_count());
     else if (bt->precision == GT355_WIDEN)
-        batch_kernel<float, double, D><<<bt->grid, 256, 0, s>>>(bt->doms, bt->items, bt->nitems,
+        batch_kernel<float, double, D><<<bt->grid, 256, 0, s>>>(bt->doms, it, bt->nitems,
             double(float(dtr_stage)), (double *)scr, bt->rows, xcd_count());
     else
-        batch_kernel<float, float, D><<<bt->grid, 256, 0, s>>>(bt->doms, bt->items, bt->nitems, float(dtr_stage),
+        batch_kernel<float, float, D><<<bt->grid, 256, 0, s>>>(bt->doms, it, bt->nitems, float(dtr_stage),
             (float *)scr, bt->rows, xcd_count());
     return check_launch("gt355_batch_run");
 }
@@ -369,8 +440,8 @@ extern "C" int gt355_batch_tune(gt355_batch *bt, int reps, void *stream, int *wg
     int best = 0, rc = GT355_OK;
     double best_t = 1e300;
     for (int w = 1; w <= BWG_MAX && rc == GT355_OK; ++w) {
-        const int g = std::min<int>(bt->nitems, bt->cus * w);
-        if (w > 1 && g == std::min<int>(bt->nitems, bt->cus * (w - 1)))
+        const int g = batch_grid(bt, w);
+        if (w > 1 && g == batch_grid(bt, w - 1))
             break; // fewer items than workgroups: larger grids are the same grid
         bt->grid = g;
         if ((rc = gt355_batch_run(bt, 0.15, stream)) != GT355_OK) // warm-up
@@ -396,7 +467,7 @@ extern "C" int gt355_batch_tune(gt355_batch *bt, int reps, void *stream, int *wg
     (void)hipEventDestroy(e1);
     if (rc != GT355_OK)
         return rc;
-    bt->grid = std::min<int>(bt->nitems, bt->cus * best);
+    bt->grid = batch_grid(bt, best);
     if (wg_per_cu)
         *wg_per_cu = best;
     if (best_us)

@@ -64,4 +64,41 @@ namespace gt355 {
         return (n % nxcd) * per + n / nxcd;
     }
 
+    // Buffer-resource access: a field is a buffer descriptor (base in scalar registers), a level is a uniform
+    // 32-bit byte offset (soffset) and a lane's column a 32-bit byte offset (voffset).  One level offset serves
+    // every field of one layout, so a load costs no per-lane or per-field address arithmetic.  Offsets are byte
+    // offsets below 4 GiB (launchers check the field span).  Data format 32 (word 3 = 0x00020000) enables
+    // untyped dword access; num_records 0xFFFFFFFF disables the range clamp.
+    using rsrc_t = __amdgpu_buffer_rsrc_t;
+    __device__ __forceinline__ rsrc_t make_rsrc(const void *base) {
+        return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)0xFFFFFFFF, 0x00020000);
+    }
+    template <class T>
+    __device__ __forceinline__ T bload(rsrc_t r, uint32_t voff, uint32_t soff) {
+        if constexpr (sizeof(T) == 4)
+            return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+        else
+            return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+    }
+    template <class T>
+    struct two {
+        T a, b;
+    };
+    // two consecutive elements (columns i and i+1) in one load
+    template <class T>
+    __device__ __forceinline__ two<T> bload2(rsrc_t r, uint32_t voff, uint32_t soff) {
+        if constexpr (sizeof(T) == 4)
+            return __builtin_bit_cast(two<T>, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+        else
+            return __builtin_bit_cast(two<T>, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+    }
+    template <class T>
+    __device__ __forceinline__ void bstore(rsrc_t r, T v, uint32_t voff, uint32_t soff) {
+        if constexpr (sizeof(T) == 4)
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, voff, soff, 0);
+        else
+            __builtin_amdgcn_raw_buffer_store_b64(
+                __builtin_bit_cast(decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0)), v), r, voff, soff, 0);
+    }
+
 } // namespace gt355

@@ -20,6 +20,7 @@
 
 #include "common.hpp"
 #include "functors.hpp"
+#include "strip.hpp"
 #include "tuning.hpp"
 
 namespace gt355 {
@@ -201,6 +202,65 @@ namespace gt355 {
             }
         }
 
+
+        // hv_dpp_kernel: the composition with no LDS tile and no barrier (the strip sweep of strip.hpp).  One
+        // wavefront per workgroup owns 64 columns of one row; NARROW (ni not a multiple of 64) lets the lanes past
+        // ni of the last strip repeat column ni-1.
+        template <class ST, class CT, int NK, int D, bool NARROW>
+        __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
+        hv_dpp_kernel(strip_fields<ST> f, CT dtr, int ni, int nj, int nk, int strips_i, int nxcd) {
+            __shared__ ST s_up[NK][64];
+            // consecutive ids are consecutive rows of one i-strip: five neighbouring waves share each row of `in`
+            const int id = xcd_remap(blockIdx.x, gridDim.x, nxcd);
+            const int si = id / nj, j = id % nj;
+            hv_strip<ST, CT, NK, D, NARROW, false, false>(f, dtr, ni, nj, nk, si * 64, j, s_up, nullptr);
+        }
+
+        // largest byte offset a strip sweep forms in any of its fields (buffer offsets are 32-bit)
+        inline int64_t strip_span(const gt355_field *f, int elem, int ni, int nj, int nk, int lo, int hi_i) {
+            return (((int64_t)(nk - 1)) * f->stride[2] + (int64_t)(nj + 1 - lo) * f->stride[1] + hi_i - lo) * elem;
+        }
+
+        template <class ST, class CT>
+        int launch_dpp(const gt355_field *in, const gt355_field *coeff, const gt355_field *u_stage,
+            const gt355_field *uts, const gt355_field *wc, const gt355_field *up, const gt355_field *ut, CT dtr,
+            int ni, int nj, int nk, hipStream_t stream) {
+            const int strips_i = (ni + 63) / 64;
+            const int64_t blocks = (int64_t)strips_i * nj;
+            GT355_CHECK(blocks < (1LL << 31), "gt355_hdiff_vadv: domain too large");
+            const int64_t lim = int64_t(1) << 32;
+            GT355_CHECK(strip_span(in, sizeof(ST), ni, nj, nk, -2, ni + 2) < lim &&
+                            strip_span(coeff, sizeof(ST), ni, nj, nk, 0, ni) < lim &&
+                            strip_span(u_stage, sizeof(ST), ni, nj, nk, 0, ni) < lim &&
+                            strip_span(uts, sizeof(ST), ni, nj, nk, 0, ni) < lim &&
+                            strip_span(up, sizeof(ST), ni, nj, nk, 0, ni) < lim &&
+                            strip_span(ut, sizeof(ST), ni, nj, nk, 0, ni) < lim &&
+                            strip_span(wc, sizeof(ST), ni, nj, nk, 0, ni + 1) < lim,
+                "gt355_hdiff_vadv: the fused strip kernel addresses fields below 4 GiB");
+            strip_fields<ST> f{(const ST *)in->origin, (const ST *)coeff->origin, (const ST *)wc->origin,
+                (const ST *)up->origin, (const ST *)ut->origin, (ST *)u_stage->origin, (ST *)uts->origin, nullptr,
+                in->stride[1], in->stride[2], coeff->stride[1], coeff->stride[2], u_stage->stride[1],
+                u_stage->stride[2], uts->stride[1], uts->stride[2], wc->stride[1], wc->stride[2]};
+#define GT355_DPP(NKB)                                                                                           \
+    if (nk <= NKB) {                                                                                             \
+        if (ni % 64 != 0)                                                                                        \
+            hv_dpp_kernel<ST, CT, NKB, 4, true><<<(int)blocks, 64, 0, stream>>>(f, dtr, ni, nj, nk, strips_i,     \
+                xcd_count());                                                                                    \
+        else                                                                                                     \
+            hv_dpp_kernel<ST, CT, NKB, 4, false><<<(int)blocks, 64, 0, stream>>>(f, dtr, ni, nj, nk, strips_i,    \
+                xcd_count());                                                                                    \
+        return check_launch("gt355_hdiff_vadv");                                                                 \
+    }
+            GT355_DPP(16)
+            GT355_DPP(32)
+            GT355_DPP(48)
+            GT355_DPP(64)
+            GT355_DPP(80)
+#undef GT355_DPP
+            set_error("gt355_hdiff_vadv: nk %d exceeds the fused kernel's deepest bucket", nk);
+            return GT355_EUNSUP;
+        }
+
         template <class ST, class CT, int TJ>
         int launch_fused_tj(const gt355_field *in, const gt355_field *coeff, const gt355_field *u_stage,
             const gt355_field *uts, const gt355_field *wc, const gt355_field *up, const gt355_field *ut, CT dtr,
@@ -238,6 +298,8 @@ namespace gt355 {
         int launch_fused(const gt355_field *in, const gt355_field *coeff, const gt355_field *u_stage,
             const gt355_field *uts, const gt355_field *wc, const gt355_field *up, const gt355_field *ut, CT dtr,
             int ni, int nj, int nk, hipStream_t stream) {
+            if (tune_param("hv_fused", 0) == 2)
+                return launch_dpp<ST, CT>(in, coeff, u_stage, uts, wc, up, ut, dtr, ni, nj, nk, stream);
             if (tune_param("hv_tj", 4) == 1)
                 return launch_fused_tj<ST, CT, 1>(in, coeff, u_stage, uts, wc, up, ut, dtr, ni, nj, nk, stream);
             return launch_fused_tj<ST, CT, 4>(in, coeff, u_stage, uts, wc, up, ut, dtr, ni, nj, nk, stream);

@@ -142,22 +142,6 @@ namespace gt355 {
         // the wave, and that old value is exactly the side column.  Loads run P rows ahead through register
         // rings; all loads are unconditional (clamped to the valid box) so the load counter stays exact.
         // ---------------------------------------------------------------------------------------------------------
-        constexpr int DPP_WF_SL1 = 0x130; // lane L reads lane L+1; lane 63 keeps its old value
-        constexpr int DPP_WF_SR1 = 0x138; // lane L reads lane L-1; lane 0 keeps its old value
-
-        template <int CTRL>
-        __device__ __forceinline__ float dpp_keep(float old, float src) {
-            return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old),
-                                                 __builtin_bit_cast(int, src), CTRL, 0xF, 0xF, false));
-        }
-        template <int CTRL>
-        __device__ __forceinline__ double dpp_keep(double old, double src) {
-            const long long o = __builtin_bit_cast(long long, old), s = __builtin_bit_cast(long long, src);
-            const int lo = __builtin_amdgcn_update_dpp(int(o), int(s), CTRL, 0xF, 0xF, false);
-            const int hi = __builtin_amdgcn_update_dpp(int(o >> 32), int(s >> 32), CTRL, 0xF, 0xF, false);
-            return __builtin_bit_cast(double, (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
-        }
-
         template <class ST, class CT, int JB, int P>
         __global__ void __launch_bounds__(256) hdiff_sweep_kernel(ST *__restrict__ out, const ST *__restrict__ in,
             const ST *__restrict__ coeff, int ni, int nj, int nk, int64_t osj, int64_t osk, int64_t isj, int64_t isk,

@@ -116,19 +116,18 @@ namespace gt355 {
             const int i = min(int(blockIdx.x * 64 + threadIdx.x), ni - 1);
             const int j = blockIdx.y;
             const int top = nk - 1;
-            const int64_t col = (int64_t)j * sj + i;
-            const int64_t wcol = (int64_t)j * wsj + i;
-            const ST *us = us_f + col;
-            const ST *up = up_f + col;
-            const ST *ut = ut_f + col;
-            ST *uts = uts_f + col;
-            const ST *wc = wc_f + wcol;
-            auto lvl = [&](int L) { return (int64_t)min(L, top); };
+            // buffer addressing: per-field descriptors, a per-level byte offset shared by the fields of one
+            // layout, the lane's column as a byte offset (the launcher checks every offset stays below 4 GiB)
+            const uint32_t colb = (uint32_t)(((int64_t)j * sj + i) * (int64_t)sizeof(ST));
+            const uint32_t wcolb = (uint32_t)(((int64_t)j * wsj + i) * (int64_t)sizeof(ST));
+            const rsrc_t r_us = make_rsrc(us_f), r_wc = make_rsrc(wc_f), rs_up = make_rsrc(up_f),
+                         r_ut = make_rsrc(ut_f), r_uts = make_rsrc(uts_f);
+            auto lvl = [&](int L) { return (uint32_t)min(L, top); };
             // The loads read memory nothing in the forward sweep writes, so the compiler would treat them as
             // invariant and hoist all of them to the kernel entry (one register per in-flight value: ~800 VGPRs).
             // Routing the strides through an empty volatile asm at every step gives each step's loads an opaque
             // dependency that only exists from that step on.
-            int64_t sk_o = sk, wsk_o = wsk;
+            uint32_t sk_o = (uint32_t)(sk * (int64_t)sizeof(ST)), wsk_o = (uint32_t)(wsk * (int64_t)sizeof(ST));
             auto pin = [&]() {
                 asm volatile("" : "+s"(sk_o));
                 asm volatile("" : "+s"(wsk_o));
@@ -137,16 +136,17 @@ namespace gt355 {
             ST a_us[D], a_wi[D], a_wip[D]; // ring A
             ST b_uts[D], b_up[D], b_ut[D]; // ring B
             auto load_a = [&](int L) {
-                const int64_t l = lvl(L);
-                a_us[L % D] = us[l * sk_o];
-                a_wi[L % D] = wc[l * wsk_o];
-                a_wip[L % D] = wc[l * wsk_o + 1];
+                const uint32_t o = lvl(L) * sk_o, w = lvl(L) * wsk_o;
+                a_us[L % D] = bload<ST>(r_us, colb, o);
+                const two<ST> ww = bload2<ST>(r_wc, wcolb, w); // wcon at i and i+1
+                a_wi[L % D] = ww.a;
+                a_wip[L % D] = ww.b;
             };
             auto load_b = [&](int L) {
-                const int64_t l = lvl(L);
-                b_uts[L % D] = uts[l * sk_o];
-                b_up[L % D] = up[l * sk_o];
-                b_ut[L % D] = ut[l * sk_o];
+                const uint32_t o = lvl(L) * sk_o;
+                b_uts[L % D] = bload<ST>(r_uts, colb, o);
+                b_up[L % D] = bload<ST>(rs_up, colb, o);
+                b_ut[L % D] = bload<ST>(r_ut, colb, o);
             };
 #pragma clang loop unroll(full)
             for (int L = 0; L < D; ++L) {
@@ -156,7 +156,7 @@ namespace gt355 {
 
             __shared__ ST s_up[UPOS_LDS ? NK : 1][64];
             CT c[NK], d[NK];
-            CT us_m = CT(0), us_c = CT(us[0]), w_c_i = CT(0), w_c_ip = CT(0);
+            CT us_m = CT(0), us_c = CT(bload<ST>(r_us, colb, 0)), w_c_i = CT(0), w_c_ip = CT(0);
 #pragma clang loop unroll(full)
             for (int k = 0; k < NK; ++k) {
                 if constexpr (UPOS_LDS)
@@ -200,7 +200,7 @@ namespace gt355 {
             if constexpr (!UPOS_LDS) {
 #pragma clang loop unroll(full)
                 for (int s = 0; s < D; ++s)
-                    r_up[(NK - 1 - s) % D] = up[lvl(NK - 1 - s) * sk];
+                    r_up[(NK - 1 - s) % D] = bload<ST>(rs_up, colb, lvl(NK - 1 - s) * sk_o);
             }
             CT data = CT(0);
 #pragma clang loop unroll(full)
@@ -212,7 +212,7 @@ namespace gt355 {
                     v_up = CT(r_up[k % D]);
                     pin();
                     if (k - D >= 0)
-                        r_up[k % D] = up[lvl(k - D) * sk_o];
+                        r_up[k % D] = bload<ST>(rs_up, colb, lvl(k - D) * sk_o);
                 }
                 // u_backward_function (vertical_advection_dycore.cpp:110-121): the top level starts the recurrence
                 const CT dt = d[k] - c[k] * data;
@@ -222,7 +222,7 @@ namespace gt355 {
                 // Unconditional store to the clamped level: a step above the top writes a value that the step at
                 // the top overwrites later in program order, and lanes past ni compute column ni-1 and write its
                 // bits again.  No branch, so the load counter stays exact.
-                uts[lvl(k) * sk_o] = ST(out);
+                bstore<ST>(r_uts, ST(out), colb, lvl(k) * sk_o);
                 __builtin_amdgcn_sched_barrier(SBM);
             }
         }
@@ -277,7 +277,11 @@ namespace gt355 {
                 "gt355_vadv: utens_stage, u_stage, u_pos and utens must share one layout");
             // the reference keeps dtr_stage as a global_parameter of the storage type (vertical_advection_dycore.cpp:139)
             const CT dtr_c = CT(ST(dtr_stage));
-            if (nk <= onchip_max_nk<CT>()) {
+            // the on-chip kernel addresses fields through buffer descriptors with 32-bit byte offsets
+            const int64_t span = std::max(((int64_t)(nk - 1) * sk + (int64_t)(nj - 1) * sj + ni) * (int64_t)sizeof(ST),
+                ((int64_t)(nk - 1) * wc->stride[2] + (int64_t)(nj - 1) * wc->stride[1] + ni + 1) * (int64_t)sizeof(ST));
+            const bool fits32 = span < (int64_t(1) << 32);
+            if (nk <= onchip_max_nk<CT>() && fits32) {
                 GT355_CHECK(nj <= 65535, "gt355_vadv: nj too large");
                 dim3 grid((ni + 63) / 64, nj);
 #define GT355_VADV_ONCHIP(NKB)                                                                                   \

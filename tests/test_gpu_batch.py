@@ -15,6 +15,8 @@ pytestmark = pytest.mark.gpu
 H = 3
 SIZES = [(23, 11, 43), (70, 9, 81), (130, 37, 2), (64, 4, 137), (65, 5, 96), (48, 48, 144), (1, 1, 3)]
 WIDE = [(512, 256, 40), (97, 300, 61)]
+# strip-form edge cases: ni of exactly one strip, one column past a strip, depths 78..81 around the register part
+NARROW = [(64, 3, 80), (65, 2, 78), (128, 2, 81), (63, 4, 79), (2, 2, 137)]
 
 
 def oracle_domain(oracle, host, ni, nj, nk, dtr, mode):
@@ -87,13 +89,19 @@ def check_batch(gt, oracle, sizes, dtype, precision, wg_per_cu):
     b.close()
 
 
+@pytest.mark.parametrize("impl", [2, 1])
 @pytest.mark.parametrize("dtype,precision", [(np.float64, 0), (np.float32, 1), (np.float32, 0)])
-def test_batch_uneven_domains_bit_exact(gpu, oracle, dtype, precision):
-    check_batch(gpu, oracle, SIZES, dtype, precision, wg_per_cu=2)
+def test_batch_uneven_domains_bit_exact(gpu, oracle, monkeypatch, dtype, precision, impl):
+    """impl 2: 64-column strips with ccol/dcol in registers (tail scratch past 80 levels); impl 1: 64x4 tiles
+    with ccol/dcol in scratch."""
+    monkeypatch.setenv("GT355_TUNE_batch_impl", str(impl))
+    check_batch(gpu, oracle, SIZES + NARROW, dtype, precision, wg_per_cu=2)
 
 
+@pytest.mark.parametrize("impl", [2, 1])
 @pytest.mark.parametrize("dtype,precision", [(np.float64, 0), (np.float32, 1)])
-def test_batch_more_items_than_workgroups(gpu, oracle, dtype, precision):
+def test_batch_more_items_than_workgroups(gpu, oracle, monkeypatch, dtype, precision, impl):
+    monkeypatch.setenv("GT355_TUNE_batch_impl", str(impl))
     check_batch(gpu, oracle, WIDE + SIZES[:3], dtype, precision, wg_per_cu=1)
 
 

@@ -60,8 +60,9 @@ def run_gpu(f, grid, dtr, precision):
             f["utens_stage"].padded_numpy()[:nk, H:H + nj, H:H + ni])
 
 
-@pytest.mark.parametrize("fused", [1, 0])
-@pytest.mark.parametrize("ni,nj,nk", [(23, 11, 43), (70, 9, 80), (130, 37, 2), (256, 256, 80)])
+@pytest.mark.parametrize("fused", [1, 0, 2])
+@pytest.mark.parametrize("ni,nj,nk", [(23, 11, 43), (70, 9, 80), (130, 37, 2), (256, 256, 80), (64, 5, 17), (1, 3, 5),
+                                      (127, 4, 61)])
 def test_compose_fp64_bit_exact(gpu, oracle, monkeypatch, fused, ni, nj, nk):
     monkeypatch.setenv("GT355_TUNE_hv_fused", str(fused))
     print(f"[case] compose fp64 fused={fused} {ni}x{nj}x{nk}", flush=True)
@@ -74,8 +75,10 @@ def test_compose_fp64_bit_exact(gpu, oracle, monkeypatch, fused, ni, nj, nk):
     assert np.array_equal(got_t, ref_t)
 
 
+@pytest.mark.parametrize("fused", [0, 1, 2])
 @pytest.mark.parametrize("ni,nj,nk", [(23, 11, 43), (96, 20, 17)])
-def test_compose_fp32_native_bit_exact(gpu, oracle, ni, nj, nk):
+def test_compose_fp32_native_bit_exact(gpu, oracle, monkeypatch, fused, ni, nj, nk):
+    monkeypatch.setenv("GT355_TUNE_hv_fused", str(fused))
     f, grid = make_case(gpu, ni, nj, nk, np.float32)
     host = {k: v.padded_numpy() for k, v in f.items()}
     dtr = float(np.float32(0.15))
@@ -94,16 +97,18 @@ def test_compose_precision_form(gpu, oracle, monkeypatch, ni, nj, nk):
     dtr = float(np.float32(0.15))
     # the FP64 oracle on the same FP32-rounded inputs
     ref_u, ref_t = oracle_chain(oracle, host64, ni, nj, nk, dtr, "native", "native")
-    monkeypatch.setenv("GT355_TUNE_hv_fused", "1")
-    got_u, got_t = run_gpu(f, grid, 0.15, 1)
-    # bit-exact: FP64 chain, outputs rounded once
-    assert np.array_equal(got_u, ref_u.astype(np.float32))
-    assert np.array_equal(got_t, ref_t.astype(np.float32))
-    for name, got, ref in (("u_stage", got_u, ref_u), ("utens_stage", got_t, ref_t)):
-        e = rel_err(got, ref)
-        rms = np.sqrt(np.mean((got.astype(np.float64) - ref) ** 2)) / np.sqrt(np.mean(ref ** 2))
-        print(f"[precision] fused {name}: max rel {e:.3e}, rms rel {rms:.3e}", flush=True)
-        assert e <= 1e-5
+    for fused in ("1", "2"):
+        # both fused kernels (LDS tile, DPP register sweep): the FP64 chain, outputs rounded once, bit for bit
+        f1, _ = make_case(gpu, ni, nj, nk, np.float32)
+        monkeypatch.setenv("GT355_TUNE_hv_fused", fused)
+        got_u, got_t = run_gpu(f1, grid, 0.15, 1)
+        assert np.array_equal(got_u, ref_u.astype(np.float32)), fused
+        assert np.array_equal(got_t, ref_t.astype(np.float32)), fused
+        for name, got, ref in (("u_stage", got_u, ref_u), ("utens_stage", got_t, ref_t)):
+            e = rel_err(got, ref)
+            rms = np.sqrt(np.mean((got.astype(np.float64) - ref) ** 2)) / np.sqrt(np.mean(ref ** 2))
+            print(f"[precision] fused={fused} {name}: max rel {e:.3e}, rms rel {rms:.3e}", flush=True)
+            assert e <= 1e-5
     # two-launch sequence: u_stage is rounded to FP32 storage before the ill-conditioned solve
     f2, _ = make_case(gpu, ni, nj, nk, np.float32)
     monkeypatch.setenv("GT355_TUNE_hv_fused", "0")
@@ -113,3 +118,17 @@ def test_compose_precision_form(gpu, oracle, monkeypatch, ni, nj, nk):
     rms2 = np.sqrt(np.mean((got_t2.astype(np.float64) - ref_t) ** 2)) / np.sqrt(np.mean(ref_t ** 2))
     print(f"[precision] two-launch utens_stage: max rel {e2:.3e}, rms rel {rms2:.3e}", flush=True)
     assert e2 <= 1e-5
+
+
+@pytest.mark.parametrize("fused", [2, 1, 0])
+def test_compose_deterministic_with_partial_strips(gpu, monkeypatch, fused):
+    """utens_stage is read and rewritten in place, so each column must be owned by exactly one wavefront: repeated
+    runs on identical inputs, with domains whose last i-strip is partial, give identical bits."""
+    monkeypatch.setenv("GT355_TUNE_hv_fused", str(fused))
+    outs = []
+    for _ in range(3):
+        f, grid = make_case(gpu, 130, 40, 61, np.float64)
+        outs.append(run_gpu(f, grid, 0.15, 0))
+    for u, t in outs[1:]:
+        assert np.array_equal(u, outs[0][0])
+        assert np.array_equal(t, outs[0][1])

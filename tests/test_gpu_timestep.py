@@ -42,7 +42,7 @@ def oracle_steps(oracle, ts, nsteps, fused):
     return f
 
 
-@pytest.mark.parametrize("fused", [0, 1])
+@pytest.mark.parametrize("fused", [0, 1, 2])
 @pytest.mark.parametrize("dtype,precision", [(np.float64, 0), (np.float32, 1)])
 def test_steps_match_oracle(gpu, oracle, monkeypatch, dtype, precision, fused):
     from gridtools_amd.timestep import HdiffVadvTimestepper
