@@ -187,7 +187,9 @@ int gt355_event_elapsed_ms(void *start, void *stop, float *ms);
  * No counterpart in the reference, whose tile shape is a backend template parameter fixed at compile time
  * (stencil/gpu/entry_point.hpp:147-150, gpu<64,8,1> by default).  gt355_tune_hdiff times every instantiated hdiff
  * configuration (LDS tiles TI x TJ x KC and register sweeps JB/P) on the given fields, `reps` launches each on
- * `stream`, records the fastest for (storage type, compute type, ni, nj, nk) so later gt355_hdiff calls with that
+ * `stream` back to back, cycling through temporary copies of the three fields so that the data exceeds the MALL
+ * (as a caller streaming several field sets sees it), records the fastest for (storage type, compute type, ni, nj,
+ * nk) so later gt355_hdiff calls with that
  * shape use it, and returns it in best[4] = {impl, a, b, c} with its mean time.  The fields' contents are
  * overwritten (out) and read (in, coeff) as by gt355_hdiff.  The table persists across processes through
  * gt355_tune_save / gt355_tune_load (text, one entry per line). */

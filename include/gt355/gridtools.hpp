@@ -349,6 +349,72 @@ namespace gt355 {
                 "run(hdiff_vadv)");
         }
 
+        // The batching form (gt355_batch_*): independent domains of uneven sizes, each computing lap =
+        // laplacian(in), u_stage = hdiff(in, coeff) and the vertical advection of u_stage into utens_stage, in one
+        // persistent launch per run().  The batch keeps the member stores alive; run() brings every member's
+        // device copy up to date first (the lazy host/target synchronisation of data_store).
+        template <class T>
+        struct batch_member {
+            std::shared_ptr<storage::data_store<T>> in, coeff, lap, u_stage, utens_stage, wcon, u_pos, utens;
+            grid g;
+        };
+
+        template <class Be, class T>
+        class batch {
+            gt355_batch *m_batch = nullptr;
+            std::vector<batch_member<T>> m_members;
+            void *m_stream;
+
+          public:
+            batch(Be be, std::vector<batch_member<T>> members, int wg_per_cu = 2)
+                : m_members(std::move(members)), m_stream(be.stream) {
+                std::vector<gt355_batch_domain> d(m_members.size());
+                for (size_t i = 0; i < m_members.size(); ++i) {
+                    auto &m = m_members[i];
+                    auto o = m.g.origin();
+                    d[i].in = m.in->sid(o, false);
+                    d[i].coeff = m.coeff->sid(o, false);
+                    d[i].lap = m.lap->sid(o, true);
+                    d[i].u_stage = m.u_stage->sid(o, true);
+                    d[i].utens_stage = m.utens_stage->sid(o, true);
+                    d[i].wcon = m.wcon->sid(o, false);
+                    d[i].u_pos = m.u_pos->sid(o, false);
+                    d[i].utens = m.utens->sid(o, false);
+                    d[i].ni = m.g.i_size;
+                    d[i].nj = m.g.j_size;
+                    d[i].nk = m.g.k_size;
+                }
+                check(gt355_batch_create(Be::precision, d.data(), (int)d.size(), wg_per_cu, &m_batch),
+                    "batch: create");
+            }
+            batch(batch const &) = delete;
+            batch &operator=(batch const &) = delete;
+            ~batch() {
+                if (m_batch)
+                    gt355_batch_destroy(m_batch);
+            }
+            void run(global_parameter_t<T> dtr_stage) {
+                for (auto &m : m_members) {
+                    m.in->get_const_target_ptr();
+                    m.coeff->get_const_target_ptr();
+                    m.wcon->get_const_target_ptr();
+                    m.u_pos->get_const_target_ptr();
+                    m.utens->get_const_target_ptr();
+                    m.lap->get_target_ptr();
+                    m.u_stage->get_target_ptr();
+                    m.utens_stage->get_target_ptr();
+                }
+                check(gt355_batch_run(m_batch, (double)dtr_stage.value, m_stream), "batch: run");
+            }
+            // the autotuner's choice of persistent grid size: (workgroups-per-CU setting, mean microseconds)
+            std::pair<int, double> tune(int reps = 5) {
+                int w = 0;
+                double us = 0;
+                check(gt355_batch_tune(m_batch, reps, m_stream, &w, &us), "batch: tune");
+                return {w, us};
+            }
+        };
+
         namespace expandable_impl {
             template <class T>
             T const &member(T const &v, size_t) {

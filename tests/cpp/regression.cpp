@@ -209,6 +209,42 @@ namespace {
         report("expandable_run horizontal_diffusion x3", ok);
     }
 
+    // the batching form: uneven domains in one launch equal laplacian + the composition run per domain (FP64,
+    // where the two-launch composition and the batch compute the same bits)
+    void batching() {
+        const int h = 3;
+        const int sizes[3][3] = {{70, 9, 81}, {23, 11, 43}, {130, 5, 2}};
+        std::vector<batch_member<double>> members;
+        std::vector<std::array<std::shared_ptr<storage::data_store<double>>, 4>> refs;
+        for (int m = 0; m < 3; ++m) {
+            const int ni = sizes[m][0], nj = sizes[m][1], nk = sizes[m][2];
+            const uint64_t seed = 300 + m;
+            auto g = make_grid(halo_descriptor(h, h, h, ni + h - 1, ni + 2 * h), halo_descriptor(h, h, h, nj + h - 1, nj + 2 * h), nk);
+            auto zero = [&] {
+                return storage::builder<storage::gpu>.type<double>().dimensions(ni + 6, nj + 6, nk).halos(h, h, 0).build();
+            };
+            batch_member<double> b{seeded<double>(ni, nj, nk, h, seed, 1, -1, 1), seeded<double>(ni, nj, nk, h, seed, 2, 0, 0.25),
+                zero(), zero(), seeded<double>(ni, nj, nk, h, seed, 6, -1, 1), seeded<double>(ni, nj, nk, h, seed, 7, -0.5, 0.5, 1),
+                seeded<double>(ni, nj, nk, h, seed, 4, -1, 1), seeded<double>(ni, nj, nk, h, seed, 5, -1, 1), g};
+            // the reference computation on separate outputs: laplacian, then hdiff -> vadv
+            auto lap = zero(), ust = zero(), uts = seeded<double>(ni, nj, nk, h, seed, 6, -1, 1);
+            run(specs::laplacian(), gfx950<>(), g, lap, b.in);
+            run(specs::hdiff_vadv(), gfx950<>(), g, b.in, b.coeff, ust, uts, b.wcon, b.u_pos, b.utens, global_parameter(0.15));
+            refs.push_back({lap, ust, uts, nullptr});
+            members.push_back(b);
+        }
+        batch<gfx950<>, double> bt(gfx950<>(), members);
+        bt.run(global_parameter(0.15));
+        bool ok = true;
+        for (int m = 0; m < 3; ++m) {
+            const auto &g = members[m].g;
+            ok &= same_interior(members[m].lap, mirror(refs[m][0]), g);
+            ok &= same_interior(members[m].u_stage, mirror(refs[m][1]), g);
+            ok &= same_interior(members[m].utens_stage, mirror(refs[m][2]), g);
+        }
+        report("batch of 3 uneven domains == per-domain runs", ok);
+    }
+
     // boundaries: zero / value / copy on the 26 regions against the oracle's per-region loop
     void boundary_conditions() {
         const int n = 20, h = 3, nk = 6;
@@ -286,6 +322,7 @@ int main(int argc, char **argv) {
             copy_known_answer();
             hdiff_vadv_precision_form(40, 30, 50);
             expandable();
+            batching();
             boundary_conditions();
         }
     } catch (std::exception const &e) {

@@ -1,6 +1,7 @@
 """The C++ front end (include/gt355/gridtools.hpp): its regression program, written like the reference's
 tests/regression/*.cpp, built by the package build.  Host-only cases run everywhere; the device cases (hdiff,
-vadv, laplacian, copy, the precision-form composition, expandable_run, boundaries and halo fill, each bit-exact
+vadv, laplacian, copy, the precision-form composition, expandable_run, the batching form, boundaries and halo fill,
+each bit-exact
 against the oracle or the reference's known answers) run on the GPU."""
 import os
 import subprocess
@@ -28,4 +29,4 @@ def test_cpp_frontend_host_cases():
 @pytest.mark.gpu
 def test_cpp_frontend_regression_on_device():
     out = run(timeout=240)
-    assert out.count("[ok]") >= 18
+    assert out.count("[ok]") >= 19
