@@ -109,7 +109,7 @@ def autotune(gt, dtype, precision, shapes):
         c_ = hb.uniform(SEED, 2, 0.0, 0.25).build()
         o_ = hb.value(0).build()
         g = gt.make_grid(gt.halo_descriptor.symmetric(2, n + 4), gt.halo_descriptor.symmetric(2, n + 4), nk)
-        cfg, us = tune.hdiff(o_, i_, c_, g, precision=precision, reps=30)
+        cfg, us = tune.hdiff(o_, i_, c_, g, precision=precision, reps=60)
         out[f"{n}x{n}x{nk}"] = {"config": cfg, "us": round(us, 2)}
     return out
 

@@ -210,9 +210,10 @@ namespace gt355 {
         __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
         hv_dpp_kernel(strip_fields<ST> f, CT dtr, int ni, int nj, int nk, int strips_i, int nxcd) {
             __shared__ ST s_up[NK][64];
-            // consecutive ids are consecutive rows of one i-strip: five neighbouring waves share each row of `in`
+            // strips fastest: the waves in flight cover whole rows (contiguous DRAM pages); the five waves that read
+            // a row of `in` are strips_i ids apart, on one XCD
             const int id = xcd_remap(blockIdx.x, gridDim.x, nxcd);
-            const int si = id / nj, j = id % nj;
+            const int si = id % strips_i, j = id / strips_i;
             hv_strip<ST, CT, NK, D, NARROW, false, false>(f, dtr, ni, nj, nk, si * 64, j, s_up, nullptr);
         }
 

@@ -155,8 +155,10 @@ namespace gt355 {
             const int total = strips_i * strips_j * nk;
             if (wid >= total)
                 return; // a whole wavefront leaves; there is no barrier in this kernel
-            const int sj_ = wid % strips_j;
-            const int si_ = (wid / strips_j) % strips_i;
+            // strips fastest: the waves in flight cover whole rows of a level (contiguous DRAM pages); the four
+            // waves of a workgroup take four neighbouring strips of one row block
+            const int si_ = wid % strips_i;
+            const int sj_ = (wid / strips_i) % strips_j;
             const int k = wid / (strips_j * strips_i);
             const int i0 = si_ * 64, j0 = sj_ * JB;
             const bool left = lane < 32;
