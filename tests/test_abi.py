@@ -134,3 +134,23 @@ def test_batch_create_validates_domains_without_a_device():
     assert "outside bounds" in _abi.last_error()
     assert lib.gt355_batch_create(_abi.WIDEN, arr, 1, 0, ctypes.byref(h)) == -1
     assert not h.value
+
+
+def test_batch_entry_points_reject_null_handles_without_a_device():
+    lib = _abi.lib()
+    n, g = ctypes.c_int(), ctypes.c_int()
+    assert lib.gt355_batch_run(None, ctypes.c_double(0.15), None) == -1
+    assert lib.gt355_batch_info(None, ctypes.byref(n), ctypes.byref(g)) == -1
+    w, us = ctypes.c_int(), ctypes.c_double()
+    assert lib.gt355_batch_tune(None, 3, None, ctypes.byref(w), ctypes.byref(us)) == -1
+    assert lib.gt355_batch_destroy(None) == 0
+    assert "null batch" in _abi.last_error()
+
+
+def test_tune_hdiff_rejects_bad_arguments_without_a_device():
+    lib = _abi.lib()
+    f = _abi.Field()
+    best = (ctypes.c_int32 * 4)()
+    us = ctypes.c_double()
+    assert lib.gt355_tune_hdiff(0, None, ctypes.byref(f), ctypes.byref(f), 8, 8, 8, 5, None, best,
+                                ctypes.byref(us)) == -1

@@ -20,7 +20,7 @@ def fields(ni, nj, nk, h, dist, seed=0, dtype=np.float64):
     rng = np.random.default_rng(seed)
     shp = (nk, nj + 2 * h, ni + 2 * h)
     wshp = (nk + 1, nj + 2 * h, ni + 2 * h)
-    if dist == "baseline":
+    if dist == "workload":
         us, up, ut, uts = (rng.uniform(-1, 1, shp) for _ in range(4))
         wc = rng.uniform(-0.5, 0.5, wshp)
     else:  # reference-like scales (vertical_advection_repository.hpp:153-174): well conditioned
@@ -168,9 +168,9 @@ def functor_order_solution(uts, us, wc, up, ut, dtr, h, ni, nj, nk):
 
 
 @pytest.mark.parametrize("ni,nj,nk", [(12, 33, 61), (64, 16, 80)])
-def test_vadv_operation_order_at_baseline_distributions(oracle, ni, nj, nk):
+def test_vadv_operation_order_at_workload_distributions(oracle, ni, nj, nk):
     h = 3
-    uts, us, wc, up, ut = fields(ni, nj, nk, h, "baseline", seed=nk)
+    uts, us, wc, up, ut = fields(ni, nj, nk, h, "workload", seed=nk)
     got = run(oracle, uts, us, wc, up, ut, 0.15, h, ni, nj, nk)[:, h:h + nj, h:h + ni]
     ref = functor_order_solution(uts, us, wc, up, ut, 0.15, h, ni, nj, nk)
     assert np.array_equal(got, ref)
@@ -182,7 +182,7 @@ def test_vadv_operation_order_at_baseline_distributions(oracle, ni, nj, nk):
 
 def test_vadv_matches_dense_lu(oracle):
     ni, nj, nk, h = 7, 5, 24, 3
-    uts, us, wc, up, ut = fields(ni, nj, nk, h, "baseline", seed=11)
+    uts, us, wc, up, ut = fields(ni, nj, nk, h, "workload", seed=11)
     wc *= 0.05  # keep every column well conditioned (cond < 1e3) so LU and Thomas agree to ~1e-12
     dtr = 0.15
     got = run(oracle, uts, us, wc, up, ut, dtr, h, ni, nj, nk)
@@ -196,7 +196,7 @@ def test_vadv_matches_dense_lu(oracle):
 
 def test_vadv_zero_wind_closed_form(oracle):
     ni, nj, nk, h = 9, 4, 10, 3
-    uts, us, wc, up, ut = fields(ni, nj, nk, h, "baseline", seed=5)
+    uts, us, wc, up, ut = fields(ni, nj, nk, h, "workload", seed=5)
     wc[:] = 0.0
     dtr = 0.15
     got = run(oracle, uts, us, wc, up, ut, dtr, h, ni, nj, nk)[:, h:-h, h:-h]
@@ -226,7 +226,7 @@ def test_tridiagonal_known_answer(oracle):
 
 def test_vadv_mixed_is_fp64_on_widened_inputs(oracle):
     ni, nj, nk, h = 40, 6, 30, 3
-    f32 = fields(ni, nj, nk, h, "baseline", seed=9, dtype=np.float32)
+    f32 = fields(ni, nj, nk, h, "workload", seed=9, dtype=np.float32)
     dtr32 = np.float32(0.15)
     mixed = run(oracle, *f32, dtr32, h, ni, nj, nk, mode="mixed")
     wide = run(oracle, *[a.astype(np.float64) for a in f32], float(dtr32), h, ni, nj, nk)
@@ -234,14 +234,14 @@ def test_vadv_mixed_is_fp64_on_widened_inputs(oracle):
 
 
 def test_vadv_rejects_single_level(oracle):
-    uts, us, wc, up, ut = fields(4, 4, 2, 3, "baseline")
+    uts, us, wc, up, ut = fields(4, 4, 2, 3, "workload")
     with pytest.raises(ValueError):
         run(oracle, uts, us, wc, up, ut, 0.15, 3, 4, 4, 1)
 
 
 def test_vadv_threads_do_not_change_bits(oracle):
     ni, nj, nk, h = 50, 21, 17, 3
-    f = fields(ni, nj, nk, h, "baseline", seed=3)
+    f = fields(ni, nj, nk, h, "workload", seed=3)
     a = run(oracle, *f, 0.15, h, ni, nj, nk, threads=1)
     b = run(oracle, *f, 0.15, h, ni, nj, nk, threads=5)
     assert np.array_equal(a, b)
