@@ -103,6 +103,8 @@ SIGNATURES = {
     "gt355_event_synchronize": [_P],
     "gt355_event_elapsed_ms": [_P, _P, ctypes.POINTER(ctypes.c_float)],
     "gt355_mall_flush": [_P],
+    "gt355_range_push": [ctypes.c_char_p],
+    "gt355_range_pop": [],
     "gt355_tune_hdiff": [_I, _FP, _FP, _FP, _I, _I, _I, _I, _P, ctypes.POINTER(ctypes.c_int32),
                          ctypes.POINTER(ctypes.c_double)],
     "gt355_tune_save": [ctypes.c_char_p],

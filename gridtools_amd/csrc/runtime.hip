@@ -2,6 +2,7 @@
 // MALL flush.  Replaces the reference's CUDA runtime glue (common/cuda_util.hpp:20-95,
 // common/cuda_runtime.hpp) and the gpu storage traits' allocation and mirroring (storage/gpu.hpp:69-105).
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <cstdarg>
 #include <cstring>
@@ -326,6 +327,17 @@ int gt355_graph_launch(void *graph_exec, void *stream) {
 int gt355_graph_destroy(void *graph_exec) {
     if (graph_exec)
         GT355_HIP(hipGraphExecDestroy((hipGraphExec_t)graph_exec));
+    return GT355_OK;
+}
+
+int gt355_range_push(const char *name) {
+    GT355_CHECK(name != nullptr, "gt355_range_push: null name");
+    roctxRangePushA(name);
+    return GT355_OK;
+}
+
+int gt355_range_pop(void) {
+    roctxRangePop();
     return GT355_OK;
 }
 

@@ -52,3 +52,18 @@ def synchronize():
 
 def mall_flush(stream=None):
     _abi.call("gt355_mall_flush", stream.handle if isinstance(stream, Stream) else stream)
+
+
+class Range:
+    """with Range("name"): ... -- a named range for tracers (gt355_range_push / gt355_range_pop)."""
+
+    def __init__(self, name):
+        self.name = name.encode()
+
+    def __enter__(self):
+        _abi.call("gt355_range_push", self.name)
+        return self
+
+    def __exit__(self, *exc):
+        _abi.call("gt355_range_pop")
+        return False

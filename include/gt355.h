@@ -239,6 +239,11 @@ int gt355_batch_tune(gt355_batch *batch, int reps, void *stream, int *wg_per_cu,
 int gt355_batch_destroy(gt355_batch *batch);
 
 /* ------------------------------------------------------------------ measurement */
+/* Named host ranges for tracers (rocprofv3 --marker-trace shows them around the launches they enclose); the role
+ * of the reference's timer hooks (common/timer/timer_cuda.hpp:21-54), which measure named regions with events. */
+int gt355_range_push(const char *name);
+int gt355_range_pop(void);
+
 /* Writes (and reads back) a device buffer larger than the MI355X MALL so that the next kernel reads
  * HBM, not Infinity Cache.  Used between timed repetitions, outside the timed region. */
 int gt355_mall_flush(void *stream);

@@ -154,3 +154,12 @@ def test_tune_hdiff_rejects_bad_arguments_without_a_device():
     us = ctypes.c_double()
     assert lib.gt355_tune_hdiff(0, None, ctypes.byref(f), ctypes.byref(f), 8, 8, 8, 5, None, best,
                                 ctypes.byref(us)) == -1
+
+
+def test_range_markers_need_no_device():
+    from gridtools_amd.timing import Range
+
+    with Range("outer"):
+        with Range("inner"):
+            pass
+    assert _abi.lib().gt355_range_push(None) == -1
