@@ -95,6 +95,7 @@ SIGNATURES = {
     "gt355_halo_buffer_size": [_I, _I, c_halo, ctypes.c_uint32, ctypes.POINTER(ctypes.c_size_t)],
     "gt355_halo_pack": [_I, ctypes.POINTER(_P), _I, c_halo, c_i64_3, ctypes.c_uint32, _P, _P],
     "gt355_halo_unpack": [_I, ctypes.POINTER(_P), _I, c_halo, c_i64_3, ctypes.c_uint32, _P, _P],
+    "gt355_halo_exchange_local": [_I, ctypes.POINTER(_P), _I, c_halo, c_i64_3, ctypes.c_uint32, _P, _P],
     "gt355_stream_create": [ctypes.POINTER(_P)],
     "gt355_stream_destroy": [_P],
     "gt355_event_create": [ctypes.POINTER(_P)],

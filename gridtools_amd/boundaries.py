@@ -107,3 +107,11 @@ def halo_unpack(fields, halos, buffer_ptr, dir_mask=ALL_DIRECTIONS, stream=None)
     dt, st = _common(fields)
     _abi.call("gt355_halo_unpack", dt, _ptrs(fields), len(fields), _halo_arg(halos), _abi.c_i64_3(*st),
               ctypes.c_uint32(dir_mask), ctypes.c_void_p(buffer_ptr), stream)
+
+
+def halo_exchange_local(fields, halos, buffer_ptr, dir_mask=ALL_DIRECTIONS, stream=None):
+    """The single-rank periodic exchange: pack the slabs of dir_mask's directions into the buffer
+    (halo_buffer_size bytes), then unpack every halo region d from the slab packed for -d."""
+    dt, st = _common(fields)
+    _abi.call("gt355_halo_exchange_local", dt, _ptrs(fields), len(fields), _halo_arg(halos), _abi.c_i64_3(*st),
+              ctypes.c_uint32(dir_mask), ctypes.c_void_p(buffer_ptr), stream)

@@ -23,6 +23,8 @@ namespace gt355 {
             int32_t n[3];    // extent
             int32_t dir[3];  // direction components (-1, 0, 1)
             int64_t start;   // prefix offset in points
+            int64_t src;     // unpack: start of the buffer segment this region reads (start, or the segment of the
+                             // opposite direction for a local periodic exchange)
         };
 
         struct region_table {
@@ -109,11 +111,10 @@ namespace gt355 {
                 int64_t p = i * a.s[0] + j * a.s[1] + k * a.s[2];
                 for (int f = 0; f < a.nfields; ++f) {
                     T *fld = (T *)a.dst[f];
-                    int64_t b = r.start * a.nfields + f * vol + loc;
                     if (UNPACK)
-                        fld[p] = buf[b];
+                        fld[p] = buf[r.src * a.nfields + f * vol + loc];
                     else
-                        buf[b] = fld[p];
+                        buf[r.start * a.nfields + f * vol + loc] = fld[p];
                 }
             }
         }
@@ -176,6 +177,7 @@ namespace gt355 {
                         if (vol == 0)
                             continue;
                         r.start = start;
+                        r.src = start;
                         start += vol;
                         t.r[t.count++] = r;
                     }
@@ -336,4 +338,46 @@ extern "C" int gt355_halo_pack(int dtype, void *const *fields, int nfields, cons
 extern "C" int gt355_halo_unpack(int dtype, void *const *fields, int nfields, const int32_t halo[3][5],
     const int64_t stride[3], uint32_t dir_mask, const void *buffer, void *stream) {
     return halo_pack_impl(true, dtype, fields, nfields, halo, stride, dir_mask, const_cast<void *>(buffer), stream);
+}
+
+extern "C" int gt355_halo_exchange_local(int dtype, void *const *fields, int nfields, const int32_t halo[3][5],
+    const int64_t stride[3], uint32_t dir_mask, void *buffer, void *stream) {
+    const char *who = "gt355_halo_exchange_local";
+    GT355_CHECK(halo && buffer, "%s: null argument", who);
+    int rc;
+    if ((rc = check_halo(halo, who)))
+        return rc;
+    bc_args a;
+    if ((rc = fill_args(a, dtype, fields, nullptr, nfields, halo, stride, who)))
+        return rc;
+    // the halo of side d receives what the neighbour at d packs for its side -d; on one rank with a periodic
+    // process grid that neighbour is this rank, so region d of the unpack reads the segment packed for -d
+    const region_table tp = make_table(halo, dir_mask, 1, nullptr);
+    region_table tu = make_table(halo, dir_mask, 0, nullptr);
+    for (int q = 0; q < tu.count; ++q) {
+        region &r = tu.r[q];
+        int found = -1;
+        for (int p = 0; p < tp.count; ++p)
+            if (tp.r[p].dir[0] == -r.dir[0] && tp.r[p].dir[1] == -r.dir[1] && tp.r[p].dir[2] == -r.dir[2])
+                found = p;
+        GT355_CHECK(found >= 0, "%s: direction (%d,%d,%d) has no opposite in the mask", who, r.dir[0], r.dir[1],
+            r.dir[2]);
+        const region &o = tp.r[found];
+        GT355_CHECK((int64_t)o.n[0] * o.n[1] * o.n[2] == (int64_t)r.n[0] * r.n[1] * r.n[2],
+            "%s: halo of direction (%d,%d,%d) and the slab packed for the opposite side differ in size (asymmetric "
+            "halo)",
+            who, r.dir[0], r.dir[1], r.dir[2]);
+        r.src = o.start;
+    }
+    if (tp.total == 0)
+        return GT355_OK;
+    hipStream_t s = (hipStream_t)stream;
+    if (dtype == GT355_F64) {
+        pack_kernel<double, false><<<blocks_for(tp.total), 256, 0, s>>>(tp, a, (double *)buffer);
+        pack_kernel<double, true><<<blocks_for(tu.total), 256, 0, s>>>(tu, a, (double *)buffer);
+    } else {
+        pack_kernel<float, false><<<blocks_for(tp.total), 256, 0, s>>>(tp, a, (float *)buffer);
+        pack_kernel<float, true><<<blocks_for(tu.total), 256, 0, s>>>(tu, a, (float *)buffer);
+    }
+    return check_launch(who);
 }

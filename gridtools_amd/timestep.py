@@ -4,8 +4,9 @@ One timestep of the horizontal-diffusion / vertical-advection operator:
 
 1. ``u_stage = hdiff(u, coeff)`` and the vertical advection of ``u_stage`` into ``utens_stage``, as one
    composition launch (gt355_hdiff_vadv);
-2. boundary conditions on the halo of ``u_stage`` and ``utens_stage``: periodic in i, zero-gradient in j, as one
-   local halo fill (gt355_halo_fill, the single-rank form of the reference's pack / periodic exchange / unpack);
+2. boundary conditions on the halo of ``u_stage`` and ``utens_stage``: periodic in i, zero-gradient in j, either
+   as one local halo fill (gt355_halo_fill) or through the halo-pack kernels (zero-gradient j, then the single-rank
+   pack / periodic exchange / unpack of the i faces, gt355_halo_exchange_local);
 3. ``u`` and ``u_stage`` swap roles, so the diffused field is the next step's input.
 
 Sequences can be captured once into a hipGraph and replayed (both swap parities are recorded, so an even number of
@@ -56,9 +57,17 @@ def capture(stream, fn):
 
 class HdiffVadvTimestepper:
     """The operator above on an ni x nj x nk interior with a ``halo``-point ij halo (>= 2), inputs from the seeded
-    generator: u, utens_stage, u_pos, utens ~ U(-1, 1), coeff ~ U(0, 0.25), wcon ~ U(-0.5, 0.5) on nk+1 levels."""
+    generator: u, utens_stage, u_pos, utens ~ U(-1, 1), coeff ~ U(0, 0.25), wcon ~ U(-0.5, 0.5) on nk+1 levels.
 
-    def __init__(self, ni, nj, nk, dtype=np.float32, precision=stencils.WIDEN, halo=3, seed=42, dtr=0.15):
+    bc="fill": one halo-fill launch per step.  bc="exchange": the halo-pack route -- zero-gradient j by halo fill,
+    then the periodic i faces by the single-rank pack / exchange / unpack (gt355_halo_exchange_local) over all rows
+    (j halo rows included, so the corners get the wrapped, clamped values); both give the same bits."""
+
+    def __init__(self, ni, nj, nk, dtype=np.float32, precision=stencils.WIDEN, halo=3, seed=42, dtr=0.15,
+                 bc="fill"):
+        if bc not in ("fill", "exchange"):
+            raise ValueError(f"bc must be 'fill' or 'exchange', not {bc!r}")
+        self.bc = bc
         if halo < 2:
             raise ValueError("hdiff needs a halo of at least 2")
         self.ni, self.nj, self.nk, self.halo = ni, nj, nk, halo
@@ -79,13 +88,28 @@ class HdiffVadvTimestepper:
         # the initial state satisfies the boundary conditions
         boundaries.halo_fill([self.u], self.halos, (boundaries.PERIODIC, boundaries.ZERO_GRADIENT, boundaries.NONE))
         self.steps_done = 0
+        self._xbuf = None
+        if bc == "exchange":
+            # the i faces over every row: j treated as halo-free for the exchange
+            self._xhalos = [hi, halo_descriptor.of_size(nj + 2 * halo), self.halos[2]]
+            self._xmask = boundaries.direction_mask(lambda di, dj, dk: di != 0 and dj == 0 and dk == 0)
+            nbytes = boundaries.halo_buffer_size(self.u.code, 2, self._xhalos, self._xmask)
+            buf = ctypes.c_void_p()
+            _abi.call("gt355_malloc", ctypes.c_size_t(nbytes), ctypes.byref(buf))
+            self._xbuf = buf
 
     def step(self, stream=None):
         s = stream.handle if stream is not None else None
         stencils.hdiff_vadv(self.u, self.coeff, self.u_stage, self.utens_stage, self.wcon, self.u_pos, self.utens,
                             self.dtr, self.grid, precision=self.precision, stream=s)
-        boundaries.halo_fill([self.u_stage, self.utens_stage], self.halos,
-                             (boundaries.PERIODIC, boundaries.ZERO_GRADIENT, boundaries.NONE), stream=s)
+        if self.bc == "fill":
+            boundaries.halo_fill([self.u_stage, self.utens_stage], self.halos,
+                                 (boundaries.PERIODIC, boundaries.ZERO_GRADIENT, boundaries.NONE), stream=s)
+        else:
+            boundaries.halo_fill([self.u_stage, self.utens_stage], self.halos,
+                                 (boundaries.NONE, boundaries.ZERO_GRADIENT, boundaries.NONE), stream=s)
+            boundaries.halo_exchange_local([self.u_stage, self.utens_stage], self._xhalos, self._xbuf.value,
+                                           self._xmask, stream=s)
         self.u, self.u_stage = self.u_stage, self.u
         self.steps_done += 1
 
@@ -105,3 +129,10 @@ class HdiffVadvTimestepper:
 
     def points(self):
         return self.ni * self.nj * self.nk
+
+    def __del__(self):
+        try:
+            if self._xbuf is not None and self._xbuf.value:
+                _abi.lib().gt355_free(self._xbuf)
+        except Exception:
+            pass

@@ -171,6 +171,12 @@ int gt355_halo_pack(int dtype, void *const *fields, int nfields, const int32_t h
     uint32_t dir_mask, void *buffer, void *stream);
 int gt355_halo_unpack(int dtype, void *const *fields, int nfields, const int32_t halo[3][5],
     const int64_t stride[3], uint32_t dir_mask, const void *buffer, void *stream);
+/* The single-rank halo exchange with a periodic process grid (gcl/low_level/proc_grids_3D.hpp:83-88): pack the
+ * slabs of the directions in dir_mask into `buffer` (gt355_halo_buffer_size bytes), then unpack every halo region d
+ * from the segment packed for -d -- the neighbour at d is this rank.  The mask must be closed under negation and
+ * the halos symmetric in the dimensions it exchanges. */
+int gt355_halo_exchange_local(int dtype, void *const *fields, int nfields, const int32_t halo[3][5],
+    const int64_t stride[3], uint32_t dir_mask, void *buffer, void *stream);
 
 /* ------------------------------------------------------------------ streams and timing
  * Replaces timer_cuda (common/timer/timer_cuda.hpp:21-54): hipEvent pairs on a stream.  Streams are created

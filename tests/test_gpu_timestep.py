@@ -42,13 +42,16 @@ def oracle_steps(oracle, ts, nsteps, fused):
     return f
 
 
+@pytest.mark.parametrize("bc", ["fill", "exchange"])
 @pytest.mark.parametrize("fused", [0, 1, 2])
 @pytest.mark.parametrize("dtype,precision", [(np.float64, 0), (np.float32, 1)])
-def test_steps_match_oracle(gpu, oracle, monkeypatch, dtype, precision, fused):
+def test_steps_match_oracle(gpu, oracle, monkeypatch, dtype, precision, fused, bc):
+    """bc="exchange" applies the boundary conditions through the halo-pack kernels (pack / single-rank periodic
+    exchange / unpack) and must give the bits of the oracle's periodic-i / zero-gradient-j halo fill."""
     from gridtools_amd.timestep import HdiffVadvTimestepper
 
     monkeypatch.setenv("GT355_TUNE_hv_fused", str(fused))
-    ts = HdiffVadvTimestepper(70, 45, 24, dtype=dtype, precision=precision, seed=3)
+    ts = HdiffVadvTimestepper(70, 45, 24, dtype=dtype, precision=precision, seed=3, bc=bc)
     ref = oracle_steps(oracle, ts, 4, fused)
     for _ in range(4):
         ts.step()
