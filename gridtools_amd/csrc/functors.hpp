@@ -15,6 +15,21 @@ namespace gt355 {
         return (res * (in_p - in_c) > CT(0)) ? CT(0) : res;
     }
 
+    // wlap_function::apply (simple_hori_diff.cpp:33-39): C++ sums a + b - c + d + e left to right.
+    template <class CT>
+    __device__ __forceinline__ CT wlap_f(CT c, CT ip, CT im, CT jp, CT jm, CT cro, CT cru) {
+        return (((ip + im) - CT(2) * c) + cro * (jp - c)) + cru * (jm - c);
+    }
+    // divflux_function::apply (simple_hori_diff.cpp:51-60); l* are the laplacians at the centre and neighbours.
+    template <class CT>
+    __device__ __forceinline__ CT divflux_f(CT c, CT l, CT lip, CT lim, CT ljp, CT ljm, CT cro, CT co) {
+        CT fluxx = lip - l;
+        CT fluxx_m = l - lim;
+        CT fluxy = cro * (ljp - l);
+        CT fluxy_m = cro * (l - ljm);
+        return c + ((fluxx_m - fluxx) + (fluxy_m - fluxy)) * co;
+    }
+
     constexpr double BET_M = 0.5, BET_P = 0.5; // vertical_advection_defs.hpp:14-16
 
     template <class CT>

@@ -60,3 +60,43 @@ def vadv(utens_stage, u_stage, wcon, u_pos, utens, dtr_stage, grid: Grid, precis
     s = [_sid(f, grid) for f in (utens_stage, u_stage, wcon, u_pos, utens)]
     _abi.call("gt355_vadv", precision, *[ctypes.byref(x) for x in s], ctypes.c_double(dtr_stage), grid.i_size,
               grid.j_size, grid.k_size, stream)
+
+
+def simple_hori_diff(out, in_, crlato, crlatu, coeff, grid: Grid, precision=NATIVE, stream=None):
+    """simple_hori_diff (tests/regression/simple_hori_diff.cpp:25-85): wlap into an ij-cached lap, then divflux.
+    crlato / crlatu are usually j-only stores (builder().selector(0, 1, 0))."""
+    s = [_sid(f, grid) for f in (out, in_, crlato, crlatu, coeff)]
+    _abi.call("gt355_simple_hori_diff", precision, *[ctypes.byref(x) for x in s], grid.i_size, grid.j_size,
+              grid.k_size, stream)
+
+
+def _field_array(fields, grid):
+    arr = (_abi.Field * len(fields))()
+    for n, f in enumerate(fields):
+        arr[n] = _sid(f, grid)
+    return arr
+
+
+def prepare_tracers(data, data_nnow, rho, grid: Grid, precision=NATIVE, stream=None):
+    """expandable_run of prepare_tracers (tests/regression/advection_pdbott_prepare_tracers.cpp:23-57):
+    data[m] = rho * data_nnow[m] for every member of the two equally long lists, up to 32 members per launch."""
+    if len(data) != len(data_nnow):
+        raise ValueError(f"expandable_run: vector arguments differ in size ({len(data)} vs {len(data_nnow)})")
+    d, n, r = _field_array(data, grid), _field_array(data_nnow, grid), _sid(rho, grid)
+    _abi.call("gt355_prepare_tracers", precision, d, n, len(data), ctypes.byref(r), grid.i_size, grid.j_size,
+              grid.k_size, stream)
+
+
+def copy_fields(out, in_, grid: Grid, stream=None):
+    """copy_stencil_tuple (tests/regression/copy_stencil_tuple.cpp:23-100): out[f] = in[f] for equally long lists
+    of stores (or of gt355_field descriptors, e.g. the members of an array-of-structs buffer)."""
+    if len(out) != len(in_):
+        raise ValueError(f"copy_fields: {len(out)} outputs for {len(in_)} inputs")
+
+    def arr(fields):
+        a = (_abi.Field * len(fields))()
+        for n, f in enumerate(fields):
+            a[n] = f if isinstance(f, _abi.Field) else _sid(f, grid)
+        return a
+
+    _abi.call("gt355_copy_fields", arr(out), arr(in_), len(out), grid.i_size, grid.j_size, grid.k_size, stream)

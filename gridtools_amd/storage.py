@@ -25,19 +25,25 @@ def dtype_code(dtype) -> int:
 class DataStore:
     """A 3-D field in device memory (lengths include the halo)."""
 
-    def __init__(self, dtype, lengths, halos=(0, 0, 0), name=""):
+    def __init__(self, dtype, lengths, halos=(0, 0, 0), name="", selector=(1, 1, 1)):
         self.dtype = np.dtype(dtype)
         self.code = dtype_code(self.dtype)
         self.lengths = tuple(int(x) for x in lengths)
         self.halos = tuple(int(x) for x in halos)
         self.name = name
-        if len(self.lengths) != 3 or len(self.halos) != 3:
+        if len(self.lengths) != 3 or len(self.halos) != 3 or len(selector) != 3:
             raise ValueError("DataStore is three-dimensional")
+        # selector (storage::builder::selector, storage/builder.hpp:168-326): a masked dimension holds one value
+        # for every index -- it is allocated with length 1 and has stride 0 in the SID
+        self.selector = tuple(bool(x) for x in selector)
+        self.alloc_lengths = tuple(n if m else 1 for n, m in zip(self.lengths, self.selector))
+        alloc_halos = tuple(h if m else 0 for h, m in zip(self.halos, self.selector))
         lay = _abi.Layout()
-        _abi.call("gt355_storage_layout", self.dtype.itemsize, _abi.c_i64_3(*self.lengths),
-                  _abi.c_i64_3(*self.halos), ctypes.byref(lay))
+        _abi.call("gt355_storage_layout", self.dtype.itemsize, _abi.c_i64_3(*self.alloc_lengths),
+                  _abi.c_i64_3(*alloc_halos), ctypes.byref(lay))
         self.layout = lay
-        self.strides = tuple(int(x) for x in lay.stride)
+        self.alloc_strides = tuple(int(x) for x in lay.stride)
+        self.strides = tuple(st if m else 0 for st, m in zip(self.alloc_strides, self.selector))
         self.padded = tuple(int(x) for x in lay.padded)
         raw = ctypes.c_void_p()
         _abi.call("gt355_malloc", ctypes.c_size_t(lay.alloc_bytes), ctypes.byref(raw))
@@ -47,7 +53,11 @@ class DataStore:
         base = ctypes.c_void_p()
         _abi.call("gt355_storage_base", ctypes.byref(lay), raw, ctypes.byref(base))
         self.base = base.value  # address of storage index (0, 0, 0)
-        self.nelem = self.strides[2] * self.padded[2]  # elements from base covered by the layout
+        self.nelem = self.alloc_strides[2] * self.padded[2]  # elements from base covered by the layout
+
+    @property
+    def masked(self):
+        return not all(self.selector)
 
     # ------------------------------------------------------------------ lifetime
     def free(self):
@@ -74,48 +84,56 @@ class DataStore:
         f = _abi.Field()
         f.origin = self.address(*o)
         f.stride = (ctypes.c_int64 * 3)(*self.strides)
-        f.lower = (ctypes.c_int32 * 3)(*[-x for x in o])
-        f.upper = (ctypes.c_int32 * 3)(*[n - x for n, x in zip(self.lengths, o)])
+        # a masked dimension can be indexed anywhere (stride 0)
+        f.lower = (ctypes.c_int32 * 3)(*[-x if m else -(1 << 30) for x, m in zip(o, self.selector)])
+        f.upper = (ctypes.c_int32 * 3)(*[n - x if m else (1 << 30) for n, x, m in zip(self.lengths, o,
+                                                                                      self.selector)])
         f.dtype = self.code
         return f
 
     # ------------------------------------------------------------------ host mirror
     def _host_buffer(self):
-        return np.zeros((self.padded[2], self.padded[1], self.strides[1]), dtype=self.dtype)
+        return np.zeros((self.padded[2], self.padded[1], self.alloc_strides[1]), dtype=self.dtype)
 
     def to_numpy(self):
-        """Host copy of the whole field indexed [i, j, k] (the reference's host_view, data_store.hpp:140-146)."""
+        """Host copy of the whole field indexed [i, j, k] (the reference's host_view, data_store.hpp:140-146);
+        masked dimensions are broadcast to the field's lengths."""
         buf = self._host_buffer()
         _abi.call("gt355_memcpy_d2h", buf.ctypes.data, ctypes.c_void_p(self.base), ctypes.c_size_t(buf.nbytes))
-        ni, nj, nk = self.lengths
-        return np.ascontiguousarray(buf[:nk, :nj, :ni].transpose(2, 1, 0))
+        ni, nj, nk = self.alloc_lengths
+        arr = np.ascontiguousarray(buf[:nk, :nj, :ni].transpose(2, 1, 0))
+        return np.broadcast_to(arr, self.lengths).copy() if self.masked else arr
 
     def padded_numpy(self):
-        """Host copy in the device layout: array [k, j, i_padded] (for passing to CPU code with the same strides)."""
+        """Host copy in the device layout: array [k, j, i_padded] (for passing to CPU code with the same strides);
+        a masked dimension has length 1 here."""
         buf = self._host_buffer()
         _abi.call("gt355_memcpy_d2h", buf.ctypes.data, ctypes.c_void_p(self.base), ctypes.c_size_t(buf.nbytes))
         return buf
 
     def from_numpy(self, arr):
-        """Uploads an [i, j, k] array of the field's lengths (storage_update_target, storage/gpu.hpp:83-90)."""
+        """Uploads an [i, j, k] array of the field's lengths (storage_update_target, storage/gpu.hpp:83-90); a
+        masked dimension takes index 0 of the array (an array of the allocation's lengths is accepted too)."""
         arr = np.asarray(arr, dtype=self.dtype)
-        if arr.shape != self.lengths:
+        if arr.shape != self.lengths and arr.shape != self.alloc_lengths:
             raise ValueError(f"shape {arr.shape} != field lengths {self.lengths}")
+        if arr.shape != self.alloc_lengths:
+            arr = arr[tuple(slice(None) if m else slice(0, 1) for m in self.selector)]
         buf = self._host_buffer()
-        ni, nj, nk = self.lengths
+        ni, nj, nk = self.alloc_lengths
         buf[:nk, :nj, :ni] = arr.transpose(2, 1, 0)
         _abi.call("gt355_memcpy_h2d", ctypes.c_void_p(self.base), buf.ctypes.data, ctypes.c_size_t(buf.nbytes))
         return self
 
     def fill_uniform(self, seed, field_id, a, b, stream=None):
         """Synthetic input: SplitMix64 -> U(a,b) over the whole field, halo included (see inputs.py)."""
-        _abi.call("gt355_fill_uniform", self.code, ctypes.c_void_p(self.base), _abi.c_i64_3(*self.lengths),
-                  _abi.c_i64_3(*self.strides), ctypes.c_uint64(seed), ctypes.c_uint64(field_id),
+        _abi.call("gt355_fill_uniform", self.code, ctypes.c_void_p(self.base), _abi.c_i64_3(*self.alloc_lengths),
+                  _abi.c_i64_3(*self.alloc_strides), ctypes.c_uint64(seed), ctypes.c_uint64(field_id),
                   ctypes.c_double(a), ctypes.c_double(b), stream)
         return self
 
     def fill_value(self, v):
-        return self.from_numpy(np.full(self.lengths, v, dtype=self.dtype))
+        return self.from_numpy(np.full(self.alloc_lengths, v, dtype=self.dtype))
 
 
 class builder:
@@ -127,6 +145,7 @@ class builder:
         self._halos = (0, 0, 0)
         self._init = None
         self._name = ""
+        self._selector = (1, 1, 1)
 
     def _copy(self, **kw):
         b = builder()
@@ -146,6 +165,13 @@ class builder:
     def name(self, name):
         return self._copy(_name=name)
 
+    def selector(self, *mask):
+        """selector<I, J, K> (storage/builder.hpp:168-326): dimensions with mask 0 hold one value for every index
+        (selector(0, 1, 0) is a j-only field)."""
+        if len(mask) != 3:
+            raise ValueError("selector takes three flags")
+        return self._copy(_selector=tuple(int(bool(m)) for m in mask))
+
     def value(self, v):
         return self._copy(_init=("value", v))
 
@@ -159,13 +185,14 @@ class builder:
     def build(self):
         if self._dims is None:
             raise ValueError("dimensions() not set")
-        ds = DataStore(self._dtype, self._dims, self._halos, self._name)
+        ds = DataStore(self._dtype, self._dims, self._halos, self._name, self._selector)
         kind = self._init[0] if self._init else "value"
         if kind == "value":
             ds.fill_value(self._init[1] if self._init else 0)
         elif kind == "fn":
-            i, j, k = np.meshgrid(*[np.arange(n) for n in ds.lengths], indexing="ij")
-            ds.from_numpy(np.broadcast_to(self._init[1](i, j, k), ds.lengths))
+            # evaluated at index 0 of every masked dimension
+            i, j, k = np.meshgrid(*[np.arange(n) for n in ds.alloc_lengths], indexing="ij")
+            ds.from_numpy(np.broadcast_to(self._init[1](i, j, k), ds.alloc_lengths))
         else:
             ds.fill_uniform(*self._init[1])
             _abi.call("gt355_synchronize")

@@ -137,6 +137,31 @@ int gt355_hdiff_vadv(int precision, const gt355_field *in, const gt355_field *co
     const gt355_field *utens_stage, const gt355_field *wcon, const gt355_field *u_pos, const gt355_field *utens,
     double dtr_stage, int ni, int nj, int nk, void *stream);
 
+/* ------------------------------------------------------------------ further cartesian regression stencils
+ * simple_hori_diff (tests/regression/simple_hori_diff.cpp:25-85): the execute_parallel().ij_cached(lap)
+ * multistage of wlap_function (lap = in[i+1] + in[i-1] - 2 in + crlato (in[j+1] - in) + crlatu (in[j-1] - in))
+ * and divflux_function (out = in + ((fluxx_m - fluxx) + (fluxy_m - fluxy)) coeff), as one launch.  crlato and
+ * crlatu are typically j-only fields (the selector<0,1,0> storages of the reference test): stride 0 in i and k,
+ * bounds checked only in j.  Full 3-D metric fields are accepted too.  Reads in over [-2, n+2) in i and j and
+ * the metric over [-1, n+1); writes out over [0, n). */
+int gt355_simple_hori_diff(int precision, const gt355_field *out, const gt355_field *in, const gt355_field *crlato,
+    const gt355_field *crlatu, const gt355_field *coeff, int ni, int nj, int nk, void *stream);
+
+/* expandable_run of prepare_tracers (tests/regression/advection_pdbott_prepare_tracers.cpp:23-57, the member
+ * splitting of stencil/frontend/expandable_run.hpp:150-183): data[m] = rho * data_nnow[m] for m < nmembers.  The
+ * reference launches one replicated spec per chunk of Factor members; here one launch covers up to 32 members and
+ * reads rho once per point.  All data members share one layout, all data_nnow members another. */
+int gt355_prepare_tracers(int precision, const gt355_field *data, const gt355_field *data_nnow, int nmembers,
+    const gt355_field *rho, int ni, int nj, int nk, void *stream);
+
+/* copy_stencil_tuple (tests/regression/copy_stencil_tuple.cpp:23-100 functors, :116-260 tests): out[f] = in[f] over
+ * the interior for nfields field pairs of one dtype, one launch per 16 pairs.  Unlike the other entry points any
+ * non-negative i stride is accepted, so each member of an array-of-structs storage (array<T, 5> or tuple<T...>
+ * elements: origin + f, i stride 5) and each slice of an extra data dimension is a field; an input may have
+ * masked (stride 0) dimensions, an output may not, and an output's index map must be injective. */
+int gt355_copy_fields(const gt355_field *out, const gt355_field *in, int nfields, int ni, int nj, int nk,
+    void *stream);
+
 /* Strided 3-D copy with element conversion between F64 and F32 (the role of layout_transformation,
  * include/gridtools/layout_transformation/gpu.hpp:72-116, plus a dtype change): dst[i + j*ds1 + k*ds2] =
  * (dst type) src[i + j*ss1 + k*ss2] over lengths[0] x lengths[1] x lengths[2]; i strides must be 1. */

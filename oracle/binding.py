@@ -29,6 +29,11 @@ _SIGS = {
     "gto_vadv_f32": [_P, _P, _P, _P, _P, ctypes.c_float, _I, _I, _I, _PD, _PD, _PD, _PD, _I],
     "gto_vadv_mixed": [_P, _P, _P, _P, _P, ctypes.c_float, _I, _I, _I, _PD, _PD, _PD, _PD, _I],
     "gto_tridiagonal_f64": [_P, _P, _P, _P, _P, _I, _I, _I, _PD, _PD],
+    "gto_shd_f64": [_P, _P, _P, _P, _PD, _PD, _PD, _P, _I, _I, _I, _PD, _PD],
+    "gto_shd_f32": [_P, _P, _P, _P, _PD, _PD, _PD, _P, _I, _I, _I, _PD, _PD],
+    "gto_shd_mixed": [_P, _P, _P, _P, _PD, _PD, _PD, _P, _I, _I, _I, _PD, _PD],
+    "gto_prepare_tracers_f64": [_P, _P, _P, _I, _I, _I, _PD, _PD],
+    "gto_prepare_tracers_f32": [_P, _P, _P, _I, _I, _I, _PD, _PD],
     "gto_bc_f64": [_I, _P, _P, ctypes.c_double, _P, _PD, _PD, ctypes.c_uint32],
     "gto_bc_f32": [_I, _P, _P, ctypes.c_float, _P, _PD, _PD, ctypes.c_uint32],
     "gto_halo_fill_f64": [_P, _P, _PD, _PD, _P],
@@ -66,6 +71,7 @@ class View:
         self.sj = arr.shape[2]
         self.sk = arr.shape[1] * arr.shape[2]
         self.off = halo_k * self.sk + halo_j * self.sj + halo_i
+        self.halo = (halo_i, halo_j, halo_k)
 
     @property
     def ptr(self):
@@ -104,6 +110,40 @@ def vadv(uts: View, us: View, wc: View, up: View, ut: View, dtr, ni, nj, nk, mod
                             threads)
     if rc != 0:
         raise ValueError(f"{fn} rejected nk={nk}")
+
+
+def simple_hori_diff(out: View, in_: View, crlato, crlatu, coeff: View, ni, nj, nk, mode="native"):
+    """simple_hori_diff.cpp:25-80.  crlato / crlatu: either 1-D C-contiguous arrays over the storage j index (the
+    j-only fields of selector<0,1,0>; element halo_j + j belongs to grid row j, rows -1 .. nj are read) or Views
+    of full fields laid out like ``in_`` (read over i in [-1, ni], j in [-1, nj])."""
+    dt = _kind([out, in_, coeff])
+    assert out.sj == in_.sj == coeff.sj and out.sk == in_.sk == coeff.sk
+    hj = in_.halo[1]
+    ptrs = []
+    for c in (crlato, crlatu):
+        if isinstance(c, View):
+            if c.arr.dtype != dt or c.sj != in_.sj or c.sk != in_.sk or min(c.halo[:2]) < 1:
+                raise TypeError("a 3-D metric field must be laid out like in and have a 1-point halo")
+            ptrs.append(c.ptr)
+            strides = (1, c.sj, c.sk)
+        else:
+            if c.dtype != dt or c.ndim != 1 or not c.flags.c_contiguous or c.shape[0] < hj + nj + 1 or hj < 1:
+                raise TypeError("crlato / crlatu: 1-D arrays of the field dtype covering rows -1 .. nj")
+            ptrs.append(c.ctypes.data + hj * c.itemsize)
+            strides = (0, 1, 0)
+    if isinstance(crlato, View) != isinstance(crlatu, View):
+        raise TypeError("crlato and crlatu must be both j-only or both 3-D")
+    fn = {("float64", "native"): "gto_shd_f64", ("float32", "native"): "gto_shd_f32",
+          ("float32", "mixed"): "gto_shd_mixed"}[(dt.name, mode)]
+    getattr(lib(), fn)(out.ptr, in_.ptr, ptrs[0], ptrs[1], *strides, coeff.ptr, ni, nj, nk, out.sj, out.sk)
+
+
+def prepare_tracers(data: View, data_nnow: View, rho: View, ni, nj, nk):
+    """advection_pdbott_prepare_tracers.cpp:23-34 for one member: data = rho * data_nnow."""
+    dt = _kind([data, data_nnow, rho])
+    assert data.sj == data_nnow.sj == rho.sj and data.sk == data_nnow.sk == rho.sk
+    fn = {"float64": "gto_prepare_tracers_f64", "float32": "gto_prepare_tracers_f32"}[dt.name]
+    getattr(lib(), fn)(data.ptr, data_nnow.ptr, rho.ptr, ni, nj, nk, data.sj, data.sk)
 
 
 def tridiagonal(out: View, inf: View, diag: View, sup: View, rhs: View, ni, nj, nk):

@@ -107,6 +107,53 @@ void gto_lap_f32(float *out, const float *in, int ni, int nj, int nk, ptrdiff_t 
         }
 }
 
+/* ---------------------------------- simple_hori_diff ---------------------------------- */
+#define ST double
+#define CT double
+#define FN gto_shd_f64
+#include "shd_body.inc"
+#undef ST
+#undef CT
+#undef FN
+
+#define ST float
+#define CT float
+#define FN gto_shd_f32
+#include "shd_body.inc"
+#undef ST
+#undef CT
+#undef FN
+
+#define ST float
+#define CT double
+#define FN gto_shd_mixed
+#include "shd_body.inc"
+#undef ST
+#undef CT
+#undef FN
+
+/* ---------------------------------- prepare_tracers ----------------------------------
+ * tests/regression/advection_pdbott_prepare_tracers.cpp:23-34: data = rho * data_nnow, one member. */
+void gto_prepare_tracers_f64(double *data, const double *data_nnow, const double *rho, int ni, int nj, int nk,
+    ptrdiff_t sj, ptrdiff_t sk) {
+    for (int k = 0; k < nk; ++k)
+        for (int j = 0; j < nj; ++j)
+            for (int i = 0; i < ni; ++i) {
+                const ptrdiff_t p = (ptrdiff_t)k * sk + (ptrdiff_t)j * sj + i;
+                data[p] = rho[p] * data_nnow[p];
+            }
+}
+
+void gto_prepare_tracers_f32(float *data, const float *data_nnow, const float *rho, int ni, int nj, int nk,
+    ptrdiff_t sj, ptrdiff_t sk) {
+    for (int k = 0; k < nk; ++k)
+        for (int j = 0; j < nj; ++j)
+            for (int i = 0; i < ni; ++i) {
+                const ptrdiff_t p = (ptrdiff_t)k * sk + (ptrdiff_t)j * sj + i;
+                data[p] = rho[p] * data_nnow[p];
+            }
+}
+
 /* ---------------------------------- tridiagonal ----------------------------------
  * tests/regression/tridiagonal.cpp:33-68.  forward_thomas: first_level sup/=diag, rhs/=diag;
  * modify<1,0>: sup = sup/(diag - sup[k-1]*inf), rhs = (rhs - inf*rhs[k-1])/(diag - sup[k-1]*inf).

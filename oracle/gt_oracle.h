@@ -1,5 +1,6 @@
 /*
- * gt_oracle.h -- CPU restatement of the GridTools hdiff / vadv / laplacian / boundary path.
+ * gt_oracle.h -- CPU restatement of the GridTools hdiff / vadv / laplacian / boundary path and of the
+ * cartesian regression stencils simple_hori_diff and prepare_tracers.
  *
  * TEST INFRASTRUCTURE ONLY.  This library is the checker: it is loaded by tests/,
  * by __graft_entry__.smoke() and by the cpu_baseline leg of bench.py.  The product
@@ -41,6 +42,23 @@ void gto_hdiff_mixed(float *out, const float *in, const float *coeff, int ni, in
 /* ---- laplacian: tests/regression/laplacian.cpp:20-29 ---- */
 void gto_lap_f64(double *out, const double *in, int ni, int nj, int nk, ptrdiff_t sj, ptrdiff_t sk);
 void gto_lap_f32(float *out, const float *in, int ni, int nj, int nk, ptrdiff_t sj, ptrdiff_t sk);
+
+/* ---- simple_hori_diff: tests/regression/simple_hori_diff.cpp:25-80 ----
+ * crlato / crlatu (pointers at grid point (0,0,0)) have strides (ci, cj, ck): (0, 1, 0) for the j-only fields of
+ * the reference test (selector<0,1,0>).  _mixed: FP32 storage, FP64 loads, lap and arithmetic, result narrowed
+ * once on store. */
+void gto_shd_f64(double *out, const double *in, const double *cro, const double *cru, ptrdiff_t ci, ptrdiff_t cj,
+    ptrdiff_t ck, const double *coeff, int ni, int nj, int nk, ptrdiff_t sj, ptrdiff_t sk);
+void gto_shd_f32(float *out, const float *in, const float *cro, const float *cru, ptrdiff_t ci, ptrdiff_t cj,
+    ptrdiff_t ck, const float *coeff, int ni, int nj, int nk, ptrdiff_t sj, ptrdiff_t sk);
+void gto_shd_mixed(float *out, const float *in, const float *cro, const float *cru, ptrdiff_t ci, ptrdiff_t cj,
+    ptrdiff_t ck, const float *coeff, int ni, int nj, int nk, ptrdiff_t sj, ptrdiff_t sk);
+
+/* ---- prepare_tracers: tests/regression/advection_pdbott_prepare_tracers.cpp:23-34 (data = rho * data_nnow) ---- */
+void gto_prepare_tracers_f64(double *data, const double *data_nnow, const double *rho, int ni, int nj, int nk,
+    ptrdiff_t sj, ptrdiff_t sk);
+void gto_prepare_tracers_f32(float *data, const float *data_nnow, const float *rho, int ni, int nj, int nk,
+    ptrdiff_t sj, ptrdiff_t sk);
 
 /* ---- vertical advection: tests/regression/vertical_advection_dycore.cpp:32-149 ----
  * wcon may have its own strides (it is often allocated with nk+1 staggered levels).

@@ -163,3 +163,51 @@ def test_range_markers_need_no_device():
         with Range("inner"):
             pass
     assert _abi.lib().gt355_range_push(None) == -1
+
+
+def _field(origin, strides, lower, upper, dtype=None):
+    f = _abi.Field()
+    f.origin = origin
+    f.stride = (ctypes.c_int64 * 3)(*strides)
+    f.lower = (ctypes.c_int32 * 3)(*lower)
+    f.upper = (ctypes.c_int32 * 3)(*upper)
+    f.dtype = _abi.F64 if dtype is None else dtype
+    return f
+
+
+def test_cartesian_entry_points_validate_without_a_device():
+    """simple_hori_diff / prepare_tracers / copy_fields reject bad fields before any launch: a masked (stride 0)
+    dimension is accepted only on read-only metric / input fields, the metric box is checked in its unmasked
+    dimension, members must share a layout, and a written field's index map must be injective."""
+    lib = _abi.lib()
+    ni, nj, nk = 64, 32, 4
+    full = _field(0x100000, (1, 80, 80 * 40), (-2, -2, 0), (66, 34, nk))
+    jonly = _field(0x200000, (0, 1, 0), (-(1 << 30), -2, -(1 << 30)), (1 << 30, 34, 1 << 30))
+    short = _field(0x200000, (0, 1, 0), (-(1 << 30), 0, -(1 << 30)), (1 << 30, 34, 1 << 30))  # no row -1
+    b = ctypes.byref
+    # a j-only metric passes validation; a null origin elsewhere then fails, so nothing launches
+    nul = _field(0, (1, 80, 80 * 40), (-2, -2, 0), (66, 34, nk))
+    assert lib.gt355_simple_hori_diff(0, b(nul), b(full), b(jonly), b(jonly), b(full), ni, nj, nk, None) == -1
+    assert "null origin" in _abi.last_error()
+    assert lib.gt355_simple_hori_diff(0, b(full), b(full), b(short), b(jonly), b(full), ni, nj, nk, None) == -1
+    assert "crlato" in _abi.last_error() and "outside bounds" in _abi.last_error()
+    # precision form needs FP32 storage
+    assert lib.gt355_simple_hori_diff(1, b(full), b(full), b(jonly), b(jonly), b(full), ni, nj, nk, None) == -1
+    # prepare_tracers: members with different layouts
+    other = _field(0x300000, (1, 96, 96 * 40), (-2, -2, 0), (66, 34, nk))
+    data = (_abi.Field * 2)(full, other)
+    nnow = (_abi.Field * 2)(full, full)
+    assert lib.gt355_prepare_tracers(0, data, nnow, 2, b(full), ni, nj, nk, None) == -1
+    assert "layout" in _abi.last_error()
+    # copy_fields: a masked output, then an output whose i stride overlaps its rows
+    outs = (_abi.Field * 1)(jonly)
+    ins = (_abi.Field * 1)(full)
+    assert lib.gt355_copy_fields(outs, ins, 1, ni, nj, nk, None) == -1
+    assert "masked" in _abi.last_error()
+    overlap = _field(0x400000, (2, 80, 80 * 40), (0, 0, 0), (ni, nj, nk))
+    outs = (_abi.Field * 1)(overlap)
+    assert lib.gt355_copy_fields(outs, ins, 1, ni, nj, nk, None) == -1
+    assert "overlap" in _abi.last_error()
+    # empty calls are no-ops
+    assert lib.gt355_copy_fields(None, None, 0, ni, nj, nk, None) == 0
+    assert lib.gt355_prepare_tracers(0, None, None, 0, b(full), ni, nj, nk, None) == 0
