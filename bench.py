@@ -163,6 +163,60 @@ def copy_leg(gt, steps, warmup, nsets):
     return {"mean_us": round(ms * 1e3, 2), "gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
 
 
+def cartesian_leg(gt, steps, warmup, nsets):
+    """The further cartesian regression stencils at 256x256x80 FP64, each timed over cycling field sets (MALL-cold):
+    simple_hori_diff (in, coeff read, out written: 24 B/point; the j-only metric is negligible), the expandable_run of
+    prepare_tracers over 11 members (rho once + 11 reads + 11 writes: 184 B/point) and the five-field copy of
+    copy_stencil_tuple (80 B/point)."""
+    from gridtools_amd import stencils
+
+    h = 2
+    b = gt.builder().type(np.float64).dimensions(NI + 2 * h, NJ + 2 * h, NK).halos(h, h, 0)
+    g = gt.make_grid(gt.halo_descriptor.symmetric(h, NI + 2 * h), gt.halo_descriptor.symmetric(h, NJ + 2 * h), NK)
+    n = max(nsets, 4)
+    shd = [(b.uniform(SEED + s, 21, -1.0, 1.0).build(), b.selector(0, 1, 0).uniform(SEED + s, 23, 0.5, 1.5).build(),
+            b.selector(0, 1, 0).uniform(SEED + s, 24, 0.5, 1.5).build(), b.uniform(SEED + s, 22, 0.0, 0.25).build(),
+            b.value(0).build()) for s in range(n)]
+
+    def run_shd(s):
+        inp, cro, cru, coeff, out = shd[s % n]
+        stencils.simple_hori_diff(out, inp, cro, cru, coeff, g)
+
+    def stats(ms, bytes_per_pt):
+        gbs = bytes_per_pt * P / (ms * 1e-3) / 1e9
+        return {"mean_us": round(ms * 1e3, 2), "gpts_per_s": round(P / (ms * 1e-3) / 1e9, 3),
+                "achieved_gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+                "bytes_per_point": bytes_per_pt}
+
+    for s in range(warmup):
+        run_shd(s)
+    res = {"simple_hori_diff_fp64": stats(_event_time(run_shd, steps), 24)}
+    del shd
+    bt = gt.builder().type(np.float64).dimensions(NI, NJ, NK)
+    gt_ = gt.make_grid(NI, NJ, NK)
+    tracers = [[bt.uniform(SEED + 11 * c + m, 31, -1.0, 1.0).build() for m in range(11)] for c in range(2)]
+    outs = [[bt.value(0).build() for _ in range(11)] for _ in range(2)]
+    rho = [bt.uniform(SEED + c, 32, 0.5, 1.5).build() for c in range(2)]
+
+    def run_tracers(s):
+        stencils.prepare_tracers(outs[s % 2], tracers[s % 2], rho[s % 2], gt_)
+
+    for s in range(min(warmup, 4)):
+        run_tracers(s)
+    res["prepare_tracers_fp64_x11"] = stats(_event_time(run_tracers, max(10, steps // 4)), (2 * 11 + 1) * 8)
+    del tracers, outs, rho
+    ins = [[bt.uniform(SEED + 5 * c + t, 41, -1.0, 1.0).build() for t in range(5)] for c in range(2)]
+    outs = [[bt.value(0).build() for _ in range(5)] for _ in range(2)]
+
+    def run_copy(s):
+        stencils.copy_fields(outs[s % 2], ins[s % 2], gt_)
+
+    for s in range(min(warmup, 4)):
+        run_copy(s)
+    res["copy_tuple_fp64_x5"] = stats(_event_time(run_copy, max(10, steps // 2)), 5 * 16)
+    return res
+
+
 def _event_time(fn, n):
     """Mean ms of n calls of fn() bracketed by one event pair on the null stream (synchronised both sides)."""
     from gridtools_amd.timing import Event, synchronize
@@ -381,10 +435,11 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--sets", type=int, default=4)
-    ap.add_argument("--legs", default="tune,fp64,fp32,copy,compose,batch,large,cpu",
+    ap.add_argument("--legs", default="tune,fp64,fp32,copy,cartesian,compose,batch,large,cpu",
                     help="comma list of legs: tune (hdiff autotuner first, untimed), fp64 (always run), fp32 "
                          "(precision form per kernel), copy (bandwidth "
-                         "yardstick), compose (precision form on the fused composition, 512x512x80), batch (64 "
+                         "yardstick), cartesian (simple_hori_diff, prepare_tracers x11, copy_stencil_tuple x5), "
+                         "compose (precision form on the fused composition, 512x512x80), batch (64 "
                          "uneven domains, persistent launch, graph of 10 timesteps), large (1536x1536x80, 20 "
                          "timesteps with boundary conditions), cpu (CPU baseline)")
     args = ap.parse_args()
@@ -435,6 +490,11 @@ def main():
         del sets32
 
     copy = copy_leg(gt, args.steps, args.warmup, args.sets) if "copy" in legs else None
+    cartesian = None
+    if "cartesian" in legs:
+        print("[bench] cartesian regression stencils leg", file=sys.stderr, flush=True)
+        with Range("bench: cartesian regression stencils 256x256x80"):
+            cartesian = cartesian_leg(gt, args.steps, args.warmup, args.sets)
     compose = None
     if "compose" in legs:
         print("[bench] compose leg", file=sys.stderr, flush=True)
@@ -478,6 +538,7 @@ def main():
         "autotuned_hdiff": tuned,
         "fp32_form": fp32,
         "copy_fp64": copy,
+        "cartesian_stencils": cartesian,
         "precision_form_composition": compose,
         "batching_form": batch,
         "large_domain": large,

@@ -1,7 +1,8 @@
 /*
  * gt355/gridtools.hpp -- header-only C++17 front end of libgt355 with the GridTools user API for the stencils
  * of the hot path: storage::builder / data_store (storage/builder.hpp:168-326, storage/data_store.hpp:33-147),
- * halo_descriptor (common/halo_descriptor.hpp:44-227), make_grid (stencil/frontend/make_grid.hpp:19-42),
+ * with selector<I, J, K> masked dimensions, halo_descriptor (common/halo_descriptor.hpp:44-227), make_grid
+ * (stencil/frontend/make_grid.hpp:19-42),
  * global_parameter, stencil::run and expandable_run (stencil/frontend/run.hpp:201-258,
  * stencil/frontend/expandable_run.hpp:150-183) with a backend tag, and boundaries::boundary /
  * make_boundary (boundaries/boundary.hpp:31-79).
@@ -82,7 +83,7 @@ namespace gt355 {
             T *base;
             int64_t stride[3];
             std::array<int, 3> len;
-            T &operator()(int i, int j, int k) const { return base[i + j * stride[1] + k * stride[2]]; }
+            T &operator()(int i, int j, int k) const { return base[i * stride[0] + j * stride[1] + k * stride[2]]; }
             std::array<int, 3> const &lengths() const { return len; }
         };
 
@@ -92,6 +93,8 @@ namespace gt355 {
         template <class T>
         class data_store {
             gt355_layout m_layout{};
+            std::array<int, 3> m_lengths{};
+            std::array<bool, 3> m_sel{{true, true, true}};
             void *m_raw = nullptr;
             T *m_target = nullptr;
             std::vector<T> m_host;
@@ -116,10 +119,13 @@ namespace gt355 {
 
           public:
             using data_t = T;
-            data_store(std::array<int, 3> lengths, std::array<int, 3> halos, std::string name)
-                : m_name(std::move(name)) {
-                const int64_t l[3] = {lengths[0], lengths[1], lengths[2]};
-                const int64_t h[3] = {halos[0], halos[1], halos[2]};
+            // sel: the selector of storage/builder.hpp:168-326 -- a masked dimension (false) holds one value for
+            // every index: it is allocated with length 1 and has stride 0
+            data_store(std::array<int, 3> lengths, std::array<int, 3> halos, std::string name,
+                std::array<bool, 3> sel = {{true, true, true}})
+                : m_lengths(lengths), m_sel(sel), m_name(std::move(name)) {
+                const int64_t l[3] = {sel[0] ? lengths[0] : 1, sel[1] ? lengths[1] : 1, sel[2] ? lengths[2] : 1};
+                const int64_t h[3] = {sel[0] ? halos[0] : 0, sel[1] ? halos[1] : 0, sel[2] ? halos[2] : 0};
                 check(gt355_storage_layout(sizeof(T), l, h, &m_layout), "data_store: layout");
                 check(gt355_malloc((size_t)m_layout.alloc_bytes, &m_raw), "data_store: allocate");
                 void *base = nullptr;
@@ -137,14 +143,14 @@ namespace gt355 {
             }
 
             std::string const &name() const { return m_name; }
-            std::array<int, 3> lengths() const {
-                return {(int)m_layout.lengths[0], (int)m_layout.lengths[1], (int)m_layout.lengths[2]};
-            }
+            std::array<int, 3> lengths() const { return m_lengths; }
+            std::array<bool, 3> selector() const { return m_sel; }
             std::array<int, 3> native_lengths() const {
                 return {(int)m_layout.padded[0], (int)m_layout.padded[1], (int)m_layout.padded[2]};
             }
             std::array<int64_t, 3> strides() const {
-                return {m_layout.stride[0], m_layout.stride[1], m_layout.stride[2]};
+                return {m_sel[0] ? m_layout.stride[0] : 0, m_sel[1] ? m_layout.stride[1] : 0,
+                    m_sel[2] ? m_layout.stride[2] : 0};
             }
             std::array<int, 3> halos() const {
                 return {(int)m_layout.halos[0], (int)m_layout.halos[1], (int)m_layout.halos[2]};
@@ -162,11 +168,13 @@ namespace gt355 {
             storage::host_view<T> host_view() {
                 to_host();
                 m_state = state::host_only;
-                return {m_host.data(), {1, m_layout.stride[1], m_layout.stride[2]}, lengths()};
+                auto st = strides();
+                return {m_host.data(), {st[0], st[1], st[2]}, lengths()};
             }
             storage::host_view<T const> const_host_view() {
                 to_host();
-                return {m_host.data(), {1, m_layout.stride[1], m_layout.stride[2]}, lengths()};
+                auto st = strides();
+                return {m_host.data(), {st[0], st[1], st[2]}, lengths()};
             }
 
             // The SID of this store after core::shift_origin (stencil/core/backend.hpp:28-34): origin at storage
@@ -175,11 +183,13 @@ namespace gt355 {
             gt355_field sid(std::array<int, 3> origin, bool write) {
                 T *p = write ? get_target_ptr() : const_cast<T *>(get_const_target_ptr());
                 gt355_field f{};
-                f.origin = p + origin[0] + origin[1] * m_layout.stride[1] + origin[2] * m_layout.stride[2];
+                auto st = strides();
+                f.origin = p + origin[0] * st[0] + origin[1] * st[1] + origin[2] * st[2];
                 for (int d = 0; d < 3; ++d) {
-                    f.stride[d] = m_layout.stride[d];
-                    f.lower[d] = -origin[d];
-                    f.upper[d] = (int32_t)m_layout.lengths[d] - origin[d];
+                    f.stride[d] = st[d];
+                    // a masked dimension can be indexed anywhere
+                    f.lower[d] = m_sel[d] ? -origin[d] : -(1 << 30);
+                    f.upper[d] = m_sel[d] ? m_lengths[d] - origin[d] : (1 << 30);
                 }
                 f.dtype = dtype_of<T>();
                 return f;
@@ -191,6 +201,7 @@ namespace gt355 {
         template <class Traits, class T = void>
         class builder_type {
             std::array<int, 3> m_dims{{0, 0, 0}}, m_halos{{0, 0, 0}};
+            std::array<bool, 3> m_sel{{true, true, true}};
             std::string m_name;
             std::function<T(int, int, int)> m_init;
 
@@ -203,6 +214,7 @@ namespace gt355 {
                 builder_type<Traits, U> b;
                 b.m_dims = m_dims;
                 b.m_halos = m_halos;
+                b.m_sel = m_sel;
                 b.m_name = m_name;
                 return b;
             }
@@ -214,6 +226,13 @@ namespace gt355 {
             builder_type halos(int i, int j, int k) const {
                 builder_type b = *this;
                 b.m_halos = {i, j, k};
+                return b;
+            }
+            // selector<I, J, K>(): dimensions with a 0 flag are masked (selector<0, 1, 0> is a j-only field)
+            template <int I, int J, int K>
+            builder_type selector() const {
+                builder_type b = *this;
+                b.m_sel = {I != 0, J != 0, K != 0};
                 return b;
             }
             builder_type name(std::string n) const {
@@ -234,13 +253,16 @@ namespace gt355 {
             }
             std::shared_ptr<data_store<T>> build() const {
                 static_assert(!std::is_void<T>::value, "builder: call type<T>() first");
-                auto ds = std::make_shared<data_store<T>>(m_dims, m_halos, m_name);
+                auto ds = std::make_shared<data_store<T>>(m_dims, m_halos, m_name, m_sel);
                 if (m_init) {
-                    // the reference initializer fills every index of the allocation box (builder.hpp:67-84)
+                    // the reference initializer fills every index of the allocation box (builder.hpp:67-84); a
+                    // masked dimension is evaluated at index 0
                     auto v = ds->host_view();
-                    for (int k = 0; k < m_dims[2]; ++k)
-                        for (int j = 0; j < m_dims[1]; ++j)
-                            for (int i = 0; i < m_dims[0]; ++i)
+                    const int n0 = m_sel[0] ? m_dims[0] : 1, n1 = m_sel[1] ? m_dims[1] : 1,
+                              n2 = m_sel[2] ? m_dims[2] : 1;
+                    for (int k = 0; k < n2; ++k)
+                        for (int j = 0; j < n1; ++j)
+                            for (int i = 0; i < n0; ++i)
                                 v(i, j, k) = m_init(i, j, k);
                 }
                 return ds;
@@ -292,6 +314,10 @@ namespace gt355 {
             struct copy {};                 // (in, out)                       copy_stencil.cpp:23-41
             struct hdiff_vadv {};           // (in, coeff, u_stage, utens_stage, wcon, u_pos, utens, dtr_stage):
                                             // u_stage = hdiff(in, coeff), then vertical_advection of u_stage
+            struct horizontal_diffusion_fused {}; // (out, in, coeff)      horizontal_diffusion_fused.cpp:97-105
+            struct simple_hori_diff {};     // (coeff, in, out, crlato, crlatu) simple_hori_diff.cpp:73-80
+            struct prepare_tracers {};      // (data, data_nnow, rho)      advection_pdbott_prepare_tracers.cpp:44-51
+            struct copy_tuple {};           // (in0 .. inN-1, out0 .. outN-1) copy_stencil_tuple.cpp:116-150
         } // namespace specs
 
         template <class Be, class T>
@@ -347,6 +373,56 @@ namespace gt355 {
             check(gt355_hdiff_vadv(Be::precision, &fi, &fc, &fu, &fs, &fw, &fp, &ft, (double)dtr_stage.value, g.i_size,
                       g.j_size, g.k_size, be.stream),
                 "run(hdiff_vadv)");
+        }
+
+        // horizontal_diffusion_fused: the hdiff out_function written with stencil functions (call<lap>, call<flx>,
+        // call<fly>); it evaluates the same operations in the same order as the multistage hdiff, so it runs on
+        // the same kernels.
+        template <class Be, class T>
+        void run(specs::horizontal_diffusion_fused, Be be, grid const &g,
+            std::shared_ptr<storage::data_store<T>> const &out, std::shared_ptr<storage::data_store<T>> const &in,
+            std::shared_ptr<storage::data_store<T>> const &coeff) {
+            run(specs::horizontal_diffusion(), be, g, in, coeff, out);
+        }
+
+        template <class Be, class T>
+        void run(specs::simple_hori_diff, Be be, grid const &g, std::shared_ptr<storage::data_store<T>> const &coeff,
+            std::shared_ptr<storage::data_store<T>> const &in, std::shared_ptr<storage::data_store<T>> const &out,
+            std::shared_ptr<storage::data_store<T>> const &crlato,
+            std::shared_ptr<storage::data_store<T>> const &crlatu) {
+            auto o = g.origin();
+            gt355_field fc = coeff->sid(o, false), fi = in->sid(o, false), fo = out->sid(o, true),
+                        fa = crlato->sid(o, false), fu = crlatu->sid(o, false);
+            check(gt355_simple_hori_diff(Be::precision, &fo, &fi, &fa, &fu, &fc, g.i_size, g.j_size, g.k_size,
+                      be.stream),
+                "run(simple_hori_diff)");
+        }
+
+        template <class Be, class T>
+        void run(specs::prepare_tracers, Be be, grid const &g, std::shared_ptr<storage::data_store<T>> const &data,
+            std::shared_ptr<storage::data_store<T>> const &data_nnow,
+            std::shared_ptr<storage::data_store<T>> const &rho) {
+            auto o = g.origin();
+            gt355_field fd = data->sid(o, true), fn = data_nnow->sid(o, false), fr = rho->sid(o, false);
+            check(gt355_prepare_tracers(Be::precision, &fd, &fn, 1, &fr, g.i_size, g.j_size, g.k_size, be.stream),
+                "run(prepare_tracers)");
+        }
+
+        // copy_stencil_tuple's unrolled functor: the first half of the stores are read, the second half written
+        template <class Be, class T, class... Rest>
+        void run(specs::copy_tuple, Be be, grid const &g, std::shared_ptr<storage::data_store<T>> const &first,
+            Rest const &... rest) {
+            static_assert(sizeof...(Rest) % 2 == 1, "copy_tuple takes N inputs followed by N outputs");
+            std::vector<std::shared_ptr<storage::data_store<T>>> all{first, rest...};
+            const size_t n = all.size() / 2;
+            auto o = g.origin();
+            std::vector<gt355_field> in(n), out(n);
+            for (size_t f = 0; f < n; ++f) {
+                in[f] = all[f]->sid(o, false);
+                out[f] = all[n + f]->sid(o, true);
+            }
+            check(gt355_copy_fields(out.data(), in.data(), (int)n, g.i_size, g.j_size, g.k_size, be.stream),
+                "run(copy_tuple)");
         }
 
         // The batching form (gt355_batch_*): independent domains of uneven sizes, each computing lap =
@@ -438,6 +514,28 @@ namespace gt355 {
         // std::vector arguments are expanded member by member, scalar arguments are shared; every vector must
         // have the same size.  Factor is kept for source compatibility; the members are launched back to back
         // on the backend's stream (the batching form, gt355_batch_*, is the one-launch path).
+        // expandable_run of prepare_tracers: every member in one library call (one launch per 32 members) instead of
+        // one run per member
+        template <size_t Factor, class Be, class T>
+        void expandable_run(specs::prepare_tracers, Be be, grid const &g,
+            std::vector<std::shared_ptr<storage::data_store<T>>> const &data,
+            std::vector<std::shared_ptr<storage::data_store<T>>> const &data_nnow,
+            std::shared_ptr<storage::data_store<T>> const &rho) {
+            static_assert(Factor > 0, "expandable_run: Factor must be positive");
+            if (data.size() != data_nnow.size())
+                throw std::invalid_argument("expandable_run: vector arguments differ in size");
+            auto o = g.origin();
+            std::vector<gt355_field> fd(data.size()), fn(data.size());
+            for (size_t m = 0; m < data.size(); ++m) {
+                fd[m] = data[m]->sid(o, true);
+                fn[m] = data_nnow[m]->sid(o, false);
+            }
+            gt355_field fr = rho->sid(o, false);
+            check(gt355_prepare_tracers(Be::precision, fd.data(), fn.data(), (int)data.size(), &fr, g.i_size, g.j_size,
+                      g.k_size, be.stream),
+                "expandable_run(prepare_tracers)");
+        }
+
         template <size_t Factor, class Spec, class Be, class... Args>
         void expandable_run(Spec spec, Be be, grid const &g, Args const &... args) {
             static_assert(Factor > 0, "expandable_run: Factor must be positive");

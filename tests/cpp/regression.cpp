@@ -8,6 +8,7 @@
 //   gt355_regression --host     only the host-side cases (no device calls)
 #include <gt355/gridtools.hpp>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -209,6 +210,163 @@ namespace {
         report("expandable_run horizontal_diffusion x3", ok);
     }
 
+    // horizontal_diffusion_repository (tests/regression/horizontal_diffusion_repository.hpp:18-80): the analytic
+    // input, j-only metric and closed forms the reference's simple_hori_diff / horizontal_diffusion_fused tests
+    // verify against, restated for this program
+    struct hd_repository {
+        int d1, d2;
+        double delta0, delta1;
+        hd_repository(int a, int b)
+            : d1(a), d2(b), delta0((0.995156 - 0.994954) / (b - 1.)), delta1((0.995143 - 0.994924) / (b - 1.)) {}
+        double crlat0(int j) const { return 0.994954 + j * delta0; }
+        double crlat1(int j) const { return 0.994924 + j * delta1; }
+        double in(int i, int j) const {
+            const double PI = std::atan(1.) * 4.;
+            double x = (1. / d1) * i, y = (1. / d2) * j;
+            return 5. + 8 * (2. + std::cos(PI * (x + 1.5 * y)) + std::sin(2 * PI * (x + 1.5 * y))) / 4.;
+        }
+        double crlato(int j) const { return crlat1(j) / crlat0(j); }
+        double crlatu(int j) const { return j == 0 ? 0 : crlat1(j - 1) / crlat0(j); }
+        double out_simple(int i, int j) const {
+            auto lap = [&](int ii, int jj) {
+                return in(ii + 1, jj) + in(ii - 1, jj) - 2 * in(ii, jj) + crlato(jj) * (in(ii, jj + 1) - in(ii, jj)) +
+                       crlatu(jj) * (in(ii, jj - 1) - in(ii, jj));
+            };
+            auto fluxx = lap(i + 1, j) - lap(i, j);
+            auto fluxx_m = lap(i, j) - lap(i - 1, j);
+            auto fluxy = crlato(j) * (lap(i, j + 1) - lap(i, j));
+            auto fluxy_m = crlato(j) * (lap(i, j) - lap(i, j - 1));
+            return in(i, j) + ((fluxx_m - fluxx) + (fluxy_m - fluxy)) * 0.025;
+        }
+        double out(int i, int j) const {
+            auto lap = [&](int ii, int jj) {
+                return 4 * in(ii, jj) - (in(ii + 1, jj) + in(ii, jj + 1) + in(ii - 1, jj) + in(ii, jj - 1));
+            };
+            auto flx = [&](int ii, int jj) {
+                double r = lap(ii + 1, jj) - lap(ii, jj);
+                return r * (in(ii + 1, jj) - in(ii, jj)) > 0 ? 0. : r;
+            };
+            auto fly = [&](int ii, int jj) {
+                double r = lap(ii, jj + 1) - lap(ii, jj);
+                return r * (in(ii, jj + 1) - in(ii, jj)) > 0 ? 0. : r;
+            };
+            return in(i, j) - 0.025 * (flx(i, j) - flx(i - 1, j) + fly(i, j) - fly(i, j - 1));
+        }
+    };
+
+    // verifier.hpp:47-51 at the double precision 1e-14
+    bool close14(double e, double a) {
+        double err = std::fabs(e - a), m = std::max(std::fabs(e), std::fabs(a));
+        return err < 1e-14 || err < m * 1e-14;
+    }
+
+    // simple_hori_diff.cpp:63-85: j-only crlato / crlatu from selector<0,1,0>, verified against out_simple; the FP64
+    // result must also equal the oracle bit for bit
+    void simple_hori_diff(int ni, int nj, int nk) {
+        const int h = 2, d1 = ni + 2 * h, d2 = nj + 2 * h;
+        hd_repository repo(d1, d2);
+        auto b = storage::builder<storage::gpu>.type<double>().dimensions(d1, d2, nk).halos(h, h, 0);
+        auto coeff = b.value(0.025).build();
+        auto in = b.initializer([&](int i, int j, int) { return repo.in(i, j); }).build();
+        auto out = b.build();
+        auto jb = b.selector<0, 1, 0>();
+        auto crlato = jb.initializer([&](int, int j, int) { return repo.crlato(j); }).build();
+        auto crlatu = jb.initializer([&](int, int j, int) { return repo.crlatu(j); }).build();
+        auto g = make_grid(halo_descriptor(h, h, h, d1 - h - 1, d1), halo_descriptor(h, h, h, d2 - h - 1, d2), nk);
+        run(specs::simple_hori_diff(), gfx950<>(), g, coeff, in, out, crlato, crlatu);
+        auto v = out->const_host_view();
+        bool ok = crlato->strides()[0] == 0 && crlato->strides()[2] == 0;
+        for (int k = 0; k < nk && ok; ++k)
+            for (int j = h; j < h + nj && ok; ++j)
+                for (int i = h; i < h + ni && ok; ++i)
+                    ok = close14(repo.out_simple(i, j), v(i, j, k));
+        auto hin = mirror(in), hco = mirror(coeff);
+        std::vector<double> cro(d2), cru(d2), ref(hin.size(), 0.);
+        for (int j = 0; j < d2; ++j) {
+            cro[j] = crlato->const_host_view()(0, j, 0);
+            cru[j] = crlatu->const_host_view()(0, j, 0);
+        }
+        auto st = in->strides();
+        const size_t o = h + h * st[1];
+        gto_shd_f64(ref.data() + o, hin.data() + o, cro.data() + h, cru.data() + h, 0, 1, 0, hco.data() + o, ni, nj,
+            nk, st[1], st[2]);
+        ok = ok && same_interior(out, ref, g);
+        char name[96];
+        std::snprintf(name, sizeof name, "simple_hori_diff %dx%dx%d", ni, nj, nk);
+        report(name, ok);
+    }
+
+    // horizontal_diffusion_fused.cpp:93-106: the stencil-function form of hdiff, verified against the
+    // repository's out
+    void horizontal_diffusion_fused(int ni, int nj, int nk) {
+        const int h = 2, d1 = ni + 2 * h, d2 = nj + 2 * h;
+        hd_repository repo(d1, d2);
+        auto b = storage::builder<storage::gpu>.type<double>().dimensions(d1, d2, nk).halos(h, h, 0);
+        auto out = b.build();
+        auto in = b.initializer([&](int i, int j, int) { return repo.in(i, j); }).build();
+        auto coeff = b.value(0.025).build();
+        auto g = make_grid(halo_descriptor(h, h, h, d1 - h - 1, d1), halo_descriptor(h, h, h, d2 - h - 1, d2), nk);
+        run(specs::horizontal_diffusion_fused(), gfx950<>(), g, out, in, coeff);
+        auto v = out->const_host_view();
+        bool ok = true;
+        for (int k = 0; k < nk && ok; ++k)
+            for (int j = h; j < h + nj && ok; ++j)
+                for (int i = h; i < h + ni && ok; ++i)
+                    ok = close14(repo.out(i, j), v(i, j, k));
+        char name[96];
+        std::snprintf(name, sizeof name, "horizontal_diffusion_fused %dx%dx%d", ni, nj, nk);
+        report(name, ok);
+    }
+
+    // advection_pdbott_prepare_tracers.cpp:36-57: expandable_run<2> over 11 members, member i holds i, rho 1.1
+    template <class T>
+    void prepare_tracers() {
+        const int ni = 12, nj = 33, nk = 61;
+        auto b = storage::builder<storage::gpu>.template type<T>().dimensions(ni, nj, nk);
+        std::vector<std::shared_ptr<storage::data_store<T>>> in, out;
+        for (int m = 0; m < 11; ++m) {
+            out.push_back(b.build());
+            in.push_back(b.value(T(m)).build());
+        }
+        auto rho = b.value(T(1.1)).build();
+        expandable_run<2>(specs::prepare_tracers(), gfx950<>(), make_grid(ni, nj, nk), out, in, rho);
+        bool ok = true;
+        for (int m = 0; m < 11 && ok; ++m) {
+            auto v = out[m]->const_host_view();
+            const double want = 1.1 * m;
+            for (int k = 0; k < nk && ok; ++k)
+                for (int j = 0; j < nj && ok; ++j)
+                    for (int i = 0; i < ni && ok; ++i) {
+                        double a = v(i, j, k), err = std::fabs(a - want);
+                        ok = sizeof(T) == 8 ? a == want : (err < 1e-6 || err < std::fabs(want) * 1e-6);
+                    }
+        }
+        report(sizeof(T) == 8 ? "expandable_run prepare_tracers x11 f64" : "expandable_run prepare_tracers x11 f32",
+            ok);
+    }
+
+    // copy_stencil_tuple.cpp:116-159 (copy_stencil_tuple_unrolled): in_t = i + j + k + t
+    void copy_tuple() {
+        const int ni = 23, nj = 11, nk = 43;
+        auto b = storage::builder<storage::gpu>.type<double>().dimensions(ni, nj, nk);
+        std::vector<std::shared_ptr<storage::data_store<double>>> in, out;
+        for (int t = 0; t < 5; ++t) {
+            in.push_back(b.initializer([t](int i, int j, int k) { return i + j + k + t; }).build());
+            out.push_back(b.build());
+        }
+        run(specs::copy_tuple(), gfx950<>(), make_grid(ni, nj, nk), in[0], in[1], in[2], in[3], in[4], out[0], out[1],
+            out[2], out[3], out[4]);
+        bool ok = true;
+        for (int t = 0; t < 5 && ok; ++t) {
+            auto v = out[t]->const_host_view();
+            for (int k = 0; k < nk && ok; ++k)
+                for (int j = 0; j < nj && ok; ++j)
+                    for (int i = 0; i < ni && ok; ++i)
+                        ok = v(i, j, k) == double(i + j + k + t);
+        }
+        report("copy_stencil_tuple unrolled x5", ok);
+    }
+
     // the batching form: uneven domains in one launch equal laplacian + the composition run per domain (FP64,
     // where the two-launch composition and the batch compute the same bits)
     void batching() {
@@ -324,6 +482,12 @@ int main(int argc, char **argv) {
             expandable();
             batching();
             boundary_conditions();
+            simple_hori_diff(12, 33, 61);
+            simple_hori_diff(23, 11, 43);
+            horizontal_diffusion_fused(12, 33, 61);
+            prepare_tracers<double>();
+            prepare_tracers<float>();
+            copy_tuple();
         }
     } catch (std::exception const &e) {
         std::printf("[FAIL] exception: %s\n", e.what());
