@@ -166,8 +166,8 @@ def copy_leg(gt, steps, warmup, nsets):
 def cartesian_leg(gt, steps, warmup, nsets):
     """The further cartesian regression stencils at 256x256x80 FP64, each timed over cycling field sets (MALL-cold):
     simple_hori_diff (in, coeff read, out written: 24 B/point; the j-only metric is negligible), the expandable_run of
-    prepare_tracers over 11 members (rho once + 11 reads + 11 writes: 184 B/point) and the five-field copy of
-    copy_stencil_tuple (80 B/point)."""
+    prepare_tracers over 11 members (rho once + 11 reads + 11 writes: 184 B/point), the five-field copy of
+    copy_stencil_tuple (80 B/point) and the extended_4D projection (144 B/point)."""
     from gridtools_amd import stencils
 
     h = 2
@@ -214,6 +214,22 @@ def cartesian_leg(gt, steps, warmup, nsets):
     for s in range(min(warmup, 4)):
         run_copy(s)
     res["copy_tuple_fp64_x5"] = stats(_event_time(run_copy, max(10, steps // 2)), 5 * 16)
+    del ins, outs
+    # extended_4D: 8 f values and 2 jac values read, 8 results written per point (144 B/point FP64)
+    b6 = gt.builder().type(np.float64).dimensions(NI, NJ, NK, 2, 2, 2)
+    rng = np.random.default_rng(SEED)
+    phi, psi = rng.uniform(-1, 1, size=(2, 2, 2, 2)), rng.uniform(-1, 1, size=(2, 2, 2, 2))
+    ext = [(b6.build(), b6.uniform(SEED + c, 51, -1.0, 1.0).build(),
+            gt.builder().type(np.float64).dimensions(NI, NJ, NK, 2).uniform(SEED + c, 52, 0.5, 1.5).build())
+           for c in range(2)]
+
+    def run_ext(s):
+        r, f, jac = ext[s % 2]
+        stencils.extended_4d(r, f, jac, phi, psi, gt_)
+
+    for s in range(min(warmup, 4)):
+        run_ext(s)
+    res["extended_4d_fp64"] = stats(_event_time(run_ext, max(10, steps // 2)), 18 * 8)
     return res
 
 
@@ -438,7 +454,7 @@ def main():
     ap.add_argument("--legs", default="tune,fp64,fp32,copy,cartesian,compose,batch,large,cpu",
                     help="comma list of legs: tune (hdiff autotuner first, untimed), fp64 (always run), fp32 "
                          "(precision form per kernel), copy (bandwidth "
-                         "yardstick), cartesian (simple_hori_diff, prepare_tracers x11, copy_stencil_tuple x5), "
+                         "yardstick), cartesian (simple_hori_diff, prepare_tracers x11, copy_stencil_tuple x5, extended_4D), "
                          "compose (precision form on the fused composition, 512x512x80), batch (64 "
                          "uneven domains, persistent launch, graph of 10 timesteps), large (1536x1536x80, 20 "
                          "timesteps with boundary conditions), cpu (CPU baseline)")

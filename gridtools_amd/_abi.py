@@ -92,6 +92,8 @@ SIGNATURES = {
     "gt355_simple_hori_diff": [_I, _FP, _FP, _FP, _FP, _FP, _I, _I, _I, _P],
     "gt355_prepare_tracers": [_I, _FP, _FP, _I, _FP, _I, _I, _I, _P],
     "gt355_copy_fields": [_FP, _FP, _I, _I, _I, _I, _P],
+    "gt355_extended_4d": [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), _FP, c_i64_3, _FP,
+                          c_i64_3, _FP, ctypes.c_int64, _I, _I, _I, _P],
     "gt355_bc_apply": [_I, _I, ctypes.POINTER(_P), ctypes.POINTER(_P), _I, ctypes.c_double, c_halo, c_i64_3,
                        ctypes.c_uint32, _P],
     "gt355_halo_fill": [_I, ctypes.POINTER(_P), _I, c_halo, c_i64_3, c_i32_3, _P],

@@ -11,6 +11,9 @@
 //  * copy_stencil_tuple (copy_stencil_tuple.cpp:23-100, 116-260): out[f] = in[f] for a tuple of fields, one
 //    launch per 16 pairs.  Any positive i stride is accepted, so the members of an array-of-structs storage
 //    (element f of array<T, 5> at i stride 5) and the slices of an extra data dimension are fields too.
+//  * extended_4D (extended_4D.cpp:58-135): per grid point, a projection over extra storage dimensions -- the 8
+//    values f(a,b,c) and 2 jac(q) in, 8 results(I,J,K) out, with the 2x2x2x2 global-parameter tables phi / psi in
+//    the kernel arguments.  One thread per point.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -274,6 +277,62 @@ namespace gt355 {
             }
         }
 
+        // -------------------------------------------------------------------------------------------------------
+        // extended_4D integration, the operation order of integration::apply (extended_4D.cpp:67-90): sum and res in
+        // the storage type, products promoted to double by the double-valued global parameters.
+        // -------------------------------------------------------------------------------------------------------
+        struct ext4d_tables {
+            double phi[16], psi[16]; // [((x*2 + y)*2 + z)*2 + q]
+        };
+
+        template <class T>
+        __global__ void __launch_bounds__(256) ext4d_kernel(ext4d_tables tab, T *__restrict__ result,
+            const T *__restrict__ f, const T *__restrict__ jac, int ni, int nj, int nk, int64_t rsj, int64_t rsk,
+            int64_t rx0, int64_t rx1, int64_t rx2, int64_t fsj, int64_t fsk, int64_t fx0, int64_t fx1, int64_t fx2,
+            int64_t jsj, int64_t jsk, int64_t jq, int segs) {
+            const int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+            const int64_t w = n >> 6;
+            const int i = int(w % segs) * 64 + int(n & 63);
+            const int64_t row = w / segs;
+            const int j = int(row % nj), k = int(row / nj);
+            if (k >= nk || i >= ni)
+                return;
+            const T *fp = f + i + j * fsj + k * fsk;
+            T fv[8];
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+#pragma unroll
+                    for (int c = 0; c < 2; ++c)
+                        fv[(a * 2 + b) * 2 + c] = fp[a * fx0 + b * fx1 + c * fx2];
+            const T *jp = jac + i + j * jsj + k * jsk;
+            const T jv[2] = {jp[0], jp[jq]};
+            // sum over (a, b, c) for each quadrature point: independent of (I, J, K)
+            T sum[2];
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                T s_ = 0;
+#pragma unroll
+                for (int e = 0; e < 8; ++e)
+                    s_ += tab.psi[e * 2 + q] * fv[e];
+                sum[q] = s_;
+            }
+            T *rp = result + i + j * rsj + k * rsk;
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y)
+#pragma unroll
+                    for (int z = 0; z < 2; ++z) {
+                        T res = 0;
+#pragma unroll
+                        for (int q = 0; q < 2; ++q)
+                            res += tab.phi[((x * 2 + y) * 2 + z) * 2 + q] * jv[q] * sum[q];
+                        rp[x * rx0 + y * rx1 + z * rx2] = res / 8;
+                    }
+        }
+
         // grid of 64-lane row segments covering ni x nj x nk points
         int row_grid(const char *who, int ni, int nj, int nk, int *segs, int *blocks) {
             *segs = (ni + 63) / 64;
@@ -407,4 +466,53 @@ extern "C" int gt355_copy_fields(const gt355_field *out, const gt355_field *in, 
             return rc;
     }
     return GT355_OK;
+}
+
+extern "C" int gt355_extended_4d(const double phi[16], const double psi[16], const gt355_field *result,
+    const int64_t result_extra[3], const gt355_field *f, const int64_t f_extra[3], const gt355_field *jac,
+    int64_t jac_extra, int ni, int nj, int nk, void *stream) {
+    GT355_CHECK(ni >= 0 && nj >= 0 && nk >= 0, "gt355_extended_4d: negative size");
+    GT355_CHECK(phi && psi && result && result_extra && f && f_extra && jac, "gt355_extended_4d: null argument");
+    const int dtype = result->dtype;
+    GT355_CHECK(dtype == GT355_F64 || dtype == GT355_F32, "gt355_extended_4d: dtype %d", dtype);
+    if (ni == 0 || nj == 0 || nk == 0)
+        return GT355_OK;
+    int rc;
+    if ((rc = check_field(result, "result", dtype, 0, ni, 0, nj, 0, nk)) ||
+        (rc = check_field(f, "f", dtype, 0, ni, 0, nj, 0, nk)) ||
+        (rc = check_field(jac, "jac", dtype, 0, ni, 0, nj, 0, nk)))
+        return rc;
+    // every extra-dimension slice must lie beyond the spatial box of the previous one (no overlapping points)
+    auto vol = [](const gt355_field *x) { return x->stride[2] * (int64_t)(x->upper[2] - x->lower[2]); };
+    GT355_CHECK(jac_extra >= vol(jac), "gt355_extended_4d: jac's quadrature stride overlaps its spatial box");
+    for (int d = 0; d < 3; ++d) {
+        GT355_CHECK(f_extra[d] >= 0, "gt355_extended_4d: negative extra stride of f");
+        GT355_CHECK(result_extra[d] > 0, "gt355_extended_4d: result extra strides must be positive");
+    }
+    {
+        int64_t r[3] = {result_extra[0], result_extra[1], result_extra[2]};
+        std::sort(r, r + 3);
+        GT355_CHECK(r[0] >= vol(result) && r[1] >= 2 * r[0] && r[2] >= 2 * r[1],
+            "gt355_extended_4d: result's extra strides make points overlap");
+    }
+    int segs, blocks;
+    if ((rc = row_grid("gt355_extended_4d", ni, nj, nk, &segs, &blocks)))
+        return rc;
+    ext4d_tables tab;
+    for (int e = 0; e < 16; ++e) {
+        tab.phi[e] = phi[e];
+        tab.psi[e] = psi[e];
+    }
+    hipStream_t s = (hipStream_t)stream;
+#define GT355_EXT4D(T)                                                                                           \
+    ext4d_kernel<T><<<blocks, 256, 0, s>>>(tab, (T *)result->origin, (const T *)f->origin, (const T *)jac->origin,  \
+        ni, nj, nk, result->stride[1], result->stride[2], result_extra[0], result_extra[1], result_extra[2],          \
+        f->stride[1], f->stride[2], f_extra[0], f_extra[1], f_extra[2], jac->stride[1], jac->stride[2], jac_extra,   \
+        segs)
+    if (dtype == GT355_F64)
+        GT355_EXT4D(double);
+    else
+        GT355_EXT4D(float);
+#undef GT355_EXT4D
+    return check_launch("gt355_extended_4d");
 }

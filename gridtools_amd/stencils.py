@@ -100,3 +100,22 @@ def copy_fields(out, in_, grid: Grid, stream=None):
         return a
 
     _abi.call("gt355_copy_fields", arr(out), arr(in_), len(out), grid.i_size, grid.j_size, grid.k_size, stream)
+
+
+def extended_4d(result, f, jac, phi, psi, grid: Grid, stream=None):
+    """extended_4D (tests/regression/extended_4D.cpp:58-135): result and f are stores with extra dimensions
+    (2, 2, 2), jac with (2,); phi / psi are the global parameters as 2x2x2x2 tables [x, y, z, q] (a scalar is the
+    reference's constant elemental)."""
+    import numpy as np
+
+    if tuple(result.extra) != (2, 2, 2) or tuple(f.extra) != (2, 2, 2) or tuple(jac.extra) != (2,):
+        raise ValueError("result and f need extra dimensions (2, 2, 2), jac needs (2,)")
+
+    def table(t):
+        a = np.broadcast_to(np.asarray(t, dtype=np.float64), (2, 2, 2, 2))
+        return (ctypes.c_double * 16)(*np.ascontiguousarray(a).reshape(16))
+
+    r, fs, js = _sid(result, grid), _sid(f, grid), _sid(jac, grid)
+    _abi.call("gt355_extended_4d", table(phi), table(psi), ctypes.byref(r), _abi.c_i64_3(*result.extra_strides),
+              ctypes.byref(fs), _abi.c_i64_3(*f.extra_strides), ctypes.byref(js), ctypes.c_int64(jac.extra_strides[0]),
+              grid.i_size, grid.j_size, grid.k_size, stream)

@@ -28,14 +28,19 @@ class DataStore:
     def __init__(self, dtype, lengths, halos=(0, 0, 0), name="", selector=(1, 1, 1)):
         self.dtype = np.dtype(dtype)
         self.code = dtype_code(self.dtype)
-        self.lengths = tuple(int(x) for x in lengths)
+        lengths = tuple(int(x) for x in lengths)
+        if len(lengths) < 3 or len(halos) != 3 or len(selector) != 3:
+            raise ValueError("DataStore has three spatial dimensions (plus optional extra data dimensions)")
+        # extra data dimensions follow k in memory (the gpu layout_map<N-1, ..., 0>, storage/gpu.hpp:27-40)
+        self.lengths = lengths[:3]
+        self.extra = lengths[3:]
         self.halos = tuple(int(x) for x in halos)
         self.name = name
-        if len(self.lengths) != 3 or len(self.halos) != 3 or len(selector) != 3:
-            raise ValueError("DataStore is three-dimensional")
         # selector (storage::builder::selector, storage/builder.hpp:168-326): a masked dimension holds one value
         # for every index -- it is allocated with length 1 and has stride 0 in the SID
         self.selector = tuple(bool(x) for x in selector)
+        if self.extra and not all(self.selector):
+            raise ValueError("masked dimensions and extra data dimensions are not combined")
         self.alloc_lengths = tuple(n if m else 1 for n, m in zip(self.lengths, self.selector))
         alloc_halos = tuple(h if m else 0 for h, m in zip(self.halos, self.selector))
         lay = _abi.Layout()
@@ -45,15 +50,19 @@ class DataStore:
         self.alloc_strides = tuple(int(x) for x in lay.stride)
         self.strides = tuple(st if m else 0 for st, m in zip(self.alloc_strides, self.selector))
         self.padded = tuple(int(x) for x in lay.padded)
+        span = self.alloc_strides[2] * self.padded[2]  # elements of one spatial slice
+        self.extra_strides = tuple(int(span * np.prod(self.extra[:d], dtype=np.int64)) for d in range(len(self.extra)))
+        nslices = int(np.prod(self.extra, dtype=np.int64))
+        alloc_bytes = lay.alloc_bytes + (nslices - 1) * span * self.dtype.itemsize
         raw = ctypes.c_void_p()
-        _abi.call("gt355_malloc", ctypes.c_size_t(lay.alloc_bytes), ctypes.byref(raw))
+        _abi.call("gt355_malloc", ctypes.c_size_t(alloc_bytes), ctypes.byref(raw))
         self._raw = raw
         # padding and alignment slack are zeroed once so that no copy of the allocation ever carries garbage
-        _abi.call("gt355_memset", raw, 0, ctypes.c_size_t(lay.alloc_bytes), None)
+        _abi.call("gt355_memset", raw, 0, ctypes.c_size_t(alloc_bytes), None)
         base = ctypes.c_void_p()
         _abi.call("gt355_storage_base", ctypes.byref(lay), raw, ctypes.byref(base))
         self.base = base.value  # address of storage index (0, 0, 0)
-        self.nelem = self.alloc_strides[2] * self.padded[2]  # elements from base covered by the layout
+        self.nelem = span * nslices  # elements from base covered by the layout
 
     @property
     def masked(self):
@@ -72,17 +81,19 @@ class DataStore:
             pass
 
     # ------------------------------------------------------------------ addressing
-    def address(self, i, j, k):
-        """Device address of storage index (i, j, k)."""
+    def address(self, i, j, k, *extra):
+        """Device address of storage index (i, j, k, extra...)."""
         off = i * self.strides[0] + j * self.strides[1] + k * self.strides[2]
+        off += sum(e * s for e, s in zip(extra, self.extra_strides))
         return self.base + off * self.dtype.itemsize
 
-    def sid(self, origin=None):
-        """gt355_field whose origin is storage index ``origin`` (default: the first interior point), with the
-        allocation box as bounds -- the SID after core::shift_origin (stencil/core/backend.hpp:28-34)."""
+    def sid(self, origin=None, extra=()):
+        """gt355_field whose origin is storage index ``origin`` (default: the first interior point) of the spatial
+        slice ``extra``, with the allocation box as bounds -- the SID after core::shift_origin
+        (stencil/core/backend.hpp:28-34)."""
         o = self.halos if origin is None else tuple(origin)
         f = _abi.Field()
-        f.origin = self.address(*o)
+        f.origin = self.address(*o, *extra)
         f.stride = (ctypes.c_int64 * 3)(*self.strides)
         # a masked dimension can be indexed anywhere (stride 0)
         f.lower = (ctypes.c_int32 * 3)(*[-x if m else -(1 << 30) for x, m in zip(o, self.selector)])
@@ -93,15 +104,16 @@ class DataStore:
 
     # ------------------------------------------------------------------ host mirror
     def _host_buffer(self):
-        return np.zeros((self.padded[2], self.padded[1], self.alloc_strides[1]), dtype=self.dtype)
+        return np.zeros(tuple(reversed(self.extra)) + (self.padded[2], self.padded[1], self.alloc_strides[1]),
+                        dtype=self.dtype)
 
     def to_numpy(self):
-        """Host copy of the whole field indexed [i, j, k] (the reference's host_view, data_store.hpp:140-146);
-        masked dimensions are broadcast to the field's lengths."""
+        """Host copy of the whole field indexed [i, j, k, extra...] (the reference's host_view,
+        data_store.hpp:140-146); masked dimensions are broadcast to the field's lengths."""
         buf = self._host_buffer()
         _abi.call("gt355_memcpy_d2h", buf.ctypes.data, ctypes.c_void_p(self.base), ctypes.c_size_t(buf.nbytes))
         ni, nj, nk = self.alloc_lengths
-        arr = np.ascontiguousarray(buf[:nk, :nj, :ni].transpose(2, 1, 0))
+        arr = np.ascontiguousarray(buf[..., :nk, :nj, :ni].transpose(*range(buf.ndim - 1, -1, -1)))
         return np.broadcast_to(arr, self.lengths).copy() if self.masked else arr
 
     def padded_numpy(self):
@@ -115,25 +127,28 @@ class DataStore:
         """Uploads an [i, j, k] array of the field's lengths (storage_update_target, storage/gpu.hpp:83-90); a
         masked dimension takes index 0 of the array (an array of the allocation's lengths is accepted too)."""
         arr = np.asarray(arr, dtype=self.dtype)
-        if arr.shape != self.lengths and arr.shape != self.alloc_lengths:
-            raise ValueError(f"shape {arr.shape} != field lengths {self.lengths}")
-        if arr.shape != self.alloc_lengths:
+        if arr.shape != self.lengths + self.extra and arr.shape != self.alloc_lengths + self.extra:
+            raise ValueError(f"shape {arr.shape} != field lengths {self.lengths + self.extra}")
+        if arr.shape[:3] != self.alloc_lengths:
             arr = arr[tuple(slice(None) if m else slice(0, 1) for m in self.selector)]
         buf = self._host_buffer()
         ni, nj, nk = self.alloc_lengths
-        buf[:nk, :nj, :ni] = arr.transpose(2, 1, 0)
+        buf[..., :nk, :nj, :ni] = arr.transpose(*range(arr.ndim - 1, -1, -1))
         _abi.call("gt355_memcpy_h2d", ctypes.c_void_p(self.base), buf.ctypes.data, ctypes.c_size_t(buf.nbytes))
         return self
 
     def fill_uniform(self, seed, field_id, a, b, stream=None):
-        """Synthetic input: SplitMix64 -> U(a,b) over the whole field, halo included (see inputs.py)."""
-        _abi.call("gt355_fill_uniform", self.code, ctypes.c_void_p(self.base), _abi.c_i64_3(*self.alloc_lengths),
+        """Synthetic input: SplitMix64 -> U(a,b) over the whole field, halo included (see inputs.py); extra
+        dimensions continue the logical order after k (k is never padded, so they merge with it)."""
+        n0, n1, n2 = self.alloc_lengths
+        n2 *= int(np.prod(self.extra, dtype=np.int64))
+        _abi.call("gt355_fill_uniform", self.code, ctypes.c_void_p(self.base), _abi.c_i64_3(n0, n1, n2),
                   _abi.c_i64_3(*self.alloc_strides), ctypes.c_uint64(seed), ctypes.c_uint64(field_id),
                   ctypes.c_double(a), ctypes.c_double(b), stream)
         return self
 
     def fill_value(self, v):
-        return self.from_numpy(np.full(self.alloc_lengths, v, dtype=self.dtype))
+        return self.from_numpy(np.full(self.alloc_lengths + self.extra, v, dtype=self.dtype))
 
 
 class builder:
@@ -190,9 +205,10 @@ class builder:
         if kind == "value":
             ds.fill_value(self._init[1] if self._init else 0)
         elif kind == "fn":
-            # evaluated at index 0 of every masked dimension
-            i, j, k = np.meshgrid(*[np.arange(n) for n in ds.alloc_lengths], indexing="ij")
-            ds.from_numpy(np.broadcast_to(self._init[1](i, j, k), ds.alloc_lengths))
+            # fn(i, j, k, extra...); evaluated at index 0 of every masked dimension
+            shape = ds.alloc_lengths + ds.extra
+            idx = np.meshgrid(*[np.arange(n) for n in shape], indexing="ij")
+            ds.from_numpy(np.broadcast_to(self._init[1](*idx), shape))
         else:
             ds.fill_uniform(*self._init[1])
             _abi.call("gt355_synchronize")

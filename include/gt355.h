@@ -39,9 +39,10 @@ enum gt355_precision { GT355_NATIVE = 0, GT355_WIDEN = 1 };
 
 /* A field as the backend sees it after core::shift_origin (stencil/core/backend.hpp:28-34): the SID
  * triple of origin, strides and bounds (sid/concept.hpp:36-120).  origin is the device address of grid point
- * (0,0,0); stride[0] must be 1 (i-innermost, storage/gpu.hpp:34-40); lower[d] <= 0 < upper[d] bound the
- * offsets from the origin that may be touched (the allocation box).  Every entry point checks the offsets
- * its stencil reads or writes against these bounds before launching. */
+ * (0,0,0); stride[0] must be 1 (i-innermost, storage/gpu.hpp:34-40) unless an entry point says otherwise (a
+ * masked dimension of a selector<> storage has stride 0; gt355_copy_fields takes any i stride); lower[d] <= 0 <
+ * upper[d] bound the offsets from the origin that may be touched (the allocation box).  Every entry point checks
+ * the offsets its stencil reads or writes against these bounds before launching. */
 typedef struct gt355_field {
     void *origin;
     int64_t stride[3];
@@ -161,6 +162,17 @@ int gt355_prepare_tracers(int precision, const gt355_field *data, const gt355_fi
  * masked (stride 0) dimensions, an output may not, and an output's index map must be injective. */
 int gt355_copy_fields(const gt355_field *out, const gt355_field *in, int nfields, int ni, int nj, int nk,
     void *stream);
+
+/* extended_4D (tests/regression/extended_4D.cpp:58-135): per grid point, the projection
+ *   result(I,J,K) = 1/8 sum_q phi(I,J,K,q) jac(q) sum_{a,b,c} psi(a,b,c,q) f(a,b,c)      (I..c, q in {0, 1})
+ * in the operation order of integration::apply.  result and f are storages with three extra dimensions of 2
+ * (element (I,J,K) at the spatial field's address + I*result_extra[0] + J*result_extra[1] + K*result_extra[2]), jac
+ * has one (q at + q*jac_extra): the gpu layout puts extra dimensions after k (storage/gpu.hpp:27-40).  phi / psi are
+ * the global parameters' values, [((x*2 + y)*2 + z)*2 + q], in double as the reference's elemental returns them
+ * (a float storage keeps sum and res in float and the products in double, like the C++ expression). */
+int gt355_extended_4d(const double phi[16], const double psi[16], const gt355_field *result,
+    const int64_t result_extra[3], const gt355_field *f, const int64_t f_extra[3], const gt355_field *jac,
+    int64_t jac_extra, int ni, int nj, int nk, void *stream);
 
 /* Strided 3-D copy with element conversion between F64 and F32 (the role of layout_transformation,
  * include/gridtools/layout_transformation/gpu.hpp:72-116, plus a dtype change): dst[i + j*ds1 + k*ds2] =

@@ -32,6 +32,8 @@ _SIGS = {
     "gto_shd_f64": [_P, _P, _P, _P, _PD, _PD, _PD, _P, _I, _I, _I, _PD, _PD],
     "gto_shd_f32": [_P, _P, _P, _P, _PD, _PD, _PD, _P, _I, _I, _I, _PD, _PD],
     "gto_shd_mixed": [_P, _P, _P, _P, _PD, _PD, _PD, _P, _I, _I, _I, _PD, _PD],
+    "gto_extended_4d_f64": [_P, _P, _P, _P, _P, _PD, _P, _P, _I, _I, _I, _PD, _PD],
+    "gto_extended_4d_f32": [_P, _P, _P, _P, _P, _PD, _P, _P, _I, _I, _I, _PD, _PD],
     "gto_prepare_tracers_f64": [_P, _P, _P, _I, _I, _I, _PD, _PD],
     "gto_prepare_tracers_f32": [_P, _P, _P, _I, _I, _I, _PD, _PD],
     "gto_bc_f64": [_I, _P, _P, ctypes.c_double, _P, _PD, _PD, ctypes.c_uint32],
@@ -144,6 +146,28 @@ def prepare_tracers(data: View, data_nnow: View, rho: View, ni, nj, nk):
     assert data.sj == data_nnow.sj == rho.sj and data.sk == data_nnow.sk == rho.sk
     fn = {"float64": "gto_prepare_tracers_f64", "float32": "gto_prepare_tracers_f32"}[dt.name]
     getattr(lib(), fn)(data.ptr, data_nnow.ptr, rho.ptr, ni, nj, nk, data.sj, data.sk)
+
+
+def extended_4d(result, f, jac, phi, psi, ni, nj, nk):
+    """extended_4D.cpp:58-91.  result / f: C-contiguous [c, b, a, k, j, i] arrays (the extra dimensions slowest, as
+    the gpu layout orders them), jac: [q, k, j, i]; all share the spatial shape.  phi / psi: 2x2x2x2 tables
+    [x, y, z, q] (float64: the reference's elemental returns double)."""
+    dt = result.dtype
+    if f.dtype != dt or jac.dtype != dt or dt not in (np.float64, np.float32):
+        raise TypeError("result, f and jac must share a float64 or float32 dtype")
+    for a in (result, f, jac):
+        if not a.flags.c_contiguous or a.shape[-3:] != result.shape[-3:]:
+            raise TypeError("C-contiguous arrays with one spatial shape")
+    if result.shape[:3] != (2, 2, 2) or f.shape[:3] != (2, 2, 2) or jac.shape[0] != 2:
+        raise TypeError("result / f have extra dimensions 2x2x2, jac has 2 quadrature points")
+    nz, ny, nx = result.shape[-3:]
+    sj, sk, vol = nx, nx * ny, nx * ny * nz
+    ext = np.array([vol, 2 * vol, 4 * vol], dtype=np.int64)  # strides of the extra dimensions (a, b, c)
+    tphi = np.ascontiguousarray(np.asarray(phi, dtype=np.float64).reshape(16))
+    tpsi = np.ascontiguousarray(np.asarray(psi, dtype=np.float64).reshape(16))
+    fn = "gto_extended_4d_f64" if dt == np.float64 else "gto_extended_4d_f32"
+    getattr(lib(), fn)(result.ctypes.data, ext.ctypes.data, f.ctypes.data, ext.ctypes.data, jac.ctypes.data, vol,
+                       tphi.ctypes.data, tpsi.ctypes.data, ni, nj, nk, sj, sk)
 
 
 def tridiagonal(out: View, inf: View, diag: View, sup: View, rhs: View, ni, nj, nk):

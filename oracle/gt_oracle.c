@@ -154,6 +154,49 @@ void gto_prepare_tracers_f32(float *data, const float *data_nnow, const float *r
             }
 }
 
+/* ---------------------------------- extended_4D ----------------------------------
+ * tests/regression/extended_4D.cpp:58-91 (integration::apply), one grid point at a time:
+ *   for I, J, K in {0,1}: res = 0; for q: sum = 0; for a, b, c: sum += psi(a,b,c,q) * f(a,b,c);
+ *                                     res += phi(I,J,K,q) * jac(q) * sum;
+ *                         result(I,J,K) = res / 8.
+ * phi / psi are the global parameters' values (the reference's elemental returns double), indexed
+ * [((x*2 + y)*2 + z)*2 + q].  jac carries the quadrature index q at stride jq; f and result carry (a,b,c) /
+ * (I,J,K) at strides fs[3] / rs[3].  The float instantiation keeps sum and res in float and the products in
+ * double (double * float promotes), exactly as the C++ expression evaluation does. */
+#define GTO_EXT4D_BODY(T)                                                                                        \
+    for (int k = 0; k < nk; ++k)                                                                                 \
+        for (int j = 0; j < nj; ++j)                                                                             \
+            for (int i = 0; i < ni; ++i) {                                                                       \
+                const ptrdiff_t p = (ptrdiff_t)k * sk + (ptrdiff_t)j * sj + i;                                   \
+                for (int I = 0; I < 2; ++I)                                                                      \
+                    for (int J = 0; J < 2; ++J)                                                                  \
+                        for (int K = 0; K < 2; ++K) {                                                            \
+                            T res = 0;                                                                           \
+                            for (int q = 0; q < 2; ++q) {                                                        \
+                                T sum = 0;                                                                       \
+                                for (int a = 0; a < 2; ++a)                                                      \
+                                    for (int b = 0; b < 2; ++b)                                                  \
+                                        for (int c = 0; c < 2; ++c)                                              \
+                                            sum += psi[((a * 2 + b) * 2 + c) * 2 + q] *                          \
+                                                   f[p + a * fs[0] + b * fs[1] + c * fs[2]];                     \
+                                res += phi[((I * 2 + J) * 2 + K) * 2 + q] * jac[p + q * jq] * sum;               \
+                            }                                                                                    \
+                            result[p + I * rs[0] + J * rs[1] + K * rs[2]] = res / 8;                             \
+                        }                                                                                        \
+            }
+
+void gto_extended_4d_f64(double *result, const ptrdiff_t rs[3], const double *f, const ptrdiff_t fs[3],
+    const double *jac, ptrdiff_t jq, const double phi[16], const double psi[16], int ni, int nj, int nk, ptrdiff_t sj,
+    ptrdiff_t sk) {
+    GTO_EXT4D_BODY(double)
+}
+
+void gto_extended_4d_f32(float *result, const ptrdiff_t rs[3], const float *f, const ptrdiff_t fs[3],
+    const float *jac, ptrdiff_t jq, const double phi[16], const double psi[16], int ni, int nj, int nk, ptrdiff_t sj,
+    ptrdiff_t sk) {
+    GTO_EXT4D_BODY(float)
+}
+
 /* ---------------------------------- tridiagonal ----------------------------------
  * tests/regression/tridiagonal.cpp:33-68.  forward_thomas: first_level sup/=diag, rhs/=diag;
  * modify<1,0>: sup = sup/(diag - sup[k-1]*inf), rhs = (rhs - inf*rhs[k-1])/(diag - sup[k-1]*inf).

@@ -1,6 +1,6 @@
 /*
  * gt_oracle.h -- CPU restatement of the GridTools hdiff / vadv / laplacian / boundary path and of the
- * cartesian regression stencils simple_hori_diff and prepare_tracers.
+ * cartesian regression stencils simple_hori_diff, prepare_tracers and extended_4D.
  *
  * TEST INFRASTRUCTURE ONLY.  This library is the checker: it is loaded by tests/,
  * by __graft_entry__.smoke() and by the cpu_baseline leg of bench.py.  The product
@@ -59,6 +59,16 @@ void gto_prepare_tracers_f64(double *data, const double *data_nnow, const double
     ptrdiff_t sj, ptrdiff_t sk);
 void gto_prepare_tracers_f32(float *data, const float *data_nnow, const float *rho, int ni, int nj, int nk,
     ptrdiff_t sj, ptrdiff_t sk);
+
+/* ---- extended_4D: tests/regression/extended_4D.cpp:58-91 ----
+ * result (I,J,K extra dims at strides rs), f (a,b,c extra dims at strides fs) and jac (q at stride jq) share the
+ * spatial strides (1, sj, sk); phi / psi are the 2x2x2x2 global-parameter tables [((x*2+y)*2+z)*2+q]. */
+void gto_extended_4d_f64(double *result, const ptrdiff_t rs[3], const double *f, const ptrdiff_t fs[3],
+    const double *jac, ptrdiff_t jq, const double phi[16], const double psi[16], int ni, int nj, int nk, ptrdiff_t sj,
+    ptrdiff_t sk);
+void gto_extended_4d_f32(float *result, const ptrdiff_t rs[3], const float *f, const ptrdiff_t fs[3],
+    const float *jac, ptrdiff_t jq, const double phi[16], const double psi[16], int ni, int nj, int nk, ptrdiff_t sj,
+    ptrdiff_t sk);
 
 /* ---- vertical advection: tests/regression/vertical_advection_dycore.cpp:32-149 ----
  * wcon may have its own strides (it is often allocated with nk+1 staggered levels).

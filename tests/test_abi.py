@@ -211,3 +211,23 @@ def test_cartesian_entry_points_validate_without_a_device():
     # empty calls are no-ops
     assert lib.gt355_copy_fields(None, None, 0, ni, nj, nk, None) == 0
     assert lib.gt355_prepare_tracers(0, None, None, 0, b(full), ni, nj, nk, None) == 0
+
+
+def test_extended_4d_validates_without_a_device():
+    lib = _abi.lib()
+    ni, nj, nk = 8, 4, 2
+    sp = _field(0x100000, (1, 16, 64), (0, 0, 0), (ni, nj, nk))
+    tab = (ctypes.c_double * 16)(*([1.0] * 16))
+    vol = 64 * nk
+    ok3 = _abi.c_i64_3(vol, 2 * vol, 4 * vol)
+    # result slices that overlap each other are rejected
+    bad3 = _abi.c_i64_3(vol, vol, 4 * vol)
+    rc = lib.gt355_extended_4d(tab, tab, ctypes.byref(sp), bad3, ctypes.byref(sp), ok3, ctypes.byref(sp),
+                               ctypes.c_int64(vol), ni, nj, nk, None)
+    assert rc == -1 and "overlap" in _abi.last_error()
+    # a quadrature stride inside the spatial box is rejected
+    rc = lib.gt355_extended_4d(tab, tab, ctypes.byref(sp), ok3, ctypes.byref(sp), ok3, ctypes.byref(sp),
+                               ctypes.c_int64(8), ni, nj, nk, None)
+    assert rc == -1 and "quadrature" in _abi.last_error()
+    assert lib.gt355_extended_4d(None, tab, ctypes.byref(sp), ok3, ctypes.byref(sp), ok3, ctypes.byref(sp),
+                                 ctypes.c_int64(vol), ni, nj, nk, None) == -1

@@ -11,6 +11,8 @@
   11 members, seeded members against the oracle, and more members than one launch holds.
 * copy_stencil_tuple (copy_stencil_tuple.cpp:116-260): the unrolled five-field copy (expected i + j + k + t), an
   array-of-structs storage, an extra data dimension, a masked (j-only) input and more pairs than one launch holds.
+* extended_4D (extended_4D.cpp:104-134): the reference's known answer (jac0 + jac1) phi psi f over 6-D stores, and
+  random 2x2x2x2 tables against the oracle bit for bit.
 """
 import ctypes
 
@@ -264,3 +266,46 @@ def test_copy_fields_masked_input_and_many_pairs(gpu):
     for t in range(1, 20):
         assert np.array_equal(outs[t].to_numpy()[h:h + ni, h:h + nj, :], ins[t].to_numpy()[h:h + ni, h:h + nj, :])
         assert not outs[t].padded_numpy()[:nk, :h, :].any()
+
+
+# ------------------------------------------------------------------------------------------------ extended_4D
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-14), (np.float32, 1e-6)])
+def test_extended_4d_reference_case(gpu, oracle, dtype, tol):
+    """extended_4D.cpp:104-134: phi = 10, psi = 11, f = 1.3 over (i,j,k,a,b,c), jac(i,j,k,q) = 1 + q; expected
+    (jac0 + jac1) phi psi f at the reference's tolerance, and the oracle's bits."""
+    from gridtools_amd import stencils
+
+    ni, nj, nk = 12, 33, 61
+    print(f"[case] extended_4d reference {np.dtype(dtype).name}", flush=True)
+    result = gpu.builder().type(dtype).dimensions(ni, nj, nk, 2, 2, 2).build()
+    f = gpu.builder().type(dtype).dimensions(ni, nj, nk, 2, 2, 2).value(1.3).build()
+    jac = gpu.builder().type(dtype).dimensions(ni, nj, nk, 2).initializer(lambda i, j, k, q: 1 + q).build()
+    stencils.extended_4d(result, f, jac, 10.0, 11.0, gpu.make_grid(ni, nj, nk))
+    got = result.to_numpy().astype(np.float64)
+    assert got.shape == (ni, nj, nk, 2, 2, 2)
+    want = (1.0 + 2.0) * 10 * 11 * float(dtype(1.3))
+    err = np.abs(got - want)
+    assert np.all((err < tol) | (err < want * tol))
+    ref = np.zeros_like(result.padded_numpy())
+    oracle.extended_4d(ref, f.padded_numpy(), jac.padded_numpy(), np.full(16, 10.0), np.full(16, 11.0), ni, nj, nk)
+    assert np.array_equal(result.padded_numpy()[..., :ni], ref[..., :ni])
+
+
+@pytest.mark.parametrize("ni,nj,nk", [(23, 11, 43), (70, 19, 5), (256, 64, 80)])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_extended_4d_parity(gpu, oracle, ni, nj, nk, dtype):
+    from gridtools_amd import stencils
+
+    print(f"[case] extended_4d {np.dtype(dtype).name} {ni}x{nj}x{nk}", flush=True)
+    rng = np.random.default_rng(ni * 1000 + nk)
+    phi = rng.uniform(-1, 1, size=(2, 2, 2, 2))
+    psi = rng.uniform(-1, 1, size=(2, 2, 2, 2))
+    result = gpu.builder().type(dtype).dimensions(ni, nj, nk, 2, 2, 2).build()
+    f = gpu.builder().type(dtype).dimensions(ni, nj, nk, 2, 2, 2).uniform(SEED, 51, -1.0, 1.0).build()
+    jac = gpu.builder().type(dtype).dimensions(ni, nj, nk, 2).uniform(SEED, 52, 0.5, 1.5).build()
+    stencils.extended_4d(result, f, jac, phi, psi, gpu.make_grid(ni, nj, nk))
+    ref = np.zeros_like(result.padded_numpy())
+    oracle.extended_4d(ref, f.padded_numpy(), jac.padded_numpy(), phi, psi, ni, nj, nk)
+    got = result.padded_numpy()
+    assert np.array_equal(got[..., :ni], ref[..., :ni])
+    assert not got[..., ni:].any()  # the i padding is never written

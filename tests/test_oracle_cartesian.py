@@ -192,3 +192,51 @@ def test_prepare_tracers_known_answer(oracle, dtype, tol):
         assert np.all((err <= tol) | (err < abs(want) * tol)), m
         if tol == 0.0:
             assert np.all(dst == want)
+
+
+# ------------------------------------------------------------------------------------------------ extended_4D
+def ext4d_closed_form(f, jac, phi, psi):
+    """The projection as a tensor contraction (numpy einsum, any summation order): result[I,J,K] =
+    1/8 sum_q phi[I,J,K,q] jac[q] sum_abc psi[a,b,c,q] f[a,b,c] per grid point; f / result [c,b,a,k,j,i]."""
+    fa = np.transpose(f, (2, 1, 0, 3, 4, 5)).astype(np.float64)  # [a,b,c,...]
+    s = np.einsum("abcq,abcxyz->qxyz", psi, fa)
+    r = np.einsum("IJKq,qxyz,qxyz->IJKxyz", phi, jac.astype(np.float64), s) / 8
+    return np.transpose(r, (2, 1, 0, 3, 4, 5))
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-14), (np.float32, 1e-6)])
+def test_extended_4d_reference_known_answer(oracle, dtype, tol):
+    """extended_4D.cpp:104-134: phi = 10, psi = 11, f = 1.3, jac(q) = 1 + q; expected (jac0 + jac1) phi psi f at every
+    point and every (I, J, K), at the reference's tolerance (verifier.hpp:25-51)."""
+    ni, nj, nk = 12, 33, 61
+    f_ = dtype(1.3)
+    f = np.full((2, 2, 2, nk, nj, ni), f_, dtype=dtype)
+    jac = np.stack([np.full((nk, nj, ni), 1 + q, dtype=dtype) for q in range(2)])
+    res = np.zeros_like(f)
+    oracle.extended_4d(res, f, jac, np.full(16, 10.0), np.full(16, 11.0), ni, nj, nk)
+    want = (1.0 + 2.0) * 10 * 11 * float(f_)
+    err = np.abs(res.astype(np.float64) - want)
+    assert np.all((err < tol) | (err < want * tol))
+
+
+def test_extended_4d_random_tables_match_contraction(oracle):
+    """Random phi / psi tables and random f / jac: every index of the contraction is exercised, so a transposed
+    table index, a wrong extra-dimension stride or a dropped quadrature point fails."""
+    ni, nj, nk = 9, 7, 5
+    rng = np.random.default_rng(21)
+    f = rng.uniform(-1, 1, size=(2, 2, 2, nk, nj, ni))
+    jac = rng.uniform(0.5, 1.5, size=(2, nk, nj, ni))
+    phi = rng.uniform(-1, 1, size=(2, 2, 2, 2))
+    psi = rng.uniform(-1, 1, size=(2, 2, 2, 2))
+    res = np.zeros_like(f)
+    oracle.extended_4d(res, f, jac, phi, psi, ni, nj, nk)
+    want = ext4d_closed_form(f, jac, phi, psi)
+    assert np.max(np.abs(res - want)) < 1e-14
+    # the tables enter with distinct roles
+    res2 = np.zeros_like(f)
+    oracle.extended_4d(res2, f, jac, psi, phi, ni, nj, nk)
+    assert np.max(np.abs(res2 - want)) > 1e-3
+    # float: sums held in float, products in double
+    res32 = np.zeros(f.shape, np.float32)
+    oracle.extended_4d(res32, f.astype(np.float32), jac.astype(np.float32), phi, psi, ni, nj, nk)
+    assert np.max(np.abs(res32 - ext4d_closed_form(f.astype(np.float32), jac.astype(np.float32), phi, psi))) < 1e-5
