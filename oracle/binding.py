@@ -29,6 +29,8 @@ _SIGS = {
     "gto_vadv_f32": [_P, _P, _P, _P, _P, ctypes.c_float, _I, _I, _I, _PD, _PD, _PD, _PD, _I],
     "gto_vadv_mixed": [_P, _P, _P, _P, _P, ctypes.c_float, _I, _I, _I, _PD, _PD, _PD, _PD, _I],
     "gto_tridiagonal_f64": [_P, _P, _P, _P, _P, _I, _I, _I, _PD, _PD],
+    "gto_hdiff_kfirst_f64": [_P, _P, _P, _I, _I, _I, _PD, _PD, _I],
+    "gto_vadv_kfirst_f64": [_P, _P, _P, _P, _P, ctypes.c_double, _I, _I, _I, _PD, _PD, _PD, _PD, _I],
     "gto_shd_f64": [_P, _P, _P, _P, _PD, _PD, _PD, _P, _I, _I, _I, _PD, _PD],
     "gto_shd_f32": [_P, _P, _P, _P, _PD, _PD, _PD, _P, _I, _I, _I, _PD, _PD],
     "gto_shd_mixed": [_P, _P, _P, _P, _PD, _PD, _PD, _P, _I, _I, _I, _PD, _PD],
@@ -88,11 +90,12 @@ def _kind(arrs):
 
 
 def hdiff(out: View, in_: View, coeff: View, ni, nj, nk, mode="native", threads=1):
-    """mode: native (compute in storage type) or mixed (FP32 storage, FP64 compute)."""
+    """mode: native (compute in storage type), mixed (FP32 storage, FP64 compute) or kfirst (FP64, the cpu_kfirst
+    traversal)."""
     dt = _kind([out, in_, coeff])
     assert out.sj == in_.sj == coeff.sj and out.sk == in_.sk == coeff.sk
     fn = {("float64", "native"): "gto_hdiff_f64", ("float32", "native"): "gto_hdiff_f32",
-          ("float32", "mixed"): "gto_hdiff_mixed"}[(dt.name, mode)]
+          ("float32", "mixed"): "gto_hdiff_mixed", ("float64", "kfirst"): "gto_hdiff_kfirst_f64"}[(dt.name, mode)]
     getattr(lib(), fn)(out.ptr, in_.ptr, coeff.ptr, ni, nj, nk, out.sj, out.sk, threads)
 
 
@@ -107,7 +110,7 @@ def laplacian(out: View, in_: View, ni, nj, nk):
 def vadv(uts: View, us: View, wc: View, up: View, ut: View, dtr, ni, nj, nk, mode="native", threads=1):
     dt = _kind([uts, us, wc, up, ut])
     fn = {("float64", "native"): "gto_vadv_f64", ("float32", "native"): "gto_vadv_f32",
-          ("float32", "mixed"): "gto_vadv_mixed"}[(dt.name, mode)]
+          ("float32", "mixed"): "gto_vadv_mixed", ("float64", "kfirst"): "gto_vadv_kfirst_f64"}[(dt.name, mode)]
     rc = getattr(lib(), fn)(uts.ptr, us.ptr, wc.ptr, up.ptr, ut.ptr, dtr, ni, nj, nk, uts.sj, uts.sk, wc.sj, wc.sk,
                             threads)
     if rc != 0:

@@ -197,6 +197,9 @@ void gto_extended_4d_f32(float *result, const ptrdiff_t rs[3], const float *f, c
     GTO_EXT4D_BODY(float)
 }
 
+/* ---------------------------------- k-first traversals (cpu_kfirst) ---------------------------------- */
+#include "kfirst.inc"
+
 /* ---------------------------------- tridiagonal ----------------------------------
  * tests/regression/tridiagonal.cpp:33-68.  forward_thomas: first_level sup/=diag, rhs/=diag;
  * modify<1,0>: sup = sup/(diag - sup[k-1]*inf), rhs = (rhs - inf*rhs[k-1])/(diag - sup[k-1]*inf).

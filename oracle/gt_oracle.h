@@ -83,6 +83,15 @@ int gto_vadv_mixed(float *utens_stage, const float *u_stage, const float *wcon, 
     const float *utens, float dtr_stage, int ni, int nj, int nk, ptrdiff_t sj, ptrdiff_t sk, ptrdiff_t wsj,
     ptrdiff_t wsk, int nthreads);
 
+/* ---- k-first traversals of the FP64 hdiff and vadv (the loop nest of stencil/cpu_kfirst.hpp:88-151: 8x8 (i, j)
+ * blocks over OpenMP threads, k innermost inside each stage).  Same operations as gto_hdiff_f64 / gto_vadv_f64, so
+ * the same bits; a second CPU baseline. */
+void gto_hdiff_kfirst_f64(double *out, const double *in, const double *coeff, int ni, int nj, int nk, ptrdiff_t sj,
+    ptrdiff_t sk, int nthreads);
+int gto_vadv_kfirst_f64(double *utens_stage, const double *u_stage, const double *wcon, const double *u_pos,
+    const double *utens, double dtr_stage, int ni, int nj, int nk, ptrdiff_t sj, ptrdiff_t sk, ptrdiff_t wsj,
+    ptrdiff_t wsk, int nthreads);
+
 /* ---- tridiagonal Thomas solve: tests/regression/tridiagonal.cpp:33-68 ----
  * inf/diag/sup/rhs/out share one layout; sup and rhs are overwritten like the reference's inout accessors. */
 void gto_tridiagonal_f64(double *out, const double *inf, const double *diag, double *sup, double *rhs, int ni,
