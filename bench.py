@@ -167,7 +167,8 @@ def cartesian_leg(gt, steps, warmup, nsets):
     """The further cartesian regression stencils at 256x256x80 FP64, each timed over cycling field sets (MALL-cold):
     simple_hori_diff (in, coeff read, out written: 24 B/point; the j-only metric is negligible), the expandable_run of
     prepare_tracers over 11 members (rho once + 11 reads + 11 writes: 184 B/point), the five-field copy of
-    copy_stencil_tuple (80 B/point) and the extended_4D projection (144 B/point)."""
+    copy_stencil_tuple (80 B/point), the extended_4D projection (144 B/point) and transform_layout of a storage
+    interior into a k-fastest dense buffer (16 B/point)."""
     from gridtools_amd import stencils
 
     h = 2
@@ -230,6 +231,24 @@ def cartesian_leg(gt, steps, warmup, nsets):
     for s in range(min(warmup, 4)):
         run_ext(s)
     res["extended_4d_fp64"] = stats(_event_time(run_ext, max(10, steps // 2)), 18 * 8)
+    del ext
+    # transform_layout: a 256x256x80 FP64 storage interior (i fastest, halo 3, padded rows) into a k-fastest dense
+    # buffer -- the transpose path (16 B/point)
+    from gridtools_amd import layout
+
+    h3 = 3
+    bs = gt.builder().type(np.float64).dimensions(NI + 2 * h3, NJ + 2 * h3, NK).halos(h3, h3, 0)
+    stores = [bs.uniform(SEED + c, 61, -1.0, 1.0).build() for c in range(4)]
+    dense = [gt.builder().type(np.float64).dimensions(NK, NJ, NI).build() for _ in range(4)]
+
+    def run_lt(s):
+        src, dst = stores[s % 4], dense[s % 4]
+        layout.transform_layout(dst.address(0, 0, 0), src.address(h3, h3, 0), [NI, NJ, NK],
+                                [dst.strides[2], dst.strides[1], 1], list(src.strides))
+
+    for s in range(min(warmup, 4)):
+        run_lt(s)
+    res["transform_layout_fp64_transpose"] = stats(_event_time(run_lt, max(10, steps // 2)), 16)
     return res
 
 
@@ -463,7 +482,8 @@ def main():
     ap.add_argument("--legs", default="tune,fp64,fp32,copy,cartesian,compose,batch,large,cpu",
                     help="comma list of legs: tune (hdiff autotuner first, untimed), fp64 (always run), fp32 "
                          "(precision form per kernel), copy (bandwidth "
-                         "yardstick), cartesian (simple_hori_diff, prepare_tracers x11, copy_stencil_tuple x5, extended_4D), "
+                         "yardstick), cartesian (simple_hori_diff, prepare_tracers x11, copy_stencil_tuple x5, extended_4D, "
+                         "transform_layout), "
                          "compose (precision form on the fused composition, 512x512x80), batch (64 "
                          "uneven domains, persistent launch, graph of 10 timesteps), large (1536x1536x80, 20 "
                          "timesteps with boundary conditions), cpu (CPU baseline)")

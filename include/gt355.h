@@ -174,6 +174,14 @@ int gt355_extended_4d(const double phi[16], const double psi[16], const gt355_fi
     const int64_t result_extra[3], const gt355_field *f, const int64_t f_extra[3], const gt355_field *jac,
     int64_t jac_extra, int ni, int nj, int nk, void *stream);
 
+/* transform_layout (include/gridtools/layout_transformation.hpp:64-75, layout_transformation/gpu.hpp:72-116):
+ * dst[sum_d i_d * dst_strides[d]] = src[sum_d i_d * src_strides[d]] for every index of the ndims-dimensional box
+ * dims (1 <= ndims <= 8), strides in elements of elem_size bytes (1, 2, 4, 8 or 16).  Strides are non-negative and
+ * the destination's index map must be injective (checked); elements outside the index map are not touched.  Both
+ * pointers are device addresses covering every index of the box (the caller's allocation, as in the reference). */
+int gt355_transform_layout(int elem_size, void *dst, const void *src, int ndims, const int64_t *dims,
+    const int64_t *dst_strides, const int64_t *src_strides, void *stream);
+
 /* Strided 3-D copy with element conversion between F64 and F32 (the role of layout_transformation,
  * include/gridtools/layout_transformation/gpu.hpp:72-116, plus a dtype change): dst[i + j*ds1 + k*ds2] =
  * (dst type) src[i + j*ss1 + k*ss2] over lengths[0] x lengths[1] x lengths[2]; i strides must be 1. */

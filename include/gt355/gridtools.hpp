@@ -550,6 +550,20 @@ namespace gt355 {
         }
     } // namespace stencil
 
+    // transform_layout(dst, src, dims, dst_strides, src_strides) (include/gridtools/layout_transformation.hpp:64-75)
+    // for device pointers: dst[sum i_d dst_strides[d]] = src[sum i_d src_strides[d]] over the box dims (any
+    // sequence types of one length, 1 to 8 dimensions), asynchronously on `stream`.
+    template <class T, class Dims, class DstStrides, class SrcStrides>
+    void transform_layout(T *dst, T const *src, Dims const &dims, DstStrides const &dst_strides,
+        SrcStrides const &src_strides, void *stream = nullptr) {
+        std::vector<int64_t> d(std::begin(dims), std::end(dims)), ds(std::begin(dst_strides), std::end(dst_strides)),
+            ss(std::begin(src_strides), std::end(src_strides));
+        if (ds.size() != d.size() || ss.size() != d.size())
+            throw std::invalid_argument("transform_layout: dims and strides differ in length");
+        check(gt355_transform_layout((int)sizeof(T), dst, src, (int)d.size(), d.data(), ds.data(), ss.data(), stream),
+            "transform_layout");
+    }
+
     namespace boundaries {
         // boundary functors (boundaries/zero.hpp:29-55, value.hpp:27-67, copy.hpp:26-47)
         struct zero_boundary {};

@@ -29,6 +29,7 @@ _SIGS = {
     "gto_vadv_f32": [_P, _P, _P, _P, _P, ctypes.c_float, _I, _I, _I, _PD, _PD, _PD, _PD, _I],
     "gto_vadv_mixed": [_P, _P, _P, _P, _P, ctypes.c_float, _I, _I, _I, _PD, _PD, _PD, _PD, _I],
     "gto_tridiagonal_f64": [_P, _P, _P, _P, _P, _I, _I, _I, _PD, _PD],
+    "gto_transform_layout": [_I, _P, _P, _I, _P, _P, _P],
     "gto_hdiff_kfirst_f64": [_P, _P, _P, _I, _I, _I, _PD, _PD, _I],
     "gto_vadv_kfirst_f64": [_P, _P, _P, _P, _P, ctypes.c_double, _I, _I, _I, _PD, _PD, _PD, _PD, _I],
     "gto_shd_f64": [_P, _P, _P, _P, _PD, _PD, _PD, _P, _I, _I, _I, _PD, _PD],
@@ -171,6 +172,19 @@ def extended_4d(result, f, jac, phi, psi, ni, nj, nk):
     fn = "gto_extended_4d_f64" if dt == np.float64 else "gto_extended_4d_f32"
     getattr(lib(), fn)(result.ctypes.data, ext.ctypes.data, f.ctypes.data, ext.ctypes.data, jac.ctypes.data, vol,
                        tphi.ctypes.data, tpsi.ctypes.data, ni, nj, nk, sj, sk)
+
+
+def transform_layout(dst, src, dims, dst_strides, src_strides):
+    """layout_transformation/cpu.hpp:20-55 on numpy buffers: dst / src are C-contiguous arrays of one dtype whose
+    flat element n is index n of the strided layouts (strides in elements)."""
+    if dst.dtype != src.dtype or not dst.flags.c_contiguous or not src.flags.c_contiguous:
+        raise TypeError("dst and src: C-contiguous arrays of one dtype")
+    n = len(dims)
+    d, ds, ss = (np.ascontiguousarray(np.asarray(x, dtype=np.int64)) for x in (dims, dst_strides, src_strides))
+    if ds.size != n or ss.size != n:
+        raise ValueError("dims and strides differ in length")
+    lib().gto_transform_layout(dst.itemsize, dst.ctypes.data, src.ctypes.data, n, d.ctypes.data, ds.ctypes.data,
+                               ss.ctypes.data)
 
 
 def tridiagonal(out: View, inf: View, diag: View, sup: View, rhs: View, ni, nj, nk):

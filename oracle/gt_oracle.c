@@ -200,6 +200,29 @@ void gto_extended_4d_f32(float *result, const ptrdiff_t rs[3], const float *f, c
 /* ---------------------------------- k-first traversals (cpu_kfirst) ---------------------------------- */
 #include "kfirst.inc"
 
+/* ---------------------------------- transform_layout ----------------------------------
+ * layout_transformation/cpu.hpp:20-55 (transform_cpu_loop): every index of the box dims, the first three dimensions
+ * in an (i, j, k) loop nest, the remaining ones as an outer hypercube; elements are moved as raw bytes. */
+void gto_transform_layout(int elem_size, void *dst, const void *src, int ndims, const int64_t *dims,
+    const int64_t *dst_strides, const int64_t *src_strides) {
+    int64_t idx[16] = {0};
+    int64_t total = 1;
+    if (ndims < 1 || ndims > 16)
+        return;
+    for (int d = 0; d < ndims; ++d)
+        total *= dims[d];
+    for (int64_t n = 0; n < total; ++n) {
+        int64_t rem = n, so = 0, dof = 0;
+        for (int d = 0; d < ndims; ++d) {
+            idx[d] = rem % dims[d];
+            rem /= dims[d];
+            so += idx[d] * src_strides[d];
+            dof += idx[d] * dst_strides[d];
+        }
+        memcpy((char *)dst + dof * elem_size, (const char *)src + so * elem_size, (size_t)elem_size);
+    }
+}
+
 /* ---------------------------------- tridiagonal ----------------------------------
  * tests/regression/tridiagonal.cpp:33-68.  forward_thomas: first_level sup/=diag, rhs/=diag;
  * modify<1,0>: sup = sup/(diag - sup[k-1]*inf), rhs = (rhs - inf*rhs[k-1])/(diag - sup[k-1]*inf).

@@ -1,6 +1,6 @@
 /*
  * gt_oracle.h -- CPU restatement of the GridTools hdiff / vadv / laplacian / boundary path and of the
- * cartesian regression stencils simple_hori_diff, prepare_tracers and extended_4D.
+ * cartesian regression stencils simple_hori_diff, prepare_tracers and extended_4D, and of transform_layout.
  *
  * TEST INFRASTRUCTURE ONLY.  This library is the checker: it is loaded by tests/,
  * by __graft_entry__.smoke() and by the cpu_baseline leg of bench.py.  The product
@@ -91,6 +91,11 @@ void gto_hdiff_kfirst_f64(double *out, const double *in, const double *coeff, in
 int gto_vadv_kfirst_f64(double *utens_stage, const double *u_stage, const double *wcon, const double *u_pos,
     const double *utens, double dtr_stage, int ni, int nj, int nk, ptrdiff_t sj, ptrdiff_t sk, ptrdiff_t wsj,
     ptrdiff_t wsk, int nthreads);
+
+/* ---- transform_layout: layout_transformation/cpu.hpp:20-55 -- dst[sum i_d dst_strides[d]] = src[sum i_d
+ * src_strides[d]] over the box dims (ndims <= 16), strides in elements of elem_size bytes. */
+void gto_transform_layout(int elem_size, void *dst, const void *src, int ndims, const int64_t *dims,
+    const int64_t *dst_strides, const int64_t *src_strides);
 
 /* ---- tridiagonal Thomas solve: tests/regression/tridiagonal.cpp:33-68 ----
  * inf/diag/sup/rhs/out share one layout; sup and rhs are overwritten like the reference's inout accessors. */

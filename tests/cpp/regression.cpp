@@ -367,6 +367,31 @@ namespace {
         report("copy_stencil_tuple unrolled x5", ok);
     }
 
+    // layout_transformation.cpp:30-40: a storage's interior into a k-fastest dense device buffer and back into a
+    // second storage (value i + j + k)
+    void layout_transformation() {
+        const int ni = 12, nj = 33, nk = 61;
+        auto b = storage::builder<storage::gpu>.type<double>().dimensions(ni, nj, nk);
+        auto src = b.initializer([](int i, int j, int k) { return i + j + k; }).build();
+        auto dst = b.build();
+        void *dense = nullptr;
+        check(gt355_malloc(sizeof(double) * ni * nj * nk, &dense), "malloc");
+        const std::array<int64_t, 3> kfirst = {(int64_t)nj * nk, nk, 1};
+        transform_layout((double *)dense, src->get_const_target_ptr(), src->lengths(), kfirst, src->strides());
+        transform_layout(dst->get_target_ptr(), (double const *)dense, src->lengths(), dst->strides(), kfirst);
+        std::vector<double> host((size_t)ni * nj * nk);
+        check(gt355_synchronize(), "sync");
+        check(gt355_memcpy_d2h(host.data(), dense, host.size() * sizeof(double)), "d2h");
+        check(gt355_free(dense), "free");
+        auto v = dst->const_host_view();
+        bool ok = true;
+        for (int i = 0; i < ni && ok; ++i)
+            for (int j = 0; j < nj && ok; ++j)
+                for (int k = 0; k < nk && ok; ++k)
+                    ok = v(i, j, k) == double(i + j + k) && host[(size_t)i * nj * nk + j * nk + k] == double(i + j + k);
+        report("layout_transformation storage -> k-fastest -> storage", ok);
+    }
+
     // the batching form: uneven domains in one launch equal laplacian + the composition run per domain (FP64,
     // where the two-launch composition and the batch compute the same bits)
     void batching() {
@@ -488,6 +513,7 @@ int main(int argc, char **argv) {
             prepare_tracers<double>();
             prepare_tracers<float>();
             copy_tuple();
+            layout_transformation();
         }
     } catch (std::exception const &e) {
         std::printf("[FAIL] exception: %s\n", e.what());

@@ -231,3 +231,19 @@ def test_extended_4d_validates_without_a_device():
     assert rc == -1 and "quadrature" in _abi.last_error()
     assert lib.gt355_extended_4d(None, tab, ctypes.byref(sp), ok3, ctypes.byref(sp), ok3, ctypes.byref(sp),
                                  ctypes.c_int64(vol), ni, nj, nk, None) == -1
+
+
+def test_transform_layout_validates_without_a_device():
+    lib = _abi.lib()
+    arr = ctypes.c_int64 * 2
+    # overlapping destination (two dimensions with one stride), bad element size, too many dimensions
+    assert lib.gt355_transform_layout(8, ctypes.c_void_p(0x1000), ctypes.c_void_p(0x2000), 2, arr(4, 5), arr(1, 1),
+                                      arr(1, 4), None) == -1
+    assert "overlap" in _abi.last_error()
+    assert lib.gt355_transform_layout(3, ctypes.c_void_p(0x1000), ctypes.c_void_p(0x2000), 2, arr(4, 5), arr(1, 4),
+                                      arr(1, 4), None) == -1
+    assert lib.gt355_transform_layout(8, ctypes.c_void_p(0x1000), ctypes.c_void_p(0x2000), 9, arr(4, 5), arr(1, 4),
+                                      arr(1, 4), None) == -1
+    # an empty box is a no-op
+    assert lib.gt355_transform_layout(8, ctypes.c_void_p(0x1000), ctypes.c_void_p(0x2000), 2, arr(0, 5), arr(1, 4),
+                                      arr(1, 4), None) == 0
