@@ -1,0 +1,920 @@
+/*
+ * gt355/stencil.hpp -- the generic stencil engine: user stencil specs written against the GridTools cartesian
+ * frontend (accessors with extents, make_param_list, execute_parallel / execute_forward / execute_backward stages,
+ * multi_pass, GT_DECLARE_TMP temporaries, interval-dispatched apply overloads, global_parameter, positional) run on
+ * the device through stencil::run / run_single_stage with the gfx950 backend tag.
+ *
+ * The reference lowers a spec to be_api matrices (stencil/be_api.hpp:51-412) that its gpu backend turns into fused
+ * kernels (stencil/gpu/entry_point.hpp:48-260).  This engine does the same job for specs the library has no
+ * hand-written kernel for:
+ *   * extent analysis: stages are walked backwards and each stage's extent is the union of what later stages read
+ *     of its outputs (the reference's stage extents, frontend/cartesian/... -> be_api::plh_info::extent);
+ *   * temporaries: GT_DECLARE_TMP placeholders get device fields sized by their extents, taken from a cached pool
+ *     (the role of sid::cached_allocator, sid/allocator.hpp:46-56);
+ *   * a parallel multistage launches one kernel per stage over the stage's extended domain, one thread per point
+ *     (each stage completes before the next reads it, as the reference's parallel execution guarantees);
+ *   * a forward / backward multistage launches one kernel in which a thread owns an (i, j) column and walks k in
+ *     order, running every stage at each level (k-caches are plain memory here);
+ *   * apply overloads are dispatched per k-level by interval (level<splitter, offset>, full_interval::modify<>,
+ *     first_level, last_level), like the reference's interval matching.
+ * Stages of a vertical multistage may not read each other's outputs at a horizontal offset (the reference needs
+ * block synchronisation for that; the engine reports it as an error).  Fields are the data_store stores of
+ * gt355/gridtools.hpp; the engine is header-only and must be compiled by hipcc (it launches kernels).
+ */
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <mutex>
+
+#include "gridtools.hpp"
+
+#ifndef GT_FUNCTION
+#define GT_FUNCTION __host__ __device__ __forceinline__
+#endif
+
+namespace gt355 {
+    namespace stencil {
+        // ------------------------------------------------------------------------------------------ extents
+        template <int IMinus = 0, int IPlus = 0, int JMinus = 0, int JPlus = 0, int KMinus = 0, int KPlus = 0>
+        struct extent {
+            static constexpr int iminus = IMinus, iplus = IPlus, jminus = JMinus, jplus = JPlus, kminus = KMinus,
+                                 kplus = KPlus;
+        };
+
+        namespace dim {
+            struct i {};
+            struct j {};
+            struct k {};
+        } // namespace dim
+
+        // ------------------------------------------------------------------------------------------ k intervals
+        // level<splitter, offset>: offset > 0 counts up from the splitter (1 = the splitter's own level), offset < 0
+        // counts down (-1 = the level below the splitter); 0 is not a level (stencil/frontend/axis.hpp:37-72).
+        template <int Splitter, int Offset>
+        struct level {
+            static_assert(Offset != 0, "level offsets are non-zero");
+            static constexpr int splitter = Splitter, offset = Offset;
+        };
+
+        constexpr int shift_level_offset(int o, int d) {
+            int r = o + d;
+            if (o > 0 && r <= 0)
+                r -= 1;
+            else if (o < 0 && r >= 0)
+                r += 1;
+            return r;
+        }
+
+        template <class From, class To>
+        struct interval {
+            using from_t = From;
+            using to_t = To;
+            template <int A, int B>
+            using modify = interval<level<From::splitter, shift_level_offset(From::offset, A)>,
+                level<To::splitter, shift_level_offset(To::offset, B)>>;
+            using first_level = interval<From, From>;
+            using last_level = interval<To, To>;
+        };
+
+        namespace axis_config {
+            template <int L>
+            struct offset_limit {};
+        } // namespace axis_config
+
+        // axis<N> (stencil/frontend/axis.hpp:37-72): N intervals between N + 1 splitters; the engine supports N = 1
+        // (apply overloads may use level offsets up to +-4)
+        template <int N, class = void>
+        struct axis {
+            static_assert(N == 1, "the generic engine supports one k interval (axis<1>)");
+            using full_interval = interval<level<0, 1>, level<1, -1>>;
+            static constexpr int n_intervals = N;
+        };
+
+        namespace engine_ {
+            // k index of a level for a domain of nk levels (splitter 0 at 0, splitter 1 at nk)
+            __host__ __device__ constexpr int level_k(int s, int o, int nk) {
+                return (s == 0 ? 0 : nk) + (o > 0 ? o - 1 : o);
+            }
+            template <class I>
+            __host__ __device__ constexpr bool in_interval(int k, int nk) {
+                return level_k(I::from_t::splitter, I::from_t::offset, nk) <= k &&
+                       k <= level_k(I::to_t::splitter, I::to_t::offset, nk);
+            }
+        } // namespace engine_
+
+        // ------------------------------------------------------------------------------------------ accessors
+        namespace cartesian {
+            template <int I, class Extent = extent<>, int NDims = 3>
+            struct in_accessor {
+                static_assert(NDims == 3, "the generic engine supports three-dimensional accessors");
+                static constexpr int index = I;
+                static constexpr bool is_const = true;
+                using extent_t = Extent;
+                int di = 0, dj = 0, dk = 0;
+                GT_FUNCTION constexpr in_accessor() = default;
+                GT_FUNCTION constexpr in_accessor(int i, int j = 0, int k = 0) : di(i), dj(j), dk(k) {}
+            };
+            template <int I, class Extent = extent<>, int NDims = 3>
+            struct inout_accessor {
+                static_assert(NDims == 3, "the generic engine supports three-dimensional accessors");
+                static constexpr int index = I;
+                static constexpr bool is_const = false;
+                using extent_t = Extent;
+                int di = 0, dj = 0, dk = 0;
+                GT_FUNCTION constexpr inout_accessor() = default;
+                GT_FUNCTION constexpr inout_accessor(int i, int j = 0, int k = 0) : di(i), dj(j), dk(k) {}
+            };
+            template <class... Accs>
+            struct param_list_t {
+                static constexpr int size = sizeof...(Accs);
+            };
+            template <class... Accs>
+            using make_param_list = param_list_t<Accs...>;
+
+            // cache policies are accepted and ignored: the engine keeps k-caches in memory
+            namespace cache_io_policy {
+                struct fill {};
+                struct flush {};
+            } // namespace cache_io_policy
+
+
+            // expressions over accessors (stencil/frontend/cartesian/expressions/expr_base.hpp:27-77): eval(expr)
+            // applies the operator to the evaluated operands, scalars passing through
+            template <class Op, class L, class R>
+            struct expr2 {
+                L l;
+                R r;
+            };
+            template <class Op, class A>
+            struct expr1 {
+                A a;
+            };
+            struct plus_f {
+                template <class A, class B>
+                GT_FUNCTION auto operator()(A const &a, B const &b) const {
+                    return a + b;
+                }
+            };
+            struct minus_f {
+                template <class A, class B>
+                GT_FUNCTION auto operator()(A const &a, B const &b) const {
+                    return a - b;
+                }
+            };
+            struct times_f {
+                template <class A, class B>
+                GT_FUNCTION auto operator()(A const &a, B const &b) const {
+                    return a * b;
+                }
+            };
+            struct divide_f {
+                template <class A, class B>
+                GT_FUNCTION auto operator()(A const &a, B const &b) const {
+                    return a / b;
+                }
+            };
+            struct negate_f {
+                template <class A>
+                GT_FUNCTION auto operator()(A const &a) const {
+                    return -a;
+                }
+            };
+            template <class T>
+            struct is_expr_operand : std::false_type {};
+            template <int I, class E, int N>
+            struct is_expr_operand<in_accessor<I, E, N>> : std::true_type {};
+            template <int I, class E, int N>
+            struct is_expr_operand<inout_accessor<I, E, N>> : std::true_type {};
+            template <class Op, class L, class R>
+            struct is_expr_operand<expr2<Op, L, R>> : std::true_type {};
+            template <class Op, class A>
+            struct is_expr_operand<expr1<Op, A>> : std::true_type {};
+            template <class L, class R>
+            using expr_enable =
+                std::enable_if_t<(is_expr_operand<L>::value || is_expr_operand<R>::value) &&
+                                     (is_expr_operand<L>::value || std::is_arithmetic<L>::value) &&
+                                     (is_expr_operand<R>::value || std::is_arithmetic<R>::value),
+                    int>;
+            template <class L, class R, expr_enable<L, R> = 0>
+            GT_FUNCTION constexpr expr2<plus_f, L, R> operator+(L l, R r) {
+                return {l, r};
+            }
+            template <class L, class R, expr_enable<L, R> = 0>
+            GT_FUNCTION constexpr expr2<minus_f, L, R> operator-(L l, R r) {
+                return {l, r};
+            }
+            template <class L, class R, expr_enable<L, R> = 0>
+            GT_FUNCTION constexpr expr2<times_f, L, R> operator*(L l, R r) {
+                return {l, r};
+            }
+            template <class L, class R, expr_enable<L, R> = 0>
+            GT_FUNCTION constexpr expr2<divide_f, L, R> operator/(L l, R r) {
+                return {l, r};
+            }
+            template <class A, std::enable_if_t<is_expr_operand<A>::value, int> = 0>
+            GT_FUNCTION constexpr expr1<negate_f, A> operator-(A a) {
+                return {a};
+            }
+            namespace expressions {}
+        } // namespace cartesian
+
+        // positional<dim>() (stencil/positional.hpp): the storage index of the evaluation point along one dimension
+        template <class Dim>
+        struct positional {};
+
+        // ------------------------------------------------------------------------------------------ placeholders
+        namespace engine_ {
+            template <class T>
+            struct field_kind {
+                using data_t = T;
+            };
+            template <class T>
+            struct global_kind {
+                using data_t = T;
+            };
+            template <class D>
+            struct positional_kind {
+                using data_t = int;
+                using dim_t = D;
+            };
+            template <int N, class Kind>
+            struct arg_plh {
+                static constexpr int key = N;
+                static constexpr bool is_tmp = false;
+                using kind = Kind;
+            };
+            template <int Id, class T>
+            struct tmp_plh {
+                static constexpr int key = 100000 + Id;
+                static constexpr bool is_tmp = true;
+                using kind = field_kind<T>;
+            };
+
+            struct parallel {};
+            struct forward {};
+            struct backward {};
+
+            template <class F, class... Plhs>
+            struct stage_desc {
+                using functor = F;
+            };
+
+            template <class Exec, class... Stages>
+            struct mss_desc {
+                using exec = Exec;
+                template <class F, class... Plhs>
+                constexpr mss_desc<Exec, Stages..., stage_desc<F, Plhs...>> stage(F, Plhs...) const {
+                    return {};
+                }
+                template <class... Ts>
+                constexpr mss_desc ij_cached(Ts...) const {
+                    return *this;
+                }
+                template <class... Ts>
+                constexpr mss_desc k_cached(Ts...) const {
+                    return *this;
+                }
+            };
+
+            template <class... Msses>
+            struct multi_pass_desc {};
+        } // namespace engine_
+
+        inline constexpr engine_::mss_desc<engine_::parallel> execute_parallel() { return {}; }
+        inline constexpr engine_::mss_desc<engine_::forward> execute_forward() { return {}; }
+        inline constexpr engine_::mss_desc<engine_::backward> execute_backward() { return {}; }
+        template <class... Msses>
+        constexpr engine_::multi_pass_desc<Msses...> multi_pass(Msses...) {
+            return {};
+        }
+
+        namespace engine_ {
+            template <class T, int C, size_t... Is>
+            constexpr auto make_tmps(std::index_sequence<Is...>) {
+                return std::tuple<tmp_plh<C * 64 + (int)Is, T>...>();
+            }
+        } // namespace engine_
+
+#define GT355_NARGS_(_1, _2, _3, _4, _5, _6, _7, _8, _9, _10, _11, _12, _13, _14, _15, _16, N, ...) N
+#define GT355_NARGS(...) GT355_NARGS_(__VA_ARGS__, 16, 15, 14, 13, 12, 11, 10, 9, 8, 7, 6, 5, 4, 3, 2, 1, 0)
+// GT_DECLARE_TMP(type, names...) (stencil/frontend/cartesian/tmp_arg.hpp:23-25): temporaries of one value type
+#define GT_DECLARE_TMP(type, ...)                                                                               \
+    auto [__VA_ARGS__] = ::gt355::stencil::engine_::make_tmps<type, __COUNTER__>(                               \
+        std::make_index_sequence<GT355_NARGS(__VA_ARGS__)>())
+
+        namespace engine_ {
+            // ------------------------------------------------------------------------------------------ views
+            template <class T>
+            struct field_view {
+                T *origin;
+                int64_t s0, s1, s2;
+                __device__ T &at(int i, int j, int k) const { return origin[i * s0 + j * s1 + k * s2]; }
+            };
+            template <class T>
+            struct global_view {
+                T v;
+                __device__ T const &at(int, int, int) const { return v; }
+            };
+            template <class D>
+            struct positional_view {
+                int origin;
+                __device__ int at(int i, int j, int k) const {
+                    return origin + (std::is_same<D, dim::i>::value ? i : std::is_same<D, dim::j>::value ? j : k);
+                }
+            };
+
+            template <class Kind>
+            struct view_of;
+            template <class T>
+            struct view_of<field_kind<T>> {
+                using type = field_view<T>;
+            };
+            template <class T>
+            struct view_of<global_kind<T>> {
+                using type = global_view<T>;
+            };
+            template <class D>
+            struct view_of<positional_kind<D>> {
+                using type = positional_view<D>;
+            };
+
+            // a minimal device tuple
+            template <int I, class T>
+            struct vleaf {
+                T val;
+            };
+            template <class Seq, class... Ts>
+            struct vtuple_impl;
+            template <int... Is, class... Ts>
+            struct vtuple_impl<std::integer_sequence<int, Is...>, Ts...> : vleaf<Is, Ts>... {};
+            template <class... Ts>
+            using vtuple = vtuple_impl<std::make_integer_sequence<int, sizeof...(Ts)>, Ts...>;
+            template <int I, class T>
+            __host__ __device__ T &vget(vleaf<I, T> &l) {
+                return l.val;
+            }
+            template <int I, class T>
+            __host__ __device__ T const &vget(vleaf<I, T> const &l) {
+                return l.val;
+            }
+
+            template <class Views>
+            struct evaluator {
+                Views const &v;
+                int i, j, k;
+                template <int I, class E, int N>
+                __device__ decltype(auto) operator()(cartesian::in_accessor<I, E, N> const &a) const {
+                    return vget<I>(v).at(i + a.di, j + a.dj, k + a.dk);
+                }
+                template <int I, class E, int N>
+                __device__ decltype(auto) operator()(cartesian::inout_accessor<I, E, N> const &a) const {
+                    return vget<I>(v).at(i + a.di, j + a.dj, k + a.dk);
+                }
+                template <class X>
+                __device__ auto operand(X const &x) const {
+                    if constexpr (std::is_arithmetic<X>::value)
+                        return x;
+                    else
+                        return std::decay_t<decltype((*this)(x))>((*this)(x));
+                }
+                template <class Op, class L, class R>
+                __device__ auto operator()(cartesian::expr2<Op, L, R> const &e) const {
+                    return Op()(operand(e.l), operand(e.r));
+                }
+                template <class Op, class A>
+                __device__ auto operator()(cartesian::expr1<Op, A> const &e) const {
+                    return Op()(operand(e.a));
+                }
+            };
+
+            template <class Stage>
+            struct stage_views;
+            template <class F, class... Plhs>
+            struct stage_views<stage_desc<F, Plhs...>> {
+                using type = vtuple<typename view_of<typename Plhs::kind>::type...>;
+            };
+
+            // ------------------------------------------------------------------------------------------ dispatch
+            template <class F, class E, class = void>
+            struct has_plain_apply : std::false_type {};
+            template <class F, class E>
+            struct has_plain_apply<F, E, std::void_t<decltype(F::apply(std::declval<E &>()))>> : std::true_type {};
+            template <class F, class E, class I, class = void>
+            struct has_interval_apply : std::false_type {};
+            template <class F, class E, class I>
+            struct has_interval_apply<F, E, I, std::void_t<decltype(F::apply(std::declval<E &>(), I()))>>
+                : std::true_type {};
+
+            template <class... Ts>
+            struct tlist {};
+            template <class A, class B>
+            struct tcat;
+            template <class... A, class... B>
+            struct tcat<tlist<A...>, tlist<B...>> {
+                using type = tlist<A..., B...>;
+            };
+
+            // candidate intervals: every [level<s1, o1>, level<s2, o2>] with s in {0, 1}, o in +-1..+-4
+            constexpr int cand_offset(int n) { return n < 4 ? n - 4 : n - 3; } // 0..7 -> -4..-1, 1..4
+            template <int N>
+            using cand_level = level<N / 8, cand_offset(N % 8)>;
+            template <class Seq>
+            struct all_candidates;
+            template <int... Ns>
+            struct all_candidates<std::integer_sequence<int, Ns...>> {
+                using type = tlist<interval<cand_level<Ns / 16>, cand_level<Ns % 16>>...>;
+            };
+            using candidates_t = typename all_candidates<std::make_integer_sequence<int, 256>>::type;
+
+            template <class F, class E, class L>
+            struct matching;
+            template <class F, class E>
+            struct matching<F, E, tlist<>> {
+                using type = tlist<>;
+            };
+            template <class F, class E, class I, class... Is>
+            struct matching<F, E, tlist<I, Is...>> {
+                using rest = typename matching<F, E, tlist<Is...>>::type;
+                using type =
+                    std::conditional_t<has_interval_apply<F, E, I>::value, typename tcat<tlist<I>, rest>::type, rest>;
+            };
+
+            // runs F at level k of an nk-level domain: the plain apply everywhere, else the interval overload whose
+            // interval contains k (none: nothing to do at this level)
+            template <class F, class E, class... Is>
+            __device__ void apply_at(E &e, int k, int nk, tlist<Is...>) {
+                if constexpr (has_plain_apply<F, E>::value) {
+                    F::apply(e);
+                } else {
+                    bool done = false;
+                    (void)std::initializer_list<int>{(
+                        (!done && in_interval<Is>(k, nk) ? (F::apply(e, Is()), done = true) : false), 0)...};
+                }
+            }
+            template <class F, class Views>
+            __device__ void run_stage_at(Views const &v, int i, int j, int k, int nk) {
+                using E = evaluator<Views>;
+                E e{v, i, j, k};
+                apply_at<F>(e, k, nk, typename matching<F, E, candidates_t>::type());
+            }
+
+            // ------------------------------------------------------------------------------------------ kernels
+            template <class F, class Views>
+            __global__ void __launch_bounds__(256) parallel_stage_kernel(Views v, int i0, int ni, int j0, int nj, int nk) {
+                const int i = i0 + (int)(blockIdx.x * 64 + (threadIdx.x & 63));
+                const int j = j0 + (int)(blockIdx.y * 4 + (threadIdx.x >> 6));
+                const int k = (int)blockIdx.z;
+                if (i >= i0 + ni || j >= j0 + nj)
+                    return;
+                run_stage_at<F>(v, i, j, k, nk);
+            }
+
+            // one vertical multistage: stage s runs on columns inside its own extent box dom[s] = {i0, i1, j0, j1}
+            template <class Exec, class StageList, class ViewsTuple>
+            struct vertical_runner;
+            template <class Exec, class... Stages, class... Views>
+            struct vertical_runner<Exec, tlist<Stages...>, vtuple<Views...>> {
+                template <int... Ss>
+                __device__ static void level(vtuple<Views...> const &v, const int (*dom)[4], int i, int j, int k, int nk,
+                    std::integer_sequence<int, Ss...>) {
+                    (void)std::initializer_list<int>{(
+                        (i >= dom[Ss][0] && i < dom[Ss][1] && j >= dom[Ss][2] && j < dom[Ss][3]
+                                ? run_stage_at<typename Stages::functor>(vget<Ss>(v), i, j, k, nk)
+                                : void()),
+                        0)...};
+                }
+            };
+            template <int NS>
+            struct domains {
+                int d[NS][4];
+            };
+            template <class Exec, class StageList, class ViewsTuple, int NS>
+            __global__ void __launch_bounds__(256) vertical_kernel(
+                ViewsTuple v, domains<NS> dom, int i0, int ni, int j0, int nj, int nk) {
+                const int i = i0 + (int)(blockIdx.x * 64 + (threadIdx.x & 63));
+                const int j = j0 + (int)(blockIdx.y * 4 + (threadIdx.x >> 6));
+                if (i >= i0 + ni || j >= j0 + nj)
+                    return;
+                using R = vertical_runner<Exec, StageList, ViewsTuple>;
+                if constexpr (std::is_same<Exec, backward>::value) {
+                    for (int k = nk - 1; k >= 0; --k)
+                        R::level(v, dom.d, i, j, k, nk, std::make_integer_sequence<int, NS>());
+                } else {
+                    for (int k = 0; k < nk; ++k)
+                        R::level(v, dom.d, i, j, k, nk, std::make_integer_sequence<int, NS>());
+                }
+            }
+
+            // ------------------------------------------------------------------------------------------ host side
+            struct ext_rt {
+                int imin = 0, imax = 0, jmin = 0, jmax = 0, kmin = 0, kmax = 0;
+                bool set = false;
+                void merge(ext_rt const &o) {
+                    if (!o.set)
+                        return;
+                    if (!set) {
+                        *this = o;
+                        return;
+                    }
+                    imin = std::min(imin, o.imin), imax = std::max(imax, o.imax);
+                    jmin = std::min(jmin, o.jmin), jmax = std::max(jmax, o.jmax);
+                    kmin = std::min(kmin, o.kmin), kmax = std::max(kmax, o.kmax);
+                }
+            };
+            struct access_rt {
+                int key;
+                bool write;
+                ext_rt ext;
+            };
+            struct stage_rt {
+                int mss;
+                bool vertical;
+                std::vector<access_rt> acc;
+                ext_rt ext; // the stage's own extent (horizontal part used)
+            };
+
+            template <class Acc>
+            access_rt make_access(int key) {
+                using E = typename Acc::extent_t;
+                access_rt a{key, !Acc::is_const, {}};
+                a.ext = {E::iminus, E::iplus, E::jminus, E::jplus, E::kminus, E::kplus, true};
+                return a;
+            }
+            template <class F, class... Plhs, class... Accs>
+            std::vector<access_rt> stage_accesses(cartesian::param_list_t<Accs...>) {
+                static_assert(sizeof...(Accs) == sizeof...(Plhs), "a stage binds one placeholder per accessor");
+                constexpr int keys[] = {Plhs::key...};
+                constexpr int idx[] = {Accs::index...};
+                for (int n = 0; n < (int)sizeof...(Accs); ++n)
+                    if (idx[n] != n)
+                        throw std::invalid_argument("param_list accessors must have indices 0, 1, 2, ... in order");
+                return {make_access<Accs>(keys[Accs::index])...};
+            }
+
+            template <class Mss>
+            struct mss_traits;
+            template <class Exec, class... Stages>
+            struct mss_traits<mss_desc<Exec, Stages...>> {
+                using exec = Exec;
+                using stages = tlist<Stages...>;
+                static constexpr int n = sizeof...(Stages);
+            };
+            template <class Spec>
+            struct msses_of {
+                using type = tlist<Spec>;
+            };
+            template <class... Msses>
+            struct msses_of<multi_pass_desc<Msses...>> {
+                using type = tlist<Msses...>;
+            };
+
+            // cached device memory for temporaries, kept for the process like the reference's allocator stash
+            // (sid/allocator.hpp:46-56): blocks go back to the pool after a run's launches are queued, so a later run
+            // on the same stream reuses them in stream order
+            class tmp_pool {
+                std::mutex m_mutex;
+                std::multimap<size_t, void *> m_free;
+
+              public:
+                static tmp_pool &get() {
+                    static tmp_pool p;
+                    return p;
+                }
+                void *take(size_t bytes, size_t &got) {
+                    std::lock_guard<std::mutex> lk(m_mutex);
+                    auto it = m_free.lower_bound(bytes);
+                    if (it != m_free.end() && it->first <= 2 * bytes) {
+                        void *p = it->second;
+                        got = it->first;
+                        m_free.erase(it);
+                        return p;
+                    }
+                    void *p = nullptr;
+                    check(gt355_malloc(bytes, &p), "stencil engine: temporary");
+                    got = bytes;
+                    return p;
+                }
+                void give(size_t bytes, void *p) {
+                    std::lock_guard<std::mutex> lk(m_mutex);
+                    m_free.emplace(bytes, p);
+                }
+            };
+
+            // ------------------------------------------------------------------------------------------ binding
+            template <class A>
+            struct kind_of;
+            template <class T>
+            struct kind_of<std::shared_ptr<storage::data_store<T>>> {
+                using type = field_kind<T>;
+            };
+            template <class T>
+            struct kind_of<global_parameter_t<T>> {
+                using type = global_kind<T>;
+            };
+            template <class D>
+            struct kind_of<positional<D>> {
+                using type = positional_kind<D>;
+            };
+
+            struct field_rt {
+                void *origin;
+                int64_t s[3];
+            };
+            struct run_ctx {
+                std::map<int, field_rt> fields; // user fields and temporaries by placeholder key
+                int origin[3];
+            };
+
+            template <class P, class ArgsTuple>
+            typename view_of<typename P::kind>::type make_view(run_ctx const &c, ArgsTuple const &args) {
+                using K = typename P::kind;
+                if constexpr (std::is_same<typename view_of<K>::type, field_view<typename K::data_t>>::value) {
+                    auto const &f = c.fields.at(P::key);
+                    return {static_cast<typename K::data_t *>(f.origin), f.s[0], f.s[1], f.s[2]};
+                } else if constexpr (std::is_same<K, global_kind<typename K::data_t>>::value) {
+                    return {std::get<P::key>(args).value};
+                } else {
+                    using D = typename K::dim_t;
+                    constexpr int d = std::is_same<D, dim::i>::value ? 0 : std::is_same<D, dim::j>::value ? 1 : 2;
+                    return {c.origin[d]};
+                }
+            }
+
+            template <class Stage>
+            struct stage_plhs;
+            template <class F, class... Plhs>
+            struct stage_plhs<stage_desc<F, Plhs...>> {
+                template <class ArgsTuple>
+                static auto views(run_ctx const &c, ArgsTuple const &args) {
+                    typename stage_views<stage_desc<F, Plhs...>>::type v{};
+                    assign(v, c, args, std::make_integer_sequence<int, sizeof...(Plhs)>());
+                    return v;
+                }
+                template <class V, class ArgsTuple, int... Is>
+                static void assign(V &v, run_ctx const &c, ArgsTuple const &args, std::integer_sequence<int, Is...>) {
+                    ((vget<Is>(v) = make_view<Plhs>(c, args)), ...);
+                }
+                static std::vector<access_rt> accesses() {
+                    return stage_accesses<F, Plhs...>(typename F::param_list());
+                }
+                // allocate each temporary of this stage that has no memory yet
+                static void alloc_tmps(run_ctx &c, std::map<int, ext_rt> const &box, grid const &g,
+                    std::vector<std::pair<size_t, void *>> &taken) {
+                    (alloc_one<Plhs>(c, box, g, taken), ...);
+                }
+                template <class P>
+                static void alloc_one(run_ctx &c, std::map<int, ext_rt> const &box, grid const &g,
+                    std::vector<std::pair<size_t, void *>> &taken) {
+                    if constexpr (P::is_tmp) {
+                        if (c.fields.count(P::key))
+                            return;
+                        using T = typename P::kind::data_t;
+                        ext_rt b = box.count(P::key) ? box.at(P::key) : ext_rt{};
+                        const int hi = std::max(-b.imin, b.imax), hj = std::max(-b.jmin, b.jmax),
+                                  hk = std::max(-b.kmin, b.kmax);
+                        gt355_layout lay{};
+                        const int64_t len[3] = {g.i_size + 2 * hi, g.j_size + 2 * hj, g.k_size + 2 * hk};
+                        const int64_t hal[3] = {hi, hj, hk};
+                        check(gt355_storage_layout(sizeof(T), len, hal, &lay), "stencil engine: temporary layout");
+                        size_t got = 0;
+                        void *raw = tmp_pool::get().take((size_t)lay.alloc_bytes, got);
+                        taken.emplace_back(got, raw);
+                        void *base = nullptr;
+                        check(gt355_storage_base(&lay, raw, &base), "stencil engine: temporary layout");
+                        T *origin = static_cast<T *>(base) + hi + hj * lay.stride[1] + hk * lay.stride[2];
+                        c.fields[P::key] = {origin, {1, lay.stride[1], lay.stride[2]}};
+                    }
+                }
+            };
+
+            inline void check_launch(const char *what) {
+                hipError_t e = hipGetLastError();
+                if (e != hipSuccess)
+                    throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+            }
+
+            inline dim3 grid_2d(int ni, int nj, int nz) {
+                return dim3((unsigned)((ni + 63) / 64), (unsigned)((nj + 3) / 4), (unsigned)nz);
+            }
+
+            template <class Exec, class... Stages, class ArgsTuple>
+            void launch_mss(tlist<Stages...>, std::vector<stage_rt> const &st, int first, run_ctx const &c,
+                ArgsTuple const &args, grid const &g, hipStream_t stream) {
+                constexpr int NS = sizeof...(Stages);
+                const int ni = g.i_size, nj = g.j_size, nk = g.k_size;
+                if constexpr (std::is_same<Exec, parallel>::value) {
+                    int s = first;
+                    auto one = [&](auto stage) {
+                        using S = decltype(stage);
+                        ext_rt const &e = st[s++].ext;
+                        auto v = stage_plhs<S>::views(c, args);
+                        const int w = ni - e.imin + e.imax, h = nj - e.jmin + e.jmax;
+                        if (w > 0 && h > 0)
+                            parallel_stage_kernel<typename S::functor, decltype(v)>
+                                <<<grid_2d(w, h, nk), 256, 0, stream>>>(v, e.imin, w, e.jmin, h, nk);
+                        check_launch("stencil engine: parallel stage launch");
+                    };
+                    (one(Stages()), ...);
+                } else {
+                    using ViewsTuple = vtuple<typename stage_views<Stages>::type...>;
+                    ViewsTuple vt{};
+                    domains<NS> dom{};
+                    ext_rt u;
+                    int s = first, n = 0;
+                    auto one = [&](auto stage) {
+                        using S = decltype(stage);
+                        ext_rt const &e = st[s++].ext;
+                        dom.d[n][0] = e.imin, dom.d[n][1] = ni + e.imax, dom.d[n][2] = e.jmin, dom.d[n][3] = nj + e.jmax;
+                        u.merge(e);
+                        ++n;
+                    };
+                    (one(Stages()), ...);
+                    assign_all(vt, c, args, tlist<Stages...>(), std::make_integer_sequence<int, NS>());
+                    const int w = ni - u.imin + u.imax, h = nj - u.jmin + u.jmax;
+                    if (w > 0 && h > 0 && nk > 0)
+                        vertical_kernel<Exec, tlist<Stages...>, ViewsTuple, NS>
+                            <<<grid_2d(w, h, 1), 256, 0, stream>>>(vt, dom, u.imin, w, u.jmin, h, nk);
+                    check_launch("stencil engine: vertical multistage launch");
+                }
+            }
+            template <class VT, class ArgsTuple, class... Stages, int... Is>
+            void assign_all(VT &vt, run_ctx const &c, ArgsTuple const &args, tlist<Stages...>,
+                std::integer_sequence<int, Is...>) {
+                ((vget<Is>(vt) = stage_plhs<Stages>::views(c, args)), ...);
+            }
+
+            template <class Spec, class PList, class = void>
+            struct spec_result {
+                using type = void;
+            };
+            template <class Spec, class... Ps>
+            struct spec_result<Spec, tlist<Ps...>, std::void_t<decltype(std::declval<Spec>()(Ps()...))>> {
+                using type = decltype(std::declval<Spec>()(Ps()...));
+            };
+            template <class T>
+            struct is_spec : std::false_type {};
+            template <class E, class... S>
+            struct is_spec<mss_desc<E, S...>> : std::true_type {};
+            template <class... M>
+            struct is_spec<multi_pass_desc<M...>> : std::true_type {};
+
+            template <class Spec, class Seq, class... Args>
+            struct plhs_for;
+            template <class Spec, size_t... Is, class... Args>
+            struct plhs_for<Spec, std::index_sequence<Is...>, Args...> {
+                using type = tlist<arg_plh<(int)Is, typename kind_of<std::decay_t<Args>>::type>...>;
+            };
+
+            template <class Spec, class... Args>
+            using spec_of = typename spec_result<Spec,
+                typename plhs_for<Spec, std::index_sequence_for<Args...>, Args...>::type>::type;
+
+            // user-field bookkeeping: bounds check and host -> device synchronisation of every field argument
+            template <int N, class A>
+            void bind_arg(run_ctx &c, A const &a, std::vector<stage_rt> const &st, grid const &g) {
+                if constexpr (std::is_same<typename kind_of<A>::type, field_kind<typename kind_of<A>::type::data_t>>::value) {
+                    bool written = false;
+                    ext_rt need;
+                    for (auto const &s : st)
+                        for (auto const &ac : s.acc)
+                            if (ac.key == N) {
+                                written = written || ac.write;
+                                ext_rt r{s.ext.imin + ac.ext.imin, s.ext.imax + ac.ext.imax, s.ext.jmin + ac.ext.jmin,
+                                    s.ext.jmax + ac.ext.jmax, ac.ext.kmin, ac.ext.kmax, true};
+                                need.merge(r);
+                            }
+                    if (!need.set)
+                        return;
+                    gt355_field f = a->sid(g.origin(), written);
+                    auto bad = [&](int d, int lo, int hi) {
+                        return f.stride[d] != 0 && (lo < f.lower[d] || hi > f.upper[d]);
+                    };
+                    if (bad(0, need.imin, g.i_size + need.imax) || bad(1, need.jmin, g.j_size + need.jmax))
+                        throw std::runtime_error("run: field argument " + std::to_string(N) + " (" + a->name() +
+                                                 ") is too small for the extents the stencil reads or writes");
+                    if (written && (f.stride[0] == 0 || f.stride[1] == 0 || f.stride[2] == 0))
+                        throw std::runtime_error("run: a written field cannot have masked dimensions");
+                    c.fields[N] = {f.origin, {f.stride[0], f.stride[1], f.stride[2]}};
+                }
+            }
+            template <class ArgsTuple, size_t... Is>
+            void bind_args(run_ctx &c, ArgsTuple const &args, std::vector<stage_rt> const &st, grid const &g,
+                std::index_sequence<Is...>) {
+                (bind_arg<(int)Is>(c, std::get<Is>(args), st, g), ...);
+            }
+
+            template <class... Msses>
+            void collect(tlist<Msses...>, std::vector<stage_rt> &st) {
+                int m = 0;
+                auto one_mss = [&](auto mss) {
+                    using T = mss_traits<decltype(mss)>;
+                    const bool vertical = !std::is_same<typename T::exec, parallel>::value;
+                    collect_stages(typename T::stages(), m, vertical, st);
+                    ++m;
+                };
+                (one_mss(Msses()), ...);
+            }
+            template <class... Stages>
+            void collect_stages(tlist<Stages...>, int m, bool vertical, std::vector<stage_rt> &st) {
+                ((st.push_back(stage_rt{m, vertical, stage_plhs<Stages>::accesses(), {}})), ...);
+            }
+
+            // extent analysis: walks the stages backwards; a stage's extent is the union of the extents at which
+            // later stages read its outputs; returns the box every placeholder is touched over (for temporaries)
+            inline std::map<int, ext_rt> analyse(std::vector<stage_rt> &st) {
+                std::map<int, ext_rt> req, box;
+                for (int s = (int)st.size() - 1; s >= 0; --s) {
+                    ext_rt e;
+                    e.set = true;
+                    for (auto const &a : st[s].acc)
+                        if (a.write && req.count(a.key)) {
+                            ext_rt r = req[a.key];
+                            r.kmin = r.kmax = 0;
+                            e.merge(r);
+                        }
+                    st[s].ext = e;
+                    // every access reads (an inout accessor sees the values earlier stages left)
+                    for (auto const &a : st[s].acc) {
+                        ext_rt r{e.imin + a.ext.imin, e.imax + a.ext.imax, e.jmin + a.ext.jmin, e.jmax + a.ext.jmax,
+                            a.ext.kmin, a.ext.kmax, true};
+                        req[a.key].merge(r);
+                        box[a.key].merge(r);
+                    }
+                }
+                // vertical multistages: no stage may read, at a horizontal offset, a field some stage of the same
+                // multistage writes (that would need synchronisation between columns)
+                for (auto const &s : st) {
+                    if (!s.vertical)
+                        continue;
+                    for (auto const &a : s.acc) {
+                        if (!(a.ext.imin || a.ext.imax || a.ext.jmin || a.ext.jmax))
+                            continue;
+                        for (auto const &t : st)
+                            if (t.mss == s.mss)
+                                for (auto const &b : t.acc)
+                                    if (b.write && b.key == a.key)
+                                        throw std::runtime_error(
+                                            "run: a vertical multistage reads, at a horizontal offset, a field it "
+                                            "writes; split it into a parallel multistage");
+                    }
+                }
+                return box;
+            }
+
+            template <class... Msses, class ArgsTuple>
+            void launch_all(tlist<Msses...>, std::vector<stage_rt> const &st, run_ctx const &c, ArgsTuple const &args,
+                grid const &g, hipStream_t stream) {
+                int first = 0;
+                auto one = [&](auto mss) {
+                    using T = mss_traits<decltype(mss)>;
+                    launch_mss<typename T::exec>(typename T::stages(), st, first, c, args, g, stream);
+                    first += T::n;
+                };
+                (one(Msses()), ...);
+            }
+            template <class... Msses>
+            void alloc_all(tlist<Msses...>, run_ctx &c, std::map<int, ext_rt> const &box, grid const &g,
+                std::vector<std::pair<size_t, void *>> &taken) {
+                auto one = [&](auto mss) { alloc_stages(typename mss_traits<decltype(mss)>::stages(), c, box, g, taken); };
+                (one(Msses()), ...);
+            }
+            template <class... Stages>
+            void alloc_stages(tlist<Stages...>, run_ctx &c, std::map<int, ext_rt> const &box, grid const &g,
+                std::vector<std::pair<size_t, void *>> &taken) {
+                (stage_plhs<Stages>::alloc_tmps(c, box, g, taken), ...);
+            }
+        } // namespace engine_
+
+        // run(spec, backend, grid, fields...) (stencil/frontend/run.hpp:201-250) for a spec lambda: the spec is called
+        // with one placeholder per field argument and returns execute_*().stage(...) chains or a multi_pass of them.
+        template <class Spec, class Be, class... Args,
+            std::enable_if_t<engine_::is_spec<engine_::spec_of<Spec, Args...>>::value, int> = 0>
+        void run(Spec, Be be, grid const &g, Args const &... args) {
+            static_assert(Be::precision == GT355_NATIVE, "the generic engine computes in the storage type");
+            using spec_t = engine_::spec_of<Spec, Args...>;
+            using msses = typename engine_::msses_of<spec_t>::type;
+            std::vector<engine_::stage_rt> st;
+            engine_::collect(msses(), st);
+            auto box = engine_::analyse(st);
+            engine_::run_ctx c;
+            c.origin[0] = g.i_start, c.origin[1] = g.j_start, c.origin[2] = 0;
+            auto args_t = std::forward_as_tuple(args...);
+            engine_::bind_args(c, args_t, st, g, std::index_sequence_for<Args...>());
+            if (g.i_size <= 0 || g.j_size <= 0 || g.k_size <= 0)
+                return;
+            std::vector<std::pair<size_t, void *>> taken;
+            engine_::alloc_all(msses(), c, box, g, taken);
+            engine_::launch_all(msses(), st, c, args_t, g, (hipStream_t)be.stream);
+            for (auto &t : taken)
+                engine_::tmp_pool::get().give(t.first, t.second);
+        }
+
+        // run_single_stage(functor, backend, grid, fields...) (stencil/frontend/run.hpp:252-258)
+        template <class F, class Be, class... Args>
+        void run_single_stage(F, Be be, grid const &g, Args const &... args) {
+            run([](auto... p) { return execute_parallel().stage(F(), p...); }, be, g, args...);
+        }
+    } // namespace stencil
+} // namespace gt355

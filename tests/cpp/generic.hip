@@ -1,0 +1,569 @@
+// The generic stencil engine (include/gt355/stencil.hpp) on user specs written like the reference's cartesian
+// regression tests: functors with accessors, extents and interval overloads, multistage specs with temporaries, run
+// on the gfx950 backend tag.  Expected values come from the oracle (oracle/gt_oracle.h) on the same seeded inputs
+// (FP64 must match bit for bit: the functors evaluate the same operations in the same order) or from the reference
+// tests' known answers.
+//
+//   gt355_generic    all cases (needs a GPU)
+#include <gt355/stencil.hpp>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../oracle/gt_oracle.h"
+
+using namespace gt355;
+using namespace gt355::stencil;
+using namespace gt355::stencil::cartesian;
+
+namespace {
+    int failures = 0;
+
+    void report(const char *name, bool ok) {
+        std::printf("[%s] %s\n", ok ? "ok" : "FAIL", name);
+        std::fflush(stdout);
+        if (!ok)
+            ++failures;
+    }
+
+    template <class T>
+    using store = std::shared_ptr<storage::data_store<T>>;
+
+    // seeded store over the whole allocation, the library's fill_uniform sequence
+    template <class T>
+    store<T> seeded(int n0, int n1, int n2, int h, uint64_t seed, uint64_t fid, double lo, double hi) {
+        std::vector<double> v((size_t)n0 * n1 * n2);
+        gto_fill_uniform_f64(v.data(), (int64_t)v.size(), seed, fid, lo, hi);
+        return storage::builder<storage::gpu>.template type<T>().dimensions(n0, n1, n2).halos(h, h, 0)
+            .initializer([v, n0, n1](int i, int j, int k) { return v[i + (size_t)n0 * (j + (size_t)n1 * k)]; })
+            .build();
+    }
+
+    // host copy in the oracle layout [k][j][i padded] (element (0,0,0) at index 0)
+    template <class T>
+    std::vector<T> mirror(store<T> const &ds) {
+        auto v = ds->const_host_view();
+        auto len = ds->lengths();
+        std::vector<T> out((size_t)v.stride[2] * len[2]);
+        for (int k = 0; k < len[2]; ++k)
+            for (int j = 0; j < len[1]; ++j)
+                for (int i = 0; i < len[0]; ++i)
+                    out[i + j * v.stride[1] + k * v.stride[2]] = v(i, j, k);
+        return out;
+    }
+
+    template <class T>
+    bool same_interior(store<T> const &ds, std::vector<T> const &ref, grid const &g) {
+        auto v = ds->const_host_view();
+        for (int k = 0; k < g.k_size; ++k)
+            for (int j = g.j_start; j < g.j_start + g.j_size; ++j)
+                for (int i = g.i_start; i < g.i_start + g.i_size; ++i) {
+                    T a = v(i, j, k), b = ref[i + j * v.stride[1] + k * v.stride[2]];
+                    if (std::memcmp(&a, &b, sizeof(T)) != 0) {
+                        std::printf("  mismatch at (%d,%d,%d): %.17g vs %.17g\n", i, j, k, (double)a, (double)b);
+                        return false;
+                    }
+                }
+        return true;
+    }
+
+    grid halo_grid(int ni, int nj, int nk, int h) {
+        return make_grid(halo_descriptor(h, h, h, ni + h - 1, ni + 2 * h), halo_descriptor(h, h, h, nj + h - 1, nj + 2 * h),
+            nk);
+    }
+
+    // ------------------------------------------------------------------------------------- copy and laplacian
+    struct copy_functor {
+        using in = in_accessor<0>;
+        using out = inout_accessor<1>;
+        using param_list = make_param_list<in, out>;
+        template <class Eval>
+        GT_FUNCTION static void apply(Eval &&eval) {
+            eval(out()) = eval(in());
+        }
+    };
+
+    struct lap_functor {
+        using out = inout_accessor<0>;
+        using in = in_accessor<1, extent<-1, 1, -1, 1>>;
+        using param_list = make_param_list<out, in>;
+        template <class Eval>
+        GT_FUNCTION static void apply(Eval &&eval) {
+            using T = std::decay_t<decltype(eval(out()))>;
+            eval(out()) = T(4) * eval(in()) - (eval(in(1, 0)) + eval(in(0, 1)) + eval(in(-1, 0)) + eval(in(0, -1)));
+        }
+    };
+
+    void copy_and_laplacian() {
+        const int ni = 23, nj = 11, nk = 43, h = 1;
+        auto in = seeded<double>(ni + 2, nj + 2, nk, h, 42, 1, -1, 1);
+        auto b = storage::builder<storage::gpu>.type<double>().dimensions(ni + 2, nj + 2, nk).halos(h, h, 0);
+        auto out = b.build(), lap = b.build();
+        auto g = halo_grid(ni, nj, nk, h);
+        run_single_stage(copy_functor(), gfx950<>(), g, in, out);
+        report("run_single_stage copy", same_interior(out, mirror(in), g));
+        run_single_stage(lap_functor(), gfx950<>(), g, lap, in);
+        auto hin = mirror(in);
+        std::vector<double> ref(hin.size(), 0.);
+        auto st = in->strides();
+        const size_t o = h + h * st[1];
+        gto_lap_f64(ref.data() + o, hin.data() + o, ni, nj, nk, st[1], st[2]);
+        report("run_single_stage laplacian (extent <-1,1,-1,1>)", same_interior(lap, ref, g));
+    }
+
+    // ------------------------------------------------------------------------------------- horizontal diffusion
+    struct hd_lap {
+        using out = inout_accessor<0>;
+        using in = in_accessor<1, extent<-1, 1, -1, 1>>;
+        using param_list = make_param_list<out, in>;
+        template <class Eval>
+        GT_FUNCTION static void apply(Eval eval) {
+            using T = std::decay_t<decltype(eval(out()))>;
+            eval(out()) = T{4} * eval(in()) - (eval(in(1, 0)) + eval(in(0, 1)) + eval(in(-1, 0)) + eval(in(0, -1)));
+        }
+    };
+    struct hd_flx {
+        using out = inout_accessor<0>;
+        using in = in_accessor<1, extent<0, 1, 0, 0>>;
+        using lap = in_accessor<2, extent<0, 1, 0, 0>>;
+        using param_list = make_param_list<out, in, lap>;
+        template <class Eval>
+        GT_FUNCTION static void apply(Eval eval) {
+            auto res = eval(lap(1, 0)) - eval(lap(0, 0));
+            eval(out()) = res * (eval(in(1, 0)) - eval(in(0, 0))) > 0 ? 0 : res;
+        }
+    };
+    struct hd_fly {
+        using out = inout_accessor<0>;
+        using in = in_accessor<1, extent<0, 0, 0, 1>>;
+        using lap = in_accessor<2, extent<0, 0, 0, 1>>;
+        using param_list = make_param_list<out, in, lap>;
+        template <class Eval>
+        GT_FUNCTION static void apply(Eval eval) {
+            auto res = eval(lap(0, 1)) - eval(lap(0, 0));
+            eval(out()) = res * (eval(in(0, 1)) - eval(in(0, 0))) > 0 ? 0 : res;
+        }
+    };
+    struct hd_out {
+        using out = inout_accessor<0>;
+        using in = in_accessor<1>;
+        using flx = in_accessor<2, extent<-1, 0, 0, 0>>;
+        using fly = in_accessor<3, extent<0, 0, -1, 0>>;
+        using coeff = in_accessor<4>;
+        using param_list = make_param_list<out, in, flx, fly, coeff>;
+        template <class Eval>
+        GT_FUNCTION static void apply(Eval eval) {
+            eval(out()) =
+                eval(in()) - eval(coeff()) * (eval(flx()) - eval(flx(-1, 0)) + eval(fly()) - eval(fly(0, -1)));
+        }
+    };
+
+    void horizontal_diffusion(int ni, int nj, int nk) {
+        const int h = 2;
+        auto in = seeded<double>(ni + 4, nj + 4, nk, h, 42, 1, -1, 1);
+        auto coeff = seeded<double>(ni + 4, nj + 4, nk, h, 42, 2, 0, 0.25);
+        auto out = storage::builder<storage::gpu>.type<double>().dimensions(ni + 4, nj + 4, nk).halos(h, h, 0).build();
+        auto g = halo_grid(ni, nj, nk, h);
+        auto spec = [](auto in, auto coeff, auto out) {
+            GT_DECLARE_TMP(double, lap, flx, fly);
+            return execute_parallel()
+                .ij_cached(lap, flx, fly)
+                .stage(hd_lap(), lap, in)
+                .stage(hd_flx(), flx, in, lap)
+                .stage(hd_fly(), fly, in, lap)
+                .stage(hd_out(), out, in, flx, fly, coeff);
+        };
+        run(spec, gfx950<>(), g, in, coeff, out);
+        auto hin = mirror(in), hco = mirror(coeff);
+        std::vector<double> ref(hin.size(), 0.);
+        auto st = in->strides();
+        const size_t o = h + h * st[1];
+        gto_hdiff_f64(ref.data() + o, hin.data() + o, hco.data() + o, ni, nj, nk, st[1], st[2], 4);
+        char name[96];
+        std::snprintf(name, sizeof name, "multistage hdiff with temporaries %dx%dx%d", ni, nj, nk);
+        report(name, same_interior(out, ref, g));
+    }
+
+    // ------------------------------------------------------------------------------------- vertical advection
+    using axis_t = axis<1, axis_config::offset_limit<3>>;
+    using full_t = axis_t::full_interval;
+
+    struct va_forward {
+        using utens_stage = in_accessor<0>;
+        using wcon = in_accessor<1, extent<0, 1, 0, 0, 0, 1>>;
+        using u_stage = in_accessor<2, extent<0, 0, 0, 0, -1, 1>>;
+        using u_pos = in_accessor<3>;
+        using utens = in_accessor<4>;
+        using dtr_stage = in_accessor<5>;
+        using ccol = inout_accessor<6, extent<0, 0, 0, 0, -1, 0>>;
+        using dcol = inout_accessor<7, extent<0, 0, 0, 0, -1, 0>>;
+        using param_list = make_param_list<utens_stage, wcon, u_stage, u_pos, utens, dtr_stage, ccol, dcol>;
+
+        template <class Eval>
+        GT_FUNCTION static auto rhs(Eval &&eval) {
+            return eval(dtr_stage()) * eval(u_pos()) + eval(utens()) + eval(utens_stage());
+        }
+        template <class Eval>
+        GT_FUNCTION static void apply(Eval &&eval, full_t::first_level) {
+            using T = std::decay_t<decltype(eval(ccol()))>;
+            auto gcv = T(.25) * (eval(wcon(1, 0, 1)) + eval(wcon(0, 0, 1)));
+            auto cs = gcv * T(0.5);
+            auto c = gcv * T(0.5);
+            auto b = eval(dtr_stage()) - c;
+            auto correction = -cs * (eval(u_stage(0, 0, 1)) - eval(u_stage()));
+            auto d = rhs(eval) + correction;
+            auto divided = T(1) / b;
+            eval(ccol()) = c * divided;
+            eval(dcol()) = d * divided;
+        }
+        template <class Eval>
+        GT_FUNCTION static void apply(Eval &&eval, full_t::modify<1, -1>) {
+            using T = std::decay_t<decltype(eval(ccol()))>;
+            auto gav = -T(.25) * (eval(wcon(1, 0, 0)) + eval(wcon(0, 0, 0)));
+            auto gcv = T(.25) * (eval(wcon(1, 0, 1)) + eval(wcon(0, 0, 1)));
+            auto as = gav * T(0.5);
+            auto cs = gcv * T(0.5);
+            auto a = gav * T(0.5);
+            auto c = gcv * T(0.5);
+            auto b = eval(dtr_stage()) - a - c;
+            auto correction =
+                -as * (eval(u_stage(0, 0, -1)) - eval(u_stage())) - cs * (eval(u_stage(0, 0, 1)) - eval(u_stage()));
+            auto d = rhs(eval) + correction;
+            auto divided = T(1) / (b - eval(ccol(0, 0, -1)) * a);
+            eval(ccol()) = c * divided;
+            eval(dcol()) = (d - eval(dcol(0, 0, -1)) * a) * divided;
+        }
+        template <class Eval>
+        GT_FUNCTION static void apply(Eval &&eval, full_t::last_level) {
+            using T = std::decay_t<decltype(eval(ccol()))>;
+            auto gav = -T(.25) * (eval(wcon(1, 0, 0)) + eval(wcon()));
+            auto as = gav * T(0.5);
+            auto a = gav * T(0.5);
+            auto b = eval(dtr_stage()) - a;
+            auto correction = -as * (eval(u_stage(0, 0, -1)) - eval(u_stage()));
+            auto d = rhs(eval) + correction;
+            auto divided = T(1) / (b - eval(ccol(0, 0, -1)) * a);
+            eval(dcol()) = (d - eval(dcol(0, 0, -1)) * a) * divided;
+        }
+    };
+
+    struct va_backward {
+        using utens_stage = inout_accessor<0>;
+        using u_pos = in_accessor<1>;
+        using dtr_stage = in_accessor<2>;
+        using ccol = in_accessor<3>;
+        using dcol = in_accessor<4>;
+        using data_col = inout_accessor<5, extent<0, 0, 0, 0, 0, 1>>;
+        using param_list = make_param_list<utens_stage, u_pos, dtr_stage, ccol, dcol, data_col>;
+
+        template <class Eval>
+        GT_FUNCTION static void apply(Eval &&eval, full_t::modify<0, -1>) {
+            auto data = eval(dcol()) - eval(ccol()) * eval(data_col(0, 0, 1));
+            eval(utens_stage()) = eval(dtr_stage()) * (data - eval(u_pos()));
+            eval(data_col()) = data;
+        }
+        template <class Eval>
+        GT_FUNCTION static void apply(Eval &&eval, full_t::last_level) {
+            eval(utens_stage()) = eval(dtr_stage()) * (eval(dcol()) - eval(u_pos()));
+            eval(data_col()) = eval(dcol());
+        }
+    };
+
+    void vertical_advection(int ni, int nj, int nk) {
+        const int h = 3;
+        auto utens_stage = seeded<double>(ni + 6, nj + 6, nk, h, 42, 6, -1, 1);
+        auto u_stage = seeded<double>(ni + 6, nj + 6, nk, h, 42, 3, -1, 1);
+        auto u_pos = seeded<double>(ni + 6, nj + 6, nk, h, 42, 4, -1, 1);
+        auto utens = seeded<double>(ni + 6, nj + 6, nk, h, 42, 5, -1, 1);
+        auto wcon = seeded<double>(ni + 6, nj + 6, nk + 1, h, 42, 7, -0.5, 0.5);
+        auto g = halo_grid(ni, nj, nk, h);
+        auto uts0 = mirror(utens_stage);
+        auto spec = [](auto utens_stage, auto u_stage, auto wcon, auto u_pos, auto utens, auto dtr_stage) {
+            GT_DECLARE_TMP(double, ccol, dcol, data_col);
+            return multi_pass(execute_forward()
+                                  .k_cached(cache_io_policy::flush(), ccol, dcol)
+                                  .k_cached(cache_io_policy::fill(), u_stage)
+                                  .stage(va_forward(), utens_stage, wcon, u_stage, u_pos, utens, dtr_stage, ccol, dcol),
+                execute_backward().k_cached(data_col).stage(
+                    va_backward(), utens_stage, u_pos, dtr_stage, ccol, dcol, data_col));
+        };
+        run(spec, gfx950<>(), g, utens_stage, u_stage, wcon, u_pos, utens, global_parameter(0.15));
+        auto hus = mirror(u_stage), hup = mirror(u_pos), hut = mirror(utens), hw = mirror(wcon);
+        auto st = u_stage->strides(), wst = wcon->strides();
+        const size_t o = h + h * st[1], wo = h + h * wst[1];
+        gto_vadv_f64(uts0.data() + o, hus.data() + o, hw.data() + wo, hup.data() + o, hut.data() + o, 0.15, ni, nj, nk,
+            st[1], st[2], wst[1], wst[2], 4);
+        char name[96];
+        std::snprintf(name, sizeof name, "multi_pass forward/backward vadv with intervals %dx%dx%d", ni, nj, nk);
+        report(name, same_interior(utens_stage, uts0, g));
+    }
+
+    // ------------------------------------------------------------------------------------- tridiagonal (expressions)
+    using full1_t = axis<1>::full_interval;
+    struct thomas_forward {
+        using inf = in_accessor<0>;
+        using diag = in_accessor<1>;
+        using sup = inout_accessor<2, extent<0, 0, 0, 0, -1, 0>>;
+        using rhs = inout_accessor<3, extent<0, 0, 0, 0, -1, 0>>;
+        using param_list = make_param_list<inf, diag, sup, rhs>;
+        template <class Eval>
+        GT_FUNCTION static void apply(Eval eval, full1_t::modify<1, 0>) {
+            eval(sup{}) = eval(sup{} / (diag{} - sup{0, 0, -1} * inf{}));
+            eval(rhs{}) = eval((rhs{} - inf{} * rhs{0, 0, -1}) / (diag{} - sup{0, 0, -1} * inf{}));
+        }
+        template <class Eval>
+        GT_FUNCTION static void apply(Eval eval, full1_t::first_level) {
+            eval(sup{}) = eval(sup{}) / eval(diag{});
+            eval(rhs{}) = eval(rhs{}) / eval(diag{});
+        }
+    };
+    struct thomas_backward {
+        using out = inout_accessor<0, extent<0, 0, 0, 0, 0, 1>>;
+        using sup = in_accessor<1>;
+        using rhs = in_accessor<2>;
+        using param_list = make_param_list<out, sup, rhs>;
+        template <class Eval>
+        GT_FUNCTION static void apply(Eval eval, full1_t::modify<0, -1>) {
+            eval(out{}) = eval(rhs{} - sup{} * out{0, 0, 1});
+        }
+        template <class Eval>
+        GT_FUNCTION static void apply(Eval eval, full1_t::last_level) {
+            eval(out{}) = eval(rhs{});
+        }
+    };
+
+    void tridiagonal(int ni, int nj, int nk, bool known_answer) {
+        auto b = storage::builder<storage::gpu>.type<double>().dimensions(ni, nj, nk);
+        store<double> inf, diag, sup, rhs;
+        if (known_answer) {
+            // tridiagonal.cpp:81-97: -1 / 3 / 1 bands and a right-hand side making the solution 1
+            inf = b.value(-1.).build();
+            diag = b.value(3.).build();
+            sup = b.value(1.).build();
+            rhs = b.initializer([nk](int, int, int k) { return k == 0 ? 4. : k == nk - 1 ? 2. : 3.; }).build();
+        } else {
+            inf = seeded<double>(ni, nj, nk, 0, 7, 11, -1, 1);
+            diag = seeded<double>(ni, nj, nk, 0, 7, 12, 3, 4);
+            sup = seeded<double>(ni, nj, nk, 0, 7, 13, -1, 1);
+            rhs = seeded<double>(ni, nj, nk, 0, 7, 14, -1, 1);
+        }
+        auto out = b.build();
+        auto hinf = mirror(inf), hdiag = mirror(diag), hsup = mirror(sup), hrhs = mirror(rhs);
+        run([](auto inf, auto diag, auto sup, auto rhs, auto out) {
+            return multi_pass(execute_forward().stage(thomas_forward(), inf, diag, sup, rhs),
+                execute_backward().stage(thomas_backward(), out, sup, rhs));
+        },
+            gfx950<>(), make_grid(ni, nj, nk), inf, diag, sup, rhs, out);
+        std::vector<double> ref(hinf.size(), 0.);
+        auto st = inf->strides();
+        gto_tridiagonal_f64(ref.data(), hinf.data(), hdiag.data(), hsup.data(), hrhs.data(), ni, nj, nk, st[1], st[2]);
+        bool ok = same_interior(out, ref, make_grid(ni, nj, nk)) && same_interior(sup, hsup, make_grid(ni, nj, nk)) &&
+                  same_interior(rhs, hrhs, make_grid(ni, nj, nk));
+        if (known_answer) {
+            auto v = out->const_host_view();
+            for (int k = 0; k < nk && ok; ++k)
+                ok = std::fabs(v(ni - 1, nj - 1, k) - 1.) < 1e-14;
+        }
+        char name[96];
+        std::snprintf(name, sizeof name, "tridiagonal (expressions, inout fields) %dx%dx%d%s", ni, nj, nk,
+            known_answer ? " known answer" : "");
+        report(name, ok);
+    }
+
+    // ------------------------------------------------------------------------------------- simple_hori_diff (j-only)
+    struct wlap {
+        using out = inout_accessor<0>;
+        using in = in_accessor<1, extent<-1, 1, -1, 1>>;
+        using crlato = in_accessor<2>;
+        using crlatu = in_accessor<3>;
+        using param_list = make_param_list<out, in, crlato, crlatu>;
+        template <class Eval>
+        GT_FUNCTION static void apply(Eval eval) {
+            using T = std::decay_t<decltype(eval(out()))>;
+            eval(out()) = eval(in(1, 0)) + eval(in(-1, 0)) - T{2} * eval(in()) +
+                          eval(crlato()) * (eval(in(0, 1)) - eval(in())) + eval(crlatu()) * (eval(in(0, -1)) - eval(in()));
+        }
+    };
+    struct divflux {
+        using out = inout_accessor<0>;
+        using in = in_accessor<1>;
+        using lap = in_accessor<2, extent<-1, 1, -1, 1>>;
+        using crlato = in_accessor<3>;
+        using coeff = in_accessor<4>;
+        using param_list = make_param_list<out, in, lap, crlato, coeff>;
+        template <class Eval>
+        GT_FUNCTION static void apply(Eval &eval) {
+            auto fluxx = eval(lap(1, 0)) - eval(lap());
+            auto fluxx_m = eval(lap()) - eval(lap(-1, 0));
+            auto fluxy = eval(crlato()) * (eval(lap(0, 1)) - eval(lap()));
+            auto fluxy_m = eval(crlato()) * (eval(lap()) - eval(lap(0, -1)));
+            eval(out()) = eval(in()) + ((fluxx_m - fluxx) + (fluxy_m - fluxy)) * eval(coeff());
+        }
+    };
+
+    void simple_hori_diff(int ni, int nj, int nk) {
+        const int h = 2;
+        auto in = seeded<double>(ni + 4, nj + 4, nk, h, 42, 21, -1, 1);
+        auto coeff = seeded<double>(ni + 4, nj + 4, nk, h, 42, 22, 0, 0.25);
+        auto jb = storage::builder<storage::gpu>.type<double>().dimensions(ni + 4, nj + 4, nk).halos(h, h, 0)
+                      .selector<0, 1, 0>();
+        std::vector<double> cro(nj + 4), cru(nj + 4);
+        gto_fill_uniform_f64(cro.data(), nj + 4, 42, 23, 0.5, 1.5);
+        gto_fill_uniform_f64(cru.data(), nj + 4, 42, 24, 0.5, 1.5);
+        auto crlato = jb.initializer([cro](int, int j, int) { return cro[j]; }).build();
+        auto crlatu = jb.initializer([cru](int, int j, int) { return cru[j]; }).build();
+        auto out = storage::builder<storage::gpu>.type<double>().dimensions(ni + 4, nj + 4, nk).halos(h, h, 0).build();
+        auto g = halo_grid(ni, nj, nk, h);
+        run([](auto coeff, auto in, auto out, auto crlato, auto crlatu) {
+            GT_DECLARE_TMP(double, lap);
+            return execute_parallel()
+                .ij_cached(lap)
+                .stage(wlap(), lap, in, crlato, crlatu)
+                .stage(divflux(), out, in, lap, crlato, coeff);
+        },
+            gfx950<>(), g, coeff, in, out, crlato, crlatu);
+        auto hin = mirror(in), hco = mirror(coeff);
+        std::vector<double> ref(hin.size(), 0.);
+        auto st = in->strides();
+        const size_t o = h + h * st[1];
+        gto_shd_f64(ref.data() + o, hin.data() + o, cro.data() + h, cru.data() + h, 0, 1, 0, hco.data() + o, ni, nj, nk,
+            st[1], st[2]);
+        char name[96];
+        std::snprintf(name, sizeof name, "simple_hori_diff spec, j-only metric %dx%dx%d", ni, nj, nk);
+        report(name, same_interior(out, ref, g));
+    }
+
+    // ------------------------------------------------------------------------------------- positional, fusion
+    struct positional_functor {
+        using out = inout_accessor<0>;
+        using i_pos = in_accessor<1>;
+        using j_pos = in_accessor<2>;
+        using k_pos = in_accessor<3>;
+        using param_list = make_param_list<out, i_pos, j_pos, k_pos>;
+        template <class Eval>
+        GT_FUNCTION static void apply(Eval &&eval) {
+            eval(out()) = eval(i_pos()) + eval(j_pos()) + eval(k_pos());
+        }
+    };
+
+    struct set_one {
+        using out = inout_accessor<0>;
+        using param_list = make_param_list<out>;
+        template <class Eval>
+        GT_FUNCTION static void apply(Eval &&eval) {
+            eval(out()) = 1;
+        }
+    };
+    struct shift_down {
+        using in = in_accessor<0, extent<0, 0, 0, 0, 0, 1>>;
+        using out = inout_accessor<1>;
+        using param_list = make_param_list<in, out>;
+        template <class Eval>
+        GT_FUNCTION static void apply(Eval &&eval, full1_t::modify<0, -1>) {
+            eval(out()) = eval(in(0, 0, 1));
+        }
+        template <class Eval>
+        GT_FUNCTION static void apply(Eval &&eval, full1_t::last_level) {
+            eval(out()) = eval(in());
+        }
+    };
+
+    void positional_and_fusion() {
+        const int ni = 12, nj = 33, nk = 61, h = 2;
+        auto b = storage::builder<storage::gpu>.type<double>().dimensions(ni + 4, nj + 4, nk).halos(h, h, 0);
+        auto out = b.value(-1.).build();
+        auto g = halo_grid(ni, nj, nk, h);
+        run_single_stage(positional_functor(), gfx950<>(), g, out, positional<dim::i>(), positional<dim::j>(),
+            positional<dim::k>());
+        bool ok = true;
+        {
+            auto v = out->const_host_view();
+            for (int k = 0; k < nk && ok; ++k)
+                for (int j = 0; j < nj + 4 && ok; ++j)
+                    for (int i = 0; i < ni + 4 && ok; ++i) {
+                        const bool inside = i >= h && i < h + ni && j >= h && j < h + nj;
+                        ok = v(i, j, k) == (inside ? double(i + j + k) : -1.);
+                    }
+        }
+        // positional_stencil.cpp:36-44: the positional value is the storage index (grid origin included)
+        report("positional<dim::i/j/k> (storage indices, halo untouched)", ok);
+        // parallel_multistage_fusion.cpp:54-66: a temporary set to 1 by one parallel multistage, copied with a
+        // vertical offset by the next
+        auto out2 = b.value(0.).build();
+        run([](auto out) {
+            GT_DECLARE_TMP(double, tmp);
+            return multi_pass(execute_parallel().stage(set_one(), tmp), execute_parallel().stage(shift_down(), tmp, out));
+        },
+            gfx950<>(), g, out2);
+        ok = true;
+        auto v = out2->const_host_view();
+        for (int k = 0; k < nk && ok; ++k)
+            for (int j = h; j < h + nj && ok; ++j)
+                for (int i = h; i < h + ni && ok; ++i)
+                    ok = v(i, j, k) == 1.;
+        report("multi_pass of parallel multistages through a temporary (k offset, intervals)", ok);
+    }
+
+    // ------------------------------------------------------------------------------------- expandable_run, errors
+    struct scale_functor {
+        using data = inout_accessor<0>;
+        using data_nnow = in_accessor<1>;
+        using rho = in_accessor<2>;
+        using param_list = make_param_list<data, data_nnow, rho>;
+        template <class Eval>
+        GT_FUNCTION static void apply(Eval &&eval) {
+            eval(data()) = eval(rho()) * eval(data_nnow());
+        }
+    };
+
+    void expandable_and_errors() {
+        const int ni = 12, nj = 33, nk = 61;
+        auto b = storage::builder<storage::gpu>.type<double>().dimensions(ni, nj, nk);
+        std::vector<store<double>> in, out;
+        for (int m = 0; m < 5; ++m) {
+            out.push_back(b.build());
+            in.push_back(b.value(double(m)).build());
+        }
+        auto rho = b.value(1.1).build();
+        expandable_run<2>([](auto out, auto in, auto rho) { return execute_parallel().stage(scale_functor(), out, in, rho); },
+            gfx950<>(), make_grid(ni, nj, nk), out, in, rho);
+        bool ok = true;
+        for (int m = 0; m < 5 && ok; ++m) {
+            auto v = out[m]->const_host_view();
+            ok = v(3, 4, 5) == 1.1 * m && v(ni - 1, nj - 1, nk - 1) == 1.1 * m;
+        }
+        report("expandable_run of a user spec x5 (1.1 * i)", ok);
+        // a field too small for the extents the spec reads is rejected before launch
+        bool threw = false;
+        try {
+            auto small = b.build(), res = b.build();
+            run_single_stage(lap_functor(), gfx950<>(), make_grid(ni, nj, nk), res, small);
+        } catch (std::runtime_error const &e) {
+            threw = std::strstr(e.what(), "too small") != nullptr;
+        }
+        report("field bounds check of the extent analysis", threw);
+    }
+} // namespace
+
+int main() {
+    try {
+        check(gt355_init(0), "init");
+        copy_and_laplacian();
+        horizontal_diffusion(23, 11, 43);
+        horizontal_diffusion(64, 64, 60);
+        vertical_advection(23, 11, 43);
+        vertical_advection(70, 9, 80);
+        tridiagonal(12, 33, 6, true);
+        tridiagonal(23, 11, 40, false);
+        simple_hori_diff(12, 33, 61);
+        positional_and_fusion();
+        expandable_and_errors();
+    } catch (std::exception const &e) {
+        std::printf("[FAIL] exception: %s\n", e.what());
+        return 1;
+    }
+    std::printf("%d failure(s)\n", failures);
+    return failures ? 1 : 0;
+}
