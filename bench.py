@@ -252,6 +252,24 @@ def cartesian_leg(gt, steps, warmup, nsets):
     return res
 
 
+def generic_engine_leg(reps=50):
+    """The generic stencil engine (include/gt355/stencil.hpp) on user specs of hdiff (one fused launch, lap / flx / fly
+    ij-cached in LDS) and vadv (forward and backward vertical launches) at 256x256x80 FP64, timed by the C++ program
+    tests/cpp/gt355_generic --bench (hipEvents over back-to-back runs on 4 rotating field sets)."""
+    import subprocess
+
+    exe = os.path.join(ROOT, "tests", "cpp", "gt355_generic")
+    if not os.path.exists(exe):
+        return None
+    res = subprocess.run([exe, "--bench", str(reps)], capture_output=True, text=True, timeout=240)
+    if res.returncode != 0:
+        raise RuntimeError(f"gt355_generic --bench failed: {res.stdout[-500:]} {res.stderr[-500:]}")
+    out = json.loads(res.stdout.strip().splitlines()[-1])
+    for k in ("hdiff_gbs", "vadv_gbs"):
+        out[k.replace("gbs", "hbm_frac")] = round(out[k] / HBM_PEAK_GBS, 4)
+    return out
+
+
 def _event_time(fn, n):
     """Mean ms of n calls of fn() bracketed by one event pair on the null stream (synchronised both sides)."""
     from gridtools_amd.timing import Event, synchronize
@@ -479,11 +497,11 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--sets", type=int, default=4)
-    ap.add_argument("--legs", default="tune,fp64,fp32,copy,cartesian,compose,batch,large,cpu",
+    ap.add_argument("--legs", default="tune,fp64,fp32,copy,cartesian,generic,compose,batch,large,cpu",
                     help="comma list of legs: tune (hdiff autotuner first, untimed), fp64 (always run), fp32 "
                          "(precision form per kernel), copy (bandwidth "
                          "yardstick), cartesian (simple_hori_diff, prepare_tracers x11, copy_stencil_tuple x5, extended_4D, "
-                         "transform_layout), "
+                         "transform_layout), generic (the generic engine on user hdiff / vadv specs), "
                          "compose (precision form on the fused composition, 512x512x80), batch (64 "
                          "uneven domains, persistent launch, graph of 10 timesteps), large (1536x1536x80, 20 "
                          "timesteps with boundary conditions), cpu (CPU baseline)")
@@ -540,6 +558,11 @@ def main():
         print("[bench] cartesian regression stencils leg", file=sys.stderr, flush=True)
         with Range("bench: cartesian regression stencils 256x256x80"):
             cartesian = cartesian_leg(gt, args.steps, args.warmup, args.sets)
+    generic = None
+    if "generic" in legs:
+        print("[bench] generic engine leg", file=sys.stderr, flush=True)
+        with Range("bench: generic stencil engine 256x256x80"):
+            generic = generic_engine_leg()
     compose = None
     if "compose" in legs:
         print("[bench] compose leg", file=sys.stderr, flush=True)
@@ -584,6 +607,7 @@ def main():
         "fp32_form": fp32,
         "copy_fp64": copy,
         "cartesian_stencils": cartesian,
+        "generic_engine": generic,
         "precision_form_composition": compose,
         "batching_form": batch,
         "large_domain": large,

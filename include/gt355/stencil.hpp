@@ -25,6 +25,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
 #include <map>
 #include <mutex>
 
@@ -261,17 +262,24 @@ namespace gt355 {
                 using functor = F;
             };
 
-            template <class Exec, class... Stages>
-            struct mss_desc {
+            template <class... Ts>
+            struct tlist {};
+
+            // Caches: the placeholders named by ij_cached (kept in LDS tiles when the multistage is fused)
+            template <class Exec, class Caches, class... Stages>
+            struct mss_desc;
+            template <class Exec, class... Cs, class... Stages>
+            struct mss_desc<Exec, tlist<Cs...>, Stages...> {
                 using exec = Exec;
                 template <class F, class... Plhs>
-                constexpr mss_desc<Exec, Stages..., stage_desc<F, Plhs...>> stage(F, Plhs...) const {
+                constexpr mss_desc<Exec, tlist<Cs...>, Stages..., stage_desc<F, Plhs...>> stage(F, Plhs...) const {
                     return {};
                 }
                 template <class... Ts>
-                constexpr mss_desc ij_cached(Ts...) const {
-                    return *this;
+                constexpr mss_desc<Exec, tlist<Cs..., Ts...>, Stages...> ij_cached(Ts...) const {
+                    return {};
                 }
+                // k-cache policies and placeholders are accepted; the engine keeps k-caches in memory
                 template <class... Ts>
                 constexpr mss_desc k_cached(Ts...) const {
                     return *this;
@@ -282,9 +290,9 @@ namespace gt355 {
             struct multi_pass_desc {};
         } // namespace engine_
 
-        inline constexpr engine_::mss_desc<engine_::parallel> execute_parallel() { return {}; }
-        inline constexpr engine_::mss_desc<engine_::forward> execute_forward() { return {}; }
-        inline constexpr engine_::mss_desc<engine_::backward> execute_backward() { return {}; }
+        inline constexpr engine_::mss_desc<engine_::parallel, engine_::tlist<>> execute_parallel() { return {}; }
+        inline constexpr engine_::mss_desc<engine_::forward, engine_::tlist<>> execute_forward() { return {}; }
+        inline constexpr engine_::mss_desc<engine_::backward, engine_::tlist<>> execute_backward() { return {}; }
         template <class... Msses>
         constexpr engine_::multi_pass_desc<Msses...> multi_pass(Msses...) {
             return {};
@@ -311,11 +319,13 @@ namespace gt355 {
                 T *origin;
                 int64_t s0, s1, s2;
                 __device__ T &at(int i, int j, int k) const { return origin[i * s0 + j * s1 + k * s2]; }
+                __device__ void rebase(int, int) {}
             };
             template <class T>
             struct global_view {
                 T v;
                 __device__ T const &at(int, int, int) const { return v; }
+                __device__ void rebase(int, int) {}
             };
             template <class D>
             struct positional_view {
@@ -323,6 +333,19 @@ namespace gt355 {
                 __device__ int at(int i, int j, int k) const {
                     return origin + (std::is_same<D, dim::i>::value ? i : std::is_same<D, dim::j>::value ? j : k);
                 }
+                __device__ void rebase(int, int) {}
+            };
+
+            // the dynamic LDS of a fused parallel multistage: its ij-cached temporaries, one tile each
+            extern __shared__ double engine_lds[];
+
+            // an ij-cached temporary: a (TI + rim) x (TJ + rim) tile in LDS, row length w, whose element (0, 0) is grid
+            // point (tile_i0 + imin, tile_j0 + jmin); off is the tile's first element in the LDS buffer
+            template <class T>
+            struct lds_view {
+                int off, w, imin, jmin, base;
+                __device__ void rebase(int ti0, int tj0) { base = off - (ti0 + imin) - (tj0 + jmin) * w; }
+                __device__ T &at(int i, int j, int) const { return reinterpret_cast<T *>(engine_lds)[base + i + j * w]; }
             };
 
             template <class Kind>
@@ -389,12 +412,27 @@ namespace gt355 {
                 }
             };
 
-            template <class Stage>
+            template <class P, class L>
+            struct in_list : std::false_type {};
+            template <class P, class... Ls>
+            struct in_list<P, tlist<Ls...>> : std::bool_constant<(std::is_same<P, Ls>::value || ...)> {};
+
+            // the view a stage gets for placeholder P: an LDS tile if P is ij-cached by a fused multistage
+            template <class P, class Caches>
+            using view_for = std::conditional_t<in_list<P, Caches>::value, lds_view<typename P::kind::data_t>,
+                typename view_of<typename P::kind>::type>;
+
+            template <class Stage, class Caches = tlist<>>
             struct stage_views;
-            template <class F, class... Plhs>
-            struct stage_views<stage_desc<F, Plhs...>> {
-                using type = vtuple<typename view_of<typename Plhs::kind>::type...>;
+            template <class F, class... Plhs, class Caches>
+            struct stage_views<stage_desc<F, Plhs...>, Caches> {
+                using type = vtuple<view_for<Plhs, Caches>...>;
             };
+
+            template <int... Is, class... Vs>
+            __device__ void rebase_views(vtuple_impl<std::integer_sequence<int, Is...>, Vs...> &t, int ti0, int tj0) {
+                (vget<Is>(t).rebase(ti0, tj0), ...);
+            }
 
             // ------------------------------------------------------------------------------------------ dispatch
             template <class F, class E, class = void>
@@ -407,8 +445,6 @@ namespace gt355 {
             struct has_interval_apply<F, E, I, std::void_t<decltype(F::apply(std::declval<E &>(), I()))>>
                 : std::true_type {};
 
-            template <class... Ts>
-            struct tlist {};
             template <class A, class B>
             struct tcat;
             template <class... A, class... B>
@@ -507,6 +543,42 @@ namespace gt355 {
                 }
             }
 
+            // one fused parallel multistage: a workgroup owns a TI x TJ tile of one k-level and runs every stage over the
+            // tile extended by the stage's extent (clamped to the stage's domain), with a barrier between stages; the
+            // ij-cached temporaries live in LDS, so stage s+1 reads what stage s left in the tile
+            template <class F, class V>
+            __device__ void stage_tile(V const &v, const int *e, int ti0, int tj0, int TI, int TJ, int ni, int nj, int k,
+                int nk) {
+                const int i_lo = ti0 + e[0], i_hi = min(ti0 + TI + e[1], ni + e[1]);
+                const int j_lo = tj0 + e[2], j_hi = min(tj0 + TJ + e[3], nj + e[3]);
+                // 64 lanes along i (coalesced rows), the 4 wavefronts of the workgroup along j
+                const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+                for (int j = j_lo + wave; j < j_hi; j += 4)
+                    for (int i = i_lo + lane; i < i_hi; i += 64)
+                        run_stage_at<F>(v, i, j, k, nk);
+                __syncthreads();
+            }
+            template <class StageList, class ViewsTuple>
+            struct fused_runner;
+            template <class... Stages, class... Views>
+            struct fused_runner<tlist<Stages...>, vtuple<Views...>> {
+                template <int... Ss>
+                __device__ static void run(vtuple<Views...> &v, const int (*ext)[4], int ti0, int tj0, int TI, int TJ,
+                    int ni, int nj, int k, int nk, std::integer_sequence<int, Ss...>) {
+                    (rebase_views(vget<Ss>(v), ti0, tj0), ...);
+                    (stage_tile<typename Stages::functor>(vget<Ss>(v), ext[Ss], ti0, tj0, TI, TJ, ni, nj, k, nk), ...);
+                }
+            };
+            template <class StageList, class ViewsTuple, int NS>
+            __global__ void __launch_bounds__(256) fused_kernel(
+                ViewsTuple v0, domains<NS> ext, int ni, int nj, int nk, int tiles_i, int TI, int TJ) {
+                const int tile = (int)blockIdx.x;
+                const int ti0 = (tile % tiles_i) * TI, tj0 = (tile / tiles_i) * TJ;
+                ViewsTuple v = v0;
+                fused_runner<StageList, ViewsTuple>::run(
+                    v, ext.d, ti0, tj0, TI, TJ, ni, nj, (int)blockIdx.z, nk, std::make_integer_sequence<int, NS>());
+            }
+
             // ------------------------------------------------------------------------------------------ host side
             struct ext_rt {
                 int imin = 0, imax = 0, jmin = 0, jmax = 0, kmin = 0, kmax = 0;
@@ -527,6 +599,7 @@ namespace gt355 {
                 int key;
                 bool write;
                 ext_rt ext;
+                int elem; // element size of the placeholder's values
             };
             struct stage_rt {
                 int mss;
@@ -536,9 +609,9 @@ namespace gt355 {
             };
 
             template <class Acc>
-            access_rt make_access(int key) {
+            access_rt make_access(int key, int elem) {
                 using E = typename Acc::extent_t;
-                access_rt a{key, !Acc::is_const, {}};
+                access_rt a{key, !Acc::is_const, {}, elem};
                 a.ext = {E::iminus, E::iplus, E::jminus, E::jplus, E::kminus, E::kplus, true};
                 return a;
             }
@@ -546,20 +619,25 @@ namespace gt355 {
             std::vector<access_rt> stage_accesses(cartesian::param_list_t<Accs...>) {
                 static_assert(sizeof...(Accs) == sizeof...(Plhs), "a stage binds one placeholder per accessor");
                 constexpr int keys[] = {Plhs::key...};
+                constexpr int elems[] = {(int)sizeof(typename Plhs::kind::data_t)...};
                 constexpr int idx[] = {Accs::index...};
                 for (int n = 0; n < (int)sizeof...(Accs); ++n)
                     if (idx[n] != n)
                         throw std::invalid_argument("param_list accessors must have indices 0, 1, 2, ... in order");
-                return {make_access<Accs>(keys[Accs::index])...};
+                return {make_access<Accs>(keys[Accs::index], elems[Accs::index])...};
             }
 
             template <class Mss>
             struct mss_traits;
-            template <class Exec, class... Stages>
-            struct mss_traits<mss_desc<Exec, Stages...>> {
+            template <class Exec, class Caches, class... Stages>
+            struct mss_traits<mss_desc<Exec, Caches, Stages...>> {
                 using exec = Exec;
+                using caches = Caches;
                 using stages = tlist<Stages...>;
                 static constexpr int n = sizeof...(Stages);
+                // a parallel multistage of several stages, or with ij-caches, runs as one fused launch
+                static constexpr bool fused =
+                    std::is_same<Exec, parallel>::value && (sizeof...(Stages) > 1 || !std::is_same<Caches, tlist<>>::value);
             };
             template <class Spec>
             struct msses_of {
@@ -622,15 +700,27 @@ namespace gt355 {
                 void *origin;
                 int64_t s[3];
             };
+            struct lds_rt {
+                int off, w, imin, jmin; // first element (in elements of the value type), row length, tile origin
+            };
+            struct fuse_rt {
+                int ti = 64, tj = 8;
+                size_t bytes = 0;
+            };
             struct run_ctx {
-                std::map<int, field_rt> fields; // user fields and temporaries by placeholder key
+                std::map<int, field_rt> fields; // user fields and HBM temporaries by placeholder key
+                std::map<int, lds_rt> lds;      // ij-cached temporaries of fused multistages
+                std::vector<fuse_rt> fuse;      // tile and LDS size per multistage
                 int origin[3];
             };
 
-            template <class P, class ArgsTuple>
-            typename view_of<typename P::kind>::type make_view(run_ctx const &c, ArgsTuple const &args) {
+            template <class P, class Caches, class ArgsTuple>
+            view_for<P, Caches> make_view(run_ctx const &c, ArgsTuple const &args) {
                 using K = typename P::kind;
-                if constexpr (std::is_same<typename view_of<K>::type, field_view<typename K::data_t>>::value) {
+                if constexpr (in_list<P, Caches>::value) {
+                    auto const &l = c.lds.at(P::key);
+                    return {l.off, l.w, l.imin, l.jmin, 0};
+                } else if constexpr (std::is_same<typename view_of<K>::type, field_view<typename K::data_t>>::value) {
                     auto const &f = c.fields.at(P::key);
                     return {static_cast<typename K::data_t *>(f.origin), f.s[0], f.s[1], f.s[2]};
                 } else if constexpr (std::is_same<K, global_kind<typename K::data_t>>::value) {
@@ -646,15 +736,15 @@ namespace gt355 {
             struct stage_plhs;
             template <class F, class... Plhs>
             struct stage_plhs<stage_desc<F, Plhs...>> {
-                template <class ArgsTuple>
+                template <class Caches = tlist<>, class ArgsTuple>
                 static auto views(run_ctx const &c, ArgsTuple const &args) {
-                    typename stage_views<stage_desc<F, Plhs...>>::type v{};
-                    assign(v, c, args, std::make_integer_sequence<int, sizeof...(Plhs)>());
+                    typename stage_views<stage_desc<F, Plhs...>, Caches>::type v{};
+                    assign<Caches>(v, c, args, std::make_integer_sequence<int, sizeof...(Plhs)>());
                     return v;
                 }
-                template <class V, class ArgsTuple, int... Is>
+                template <class Caches, class V, class ArgsTuple, int... Is>
                 static void assign(V &v, run_ctx const &c, ArgsTuple const &args, std::integer_sequence<int, Is...>) {
-                    ((vget<Is>(v) = make_view<Plhs>(c, args)), ...);
+                    ((vget<Is>(v) = make_view<Plhs, Caches>(c, args)), ...);
                 }
                 static std::vector<access_rt> accesses() {
                     return stage_accesses<F, Plhs...>(typename F::param_list());
@@ -668,7 +758,7 @@ namespace gt355 {
                 static void alloc_one(run_ctx &c, std::map<int, ext_rt> const &box, grid const &g,
                     std::vector<std::pair<size_t, void *>> &taken) {
                     if constexpr (P::is_tmp) {
-                        if (c.fields.count(P::key))
+                        if (c.fields.count(P::key) || c.lds.count(P::key))
                             return;
                         using T = typename P::kind::data_t;
                         ext_rt b = box.count(P::key) ? box.at(P::key) : ext_rt{};
@@ -699,12 +789,30 @@ namespace gt355 {
                 return dim3((unsigned)((ni + 63) / 64), (unsigned)((nj + 3) / 4), (unsigned)nz);
             }
 
-            template <class Exec, class... Stages, class ArgsTuple>
-            void launch_mss(tlist<Stages...>, std::vector<stage_rt> const &st, int first, run_ctx const &c,
+            template <class Mss, class... Stages, class ArgsTuple>
+            void launch_mss(tlist<Stages...>, std::vector<stage_rt> const &st, int first, int m, run_ctx const &c,
                 ArgsTuple const &args, grid const &g, hipStream_t stream) {
+                using T = mss_traits<Mss>;
+                using Exec = typename T::exec;
                 constexpr int NS = sizeof...(Stages);
                 const int ni = g.i_size, nj = g.j_size, nk = g.k_size;
-                if constexpr (std::is_same<Exec, parallel>::value) {
+                if constexpr (T::fused) {
+                    using Caches = typename T::caches;
+                    using ViewsTuple = vtuple<typename stage_views<Stages, Caches>::type...>;
+                    ViewsTuple vt{};
+                    domains<NS> ext{};
+                    for (int n = 0; n < NS; ++n) {
+                        ext_rt const &e = st[first + n].ext;
+                        ext.d[n][0] = e.imin, ext.d[n][1] = e.imax, ext.d[n][2] = e.jmin, ext.d[n][3] = e.jmax;
+                    }
+                    assign_all<Caches>(vt, c, args, tlist<Stages...>(), std::make_integer_sequence<int, NS>());
+                    fuse_rt const &f = c.fuse[m];
+                    const int tiles_i = (ni + f.ti - 1) / f.ti, tiles_j = (nj + f.tj - 1) / f.tj;
+                    fused_kernel<tlist<Stages...>, ViewsTuple, NS>
+                        <<<dim3((unsigned)(tiles_i * tiles_j), 1, (unsigned)nk), 256, f.bytes, stream>>>(
+                            vt, ext, ni, nj, nk, tiles_i, f.ti, f.tj);
+                    check_launch("stencil engine: fused multistage launch");
+                } else if constexpr (std::is_same<Exec, parallel>::value) {
                     int s = first;
                     auto one = [&](auto stage) {
                         using S = decltype(stage);
@@ -722,16 +830,12 @@ namespace gt355 {
                     ViewsTuple vt{};
                     domains<NS> dom{};
                     ext_rt u;
-                    int s = first, n = 0;
-                    auto one = [&](auto stage) {
-                        using S = decltype(stage);
-                        ext_rt const &e = st[s++].ext;
+                    for (int n = 0; n < NS; ++n) {
+                        ext_rt const &e = st[first + n].ext;
                         dom.d[n][0] = e.imin, dom.d[n][1] = ni + e.imax, dom.d[n][2] = e.jmin, dom.d[n][3] = nj + e.jmax;
                         u.merge(e);
-                        ++n;
-                    };
-                    (one(Stages()), ...);
-                    assign_all(vt, c, args, tlist<Stages...>(), std::make_integer_sequence<int, NS>());
+                    }
+                    assign_all<tlist<>>(vt, c, args, tlist<Stages...>(), std::make_integer_sequence<int, NS>());
                     const int w = ni - u.imin + u.imax, h = nj - u.jmin + u.jmax;
                     if (w > 0 && h > 0 && nk > 0)
                         vertical_kernel<Exec, tlist<Stages...>, ViewsTuple, NS>
@@ -739,10 +843,84 @@ namespace gt355 {
                     check_launch("stencil engine: vertical multistage launch");
                 }
             }
-            template <class VT, class ArgsTuple, class... Stages, int... Is>
+            template <class Caches, class VT, class ArgsTuple, class... Stages, int... Is>
             void assign_all(VT &vt, run_ctx const &c, ArgsTuple const &args, tlist<Stages...>,
                 std::integer_sequence<int, Is...>) {
-                ((vget<Is>(vt) = stage_plhs<Stages>::views(c, args)), ...);
+                ((vget<Is>(vt) = stage_plhs<Stages>::template views<Caches>(c, args)), ...);
+            }
+
+            // LDS plan of a fused multistage: each ij-cached temporary gets a tile of the workgroup's TI x TJ block
+            // extended by the temporary's extent; TJ shrinks until the tiles fit 64 KB of LDS
+            template <class... Cs>
+            void plan_lds(tlist<Cs...>, int m, std::vector<stage_rt> const &st, std::map<int, ext_rt> const &box,
+                run_ctx &c) {
+                const int keys[] = {0, Cs::key...};
+                const int elems[] = {0, (int)sizeof(typename Cs::kind::data_t)...};
+                constexpr bool all_tmp = (true && ... && Cs::is_tmp);
+                static_assert(all_tmp, "ij_cached takes temporaries (GT_DECLARE_TMP)");
+                for (int q = 1; q <= (int)sizeof...(Cs); ++q)
+                    for (auto const &s : st)
+                        for (auto const &a : s.acc)
+                            if (a.key == keys[q] && (s.mss != m || a.ext.kmin || a.ext.kmax))
+                                throw std::runtime_error("run: an ij_cached temporary must be used only inside its "
+                                                         "multistage and without k offsets");
+                // workgroups of one fused launch run concurrently: a field that lives in memory (not in LDS) and is
+                // read at a horizontal offset must not be written by that stage or a later one of the multistage
+                for (auto const &r : st) {
+                    if (r.mss != m)
+                        continue;
+                    for (auto const &a : r.acc) {
+                        bool cached = false;
+                        for (int q = 1; q <= (int)sizeof...(Cs); ++q)
+                            cached = cached || keys[q] == a.key;
+                        if (cached || !(a.ext.imin || a.ext.imax || a.ext.jmin || a.ext.jmax))
+                            continue;
+                        for (auto const &w : st)
+                            if (w.mss == m && &w >= &r)
+                                for (auto const &b : w.acc)
+                                    if (b.write && b.key == a.key)
+                                        throw std::runtime_error("run: a stage reads a field at a horizontal offset "
+                                                                 "that it or a later stage of the multistage writes");
+                    }
+                }
+                fuse_rt f;
+                // tile rows per workgroup: GT355_ENGINE_TJ (a power of two up to 32) or 16, halved until the tiles fit
+                const char *env = std::getenv("GT355_ENGINE_TJ");
+                const int tj0 = env ? std::max(1, std::min(32, std::atoi(env))) : 16;
+                for (f.tj = tj0;; f.tj /= 2) {
+                    size_t bytes = 0;
+                    std::map<int, lds_rt> lds;
+                    for (int q = 1; q <= (int)sizeof...(Cs); ++q) {
+                        if (!box.count(keys[q]))
+                            continue;
+                        ext_rt const &b = box.at(keys[q]);
+                        const int w = f.ti + b.imax - b.imin, h = f.tj + b.jmax - b.jmin;
+                        bytes = (bytes + 15) / 16 * 16;
+                        lds[keys[q]] = {(int)(bytes / elems[q]), w, b.imin, b.jmin};
+                        bytes += (size_t)w * h * elems[q];
+                    }
+                    if (bytes <= 65536 || f.tj == 1) {
+                        if (bytes > 65536)
+                            throw std::runtime_error("run: the ij-cached temporaries of a multistage exceed 64 KB of LDS");
+                        f.bytes = bytes;
+                        for (auto &l : lds)
+                            c.lds[l.first] = l.second;
+                        break;
+                    }
+                }
+                c.fuse[m] = f;
+            }
+            template <class... Msses>
+            void plan_all(tlist<Msses...>, std::vector<stage_rt> const &st, std::map<int, ext_rt> const &box, run_ctx &c) {
+                c.fuse.assign(sizeof...(Msses), fuse_rt{});
+                int m = 0;
+                auto one = [&](auto mss) {
+                    using T = mss_traits<decltype(mss)>;
+                    if constexpr (T::fused)
+                        plan_lds(typename T::caches(), m, st, box, c);
+                    ++m;
+                };
+                (one(Msses()), ...);
             }
 
             template <class Spec, class PList, class = void>
@@ -755,8 +933,8 @@ namespace gt355 {
             };
             template <class T>
             struct is_spec : std::false_type {};
-            template <class E, class... S>
-            struct is_spec<mss_desc<E, S...>> : std::true_type {};
+            template <class E, class C, class... S>
+            struct is_spec<mss_desc<E, C, S...>> : std::true_type {};
             template <class... M>
             struct is_spec<multi_pass_desc<M...>> : std::true_type {};
 
@@ -866,10 +1044,10 @@ namespace gt355 {
             template <class... Msses, class ArgsTuple>
             void launch_all(tlist<Msses...>, std::vector<stage_rt> const &st, run_ctx const &c, ArgsTuple const &args,
                 grid const &g, hipStream_t stream) {
-                int first = 0;
+                int first = 0, m = 0;
                 auto one = [&](auto mss) {
                     using T = mss_traits<decltype(mss)>;
-                    launch_mss<typename T::exec>(typename T::stages(), st, first, c, args, g, stream);
+                    launch_mss<decltype(mss)>(typename T::stages(), st, first, m++, c, args, g, stream);
                     first += T::n;
                 };
                 (one(Msses()), ...);
@@ -904,6 +1082,7 @@ namespace gt355 {
             engine_::bind_args(c, args_t, st, g, std::index_sequence_for<Args...>());
             if (g.i_size <= 0 || g.j_size <= 0 || g.k_size <= 0)
                 return;
+            engine_::plan_all(msses(), st, box, c);
             std::vector<std::pair<size_t, void *>> taken;
             engine_::alloc_all(msses(), c, box, g, taken);
             engine_::launch_all(msses(), st, c, args_t, g, (hipStream_t)be.stream);

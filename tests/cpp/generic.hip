@@ -4,11 +4,13 @@
 // (FP64 must match bit for bit: the functors evaluate the same operations in the same order) or from the reference
 // tests' known answers.
 //
-//   gt355_generic    all cases (needs a GPU)
+//   gt355_generic            all cases (needs a GPU)
+//   gt355_generic --bench N  times the engine's hdiff and vadv specs at 256x256x80 FP64 (one JSON line)
 #include <gt355/stencil.hpp>
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -160,29 +162,41 @@ namespace {
         }
     };
 
-    void horizontal_diffusion(int ni, int nj, int nk) {
+    void horizontal_diffusion(int ni, int nj, int nk, bool cached = true) {
         const int h = 2;
         auto in = seeded<double>(ni + 4, nj + 4, nk, h, 42, 1, -1, 1);
         auto coeff = seeded<double>(ni + 4, nj + 4, nk, h, 42, 2, 0, 0.25);
         auto out = storage::builder<storage::gpu>.type<double>().dimensions(ni + 4, nj + 4, nk).halos(h, h, 0).build();
         auto g = halo_grid(ni, nj, nk, h);
-        auto spec = [](auto in, auto coeff, auto out) {
-            GT_DECLARE_TMP(double, lap, flx, fly);
-            return execute_parallel()
-                .ij_cached(lap, flx, fly)
-                .stage(hd_lap(), lap, in)
-                .stage(hd_flx(), flx, in, lap)
-                .stage(hd_fly(), fly, in, lap)
-                .stage(hd_out(), out, in, flx, fly, coeff);
-        };
-        run(spec, gfx950<>(), g, in, coeff, out);
+        if (cached)
+            run([](auto in, auto coeff, auto out) {
+                GT_DECLARE_TMP(double, lap, flx, fly);
+                return execute_parallel()
+                    .ij_cached(lap, flx, fly)
+                    .stage(hd_lap(), lap, in)
+                    .stage(hd_flx(), flx, in, lap)
+                    .stage(hd_fly(), fly, in, lap)
+                    .stage(hd_out(), out, in, flx, fly, coeff);
+            },
+                gfx950<>(), g, in, coeff, out);
+        else
+            run([](auto in, auto coeff, auto out) {
+                GT_DECLARE_TMP(double, lap, flx, fly);
+                return execute_parallel()
+                    .stage(hd_lap(), lap, in)
+                    .stage(hd_flx(), flx, in, lap)
+                    .stage(hd_fly(), fly, in, lap)
+                    .stage(hd_out(), out, in, flx, fly, coeff);
+            },
+                gfx950<>(), g, in, coeff, out);
         auto hin = mirror(in), hco = mirror(coeff);
         std::vector<double> ref(hin.size(), 0.);
         auto st = in->strides();
         const size_t o = h + h * st[1];
         gto_hdiff_f64(ref.data() + o, hin.data() + o, hco.data() + o, ni, nj, nk, st[1], st[2], 4);
         char name[96];
-        std::snprintf(name, sizeof name, "multistage hdiff with temporaries %dx%dx%d", ni, nj, nk);
+        std::snprintf(name, sizeof name, "fused multistage hdiff, temporaries %s %dx%dx%d",
+            cached ? "ij-cached in LDS" : "in HBM", ni, nj, nk);
         report(name, same_interior(out, ref, g));
     }
 
@@ -545,14 +559,89 @@ namespace {
         }
         report("field bounds check of the extent analysis", threw);
     }
+    // --bench: the engine's hdiff and vadv specs at 256x256x80 FP64 over 4 rotating field sets (past the 256 MB
+    // MALL), mean microseconds per run from one hipEvent pair around `reps` back-to-back runs
+    int bench(int reps) {
+        const int ni = 256, nj = 256, nk = 80, nsets = 4;
+        std::vector<store<double>> hin, hco, hout, uts, us, up, ut, wc;
+        for (int s = 0; s < nsets; ++s) {
+            hin.push_back(seeded<double>(ni + 4, nj + 4, nk, 2, 42 + s, 1, -1, 1));
+            hco.push_back(seeded<double>(ni + 4, nj + 4, nk, 2, 42 + s, 2, 0, 0.25));
+            hout.push_back(storage::builder<storage::gpu>.type<double>().dimensions(ni + 4, nj + 4, nk).halos(2, 2, 0)
+                               .build());
+            uts.push_back(seeded<double>(ni + 6, nj + 6, nk, 3, 42 + s, 6, -1, 1));
+            us.push_back(seeded<double>(ni + 6, nj + 6, nk, 3, 42 + s, 3, -1, 1));
+            up.push_back(seeded<double>(ni + 6, nj + 6, nk, 3, 42 + s, 4, -1, 1));
+            ut.push_back(seeded<double>(ni + 6, nj + 6, nk, 3, 42 + s, 5, -1, 1));
+            wc.push_back(seeded<double>(ni + 6, nj + 6, nk + 1, 3, 42 + s, 7, -0.5, 0.5));
+        }
+        auto gh = halo_grid(ni, nj, nk, 2), gv = halo_grid(ni, nj, nk, 3);
+        auto hd = [&](int s) {
+            run([](auto in, auto coeff, auto out) {
+                GT_DECLARE_TMP(double, lap, flx, fly);
+                return execute_parallel()
+                    .ij_cached(lap, flx, fly)
+                    .stage(hd_lap(), lap, in)
+                    .stage(hd_flx(), flx, in, lap)
+                    .stage(hd_fly(), fly, in, lap)
+                    .stage(hd_out(), out, in, flx, fly, coeff);
+            },
+                gfx950<>(), gh, hin[s % nsets], hco[s % nsets], hout[s % nsets]);
+        };
+        auto va = [&](int s) {
+            run([](auto utens_stage, auto u_stage, auto wcon, auto u_pos, auto utens, auto dtr_stage) {
+                GT_DECLARE_TMP(double, ccol, dcol, data_col);
+                return multi_pass(
+                    execute_forward().stage(va_forward(), utens_stage, wcon, u_stage, u_pos, utens, dtr_stage, ccol, dcol),
+                    execute_backward().stage(va_backward(), utens_stage, u_pos, dtr_stage, ccol, dcol, data_col));
+            },
+                gfx950<>(), gv, uts[s % nsets], us[s % nsets], wc[s % nsets], up[s % nsets], ut[s % nsets],
+                global_parameter(0.15));
+        };
+        auto time_us = [&](auto &&fn) {
+            for (int s = 0; s < nsets; ++s)
+                fn(s);
+            hipEvent_t e0, e1;
+            check(hipEventCreate(&e0) == hipSuccess ? 0 : -2, "event");
+            check(hipEventCreate(&e1) == hipSuccess ? 0 : -2, "event");
+            check(gt355_synchronize(), "sync");
+            (void)hipEventRecord(e0, nullptr);
+            for (int s = 0; s < reps; ++s)
+                fn(s);
+            (void)hipEventRecord(e1, nullptr);
+            (void)hipEventSynchronize(e1);
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            return 1e3 * ms / reps;
+        };
+        const double hd_us = time_us(hd), va_us = time_us(va);
+        const double pts = double(ni) * nj * nk;
+        std::printf("{\"hdiff_fused_lds_us\": %.2f, \"hdiff_gbs\": %.1f, \"vadv_vertical_us\": %.2f, \"vadv_gbs\": "
+                    "%.1f, \"reps\": %d}\n",
+            hd_us, 24 * pts / (hd_us * 1e-6) / 1e9, va_us, 48 * pts / (va_us * 1e-6) / 1e9, reps);
+        return 0;
+    }
 } // namespace
 
-int main() {
+int main(int argc, char **argv) {
+    if (argc > 1 && std::strcmp(argv[1], "--bench") == 0) {
+        try {
+            check(gt355_init(0), "init");
+            return bench(argc > 2 ? std::atoi(argv[2]) : 50);
+        } catch (std::exception const &e) {
+            std::printf("[FAIL] exception: %s\n", e.what());
+            return 1;
+        }
+    }
     try {
         check(gt355_init(0), "init");
         copy_and_laplacian();
         horizontal_diffusion(23, 11, 43);
         horizontal_diffusion(64, 64, 60);
+        horizontal_diffusion(130, 37, 5, false);
+        horizontal_diffusion(256, 256, 80);
         vertical_advection(23, 11, 43);
         vertical_advection(70, 9, 80);
         tridiagonal(12, 33, 6, true);
