@@ -14,7 +14,8 @@
  *   * a parallel multistage launches one kernel per stage over the stage's extended domain, one thread per point
  *     (each stage completes before the next reads it, as the reference's parallel execution guarantees);
  *   * a forward / backward multistage launches one kernel in which a thread owns an (i, j) column and walks k in
- *     order, running every stage at each level (k-caches are plain memory here);
+ *     order, running every stage at each level; its k_cached placeholders live in a per-thread register window
+ *     (kring) that slides with k, flushed to memory under the flush policy;
  *   * apply overloads are dispatched per k-level by interval (level<splitter, offset>, full_interval::modify<>,
  *     first_level, last_level), like the reference's interval matching.
  * Stages of a vertical multistage may not read each other's outputs at a horizontal offset (the reference needs
@@ -30,6 +31,14 @@
 #include <mutex>
 
 #include "gridtools.hpp"
+
+// k_cached placeholders with the fill policy: 0 (default) reads them from memory at each level, where their loads
+// issue with the stage's other loads; 1 keeps them in the register window too (one load per level at the slide, which
+// measured slower on the vadv spec: 218 vs 159 us at 256x256x80).  Flush and policy-free k-caches always live in the
+// register window.
+#ifndef GT355_ENGINE_RING_FILL
+#define GT355_ENGINE_RING_FILL 0
+#endif
 
 #ifndef GT_FUNCTION
 #define GT_FUNCTION __host__ __device__ __forceinline__
@@ -265,7 +274,33 @@ namespace gt355 {
             template <class... Ts>
             struct tlist {};
 
-            // Caches: the placeholders named by ij_cached (kept in LDS tiles when the multistage is fused)
+            // a k-cached placeholder of a vertical multistage and its policies (cache_io_policy fill / flush)
+            template <class P, bool Fill, bool Flush>
+            struct kc {};
+            template <class T>
+            struct is_policy : std::false_type {};
+            template <>
+            struct is_policy<cartesian::cache_io_policy::fill> : std::true_type {};
+            template <>
+            struct is_policy<cartesian::cache_io_policy::flush> : std::true_type {};
+            template <class A, class B>
+            struct lcat;
+            template <class... A, class... B>
+            struct lcat<tlist<A...>, tlist<B...>> {
+                using type = tlist<A..., B...>;
+            };
+            template <bool Fi, bool Fl, class... Ts>
+            struct make_kcs {
+                using type = tlist<>;
+            };
+            template <bool Fi, bool Fl, class T, class... Ts>
+            struct make_kcs<Fi, Fl, T, Ts...> {
+                using rest = typename make_kcs<Fi, Fl, Ts...>::type;
+                using type = std::conditional_t<is_policy<T>::value, rest, typename lcat<tlist<kc<T, Fi, Fl>>, rest>::type>;
+            };
+
+            // Caches: the placeholders named by ij_cached (kept in LDS tiles when the multistage is fused) and the kc<>
+            // entries of k_cached (kept in registers by a vertical multistage)
             template <class Exec, class Caches, class... Stages>
             struct mss_desc;
             template <class Exec, class... Cs, class... Stages>
@@ -279,11 +314,17 @@ namespace gt355 {
                 constexpr mss_desc<Exec, tlist<Cs..., Ts...>, Stages...> ij_cached(Ts...) const {
                     return {};
                 }
-                // k-cache policies and placeholders are accepted; the engine keeps k-caches in memory
                 template <class... Ts>
-                constexpr mss_desc k_cached(Ts...) const {
-                    return *this;
+                constexpr auto k_cached(Ts...) const {
+                    constexpr bool fill = (std::is_same<Ts, cartesian::cache_io_policy::fill>::value || ...);
+                    constexpr bool flush = (std::is_same<Ts, cartesian::cache_io_policy::flush>::value || ...);
+                    using added = typename make_kcs<fill, flush, Ts...>::type;
+                    return typename lcat_mss<typename lcat<tlist<Cs...>, added>::type>::type();
                 }
+                template <class NewCaches>
+                struct lcat_mss {
+                    using type = mss_desc<Exec, NewCaches, Stages...>;
+                };
             };
 
             template <class... Msses>
@@ -383,17 +424,39 @@ namespace gt355 {
                 return l.val;
             }
 
-            template <class Views>
+            // a k-cached placeholder of a vertical multistage: its values live in the vertical kernel's register window
+            // (kring) for that key; the evaluator reads level k + dk as element dk - KMIN of the window
+            template <class T, int KMIN, int KMAX, int KEY>
+            struct kring_ref {
+                __device__ void rebase(int, int) {}
+            };
+
+            template <int KEY, int... Is, class... Rs>
+            __device__ __forceinline__ auto &ring_by_key(vtuple_impl<std::integer_sequence<int, Is...>, Rs...> &rs) {
+                constexpr int idx = ((Rs::key == KEY ? Is : 0) + ... + 0);
+                return vget<idx>(rs);
+            }
+
+            template <class Views, class Rings = vtuple<>>
             struct evaluator {
                 Views const &v;
+                Rings &rs;
                 int i, j, k;
-                template <int I, class E, int N>
-                __device__ decltype(auto) operator()(cartesian::in_accessor<I, E, N> const &a) const {
-                    return vget<I>(v).at(i + a.di, j + a.dj, k + a.dk);
+                template <class View, class Acc>
+                __device__ __forceinline__ decltype(auto) get(View const &view, Acc const &a) const {
+                    return view.at(i + a.di, j + a.dj, k + a.dk);
+                }
+                template <class T, int KMIN, int KMAX, int KEY, class Acc>
+                __device__ __forceinline__ T &get(kring_ref<T, KMIN, KMAX, KEY> const &, Acc const &a) const {
+                    return ring_by_key<KEY>(rs).r[a.dk - KMIN];
                 }
                 template <int I, class E, int N>
-                __device__ decltype(auto) operator()(cartesian::inout_accessor<I, E, N> const &a) const {
-                    return vget<I>(v).at(i + a.di, j + a.dj, k + a.dk);
+                __device__ __forceinline__ decltype(auto) operator()(cartesian::in_accessor<I, E, N> const &a) const {
+                    return get(vget<I>(v), a);
+                }
+                template <int I, class E, int N>
+                __device__ __forceinline__ decltype(auto) operator()(cartesian::inout_accessor<I, E, N> const &a) const {
+                    return get(vget<I>(v), a);
                 }
                 template <class X>
                 __device__ auto operand(X const &x) const {
@@ -412,15 +475,112 @@ namespace gt355 {
                 }
             };
 
+            // the register window of one k-cached placeholder: levels kcur + KMIN .. kcur + KMAX of the thread's column
+            template <class T, int KMIN, int KMAX, int KEY, bool FILL, bool FLUSH>
+            struct kring {
+                static constexpr int key = KEY;
+                T r[KMAX - KMIN + 1];
+                int kcur;
+                field_view<T> mem; // the placeholder's memory (fill / flush)
+                int fb[4], wb[4];  // column boxes {i0, i1, j0, j1}: read (fill) and written (flush) by the multistage
+                __device__ bool in(const int *b, int i, int j) const { return i >= b[0] && i < b[1] && j >= b[2] && j < b[3]; }
+                // the whole window at the sweep's first level
+                __device__ void start(int i, int j, int k, int nk) {
+                    kcur = k;
+                    if constexpr (FILL) {
+                        if (in(fb, i, j)) {
+#pragma unroll
+                            for (int o = KMIN; o <= KMAX; ++o)
+                                if (k + o >= 0 && k + o < nk)
+                                    r[o - KMIN] = mem.at(i, j, k + o);
+                        }
+                    }
+                }
+                __device__ void flush(int i, int j) {
+                    if constexpr (FLUSH) {
+                        if (in(wb, i, j))
+                            mem.at(i, j, kcur) = r[-KMIN];
+                    }
+                }
+                // move the window one level in the sweep direction and fill the level it uncovers
+                template <bool Up>
+                __device__ void slide(int i, int j, int nk) {
+                    if constexpr (Up) {
+#pragma unroll
+                        for (int o = KMIN; o < KMAX; ++o)
+                            r[o - KMIN] = r[o + 1 - KMIN];
+                        ++kcur;
+                        if constexpr (FILL) {
+                            if (in(fb, i, j) && kcur + KMAX < nk)
+                                r[KMAX - KMIN] = mem.at(i, j, kcur + KMAX);
+                        }
+                    } else {
+#pragma unroll
+                        for (int o = KMAX; o > KMIN; --o)
+                            r[o - KMIN] = r[o - 1 - KMIN];
+                        --kcur;
+                        if constexpr (FILL) {
+                            if (in(fb, i, j) && kcur + KMIN >= 0)
+                                r[0] = mem.at(i, j, kcur + KMIN);
+                        }
+                    }
+                }
+            };
+
             template <class P, class L>
             struct in_list : std::false_type {};
             template <class P, class... Ls>
             struct in_list<P, tlist<Ls...>> : std::bool_constant<(std::is_same<P, Ls>::value || ...)> {};
 
-            // the view a stage gets for placeholder P: an LDS tile if P is ij-cached by a fused multistage
-            template <class P, class Caches>
-            using view_for = std::conditional_t<in_list<P, Caches>::value, lds_view<typename P::kind::data_t>,
-                typename view_of<typename P::kind>::type>;
+            // the k extent window of placeholder P over the stages of a multistage (union of its accessors' k extents)
+            template <class P, class PL, class AL>
+            struct kext_pair;
+            template <class P, class... Ps, class... As>
+            struct kext_pair<P, tlist<Ps...>, cartesian::param_list_t<As...>> {
+                static constexpr int kmin = std::min({0, (std::is_same<P, Ps>::value ? As::extent_t::kminus : 0)...});
+                static constexpr int kmax = std::max({0, (std::is_same<P, Ps>::value ? As::extent_t::kplus : 0)...});
+            };
+            template <class P, class Stage>
+            struct stage_kext;
+            template <class P, class F, class... Ps>
+            struct stage_kext<P, stage_desc<F, Ps...>> : kext_pair<P, tlist<Ps...>, typename F::param_list> {};
+            template <class P, class... Stages>
+            struct mss_kext {
+                static constexpr int kmin = std::min({0, stage_kext<P, Stages>::kmin...});
+                static constexpr int kmax = std::max({0, stage_kext<P, Stages>::kmax...});
+            };
+
+            // the kc<> entry of P in a cache list, if any
+            template <class P, class L>
+            struct find_kc {
+                static constexpr bool found = false, fill = false, flush = false;
+            };
+            template <class P, bool Fi, bool Fl, class... Ls>
+            struct find_kc<P, tlist<kc<P, Fi, Fl>, Ls...>> {
+                static constexpr bool found = true, fill = Fi, flush = Fl;
+            };
+            template <class P, class L0, class... Ls>
+            struct find_kc<P, tlist<L0, Ls...>> : find_kc<P, tlist<Ls...>> {};
+
+            // the view context of a vertical multistage: its cache list and stages
+            template <class Caches, class... Stages>
+            struct vctx {};
+
+            // the view a stage gets for placeholder P: an LDS tile if P is ij-cached by a fused multistage, a register
+            // window if P is k-cached by a vertical multistage, else the placeholder's own view
+            template <class P, class Ctx>
+            struct view_for_t {
+                using type = std::conditional_t<in_list<P, Ctx>::value, lds_view<typename P::kind::data_t>,
+                    typename view_of<typename P::kind>::type>;
+            };
+            template <class P, class Caches, class... Stages>
+            struct view_for_t<P, vctx<Caches, Stages...>> {
+                using type = std::conditional_t<find_kc<P, Caches>::found && !(GT355_ENGINE_RING_FILL == 0 && find_kc<P, Caches>::fill),
+                    kring_ref<typename P::kind::data_t, mss_kext<P, Stages...>::kmin, mss_kext<P, Stages...>::kmax, P::key>,
+                    typename view_of<typename P::kind>::type>;
+            };
+            template <class P, class Ctx>
+            using view_for = typename view_for_t<P, Ctx>::type;
 
             template <class Stage, class Caches = tlist<>>
             struct stage_views;
@@ -489,11 +649,16 @@ namespace gt355 {
                         (!done && in_interval<Is>(k, nk) ? (F::apply(e, Is()), done = true) : false), 0)...};
                 }
             }
-            template <class F, class Views>
-            __device__ void run_stage_at(Views const &v, int i, int j, int k, int nk) {
-                using E = evaluator<Views>;
-                E e{v, i, j, k};
+            template <class F, class Views, class Rings>
+            __device__ __forceinline__ void run_stage_at(Views const &v, Rings &rs, int i, int j, int k, int nk) {
+                using E = evaluator<Views, Rings>;
+                E e{v, rs, i, j, k};
                 apply_at<F>(e, k, nk, typename matching<F, E, candidates_t>::type());
+            }
+            template <class F, class Views>
+            __device__ __forceinline__ void run_stage_at(Views const &v, int i, int j, int k, int nk) {
+                vtuple<> none;
+                run_stage_at<F>(v, none, i, j, k, nk);
             }
 
             // ------------------------------------------------------------------------------------------ kernels
@@ -512,34 +677,72 @@ namespace gt355 {
             struct vertical_runner;
             template <class Exec, class... Stages, class... Views>
             struct vertical_runner<Exec, tlist<Stages...>, vtuple<Views...>> {
-                template <int... Ss>
-                __device__ static void level(vtuple<Views...> const &v, const int (*dom)[4], int i, int j, int k, int nk,
-                    std::integer_sequence<int, Ss...>) {
-                    (void)std::initializer_list<int>{(
-                        (i >= dom[Ss][0] && i < dom[Ss][1] && j >= dom[Ss][2] && j < dom[Ss][3]
-                                ? run_stage_at<typename Stages::functor>(vget<Ss>(v), i, j, k, nk)
-                                : void()),
-                        0)...};
+                template <class Rings, int... Ss>
+                __device__ static void level(vtuple<Views...> const &v, Rings &rs, const int (*dom)[4], int i, int j, int k,
+                    int nk, std::integer_sequence<int, Ss...>) {
+                    ((i >= dom[Ss][0] && i < dom[Ss][1] && j >= dom[Ss][2] && j < dom[Ss][3]
+                             ? run_stage_at<typename Stages::functor>(vget<Ss>(v), rs, i, j, k, nk)
+                             : void()),
+                        ...);
                 }
             };
             template <int NS>
             struct domains {
                 int d[NS][4];
             };
-            template <class Exec, class StageList, class ViewsTuple, int NS>
+            // the register windows of a vertical multistage: one kring per kc<> entry of its cache list
+            template <class Entry, class... Stages>
+            struct ring_of;
+            template <class P, bool Fi, bool Fl, class... Stages>
+            struct ring_of<kc<P, Fi, Fl>, Stages...> {
+                using type = kring<typename P::kind::data_t, mss_kext<P, Stages...>::kmin, mss_kext<P, Stages...>::kmax,
+                    P::key, Fi, Fl>;
+            };
+            template <class Caches, class StageList>
+            struct ring_list {
+                using type = tlist<>;
+            };
+            template <class C0, class... Cs, class... Stages>
+            struct ring_list<tlist<C0, Cs...>, tlist<Stages...>> {
+                using rest = typename ring_list<tlist<Cs...>, tlist<Stages...>>::type;
+                using type = rest;
+            };
+            template <class P, bool Fi, bool Fl, class... Cs, class... Stages>
+            struct ring_list<tlist<kc<P, Fi, Fl>, Cs...>, tlist<Stages...>> {
+                using rest = typename ring_list<tlist<Cs...>, tlist<Stages...>>::type;
+                using type = std::conditional_t<GT355_ENGINE_RING_FILL == 0 && Fi, rest,
+                    typename lcat<tlist<typename ring_of<kc<P, Fi, Fl>, Stages...>::type>, rest>::type>;
+            };
+            template <class L>
+            struct to_vtuple;
+            template <class... Ts>
+            struct to_vtuple<tlist<Ts...>> {
+                using type = vtuple<Ts...>;
+            };
+
+            template <class F, int... Is, class... Rs>
+            __device__ void for_rings(vtuple_impl<std::integer_sequence<int, Is...>, Rs...> &rs, F &&f) {
+                (f(vget<Is>(rs)), ...);
+            }
+
+            template <class Exec, class StageList, class ViewsTuple, class RingSet, int NS>
             __global__ void __launch_bounds__(256) vertical_kernel(
-                ViewsTuple v, domains<NS> dom, int i0, int ni, int j0, int nj, int nk) {
+                ViewsTuple v0, RingSet rs0, domains<NS> dom, int i0, int ni, int j0, int nj, int nk) {
                 const int i = i0 + (int)(blockIdx.x * 64 + (threadIdx.x & 63));
                 const int j = j0 + (int)(blockIdx.y * 4 + (threadIdx.x >> 6));
                 if (i >= i0 + ni || j >= j0 + nj)
                     return;
                 using R = vertical_runner<Exec, StageList, ViewsTuple>;
-                if constexpr (std::is_same<Exec, backward>::value) {
-                    for (int k = nk - 1; k >= 0; --k)
-                        R::level(v, dom.d, i, j, k, nk, std::make_integer_sequence<int, NS>());
-                } else {
-                    for (int k = 0; k < nk; ++k)
-                        R::level(v, dom.d, i, j, k, nk, std::make_integer_sequence<int, NS>());
+                constexpr bool up = !std::is_same<Exec, backward>::value;
+                RingSet rs = rs0;
+                const int k0 = up ? 0 : nk - 1;
+                for_rings(rs, [&](auto &r) { r.start(i, j, k0, nk); });
+                for (int n = 0; n < nk; ++n) {
+                    const int k = up ? n : nk - 1 - n;
+                    R::level(v0, rs, dom.d, i, j, k, nk, std::make_integer_sequence<int, NS>());
+                    for_rings(rs, [&](auto &r) { r.flush(i, j); });
+                    if (n + 1 < nk)
+                        for_rings(rs, [&](auto &r) { r.template slide<up>(i, j, nk); });
                 }
             }
 
@@ -720,6 +923,8 @@ namespace gt355 {
                 if constexpr (in_list<P, Caches>::value) {
                     auto const &l = c.lds.at(P::key);
                     return {l.off, l.w, l.imin, l.jmin, 0};
+                } else if constexpr (!std::is_same<view_for<P, Caches>, typename view_of<K>::type>::value) {
+                    return {}; // a register window of the vertical kernel
                 } else if constexpr (std::is_same<typename view_of<K>::type, field_view<typename K::data_t>>::value) {
                     auto const &f = c.fields.at(P::key);
                     return {static_cast<typename K::data_t *>(f.origin), f.s[0], f.s[1], f.s[2]};
@@ -826,8 +1031,11 @@ namespace gt355 {
                     };
                     (one(Stages()), ...);
                 } else {
-                    using ViewsTuple = vtuple<typename stage_views<Stages>::type...>;
+                    using Ctx = vctx<typename T::caches, Stages...>;
+                    using ViewsTuple = vtuple<typename stage_views<Stages, Ctx>::type...>;
+                    using RingSet = typename to_vtuple<typename ring_list<typename T::caches, tlist<Stages...>>::type>::type;
                     ViewsTuple vt{};
+                    RingSet rs{};
                     domains<NS> dom{};
                     ext_rt u;
                     for (int n = 0; n < NS; ++n) {
@@ -835,13 +1043,55 @@ namespace gt355 {
                         dom.d[n][0] = e.imin, dom.d[n][1] = ni + e.imax, dom.d[n][2] = e.jmin, dom.d[n][3] = nj + e.jmax;
                         u.merge(e);
                     }
-                    assign_all<tlist<>>(vt, c, args, tlist<Stages...>(), std::make_integer_sequence<int, NS>());
+                    assign_all<Ctx>(vt, c, args, tlist<Stages...>(), std::make_integer_sequence<int, NS>());
+                    init_rings(rs, c, st, first, NS, dom.d);
                     const int w = ni - u.imin + u.imax, h = nj - u.jmin + u.jmax;
                     if (w > 0 && h > 0 && nk > 0)
-                        vertical_kernel<Exec, tlist<Stages...>, ViewsTuple, NS>
-                            <<<grid_2d(w, h, 1), 256, 0, stream>>>(vt, dom, u.imin, w, u.jmin, h, nk);
+                        vertical_kernel<Exec, tlist<Stages...>, ViewsTuple, RingSet, NS>
+                            <<<grid_2d(w, h, 1), 256, 0, stream>>>(vt, rs, dom, u.imin, w, u.jmin, h, nk);
                     check_launch("stencil engine: vertical multistage launch");
                 }
+            }
+
+            // a register window's memory and column boxes: it fills where its multistage reads the placeholder and
+            // flushes where it writes it
+            template <class Ring>
+            void init_ring(Ring &r, run_ctx const &c, std::vector<stage_rt> const &st, int first, int ns,
+                const int (*dom)[4]) {
+                using T = std::decay_t<decltype(r.r[0])>;
+                bool any_r = false, any_w = false;
+                for (int n = 0; n < ns; ++n)
+                    for (auto const &a : st[first + n].acc) {
+                        if (a.key != Ring::key)
+                            continue;
+                        if (a.ext.imin || a.ext.imax || a.ext.jmin || a.ext.jmax)
+                            throw std::runtime_error("run: a k_cached placeholder is accessed at a horizontal offset");
+                        auto grow = [&](int *b, bool &any) {
+                            if (!any) {
+                                for (int q = 0; q < 4; ++q)
+                                    b[q] = dom[n][q];
+                                any = true;
+                            } else {
+                                b[0] = std::min(b[0], dom[n][0]), b[1] = std::max(b[1], dom[n][1]);
+                                b[2] = std::min(b[2], dom[n][2]), b[3] = std::max(b[3], dom[n][3]);
+                            }
+                        };
+                        grow(r.fb, any_r);
+                        if (a.write)
+                            grow(r.wb, any_w);
+                    }
+                if (!any_w)
+                    r.wb[0] = r.wb[1] = r.wb[2] = r.wb[3] = 0;
+                auto it = c.fields.find(Ring::key);
+                if (it != c.fields.end())
+                    r.mem = {static_cast<T *>(it->second.origin), it->second.s[0], it->second.s[1], it->second.s[2]};
+                else
+                    r.fb[0] = r.fb[1] = r.wb[0] = r.wb[1] = 0; // no memory: a purely register-held temporary
+            }
+            template <int... Is, class... Rs>
+            void init_rings(vtuple_impl<std::integer_sequence<int, Is...>, Rs...> &rs, run_ctx const &c,
+                std::vector<stage_rt> const &st, int first, int ns, const int (*dom)[4]) {
+                (init_ring(vget<Is>(rs), c, st, first, ns, dom), ...);
             }
             template <class Caches, class VT, class ArgsTuple, class... Stages, int... Is>
             void assign_all(VT &vt, run_ctx const &c, ArgsTuple const &args, tlist<Stages...>,

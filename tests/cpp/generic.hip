@@ -591,9 +591,12 @@ namespace {
         auto va = [&](int s) {
             run([](auto utens_stage, auto u_stage, auto wcon, auto u_pos, auto utens, auto dtr_stage) {
                 GT_DECLARE_TMP(double, ccol, dcol, data_col);
-                return multi_pass(
-                    execute_forward().stage(va_forward(), utens_stage, wcon, u_stage, u_pos, utens, dtr_stage, ccol, dcol),
-                    execute_backward().stage(va_backward(), utens_stage, u_pos, dtr_stage, ccol, dcol, data_col));
+                return multi_pass(execute_forward()
+                                      .k_cached(cache_io_policy::flush(), ccol, dcol)
+                                      .k_cached(cache_io_policy::fill(), u_stage)
+                                      .stage(va_forward(), utens_stage, wcon, u_stage, u_pos, utens, dtr_stage, ccol, dcol),
+                    execute_backward().k_cached(data_col).stage(
+                        va_backward(), utens_stage, u_pos, dtr_stage, ccol, dcol, data_col));
             },
                 gfx950<>(), gv, uts[s % nsets], us[s % nsets], wc[s % nsets], up[s % nsets], ut[s % nsets],
                 global_parameter(0.15));
