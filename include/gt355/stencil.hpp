@@ -579,8 +579,104 @@ namespace gt355 {
                     kring_ref<typename P::kind::data_t, mss_kext<P, Stages...>::kmin, mss_kext<P, Stages...>::kmax, P::key>,
                     typename view_of<typename P::kind>::type>;
             };
+            // fused parallel multistages stage every field they only read, at a horizontal offset and without k offsets,
+            // into an LDS tile per workgroup (one coalesced load), so that the stages' neighbour reads hit LDS
+            template <class P, class PL, class AL>
+            struct acc_props;
+            template <class P, class... Ps, class... As>
+            struct acc_props<P, tlist<Ps...>, cartesian::param_list_t<As...>> {
+                static constexpr bool written = ((std::is_same<P, Ps>::value && !As::is_const) || ...);
+                static constexpr bool ij = ((std::is_same<P, Ps>::value &&
+                                                (As::extent_t::iminus || As::extent_t::iplus || As::extent_t::jminus ||
+                                                    As::extent_t::jplus)) ||
+                                            ...);
+                static constexpr bool kofs =
+                    ((std::is_same<P, Ps>::value && (As::extent_t::kminus || As::extent_t::kplus)) || ...);
+            };
+            template <class P, class Stage>
+            struct stage_props;
+            template <class P, class F, class... Ps>
+            struct stage_props<P, stage_desc<F, Ps...>> : acc_props<P, tlist<Ps...>, typename F::param_list> {};
+            template <class P, class... Stages>
+            struct auto_stage {
+                static constexpr bool value =
+                    std::is_same<typename view_of<typename P::kind>::type, field_view<typename P::kind::data_t>>::value &&
+                    !(stage_props<P, Stages>::written || ...) && (stage_props<P, Stages>::ij || ...) &&
+                    !(stage_props<P, Stages>::kofs || ...);
+            };
+
+            template <class Caches, class... Stages>
+            struct fctx {};
+            template <class P, class Caches, class... Stages>
+            struct view_for_t<P, fctx<Caches, Stages...>> {
+                using type = std::conditional_t<in_list<P, Caches>::value || auto_stage<P, Stages...>::value,
+                    lds_view<typename P::kind::data_t>, typename view_of<typename P::kind>::type>;
+            };
+
             template <class P, class Ctx>
             using view_for = typename view_for_t<P, Ctx>::type;
+
+            // the distinct placeholders of a list of stages, and those a fused multistage stages into LDS
+            template <class L, class P>
+            struct tset_add {
+                using type = std::conditional_t<in_list<P, L>::value, L, typename lcat<L, tlist<P>>::type>;
+            };
+            template <class L, class... Ps>
+            struct tset_add_all {
+                using type = L;
+            };
+            template <class L, class P, class... Ps>
+            struct tset_add_all<L, P, Ps...> : tset_add_all<typename tset_add<L, P>::type, Ps...> {};
+            template <class L, class... Stages>
+            struct plhs_of_stages {
+                using type = L;
+            };
+            template <class L, class F, class... Ps, class... Stages>
+            struct plhs_of_stages<L, stage_desc<F, Ps...>, Stages...>
+                : plhs_of_stages<typename tset_add_all<L, Ps...>::type, Stages...> {};
+            template <class L, class Stages>
+            struct staged_of;
+            template <class... Stages>
+            struct staged_of<tlist<>, tlist<Stages...>> {
+                using type = tlist<>;
+            };
+            template <class P, class... Ps, class... Stages>
+            struct staged_of<tlist<P, Ps...>, tlist<Stages...>> {
+                using rest = typename staged_of<tlist<Ps...>, tlist<Stages...>>::type;
+                using type =
+                    std::conditional_t<auto_stage<P, Stages...>::value, typename lcat<tlist<P>, rest>::type, rest>;
+            };
+            template <class... Stages>
+            using staged_inputs = typename staged_of<typename plhs_of_stages<tlist<>, Stages...>::type, tlist<Stages...>>::type;
+
+            // one staged input: its memory and its LDS tile (first element, row length, tile box relative to the tile)
+            template <class T>
+            struct stage_in {
+                field_view<T> src;
+                int off, w, imin, imax, jmin, jmax;
+            };
+            template <class T>
+            __device__ void load_stage_in(
+                stage_in<T> const &s, int ti0, int tj0, int TI, int TJ, int ni, int nj, int k) {
+                const int i_lo = ti0 + s.imin, i_hi = min(ti0 + TI + s.imax, ni + s.imax);
+                const int j_lo = tj0 + s.jmin, j_hi = min(tj0 + TJ + s.jmax, nj + s.jmax);
+                const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+                T *lds = reinterpret_cast<T *>(engine_lds) + s.off;
+                for (int j = j_lo + wave; j < j_hi; j += 4)
+                    for (int i = i_lo + lane; i < i_hi; i += 64)
+                        lds[(j - j_lo) * s.w + (i - i_lo)] = s.src.at(i, j, k);
+            }
+            template <int... Is, class... Ss>
+            __device__ void load_stage_ins(vtuple_impl<std::integer_sequence<int, Is...>, Ss...> const &si, int ti0,
+                int tj0, int TI, int TJ, int ni, int nj, int k) {
+                (load_stage_in(vget<Is>(si), ti0, tj0, TI, TJ, ni, nj, k), ...);
+            }
+            template <class L>
+            struct stage_in_tuple;
+            template <class... Ps>
+            struct stage_in_tuple<tlist<Ps...>> {
+                using type = vtuple<stage_in<typename Ps::kind::data_t>...>;
+            };
 
             template <class Stage, class Caches = tlist<>>
             struct stage_views;
@@ -772,11 +868,13 @@ namespace gt355 {
                     (stage_tile<typename Stages::functor>(vget<Ss>(v), ext[Ss], ti0, tj0, TI, TJ, ni, nj, k, nk), ...);
                 }
             };
-            template <class StageList, class ViewsTuple, int NS>
+            template <class StageList, class ViewsTuple, class StageIns, int NS>
             __global__ void __launch_bounds__(256) fused_kernel(
-                ViewsTuple v0, domains<NS> ext, int ni, int nj, int nk, int tiles_i, int TI, int TJ) {
+                ViewsTuple v0, StageIns si, domains<NS> ext, int ni, int nj, int nk, int tiles_i, int TI, int TJ) {
                 const int tile = (int)blockIdx.x;
                 const int ti0 = (tile % tiles_i) * TI, tj0 = (tile / tiles_i) * TJ;
+                load_stage_ins(si, ti0, tj0, TI, TJ, ni, nj, (int)blockIdx.z);
+                __syncthreads();
                 ViewsTuple v = v0;
                 fused_runner<StageList, ViewsTuple>::run(
                     v, ext.d, ti0, tj0, TI, TJ, ni, nj, (int)blockIdx.z, nk, std::make_integer_sequence<int, NS>());
@@ -905,6 +1003,7 @@ namespace gt355 {
             };
             struct lds_rt {
                 int off, w, imin, jmin; // first element (in elements of the value type), row length, tile origin
+                int imax, jmax;         // the tile's far edges relative to the workgroup's block
             };
             struct fuse_rt {
                 int ti = 64, tj = 8;
@@ -920,7 +1019,7 @@ namespace gt355 {
             template <class P, class Caches, class ArgsTuple>
             view_for<P, Caches> make_view(run_ctx const &c, ArgsTuple const &args) {
                 using K = typename P::kind;
-                if constexpr (in_list<P, Caches>::value) {
+                if constexpr (std::is_same<view_for<P, Caches>, lds_view<typename K::data_t>>::value) {
                     auto const &l = c.lds.at(P::key);
                     return {l.off, l.w, l.imin, l.jmin, 0};
                 } else if constexpr (!std::is_same<view_for<P, Caches>, typename view_of<K>::type>::value) {
@@ -1002,20 +1101,24 @@ namespace gt355 {
                 constexpr int NS = sizeof...(Stages);
                 const int ni = g.i_size, nj = g.j_size, nk = g.k_size;
                 if constexpr (T::fused) {
-                    using Caches = typename T::caches;
-                    using ViewsTuple = vtuple<typename stage_views<Stages, Caches>::type...>;
+                    using Ctx = fctx<typename T::caches, Stages...>;
+                    using ViewsTuple = vtuple<typename stage_views<Stages, Ctx>::type...>;
+                    using Staged = staged_inputs<Stages...>;
+                    using StageIns = typename stage_in_tuple<Staged>::type;
+                    StageIns si{};
+                    init_stage_ins(si, Staged(), c);
                     ViewsTuple vt{};
                     domains<NS> ext{};
                     for (int n = 0; n < NS; ++n) {
                         ext_rt const &e = st[first + n].ext;
                         ext.d[n][0] = e.imin, ext.d[n][1] = e.imax, ext.d[n][2] = e.jmin, ext.d[n][3] = e.jmax;
                     }
-                    assign_all<Caches>(vt, c, args, tlist<Stages...>(), std::make_integer_sequence<int, NS>());
+                    assign_all<Ctx>(vt, c, args, tlist<Stages...>(), std::make_integer_sequence<int, NS>());
                     fuse_rt const &f = c.fuse[m];
                     const int tiles_i = (ni + f.ti - 1) / f.ti, tiles_j = (nj + f.tj - 1) / f.tj;
-                    fused_kernel<tlist<Stages...>, ViewsTuple, NS>
+                    fused_kernel<tlist<Stages...>, ViewsTuple, StageIns, NS>
                         <<<dim3((unsigned)(tiles_i * tiles_j), 1, (unsigned)nk), 256, f.bytes, stream>>>(
-                            vt, ext, ni, nj, nk, tiles_i, f.ti, f.tj);
+                            vt, si, ext, ni, nj, nk, tiles_i, f.ti, f.tj);
                     check_launch("stencil engine: fused multistage launch");
                 } else if constexpr (std::is_same<Exec, parallel>::value) {
                     int s = first;
@@ -1099,13 +1202,34 @@ namespace gt355 {
                 ((vget<Is>(vt) = stage_plhs<Stages>::template views<Caches>(c, args)), ...);
             }
 
+            template <class Mss>
+            struct staged_list;
+            template <class Exec, class Caches, class... Stages>
+            struct staged_list<mss_desc<Exec, Caches, Stages...>> {
+                using type = staged_inputs<Stages...>;
+            };
+            template <class P, class T>
+            void init_stage_in(stage_in<T> &s, run_ctx const &c) {
+                auto const &f = c.fields.at(P::key);
+                auto const &l = c.lds.at(P::key);
+                s.src = {static_cast<T *>(f.origin), f.s[0], f.s[1], f.s[2]};
+                s.off = l.off, s.w = l.w, s.imin = l.imin, s.jmin = l.jmin;
+                s.imax = l.imax, s.jmax = l.jmax;
+            }
+            template <int... Is, class... Ss, class... Ps>
+            void init_stage_ins(vtuple_impl<std::integer_sequence<int, Is...>, Ss...> &si, tlist<Ps...>, run_ctx const &c) {
+                (init_stage_in<Ps>(vget<Is>(si), c), ...);
+            }
+
             // LDS plan of a fused multistage: each ij-cached temporary gets a tile of the workgroup's TI x TJ block
             // extended by the temporary's extent; TJ shrinks until the tiles fit 64 KB of LDS
-            template <class... Cs>
-            void plan_lds(tlist<Cs...>, int m, std::vector<stage_rt> const &st, std::map<int, ext_rt> const &box,
-                run_ctx &c) {
+            template <class... Cs, class... Ins>
+            void plan_lds(tlist<Cs...>, tlist<Ins...>, int m, std::vector<stage_rt> const &st,
+                std::map<int, ext_rt> const &box, run_ctx &c) {
                 const int keys[] = {0, Cs::key...};
                 const int elems[] = {0, (int)sizeof(typename Cs::kind::data_t)...};
+                const int in_keys[] = {0, Ins::key...};
+                const int in_elems[] = {0, (int)sizeof(typename Ins::kind::data_t)...};
                 constexpr bool all_tmp = (true && ... && Cs::is_tmp);
                 static_assert(all_tmp, "ij_cached takes temporaries (GT_DECLARE_TMP)");
                 for (int q = 1; q <= (int)sizeof...(Cs); ++q)
@@ -1134,24 +1258,43 @@ namespace gt355 {
                     }
                 }
                 fuse_rt f;
-                // tile rows per workgroup: GT355_ENGINE_TJ (a power of two up to 32) or 16, halved until the tiles fit
+                // tile columns: 64 less the widest rim of the multistage's LDS tiles and stage extents, so that every row
+                // a stage or a staging load walks fits one pass of the 64 lanes (hdiff: 60); at least 32
+                int rim = 0;
+                for (auto const &s : st)
+                    if (s.mss == m)
+                        rim = std::max(rim, s.ext.imax - s.ext.imin);
+                auto widen = [&](int key) {
+                    if (box.count(key))
+                        rim = std::max(rim, box.at(key).imax - box.at(key).imin);
+                };
+                for (int q = 1; q <= (int)sizeof...(Cs); ++q)
+                    widen(keys[q]);
+                for (int q = 1; q <= (int)sizeof...(Ins); ++q)
+                    widen(in_keys[q]);
+                f.ti = std::max(32, 64 - rim);
+                // tile rows per workgroup: GT355_ENGINE_TJ (a power of two up to 32) or 8, halved until the tiles fit
                 const char *env = std::getenv("GT355_ENGINE_TJ");
-                const int tj0 = env ? std::max(1, std::min(32, std::atoi(env))) : 16;
+                const int tj0 = env ? std::max(1, std::min(32, std::atoi(env))) : 8;
                 for (f.tj = tj0;; f.tj /= 2) {
                     size_t bytes = 0;
                     std::map<int, lds_rt> lds;
-                    for (int q = 1; q <= (int)sizeof...(Cs); ++q) {
-                        if (!box.count(keys[q]))
-                            continue;
-                        ext_rt const &b = box.at(keys[q]);
+                    auto place = [&](int key, int elem) {
+                        if (!box.count(key))
+                            return;
+                        ext_rt const &b = box.at(key);
                         const int w = f.ti + b.imax - b.imin, h = f.tj + b.jmax - b.jmin;
                         bytes = (bytes + 15) / 16 * 16;
-                        lds[keys[q]] = {(int)(bytes / elems[q]), w, b.imin, b.jmin};
-                        bytes += (size_t)w * h * elems[q];
-                    }
+                        lds[key] = {(int)(bytes / elem), w, b.imin, b.jmin, b.imax, b.jmax};
+                        bytes += (size_t)w * h * elem;
+                    };
+                    for (int q = 1; q <= (int)sizeof...(Cs); ++q)
+                        place(keys[q], elems[q]);
+                    for (int q = 1; q <= (int)sizeof...(Ins); ++q)
+                        place(in_keys[q], in_elems[q]);
                     if (bytes <= 65536 || f.tj == 1) {
                         if (bytes > 65536)
-                            throw std::runtime_error("run: the ij-cached temporaries of a multistage exceed 64 KB of LDS");
+                            throw std::runtime_error("run: the LDS tiles of a multistage exceed 64 KB");
                         f.bytes = bytes;
                         for (auto &l : lds)
                             c.lds[l.first] = l.second;
@@ -1167,7 +1310,7 @@ namespace gt355 {
                 auto one = [&](auto mss) {
                     using T = mss_traits<decltype(mss)>;
                     if constexpr (T::fused)
-                        plan_lds(typename T::caches(), m, st, box, c);
+                        plan_lds(typename T::caches(), typename staged_list<decltype(mss)>::type(), m, st, box, c);
                     ++m;
                 };
                 (one(Msses()), ...);
