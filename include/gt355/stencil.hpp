@@ -1458,6 +1458,123 @@ namespace gt355 {
             }
         } // namespace engine_
 
+        // ------------------------------------------------------------------------------------------ stencil functions
+        // call<F, Region, ReturnType>::[at<i, j, k>::]with(eval, args...) and call_proc<F, Region>::with(eval, args...)
+        // (stencil/frontend/cartesian/stencil_functions.hpp:139-247): run functor F at the evaluation point (shifted by
+        // at<>) with its accessors bound to the caller's arguments -- an accessor argument is read by the caller's
+        // evaluator at the sum of both offsets, a value argument is read (or written) directly.  call returns what F
+        // writes to its single inout accessor; call_proc passes every argument, outputs included.
+        namespace engine_ {
+            template <class T>
+            struct is_accessor : std::false_type {};
+            template <int I, class E, int N>
+            struct is_accessor<cartesian::in_accessor<I, E, N>> : std::true_type {};
+            template <int I, class E, int N>
+            struct is_accessor<cartesian::inout_accessor<I, E, N>> : std::true_type {};
+
+            template <class AL>
+            struct out_index;
+            template <class... As>
+            struct out_index<cartesian::param_list_t<As...>> {
+                static constexpr int count = ((!As::is_const ? 1 : 0) + ... + 0);
+                static constexpr int value = ((!As::is_const ? As::index : 0) + ... + 0);
+            };
+
+            // Args: a vtuple of references to the arguments in the callee's parameter order (OUT: the position of the
+            // result, which is not among Args; -1 for call_proc)
+            template <class Eval, class Args, class Res, int OUT, int OI, int OJ, int OK>
+            struct call_eval {
+                Eval &e;
+                Args const &args;
+                Res *res;
+                template <int I>
+                __device__ __forceinline__ decltype(auto) param() const {
+                    if constexpr (I == OUT)
+                        return (*res);
+                    else
+                        return vget<(OUT >= 0 && I > OUT ? I - 1 : I)>(args);
+                }
+                template <class P, class Acc>
+                __device__ __forceinline__ decltype(auto) bind(P &&p, Acc const &a) const {
+                    using A = std::decay_t<P>;
+                    if constexpr (is_accessor<A>::value)
+                        return e(A(p.di + a.di + OI, p.dj + a.dj + OJ, p.dk + a.dk + OK));
+                    else
+                        return std::forward<P>(p);
+                }
+                template <int I, class E, int N>
+                __device__ __forceinline__ decltype(auto) operator()(cartesian::in_accessor<I, E, N> const &a) const {
+                    return bind(param<I>(), a);
+                }
+                template <int I, class E, int N>
+                __device__ __forceinline__ decltype(auto) operator()(cartesian::inout_accessor<I, E, N> const &a) const {
+                    return bind(param<I>(), a);
+                }
+                template <class X>
+                __device__ auto operand(X const &x) const {
+                    if constexpr (std::is_arithmetic<X>::value)
+                        return x;
+                    else
+                        return std::decay_t<decltype((*this)(x))>((*this)(x));
+                }
+                template <class Op, class L, class R>
+                __device__ auto operator()(cartesian::expr2<Op, L, R> const &x) const {
+                    return Op()(operand(x.l), operand(x.r));
+                }
+                template <class Op, class A>
+                __device__ auto operator()(cartesian::expr1<Op, A> const &x) const {
+                    return Op()(operand(x.a));
+                }
+            };
+
+            template <class F, class Region, class E>
+            __device__ __forceinline__ void call_functor(E &ce) {
+                if constexpr (std::is_void<Region>::value)
+                    F::apply(ce);
+                else
+                    F::apply(ce, Region());
+            }
+        } // namespace engine_
+
+        namespace cartesian {
+            template <class F, class Region = void, class ReturnType = void, int OI = 0, int OJ = 0, int OK = 0>
+            struct call {
+                template <int I, int J, int K>
+                using at = call<F, Region, ReturnType, I, J, K>;
+                template <class R>
+                using return_type = call<F, Region, R, OI, OJ, OK>;
+
+                template <class Eval, class A0, class... Args>
+                __device__ __forceinline__ static auto with(Eval &eval, A0 &&a0, Args &&... args) {
+                    using out_t = engine_::out_index<typename F::param_list>;
+                    static_assert(out_t::count == 1, "call<>: the stencil function must have exactly one inout accessor");
+                    using deduced = std::conditional_t<engine_::is_accessor<std::decay_t<A0>>::value,
+                        std::decay_t<decltype(eval(a0))>, std::decay_t<A0>>;
+                    using Res = std::conditional_t<std::is_void<ReturnType>::value, deduced, ReturnType>;
+                    Res res{};
+                    using Tup = engine_::vtuple<A0 &&, Args &&...>;
+                    Tup t{{std::forward<A0>(a0)}, {std::forward<Args>(args)}...};
+                    engine_::call_eval<Eval, Tup, Res, out_t::value, OI, OJ, OK> ce{eval, t, &res};
+                    engine_::call_functor<F, Region>(ce);
+                    return res;
+                }
+            };
+
+            template <class F, class Region = void, int OI = 0, int OJ = 0, int OK = 0>
+            struct call_proc {
+                template <int I, int J, int K>
+                using at = call_proc<F, Region, I, J, K>;
+
+                template <class Eval, class... Args>
+                __device__ __forceinline__ static void with(Eval &eval, Args &&... args) {
+                    using Tup = engine_::vtuple<Args &&...>;
+                    Tup t{{std::forward<Args>(args)}...};
+                    engine_::call_eval<Eval, Tup, int, -1, OI, OJ, OK> ce{eval, t, nullptr};
+                    engine_::call_functor<F, Region>(ce);
+                }
+            };
+        } // namespace cartesian
+
         // run(spec, backend, grid, fields...) (stencil/frontend/run.hpp:201-250) for a spec lambda: the spec is called
         // with one placeholder per field argument and returns execute_*().stage(...) chains or a multi_pass of them.
         template <class Spec, class Be, class... Args,

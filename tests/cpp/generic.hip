@@ -200,6 +200,109 @@ namespace {
         report(name, same_interior(out, ref, g));
     }
 
+    // ------------------------------------------------------------------------------------- stencil functions
+    // horizontal_diffusion_fused.cpp:23-91: the flux and output functors call the laplacian / flux functors as
+    // stencil functions at shifted points; run as one stage
+    struct sf_lap {
+        using out = inout_accessor<0>;
+        using in = in_accessor<1, extent<-1, 1, -1, 1>>;
+        using param_list = make_param_list<out, in>;
+        template <class Eval>
+        GT_FUNCTION static void apply(Eval eval) {
+            using T = std::decay_t<decltype(eval(out()))>;
+            eval(out()) = T{4} * eval(in()) - (eval(in(1, 0)) + eval(in(0, 1)) + eval(in(-1, 0)) + eval(in(0, -1)));
+        }
+    };
+    struct sf_flx {
+        using out = inout_accessor<0>;
+        using in = in_accessor<1, extent<-1, 2, -1, 1>>;
+        using param_list = make_param_list<out, in>;
+        template <class Eval>
+        GT_FUNCTION static void apply(Eval eval) {
+            auto hi = call<sf_lap>::with(eval, in(1, 0));
+            auto lo = call<sf_lap>::with(eval, in(0, 0));
+            auto flx = hi - lo;
+            eval(out()) = flx * (eval(in(1, 0)) - eval(in(0, 0))) > 0 ? 0 : flx;
+        }
+    };
+    struct sf_fly {
+        using out = inout_accessor<0>;
+        using in = in_accessor<1, extent<-1, 1, -1, 2>>;
+        using param_list = make_param_list<out, in>;
+        template <class Eval>
+        GT_FUNCTION static void apply(Eval eval) {
+            auto hi = call<sf_lap>::with(eval, in(0, 1));
+            auto lo = call<sf_lap>::with(eval, in(0, 0));
+            auto fly = hi - lo;
+            eval(out()) = fly * (eval(in(0, 1)) - eval(in(0, 0))) > 0 ? 0 : fly;
+        }
+    };
+    struct sf_out {
+        using out = inout_accessor<0>;
+        using in = in_accessor<1, extent<-2, 2, -2, 2>>;
+        using coeff = in_accessor<2>;
+        using param_list = make_param_list<out, in, coeff>;
+        template <class Eval>
+        GT_FUNCTION static void apply(Eval eval) {
+            auto flx_hi = call<sf_flx>::with(eval, in(0, 0));
+            auto flx_lo = call<sf_flx>::with(eval, in(-1, 0));
+            auto fly_hi = call<sf_fly>::with(eval, in(0, 0));
+            auto fly_lo = call<sf_fly>::with(eval, in(0, -1));
+            eval(out()) = eval(in()) - eval(coeff()) * (flx_hi - flx_lo + fly_hi - fly_lo);
+        }
+    };
+    // call_proc and at<>: a procedure writing two outputs, called one point to the east
+    struct sf_pair {
+        using a = inout_accessor<0>;
+        using b = inout_accessor<1>;
+        using in = in_accessor<2, extent<0, 1, 0, 0>>;
+        using param_list = make_param_list<a, b, in>;
+        template <class Eval>
+        GT_FUNCTION static void apply(Eval eval) {
+            eval(a()) = 2 * eval(in());
+            eval(b()) = eval(in()) + 1;
+        }
+    };
+    struct sf_use_proc {
+        using out = inout_accessor<0>;
+        using in = in_accessor<1, extent<0, 1, 0, 0>>;
+        using param_list = make_param_list<out, in>;
+        template <class Eval>
+        GT_FUNCTION static void apply(Eval eval) {
+            double a = 0, b = 0;
+            call_proc<sf_pair>::at<1, 0, 0>::with(eval, a, b, in());
+            eval(out()) = a - b; // 2 in(i+1) - (in(i+1) + 1)
+        }
+    };
+
+    void stencil_functions(int ni, int nj, int nk) {
+        const int h = 2;
+        auto in = seeded<double>(ni + 4, nj + 4, nk, h, 42, 1, -1, 1);
+        auto coeff = seeded<double>(ni + 4, nj + 4, nk, h, 42, 2, 0, 0.25);
+        auto b = storage::builder<storage::gpu>.type<double>().dimensions(ni + 4, nj + 4, nk).halos(h, h, 0);
+        auto out = b.build(), out2 = b.build();
+        auto g = halo_grid(ni, nj, nk, h);
+        run_single_stage(sf_out(), gfx950<>(), g, out, in, coeff);
+        auto hin = mirror(in), hco = mirror(coeff);
+        std::vector<double> ref(hin.size(), 0.);
+        auto st = in->strides();
+        const size_t o = h + h * st[1];
+        gto_hdiff_f64(ref.data() + o, hin.data() + o, hco.data() + o, ni, nj, nk, st[1], st[2], 4);
+        char name[96];
+        std::snprintf(name, sizeof name, "stencil functions call<>: hdiff as one stage %dx%dx%d", ni, nj, nk);
+        report(name, same_interior(out, ref, g));
+        run_single_stage(sf_use_proc(), gfx950<>(), g, out2, in);
+        auto v = out2->const_host_view();
+        bool ok = true;
+        for (int k = 0; k < nk && ok; ++k)
+            for (int j = h; j < h + nj && ok; ++j)
+                for (int i = h; i < h + ni && ok; ++i) {
+                    const double x = hin[i + 1 + j * st[1] + k * st[2]];
+                    ok = v(i, j, k) == 2 * x - (x + 1);
+                }
+        report("call_proc<>::at<1,0,0> with value outputs", ok);
+    }
+
     // ------------------------------------------------------------------------------------- vertical advection
     using axis_t = axis<1, axis_config::offset_limit<3>>;
     using full_t = axis_t::full_interval;
@@ -650,6 +753,7 @@ int main(int argc, char **argv) {
         tridiagonal(12, 33, 6, true);
         tridiagonal(23, 11, 40, false);
         simple_hori_diff(12, 33, 61);
+        stencil_functions(23, 11, 43);
         positional_and_fusion();
         expandable_and_errors();
     } catch (std::exception const &e) {

@@ -20,4 +20,4 @@ def test_generic_engine_on_device():
     print(res.stdout)
     assert res.returncode == 0, res.stdout + res.stderr
     assert "[FAIL]" not in res.stdout
-    assert res.stdout.count("[ok]") >= 15
+    assert res.stdout.count("[ok]") >= 17
