@@ -272,8 +272,9 @@ namespace gt355 {
                 return;
             for (int p = 0; p < np; ++p) {
                 const copy_pair &c = tab.p[p];
-                const W v = ((const W *)c.in)[i * c.is[0] + j * c.is[1] + k * c.is[2]];
-                ((W *)c.out)[i * c.os[0] + j * c.os[1] + k * c.os[2]] = v;
+                // every value is read and written once: non-temporal on both sides
+                const W v = __builtin_nontemporal_load((const W *)c.in + (i * c.is[0] + j * c.is[1] + k * c.is[2]));
+                __builtin_nontemporal_store(v, (W *)c.out + (i * c.os[0] + j * c.os[1] + k * c.os[2]));
             }
         }
 
@@ -305,9 +306,9 @@ namespace gt355 {
                 for (int b = 0; b < 2; ++b)
 #pragma unroll
                     for (int c = 0; c < 2; ++c)
-                        fv[(a * 2 + b) * 2 + c] = fp[a * fx0 + b * fx1 + c * fx2];
+                        fv[(a * 2 + b) * 2 + c] = __builtin_nontemporal_load(fp + (a * fx0 + b * fx1 + c * fx2));
             const T *jp = jac + i + j * jsj + k * jsk;
-            const T jv[2] = {jp[0], jp[jq]};
+            const T jv[2] = {__builtin_nontemporal_load(jp), __builtin_nontemporal_load(jp + jq)};
             // sum over (a, b, c) for each quadrature point: independent of (I, J, K)
             T sum[2];
 #pragma unroll
@@ -329,7 +330,7 @@ namespace gt355 {
 #pragma unroll
                         for (int q = 0; q < 2; ++q)
                             res += tab.phi[((x * 2 + y) * 2 + z) * 2 + q] * jv[q] * sum[q];
-                        rp[x * rx0 + y * rx1 + z * rx2] = res / 8;
+                        __builtin_nontemporal_store(T(res / 8), rp + (x * rx0 + y * rx1 + z * rx2));
                     }
         }
 

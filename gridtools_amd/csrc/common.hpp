@@ -73,32 +73,38 @@ namespace gt355 {
     __device__ __forceinline__ rsrc_t make_rsrc(const void *base) {
         return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)0xFFFFFFFF, 0x00020000);
     }
-    template <class T>
+    // Cache policy of a buffer access (the instruction's cache-policy bits on gfx950): CPOL_NT marks a stream
+    // that is touched once -- non-temporal, so the L2 and the Infinity Cache do not keep it at the expense of
+    // lines that are re-read.  Measured on vadv FP64 256x256x80 (five input streams and one output stream):
+    // 54 us with the default policy, 41 us with non-temporal loads and stores.
+    constexpr int CPOL_NT = 2;
+    template <class T, int CPOL = 0>
     __device__ __forceinline__ T bload(rsrc_t r, uint32_t voff, uint32_t soff) {
         if constexpr (sizeof(T) == 4)
-            return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+            return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, CPOL));
         else
-            return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+            return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, CPOL));
     }
     template <class T>
     struct two {
         T a, b;
     };
     // two consecutive elements (columns i and i+1) in one load
-    template <class T>
+    template <class T, int CPOL = 0>
     __device__ __forceinline__ two<T> bload2(rsrc_t r, uint32_t voff, uint32_t soff) {
         if constexpr (sizeof(T) == 4)
-            return __builtin_bit_cast(two<T>, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+            return __builtin_bit_cast(two<T>, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, CPOL));
         else
-            return __builtin_bit_cast(two<T>, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+            return __builtin_bit_cast(two<T>, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, CPOL));
     }
-    template <class T>
+    template <class T, int CPOL = 0>
     __device__ __forceinline__ void bstore(rsrc_t r, T v, uint32_t voff, uint32_t soff) {
         if constexpr (sizeof(T) == 4)
-            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, voff, soff, 0);
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, voff, soff, CPOL);
         else
             __builtin_amdgcn_raw_buffer_store_b64(
-                __builtin_bit_cast(decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0)), v), r, voff, soff, 0);
+                __builtin_bit_cast(decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0)), v), r, voff, soff,
+                CPOL);
     }
 
 } // namespace gt355

@@ -7,27 +7,29 @@
 
 namespace gt355 {
     namespace {
-        // One row segment of V elements per thread; rows are the i-lines of every (j, k).
-        template <class T, int V>
+        // One row segment of V consecutive elements per lane, moved as one 16-byte vector when the row start is
+        // 16-byte aligned (ALIGNED; the launcher checks origins and strides), else element by element.  Lanes of a
+        // wavefront take consecutive segments, so one load instruction covers 1 KiB of a row.  Every value is read
+        // and written once: non-temporal on both sides.
+        template <class T, int V, bool ALIGNED>
         __global__ void __launch_bounds__(256) copy_kernel(T *__restrict__ out, const T *__restrict__ in, int ni,
             int nj, int nk, int64_t osj, int64_t osk, int64_t isj, int64_t isk, int segs) {
-            int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-            int64_t rows = (int64_t)nj * nk;
+            typedef T vec __attribute__((ext_vector_type(V)));
+            const int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+            const int64_t rows = (int64_t)nj * nk;
             if (n >= rows * segs)
                 return;
-            int seg = int(n % segs);
-            int64_t row = n / segs;
-            int j = int(row % nj), k = int(row / nj);
-            int i = seg * V;
+            const int seg = int(n % segs);
+            const int64_t row = n / segs;
+            const int j = int(row % nj), k = int(row / nj);
+            const int i = seg * V;
             const T *src = in + i + j * isj + k * isk;
             T *dst = out + i + j * osj + k * osk;
-            if (i + V <= ni) {
-#pragma unroll
-                for (int v = 0; v < V; ++v)
-                    dst[v] = __builtin_nontemporal_load(src + v);
+            if (ALIGNED && i + V <= ni) {
+                __builtin_nontemporal_store(__builtin_nontemporal_load((const vec *)src), (vec *)dst);
             } else {
-                for (int v = 0; v < ni - i; ++v)
-                    dst[v] = src[v];
+                for (int v = 0; v < V && i + v < ni; ++v)
+                    __builtin_nontemporal_store(__builtin_nontemporal_load(src + v), dst + v);
             }
         }
         // Strided 3-D copy with element conversion (one thread per point, i fastest).
@@ -90,16 +92,31 @@ extern "C" int gt355_copy(const gt355_field *out, const gt355_field *in, int ni,
     if ((rc = check_field(out, "out", dtype, 0, ni, 0, nj, 0, nk)))
         return rc;
     hipStream_t s = (hipStream_t)stream;
-    constexpr int V = 4;
-    int segs = (ni + V - 1) / V;
-    int64_t total = (int64_t)segs * nj * nk;
-    int64_t blocks = (total + 255) / 256;
+    // 16-byte segments when every row of both fields starts 16-byte aligned
+    const auto a16 = [](const gt355_field *f, int64_t es) {
+        return (uintptr_t)f->origin % 16 == 0 && (f->stride[1] * es) % 16 == 0 && (f->stride[2] * es) % 16 == 0;
+    };
+    const int64_t es = dtype == GT355_F64 ? 8 : 4;
+    const bool aligned = a16(in, es) && a16(out, es) && in->stride[0] == 1 && out->stride[0] == 1;
+    const int V = int(16 / es);
+    const int segs = (ni + V - 1) / V;
+    const int64_t total = (int64_t)segs * nj * nk;
+    const int64_t blocks = (total + 255) / 256;
     GT355_CHECK(blocks < (1LL << 31), "gt355_copy: domain too large");
-    if (dtype == GT355_F64)
-        copy_kernel<double, V><<<(int)blocks, 256, 0, s>>>((double *)out->origin, (const double *)in->origin, ni, nj,
-            nk, out->stride[1], out->stride[2], in->stride[1], in->stride[2], segs);
-    else
-        copy_kernel<float, V><<<(int)blocks, 256, 0, s>>>((float *)out->origin, (const float *)in->origin, ni, nj, nk,
-            out->stride[1], out->stride[2], in->stride[1], in->stride[2], segs);
+#define GT355_COPY(T, VV, AL)                                                                                    \
+    copy_kernel<T, VV, AL><<<(int)blocks, 256, 0, s>>>((T *)out->origin, (const T *)in->origin, ni, nj, nk,      \
+        out->stride[1], out->stride[2], in->stride[1], in->stride[2], segs)
+    if (dtype == GT355_F64) {
+        if (aligned)
+            GT355_COPY(double, 2, true);
+        else
+            GT355_COPY(double, 2, false);
+    } else {
+        if (aligned)
+            GT355_COPY(float, 4, true);
+        else
+            GT355_COPY(float, 4, false);
+    }
+#undef GT355_COPY
     return check_launch("gt355_copy");
 }

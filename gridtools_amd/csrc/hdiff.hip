@@ -174,6 +174,7 @@ namespace gt355 {
             ST reL[P], rfL[P], reR[P], rfR[P]; // wave-uniform rings: side columns (scalar loads)
             auto load = [&](int t) {
                 const int64_t r = row(t);
+                // in rows are re-read by the neighbouring strips (rim rows) and side columns: default cache policy
                 rc[t % P] = inp[r * isj + ic];
                 reL[t % P] = inp[r * isj + ieL];
                 rfL[t % P] = inp[r * isj + ifL];
@@ -182,7 +183,7 @@ namespace gt355 {
                 // coeff of the row that is output at ring step t, R-2 = j0-4+t (clamped into the interior; only
                 // steps 4..NR-1 use it)
                 const int64_t ro = min(max(j0 - 4 + t, 0), nj - 1);
-                rk[t % P] = cop[ro * csj + min(ic, ni - 1)];
+                rk[t % P] = __builtin_nontemporal_load(cop + ro * csj + min(ic, ni - 1)); // read once
             };
 #pragma clang loop unroll(full)
             for (int t = 0; t < P; ++t)

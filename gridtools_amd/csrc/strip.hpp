@@ -110,16 +110,16 @@ namespace gt355 {
             h_e1[1][s] = bload<ST>(r_in, v_e10, o);
             h_e1[2][s] = bload<ST>(r_in, v_e1p, o);
             h_e2[s] = bload<ST>(r_in, v_e2, o);
-            h_co[s] = bload<ST>(r_co, v_co, lvl(L) * csk_b);
+            h_co[s] = bload<ST, CPOL_NT>(r_co, v_co, lvl(L) * csk_b);
         };
         auto load_b = [&](int L, int s) {
             const uint32_t o = lvl(L) * sk_b;
-            b_uts[s] = bload<ST>(r_uts, v_col, o);
-            b_up[s] = bload<ST>(r_up, v_col, o);
-            b_ut[s] = bload<ST>(r_ut, v_col, o);
+            b_uts[s] = bload<ST, CPOL_NT>(r_uts, v_col, o);
+            b_up[s] = bload<ST, CPOL_NT>(r_up, v_col, o);
+            b_ut[s] = bload<ST, CPOL_NT>(r_ut, v_col, o);
         };
         auto load_a = [&](int L, int s) {
-            const two<ST> w = bload2<ST>(r_wc, v_wc, lvl(L) * wsk_b);
+            const two<ST> w = bload2<ST, CPOL_NT>(r_wc, v_wc, lvl(L) * wsk_b);
             a_wi[s] = w.a;
             a_wip[s] = w.b;
         };
@@ -148,11 +148,11 @@ namespace gt355 {
             if constexpr (NARROW)
                 res = lane_value(res, src_lane);
             const uint32_t o = lvl(L) * osk_b;
-            bstore<ST>(r_ust, ST(res), v_o, o);
+            bstore<ST, CPOL_NT>(r_ust, ST(res), v_o, o);
             if constexpr (LAP) {
                 if constexpr (NARROW)
                     lap_c = lane_value(lap_c, src_lane);
-                bstore<ST>(r_lap, ST(lap_c), v_o, o);
+                bstore<ST, CPOL_NT>(r_lap, ST(lap_c), v_o, o);
             }
             return res;
         };
@@ -230,7 +230,7 @@ namespace gt355 {
                     const CT *t = tail + (int64_t)(kb + s - NK) * 128;
                     rc[s] = t[lane];
                     rd[s] = t[64 + lane];
-                    rp[s] = CT(bload<ST>(r_up, v_col, lvl(kb + s) * sk_b));
+                    rp[s] = CT(bload<ST, CPOL_NT>(r_up, v_col, lvl(kb + s) * sk_b));
                 }
 #pragma clang loop unroll(full)
                 for (int s = D - 1; s >= 0; --s) {
@@ -238,7 +238,7 @@ namespace gt355 {
                     const CT dt = rd[s] - rc[s] * data;
                     const CT nd = k == top ? rd[s] : dt;
                     data = nd;
-                    bstore<ST>(r_uts, ST(dtr * (nd - rp[s])), v_col, lvl(k) * sk_b);
+                    bstore<ST, CPOL_NT>(r_uts, ST(dtr * (nd - rp[s])), v_col, lvl(k) * sk_b);
                 }
             }
         }
@@ -248,7 +248,7 @@ namespace gt355 {
             const CT dt = d[k] - c[k] * data;
             const CT nd = k == top ? d[k] : dt;
             data = nd;
-            bstore<ST>(r_uts, ST(dtr * (nd - v_up)), v_col, lvl(k) * sk_b);
+            bstore<ST, CPOL_NT>(r_uts, ST(dtr * (nd - v_up)), v_col, lvl(k) * sk_b);
             __builtin_amdgcn_sched_barrier(6);
         }
     }

@@ -137,16 +137,16 @@ namespace gt355 {
             ST b_uts[D], b_up[D], b_ut[D]; // ring B
             auto load_a = [&](int L) {
                 const uint32_t o = lvl(L) * sk_o, w = lvl(L) * wsk_o;
-                a_us[L % D] = bload<ST>(r_us, colb, o);
-                const two<ST> ww = bload2<ST>(r_wc, wcolb, w); // wcon at i and i+1
+                a_us[L % D] = bload<ST, CPOL_NT>(r_us, colb, o);
+                const two<ST> ww = bload2<ST, CPOL_NT>(r_wc, wcolb, w); // wcon at i and i+1
                 a_wi[L % D] = ww.a;
                 a_wip[L % D] = ww.b;
             };
             auto load_b = [&](int L) {
                 const uint32_t o = lvl(L) * sk_o;
-                b_uts[L % D] = bload<ST>(r_uts, colb, o);
-                b_up[L % D] = bload<ST>(rs_up, colb, o);
-                b_ut[L % D] = bload<ST>(r_ut, colb, o);
+                b_uts[L % D] = bload<ST, CPOL_NT>(r_uts, colb, o);
+                b_up[L % D] = bload<ST, CPOL_NT>(rs_up, colb, o);
+                b_ut[L % D] = bload<ST, CPOL_NT>(r_ut, colb, o);
             };
 #pragma clang loop unroll(full)
             for (int L = 0; L < D; ++L) {
@@ -156,7 +156,7 @@ namespace gt355 {
 
             __shared__ ST s_up[UPOS_LDS ? NK : 1][64];
             CT c[NK], d[NK];
-            CT us_m = CT(0), us_c = CT(bload<ST>(r_us, colb, 0)), w_c_i = CT(0), w_c_ip = CT(0);
+            CT us_m = CT(0), us_c = CT(bload<ST, CPOL_NT>(r_us, colb, 0)), w_c_i = CT(0), w_c_ip = CT(0);
 #pragma clang loop unroll(full)
             for (int k = 0; k < NK; ++k) {
                 if constexpr (UPOS_LDS)
@@ -200,7 +200,7 @@ namespace gt355 {
             if constexpr (!UPOS_LDS) {
 #pragma clang loop unroll(full)
                 for (int s = 0; s < D; ++s)
-                    r_up[(NK - 1 - s) % D] = bload<ST>(rs_up, colb, lvl(NK - 1 - s) * sk_o);
+                    r_up[(NK - 1 - s) % D] = bload<ST, CPOL_NT>(rs_up, colb, lvl(NK - 1 - s) * sk_o);
             }
             CT data = CT(0);
 #pragma clang loop unroll(full)
@@ -212,7 +212,7 @@ namespace gt355 {
                     v_up = CT(r_up[k % D]);
                     pin();
                     if (k - D >= 0)
-                        r_up[k % D] = bload<ST>(rs_up, colb, lvl(k - D) * sk_o);
+                        r_up[k % D] = bload<ST, CPOL_NT>(rs_up, colb, lvl(k - D) * sk_o);
                 }
                 // u_backward_function (vertical_advection_dycore.cpp:110-121): the top level starts the recurrence
                 const CT dt = d[k] - c[k] * data;
@@ -222,7 +222,7 @@ namespace gt355 {
                 // Unconditional store to the clamped level: a step above the top writes a value that the step at
                 // the top overwrites later in program order, and lanes past ni compute column ni-1 and write its
                 // bits again.  No branch, so the load counter stays exact.
-                bstore<ST>(r_uts, ST(out), colb, lvl(k) * sk_o);
+                bstore<ST, CPOL_NT>(r_uts, ST(out), colb, lvl(k) * sk_o);
                 __builtin_amdgcn_sched_barrier(SBM);
             }
         }
