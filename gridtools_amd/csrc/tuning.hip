@@ -32,9 +32,10 @@ namespace gt355 {
     }
 
     hdiff_cfg hdiff_cfg_for(int st_size, int ct_size, int ni, int nj, int nk) {
-        // measured defaults at 256x256x80: FP64 storage is fastest with the LDS tile 64x8 with 5-level chunks (2048
-        // workgroups, one full round at 8 waves per SIMD; 24.9 us), FP32 storage with the register sweep (17.1 us)
-        hdiff_cfg c = st_size == 8 ? hdiff_cfg{0, 64, 8, 5} : hdiff_cfg{1, 16, 4, 0};
+        // measured defaults (MI355X, x80 levels, autotuner winners): FP32 storage takes the column-pair sweep with
+        // 8-row strips (256x256: 13.5 us, 512x512: 41 us); FP64 storage the one-column sweep with 8-row strips at
+        // 256x256 (22.2 us) and the column-pair sweep from 512 columns on (88 us against 97 us)
+        hdiff_cfg c = st_size == 4 ? hdiff_cfg{2, 8, 4, 0} : ni >= 512 ? hdiff_cfg{2, 8, 4, 0} : hdiff_cfg{1, 8, 4, 0};
         {
             std::lock_guard<std::mutex> lock(g_tune_mutex);
             auto it = g_hdiff.find(tune_key{st_size, ct_size, ni, nj, nk});
@@ -43,8 +44,8 @@ namespace gt355 {
         }
         const int impl = tune_param("hdiff_impl", c.impl);
         if (impl != c.impl)
-            c = impl == 1 ? hdiff_cfg{1, 16, 4, 0} : hdiff_cfg{0, 64, 8, 5};
-        if (c.impl == 1) {
+            c = impl == 2 ? hdiff_cfg{2, 8, 4, 0} : impl == 1 ? hdiff_cfg{1, 16, 4, 0} : hdiff_cfg{0, 64, 8, 5};
+        if (c.impl >= 1) {
             c.a = tune_param("hdiff_jb", c.a);
             c.b = tune_param("hdiff_p", c.b);
         } else {

@@ -9,7 +9,7 @@ import ctypes
 from . import _abi
 from .stencils import NATIVE
 
-IMPL_NAMES = {0: "lds-tile", 1: "register-sweep"}
+IMPL_NAMES = {0: "lds-tile", 1: "register-sweep", 2: "register-sweep-pairs"}
 
 
 def hdiff(out, in_, coeff, grid, precision=NATIVE, reps=20, stream=None):

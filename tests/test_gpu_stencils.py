@@ -83,12 +83,12 @@ def test_hdiff_precision_form(gpu, oracle, ni, nj, nk):
     assert rel_err(got, ref64) <= MIXED_TOL
 
 
-@pytest.mark.parametrize("impl", [0, 1])
+@pytest.mark.parametrize("impl", [0, 1, 2])
 @pytest.mark.parametrize("dtype,precision", [(np.float64, 0), (np.float32, 0), (np.float32, 1)])
 @pytest.mark.parametrize("ni,nj,nk", [(12, 33, 5), (130, 70, 3), (256, 256, 4)])
 def test_hdiff_both_kernels_all_precisions(gpu, oracle, monkeypatch, impl, dtype, precision, ni, nj, nk):
-    """The LDS-tile kernel and the DPP register-sweep kernel are each the default for one storage type; both
-    must agree with the oracle for every precision (selected through the GT355_TUNE_hdiff_impl knob)."""
+    """The LDS-tile kernel, the DPP register sweep and its column-pair form are each picked by the tuner for some
+    shapes; all must agree with the oracle for every precision (selected through the GT355_TUNE_hdiff_impl knob)."""
     from gridtools_amd import stencils
 
     monkeypatch.setenv("GT355_TUNE_hdiff_impl", str(impl))
@@ -112,6 +112,26 @@ def test_hdiff_sweep_variants(gpu, oracle, monkeypatch, jb, p):
     stencils.hdiff(out, inp, coeff, grid)
     got = out.padded_numpy()[:nk, 2:2 + nj, 2:2 + ni]
     assert np.array_equal(got, oracle_hdiff(oracle, inp, coeff, 2, ni, nj, nk, "native"))
+
+
+@pytest.mark.parametrize("jb,p", [(8, 4), (8, 8), (16, 4), (16, 8)])
+@pytest.mark.parametrize("ni,nj", [(130, 37), (256, 20), (2, 9), (97, 11)])
+@pytest.mark.parametrize("dtype,precision", [(np.float64, 0), (np.float32, 1)])
+def test_hdiff_pair_sweep_variants(gpu, oracle, monkeypatch, jb, p, ni, nj, dtype, precision):
+    """Column-pair sweep shapes, with a partial last 128-column strip (130), exactly two strips (256), a single pair
+    (2) and an odd ni (97, which the launcher hands to the one-column sweep)."""
+    from gridtools_amd import stencils
+
+    monkeypatch.setenv("GT355_TUNE_hdiff_impl", "2")
+    monkeypatch.setenv("GT355_TUNE_hdiff_jb", str(jb))
+    monkeypatch.setenv("GT355_TUNE_hdiff_p", str(p))
+    nk = 3
+    print(f"[case] hdiff pairs jb={jb} p={p} {np.dtype(dtype).name} {ni}x{nj}x{nk}", flush=True)
+    inp, coeff, out, grid = hdiff_case(gpu, ni, nj, nk, dtype)
+    stencils.hdiff(out, inp, coeff, grid, precision=precision)
+    got = out.padded_numpy()[:nk, 2:2 + nj, 2:2 + ni]
+    mode = "mixed" if precision == 1 else "native"
+    assert np.array_equal(got, oracle_hdiff(oracle, inp, coeff, 2, ni, nj, nk, mode))
 
 
 def vadv_case(gt, ni, nj, nk, dtype, h=3):
@@ -250,7 +270,7 @@ def test_autotuner_picks_a_config_and_dispatch_stays_exact(gpu, oracle, tmp_path
     inp, coeff, out, grid = hdiff_case(gpu, ni, nj, nk, np.float64)
     cfg, us = tune.hdiff(out, inp, coeff, grid, reps=3)
     print(f"[tune] {ni}x{nj}x{nk} fp64 -> {cfg} {us:.2f} us", flush=True)
-    assert cfg["impl"] in ("lds-tile", "register-sweep") and us > 0
+    assert cfg["impl"] in ("lds-tile", "register-sweep", "register-sweep-pairs") and us > 0
     stencils.hdiff(out, inp, coeff, grid)  # now dispatched with the tuned configuration
     got = out.padded_numpy()[:nk, 2:2 + nj, 2:2 + ni]
     assert np.array_equal(got, oracle_hdiff(oracle, inp, coeff, 2, ni, nj, nk, "native"))
