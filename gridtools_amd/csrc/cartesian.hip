@@ -104,7 +104,7 @@ namespace gt355 {
                 reR[t % P] = inp[r * isj + ieR];
                 rfR[t % P] = inp[r * isj + ifR];
                 const int64_t rout = min(max(j0 - 4 + t, 0), nj - 1); // the row output at step t
-                rk[t % P] = cop[rout * csj + min(ic, ni - 1)];
+                rk[t % P] = __builtin_nontemporal_load(cop + rout * csj + min(ic, ni - 1)); // read once
                 const int64_t rm = min(max(j0 - 3 + t, -1), nj); // the row whose laplacian step t evaluates
                 if constexpr (CJ) {
                     ro[t % P] = rop[rm * ao1];
