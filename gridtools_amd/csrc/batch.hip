@@ -9,6 +9,12 @@
 //   u_stage = hdiff(in, coeff)                     (horizontal_diffusion.cpp:35-107)
 //   vadv(utens_stage, u_stage, wcon, u_pos, utens) (vertical_advection_dycore.cpp:32-150), u_stage taken from the
 //                                                 diffusion in the compute type, not re-read
+// The two-phase form runs the same chain in two persistent launches per timestep, each over every domain: the
+// stencil phase (laplacian and hdiff by the one-column register sweep of sweep.hpp, one wavefront per 64-column x
+// JB-row x level item; lap and u_stage stored) and the vertical phase (the strip sweep of strip.hpp with u_stage read
+// back instead of evaluated).  It evaluates each laplacian once instead of once per row that needs it, at the cost
+// of the u_stage round trip through memory; in the precision form u_stage reaches the vertical phase rounded to
+// FP32 (as in the two-launch composition), where the single-launch forms pass it on in FP64.
 // The k depth is a runtime value (up to 144 levels), so ccol/dcol go to the workgroup's own slot of a scratch
 // buffer (k-major, coalesced; written and read back by the same workgroup, so they stay in L2 / Infinity Cache).
 // Levels are processed in blocks of D with static ring slots inside a block; loads are unconditional (clamped to
@@ -21,6 +27,7 @@
 #include "common.hpp"
 #include "functors.hpp"
 #include "strip.hpp"
+#include "sweep.hpp"
 #include "tuning.hpp"
 
 namespace gt355 {
@@ -217,21 +224,26 @@ namespace gt355 {
         struct sitem {
             int32_t dom, i0, j, pad;
         };
-        constexpr int SD = 4;                       // load-ahead depth of the strip sweep
-        constexpr int STAIL_ROWS = BMAXK - 80 + SD; // tail levels a wavefront's scratch holds
+        constexpr int SD = 4; // load-ahead depth of the strip sweep
+        // tail levels a wavefront's scratch holds, for a register part of NK levels
+        constexpr int stail_rows(int nk_reg) { return BMAXK - nk_reg + SD; }
 
         // The batching form on the strip sweep (strip.hpp): one wavefront per workgroup and per item, no LDS
-        // tile, no barrier, ccol/dcol in registers for the first 80 levels and in the wavefront's tail scratch
-        // beyond.  One sweep variant serves every depth (a column shorter than 80 levels runs the register part
+        // tile, no barrier, ccol/dcol in registers for the first NK levels and in the wavefront's tail scratch
+        // beyond.  One sweep variant serves every depth (a column shorter than NK levels runs the register part
         // with its levels past the top clamped; separate shorter variants in the same kernel would share one
         // register allocation and spill).  Items are taken with a static stride (no counters), longest columns
         // first.
-        template <class ST, class CT>
-        __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
+        //   NK = 80, one wavefront per SIMD: the whole column of the baseline depth in registers;
+        //   NK = 32, two wavefronts per SIMD (ccol/dcol of 32 levels fit half the register file): every level
+        //   past 32 goes through the tail scratch, but no column runs clamped steps past its top (the batch's
+        //   depths start at 40) and two wavefronts hide each other's latency.
+        template <class ST, class CT, int NK, int WPE, bool HD>
+        __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
         batch_strip_kernel(const bdom *__restrict__ doms, const sitem *__restrict__ items, int nitems, CT dtr,
             CT *__restrict__ scratch, int nxcd) {
-            __shared__ ST s_up[80][64];
-            CT *tail = scratch + (int64_t)blockIdx.x * (STAIL_ROWS * 128);
+            __shared__ ST s_up[NK][64];
+            CT *tail = scratch + (int64_t)blockIdx.x * (stail_rows(NK) * 128);
             const int wid = xcd_remap(blockIdx.x, gridDim.x, nxcd);
             for (int it = wid; it < nitems; it += gridDim.x) {
                 const sitem item = items[it];
@@ -239,7 +251,32 @@ namespace gt355 {
                 const strip_fields<ST> f{(const ST *)dm.in, (const ST *)dm.coeff, (const ST *)dm.wcon,
                     (const ST *)dm.u_pos, (const ST *)dm.utens, (ST *)dm.u_stage, (ST *)dm.utens_stage,
                     (ST *)dm.lap, dm.sj, dm.sk, dm.sj, dm.sk, dm.sj, dm.sk, dm.sj, dm.sk, dm.wsj, dm.wsk};
-                hv_strip<ST, CT, 80, SD, true, true, true>(f, dtr, dm.ni, dm.nj, dm.nk, item.i0, item.j, s_up, tail);
+                hv_strip<ST, CT, NK, SD, true, true, true, HD>(f, dtr, dm.ni, dm.nj, dm.nk, item.i0, item.j, s_up,
+                    tail);
+            }
+        }
+
+        // Work item of the two-phase form's stencil phase: columns [i0, i0+64) x rows [j0, j0+JB) of level k.
+        struct hitem {
+            int32_t dom, i0, j0, k;
+        };
+
+        // The stencil phase: laplacian (stored as lap) and hdiff (stored as u_stage) of every domain, one register
+        // sweep per item, items taken by wavefronts with a static stride; the item order (domain, level, row block,
+        // strip) keeps the wavefronts in flight on neighbouring strips, whose rim rows they share through L2.
+        template <class ST, class CT, int JB, int P>
+        __global__ void __launch_bounds__(256) batch_hdiff_kernel(const bdom *__restrict__ doms,
+            const hitem *__restrict__ items, int nitems, int nxcd) {
+            const int nw = gridDim.x * 4;
+            const int w0 = __builtin_amdgcn_readfirstlane(xcd_remap(blockIdx.x, gridDim.x, nxcd) * 4 + (threadIdx.x >> 6));
+            for (int it = w0; it < nitems; it += nw) {
+                const hitem item = items[it];
+                const bdom dm = doms[item.dom];
+                // global-address-space views of the table's pointers (global_* rather than flat_* accesses)
+                const gptr<ST> ust = (gptr<ST>)dm.u_stage, lap = (gptr<ST>)dm.lap;
+                const gptr<const ST> in = (gptr<const ST>)dm.in, co = (gptr<const ST>)dm.coeff;
+                hdiff_sweep_wave<ST, CT, JB, P, true, gptr<ST>, gptr<const ST>>(ust, in, co, lap, dm.ni, dm.nj, dm.sj,
+                    dm.sk, dm.sj, dm.sk, dm.sj, dm.sk, item.i0, item.j0, item.k);
             }
         }
     } // namespace
@@ -249,10 +286,14 @@ using namespace gt355;
 
 struct gt355_batch {
     bdom *doms;    // device table
-    void *items;   // device table: bitem (tile form) or sitem (strip form)
+    void *items;   // device table: bitem (tile form) or sitem (strip and two-phase forms)
+    void *hitems;  // device table: hitem (the two-phase form's stencil phase)
     void *scratch; // ccol/dcol of every workgroup the grid may have, in the compute type
-    int impl;      // 1: 64x4 tiles, ccol/dcol in scratch; 2: 64-column strips, ccol/dcol in registers
+    int impl;      // 1: 64x4 tiles, ccol/dcol in scratch; 2: 64-column strips, ccol/dcol in registers;
+                   // 3: two phases (stencil sweep, then the strips' vertical advection)
+    int nk_reg;    // strip form: levels of ccol/dcol held in registers (80: one wavefront per SIMD; 32: two)
     int ndoms, nitems, dtype, precision, grid, gridmax, cus, rows;
+    int nhitems, hgrid, hjb; // the two-phase form's stencil phase: items, workgroups, rows per item
 };
 
 namespace gt355 {
@@ -262,7 +303,7 @@ namespace gt355 {
         // (2w per CU: 2 fills half the SIMDs, 4 all of them, 6 and 8 queue a second round)
         constexpr int BWG_MAX = 4;
         inline int batch_grid(const gt355_batch *bt, int w) {
-            return std::min<int>(bt->nitems, bt->cus * (bt->impl == 2 ? 2 * w : w));
+            return std::min<int>(bt->nitems, bt->cus * (bt->impl >= 2 ? (bt->nk_reg == 32 ? 4 : 2) * w : w));
         }
     } // namespace
 } // namespace gt355
@@ -278,10 +319,18 @@ extern "C" int gt355_batch_create(int precision, const gt355_batch_domain *domai
     GT355_CHECK(wg_per_cu >= 1 && wg_per_cu <= BWG_MAX, "gt355_batch_create: wg_per_cu %d (1..%d)", wg_per_cu,
         BWG_MAX);
     // the strip form (default) or the tile form (GT355_TUNE_batch_impl=1)
-    const int impl = tune_param("batch_impl", 2) == 1 ? 1 : 2;
+    // the form (GT355_TUNE_batch_impl): two-phase by default (measured fastest on MI355X: 64 domains of seed 2024,
+    // FP32 storage / FP64 compute, 2.41 ms per timestep against 2.73-2.87 ms for the single-launch strip form)
+    const int impl_p = tune_param("batch_impl", 3);
+    const int impl = impl_p == 1 ? 1 : impl_p == 3 ? 3 : 2;
+    // rows per stencil-phase item of the two-phase form (GT355_TUNE_batch_hjb: 8 or 16)
+    const int hjb = tune_param("batch_hjb", 16) == 8 ? 8 : 16;
+    // register depth of the strip form (GT355_TUNE_batch_nk: 80 or 32)
+    const int nk_reg = tune_param("batch_nk", 80) == 32 ? 32 : 80;
     std::vector<bdom> doms(ndomains);
     std::vector<bitem> items;
     std::vector<sitem> sitems;
+    std::vector<hitem> hitems;
     for (int d = 0; d < ndomains; ++d) {
         const gt355_batch_domain &b = domains[d];
         const int ni = b.ni, nj = b.nj, nk = b.nk;
@@ -319,7 +368,14 @@ extern "C" int gt355_batch_create(int precision, const gt355_batch_domain *domai
         m.nj = nj;
         m.nk = nk;
         m.tiles_i = (ni + BTI - 1) / BTI;
-        if (impl == 2) {
+        if (impl == 3) {
+            const int strips = (ni + 63) / 64, rowblocks = (nj + hjb - 1) / hjb;
+            for (int k = 0; k < nk; ++k)
+                for (int rb = 0; rb < rowblocks; ++rb)
+                    for (int si = 0; si < strips; ++si)
+                        hitems.push_back(hitem{d, si * 64, rb * hjb, k});
+        }
+        if (impl >= 2) {
             const int64_t lim = int64_t(1) << 32, es = dt == GT355_F64 ? 8 : 4;
             GT355_CHECK(((int64_t)(nk - 1) * m.sk + (int64_t)(nj + 3) * m.sj + ni + 4) * es < lim &&
                             ((int64_t)(nk - 1) * m.wsk + (int64_t)(nj + 1) * m.wsj + ni + 1) * es < lim,
@@ -344,15 +400,21 @@ extern "C" int gt355_batch_create(int precision, const gt355_batch_domain *domai
     GT355_HIP(hipGetDeviceProperties(&p, current_device()));
     gt355_batch *bt = new gt355_batch{};
     bt->impl = impl;
+    bt->nk_reg = nk_reg;
+    bt->hjb = hjb;
+    bt->nhitems = (int)hitems.size();
+    // the stencil phase: four wavefronts per workgroup, at most 8 workgroups per CU (one round at 8 wavefronts
+    // per SIMD), each wavefront striding through the items
+    bt->hgrid = (int)std::max<int64_t>(1, std::min<int64_t>(((int64_t)hitems.size() + 3) / 4, (int64_t)p.multiProcessorCount * 8));
     bt->ndoms = ndomains;
-    bt->nitems = impl == 2 ? (int)sitems.size() : (int)items.size();
+    bt->nitems = impl >= 2 ? (int)sitems.size() : (int)items.size();
     bt->dtype = dt;
     bt->precision = precision;
     bt->cus = p.multiProcessorCount;
     bt->grid = batch_grid(bt, wg_per_cu);
     bt->gridmax = batch_grid(bt, BWG_MAX); // scratch for any grid the tuner may pick
-    const size_t item_bytes = impl == 2 ? sizeof(sitem) * sitems.size() : sizeof(bitem) * items.size();
-    const void *item_src = impl == 2 ? (const void *)sitems.data() : (const void *)items.data();
+    const size_t item_bytes = impl >= 2 ? sizeof(sitem) * sitems.size() : sizeof(bitem) * items.size();
+    const void *item_src = impl >= 2 ? (const void *)sitems.data() : (const void *)items.data();
     hipError_t e = hipMalloc(&bt->doms, sizeof(bdom) * doms.size());
     if (e == hipSuccess)
         e = hipMalloc(&bt->items, item_bytes);
@@ -360,18 +422,23 @@ extern "C" int gt355_batch_create(int precision, const gt355_batch_domain *domai
         e = hipMemcpy(bt->doms, doms.data(), sizeof(bdom) * doms.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(bt->items, item_src, item_bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess && !hitems.empty())
+        e = hipMalloc(&bt->hitems, sizeof(hitem) * hitems.size());
+    if (e == hipSuccess && !hitems.empty())
+        e = hipMemcpy(bt->hitems, hitems.data(), sizeof(hitem) * hitems.size(), hipMemcpyHostToDevice);
     // each batch owns its scratch, so batches on different streams never share it
     int maxnk = 0;
     for (const bdom &m : doms)
         maxnk = std::max(maxnk, m.nk);
     bt->rows = maxnk + BPAST;
     const size_t ct = (dt == GT355_F64 || precision == GT355_WIDEN) ? 8 : 4;
-    const size_t per_wg = impl == 2 ? (size_t)STAIL_ROWS * 128 : 2 * (size_t)bt->rows * 256;
+    const size_t per_wg = impl >= 2 ? (size_t)stail_rows(nk_reg) * 128 : 2 * (size_t)bt->rows * 256;
     if (e == hipSuccess)
         e = hipMalloc(&bt->scratch, ct * per_wg * (size_t)bt->gridmax);
     if (e != hipSuccess) {
         (void)hipFree(bt->doms);
         (void)hipFree(bt->items);
+        (void)hipFree(bt->hitems);
         (void)hipFree(bt->scratch);
         delete bt;
         return hip_fail(e, "gt355_batch_create");
@@ -384,17 +451,52 @@ extern "C" int gt355_batch_run(gt355_batch *bt, double dtr_stage, void *stream) 
     GT355_CHECK(bt != nullptr, "gt355_batch_run: null batch");
     void *scr = bt->scratch;
     hipStream_t s = (hipStream_t)stream;
-    if (bt->impl == 2) {
+    if (bt->impl >= 2) {
         const sitem *it = (const sitem *)bt->items;
-        if (bt->dtype == GT355_F64)
-            batch_strip_kernel<double, double><<<bt->grid, 64, 0, s>>>(bt->doms, it, bt->nitems, dtr_stage,
-                (double *)scr, xcd_count());
-        else if (bt->precision == GT355_WIDEN)
-            batch_strip_kernel<float, double><<<bt->grid, 64, 0, s>>>(bt->doms, it, bt->nitems,
-                double(float(dtr_stage)), (double *)scr, xcd_count());
-        else
-            batch_strip_kernel<float, float><<<bt->grid, 64, 0, s>>>(bt->doms, it, bt->nitems, float(dtr_stage),
-                (float *)scr, xcd_count());
+        if (bt->impl == 3 && bt->nhitems > 0) {
+            const hitem *hi = (const hitem *)bt->hitems;
+#define GT355_BATCH_HDIFF(JB)                                                                                   \
+    if (bt->dtype == GT355_F64)                                                                                 \
+        batch_hdiff_kernel<double, double, JB, 4><<<bt->hgrid, 256, 0, s>>>(bt->doms, hi, bt->nhitems, xcd_count()); \
+    else if (bt->precision == GT355_WIDEN)                                                                      \
+        batch_hdiff_kernel<float, double, JB, 4><<<bt->hgrid, 256, 0, s>>>(bt->doms, hi, bt->nhitems, xcd_count());  \
+    else                                                                                                        \
+        batch_hdiff_kernel<float, float, JB, 4><<<bt->hgrid, 256, 0, s>>>(bt->doms, hi, bt->nhitems, xcd_count());
+            if (bt->hjb == 8) {
+                GT355_BATCH_HDIFF(8)
+            } else {
+                GT355_BATCH_HDIFF(16)
+            }
+#undef GT355_BATCH_HDIFF
+            int rc = check_launch("gt355_batch_run");
+            if (rc)
+                return rc;
+        }
+#define GT355_BATCH_STRIP(NK, WPE, HD)                                                                          \
+    if (bt->dtype == GT355_F64)                                                                                 \
+        batch_strip_kernel<double, double, NK, WPE, HD><<<bt->grid, 64, 0, s>>>(bt->doms, it, bt->nitems,          \
+            dtr_stage, (double *)scr, xcd_count());                                                             \
+    else if (bt->precision == GT355_WIDEN)                                                                      \
+        batch_strip_kernel<float, double, NK, WPE, HD><<<bt->grid, 64, 0, s>>>(bt->doms, it, bt->nitems,           \
+            double(float(dtr_stage)), (double *)scr, xcd_count());                                              \
+    else                                                                                                        \
+        batch_strip_kernel<float, float, NK, WPE, HD><<<bt->grid, 64, 0, s>>>(bt->doms, it, bt->nitems,            \
+            float(dtr_stage), (float *)scr, xcd_count());
+        const bool hd = bt->impl == 2;
+        if (bt->nk_reg == 32) {
+            if (hd) {
+                GT355_BATCH_STRIP(32, 2, true)
+            } else {
+                GT355_BATCH_STRIP(32, 2, false)
+            }
+        } else {
+            if (hd) {
+                GT355_BATCH_STRIP(80, 1, true)
+            } else {
+                GT355_BATCH_STRIP(80, 1, false)
+            }
+        }
+#undef GT355_BATCH_STRIP
         return check_launch("gt355_batch_run");
     }
     const bitem *it = (const bitem *)bt->items;
@@ -416,6 +518,7 @@ extern "C" int gt355_batch_destroy(gt355_batch *bt) {
         return GT355_OK;
     (void)hipFree(bt->doms);
     (void)hipFree(bt->items);
+    (void)hipFree(bt->hitems);
     (void)hipFree(bt->scratch);
     delete bt;
     return GT355_OK;

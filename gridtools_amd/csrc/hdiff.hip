@@ -21,6 +21,7 @@
 
 #include "common.hpp"
 #include "functors.hpp"
+#include "sweep.hpp"
 #include "tuning.hpp"
 
 namespace gt355 {
@@ -132,25 +133,12 @@ namespace gt355 {
             }
         }
 
-        // ---------------------------------------------------------------------------------------------------------
-        // Register sweep variant: no LDS, no barriers.
-        //
-        // One wavefront owns 64 consecutive columns [i0, i0+64) of one k-level and sweeps JB output rows in j.
-        // Rows of in live in registers (a three-row window), the i-neighbours of in and lap come from DPP
-        // wavefront shifts, and the two lanes at the wave edges take their out-of-wave neighbour from a "side"
-        // column the wave streams with scalar loads (columns i0-1 / i0+64, plus i0-2 / i0+65 for the side
-        // laplacian): the DPP bound control keeps the destination's old value for a source lane outside
-        // the wave, and that old value is exactly the side column.  Loads run P rows ahead through register
-        // rings; all loads are unconditional (clamped to the valid box) so the load counter stays exact.
-        // ---------------------------------------------------------------------------------------------------------
+        // Register sweep variant: one wavefront per 64 columns x JB rows x 1 level (sweep.hpp: hdiff_sweep_wave).
         template <class ST, class CT, int JB, int P>
         __global__ void __launch_bounds__(256) hdiff_sweep_kernel(ST *__restrict__ out, const ST *__restrict__ in,
             const ST *__restrict__ coeff, int ni, int nj, int nk, int64_t osj, int64_t osk, int64_t isj, int64_t isk,
             int64_t csj, int64_t csk, int strips_i, int strips_j, int nblocks, int nxcd) {
-            constexpr int NR = JB + 4; // in rows j0-2 .. j0+JB+1
-            const int lane = threadIdx.x & 63;
-            // wave-uniform by construction; readfirstlane tells the compiler, so the side columns below become
-            // scalar loads
+            // wave-uniform by construction; readfirstlane tells the compiler, so the side columns become scalar loads
             const int wid = __builtin_amdgcn_readfirstlane(xcd_remap(blockIdx.x, nblocks, nxcd) * 4 + (threadIdx.x >> 6));
             const int total = strips_i * strips_j * nk;
             if (wid >= total)
@@ -160,92 +148,8 @@ namespace gt355 {
             const int si_ = wid % strips_i;
             const int sj_ = (wid / strips_i) % strips_j;
             const int k = wid / (strips_j * strips_i);
-            const int i0 = si_ * 64, j0 = sj_ * JB;
-            const bool left = lane < 32;
-            // columns, clamped to the readable box [-2, ni+2): own column per lane, side columns per wave
-            const int ic = min(i0 + lane, ni + 1);
-            const int ieL = i0 - 1, ifL = i0 - 2;
-            const int ieR = min(i0 + 64, ni + 1), ifR = min(i0 + 65, ni + 1);
-            const ST *inp = in + (int64_t)k * isk;
-            const ST *cop = coeff + (int64_t)k * csk;
-            auto row = [&](int t) { return (int64_t)min(j0 - 2 + t, nj + 1); }; // absolute row of ring step t
-
-            ST rc[P], rk[P];                 // per-lane rings: own column, coeff
-            ST reL[P], rfL[P], reR[P], rfR[P]; // wave-uniform rings: side columns (scalar loads)
-            auto load = [&](int t) {
-                const int64_t r = row(t);
-                // in rows are re-read by the neighbouring strips (rim rows) and side columns: default cache policy
-                rc[t % P] = inp[r * isj + ic];
-                reL[t % P] = inp[r * isj + ieL];
-                rfL[t % P] = inp[r * isj + ifL];
-                reR[t % P] = inp[r * isj + ieR];
-                rfR[t % P] = inp[r * isj + ifR];
-                // coeff of the row that is output at ring step t, R-2 = j0-4+t (clamped into the interior; only
-                // steps 4..NR-1 use it)
-                const int64_t ro = min(max(j0 - 4 + t, 0), nj - 1);
-                rk[t % P] = __builtin_nontemporal_load(cop + ro * csj + min(ic, ni - 1)); // read once
-            };
-#pragma clang loop unroll(full)
-            for (int t = 0; t < P; ++t)
-                load(t);
-
-            CT c0 = 0, c1 = 0, c2 = 0;       // in rows R-2, R-1, R of the own column
-            CT eL0 = 0, eL1 = 0, eL2 = 0;    // the same rows of column i0-1 (uniform)
-            CT eR0 = 0, eR1 = 0, eR2 = 0;    // the same rows of column i0+64 (uniform)
-            CT fL1 = 0, fR1 = 0;             // columns i0-2 and i0+65 of row R-1 (uniform)
-            CT m1 = 0, p1 = 0;               // in(i-1), in(i+1) of row R-1
-            CT m0 = 0, p0 = 0;               // in(i-1), in(i+1) of row R-2
-            CT lap1 = 0, lap0 = 0;           // lap of rows R-1 and R-2 (own column)
-            CT sl1 = 0, sl0 = 0;             // side laplacian of rows R-1 and R-2 (i0-1 on lane 0, i0+64 on lane 63)
-            CT fly_m = 0;                    // fly of row R-3
-            ST *orow = out + (int64_t)k * osk + (i0 + lane);
-            const bool col_ok = i0 + lane < ni;
-#pragma clang loop unroll(full)
-            for (int t = 0; t < NR; ++t) {
-                // shift the windows and take row R = j0 - 2 + t from the rings
-                c0 = c1; c1 = c2; c2 = CT(rc[t % P]);
-                eL0 = eL1; eL1 = eL2; eL2 = CT(reL[t % P]);
-                eR0 = eR1; eR1 = eR2; eR2 = CT(reR[t % P]);
-                const CT kout = CT(rk[t % P]); // coeff of row R-2
-                const CT fL_r = CT(rfL[t % P]), fR_r = CT(rfR[t % P]);
-                if (t + P < NR)
-                    load(t + P);
-                // in(i-1), in(i+1) of row R: lane 0 keeps column i0-1, lane 63 keeps column i0+64
-                const CT m2 = dpp_keep<DPP_WF_SR1>(eL2, c2), p2 = dpp_keep<DPP_WF_SL1>(eR2, c2);
-                lap0 = lap1;
-                sl0 = sl1;
-                if (t >= 2) {
-                    // lap of row R-1 (lap_function, horizontal_diffusion.cpp:41-46): needs rows R-2..R
-                    lap1 = lap_f<CT>(c1, p1, c2, m1, c0);
-                    // side laplacian at i0-1 (left lanes) or i0+64 (right lanes); the own column is the i+1
-                    // neighbour on the left side and the i-1 neighbour on the right side
-                    const CT e0 = left ? eL0 : eR0, e1 = left ? eL1 : eR1, e2 = left ? eL2 : eR2;
-                    const CT ip = left ? c1 : fR1, im = left ? fL1 : c1;
-                    sl1 = lap_f<CT>(e1, ip, e2, im, e0);
-                }
-                if (t >= 4) {
-                    // output row R-2: flx, flx[i-1], fly, fly[j-1], out (horizontal_diffusion.cpp:56-60, 70-74, 86-90)
-                    const CT lm = dpp_keep<DPP_WF_SR1>(sl0, lap0), lp = dpp_keep<DPP_WF_SL1>(sl0, lap0);
-                    const CT flx_c = flux_f<CT>(lp, lap0, p0, c0);
-                    const CT flx_m = flux_f<CT>(lap0, lm, c0, m0);
-                    const CT fly_c = flux_f<CT>(lap1, lap0, c1, c0);
-                    const CT res = c0 - kout * (((flx_c - flx_m) + fly_c) - fly_m);
-                    fly_m = fly_c;
-                    const int r_out = j0 + t - 4;
-                    if (col_ok && r_out < nj)
-                        __builtin_nontemporal_store(static_cast<ST>(res), orow + (int64_t)r_out * osj);
-                } else if (t == 3) {
-                    // fly of row j0-1, the fly[j-1] of the first output row
-                    fly_m = flux_f<CT>(lap1, lap0, c1, c0);
-                }
-                // rows move down one: (R-1) -> R-2 for the next step's output, R -> R-1 for its laplacian
-                m0 = m1;
-                p0 = p1;
-                m1 = m2;
-                p1 = p2;
-                fL1 = fL_r;
-                fR1 = fR_r;
-            }
+            hdiff_sweep_wave<ST, CT, JB, P, false>(out, in, coeff, (ST *)nullptr, ni, nj, osj, osk, isj, isk, csj, csk,
+                si_ * 64, sj_ * JB, k);
         }
 
         // Column-pair register sweep: the register sweep above with two adjacent columns per lane, so one wavefront

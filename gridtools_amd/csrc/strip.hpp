@@ -51,7 +51,9 @@ namespace gt355 {
     // column must belong to exactly one wavefront.  NARROW: in a strip that ends past ni, lanes past ni take lane
     // ni-1's results and store the same bits to column ni-1 as that lane (ds_bpermute per level); callers pass
     // NARROW = false only when every strip is full (ni a multiple of 64).
-    template <class ST, class CT, int NK, int D, bool NARROW, bool LAP, bool TAIL>
+    // HD = false: the vertical advection alone, with u_stage read from memory (written beforehand by the batching
+    // form's stencil phase) instead of evaluated; nothing but utens_stage is stored.
+    template <class ST, class CT, int NK, int D, bool NARROW, bool LAP, bool TAIL, bool HD = true>
     __device__ __forceinline__ void hv_strip(const strip_fields<ST> &f, CT dtr, int ni, int nj, int nk, int i0,
         int j, ST (*s_up)[64], CT *tail) {
         static_assert(NK % D == 0, "tail blocks start on a ring-slot boundary");
@@ -92,6 +94,8 @@ namespace gt355 {
             asm volatile("" : "+s"(csk_b));
             asm volatile("" : "+s"(sk_b));
             asm volatile("" : "+s"(wsk_b));
+            if constexpr (!HD)
+                asm volatile("" : "+s"(osk_b));
         };
 
         // register rings, level L in slot L % D: hdiff inputs of level L (consumed at step L-1), vadv inputs of
@@ -100,6 +104,11 @@ namespace gt355 {
         ST b_uts[D], b_up[D], b_ut[D];
         ST a_wi[D], a_wip[D];
         auto load_h = [&](int L, int s) {
+            if constexpr (!HD) {
+                // u_stage of the lane's column (default cache policy: the stencil phase has just written it)
+                h_c[2][s] = bload<ST>(r_ust, v_o, lvl(L) * osk_b);
+                return;
+            }
             const uint32_t o = lvl(L) * isk_b;
             h_c[0][s] = bload<ST>(r_in, v_c0, o);
             h_c[1][s] = bload<ST>(r_in, v_c1, o);
@@ -127,6 +136,8 @@ namespace gt355 {
         // hdiff at level L for the lane's column (horizontal_diffusion.cpp:41-46, 56-60, 70-74, 86-90), inputs in
         // slot s; stored as u_stage (and lap), returned in the compute type
         auto hdiff_level = [&](int L, int s) -> CT {
+            if constexpr (!HD)
+                return CT(h_c[2][s]);
             const CT c0 = CT(h_c[0][s]), c1 = CT(h_c[1][s]), c2 = CT(h_c[2][s]), c3 = CT(h_c[3][s]),
                      c4 = CT(h_c[4][s]);
             const CT e1m = CT(h_e1[0][s]), e10 = CT(h_e1[1][s]), e1p = CT(h_e1[2][s]), e2 = CT(h_e2[s]);

@@ -1,0 +1,120 @@
+// The hdiff register sweep of one wavefront (horizontal_diffusion.cpp:35-107): used by the single-domain hdiff
+// kernel (hdiff.hip) and by the batching form's stencil phase (batch.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+#include "functors.hpp"
+
+namespace gt355 {
+    // Register sweep: no LDS, no barriers.
+    //
+    // One wavefront owns 64 consecutive columns [i0, i0+64) of one k-level and sweeps JB output rows in j.
+    // Rows of in live in registers (a three-row window), the i-neighbours of in and lap come from DPP
+    // wavefront shifts, and the two lanes at the wave edges take their out-of-wave neighbour from a "side"
+    // column the wave streams with scalar loads (columns i0-1 / i0+64, plus i0-2 / i0+65 for the side
+    // laplacian): the DPP bound control keeps the destination's old value for a source lane outside
+    // the wave, and that old value is exactly the side column.  Loads run P rows ahead through register
+    // rings; all loads are unconditional (clamped to the valid box) so the load counter stays exact.
+    // The wavefront's strip is columns [i0, i0+64) x rows [j0, j0+JB) of level k (i0, j0, k wave-uniform).  LAP also
+    // stores the laplacian of every output point (the lap ij-cache value there) into `lap`, laid out like `out`.
+    // PO / PI: the output and input pointer types (ST * and const ST * from kernel arguments; global-address-space
+    // pointers when they come from a table in memory, so that accesses are global_* rather than flat_*).
+    template <class ST, class CT, int JB, int P, bool LAP, class PO = ST *, class PI = const ST *>
+    __device__ __forceinline__ void hdiff_sweep_wave(PO __restrict__ out, PI __restrict__ in, PI __restrict__ coeff,
+        PO __restrict__ lap, int ni, int nj, int64_t osj, int64_t osk, int64_t isj, int64_t isk, int64_t csj,
+        int64_t csk, int i0, int j0, int k) {
+        constexpr int NR = JB + 4; // in rows j0-2 .. j0+JB+1
+        const int lane = threadIdx.x & 63;
+        const bool left = lane < 32;
+        // columns, clamped to the readable box [-2, ni+2): own column per lane, side columns per wave
+        const int ic = min(i0 + lane, ni + 1);
+        const int ieL = i0 - 1, ifL = i0 - 2;
+        const int ieR = min(i0 + 64, ni + 1), ifR = min(i0 + 65, ni + 1);
+        const PI inp = in + (int64_t)k * isk;
+        const PI cop = coeff + (int64_t)k * csk;
+        auto row = [&](int t) { return (int64_t)min(j0 - 2 + t, nj + 1); }; // absolute row of ring step t
+
+        ST rc[P], rk[P];                 // per-lane rings: own column, coeff
+        ST reL[P], rfL[P], reR[P], rfR[P]; // wave-uniform rings: side columns (scalar loads)
+        auto load = [&](int t) {
+            const int64_t r = row(t);
+            // in rows are re-read by the neighbouring strips (rim rows) and side columns: default cache policy
+            rc[t % P] = inp[r * isj + ic];
+            reL[t % P] = inp[r * isj + ieL];
+            rfL[t % P] = inp[r * isj + ifL];
+            reR[t % P] = inp[r * isj + ieR];
+            rfR[t % P] = inp[r * isj + ifR];
+            // coeff of the row that is output at ring step t, R-2 = j0-4+t (clamped into the interior; only
+            // steps 4..NR-1 use it)
+            const int64_t ro = min(max(j0 - 4 + t, 0), nj - 1);
+            rk[t % P] = __builtin_nontemporal_load(cop + ro * csj + min(ic, ni - 1)); // read once
+        };
+#pragma clang loop unroll(full)
+        for (int t = 0; t < P; ++t)
+            load(t);
+
+        CT c0 = 0, c1 = 0, c2 = 0;       // in rows R-2, R-1, R of the own column
+        CT eL0 = 0, eL1 = 0, eL2 = 0;    // the same rows of column i0-1 (uniform)
+        CT eR0 = 0, eR1 = 0, eR2 = 0;    // the same rows of column i0+64 (uniform)
+        CT fL1 = 0, fR1 = 0;             // columns i0-2 and i0+65 of row R-1 (uniform)
+        CT m1 = 0, p1 = 0;               // in(i-1), in(i+1) of row R-1
+        CT m0 = 0, p0 = 0;               // in(i-1), in(i+1) of row R-2
+        CT lap1 = 0, lap0 = 0;           // lap of rows R-1 and R-2 (own column)
+        CT sl1 = 0, sl0 = 0;             // side laplacian of rows R-1 and R-2 (i0-1 on lane 0, i0+64 on lane 63)
+        CT fly_m = 0;                    // fly of row R-3
+        const PO orow = out + (int64_t)k * osk + (i0 + lane);
+        const bool col_ok = i0 + lane < ni;
+#pragma clang loop unroll(full)
+        for (int t = 0; t < NR; ++t) {
+            // shift the windows and take row R = j0 - 2 + t from the rings
+            c0 = c1; c1 = c2; c2 = CT(rc[t % P]);
+            eL0 = eL1; eL1 = eL2; eL2 = CT(reL[t % P]);
+            eR0 = eR1; eR1 = eR2; eR2 = CT(reR[t % P]);
+            const CT kout = CT(rk[t % P]); // coeff of row R-2
+            const CT fL_r = CT(rfL[t % P]), fR_r = CT(rfR[t % P]);
+            if (t + P < NR)
+                load(t + P);
+            // in(i-1), in(i+1) of row R: lane 0 keeps column i0-1, lane 63 keeps column i0+64
+            const CT m2 = dpp_keep<DPP_WF_SR1>(eL2, c2), p2 = dpp_keep<DPP_WF_SL1>(eR2, c2);
+            lap0 = lap1;
+            sl0 = sl1;
+            if (t >= 2) {
+                // lap of row R-1 (lap_function, horizontal_diffusion.cpp:41-46): needs rows R-2..R
+                lap1 = lap_f<CT>(c1, p1, c2, m1, c0);
+                // side laplacian at i0-1 (left lanes) or i0+64 (right lanes); the own column is the i+1
+                // neighbour on the left side and the i-1 neighbour on the right side
+                const CT e0 = left ? eL0 : eR0, e1 = left ? eL1 : eR1, e2 = left ? eL2 : eR2;
+                const CT ip = left ? c1 : fR1, im = left ? fL1 : c1;
+                sl1 = lap_f<CT>(e1, ip, e2, im, e0);
+            }
+            if (t >= 4) {
+                // output row R-2: flx, flx[i-1], fly, fly[j-1], out (horizontal_diffusion.cpp:56-60, 70-74, 86-90)
+                const CT lm = dpp_keep<DPP_WF_SR1>(sl0, lap0), lp = dpp_keep<DPP_WF_SL1>(sl0, lap0);
+                const CT flx_c = flux_f<CT>(lp, lap0, p0, c0);
+                const CT flx_m = flux_f<CT>(lap0, lm, c0, m0);
+                const CT fly_c = flux_f<CT>(lap1, lap0, c1, c0);
+                const CT res = c0 - kout * (((flx_c - flx_m) + fly_c) - fly_m);
+                fly_m = fly_c;
+                const int r_out = j0 + t - 4;
+                if (col_ok && r_out < nj) {
+                    __builtin_nontemporal_store(static_cast<ST>(res), orow + (int64_t)r_out * osj);
+                    if constexpr (LAP)
+                        __builtin_nontemporal_store(static_cast<ST>(lap0), lap + (orow - out) + (int64_t)r_out * osj);
+                }
+            } else if (t == 3) {
+                // fly of row j0-1, the fly[j-1] of the first output row
+                fly_m = flux_f<CT>(lap1, lap0, c1, c0);
+            }
+            // rows move down one: (R-1) -> R-2 for the next step's output, R -> R-1 for its laplacian
+            m0 = m1;
+            p0 = p1;
+            m1 = m2;
+            p1 = p2;
+            fL1 = fL_r;
+            fR1 = fR_r;
+        }
+    }
+
+} // namespace gt355

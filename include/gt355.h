@@ -261,18 +261,23 @@ int gt355_graph_launch(void *graph_exec, void *stream);
 int gt355_graph_destroy(void *graph_exec);
 
 /* ------------------------------------------------------------------ batching
- * Many independent domains of uneven sizes in one persistent launch.  Replaces looping the reference's run()
- * (stencil/frontend/run.hpp:34-60) over domains, and generalises expandable_run
+ * Many independent domains of uneven sizes, every domain in each persistent launch.  Replaces looping the
+ * reference's run() (stencil/frontend/run.hpp:34-60) over domains, and generalises expandable_run
  * (stencil/frontend/expandable_run.hpp:80-183), which batches fields of ONE grid size only.  Per domain and level
  * the batch computes lap = laplacian(in) (laplacian.cpp:20-29), u_stage = hdiff(in, coeff) and
- * utens_stage = vadv(utens_stage, u_stage, wcon, u_pos, utens) with u_stage taken from the diffusion in the
- * compute type; results equal gt355_laplacian + gt355_hdiff_vadv bit for bit.  Domains: ni, nj >= 1,
- * 2 <= nk <= 144; `in` readable on [-2, n+2) in i and j, wcon on [0, ni+1) in i; every field but wcon shares
- * the layout of `in`.  All domains have the dtype of domains[0].in.  gt355_batch_create builds the work table
- * (one item per domain and 64x4 column tile, longest columns first) and the per-workgroup scratch;
- * gt355_batch_run launches one persistent grid of cu_count * wg_per_cu workgroups (no counters or memsets, so
- * it may be captured into a graph once the batch exists).  Each batch owns its ccol/dcol scratch (workgroups x
- * 2 x (deepest nk + 8) x 256 values of the compute type): two runs of ONE batch must not overlap in time, runs of
+ * utens_stage = vadv(utens_stage, u_stage, wcon, u_pos, utens).  Domains: ni, nj >= 1, 2 <= nk <= 144; `in`
+ * readable on [-2, n+2) in i and j, wcon on [0, ni+1) in i; every field but wcon shares the layout of `in`.  All
+ * domains have the dtype of domains[0].in.  Forms (chosen at create time, GT355_TUNE_batch_impl):
+ *   3 (default), two phases: one persistent launch of the laplacian and hdiff register sweep over every domain's
+ *     (64-column strip, row block, level) items, then one of the vertical advection over every domain's 64-column
+ *     strips, u_stage read back from memory; results equal gt355_laplacian + gt355_hdiff_vadv's two-launch
+ *     composition bit for bit (in the precision form u_stage reaches the vertical phase rounded to FP32);
+ *   2, one launch: strips evaluate hdiff and the vertical advection together, u_stage passed on in the compute
+ *     type; results equal gt355_laplacian + the fused gt355_hdiff_vadv bit for bit;
+ *   1, one launch over 64x4 column tiles with ccol/dcol in scratch (kept for comparison).
+ * gt355_batch_create builds the work tables (longest columns first) and the per-workgroup scratch; gt355_batch_run
+ * launches the persistent grids (no counters or memsets, so a run may be captured into a graph once the batch
+ * exists).  Each batch owns its ccol/dcol scratch: two runs of ONE batch must not overlap in time, runs of
  * different batches may. */
 typedef struct gt355_batch_domain {
     gt355_field in, coeff, lap, u_stage, utens_stage, wcon, u_pos, utens;

@@ -17,6 +17,8 @@ SIZES = [(23, 11, 43), (70, 9, 81), (130, 37, 2), (64, 4, 137), (65, 5, 96), (48
 WIDE = [(512, 256, 40), (97, 300, 61)]
 # strip-form edge cases: ni of exactly one strip, one column past a strip, depths 78..81 around the register part
 NARROW = [(64, 3, 80), (65, 2, 78), (128, 2, 81), (63, 4, 79), (2, 2, 137)]
+# depths around a 32-level register part
+NARROW32 = [(64, 3, 32), (65, 2, 31), (128, 2, 33), (63, 4, 36), (30, 5, 2)]
 
 
 def oracle_domain(oracle, host, ni, nj, nk, dtr, mode):
@@ -31,6 +33,22 @@ def oracle_domain(oracle, host, ni, nj, nk, dtr, mode):
                 dtr, ni, nj, nk, mode=mode, threads=8)
     box = np.s_[:nk, H:H + nj, H:H + ni]
     return lap[box], ust[box], uts[box]
+
+
+def oracle_domain_rounded_ustage(oracle, host, ni, nj, nk, dtr):
+    """The precision form of the two-phase batch: lap and u_stage are the FP64 chain on the FP32-rounded inputs,
+    each rounded once; the vertical phase reads u_stage back rounded to FP32, so vadv runs in FP64 on that."""
+    V = oracle.View
+    lap = np.zeros_like(host["in_"])
+    ust = np.zeros_like(host["in_"])
+    oracle.laplacian(V(lap, H, H), V(host["in_"], H, H), ni, nj, nk)
+    oracle.hdiff(V(ust, H, H), V(host["in_"], H, H), V(host["coeff"], H, H), ni, nj, nk, mode="native", threads=8)
+    ust_r = ust.astype(np.float32).astype(np.float64)
+    uts = host["utens_stage"].copy()
+    oracle.vadv(V(uts, H, H), V(ust_r, H, H), V(host["wcon"], H, H), V(host["u_pos"], H, H), V(host["utens"], H, H),
+                dtr, ni, nj, nk, mode="native", threads=8)
+    box = np.s_[:nk, H:H + nj, H:H + ni]
+    return tuple(r[box].astype(np.float32) for r in (lap, ust, uts))
 
 
 def gpu_domain(m):
@@ -62,7 +80,7 @@ def make_members(gt, sizes, dtype, seed=11):
     return members
 
 
-def check_batch(gt, oracle, sizes, dtype, precision, wg_per_cu):
+def check_batch(gt, oracle, sizes, dtype, precision, wg_per_cu, rounded_ustage=False):
     from gridtools_amd.batch import Batch
 
     print(f"[case] batch {np.dtype(dtype).name} precision={precision} wg_per_cu={wg_per_cu} {sizes}", flush=True)
@@ -77,7 +95,10 @@ def check_batch(gt, oracle, sizes, dtype, precision, wg_per_cu):
     dtr = 0.15 if dtype == np.float64 else float(np.float32(0.15))
     for (ni, nj, nk), m, host in zip(sizes, members, hosts):
         got = gpu_domain(m)
-        if precision == 1:
+        if precision == 1 and rounded_ustage:
+            ref = oracle_domain_rounded_ustage(oracle, {k: v.astype(np.float64) for k, v in host.items()}, ni, nj,
+                                               nk, dtr)
+        elif precision == 1:
             # FP64 chain on the FP32-rounded inputs, each output rounded once
             ref = oracle_domain(oracle, {k: v.astype(np.float64) for k, v in host.items()}, ni, nj, nk, dtr,
                                 "native")
@@ -98,6 +119,31 @@ def test_batch_uneven_domains_bit_exact(gpu, oracle, monkeypatch, dtype, precisi
     check_batch(gpu, oracle, SIZES + NARROW, dtype, precision, wg_per_cu=2)
 
 
+@pytest.mark.parametrize("nk_reg", [80, 32])
+@pytest.mark.parametrize("hjb", [16, 8])
+@pytest.mark.parametrize("dtype,precision", [(np.float64, 0), (np.float32, 1), (np.float32, 0)])
+def test_batch_two_phase_bit_exact(gpu, oracle, monkeypatch, dtype, precision, hjb, nk_reg):
+    """impl 3: the stencil phase (one-column register sweep over every domain's items) then the vertical phase
+    (strip sweep reading u_stage back), with both register depths of the vertical phase and both item heights."""
+    monkeypatch.setenv("GT355_TUNE_batch_impl", "3")
+    monkeypatch.setenv("GT355_TUNE_batch_hjb", str(hjb))
+    monkeypatch.setenv("GT355_TUNE_batch_nk", str(nk_reg))
+    print(f"[case] batch two-phase hjb={hjb} nk_reg={nk_reg} {np.dtype(dtype).name} prec={precision}", flush=True)
+    check_batch(gpu, oracle, SIZES + NARROW + NARROW32, dtype, precision, wg_per_cu=2, rounded_ustage=True)
+    check_batch(gpu, oracle, WIDE + SIZES[:3], dtype, precision, wg_per_cu=1, rounded_ustage=True)
+
+
+@pytest.mark.parametrize("dtype,precision", [(np.float64, 0), (np.float32, 1), (np.float32, 0)])
+def test_batch_strip_32_register_levels_bit_exact(gpu, oracle, monkeypatch, dtype, precision):
+    """The strip form with 32 levels of ccol/dcol in registers (two wavefronts per SIMD): every level past 32 goes
+    through the tail scratch; the depths cover 2..144, around 32 and around 80."""
+    monkeypatch.setenv("GT355_TUNE_batch_impl", "2")
+    monkeypatch.setenv("GT355_TUNE_batch_nk", "32")
+    print(f"[case] batch nk_reg=32 {np.dtype(dtype).name} prec={precision}", flush=True)
+    check_batch(gpu, oracle, SIZES + NARROW + NARROW32, dtype, precision, wg_per_cu=2)
+    check_batch(gpu, oracle, WIDE + SIZES[:3], dtype, precision, wg_per_cu=1)
+
+
 @pytest.mark.parametrize("impl", [2, 1])
 @pytest.mark.parametrize("dtype,precision", [(np.float64, 0), (np.float32, 1)])
 def test_batch_more_items_than_workgroups(gpu, oracle, monkeypatch, dtype, precision, impl):
@@ -105,10 +151,14 @@ def test_batch_more_items_than_workgroups(gpu, oracle, monkeypatch, dtype, preci
     check_batch(gpu, oracle, WIDE + SIZES[:3], dtype, precision, wg_per_cu=1)
 
 
-def test_batch_matches_single_domain_entry_points(gpu, monkeypatch):
-    """On the GPU the batch equals gt355_laplacian + gt355_hdiff_vadv run per domain with the fused composition
-    (nk <= 80), which also keeps u_stage in FP64 between the stages in the precision form."""
-    monkeypatch.setenv("GT355_TUNE_hv_fused", "1")
+@pytest.mark.parametrize("impl", [3, 2])
+def test_batch_matches_single_domain_entry_points(gpu, monkeypatch, impl):
+    """On the GPU the batch equals gt355_laplacian + gt355_hdiff_vadv run per domain: the two-phase form (impl 3)
+    the two-launch composition (u_stage stored in FP32 between the launches in the precision form), the
+    single-launch strip form (impl 2) the fused composition (nk <= 80), which keeps u_stage in FP64."""
+    monkeypatch.setenv("GT355_TUNE_batch_impl", str(impl))
+    if impl == 2:
+        monkeypatch.setenv("GT355_TUNE_hv_fused", "1")
     from gridtools_amd import stencils
     from gridtools_amd.batch import Batch
 
@@ -135,7 +185,8 @@ def test_batch_timesteps_graph_replays_match_eager(gpu, oracle):
 
     sizes = [(40, 30, 50), (90, 17, 96), (64, 64, 41)]
     eager = BatchedTimesteps(sizes, seed=5, halo=H)
-    # step 1 against the oracle (FP64 chain on the FP32 inputs, rounded once)
+    # step 1 against the oracle: the FP64 chain on the FP32 inputs with every output rounded once, the vertical
+    # phase reading u_stage back in FP32 (the default two-phase form)
     host = []
     for f in eager.domains:
         h = {n: f[n].padded_numpy().astype(np.float64) for n in ("coeff", "utens_stage", "u_pos", "utens", "wcon")}
@@ -144,10 +195,10 @@ def test_batch_timesteps_graph_replays_match_eager(gpu, oracle):
     eager.step()
     gpu.synchronize()
     for d, ((ni, nj, nk), h) in enumerate(zip(sizes, host)):
-        _, ref_u, ref_t = oracle_domain(oracle, h, ni, nj, nk, float(np.float32(0.15)), "native")
+        _, ref_u, ref_t = oracle_domain_rounded_ustage(oracle, h, ni, nj, nk, float(np.float32(0.15)))
         box = np.s_[:nk, H:H + nj, H:H + ni]
-        assert np.array_equal(eager.current(d).padded_numpy()[box], ref_u.astype(np.float32))
-        assert np.array_equal(eager.domains[d]["utens_stage"].padded_numpy()[box], ref_t.astype(np.float32))
+        assert np.array_equal(eager.current(d).padded_numpy()[box], ref_u)
+        assert np.array_equal(eager.domains[d]["utens_stage"].padded_numpy()[box], ref_t)
     twin = BatchedTimesteps(sizes, seed=5, halo=H)
     graphed = BatchedTimesteps(sizes, seed=5, halo=H)
     st = Stream()
