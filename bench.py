@@ -409,8 +409,9 @@ def gt_sync():
 
 def batch_leg(gt, nsteps=10, reps=3, seed=2024):
     """The batching form: 64 domains, ni, nj ~ U{48..384}, nk ~ U{40..137} (seeded), laplacian + hdiff + vadv in
-    FP32 storage / FP64 compute, one persistent launch per timestep; the grid size from the batch tuner; a graph
-    of nsteps timesteps (u_stage -> in swap) replayed `reps` times."""
+    FP32 storage / FP64 compute, in the batch's default two-phase form (per timestep one persistent launch of the
+    stencil phase and one of the vertical phase, each over every domain); the grid size from the batch tuner; a
+    graph of nsteps timesteps (u_stage -> in swap) replayed `reps` times."""
     from gridtools_amd.batch import BatchedTimesteps, draw_sizes
     from gridtools_amd.timing import Event, Stream
 
@@ -435,7 +436,8 @@ def batch_leg(gt, nsteps=10, reps=3, seed=2024):
     # lap, u_stage, utens_stage written
     gbs = 9 * 4 * pts * nsteps / (ms * 1e-3) / 1e9
     return {
-        "workload": f"64 domains (seed {seed}), {pts / 1e6:.1f} M points, {nsteps}-timestep graph",
+        "workload": f"64 domains (seed {seed}), {pts / 1e6:.1f} M points, {nsteps}-timestep graph, two-phase form "
+                    "(stencil-phase + vertical-phase launch per timestep, each over every domain)",
         "ms_per_graph": round(ms, 4),
         "ms_per_timestep": round(ms / nsteps, 4),
         "gpts_per_s": round(pts * nsteps / (ms * 1e-3) / 1e9, 3),
