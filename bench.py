@@ -642,6 +642,9 @@ def main():
         "config": {"workload": "hdiff + vadv FP64, 256x256x80 interior (halo 2 / 3), one launch each per step",
                    "field_sets": args.sets, "parallelism": "single GPU"},
         "kernels": {"hdiff_fp64": hd64, "vadv_fp64": va64},
+        # the whole step: hdiff's 24 B/point and vadv's 48 B/point of algorithmic traffic over the step time
+        "step_fp64": {"algo_bytes": 72 * P, "achieved_gbs": round(72 * P / (ms_per_step * 1e-3) / 1e9, 1),
+                      "hbm_frac": round(72 * P / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "autotuned_hdiff": tuned,
         "fp32_form": fp32,
         "copy_fp64": copy,
