@@ -15,10 +15,10 @@ namespace gt355 {
     X(32, 8, 8) X(32, 16, 8)
 
     // Every register-sweep shape instantiated for hdiff: X(JB, P).
-#define GT355_HDIFF_SWEEPS(X) X(8, 4) X(16, 4) X(16, 8) X(32, 4) X(32, 8)
+#define GT355_HDIFF_SWEEPS(X) X(4, 4) X(8, 4) X(16, 4) X(16, 8) X(32, 4) X(32, 8)
 
     // Every column-pair sweep shape instantiated for hdiff: X(JB, P).
-#define GT355_HDIFF_SWEEPS2(X) X(8, 4) X(8, 8) X(16, 4) X(16, 8)
+#define GT355_HDIFF_SWEEPS2(X) X(4, 4) X(8, 4) X(8, 8) X(16, 4) X(16, 8)
 
     // One hdiff launch configuration: impl 0 = LDS tile (a, b, c) = (TI, TJ, KC); impl 1 = register sweep
     // (a, b) = (JB, P); impl 2 = column-pair register sweep (a, b) = (JB, P).

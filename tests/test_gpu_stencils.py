@@ -100,7 +100,7 @@ def test_hdiff_both_kernels_all_precisions(gpu, oracle, monkeypatch, impl, dtype
     assert np.array_equal(got, oracle_hdiff(oracle, inp, coeff, 2, ni, nj, nk, mode))
 
 
-@pytest.mark.parametrize("jb,p", [(8, 4), (16, 8), (32, 4), (32, 8)])
+@pytest.mark.parametrize("jb,p", [(4, 4), (8, 4), (16, 8), (32, 4), (32, 8)])
 def test_hdiff_sweep_variants(gpu, oracle, monkeypatch, jb, p):
     from gridtools_amd import stencils
 
@@ -114,7 +114,7 @@ def test_hdiff_sweep_variants(gpu, oracle, monkeypatch, jb, p):
     assert np.array_equal(got, oracle_hdiff(oracle, inp, coeff, 2, ni, nj, nk, "native"))
 
 
-@pytest.mark.parametrize("jb,p", [(8, 4), (8, 8), (16, 4), (16, 8)])
+@pytest.mark.parametrize("jb,p", [(4, 4), (8, 4), (8, 8), (16, 4), (16, 8)])
 @pytest.mark.parametrize("ni,nj", [(130, 37), (256, 20), (2, 9), (97, 11)])
 @pytest.mark.parametrize("dtype,precision", [(np.float64, 0), (np.float32, 1)])
 def test_hdiff_pair_sweep_variants(gpu, oracle, monkeypatch, jb, p, ni, nj, dtype, precision):
