@@ -237,7 +237,8 @@ int gt355_event_elapsed_ms(void *start, void *stop, float *ms);
 /* ------------------------------------------------------------------ autotuner
  * No counterpart in the reference, whose tile shape is a backend template parameter fixed at compile time
  * (stencil/gpu/entry_point.hpp:147-150, gpu<64,8,1> by default).  gt355_tune_hdiff times every instantiated hdiff
- * configuration (LDS tiles TI x TJ x KC and register sweeps JB/P) on the given fields, `reps` launches each on
+ * configuration (impl 0: LDS tiles TI x TJ x KC; impl 1: one-column register sweeps JB/P; impl 2: column-pair
+ * register sweeps JB/P, for an even ni and 2-element-aligned rows) on the given fields, `reps` launches each on
  * `stream` back to back, cycling through temporary copies of the three fields so that the data exceeds the MALL
  * (as a caller streaming several field sets sees it), records the fastest for (storage type, compute type, ni, nj,
  * nk) so later gt355_hdiff calls with that
