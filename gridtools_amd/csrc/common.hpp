@@ -21,6 +21,9 @@ namespace gt355 {
     // Number of XCDs the launch is spread over (workgroups are dispatched round-robin over XCDs).
     int xcd_count();
 
+    // Number of compute units of the current device (cached on first use).
+    int cu_count();
+
     inline int hip_fail(hipError_t e, const char *what) {
         set_error("%s: %s", what, hipGetErrorString(e));
         return GT355_EHIP;

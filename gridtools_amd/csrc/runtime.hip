@@ -41,6 +41,18 @@ namespace gt355 {
         return d;
     }
 
+    int cu_count() {
+        static int cached = -1;
+        if (cached > 0)
+            return cached;
+        hipDeviceProp_t p;
+        int n = 1;
+        if (hipGetDeviceProperties(&p, current_device()) == hipSuccess && p.multiProcessorCount > 0)
+            n = p.multiProcessorCount;
+        cached = n;
+        return n;
+    }
+
     int xcd_count() {
         // MI355X (gfx950) and MI300 (gfx942) spread 256/304 CUs over 8 XCDs.
         static int cached = -1;

@@ -227,6 +227,127 @@ namespace gt355 {
             }
         }
 
+        // Persistent form of vadv_onchip_kernel (u_pos parked in LDS) for domains with more 64-column strips than
+        // one wavefront per SIMD covers at once: a grid of one wavefront per SIMD walks the strips (64 columns of one
+        // row; the strips of a row consecutive) with a static stride, so the wavefronts in flight stay on
+        // neighbouring strips of the same levels.  Each wavefront issues the next strip's first D levels of loads
+        // between the current strip's forward and backward sweeps; the backward sweep only stores, so those loads
+        // are in flight while it runs and the next forward sweep starts without a pipeline fill (one wavefront per
+        // SIMD has nothing else to hide it).  The column sweep itself is vadv_onchip_kernel's, operation by
+        // operation.
+        template <class ST, class CT, int NK, int D>
+        __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
+        vadv_persist_kernel(ST *__restrict__ uts_f, const ST *__restrict__ us_f, const ST *__restrict__ wc_f,
+            const ST *__restrict__ up_f, const ST *__restrict__ ut_f, CT dtr, int ni, int nk, int64_t sj, int64_t sk,
+            int64_t wsj, int64_t wsk, int strips_i, int nitems, int nxcd) {
+            static_assert(D >= 1 && D <= NK, "prefetch depth");
+            const int top = nk - 1;
+            const rsrc_t r_us = make_rsrc(us_f), r_wc = make_rsrc(wc_f), rs_up = make_rsrc(up_f),
+                         r_ut = make_rsrc(ut_f), r_uts = make_rsrc(uts_f);
+            auto lvl = [&](int L) { return (uint32_t)min(L, top); };
+            uint32_t sk_o = (uint32_t)(sk * (int64_t)sizeof(ST)), wsk_o = (uint32_t)(wsk * (int64_t)sizeof(ST));
+            auto pin = [&]() {
+                asm volatile("" : "+s"(sk_o));
+                asm volatile("" : "+s"(wsk_o));
+            };
+            // the lane's byte offsets in strip `it` (lanes past ni take column ni-1)
+            auto offsets = [&](int it, uint32_t &cb, uint32_t &wb) {
+                const int si = it % strips_i, j = it / strips_i;
+                const int i = min(si * 64 + (int)threadIdx.x, ni - 1);
+                cb = (uint32_t)(((int64_t)j * sj + i) * (int64_t)sizeof(ST));
+                wb = (uint32_t)(((int64_t)j * wsj + i) * (int64_t)sizeof(ST));
+            };
+            ST a_us[D], a_wi[D], a_wip[D]; // ring A: u_stage, wcon at i and i+1 of level L, consumed at step L-1
+            ST b_uts[D], b_up[D], b_ut[D]; // ring B: consumed at step L
+            ST us0;                        // u_stage of level 0
+            auto load_a = [&](int L, uint32_t cb, uint32_t wb) {
+                const uint32_t o = lvl(L) * sk_o, w = lvl(L) * wsk_o;
+                a_us[L % D] = bload<ST, CPOL_NT>(r_us, cb, o);
+                const two<ST> ww = bload2<ST, CPOL_NT>(r_wc, wb, w);
+                a_wi[L % D] = ww.a;
+                a_wip[L % D] = ww.b;
+            };
+            auto load_b = [&](int L, uint32_t cb) {
+                const uint32_t o = lvl(L) * sk_o;
+                b_uts[L % D] = bload<ST, CPOL_NT>(r_uts, cb, o);
+                b_up[L % D] = bload<ST, CPOL_NT>(rs_up, cb, o);
+                b_ut[L % D] = bload<ST, CPOL_NT>(r_ut, cb, o);
+            };
+            auto prologue = [&](uint32_t cb, uint32_t wb) {
+                us0 = bload<ST, CPOL_NT>(r_us, cb, 0);
+#pragma clang loop unroll(full)
+                for (int L = 0; L < D; ++L) {
+                    load_b(L, cb);
+                    load_a(L + 1, cb, wb);
+                }
+            };
+            __shared__ ST s_up[NK][64];
+            int it = xcd_remap(blockIdx.x, gridDim.x, nxcd);
+            if (it >= nitems)
+                return; // a whole wavefront leaves; there is no barrier in this kernel
+            uint32_t colb, wcolb;
+            offsets(it, colb, wcolb);
+            prologue(colb, wcolb);
+            for (; it < nitems; it += gridDim.x) {
+                CT c[NK], d[NK];
+                CT us_m = CT(0), us_c = CT(us0), w_c_i = CT(0), w_c_ip = CT(0);
+#pragma clang loop unroll(full)
+                for (int k = 0; k < NK; ++k) {
+                    s_up[k][threadIdx.x] = b_up[k % D];
+                    const CT v_uts = CT(b_uts[k % D]), v_up = CT(b_up[k % D]), v_ut = CT(b_ut[k % D]);
+                    // top level: the k+1 wind taken as zero reproduces last_level exactly (see vadv_onchip_kernel)
+                    const bool at_top = k == top;
+                    const CT us_p = at_top ? us_c : CT(a_us[(k + 1) % D]);
+                    const CT w_p_i = at_top ? CT(0) : CT(a_wi[(k + 1) % D]);
+                    const CT w_p_ip = at_top ? CT(0) : CT(a_wip[(k + 1) % D]);
+                    pin();
+                    if (k + D < NK)
+                        load_b(k + D, colb);
+                    if (k + 1 + D < NK)
+                        load_a(k + 1 + D, colb, wcolb);
+                    if (k == 0) {
+                        fwd_out<CT> r = forward_first<CT>(dtr, w_p_i, w_p_ip, us_c, us_p, v_up, v_ut, v_uts);
+                        asm volatile("" : "+v"(r.c), "+v"(r.d)); // keeps the chain at its step (no sinking)
+                        c[0] = r.c;
+                        d[0] = r.d;
+                    } else {
+                        const CT cm = c[k > 0 ? k - 1 : 0], dm = d[k > 0 ? k - 1 : 0];
+                        fwd_out<CT> r = forward_body<CT>(
+                            dtr, w_c_i, w_c_ip, w_p_i, w_p_ip, us_m, us_c, us_p, v_up, v_ut, v_uts, cm, dm);
+                        asm volatile("" : "+v"(r.c), "+v"(r.d));
+                        c[k] = r.c;
+                        d[k] = r.d;
+                    }
+                    us_m = us_c;
+                    us_c = us_p;
+                    w_c_i = w_p_i;
+                    w_c_ip = w_p_ip;
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+                // the next strip's first levels, in flight during this strip's backward sweep
+                const int nx = it + (int)gridDim.x;
+                uint32_t ncolb = colb, nwcolb = wcolb;
+                if (nx < nitems) {
+                    offsets(nx, ncolb, nwcolb);
+                    pin();
+                    prologue(ncolb, nwcolb);
+                }
+                CT data = CT(0);
+#pragma clang loop unroll(full)
+                for (int k = NK - 1; k >= 0; --k) {
+                    const CT v_up = CT(s_up[k][threadIdx.x]);
+                    // u_backward_function (vertical_advection_dycore.cpp:110-121)
+                    const CT dt = d[k] - c[k] * data;
+                    const CT nd = k == top ? d[k] : dt;
+                    data = nd;
+                    bstore<ST, CPOL_NT>(r_uts, ST(dtr * (nd - v_up)), colb, lvl(k) * sk_o);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+                colb = ncolb;
+                wcolb = nwcolb;
+            }
+        }
+
         // Scratch pool (the role of the reference's cached_allocator, sid/allocator.hpp:46-56): grows, never
         // shrinks, one per device; calls on one device are serialised by the caller's stream order.
         struct scratch_pool {
@@ -296,6 +417,20 @@ namespace gt355 {
                 (const ST *)ut->origin, dtr_c, ni, nk, sj, sk, wc->stride[1], wc->stride[2]);                    \
         return check_launch("gt355_vadv");                                                                       \
     }
+                // more strips than two rounds of one wavefront per SIMD: the persistent form (GT355_TUNE_vadv_persist=0
+                // selects the one-strip-per-wavefront grid)
+                if (nk > 64 && tune_param("vadv_upos_lds", 1) != 0 && tune_param("vadv_persist", 1) != 0) {
+                    const int strips_i = (ni + 63) / 64;
+                    const int64_t items = (int64_t)strips_i * nj;
+                    const int waves = cu_count() * 4;
+                    if (items >= 2 * (int64_t)waves && items < (int64_t(1) << 31)) {
+                        vadv_persist_kernel<ST, CT, 80, 8><<<waves, 64, 0, stream>>>((ST *)uts->origin,
+                            (const ST *)us->origin, (const ST *)wc->origin, (const ST *)up->origin,
+                            (const ST *)ut->origin, dtr_c, ni, nk, sj, sk, wc->stride[1], wc->stride[2], strips_i,
+                            (int)items, xcd_count());
+                        return check_launch("gt355_vadv");
+                    }
+                }
                 // u_pos parked in LDS costs NK*64*sizeof(ST) bytes per wavefront (40 KB for FP64 at NK=80: four
                 // wavefronts still fill a CU's 160 KB) and saves the backward re-read; measured faster for FP64 and
                 // FP32 storage at 256x256x80 (54.5 vs 56.3 us FP64).

@@ -202,6 +202,20 @@ def test_vadv_register_buckets_and_upos_placement(gpu, oracle, monkeypatch, upos
     assert np.array_equal(got, ref)
 
 
+@pytest.mark.parametrize("dtype,precision", [(np.float64, 0), (np.float32, 1), (np.float32, 0)])
+@pytest.mark.parametrize("ni,nj,nk", [(130, 700, 67), (256, 512, 80), (1, 2100, 72)])
+def test_vadv_persistent_strips(gpu, oracle, monkeypatch, dtype, precision, ni, nj, nk):
+    """Domains with at least two rounds of 64-column strips per SIMD take the persistent kernel (next strip's first
+    levels prefetched during the backward sweep): bit-identical to the oracle, with a partial last strip (130), a
+    one-column domain (1) and a depth below the bucket (67, 72)."""
+    print(f"[case] vadv persistent {np.dtype(dtype).name} prec={precision} {ni}x{nj}x{nk}", flush=True)
+    f, grid = vadv_case(gpu, ni, nj, nk, dtype)
+    dtr = 0.15 if dtype == np.float64 else float(np.float32(0.15))
+    ref = oracle_vadv(oracle, f, dtr, 3, ni, nj, nk, "native" if precision == 0 else "mixed")
+    got = run_vadv(f, grid, 0.15, precision)[:nk, 3:3 + nj, 3:3 + ni]
+    assert np.array_equal(got, ref)
+
+
 @pytest.mark.parametrize("ni,nj,nk", [(23, 11, 43), (256, 256, 80)])
 def test_vadv_fp32_native_parity(gpu, oracle, ni, nj, nk):
     print(f"[case] vadv fp32 {ni}x{nj}x{nk}", flush=True)
