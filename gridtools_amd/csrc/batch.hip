@@ -251,8 +251,22 @@ namespace gt355 {
                 const strip_fields<ST> f{(const ST *)dm.in, (const ST *)dm.coeff, (const ST *)dm.wcon,
                     (const ST *)dm.u_pos, (const ST *)dm.utens, (ST *)dm.u_stage, (ST *)dm.utens_stage,
                     (ST *)dm.lap, dm.sj, dm.sk, dm.sj, dm.sk, dm.sj, dm.sk, dm.sj, dm.sk, dm.wsj, dm.wsk};
-                hv_strip<ST, CT, NK, SD, true, true, true, HD>(f, dtr, dm.ni, dm.nj, dm.nk, item.i0, item.j, s_up,
-                    tail);
+                if constexpr (!HD && NK == 80) {
+                    // the vertical phase: columns of at most 48 or 64 levels run a register part of that depth
+                    // (a uniform branch per item), not 80 steps with the levels past their top clamped
+                    if (dm.nk <= 48)
+                        hv_strip<ST, CT, 48, SD, true, true, false, HD>(f, dtr, dm.ni, dm.nj, dm.nk, item.i0, item.j,
+                            s_up, tail);
+                    else if (dm.nk <= 64)
+                        hv_strip<ST, CT, 64, SD, true, true, false, HD>(f, dtr, dm.ni, dm.nj, dm.nk, item.i0, item.j,
+                            s_up, tail);
+                    else
+                        hv_strip<ST, CT, NK, SD, true, true, true, HD>(f, dtr, dm.ni, dm.nj, dm.nk, item.i0, item.j,
+                            s_up, tail);
+                } else {
+                    hv_strip<ST, CT, NK, SD, true, true, true, HD>(f, dtr, dm.ni, dm.nj, dm.nk, item.i0, item.j, s_up,
+                        tail);
+                }
             }
         }
 
