@@ -152,25 +152,11 @@ namespace gt355 {
                 si_ * 64, sj_ * JB, k);
         }
 
-        // Column-pair register sweep: the register sweep above with two adjacent columns per lane, so one wavefront
-        // covers 128 columns [i0, i0+128) and every row load is one 2-element vector per lane (8 bytes per lane for
-        // FP32 storage: 512-byte wavefront requests, the size at which the FP64 sweep streams best; measured at
-        // 256x256x80 FP32 storage: 14.2 us against 16.5 us for one column per lane).  Lane l owns columns
-        // a = i0+2l and b = a+1: the i-neighbours inside a lane are the other column of the pair, across lanes they
-        // come from DPP shifts (a's i-1 is the b column of lane l-1, b's i+1 the a column of lane l+1), and lanes
-        // 0 / 63 keep the side columns i0-1 / i0+128 (and i0-2 / i0+129 for the side laplacian), streamed with
-        // scalar loads.  flx at b-1 is flx at a, so each lane evaluates three x-fluxes, not four.  Lanes past the
-        // domain load the last pair (clamped) and store nothing; since ni is even, the lane after the last valid
-        // one loads exactly columns ni and ni+1, the right neighbours the last valid lane needs.  The launcher uses
-        // it for even ni with 2-element-aligned rows (origin and j/k strides); other shapes take the one-column
-        // sweep.
+        // Column-pair register sweep: one wavefront per 128 columns x JB rows x 1 level (sweep.hpp: hdiff_pair_wave).
         template <class ST, class CT, int JB, int P>
         __global__ void __launch_bounds__(256) hdiff_sweep2_kernel(ST *__restrict__ out, const ST *__restrict__ in,
             const ST *__restrict__ coeff, int ni, int nj, int nk, int64_t osj, int64_t osk, int64_t isj, int64_t isk,
             int64_t csj, int64_t csk, int strips_i, int strips_j, int nblocks, int nxcd) {
-            typedef ST vec2 __attribute__((ext_vector_type(2)));
-            constexpr int NR = JB + 4; // in rows j0-2 .. j0+JB+1
-            const int lane = threadIdx.x & 63;
             const int wid = __builtin_amdgcn_readfirstlane(xcd_remap(blockIdx.x, nblocks, nxcd) * 4 + (threadIdx.x >> 6));
             const int total = strips_i * strips_j * nk;
             if (wid >= total)
@@ -178,98 +164,8 @@ namespace gt355 {
             const int si_ = wid % strips_i;
             const int sj_ = (wid / strips_i) % strips_j;
             const int k = wid / (strips_j * strips_i);
-            const int i0 = si_ * 128, j0 = sj_ * JB;
-            const bool left = lane < 32;
-            const int ca = i0 + 2 * lane;
-            // pair starts, clamped: in is readable on [-2, ni+2), coeff on [0, ni); ni is even
-            const int pin_ = min(ca, ni), pco = min(ca, ni - 2);
-            const int ieL = i0 - 1, ifL = i0 - 2;
-            const int ieR = min(i0 + 128, ni + 1), ifR = min(i0 + 129, ni + 1);
-            const ST *inp = in + (int64_t)k * isk;
-            const ST *cop = coeff + (int64_t)k * csk;
-            auto row = [&](int t) { return (int64_t)min(j0 - 2 + t, nj + 1); };
-
-            vec2 rc[P], rk[P];
-            ST reL[P], rfL[P], reR[P], rfR[P];
-            auto load = [&](int t) {
-                const int64_t r = row(t);
-                rc[t % P] = *(const vec2 *)(inp + r * isj + pin_);
-                reL[t % P] = inp[r * isj + ieL];
-                rfL[t % P] = inp[r * isj + ifL];
-                reR[t % P] = inp[r * isj + ieR];
-                rfR[t % P] = inp[r * isj + ifR];
-                const int64_t ro = min(max(j0 - 4 + t, 0), nj - 1);
-                rk[t % P] = __builtin_nontemporal_load((const vec2 *)(cop + ro * csj + pco)); // read once
-            };
-#pragma clang loop unroll(full)
-            for (int t = 0; t < P; ++t)
-                load(t);
-
-            CT a0 = 0, a1 = 0, a2 = 0, b0 = 0, b1 = 0, b2 = 0; // in rows R-2, R-1, R of columns a and b
-            CT eL0 = 0, eL1 = 0, eL2 = 0, eR0 = 0, eR1 = 0, eR2 = 0;
-            CT fL1 = 0, fR1 = 0;
-            CT ma1 = 0, ma0 = 0; // in(a-1) of rows R-1, R-2
-            CT pb1 = 0, pb0 = 0; // in(b+1) of rows R-1, R-2
-            CT la1 = 0, la0 = 0, lb1 = 0, lb0 = 0; // lap of rows R-1, R-2 at a and b
-            CT sl1 = 0, sl0 = 0;                  // side laplacian (i0-1 on lane 0, i0+128 on lane 63)
-            CT flya_m = 0, flyb_m = 0;            // fly of row R-3
-            ST *orow = out + (int64_t)k * osk + ca;
-            const bool col_ok = ca < ni; // ni even: b = a+1 is inside too
-#pragma clang loop unroll(full)
-            for (int t = 0; t < NR; ++t) {
-                const vec2 v = rc[t % P];
-                a0 = a1; a1 = a2; a2 = CT(v.x);
-                b0 = b1; b1 = b2; b2 = CT(v.y);
-                eL0 = eL1; eL1 = eL2; eL2 = CT(reL[t % P]);
-                eR0 = eR1; eR1 = eR2; eR2 = CT(reR[t % P]);
-                const vec2 kv = rk[t % P];
-                const CT ka = CT(kv.x), kb = CT(kv.y);
-                const CT fL_r = CT(rfL[t % P]), fR_r = CT(rfR[t % P]);
-                if (t + P < NR)
-                    load(t + P);
-                const CT ma2 = dpp_keep<DPP_WF_SR1>(eL2, b2), pb2 = dpp_keep<DPP_WF_SL1>(eR2, a2);
-                la0 = la1;
-                lb0 = lb1;
-                sl0 = sl1;
-                if (t >= 2) {
-                    // lap of row R-1 at a and b (lap_function, horizontal_diffusion.cpp:41-46)
-                    la1 = lap_f<CT>(a1, b1, a2, ma1, a0);
-                    lb1 = lap_f<CT>(b1, pb1, b2, a1, b0);
-                    const CT e0 = left ? eL0 : eR0, e1 = left ? eL1 : eR1, e2 = left ? eL2 : eR2;
-                    const CT ip = left ? a1 : fR1, im = left ? fL1 : b1;
-                    sl1 = lap_f<CT>(e1, ip, e2, im, e0);
-                }
-                if (t >= 4) {
-                    // output row R-2 (horizontal_diffusion.cpp:56-60, 70-74, 86-90)
-                    const CT lam = dpp_keep<DPP_WF_SR1>(sl0, lb0), lbp = dpp_keep<DPP_WF_SL1>(sl0, la0);
-                    const CT flx_a = flux_f<CT>(lb0, la0, b0, a0);
-                    const CT flx_am = flux_f<CT>(la0, lam, a0, ma0);
-                    const CT flx_b = flux_f<CT>(lbp, lb0, pb0, b0);
-                    const CT fly_a = flux_f<CT>(la1, la0, a1, a0);
-                    const CT fly_b = flux_f<CT>(lb1, lb0, b1, b0);
-                    const CT ra = a0 - ka * (((flx_a - flx_am) + fly_a) - flya_m);
-                    const CT rb = b0 - kb * (((flx_b - flx_a) + fly_b) - flyb_m);
-                    flya_m = fly_a;
-                    flyb_m = fly_b;
-                    const int r_out = j0 + t - 4;
-                    if (col_ok && r_out < nj) {
-                        vec2 o;
-                        o.x = static_cast<ST>(ra);
-                        o.y = static_cast<ST>(rb);
-                        __builtin_nontemporal_store(o, (vec2 *)(orow + (int64_t)r_out * osj));
-                    }
-                } else if (t == 3) {
-                    // fly of row j0-1, the fly[j-1] of the first output row
-                    flya_m = flux_f<CT>(la1, la0, a1, a0);
-                    flyb_m = flux_f<CT>(lb1, lb0, b1, b0);
-                }
-                ma0 = ma1;
-                pb0 = pb1;
-                ma1 = ma2;
-                pb1 = pb2;
-                fL1 = fL_r;
-                fR1 = fR_r;
-            }
+            hdiff_pair_wave<ST, CT, JB, P, false>(out, in, coeff, (ST *)nullptr, ni, nj, osj, osk, isj, isk, csj, csk,
+                si_ * 128, sj_ * JB, k);
         }
 
         // laplacian.cpp:20-29, one point per thread, one k-level per grid z
