@@ -202,6 +202,20 @@ def test_vadv_register_buckets_and_upos_placement(gpu, oracle, monkeypatch, upos
     assert np.array_equal(got, ref)
 
 
+@pytest.mark.parametrize("depth", [4, 8, 12, 16])
+@pytest.mark.parametrize("dtype,precision", [(np.float64, 0), (np.float32, 1)])
+@pytest.mark.parametrize("nk", [70, 80])
+def test_vadv_load_ahead_depths(gpu, oracle, monkeypatch, depth, dtype, precision, nk):
+    """Every load-ahead depth of the 80-level bucket (GT355_TUNE_vadv_d) is bit-identical to the oracle."""
+    monkeypatch.setenv("GT355_TUNE_vadv_d", str(depth))
+    ni, nj = 67, 5
+    f, grid = vadv_case(gpu, ni, nj, nk, dtype)
+    dtr = 0.15 if dtype == np.float64 else float(np.float32(0.15))
+    ref = oracle_vadv(oracle, f, dtr, 3, ni, nj, nk, "native" if precision == 0 else "mixed")
+    got = run_vadv(f, grid, 0.15, precision)[:nk, 3:3 + nj, 3:3 + ni]
+    assert np.array_equal(got, ref)
+
+
 @pytest.mark.parametrize("dtype,precision", [(np.float64, 0), (np.float32, 1), (np.float32, 0)])
 @pytest.mark.parametrize("ni,nj,nk", [(130, 700, 67), (256, 512, 80), (1, 2100, 72)])
 def test_vadv_persistent_strips(gpu, oracle, monkeypatch, dtype, precision, ni, nj, nk):
