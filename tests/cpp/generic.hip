@@ -8,6 +8,7 @@
 //   gt355_generic --bench N  times the engine's hdiff and vadv specs at 256x256x80 FP64 (one JSON line)
 #include <gt355/stencil.hpp>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -251,6 +252,196 @@ namespace {
             eval(out()) = eval(in()) - eval(coeff()) * (flx_hi - flx_lo + fly_hi - fly_lo);
         }
     };
+    // horizontal_diffusion_functions.cpp:22-205: flx / fly evaluate the laplacian at the point and its east or
+    // north neighbour in five ways (monolithic expressions, call<>::at<>, call<> with offset accessors,
+    // call_proc<>::at<>, call_proc<> with offset accessors); flx and fly are ij-cached, lap is never stored
+    struct hf_lap {
+        using out = inout_accessor<0>;
+        using in = in_accessor<1, extent<-1, 1, -1, 1>>;
+        using param_list = make_param_list<out, in>;
+        template <class Eval>
+        GT_FUNCTION static void apply(Eval eval) {
+            eval(out()) = 4. * eval(in()) - (eval(in(-1, 0)) + eval(in(0, -1)) + eval(in(0, 1)) + eval(in(1, 0)));
+        }
+    };
+    template <class Eval, class Acc>
+    using hf_float = std::decay_t<decltype(std::declval<Eval>()(Acc()))>;
+    enum class hf_var { monolithic, call, call_offsets, procedures, procedures_offsets };
+    template <hf_var V, class Acc>
+    struct hf_lap_at;
+    template <class Acc>
+    struct hf_lap_at<hf_var::monolithic, Acc> {
+        template <class Eval>
+        GT_FUNCTION static auto do00(Eval &eval) {
+            return hf_float<Eval, Acc>(4) * eval(Acc()) -
+                   (eval(Acc(-1, 0)) + eval(Acc(0, -1)) + eval(Acc(0, 1)) + eval(Acc(1, 0)));
+        }
+        template <class Eval>
+        GT_FUNCTION static auto do10(Eval &eval) {
+            return hf_float<Eval, Acc>(4) * eval(Acc(1, 0)) -
+                   (eval(Acc(0, 0)) + eval(Acc(1, -1)) + eval(Acc(1, 1)) + eval(Acc(2, 0)));
+        }
+        template <class Eval>
+        GT_FUNCTION static auto do01(Eval &eval) {
+            return hf_float<Eval, Acc>(4) * eval(Acc(0, 1)) -
+                   (eval(Acc(-1, 1)) + eval(Acc(0, 0)) + eval(Acc(0, 2)) + eval(Acc(1, 1)));
+        }
+    };
+    template <class Acc>
+    struct hf_lap_at<hf_var::call, Acc> {
+        template <class Eval>
+        GT_FUNCTION static auto do00(Eval &eval) {
+            return call<hf_lap>::at<0, 0, 0>::with(eval, Acc());
+        }
+        template <class Eval>
+        GT_FUNCTION static auto do10(Eval &eval) {
+            return call<hf_lap>::at<1, 0, 0>::with(eval, Acc());
+        }
+        template <class Eval>
+        GT_FUNCTION static auto do01(Eval &eval) {
+            return call<hf_lap>::at<0, 1, 0>::with(eval, Acc());
+        }
+    };
+    template <class Acc>
+    struct hf_lap_at<hf_var::call_offsets, Acc> {
+        template <class Eval>
+        GT_FUNCTION static auto do00(Eval &eval) {
+            return call<hf_lap>::with(eval, Acc(0, 0));
+        }
+        template <class Eval>
+        GT_FUNCTION static auto do10(Eval &eval) {
+            return call<hf_lap>::with(eval, Acc(1, 0));
+        }
+        template <class Eval>
+        GT_FUNCTION static auto do01(Eval &eval) {
+            return call<hf_lap>::with(eval, Acc(0, 1));
+        }
+    };
+    template <class Acc>
+    struct hf_lap_at<hf_var::procedures, Acc> {
+        template <class Eval>
+        GT_FUNCTION static auto do00(Eval &eval) {
+            hf_float<Eval, Acc> res;
+            call_proc<hf_lap>::at<0, 0, 0>::with(eval, res, Acc());
+            return res;
+        }
+        template <class Eval>
+        GT_FUNCTION static auto do10(Eval &eval) {
+            hf_float<Eval, Acc> res;
+            call_proc<hf_lap>::at<1, 0, 0>::with(eval, res, Acc());
+            return res;
+        }
+        template <class Eval>
+        GT_FUNCTION static auto do01(Eval &eval) {
+            hf_float<Eval, Acc> res;
+            call_proc<hf_lap>::at<0, 1, 0>::with(eval, res, Acc());
+            return res;
+        }
+    };
+    template <class Acc>
+    struct hf_lap_at<hf_var::procedures_offsets, Acc> {
+        template <class Eval>
+        GT_FUNCTION static auto do00(Eval &eval) {
+            hf_float<Eval, Acc> res;
+            call_proc<hf_lap>::with(eval, res, Acc(0, 0));
+            return res;
+        }
+        template <class Eval>
+        GT_FUNCTION static auto do10(Eval &eval) {
+            hf_float<Eval, Acc> res;
+            call_proc<hf_lap>::with(eval, res, Acc(1, 0));
+            return res;
+        }
+        template <class Eval>
+        GT_FUNCTION static auto do01(Eval &eval) {
+            hf_float<Eval, Acc> res;
+            call_proc<hf_lap>::with(eval, res, Acc(0, 1));
+            return res;
+        }
+    };
+    template <hf_var V>
+    struct hf_flx {
+        using out = inout_accessor<0>;
+        using in = in_accessor<1, extent<-1, 2, -1, 1>>;
+        using param_list = make_param_list<out, in>;
+        template <class Eval>
+        GT_FUNCTION static void apply(Eval eval) {
+            eval(out()) = hf_lap_at<V, in>::do10(eval) - hf_lap_at<V, in>::do00(eval);
+            eval(out()) = eval(out()) * (eval(in(1, 0)) - eval(in(0, 0))) > 0 ? 0.0 : eval(out());
+        }
+    };
+    template <hf_var V>
+    struct hf_fly {
+        using out = inout_accessor<0>;
+        using in = in_accessor<1, extent<-1, 1, -1, 2>>;
+        using param_list = make_param_list<out, in>;
+        template <class Eval>
+        GT_FUNCTION static void apply(Eval eval) {
+            eval(out()) = hf_lap_at<V, in>::do01(eval) - hf_lap_at<V, in>::do00(eval);
+            eval(out()) = eval(out()) * (eval(in(0, 1, 0)) - eval(in(0, 0, 0))) > 0 ? 0.0 : eval(out());
+        }
+    };
+
+    template <hf_var V>
+    void hdiff_functions(const char *vname, int ni, int nj, int nk) {
+        const int h = 2;
+        auto in = seeded<double>(ni + 4, nj + 4, nk, h, 42, 1, -1, 1);
+        auto coeff = seeded<double>(ni + 4, nj + 4, nk, h, 42, 2, 0, 0.25);
+        auto out = storage::builder<storage::gpu>.type<double>().dimensions(ni + 4, nj + 4, nk).halos(h, h, 0).build();
+        auto g = halo_grid(ni, nj, nk, h);
+        run([](auto coeff, auto in, auto out) {
+            GT_DECLARE_TMP(double, flx, fly);
+            return execute_parallel()
+                .ij_cached(flx, fly)
+                .stage(hf_flx<V>(), flx, in)
+                .stage(hf_fly<V>(), fly, in)
+                .stage(hd_out(), out, in, flx, fly, coeff);
+        },
+            gfx950<>(), g, coeff, in, out);
+        // expected: the same operations in the same order on the host (the laplacian sums its neighbours as
+        // i-1, j-1, j+1, i+1 here, unlike lap_function of horizontal_diffusion.cpp), and the oracle's hdiff within
+        // 1e-12 relative (same mathematics, other summation order)
+        auto hin = mirror(in), hco = mirror(coeff);
+        auto st = in->strides();
+        const int64_t sj = st[1], sk = st[2];
+        auto at = [&](std::vector<double> const &a, int i, int j, int k) { return a[i + j * sj + k * sk]; };
+        auto lap = [&](int i, int j, int k) {
+            return 4. * at(hin, i, j, k) -
+                   (at(hin, i - 1, j, k) + at(hin, i, j - 1, k) + at(hin, i, j + 1, k) + at(hin, i + 1, j, k));
+        };
+        auto flx = [&](int i, int j, int k) {
+            const double f = lap(i + 1, j, k) - lap(i, j, k);
+            return f * (at(hin, i + 1, j, k) - at(hin, i, j, k)) > 0 ? 0.0 : f;
+        };
+        auto fly = [&](int i, int j, int k) {
+            const double f = lap(i, j + 1, k) - lap(i, j, k);
+            return f * (at(hin, i, j + 1, k) - at(hin, i, j, k)) > 0 ? 0.0 : f;
+        };
+        std::vector<double> ref(hin.size(), 0.), ora(hin.size(), 0.);
+        for (int k = 0; k < nk; ++k)
+            for (int j = h; j < h + nj; ++j)
+                for (int i = h; i < h + ni; ++i)
+                    ref[i + j * sj + k * sk] = at(hin, i, j, k) - at(hco, i, j, k) * (flx(i, j, k) - flx(i - 1, j, k) +
+                                                                                          fly(i, j, k) - fly(i, j - 1, k));
+        const size_t o = h + h * sj;
+        gto_hdiff_f64(ora.data() + o, hin.data() + o, hco.data() + o, ni, nj, nk, sj, sk, 4);
+        auto v = out->const_host_view();
+        bool same = true;
+        double maxref = 0, maxdiff = 0;
+        for (int k = 0; k < nk; ++k)
+            for (int j = h; j < h + nj; ++j)
+                for (int i = h; i < h + ni; ++i) {
+                    const double got = v(i, j, k), r = ref[i + j * sj + k * sk], q = ora[i + j * sj + k * sk];
+                    same = same && got == r;
+                    maxref = std::max(maxref, std::fabs(q));
+                    maxdiff = std::max(maxdiff, std::fabs(got - q));
+                }
+        char name[128];
+        std::snprintf(name, sizeof name, "horizontal_diffusion_functions (%s) %dx%dx%d: bit-identical; %.1e of the oracle",
+            vname, ni, nj, nk, maxdiff / maxref);
+        report(name, same && maxdiff <= 1e-12 * maxref);
+    }
+
     // call_proc and at<>: a procedure writing two outputs, called one point to the east
     struct sf_pair {
         using a = inout_accessor<0>;
@@ -754,6 +945,11 @@ int main(int argc, char **argv) {
         tridiagonal(23, 11, 40, false);
         simple_hori_diff(12, 33, 61);
         stencil_functions(23, 11, 43);
+        hdiff_functions<hf_var::monolithic>("monolithic", 23, 11, 43);
+        hdiff_functions<hf_var::call>("call", 23, 11, 43);
+        hdiff_functions<hf_var::call_offsets>("call_offsets", 23, 11, 43);
+        hdiff_functions<hf_var::procedures>("procedures", 23, 11, 43);
+        hdiff_functions<hf_var::procedures_offsets>("procedures_offsets", 64, 64, 8);
         positional_and_fusion();
         expandable_and_errors();
     } catch (std::exception const &e) {

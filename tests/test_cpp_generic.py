@@ -1,8 +1,9 @@
 """The generic stencil engine (include/gt355/stencil.hpp): user specs written like the reference's cartesian
 regression tests -- copy, laplacian, the four-stage hdiff with ij-cached temporaries, the forward / backward vadv with
 interval overloads and k-cached temporaries, the tridiagonal solve with expressions and inout fields,
-simple_hori_diff with j-only metric fields, positional, multistage fusion through a temporary, expandable_run of a
-user spec -- run on the GPU and checked against the oracle bit for bit or the reference's known answers
+simple_hori_diff with j-only metric fields, stencil functions (horizontal_diffusion_fused and the five variations of
+horizontal_diffusion_functions), positional, multistage fusion through a temporary, expandable_run of a user spec --
+run on the GPU and checked against the oracle bit for bit or the reference's known answers
 (tests/cpp/generic.hip, built by the package build)."""
 import os
 import subprocess
@@ -20,4 +21,4 @@ def test_generic_engine_on_device():
     print(res.stdout)
     assert res.returncode == 0, res.stdout + res.stderr
     assert "[FAIL]" not in res.stdout
-    assert res.stdout.count("[ok]") >= 17
+    assert res.stdout.count("[ok]") >= 22
