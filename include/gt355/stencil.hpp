@@ -136,6 +136,11 @@ namespace gt355 {
                 GT_FUNCTION constexpr inout_accessor() = default;
                 GT_FUNCTION constexpr inout_accessor(int i, int j = 0, int k = 0) : di(i), dj(j), dk(k) {}
             };
+            // accessor<I, intent, extent> (stencil/frontend/cartesian/accessor.hpp): the same two accessor kinds
+            enum class intent { in, inout };
+            template <int I, intent Intent = intent::in, class Extent = extent<>, int NDims = 3>
+            using accessor = std::conditional_t<Intent == intent::in, in_accessor<I, Extent, NDims>,
+                inout_accessor<I, Extent, NDims>>;
             template <class... Accs>
             struct param_list_t {
                 static constexpr int size = sizeof...(Accs);
@@ -352,6 +357,9 @@ namespace gt355 {
 #define GT_DECLARE_TMP(type, ...)                                                                               \
     auto [__VA_ARGS__] = ::gt355::stencil::engine_::make_tmps<type, __COUNTER__>(                               \
         std::make_index_sequence<GT355_NARGS(__VA_ARGS__)>())
+// GT_DECLARE_EXPANDABLE_TMP (stencil/frontend/cartesian/expandable_parameters): temporaries of a spec given to
+// expandable_run.  expandable_run here runs the spec once per member, so each run's temporaries are the member's.
+#define GT_DECLARE_EXPANDABLE_TMP(type, ...) GT_DECLARE_TMP(type, __VA_ARGS__)
 
         namespace engine_ {
             // ------------------------------------------------------------------------------------------ views

@@ -442,6 +442,82 @@ namespace {
         report(name, same && maxdiff <= 1e-12 * maxref);
     }
 
+    // expandable_parameters.cpp:22-60 and expandable_parameters_single_kernel.cpp:22-100: copies through an ij-cached
+    // temporary, over five members by expandable_run (expandable temporary) and as one spec of ten fields
+    struct ep_copy {
+        using parameters_out = inout_accessor<0>;
+        using parameters_in = accessor<1>;
+        using param_list = make_param_list<parameters_out, parameters_in>;
+        template <class Eval>
+        GT_FUNCTION static void apply(Eval &eval) {
+            eval(parameters_out{}) = eval(parameters_in{});
+        }
+    };
+    struct ep_single {
+        using o1 = inout_accessor<0>;
+        using o2 = inout_accessor<1>;
+        using o3 = inout_accessor<2>;
+        using o4 = inout_accessor<3>;
+        using o5 = inout_accessor<4>;
+        using i1 = in_accessor<5>;
+        using i2 = in_accessor<6>;
+        using i3 = in_accessor<7>;
+        using i4 = in_accessor<8>;
+        using i5 = in_accessor<9>;
+        using param_list = make_param_list<o1, o2, o3, o4, o5, i1, i2, i3, i4, i5>;
+        template <class Eval>
+        GT_FUNCTION static void apply(Eval eval) {
+            eval(o1()) = eval(i1());
+            eval(o2()) = eval(i2());
+            eval(o3()) = eval(i3());
+            eval(o4()) = eval(i4());
+            eval(o5()) = eval(i5());
+        }
+    };
+
+    void expandable_parameters(int ni, int nj, int nk) {
+        auto b = storage::builder<storage::gpu>.type<double>().dimensions(ni, nj, nk);
+        auto g = make_grid(ni, nj, nk);
+        auto make = [&](double v) { return b.value(v).build(); };
+        auto check_copies = [&](std::vector<store<double>> const &in, std::vector<store<double>> const &out) {
+            bool ok = true;
+            for (size_t m = 0; m < in.size(); ++m) {
+                auto vi = in[m]->const_host_view();
+                auto vo = out[m]->const_host_view();
+                for (int k = 0; k < nk && ok; ++k)
+                    for (int j = 0; j < nj && ok; ++j)
+                        for (int i = 0; i < ni && ok; ++i)
+                            ok = vo(i, j, k) == vi(i, j, k);
+            }
+            return ok;
+        };
+        {
+            std::vector<store<double>> out = {make(1.), make(2.), make(3.), make(4.), make(5.)};
+            std::vector<store<double>> in = {make(-1.), make(-2.), make(-3.), make(-4.), make(-5.)};
+            expandable_run<2>([](auto in, auto out) {
+                GT_DECLARE_EXPANDABLE_TMP(double, tmp);
+                return execute_parallel().ij_cached(tmp).stage(ep_copy(), tmp, in).stage(ep_copy(), out, tmp);
+            },
+                gfx950<>(), g, in, out);
+            report("expandable_parameters: expandable_run<2> over 5 members, expandable ij-cached temporary",
+                check_copies(in, out));
+        }
+        {
+            std::vector<store<double>> out = {make(1.), make(2.), make(3.), make(4.), make(5.)};
+            std::vector<store<double>> in = {make(-1.), make(-2.), make(-3.), make(-4.), make(-5.)};
+            run([](auto out0, auto out1, auto out2, auto out3, auto out4, auto in0, auto in1, auto in2, auto in3,
+                    auto in4) {
+                GT_DECLARE_TMP(double, tmp0, tmp1, tmp2, tmp3, tmp4);
+                return execute_parallel()
+                    .ij_cached(tmp0, tmp1, tmp2, tmp3, tmp4)
+                    .stage(ep_single(), tmp0, tmp1, tmp2, tmp3, tmp4, in0, in1, in2, in3, in4)
+                    .stage(ep_single(), out0, out1, out2, out3, out4, tmp0, tmp1, tmp2, tmp3, tmp4);
+            },
+                gfx950<>(), g, out[0], out[1], out[2], out[3], out[4], in[0], in[1], in[2], in[3], in[4]);
+            report("expandable_parameters_single_kernel: ten fields, five ij-cached temporaries", check_copies(in, out));
+        }
+    }
+
     // call_proc and at<>: a procedure writing two outputs, called one point to the east
     struct sf_pair {
         using a = inout_accessor<0>;
@@ -950,6 +1026,7 @@ int main(int argc, char **argv) {
         hdiff_functions<hf_var::call_offsets>("call_offsets", 23, 11, 43);
         hdiff_functions<hf_var::procedures>("procedures", 23, 11, 43);
         hdiff_functions<hf_var::procedures_offsets>("procedures_offsets", 64, 64, 8);
+        expandable_parameters(23, 11, 43);
         positional_and_fusion();
         expandable_and_errors();
     } catch (std::exception const &e) {

@@ -21,4 +21,4 @@ def test_generic_engine_on_device():
     print(res.stdout)
     assert res.returncode == 0, res.stdout + res.stderr
     assert "[FAIL]" not in res.stdout
-    assert res.stdout.count("[ok]") >= 22
+    assert res.stdout.count("[ok]") >= 24
