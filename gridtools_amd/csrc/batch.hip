@@ -238,11 +238,15 @@ namespace gt355 {
         //   NK = 32, two wavefronts per SIMD (ccol/dcol of 32 levels fit half the register file): every level
         //   past 32 goes through the tail scratch, but no column runs clamped steps past its top (the batch's
         //   depths start at 40) and two wavefronts hide each other's latency.
+        // The vertical phase with FP32 storage also has a 96-level register part, taken by columns of at least
+        // deep_nk levels: it halves the tail-scratch levels of the deepest columns (ccol/dcol of 96 levels still fit
+        // the register file of one wavefront per SIMD; u_pos of 96 levels in LDS is 24 KiB per wavefront).
         template <class ST, class CT, int NK, int WPE, bool HD>
         __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
         batch_strip_kernel(const bdom *__restrict__ doms, const sitem *__restrict__ items, int nitems, CT dtr,
-            CT *__restrict__ scratch, int nxcd) {
-            __shared__ ST s_up[NK][64];
+            CT *__restrict__ scratch, int nxcd, int deep_nk) {
+            constexpr bool DEEP = !HD && NK == 80 && sizeof(ST) == 4;
+            __shared__ ST s_up[DEEP ? 96 : NK][64];
             CT *tail = scratch + (int64_t)blockIdx.x * (stail_rows(NK) * 128);
             const int wid = xcd_remap(blockIdx.x, gridDim.x, nxcd);
             for (int it = wid; it < nitems; it += gridDim.x) {
@@ -260,6 +264,9 @@ namespace gt355 {
                     else if (dm.nk <= 64)
                         hv_strip<ST, CT, 64, SD, true, true, false, HD>(f, dtr, dm.ni, dm.nj, dm.nk, item.i0, item.j,
                             s_up, tail);
+                    else if (DEEP && dm.nk >= deep_nk)
+                        hv_strip<ST, CT, DEEP ? 96 : NK, SD, true, true, true, HD>(f, dtr, dm.ni, dm.nj, dm.nk,
+                            item.i0, item.j, s_up, tail);
                     else
                         hv_strip<ST, CT, NK, SD, true, true, true, HD>(f, dtr, dm.ni, dm.nj, dm.nk, item.i0, item.j,
                             s_up, tail);
@@ -306,6 +313,7 @@ struct gt355_batch {
     int impl;      // 1: 64x4 tiles, ccol/dcol in scratch; 2: 64-column strips, ccol/dcol in registers;
                    // 3: two phases (stencil sweep, then the strips' vertical advection)
     int nk_reg;    // strip form: levels of ccol/dcol held in registers (80: one wavefront per SIMD; 32: two)
+    int deep_nk;   // vertical phase, FP32 storage: columns of at least this many levels keep 96 in registers
     int ndoms, nitems, dtype, precision, grid, gridmax, cus, rows;
     int nhitems, hgrid, hjb; // the two-phase form's stencil phase: items, workgroups, rows per item
 };
@@ -415,6 +423,7 @@ extern "C" int gt355_batch_create(int precision, const gt355_batch_domain *domai
     gt355_batch *bt = new gt355_batch{};
     bt->impl = impl;
     bt->nk_reg = nk_reg;
+    bt->deep_nk = tune_param("batch_deep_nk", 89);
     bt->hjb = hjb;
     bt->nhitems = (int)hitems.size();
     // the stencil phase: four wavefronts per workgroup, at most 8 workgroups per CU (one round at 8 wavefronts
@@ -486,16 +495,17 @@ extern "C" int gt355_batch_run(gt355_batch *bt, double dtr_stage, void *stream) 
             if (rc)
                 return rc;
         }
+        const int deep_nk = bt->deep_nk;
 #define GT355_BATCH_STRIP(NK, WPE, HD)                                                                          \
     if (bt->dtype == GT355_F64)                                                                                 \
         batch_strip_kernel<double, double, NK, WPE, HD><<<bt->grid, 64, 0, s>>>(bt->doms, it, bt->nitems,          \
-            dtr_stage, (double *)scr, xcd_count());                                                             \
+            dtr_stage, (double *)scr, xcd_count(), deep_nk);                                                    \
     else if (bt->precision == GT355_WIDEN)                                                                      \
         batch_strip_kernel<float, double, NK, WPE, HD><<<bt->grid, 64, 0, s>>>(bt->doms, it, bt->nitems,           \
-            double(float(dtr_stage)), (double *)scr, xcd_count());                                              \
+            double(float(dtr_stage)), (double *)scr, xcd_count(), deep_nk);                                     \
     else                                                                                                        \
         batch_strip_kernel<float, float, NK, WPE, HD><<<bt->grid, 64, 0, s>>>(bt->doms, it, bt->nitems,            \
-            float(dtr_stage), (float *)scr, xcd_count());
+            float(dtr_stage), (float *)scr, xcd_count(), deep_nk);
         const bool hd = bt->impl == 2;
         if (bt->nk_reg == 32) {
             if (hd) {

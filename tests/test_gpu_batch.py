@@ -19,6 +19,8 @@ WIDE = [(512, 256, 40), (97, 300, 61)]
 NARROW = [(64, 3, 80), (65, 2, 78), (128, 2, 81), (63, 4, 79), (2, 2, 137)]
 # depths around a 32-level register part
 NARROW32 = [(64, 3, 32), (65, 2, 31), (128, 2, 33), (63, 4, 36), (30, 5, 2)]
+# depths around the vertical phase's 96-level register part (taken from 89 levels on by default)
+DEEP96 = [(64, 3, 88), (65, 2, 89), (128, 2, 95), (63, 4, 97), (2, 2, 100), (70, 3, 144)]
 
 
 def oracle_domain(oracle, host, ni, nj, nk, dtr, mode):
@@ -131,6 +133,18 @@ def test_batch_two_phase_bit_exact(gpu, oracle, monkeypatch, dtype, precision, h
     print(f"[case] batch two-phase hjb={hjb} nk_reg={nk_reg} {np.dtype(dtype).name} prec={precision}", flush=True)
     check_batch(gpu, oracle, SIZES + NARROW + NARROW32, dtype, precision, wg_per_cu=2, rounded_ustage=True)
     check_batch(gpu, oracle, WIDE + SIZES[:3], dtype, precision, wg_per_cu=1, rounded_ustage=True)
+
+
+@pytest.mark.parametrize("deep_nk", [89, 65, 145])
+@pytest.mark.parametrize("dtype,precision", [(np.float32, 1), (np.float32, 0)])
+def test_batch_vertical_96_register_levels_bit_exact(gpu, oracle, monkeypatch, dtype, precision, deep_nk):
+    """The two-phase form's vertical phase with FP32 storage keeps 96 levels of ccol/dcol in registers for columns
+    of at least deep_nk levels (89 by default; 65 sends every column past 64 levels there; 145 none): the depths
+    cover 78..100 and 137..144, with and without tail levels."""
+    monkeypatch.setenv("GT355_TUNE_batch_impl", "3")
+    monkeypatch.setenv("GT355_TUNE_batch_deep_nk", str(deep_nk))
+    print(f"[case] batch vertical deep_nk={deep_nk} {np.dtype(dtype).name} prec={precision}", flush=True)
+    check_batch(gpu, oracle, SIZES + NARROW + DEEP96, dtype, precision, wg_per_cu=2, rounded_ustage=True)
 
 
 @pytest.mark.parametrize("dtype,precision", [(np.float64, 0), (np.float32, 1), (np.float32, 0)])
