@@ -345,8 +345,9 @@ extern "C" int gt355_batch_create(int precision, const gt355_batch_domain *domai
     // FP32 storage / FP64 compute, 2.41 ms per timestep against 2.73-2.87 ms for the single-launch strip form)
     const int impl_p = tune_param("batch_impl", 3);
     const int impl = impl_p == 1 ? 1 : impl_p == 3 ? 3 : 2;
-    // rows per stencil-phase item of the two-phase form (GT355_TUNE_batch_hjb: 8 or 16)
-    const int hjb = tune_param("batch_hjb", 16) == 8 ? 8 : 16;
+    // rows per stencil-phase item of the two-phase form (GT355_TUNE_batch_hjb: 8, 16 or 32)
+    const int hjb_p = tune_param("batch_hjb", 16);
+    const int hjb = hjb_p == 8 ? 8 : hjb_p == 32 ? 32 : 16;
     // register depth of the strip form (GT355_TUNE_batch_nk: 80 or 32)
     const int nk_reg = tune_param("batch_nk", 80) == 32 ? 32 : 80;
     std::vector<bdom> doms(ndomains);
@@ -487,6 +488,8 @@ extern "C" int gt355_batch_run(gt355_batch *bt, double dtr_stage, void *stream) 
         batch_hdiff_kernel<float, float, JB, 4><<<bt->hgrid, 256, 0, s>>>(bt->doms, hi, bt->nhitems, xcd_count());
             if (bt->hjb == 8) {
                 GT355_BATCH_HDIFF(8)
+            } else if (bt->hjb == 32) {
+                GT355_BATCH_HDIFF(32)
             } else {
                 GT355_BATCH_HDIFF(16)
             }

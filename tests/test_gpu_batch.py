@@ -122,7 +122,7 @@ def test_batch_uneven_domains_bit_exact(gpu, oracle, monkeypatch, dtype, precisi
 
 
 @pytest.mark.parametrize("nk_reg", [80, 32])
-@pytest.mark.parametrize("hjb", [16, 8])
+@pytest.mark.parametrize("hjb", [16, 8, 32])
 @pytest.mark.parametrize("dtype,precision", [(np.float64, 0), (np.float32, 1), (np.float32, 0)])
 def test_batch_two_phase_bit_exact(gpu, oracle, monkeypatch, dtype, precision, hjb, nk_reg):
     """impl 3: the stencil phase (one-column register sweep over every domain's items) then the vertical phase
