@@ -424,6 +424,19 @@ namespace gt355 {
                     const int64_t items = (int64_t)strips_i * nj;
                     const int waves = cu_count() * 4;
                     if (items >= 2 * (int64_t)waves && items < (int64_t(1) << 31)) {
+                        if constexpr (sizeof(ST) == 4) {
+                            // FP32 storage: load-ahead depth 16 from 8 rounds of strips per wavefront on
+                            // (GT355_TUNE_vadv_persist_d = 8 / 16 forces one).  Measured: 1536x1536x80 1005 -> 964 us
+                            // (36 rounds); 512x512x80 (4 rounds) 92.0 us at depth 8 against 94.4 us at 16.
+                            const int pd = tune_param("vadv_persist_d", 0);
+                            if (pd == 16 || (pd == 0 && items >= 8 * (int64_t)waves)) {
+                                vadv_persist_kernel<ST, CT, 80, 16><<<waves, 64, 0, stream>>>((ST *)uts->origin,
+                                    (const ST *)us->origin, (const ST *)wc->origin, (const ST *)up->origin,
+                                    (const ST *)ut->origin, dtr_c, ni, nk, sj, sk, wc->stride[1], wc->stride[2],
+                                    strips_i, (int)items, xcd_count());
+                                return check_launch("gt355_vadv");
+                            }
+                        }
                         vadv_persist_kernel<ST, CT, 80, 8><<<waves, 64, 0, stream>>>((ST *)uts->origin,
                             (const ST *)us->origin, (const ST *)wc->origin, (const ST *)up->origin,
                             (const ST *)ut->origin, dtr_c, ni, nk, sj, sk, wc->stride[1], wc->stride[2], strips_i,

@@ -230,6 +230,18 @@ def test_vadv_persistent_strips(gpu, oracle, monkeypatch, dtype, precision, ni, 
     assert np.array_equal(got, ref)
 
 
+@pytest.mark.parametrize("precision", [1, 0])
+@pytest.mark.parametrize("ni,nj,nk", [(130, 700, 67), (256, 512, 80), (1, 2100, 72)])
+def test_vadv_persistent_strips_depth16(gpu, oracle, monkeypatch, precision, ni, nj, nk):
+    """The persistent kernel with 16 levels of load-ahead (FP32 storage, GT355_TUNE_vadv_persist_d=16)."""
+    monkeypatch.setenv("GT355_TUNE_vadv_persist_d", "16")
+    print(f"[case] vadv persistent D=16 prec={precision} {ni}x{nj}x{nk}", flush=True)
+    f, grid = vadv_case(gpu, ni, nj, nk, np.float32)
+    ref = oracle_vadv(oracle, f, float(np.float32(0.15)), 3, ni, nj, nk, "native" if precision == 0 else "mixed")
+    got = run_vadv(f, grid, 0.15, precision)[:nk, 3:3 + nj, 3:3 + ni]
+    assert np.array_equal(got, ref)
+
+
 @pytest.mark.parametrize("ni,nj,nk", [(23, 11, 43), (256, 256, 80)])
 def test_vadv_fp32_native_parity(gpu, oracle, ni, nj, nk):
     print(f"[case] vadv fp32 {ni}x{nj}x{nk}", flush=True)
