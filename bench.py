@@ -30,6 +30,8 @@ HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
 # utens_stage, u_stage, wcon, u_pos, utens and writes utens_stage
 BYTES_PER_PT = {"hdiff": 3, "vadv": 6}
 SEED = 42
+# launches per per-kernel timing window (hdiff FP64 256x256x80: 250 x 22 us = 5.5 ms)
+KERNEL_WINDOW = 250
 
 
 def make_set(gt, dtype, s):
@@ -110,8 +112,11 @@ def timed_steps(gt, sets, precision, steps, warmup):
         hdiff(s)
         vadv(s)
     total = timed([hdiff, vadv], steps, warmup)
-    hd_ms = timed([hdiff], steps, warmup + steps) / steps
-    va_ms = timed([vadv], steps, warmup + 2 * steps) / steps
+    # per-kernel windows of at least KERNEL_WINDOW launches (several ms), whatever --steps is, so that the start-up
+    # of a window does not weigh on the mean
+    n = max(steps, KERNEL_WINDOW)
+    hd_ms = timed([hdiff], n, warmup + steps) / n
+    va_ms = timed([vadv], n, warmup + steps + n) / n
     return total, hd_ms, va_ms
 
 
@@ -302,22 +307,30 @@ def generic_engine_leg(reps=50):
     return out
 
 
-def _event_time(fn, n):
+def _event_time(fn, n, min_ms=5.0):
     """Mean ms of n calls of fn() (which launch on launch_stream()) bracketed by one event pair on that stream
-    (synchronised both sides)."""
+    (synchronised both sides).  A window shorter than min_ms is measured again with enough calls to fill it."""
     from gridtools_amd.timing import Event, synchronize
 
     st = launch_stream()
-    synchronize()
-    st.synchronize()
-    start, stop = Event(), Event()
-    start.record(st)
-    for s in range(n):
-        fn(s)
-    stop.record(st)
-    stop.synchronize()
-    st.synchronize()
-    return start.elapsed_ms(stop) / n
+
+    def window(count):
+        synchronize()
+        st.synchronize()
+        start, stop = Event(), Event()
+        start.record(st)
+        for s in range(count):
+            fn(s)
+        stop.record(st)
+        stop.synchronize()
+        st.synchronize()
+        return start.elapsed_ms(stop)
+
+    total = window(n)
+    if total < min_ms:
+        n = int(np.ceil(min_ms / max(total / n, 1e-4)))
+        total = window(n)
+    return total / n
 
 
 def compose_leg(gt, steps, warmup, n=512, nk=80, nsets=2):
@@ -326,7 +339,7 @@ def compose_leg(gt, steps, warmup, n=512, nk=80, nsets=2):
     kernel is timed beside it.  Errors: each
     output of the FP32 form against the FP64 path run on the same FP32-rounded inputs (the FP64 path equals the
     FP64 oracle bit for bit), max and RMS, relative to the field's max |value| and RMS."""
-    from gridtools_amd import inputs, stencils
+    from gridtools_amd import _abi, inputs, stencils
 
     h = 3
     grid = gt.make_grid(gt.halo_descriptor.symmetric(h, n + 2 * h), gt.halo_descriptor.symmetric(h, n + 2 * h), nk)
@@ -361,12 +374,14 @@ def compose_leg(gt, steps, warmup, n=512, nk=80, nsets=2):
         two[name] = {}
         for variant, key in (("1", "lds_tile"), ("2", "dpp_sweep")):
             os.environ["GT355_TUNE_hv_fused"] = variant
+            _abi.call("gt355_tune_env_refresh")
             try:
                 for s in range(max(warmup, 2)):  # first launches load the variant's code object
                     run(sets[s % nsets], prec)
                 two[name][key] = round(_event_time(lambda s: run(sets[s % nsets], prec), steps) * 1e3, 2)
             finally:
                 del os.environ["GT355_TUNE_hv_fused"]
+                _abi.call("gt355_tune_env_refresh")
         del sets
     pts = n * n * nk
     # errors on one fresh input set: FP32 form vs the FP64 path on the widened FP32 inputs
@@ -484,14 +499,38 @@ def large_leg(gt, nsteps=20, n=1536, nk=80):
     }
 
 
+def host_cpu():
+    """CPU model, physical cores of the host, and the CPUs this process may run on."""
+    model, cores = "unknown", set()
+    try:
+        phys = None
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name") and model == "unknown":
+                    model = ln.split(":", 1)[1].strip()
+                elif ln.startswith("physical id"):
+                    phys = ln.split(":", 1)[1].strip()
+                elif ln.startswith("core id"):
+                    cores.add((phys, ln.split(":", 1)[1].strip()))
+    except OSError:
+        pass
+    allowed = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return model, len(cores) or None, allowed
+
+
 def cpu_baseline(budget_s=12.0):
     """The oracle ports (OpenMP, same arithmetic) on this host, hdiff+vadv FP64 256x256x80, bounded samples: the
-    cpu_ifirst traversal and the cpu_kfirst traversal (8x8 blocks, k innermost); the faster one is the value."""
+    cpu_ifirst traversal and the cpu_kfirst traversal (8x8 blocks, k innermost); the faster one is the value.
+
+    Threads: OMP_NUM_THREADS, the host-CPU share of one GPU on the measuring machine (16 there), else the CPUs this
+    process may use; bound one per core (OMP_PROC_BIND=true, OMP_PLACES=cores, set in main() before the OpenMP
+    runtime loads), as the reference's perf runs bind them (jenkins/envs/base.sh:18-19)."""
     from gridtools_amd import inputs
     from oracle import binding as ob
 
-    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    threads = max(1, min(threads, 16))
+    model, phys, allowed = host_cpu()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or allowed
+    threads = max(1, min(threads, allowed))
     h = 3
     shp = (NK, NJ + 2 * h, NI + 2 * h)
     rnd = lambda fid, a, b, nk=NK: np.ascontiguousarray(  # noqa: E731
@@ -522,10 +561,15 @@ def cpu_baseline(budget_s=12.0):
         "unit": "Ggridpoint-updates/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"hdiff+vadv FP64 256x256x80, oracle/gt_oracle.c OpenMP ports on {threads} threads: cpu_ifirst "
-                  f"traversal {res['cpu_ifirst']['reps']} reps (median {res['cpu_ifirst']['median_ms']} ms), "
-                  f"cpu_kfirst traversal {res['cpu_kfirst']['reps']} reps (median {res['cpu_kfirst']['median_ms']} "
-                  f"ms); value = the faster ({best})",
+        "sample": f"hdiff+vadv FP64 256x256x80, oracle/gt_oracle.c OpenMP ports (the reference's cpu_ifirst "
+                  f"cannot be built here: its headers need Boost.Preprocessor, absent from the image) on {threads} "
+                  f"threads bound to cores ({model}; {phys} physical cores in the host, {allowed} usable by this "
+                  f"process): cpu_ifirst traversal {res['cpu_ifirst']['reps']} reps (median "
+                  f"{res['cpu_ifirst']['median_ms']} ms), cpu_kfirst traversal {res['cpu_kfirst']['reps']} reps "
+                  f"(median {res['cpu_kfirst']['median_ms']} ms); value = the faster ({best})",
+        "cpu_model": model,
+        "host_physical_cores": phys,
+        "omp": {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES")},
         "variants": res,
     }
 
@@ -553,6 +597,10 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
               file=sys.stderr)
         sys.exit(2)
+
+    # the CPU baseline's OpenMP threads are bound one per core; libgomp reads these when the oracle loads
+    os.environ.setdefault("OMP_PROC_BIND", "true")
+    os.environ.setdefault("OMP_PLACES", "cores")
 
     import gridtools_amd as gt
     from gridtools_amd.timing import Range

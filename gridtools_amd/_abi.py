@@ -68,6 +68,13 @@ _FP = ctypes.POINTER(Field)
 # name -> argtypes (every entry point returns int unless listed in _RESTYPES)
 SIGNATURES = {
     "gt355_init": [_I],
+    "gt355_finalize": [],
+    "gt355_tmp_acquire": [ctypes.c_size_t, _P, ctypes.POINTER(_P)],
+    "gt355_tmp_release": [_P, _P],
+    "gt355_stream_set_current": [_P],
+    "gt355_stream_current": [ctypes.POINTER(_P)],
+    "gt355_debug_sync": [_I],
+    "gt355_tune_env_refresh": [],
     "gt355_last_error": [],
     "gt355_version": [],
     "gt355_get_device_info": [ctypes.POINTER(DeviceInfo)],

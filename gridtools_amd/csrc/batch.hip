@@ -174,7 +174,7 @@ namespace gt355 {
                         r = forward_first<CT>(dtr, w_p_i, w_p_ip, us_c, us_p, v_up, v_ut, v_uts);
                     else
                         r = forward_body<CT>(
-                            dtr, w_c_i, w_c_ip, w_p_i, w_p_ip, us_m, us_c, us_p, v_up, v_ut, v_uts, cm, dmm);
+                            dtr, w_c_i, w_c_ip, w_p_i, w_p_ip, us_m, us_c, us_p, v_up, v_ut, v_uts, cm, dmm, at_top);
                     sc[k * 256 + tid] = r.c;
                     sd[k * 256 + tid] = r.d;
                     cm = r.c;

@@ -48,11 +48,24 @@ namespace gt355 {
     int check_field(const gt355_field *f, const char *name, int dtype, int ilo, int ihi, int jlo, int jhi, int klo,
         int khi);
 
-    // After a launch: report a launch-configuration error as a status.
+    // Debug launch mode (GT355_DEBUG_SYNC=1 or gt355_debug_sync): a device synchronisation after every launch.
+    bool debug_sync_enabled();
+    int debug_sync_after(const char *what);
+    // Marks the calling thread as capturing a graph (gt355_graph_begin .. gt355_graph_end).
+    void set_capturing(bool on);
+    // The deep-column scratch buffer of (current device, stream), grown to at least `bytes` (pools.hip).
+    int scratch_for(hipStream_t stream, size_t bytes, void **out);
+    // Frees the pools and retired buffers (gt355_finalize).
+    void release_pools();
+
+    // After a launch: report a launch-configuration error as a status; in the debug launch mode also wait for the
+    // kernel and report its failure here.
     inline int check_launch(const char *what) {
         hipError_t e = hipGetLastError();
         if (e != hipSuccess)
             return hip_fail(e, what);
+        if (debug_sync_enabled())
+            return debug_sync_after(what);
         return GT355_OK;
     }
 

@@ -165,7 +165,7 @@ namespace gt355 {
                 } else {
                     const CT cm = c[k > 0 ? k - 1 : 0], dm = d[k > 0 ? k - 1 : 0];
                     fwd_out<CT> r = forward_body<CT>(
-                        dtr, w_c_i, w_c_ip, w_p_i, w_p_ip, us_m, us_c, us_p, v_up, v_ut, v_uts, cm, dm);
+                        dtr, w_c_i, w_c_ip, w_p_i, w_p_ip, us_m, us_c, us_p, v_up, v_ut, v_uts, cm, dm, at_top);
                     c[k] = r.c;
                     d[k] = r.d;
                 }

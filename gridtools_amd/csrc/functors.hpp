@@ -51,10 +51,14 @@ namespace gt355 {
         return {c * divided, d * divided};
     }
 
-    // u_forward_function::apply(modify<1,-1>) (vertical_advection_dycore.cpp:50-67)
+    // u_forward_function::apply(modify<1,-1>) (vertical_advection_dycore.cpp:50-67).
+    // top: evaluate apply(last_level) (:69-81) instead, for callers that pass zero k+1 winds at the top level: then
+    // c = +0, b = (dtr - a) - (+0) == dtr - a, and the k+1 correction term is replaced by +0 (x - (+0) == x for every
+    // x), so the result has last_level's bits whatever u_stage holds (an infinite u_stage would make cs * (us_p -
+    // us_c) a NaN).
     template <class CT>
     __device__ __forceinline__ fwd_out<CT> forward_body(CT dtr, CT w_c_i, CT w_c_ip, CT w_p_i, CT w_p_ip,
-        CT us_m, CT us_c, CT us_p, CT up, CT ut, CT uts, CT cc_m, CT dc_m) {
+        CT us_m, CT us_c, CT us_p, CT up, CT ut, CT uts, CT cc_m, CT dc_m, bool top) {
         CT gav = -CT(.25) * (w_c_ip + w_c_i);
         CT gcv = CT(.25) * (w_p_ip + w_p_i);
         CT as = gav * CT(BET_M);
@@ -62,7 +66,8 @@ namespace gt355 {
         CT a = gav * CT(BET_P);
         CT c = gcv * CT(BET_P);
         CT b = (dtr - a) - c;
-        CT correction = -as * (us_m - us_c) - cs * (us_p - us_c);
+        CT up_term = cs * (us_p - us_c);
+        CT correction = -as * (us_m - us_c) - (top ? CT(0) : up_term);
         CT d = ((dtr * up + ut) + uts) + correction;
         CT divided = CT(1) / (b - cc_m * a);
         return {c * divided, (d - dc_m * a) * divided};

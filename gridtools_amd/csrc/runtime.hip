@@ -137,6 +137,20 @@ int gt355_init(int device) {
     return GT355_OK;
 }
 
+int gt355_finalize(void) {
+    GT355_HIP(hipDeviceSynchronize());
+    release_pools();
+    std::lock_guard<std::mutex> lock(g_mutex);
+    for (auto &f : g_flush) {
+        if (f.buf)
+            (void)hipFree(f.buf);
+        f.buf = nullptr;
+        f.bytes = 0;
+    }
+    g_device = -1;
+    return GT355_OK;
+}
+
 int gt355_get_device_info(gt355_device_info *out) {
     GT355_CHECK(out != nullptr, "gt355_get_device_info: null output");
     hipDeviceProp_t p;
@@ -314,12 +328,14 @@ int gt355_event_elapsed_ms(void *start, void *stop, float *ms) {
 int gt355_graph_begin(void *stream) {
     GT355_CHECK(stream != nullptr, "gt355_graph_begin: capture needs a non-null stream");
     GT355_HIP(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal));
+    set_capturing(true);
     return GT355_OK;
 }
 
 int gt355_graph_end(void *stream, void **graph_exec) {
     GT355_CHECK(stream != nullptr && graph_exec != nullptr, "gt355_graph_end: null argument");
     hipGraph_t g = nullptr;
+    set_capturing(false);
     GT355_HIP(hipStreamEndCapture((hipStream_t)stream, &g));
     hipGraphExec_t e = nullptr;
     hipError_t err = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);

@@ -198,7 +198,7 @@ namespace gt355 {
             fwd_out<CT> r = first
                                 ? forward_first<CT>(dtr, w_p_i, w_p_ip, us_c, us_p, v_up, v_ut, v_uts)
                                 : forward_body<CT>(dtr, w_c_i, w_c_ip, w_p_i, w_p_ip, us_m, us_c, us_p, v_up, v_ut,
-                                      v_uts, cm, dm);
+                                      v_uts, cm, dm, at_top);
             cm = r.c;
             dm = r.d;
             us_m = us_c;

@@ -6,6 +6,8 @@
 //   hdiff <storage bytes> <compute bytes> <ni> <nj> <nk> <impl> <a> <b> <c> <microseconds>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
+#include <unistd.h>
 #include <map>
 #include <mutex>
 #include <string>
@@ -25,10 +27,55 @@ namespace gt355 {
         std::map<tune_key, entry> g_hdiff;
     } // namespace
 
+    // The GT355_TUNE_* knobs, read from the environment once (at the first lookup) and again only on
+    // gt355_tune_env_refresh: a launch looks its knobs up in this snapshot, with no getenv and no allocation.
+    namespace {
+        struct knob {
+            char name[48];
+            int value;
+        };
+        struct knob_table {
+            knob k[64];
+            int n = 0;
+        };
+        std::mutex g_knob_mutex;
+        knob_table g_knobs;
+        bool g_knobs_read = false;
+
+        void read_knobs_locked() {
+            g_knobs.n = 0;
+            const size_t plen = std::strlen("GT355_TUNE_");
+            for (char **e = environ; e && *e; ++e) {
+                if (std::strncmp(*e, "GT355_TUNE_", plen) != 0)
+                    continue;
+                const char *eq = std::strchr(*e, '=');
+                if (!eq || g_knobs.n == 64)
+                    continue;
+                const size_t len = (size_t)(eq - *e) - plen;
+                if (len == 0 || len >= sizeof(g_knobs.k[0].name))
+                    continue;
+                knob &kn = g_knobs.k[g_knobs.n++];
+                std::memcpy(kn.name, *e + plen, len);
+                kn.name[len] = 0;
+                kn.value = std::atoi(eq + 1);
+            }
+            g_knobs_read = true;
+        }
+    } // namespace
+
     int tune_param(const char *name, int dflt) {
-        std::string key = std::string("GT355_TUNE_") + name;
-        const char *v = std::getenv(key.c_str());
-        return v ? std::atoi(v) : dflt;
+        std::lock_guard<std::mutex> lock(g_knob_mutex);
+        if (!g_knobs_read)
+            read_knobs_locked();
+        for (int n = 0; n < g_knobs.n; ++n)
+            if (std::strcmp(g_knobs.k[n].name, name) == 0)
+                return g_knobs.k[n].value;
+        return dflt;
+    }
+
+    void tune_env_refresh() {
+        std::lock_guard<std::mutex> lock(g_knob_mutex);
+        read_knobs_locked();
     }
 
     hdiff_cfg hdiff_cfg_for(int st_size, int ct_size, int ni, int nj, int nk) {
@@ -99,6 +146,11 @@ extern "C" int gt355_tune_load(const char *path, int *entries) {
     std::fclose(f);
     if (entries)
         *entries = n;
+    return GT355_OK;
+}
+
+extern "C" int gt355_tune_env_refresh(void) {
+    gt355::tune_env_refresh();
     return GT355_OK;
 }
 

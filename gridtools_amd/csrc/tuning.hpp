@@ -31,7 +31,9 @@ namespace gt355 {
     hdiff_cfg hdiff_cfg_for(int st_size, int ct_size, int ni, int nj, int nk);
     void record_hdiff_cfg(int st_size, int ct_size, int ni, int nj, int nk, hdiff_cfg cfg, double us);
 
-    // A named integer tuning knob: the environment variable GT355_TUNE_<name> if set, otherwise dflt.
+    // A named integer tuning knob: the environment variable GT355_TUNE_<name> if set, otherwise dflt.  The
+    // environment is read once per process; tune_env_refresh (gt355_tune_env_refresh) reads it again.
     int tune_param(const char *name, int dflt);
+    void tune_env_refresh();
 
 } // namespace gt355

@@ -60,7 +60,7 @@ namespace gt355 {
                 w_p_ip = CT(wc_f[wcol + (int64_t)(k + 1) * wsk + 1]);
                 up = CT(up_f[p]);
                 r = forward_body<CT>(dtr, w_c_i, w_c_ip, w_p_i, w_p_ip, us_m, us_c, us_p, up, CT(ut_f[p]),
-                    CT(uts_f[p]), cc_m, dc_m);
+                    CT(uts_f[p]), cc_m, dc_m, false);
                 sc[(int64_t)k * plane] = r.c;
                 sd[(int64_t)k * plane] = r.d;
                 cc_m = r.c;
@@ -182,7 +182,7 @@ namespace gt355 {
                 } else {
                     const CT cm = c[k > 0 ? k - 1 : 0], dm = d[k > 0 ? k - 1 : 0];
                     fwd_out<CT> r = forward_body<CT>(
-                        dtr, w_c_i, w_c_ip, w_p_i, w_p_ip, us_m, us_c, us_p, v_up, v_ut, v_uts, cm, dm);
+                        dtr, w_c_i, w_c_ip, w_p_i, w_p_ip, us_m, us_c, us_p, v_up, v_ut, v_uts, cm, dm, at_top);
                     c[k] = r.c; // never read at the top level
                     d[k] = r.d;
                 }
@@ -313,7 +313,7 @@ namespace gt355 {
                     } else {
                         const CT cm = c[k > 0 ? k - 1 : 0], dm = d[k > 0 ? k - 1 : 0];
                         fwd_out<CT> r = forward_body<CT>(
-                            dtr, w_c_i, w_c_ip, w_p_i, w_p_ip, us_m, us_c, us_p, v_up, v_ut, v_uts, cm, dm);
+                            dtr, w_c_i, w_c_ip, w_p_i, w_p_ip, us_m, us_c, us_p, v_up, v_ut, v_uts, cm, dm, at_top);
                         asm volatile("" : "+v"(r.c), "+v"(r.d));
                         c[k] = r.c;
                         d[k] = r.d;
@@ -346,40 +346,6 @@ namespace gt355 {
                 colb = ncolb;
                 wcolb = nwcolb;
             }
-        }
-
-        // Scratch pool (the role of the reference's cached_allocator, sid/allocator.hpp:46-56): grows, never
-        // shrinks, one per device; calls on one device are serialised by the caller's stream order.
-        struct scratch_pool {
-            std::mutex m;
-            void *ptr[64] = {};
-            size_t bytes[64] = {};
-        };
-        scratch_pool g_pool;
-
-        int get_scratch(size_t bytes, void **out) {
-            int dev = current_device();
-            GT355_CHECK(dev >= 0 && dev < 64, "scratch: device %d", dev);
-            std::lock_guard<std::mutex> lock(g_pool.m);
-            if (g_pool.bytes[dev] < bytes) {
-                if (g_pool.ptr[dev]) {
-                    // the old buffer may still be in use by queued work on any stream
-                    GT355_HIP(hipDeviceSynchronize());
-                    GT355_HIP(hipFree(g_pool.ptr[dev]));
-                    g_pool.ptr[dev] = nullptr;
-                    g_pool.bytes[dev] = 0;
-                }
-                void *p = nullptr;
-                hipError_t e = hipMalloc(&p, bytes);
-                if (e != hipSuccess) {
-                    set_error("scratch: hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
-                    return GT355_ENOMEM;
-                }
-                g_pool.ptr[dev] = p;
-                g_pool.bytes[dev] = bytes;
-            }
-            *out = g_pool.ptr[dev];
-            return GT355_OK;
         }
 
         // Largest k-depth whose ccol/dcol fit the register file next to the prefetch rings without private-memory
@@ -481,7 +447,7 @@ namespace gt355 {
             int64_t plane = (int64_t)ni * nj;
             size_t bytes = sizeof(CT) * (size_t)plane * (size_t)(2 * nk - 1);
             void *scratch = nullptr;
-            int rc = get_scratch(bytes, &scratch);
+            int rc = scratch_for(stream, bytes, &scratch);
             if (rc)
                 return rc;
             CT dtr = dtr_c;

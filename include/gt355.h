@@ -56,6 +56,11 @@ typedef struct gt355_field {
 /* Select and initialise a device (the reference leaves this to the CUDA runtime; gt355 keeps per-device
  * state: temporary pool, flush buffer, tuning table). */
 int gt355_init(int device);
+/* Waits for the device, then frees what the library keeps between calls: the temporary pool, the deep-column
+ * scratch buffers (retired ones included) and the MALL-flush buffer.  The reference frees its allocator stash at
+ * process exit (sid/allocator.hpp:44-57); a caller that must return the memory earlier calls this.  Graphs that use
+ * library scratch must be destroyed first.  gt355_init (or any call) may follow: pools are re-created on use. */
+int gt355_finalize(void);
 const char *gt355_last_error(void);
 const char *gt355_version(void);
 
@@ -94,6 +99,13 @@ int gt355_storage_layout(int64_t elem_size, const int64_t lengths[3], const int6
 int gt355_storage_base(const gt355_layout *layout, void *raw, void **base);
 
 int gt355_malloc(size_t bytes, void **ptr);   /* storage_allocate (storage/gpu.hpp:78-81) */
+/* Pooled temporaries, the cached allocator of the reference's temporaries (sid/allocator.hpp:44-57, whose static
+ * stash hands freed blocks out again).  A released block is reused at once by an acquire on the same stream (stream
+ * order keeps its users apart) and by an acquire on another stream only after the work queued before its release
+ * has completed (an event recorded at the release).  Blocks released while the stream is captured into a graph
+ * are reused on that stream only.  Blocks are freed by gt355_finalize. */
+int gt355_tmp_acquire(size_t bytes, void *stream, void **ptr);
+int gt355_tmp_release(void *ptr, void *stream);
 int gt355_free(void *ptr);
 int gt355_memcpy_h2d(void *dst, const void *src, size_t bytes); /* storage_update_target (gpu.hpp:83-90) */
 int gt355_memcpy_d2h(void *dst, const void *src, size_t bytes); /* storage_update_host (gpu.hpp:92-99) */
@@ -227,6 +239,15 @@ int gt355_halo_exchange_local(int dtype, void *const *fields, int nfields, const
  * Replaces timer_cuda (common/timer/timer_cuda.hpp:21-54): hipEvent pairs on a stream.  Streams are created
  * with default priority. */
 int gt355_stream_create(void **stream);
+/* The calling thread's current stream: the reference launches on the default stream and leaves propagation of a
+ * stream open ("if required propagate CUDA stream", stencil/gpu/launch_kernel.hpp:158-161).  The C++ front end's
+ * backend tag launches on it when the tag carries no stream of its own.  NULL (the null stream) until set. */
+int gt355_stream_set_current(void *stream);
+int gt355_stream_current(void **stream);
+/* Debug launch mode, the reference's debug-build synchronisation after each launch (common/cuda_util.hpp:89-93):
+ * on != 0 makes every entry point wait for its kernels and report a kernel failure at that call.  Also switched on
+ * by GT355_DEBUG_SYNC=1 in the environment.  Not applied while the calling thread captures a graph. */
+int gt355_debug_sync(int on);
 int gt355_stream_destroy(void *stream);
 int gt355_event_create(void **event);
 int gt355_event_destroy(void *event);
@@ -250,6 +271,9 @@ int gt355_tune_hdiff(int precision, const gt355_field *out, const gt355_field *i
 int gt355_tune_save(const char *path);
 int gt355_tune_load(const char *path, int *entries);
 int gt355_tune_clear(void);
+/* Launch-configuration knobs GT355_TUNE_<name> are read from the environment once per process; this reads them
+ * again (for a caller that changes them at run time). */
+int gt355_tune_env_refresh(void);
 
 /* ------------------------------------------------------------------ graphs
  * Stream capture of a launch sequence into a hipGraph (the reference has no equivalent: it launches every

@@ -305,6 +305,15 @@ namespace gt355 {
         };
         using gfx950_widen = gfx950<GT355_WIDEN>;
 
+        // The stream a run launches on: the tag's own, else the calling thread's current stream (gt355_stream_current)
+        template <class Be>
+        void *stream_of(Be const &be) {
+            void *s = be.stream;
+            if (!s)
+                check(gt355_stream_current(&s), "run: current stream");
+            return s;
+        }
+
         // The path's stencil specs; fields in the order of the reference regression specs.
         namespace specs {
             struct horizontal_diffusion {}; // (in, coeff, out)                horizontal_diffusion.cpp:97-107
@@ -325,7 +334,7 @@ namespace gt355 {
             std::shared_ptr<storage::data_store<T>> const &coeff, std::shared_ptr<storage::data_store<T>> const &out) {
             auto o = g.origin();
             gt355_field fi = in->sid(o, false), fc = coeff->sid(o, false), fo = out->sid(o, true);
-            check(gt355_hdiff(Be::precision, &fo, &fi, &fc, g.i_size, g.j_size, g.k_size, be.stream),
+            check(gt355_hdiff(Be::precision, &fo, &fi, &fc, g.i_size, g.j_size, g.k_size, stream_of(be)),
                 "run(horizontal_diffusion)");
         }
 
@@ -339,7 +348,7 @@ namespace gt355 {
             gt355_field fs = utens_stage->sid(o, true), fu = u_stage->sid(o, false), fw = wcon->sid(o, false),
                         fp = u_pos->sid(o, false), ft = utens->sid(o, false);
             check(gt355_vadv(Be::precision, &fs, &fu, &fw, &fp, &ft, (double)dtr_stage.value, g.i_size, g.j_size,
-                      g.k_size, be.stream),
+                      g.k_size, stream_of(be)),
                 "run(vertical_advection)");
         }
 
@@ -348,7 +357,7 @@ namespace gt355 {
             std::shared_ptr<storage::data_store<T>> const &in) {
             auto o = g.origin();
             gt355_field fo = out->sid(o, true), fi = in->sid(o, false);
-            check(gt355_laplacian(Be::precision, &fo, &fi, g.i_size, g.j_size, g.k_size, be.stream), "run(laplacian)");
+            check(gt355_laplacian(Be::precision, &fo, &fi, g.i_size, g.j_size, g.k_size, stream_of(be)), "run(laplacian)");
         }
 
         template <class Be, class T>
@@ -357,7 +366,7 @@ namespace gt355 {
             static_assert(Be::precision == GT355_NATIVE, "copy has no compute type");
             auto o = g.origin();
             gt355_field fo = out->sid(o, true), fi = in->sid(o, false);
-            check(gt355_copy(&fo, &fi, g.i_size, g.j_size, g.k_size, be.stream), "run(copy)");
+            check(gt355_copy(&fo, &fi, g.i_size, g.j_size, g.k_size, stream_of(be)), "run(copy)");
         }
 
         template <class Be, class T>
@@ -371,7 +380,7 @@ namespace gt355 {
                         fs = utens_stage->sid(o, true), fw = wcon->sid(o, false), fp = u_pos->sid(o, false),
                         ft = utens->sid(o, false);
             check(gt355_hdiff_vadv(Be::precision, &fi, &fc, &fu, &fs, &fw, &fp, &ft, (double)dtr_stage.value, g.i_size,
-                      g.j_size, g.k_size, be.stream),
+                      g.j_size, g.k_size, stream_of(be)),
                 "run(hdiff_vadv)");
         }
 
@@ -394,7 +403,7 @@ namespace gt355 {
             gt355_field fc = coeff->sid(o, false), fi = in->sid(o, false), fo = out->sid(o, true),
                         fa = crlato->sid(o, false), fu = crlatu->sid(o, false);
             check(gt355_simple_hori_diff(Be::precision, &fo, &fi, &fa, &fu, &fc, g.i_size, g.j_size, g.k_size,
-                      be.stream),
+                      stream_of(be)),
                 "run(simple_hori_diff)");
         }
 
@@ -404,7 +413,7 @@ namespace gt355 {
             std::shared_ptr<storage::data_store<T>> const &rho) {
             auto o = g.origin();
             gt355_field fd = data->sid(o, true), fn = data_nnow->sid(o, false), fr = rho->sid(o, false);
-            check(gt355_prepare_tracers(Be::precision, &fd, &fn, 1, &fr, g.i_size, g.j_size, g.k_size, be.stream),
+            check(gt355_prepare_tracers(Be::precision, &fd, &fn, 1, &fr, g.i_size, g.j_size, g.k_size, stream_of(be)),
                 "run(prepare_tracers)");
         }
 
@@ -421,7 +430,7 @@ namespace gt355 {
                 in[f] = all[f]->sid(o, false);
                 out[f] = all[n + f]->sid(o, true);
             }
-            check(gt355_copy_fields(out.data(), in.data(), (int)n, g.i_size, g.j_size, g.k_size, be.stream),
+            check(gt355_copy_fields(out.data(), in.data(), (int)n, g.i_size, g.j_size, g.k_size, stream_of(be)),
                 "run(copy_tuple)");
         }
 
@@ -443,7 +452,7 @@ namespace gt355 {
 
           public:
             batch(Be be, std::vector<batch_member<T>> members, int wg_per_cu = 2)
-                : m_members(std::move(members)), m_stream(be.stream) {
+                : m_members(std::move(members)), m_stream(stream_of(be)) {
                 std::vector<gt355_batch_domain> d(m_members.size());
                 for (size_t i = 0; i < m_members.size(); ++i) {
                     auto &m = m_members[i];
@@ -532,7 +541,7 @@ namespace gt355 {
             }
             gt355_field fr = rho->sid(o, false);
             check(gt355_prepare_tracers(Be::precision, fd.data(), fn.data(), (int)data.size(), &fr, g.i_size, g.j_size,
-                      g.k_size, be.stream),
+                      g.k_size, stream_of(be)),
                 "expandable_run(prepare_tracers)");
         }
 

@@ -957,38 +957,6 @@ namespace gt355 {
                 using type = tlist<Msses...>;
             };
 
-            // cached device memory for temporaries, kept for the process like the reference's allocator stash
-            // (sid/allocator.hpp:46-56): blocks go back to the pool after a run's launches are queued, so a later run
-            // on the same stream reuses them in stream order
-            class tmp_pool {
-                std::mutex m_mutex;
-                std::multimap<size_t, void *> m_free;
-
-              public:
-                static tmp_pool &get() {
-                    static tmp_pool p;
-                    return p;
-                }
-                void *take(size_t bytes, size_t &got) {
-                    std::lock_guard<std::mutex> lk(m_mutex);
-                    auto it = m_free.lower_bound(bytes);
-                    if (it != m_free.end() && it->first <= 2 * bytes) {
-                        void *p = it->second;
-                        got = it->first;
-                        m_free.erase(it);
-                        return p;
-                    }
-                    void *p = nullptr;
-                    check(gt355_malloc(bytes, &p), "stencil engine: temporary");
-                    got = bytes;
-                    return p;
-                }
-                void give(size_t bytes, void *p) {
-                    std::lock_guard<std::mutex> lk(m_mutex);
-                    m_free.emplace(bytes, p);
-                }
-            };
-
             // ------------------------------------------------------------------------------------------ binding
             template <class A>
             struct kind_of;
@@ -1062,13 +1030,13 @@ namespace gt355 {
                     return stage_accesses<F, Plhs...>(typename F::param_list());
                 }
                 // allocate each temporary of this stage that has no memory yet
-                static void alloc_tmps(run_ctx &c, std::map<int, ext_rt> const &box, grid const &g,
-                    std::vector<std::pair<size_t, void *>> &taken) {
-                    (alloc_one<Plhs>(c, box, g, taken), ...);
+                static void alloc_tmps(run_ctx &c, std::map<int, ext_rt> const &box, grid const &g, void *stream,
+                    std::vector<void *> &taken) {
+                    (alloc_one<Plhs>(c, box, g, stream, taken), ...);
                 }
                 template <class P>
-                static void alloc_one(run_ctx &c, std::map<int, ext_rt> const &box, grid const &g,
-                    std::vector<std::pair<size_t, void *>> &taken) {
+                static void alloc_one(run_ctx &c, std::map<int, ext_rt> const &box, grid const &g, void *stream,
+                    std::vector<void *> &taken) {
                     if constexpr (P::is_tmp) {
                         if (c.fields.count(P::key) || c.lds.count(P::key))
                             return;
@@ -1080,9 +1048,9 @@ namespace gt355 {
                         const int64_t len[3] = {g.i_size + 2 * hi, g.j_size + 2 * hj, g.k_size + 2 * hk};
                         const int64_t hal[3] = {hi, hj, hk};
                         check(gt355_storage_layout(sizeof(T), len, hal, &lay), "stencil engine: temporary layout");
-                        size_t got = 0;
-                        void *raw = tmp_pool::get().take((size_t)lay.alloc_bytes, got);
-                        taken.emplace_back(got, raw);
+                        void *raw = nullptr;
+                        check(gt355_tmp_acquire((size_t)lay.alloc_bytes, stream, &raw), "stencil engine: temporary");
+                        taken.push_back(raw);
                         void *base = nullptr;
                         check(gt355_storage_base(&lay, raw, &base), "stencil engine: temporary layout");
                         T *origin = static_cast<T *>(base) + hi + hj * lay.stride[1] + hk * lay.stride[2];
@@ -1453,16 +1421,21 @@ namespace gt355 {
                 };
                 (one(Msses()), ...);
             }
+            // temporaries come from the library's pool (gt355_tmp_acquire: the cached allocator of
+            // sid/allocator.hpp:44-57, made safe across streams) and go back to it after the run's launches are
+            // queued on `stream`
             template <class... Msses>
-            void alloc_all(tlist<Msses...>, run_ctx &c, std::map<int, ext_rt> const &box, grid const &g,
-                std::vector<std::pair<size_t, void *>> &taken) {
-                auto one = [&](auto mss) { alloc_stages(typename mss_traits<decltype(mss)>::stages(), c, box, g, taken); };
+            void alloc_all(tlist<Msses...>, run_ctx &c, std::map<int, ext_rt> const &box, grid const &g, void *stream,
+                std::vector<void *> &taken) {
+                auto one = [&](auto mss) {
+                    alloc_stages(typename mss_traits<decltype(mss)>::stages(), c, box, g, stream, taken);
+                };
                 (one(Msses()), ...);
             }
             template <class... Stages>
             void alloc_stages(tlist<Stages...>, run_ctx &c, std::map<int, ext_rt> const &box, grid const &g,
-                std::vector<std::pair<size_t, void *>> &taken) {
-                (stage_plhs<Stages>::alloc_tmps(c, box, g, taken), ...);
+                void *stream, std::vector<void *> &taken) {
+                (stage_plhs<Stages>::alloc_tmps(c, box, g, stream, taken), ...);
             }
         } // namespace engine_
 
@@ -1601,11 +1574,14 @@ namespace gt355 {
             if (g.i_size <= 0 || g.j_size <= 0 || g.k_size <= 0)
                 return;
             engine_::plan_all(msses(), st, box, c);
-            std::vector<std::pair<size_t, void *>> taken;
-            engine_::alloc_all(msses(), c, box, g, taken);
-            engine_::launch_all(msses(), st, c, args_t, g, (hipStream_t)be.stream);
-            for (auto &t : taken)
-                engine_::tmp_pool::get().give(t.first, t.second);
+            void *stream = be.stream;
+            if (!stream)
+                check(gt355_stream_current(&stream), "run: current stream");
+            std::vector<void *> taken;
+            engine_::alloc_all(msses(), c, box, g, stream, taken);
+            engine_::launch_all(msses(), st, c, args_t, g, (hipStream_t)stream);
+            for (void *t : taken)
+                check(gt355_tmp_release(t, stream), "stencil engine: temporary");
         }
 
         // run_single_stage(functor, backend, grid, fields...) (stencil/frontend/run.hpp:252-258)

@@ -32,6 +32,41 @@ def gpu():
     return gt
 
 
+class _TuneAwareMonkeypatch:
+    """monkeypatch whose environment changes of GT355_TUNE_* knobs reach the library, which reads them once per
+    process (gt355_tune_env_refresh)."""
+
+    def __init__(self, mp):
+        self._mp = mp
+
+    @staticmethod
+    def _refresh(name):
+        if str(name).startswith("GT355_TUNE_"):
+            from gridtools_amd import _abi
+
+            _abi.call("gt355_tune_env_refresh")
+
+    def setenv(self, name, value, prepend=None):
+        self._mp.setenv(name, value, prepend)
+        self._refresh(name)
+
+    def delenv(self, name, raising=True):
+        self._mp.delenv(name, raising)
+        self._refresh(name)
+
+    def __getattr__(self, attr):
+        return getattr(self._mp, attr)
+
+
+@pytest.fixture
+def monkeypatch(monkeypatch):
+    wrapped = _TuneAwareMonkeypatch(monkeypatch)
+    yield wrapped
+    monkeypatch.undo()
+    if "gridtools_amd._abi" in sys.modules:
+        sys.modules["gridtools_amd._abi"].call("gt355_tune_env_refresh")
+
+
 def rel_err(a, b):
     """max |a-b| / max |b| over finite entries."""
     a = np.asarray(a, dtype=np.float64)

@@ -1,8 +1,9 @@
 """Pins the CPU oracle's vertical advection and tridiagonal solve.
 
-  * the reference's own known answer: vertical_advection_repository's analytic fields and its closed-form
-    Thomas solve (tests/regression/vertical_advection_repository.hpp:72-151), at the reference verifier's
-    double tolerance 1e-14 (tests/include/verifier.hpp:25-51) and test sizes (test_environment.hpp:69-77);
+  * the reference's own known answer: vertical_advection_repository's closed-form Thomas solve
+    (tests/regression/vertical_advection_repository.hpp:95-151), applied to random fields drawn at the
+    repository's scales (:153-174), at the reference verifier's double tolerance 1e-14
+    (tests/include/verifier.hpp:25-51) and test sizes (test_environment.hpp:69-77);
   * a textbook check independent of the recurrence: the tridiagonal system a x[k-1] + b x[k] + c x[k+1] = d
     assembled from the inputs and solved by dense LU (numpy.linalg.solve), on well-conditioned columns;
   * wcon = 0 reduces the solve to a diagonal one with a closed form;
@@ -40,7 +41,7 @@ def run(oracle, uts, us, wc, up, ut, dtr, h, ni, nj, nk, mode="native", threads=
 
 
 def repository_solution(uts, us, wc, up, ut, dtr, h, ni, nj, nk):
-    """vertical_advection_repository::utens_stage_out (:176-232), column by column, vectorised over (j, i)."""
+    """vertical_advection_repository::utens_stage_out (:95-151), column by column, vectorised over (j, i)."""
     J, I = slice(h, h + nj), slice(h, h + ni)
     Ip = slice(h + 1, h + 1 + ni)
     c = np.zeros((nk, nj, ni))
@@ -245,3 +246,20 @@ def test_vadv_threads_do_not_change_bits(oracle):
     a = run(oracle, *f, 0.15, h, ni, nj, nk, threads=1)
     b = run(oracle, *f, 0.15, h, ni, nj, nk, threads=5)
     assert np.array_equal(a, b)
+
+
+def test_vadv_infinite_top_level_matches_fixture(oracle):
+    """Infinite / NaN u_stage at the top level and infinite winds on the unread staggered level nk: the oracle
+    equals the fixture written by an independent pure-Python transcription of the reference functors
+    (tests/golden/make_vadv_inf_top.py), NaN and infinity positions included."""
+    import os
+
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "vadv_inf_top.npz"))
+    ni, nj, nk, h = (int(x) for x in z["dims"])
+    fl = [np.ascontiguousarray(z[n]) for n in ("utens_stage", "u_stage", "wcon", "u_pos", "utens")]
+    with np.errstate(invalid="ignore", over="ignore"):
+        got = run(oracle, *fl, float(z["dtr"]), h, ni, nj, nk)
+    exp = z["expected"]
+    sl = (slice(None), slice(h, h + nj), slice(h, h + ni))
+    assert np.isnan(exp[sl]).any() and np.isinf(exp[sl]).any()
+    np.testing.assert_array_equal(got[sl], exp[sl])
