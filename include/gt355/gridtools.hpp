@@ -274,18 +274,148 @@ namespace gt355 {
         inline const builder_type<Traits> builder = {};
     } // namespace storage
 
+    // literals::operator""_c (common/integral_constant.hpp): 3_c is std::integral_constant<int, 3>
+    namespace literals {
+        namespace literals_impl {
+            template <char... Cs>
+            constexpr int digits() {
+                int v = 0;
+                ((v = v * 10 + (Cs - '0')), ...);
+                return v;
+            }
+        } // namespace literals_impl
+        template <char... Cs>
+        constexpr std::integral_constant<int, literals_impl::digits<Cs...>()> operator""_c() {
+            return {};
+        }
+    } // namespace literals
+
     namespace stencil {
-        // core::grid (stencil/core/grid.hpp:30-128) for one k interval starting at 0
-        struct grid {
-            int i_start, i_size, j_start, j_size, k_size;
-            std::array<int, 3> origin() const { return {i_start, j_start, 0}; }
+        // ------------------------------------------------------------------------------------------ k levels
+        // level<splitter, offset> (stencil/core/level.hpp): offset > 0 counts up from the splitter (1 = the
+        // splitter's own level), offset < 0 counts down (-1 = the level below the splitter); 0 is not a level.
+        template <int Splitter, int Offset>
+        struct level {
+            static_assert(Splitter >= 0 && Offset != 0, "level: splitter >= 0 and a non-zero offset");
+            static_assert(Offset >= -4 && Offset <= 4, "level offsets are limited to +-4");
+            static constexpr int splitter = Splitter, offset = Offset;
         };
 
-        // make_grid(halo_descriptor, halo_descriptor, k_size) (stencil/frontend/make_grid.hpp:19-42)
-        inline grid make_grid(halo_descriptor const &hi, halo_descriptor const &hj, int k_size) {
-            return {hi.begin(), hi.end() - hi.begin() + 1, hj.begin(), hj.end() - hj.begin() + 1, k_size};
+        // offset arithmetic that skips 0 (interval_impl::add_offset, stencil/core/interval.hpp)
+        constexpr int shift_level_offset(int o, int d) {
+            int r = o + d;
+            if (o > 0 && r <= 0)
+                r -= 1;
+            else if (o < 0 && r >= 0)
+                r += 1;
+            return r;
         }
-        inline grid make_grid(int ni, int nj, int nk) { return {0, ni, 0, nj, nk}; }
+
+        // interval<from, to> (stencil/core/interval.hpp): a closed range of levels; modify<l, r> moves its ends,
+        // shift<d> moves both, first_level / last_level are its end levels
+        template <class From, class To>
+        struct interval {
+            static_assert(From::splitter < To::splitter ||
+                              (From::splitter == To::splitter && From::offset <= To::offset),
+                "interval: the from level must not be above the to level");
+            using from_t = From;
+            using to_t = To;
+            template <int A, int B>
+            using modify = interval<level<From::splitter, shift_level_offset(From::offset, A)>,
+                level<To::splitter, shift_level_offset(To::offset, B)>>;
+            template <int D>
+            using shift = modify<D, D>;
+            using first_level = interval<From, From>;
+            using last_level = interval<To, To>;
+        };
+
+        namespace axis_config {
+            template <int L>
+            struct offset_limit {
+                static constexpr int value = L;
+            };
+        } // namespace axis_config
+
+        // axis<N, offset_limit<L>> (stencil/frontend/axis.hpp:37-72): N consecutive k intervals between N + 1
+        // splitters, built from the intervals' sizes; get_interval<i, ...> is interval i (or consecutive ones)
+        template <int N, class Limit = axis_config::offset_limit<2>>
+        class axis {
+            static_assert(N >= 1 && N <= 8, "axis: 1 to 8 intervals");
+            static_assert(Limit::value >= 1 && Limit::value <= 4, "axis: level offsets are limited to +-4");
+            std::array<int, N> m_sizes;
+
+            template <int I, int... Is>
+            struct interval_of {
+                static constexpr bool consecutive() {
+                    int e = I;
+                    bool ok = true;
+                    ((ok = ok && Is == ++e), ...);
+                    return ok;
+                }
+                static_assert(consecutive(), "axis::get_interval: the intervals must be consecutive");
+                static_assert(I >= 0 && I + (int)sizeof...(Is) < N, "axis::get_interval: interval id out of range");
+                using type = interval<level<I, 1>, level<I + (int)sizeof...(Is) + 1, -1>>;
+            };
+
+          public:
+            static constexpr int n_intervals = N;
+            using full_interval = interval<level<0, 1>, level<N, -1>>;
+            using axis_interval_t = full_interval;
+
+            template <class... Sizes,
+                std::enable_if_t<sizeof...(Sizes) == N && (std::is_convertible<Sizes, int>::value && ...), int> = 0>
+            axis(Sizes... sizes) : m_sizes{{static_cast<int>(sizes)...}} {}
+
+            int interval_size(int i) const { return m_sizes.at(i); }
+            std::array<int, N> const &interval_sizes() const { return m_sizes; }
+
+            template <int I, int... Is>
+            using get_interval = typename interval_of<I, Is...>::type;
+        };
+
+        // core::grid (stencil/core/grid.hpp:30-128): the (i, j) compute box and the k positions of the axis
+        // splitters (splitter 0 at k = 0, splitter N at k = k_size)
+        template <int NIntervals = 1>
+        struct grid_t {
+            int i_start = 0, i_size = 0, j_start = 0, j_size = 0, k_size = 0;
+            std::array<int, NIntervals + 1> splitters{};
+            static constexpr int n_intervals = NIntervals;
+            std::array<int, 3> origin() const { return {i_start, j_start, 0}; }
+        };
+        using grid = grid_t<1>;
+
+        namespace grid_impl {
+            template <int N>
+            grid_t<N> make(int i0, int ni, int j0, int nj, std::array<int, N> const &sizes) {
+                grid_t<N> g;
+                g.i_start = i0, g.i_size = ni, g.j_start = j0, g.j_size = nj;
+                g.splitters[0] = 0;
+                for (int n = 0; n < N; ++n) {
+                    if (sizes[n] < 0)
+                        throw std::invalid_argument("make_grid: negative interval size");
+                    g.splitters[n + 1] = g.splitters[n] + sizes[n];
+                }
+                g.k_size = g.splitters[N];
+                return g;
+            }
+        } // namespace grid_impl
+
+        // make_grid (stencil/frontend/make_grid.hpp:19-42): from halo descriptors or sizes, with a k size (one
+        // interval) or an axis
+        inline grid make_grid(halo_descriptor const &hi, halo_descriptor const &hj, int k_size) {
+            return grid_impl::make<1>(
+                hi.begin(), hi.end() - hi.begin() + 1, hj.begin(), hj.end() - hj.begin() + 1, {{k_size}});
+        }
+        inline grid make_grid(int ni, int nj, int nk) { return grid_impl::make<1>(0, ni, 0, nj, {{nk}}); }
+        template <int N, class L>
+        grid_t<N> make_grid(halo_descriptor const &hi, halo_descriptor const &hj, axis<N, L> const &ax) {
+            return grid_impl::make<N>(
+                hi.begin(), hi.end() - hi.begin() + 1, hj.begin(), hj.end() - hj.begin() + 1, ax.interval_sizes());
+        }
+        template <int N, class L>
+        grid_t<N> make_grid(int ni, int nj, axis<N, L> const &ax) {
+            return grid_impl::make<N>(0, ni, 0, nj, ax.interval_sizes());
+        }
 
         template <class T>
         struct global_parameter_t {
@@ -545,8 +675,8 @@ namespace gt355 {
                 "expandable_run(prepare_tracers)");
         }
 
-        template <size_t Factor, class Spec, class Be, class... Args>
-        void expandable_run(Spec spec, Be be, grid const &g, Args const &... args) {
+        template <size_t Factor, class Spec, class Be, int N, class... Args>
+        void expandable_run(Spec spec, Be be, grid_t<N> const &g, Args const &... args) {
             static_assert(Factor > 0, "expandable_run: Factor must be positive");
             size_t n = 0;
             for (size_t c : {expandable_impl::count(args)...}) {

@@ -60,81 +60,51 @@ namespace gt355 {
         } // namespace dim
 
         // ------------------------------------------------------------------------------------------ k intervals
-        // level<splitter, offset>: offset > 0 counts up from the splitter (1 = the splitter's own level), offset < 0
-        // counts down (-1 = the level below the splitter); 0 is not a level (stencil/frontend/axis.hpp:37-72).
-        template <int Splitter, int Offset>
-        struct level {
-            static_assert(Offset != 0, "level offsets are non-zero");
-            static constexpr int splitter = Splitter, offset = Offset;
-        };
-
-        constexpr int shift_level_offset(int o, int d) {
-            int r = o + d;
-            if (o > 0 && r <= 0)
-                r -= 1;
-            else if (o < 0 && r >= 0)
-                r += 1;
-            return r;
-        }
-
-        template <class From, class To>
-        struct interval {
-            using from_t = From;
-            using to_t = To;
-            template <int A, int B>
-            using modify = interval<level<From::splitter, shift_level_offset(From::offset, A)>,
-                level<To::splitter, shift_level_offset(To::offset, B)>>;
-            using first_level = interval<From, From>;
-            using last_level = interval<To, To>;
-        };
-
-        namespace axis_config {
-            template <int L>
-            struct offset_limit {};
-        } // namespace axis_config
-
-        // axis<N> (stencil/frontend/axis.hpp:37-72): N intervals between N + 1 splitters; the engine supports N = 1
-        // (apply overloads may use level offsets up to +-4)
-        template <int N, class = void>
-        struct axis {
-            static_assert(N == 1, "the generic engine supports one k interval (axis<1>)");
-            using full_interval = interval<level<0, 1>, level<1, -1>>;
-            static constexpr int n_intervals = N;
-        };
-
+        // level, interval, axis and grid_t are the front end's (gt355/gridtools.hpp)
         namespace engine_ {
-            // k index of a level for a domain of nk levels (splitter 0 at 0, splitter 1 at nk)
-            __host__ __device__ constexpr int level_k(int s, int o, int nk) {
-                return (s == 0 ? 0 : nk) + (o > 0 ? o - 1 : o);
+            // the k positions of a run's axis splitters: splitter s sits at pos[s] (0 at k = 0, the last at nk)
+            struct ksplit {
+                int pos[9];
+            };
+            __host__ __device__ constexpr int level_k(int s, int o, ksplit const &ks) {
+                return ks.pos[s] + (o > 0 ? o - 1 : o);
             }
             template <class I>
-            __host__ __device__ constexpr bool in_interval(int k, int nk) {
-                return level_k(I::from_t::splitter, I::from_t::offset, nk) <= k &&
-                       k <= level_k(I::to_t::splitter, I::to_t::offset, nk);
+            __host__ __device__ bool in_interval(int k, ksplit const &ks) {
+                return level_k(I::from_t::splitter, I::from_t::offset, ks) <= k &&
+                       k <= level_k(I::to_t::splitter, I::to_t::offset, ks);
             }
         } // namespace engine_
 
         // ------------------------------------------------------------------------------------------ accessors
         namespace cartesian {
+            // in_accessor / inout_accessor (stencil/frontend/cartesian/accessor.hpp): offsets in i, j, k and, for
+            // accessors of more than three dimensions, in up to three extra data dimensions (x[0..2])
             template <int I, class Extent = extent<>, int NDims = 3>
             struct in_accessor {
-                static_assert(NDims == 3, "the generic engine supports three-dimensional accessors");
+                static_assert(NDims >= 1 && NDims <= 6, "accessors have 1 to 6 dimensions");
                 static constexpr int index = I;
                 static constexpr bool is_const = true;
+                static constexpr int ndims = NDims;
                 using extent_t = Extent;
                 int di = 0, dj = 0, dk = 0;
+                int x[3] = {0, 0, 0};
                 GT_FUNCTION constexpr in_accessor() = default;
-                GT_FUNCTION constexpr in_accessor(int i, int j = 0, int k = 0) : di(i), dj(j), dk(k) {}
+                GT_FUNCTION constexpr in_accessor(int i, int j = 0, int k = 0, int x0 = 0, int x1 = 0, int x2 = 0)
+                    : di(i), dj(j), dk(k), x{x0, x1, x2} {}
             };
             template <int I, class Extent = extent<>, int NDims = 3>
             struct inout_accessor {
-                static_assert(NDims == 3, "the generic engine supports three-dimensional accessors");
+                static_assert(NDims >= 1 && NDims <= 6, "accessors have 1 to 6 dimensions");
                 static constexpr int index = I;
                 static constexpr bool is_const = false;
+                static constexpr int ndims = NDims;
                 using extent_t = Extent;
                 int di = 0, dj = 0, dk = 0;
+                int x[3] = {0, 0, 0};
                 GT_FUNCTION constexpr inout_accessor() = default;
-                GT_FUNCTION constexpr inout_accessor(int i, int j = 0, int k = 0) : di(i), dj(j), dk(k) {}
+                GT_FUNCTION constexpr inout_accessor(int i, int j = 0, int k = 0, int x0 = 0, int x1 = 0, int x2 = 0)
+                    : di(i), dj(j), dk(k), x{x0, x1, x2} {}
             };
             // accessor<I, intent, extent> (stencil/frontend/cartesian/accessor.hpp): the same two accessor kinds
             enum class intent { in, inout };
@@ -238,6 +208,44 @@ namespace gt355 {
         // positional<dim>() (stencil/positional.hpp): the storage index of the evaluation point along one dimension
         template <class Dim>
         struct positional {};
+
+        // sid::rename_dimensions<Old, New>(field) (sid/rename_dimensions.hpp): the field's dimension Old (i, j or k)
+        // is seen as data dimension New (3, 4 or 5; an accessor of more than three dimensions indexes it with its
+        // extra offsets), and the grid no longer moves along it -- whole_axis_access reads a whole k column at every
+        // point this way.  The renamed dimension is not bounds-checked.
+        namespace sid {
+            template <class T, int From, int To>
+            struct renamed {
+                std::shared_ptr<storage::data_store<T>> store;
+            };
+            template <class D>
+            constexpr int dim_index() {
+                return std::is_same<D, dim::i>::value ? 0 : std::is_same<D, dim::j>::value ? 1 : 2;
+            }
+            template <class From, class To, class T>
+            renamed<T, dim_index<From>(), (int)To::value> rename_dimensions(
+                std::shared_ptr<storage::data_store<T>> const &s) {
+                static_assert(To::value >= 3 && To::value <= 5, "rename_dimensions: the new dimension is 3, 4 or 5");
+                return {s};
+            }
+            template <class T>
+            std::shared_ptr<storage::data_store<T>> const &store_of(std::shared_ptr<storage::data_store<T>> const &s) {
+                return s;
+            }
+            template <class T, int F, int To>
+            std::shared_ptr<storage::data_store<T>> const &store_of(renamed<T, F, To> const &r) {
+                return r.store;
+            }
+            template <class T>
+            void rename(std::shared_ptr<storage::data_store<T>> const &, gt355_field &, int64_t *) {}
+            template <class T, int F, int To>
+            void rename(renamed<T, F, To> const &, gt355_field &f, int64_t *sx) {
+                sx[To - 3] = f.stride[F];
+                f.stride[F] = 0;
+                f.lower[F] = -(1 << 30);
+                f.upper[F] = 1 << 30;
+            }
+        } // namespace sid
 
         // ------------------------------------------------------------------------------------------ placeholders
         namespace engine_ {
@@ -367,9 +375,17 @@ namespace gt355 {
             struct field_view {
                 T *origin;
                 int64_t s0, s1, s2;
+                int64_t sx[3]; // strides of the extra data dimensions (0 where the field has none)
                 __device__ T &at(int i, int j, int k) const { return origin[i * s0 + j * s1 + k * s2]; }
+                __device__ T &at(int i, int j, int k, const int (&x)[3]) const {
+                    return origin[i * s0 + j * s1 + k * s2 + x[0] * sx[0] + x[1] * sx[1] + x[2] * sx[2]];
+                }
                 __device__ void rebase(int, int) {}
             };
+            template <class V>
+            struct is_field_view : std::false_type {};
+            template <class T>
+            struct is_field_view<field_view<T>> : std::true_type {};
             template <class T>
             struct global_view {
                 T v;
@@ -445,14 +461,31 @@ namespace gt355 {
                 return vget<idx>(rs);
             }
 
-            template <class Views, class Rings = vtuple<>>
+            // The precision policy of a read-only access (the backend's Deref hook, be_api.hpp:160-169,
+            // frontend/cartesian/stage.hpp:53-63): with W (GT355_WIDEN) a float value is read as a double; otherwise
+            // the value or reference passes through.  Writable accesses always see the storage type, so a store
+            // narrows once.
+            template <bool W, class X>
+            __device__ __forceinline__ decltype(auto) widen_read(X &&x) {
+                if constexpr (W && std::is_same<std::decay_t<X>, float>::value)
+                    return double(x);
+                else if constexpr (std::is_lvalue_reference<X>::value)
+                    return static_cast<X>(x);
+                else
+                    return std::decay_t<X>(x);
+            }
+
+            template <class Views, class Rings = vtuple<>, bool W = false>
             struct evaluator {
                 Views const &v;
                 Rings &rs;
                 int i, j, k;
                 template <class View, class Acc>
                 __device__ __forceinline__ decltype(auto) get(View const &view, Acc const &a) const {
-                    return view.at(i + a.di, j + a.dj, k + a.dk);
+                    if constexpr (is_field_view<View>::value)
+                        return view.at(i + a.di, j + a.dj, k + a.dk, a.x);
+                    else
+                        return view.at(i + a.di, j + a.dj, k + a.dk);
                 }
                 template <class T, int KMIN, int KMAX, int KEY, class Acc>
                 __device__ __forceinline__ T &get(kring_ref<T, KMIN, KMAX, KEY> const &, Acc const &a) const {
@@ -460,7 +493,7 @@ namespace gt355 {
                 }
                 template <int I, class E, int N>
                 __device__ __forceinline__ decltype(auto) operator()(cartesian::in_accessor<I, E, N> const &a) const {
-                    return get(vget<I>(v), a);
+                    return widen_read<W>(get(vget<I>(v), a));
                 }
                 template <int I, class E, int N>
                 __device__ __forceinline__ decltype(auto) operator()(cartesian::inout_accessor<I, E, N> const &a) const {
@@ -716,76 +749,77 @@ namespace gt355 {
                 using type = tlist<A..., B...>;
             };
 
-            // candidate intervals: every [level<s1, o1>, level<s2, o2>] with s in {0, 1}, o in +-1..+-4
-            constexpr int cand_offset(int n) { return n < 4 ? n - 4 : n - 3; } // 0..7 -> -4..-1, 1..4
+            // candidate intervals of an axis with NSPL splitters: every [level<s1, o1>, level<s2, o2>] with s in
+            // [0, NSPL), o in +-1..+-4 and the first level not above the second.  Level n of the enumeration is
+            // splitter n / 8 with offsets -4, -3, -2, -1, 1, 2, 3, 4 in k order.
+            constexpr int cand_offset(int n) { return n < 4 ? n - 4 : n - 3; }
             template <int N>
             using cand_level = level<N / 8, cand_offset(N % 8)>;
-            template <class Seq>
-            struct all_candidates;
-            template <int... Ns>
-            struct all_candidates<std::integer_sequence<int, Ns...>> {
-                using type = tlist<interval<cand_level<Ns / 16>, cand_level<Ns % 16>>...>;
-            };
-            using candidates_t = typename all_candidates<std::make_integer_sequence<int, 256>>::type;
-
-            template <class F, class E, class L>
-            struct matching;
-            template <class F, class E>
-            struct matching<F, E, tlist<>> {
-                using type = tlist<>;
-            };
-            template <class F, class E, class I, class... Is>
-            struct matching<F, E, tlist<I, Is...>> {
-                using rest = typename matching<F, E, tlist<Is...>>::type;
-                using type =
-                    std::conditional_t<has_interval_apply<F, E, I>::value, typename tcat<tlist<I>, rest>::type, rest>;
+            template <int M, int N>
+            struct cand {
+                static constexpr bool valid = N / M <= N % M;
+                using type = interval<cand_level<N / M>, cand_level<N % M>>;
             };
 
-            // runs F at level k of an nk-level domain: the plain apply everywhere, else the interval overload whose
-            // interval contains k (none: nothing to do at this level)
-            template <class F, class E, class... Is>
-            __device__ void apply_at(E &e, int k, int nk, tlist<Is...>) {
+            // runs F at level k: the plain apply everywhere, else the first interval overload whose interval
+            // contains k (none: nothing to do at this level), like the reference's interval matching
+            template <class F, class E, int M, int N>
+            __device__ __forceinline__ void try_interval(E &e, int k, ksplit const &ks, bool &done) {
+                if constexpr (cand<M, N>::valid) {
+                    using I = typename cand<M, N>::type;
+                    if constexpr (has_interval_apply<F, E, I>::value) {
+                        if (!done && in_interval<I>(k, ks)) {
+                            F::apply(e, I());
+                            done = true;
+                        }
+                    }
+                }
+            }
+            template <class F, class E, int M, int... Ns>
+            __device__ __forceinline__ void apply_at(E &e, int k, ksplit const &ks, std::integer_sequence<int, Ns...>) {
                 if constexpr (has_plain_apply<F, E>::value) {
                     F::apply(e);
                 } else {
                     bool done = false;
-                    (void)std::initializer_list<int>{(
-                        (!done && in_interval<Is>(k, nk) ? (F::apply(e, Is()), done = true) : false), 0)...};
+                    (try_interval<F, E, M, Ns>(e, k, ks, done), ...);
                 }
             }
-            template <class F, class Views, class Rings>
-            __device__ __forceinline__ void run_stage_at(Views const &v, Rings &rs, int i, int j, int k, int nk) {
-                using E = evaluator<Views, Rings>;
+            template <class F, int NSPL, bool W, class Views, class Rings>
+            __device__ __forceinline__ void run_stage_at(
+                Views const &v, Rings &rs, int i, int j, int k, ksplit const &ks) {
+                using E = evaluator<Views, Rings, W>;
                 E e{v, rs, i, j, k};
-                apply_at<F>(e, k, nk, typename matching<F, E, candidates_t>::type());
+                constexpr int M = NSPL * 8;
+                apply_at<F, E, M>(e, k, ks, std::make_integer_sequence<int, M * M>());
             }
-            template <class F, class Views>
-            __device__ __forceinline__ void run_stage_at(Views const &v, int i, int j, int k, int nk) {
+            template <class F, int NSPL, bool W, class Views>
+            __device__ __forceinline__ void run_stage_at(Views const &v, int i, int j, int k, ksplit const &ks) {
                 vtuple<> none;
-                run_stage_at<F>(v, none, i, j, k, nk);
+                run_stage_at<F, NSPL, W>(v, none, i, j, k, ks);
             }
 
             // ------------------------------------------------------------------------------------------ kernels
-            template <class F, class Views>
-            __global__ void __launch_bounds__(256) parallel_stage_kernel(Views v, int i0, int ni, int j0, int nj, int nk) {
+            template <class F, int NSPL, bool W, class Views>
+            __global__ void __launch_bounds__(256) parallel_stage_kernel(
+                Views v, int i0, int ni, int j0, int nj, ksplit ks) {
                 const int i = i0 + (int)(blockIdx.x * 64 + (threadIdx.x & 63));
                 const int j = j0 + (int)(blockIdx.y * 4 + (threadIdx.x >> 6));
                 const int k = (int)blockIdx.z;
                 if (i >= i0 + ni || j >= j0 + nj)
                     return;
-                run_stage_at<F>(v, i, j, k, nk);
+                run_stage_at<F, NSPL, W>(v, i, j, k, ks);
             }
 
             // one vertical multistage: stage s runs on columns inside its own extent box dom[s] = {i0, i1, j0, j1}
-            template <class Exec, class StageList, class ViewsTuple>
+            template <class Exec, int NSPL, bool W, class StageList, class ViewsTuple>
             struct vertical_runner;
-            template <class Exec, class... Stages, class... Views>
-            struct vertical_runner<Exec, tlist<Stages...>, vtuple<Views...>> {
+            template <class Exec, int NSPL, bool W, class... Stages, class... Views>
+            struct vertical_runner<Exec, NSPL, W, tlist<Stages...>, vtuple<Views...>> {
                 template <class Rings, int... Ss>
                 __device__ static void level(vtuple<Views...> const &v, Rings &rs, const int (*dom)[4], int i, int j, int k,
-                    int nk, std::integer_sequence<int, Ss...>) {
+                    ksplit const &ks, std::integer_sequence<int, Ss...>) {
                     ((i >= dom[Ss][0] && i < dom[Ss][1] && j >= dom[Ss][2] && j < dom[Ss][3]
-                             ? run_stage_at<typename Stages::functor>(vget<Ss>(v), rs, i, j, k, nk)
+                             ? run_stage_at<typename Stages::functor, NSPL, W>(vget<Ss>(v), rs, i, j, k, ks)
                              : void()),
                         ...);
                 }
@@ -829,21 +863,22 @@ namespace gt355 {
                 (f(vget<Is>(rs)), ...);
             }
 
-            template <class Exec, class StageList, class ViewsTuple, class RingSet, int NS>
+            template <class Exec, int NSPL, bool W, class StageList, class ViewsTuple, class RingSet, int NS>
             __global__ void __launch_bounds__(256) vertical_kernel(
-                ViewsTuple v0, RingSet rs0, domains<NS> dom, int i0, int ni, int j0, int nj, int nk) {
+                ViewsTuple v0, RingSet rs0, domains<NS> dom, int i0, int ni, int j0, int nj, ksplit ks) {
                 const int i = i0 + (int)(blockIdx.x * 64 + (threadIdx.x & 63));
                 const int j = j0 + (int)(blockIdx.y * 4 + (threadIdx.x >> 6));
                 if (i >= i0 + ni || j >= j0 + nj)
                     return;
-                using R = vertical_runner<Exec, StageList, ViewsTuple>;
+                using R = vertical_runner<Exec, NSPL, W, StageList, ViewsTuple>;
                 constexpr bool up = !std::is_same<Exec, backward>::value;
+                const int nk = ks.pos[NSPL - 1];
                 RingSet rs = rs0;
                 const int k0 = up ? 0 : nk - 1;
                 for_rings(rs, [&](auto &r) { r.start(i, j, k0, nk); });
                 for (int n = 0; n < nk; ++n) {
                     const int k = up ? n : nk - 1 - n;
-                    R::level(v0, rs, dom.d, i, j, k, nk, std::make_integer_sequence<int, NS>());
+                    R::level(v0, rs, dom.d, i, j, k, ks, std::make_integer_sequence<int, NS>());
                     for_rings(rs, [&](auto &r) { r.flush(i, j); });
                     if (n + 1 < nk)
                         for_rings(rs, [&](auto &r) { r.template slide<up>(i, j, nk); });
@@ -853,39 +888,40 @@ namespace gt355 {
             // one fused parallel multistage: a workgroup owns a TI x TJ tile of one k-level and runs every stage over the
             // tile extended by the stage's extent (clamped to the stage's domain), with a barrier between stages; the
             // ij-cached temporaries live in LDS, so stage s+1 reads what stage s left in the tile
-            template <class F, class V>
+            template <class F, int NSPL, bool W, class V>
             __device__ void stage_tile(V const &v, const int *e, int ti0, int tj0, int TI, int TJ, int ni, int nj, int k,
-                int nk) {
+                ksplit const &ks) {
                 const int i_lo = ti0 + e[0], i_hi = min(ti0 + TI + e[1], ni + e[1]);
                 const int j_lo = tj0 + e[2], j_hi = min(tj0 + TJ + e[3], nj + e[3]);
                 // 64 lanes along i (coalesced rows), the 4 wavefronts of the workgroup along j
                 const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
                 for (int j = j_lo + wave; j < j_hi; j += 4)
                     for (int i = i_lo + lane; i < i_hi; i += 64)
-                        run_stage_at<F>(v, i, j, k, nk);
+                        run_stage_at<F, NSPL, W>(v, i, j, k, ks);
                 __syncthreads();
             }
-            template <class StageList, class ViewsTuple>
+            template <int NSPL, bool W, class StageList, class ViewsTuple>
             struct fused_runner;
-            template <class... Stages, class... Views>
-            struct fused_runner<tlist<Stages...>, vtuple<Views...>> {
+            template <int NSPL, bool W, class... Stages, class... Views>
+            struct fused_runner<NSPL, W, tlist<Stages...>, vtuple<Views...>> {
                 template <int... Ss>
                 __device__ static void run(vtuple<Views...> &v, const int (*ext)[4], int ti0, int tj0, int TI, int TJ,
-                    int ni, int nj, int k, int nk, std::integer_sequence<int, Ss...>) {
+                    int ni, int nj, int k, ksplit const &ks, std::integer_sequence<int, Ss...>) {
                     (rebase_views(vget<Ss>(v), ti0, tj0), ...);
-                    (stage_tile<typename Stages::functor>(vget<Ss>(v), ext[Ss], ti0, tj0, TI, TJ, ni, nj, k, nk), ...);
+                    (stage_tile<typename Stages::functor, NSPL, W>(vget<Ss>(v), ext[Ss], ti0, tj0, TI, TJ, ni, nj, k, ks),
+                        ...);
                 }
             };
-            template <class StageList, class ViewsTuple, class StageIns, int NS>
+            template <int NSPL, bool W, class StageList, class ViewsTuple, class StageIns, int NS>
             __global__ void __launch_bounds__(256) fused_kernel(
-                ViewsTuple v0, StageIns si, domains<NS> ext, int ni, int nj, int nk, int tiles_i, int TI, int TJ) {
+                ViewsTuple v0, StageIns si, domains<NS> ext, int ni, int nj, ksplit ks, int tiles_i, int TI, int TJ) {
                 const int tile = (int)blockIdx.x;
                 const int ti0 = (tile % tiles_i) * TI, tj0 = (tile / tiles_i) * TJ;
                 load_stage_ins(si, ti0, tj0, TI, TJ, ni, nj, (int)blockIdx.z);
                 __syncthreads();
                 ViewsTuple v = v0;
-                fused_runner<StageList, ViewsTuple>::run(
-                    v, ext.d, ti0, tj0, TI, TJ, ni, nj, (int)blockIdx.z, nk, std::make_integer_sequence<int, NS>());
+                fused_runner<NSPL, W, StageList, ViewsTuple>::run(
+                    v, ext.d, ti0, tj0, TI, TJ, ni, nj, (int)blockIdx.z, ks, std::make_integer_sequence<int, NS>());
             }
 
             // ------------------------------------------------------------------------------------------ host side
@@ -964,6 +1000,10 @@ namespace gt355 {
             struct kind_of<std::shared_ptr<storage::data_store<T>>> {
                 using type = field_kind<T>;
             };
+            template <class T, int From, int To>
+            struct kind_of<sid::renamed<T, From, To>> {
+                using type = field_kind<T>;
+            };
             template <class T>
             struct kind_of<global_parameter_t<T>> {
                 using type = global_kind<T>;
@@ -976,7 +1016,18 @@ namespace gt355 {
             struct field_rt {
                 void *origin;
                 int64_t s[3];
+                int64_t sx[3]; // extra data dimensions (0 where absent)
             };
+            // the run's compute box and axis splitters, as the host side of the engine sees them
+            struct run_geom {
+                int i_start, i_size, j_start, j_size, k_size;
+                ksplit ks;
+                std::array<int, 3> origin() const { return {i_start, j_start, 0}; }
+            };
+            template <class T>
+            field_view<T> view_of_rt(field_rt const &f) {
+                return {static_cast<T *>(f.origin), f.s[0], f.s[1], f.s[2], {f.sx[0], f.sx[1], f.sx[2]}};
+            }
             struct lds_rt {
                 int off, w, imin, jmin; // first element (in elements of the value type), row length, tile origin
                 int imax, jmax;         // the tile's far edges relative to the workgroup's block
@@ -1002,7 +1053,7 @@ namespace gt355 {
                     return {}; // a register window of the vertical kernel
                 } else if constexpr (std::is_same<typename view_of<K>::type, field_view<typename K::data_t>>::value) {
                     auto const &f = c.fields.at(P::key);
-                    return {static_cast<typename K::data_t *>(f.origin), f.s[0], f.s[1], f.s[2]};
+                    return view_of_rt<typename K::data_t>(f);
                 } else if constexpr (std::is_same<K, global_kind<typename K::data_t>>::value) {
                     return {std::get<P::key>(args).value};
                 } else {
@@ -1030,12 +1081,12 @@ namespace gt355 {
                     return stage_accesses<F, Plhs...>(typename F::param_list());
                 }
                 // allocate each temporary of this stage that has no memory yet
-                static void alloc_tmps(run_ctx &c, std::map<int, ext_rt> const &box, grid const &g, void *stream,
+                static void alloc_tmps(run_ctx &c, std::map<int, ext_rt> const &box, run_geom const &g, void *stream,
                     std::vector<void *> &taken) {
                     (alloc_one<Plhs>(c, box, g, stream, taken), ...);
                 }
                 template <class P>
-                static void alloc_one(run_ctx &c, std::map<int, ext_rt> const &box, grid const &g, void *stream,
+                static void alloc_one(run_ctx &c, std::map<int, ext_rt> const &box, run_geom const &g, void *stream,
                     std::vector<void *> &taken) {
                     if constexpr (P::is_tmp) {
                         if (c.fields.count(P::key) || c.lds.count(P::key))
@@ -1054,7 +1105,7 @@ namespace gt355 {
                         void *base = nullptr;
                         check(gt355_storage_base(&lay, raw, &base), "stencil engine: temporary layout");
                         T *origin = static_cast<T *>(base) + hi + hj * lay.stride[1] + hk * lay.stride[2];
-                        c.fields[P::key] = {origin, {1, lay.stride[1], lay.stride[2]}};
+                        c.fields[P::key] = {origin, {1, lay.stride[1], lay.stride[2]}, {0, 0, 0}};
                     }
                 }
             };
@@ -1069,9 +1120,9 @@ namespace gt355 {
                 return dim3((unsigned)((ni + 63) / 64), (unsigned)((nj + 3) / 4), (unsigned)nz);
             }
 
-            template <class Mss, class... Stages, class ArgsTuple>
+            template <class Mss, int NSPL, bool W, class... Stages, class ArgsTuple>
             void launch_mss(tlist<Stages...>, std::vector<stage_rt> const &st, int first, int m, run_ctx const &c,
-                ArgsTuple const &args, grid const &g, hipStream_t stream) {
+                ArgsTuple const &args, run_geom const &g, hipStream_t stream) {
                 using T = mss_traits<Mss>;
                 using Exec = typename T::exec;
                 constexpr int NS = sizeof...(Stages);
@@ -1092,9 +1143,9 @@ namespace gt355 {
                     assign_all<Ctx>(vt, c, args, tlist<Stages...>(), std::make_integer_sequence<int, NS>());
                     fuse_rt const &f = c.fuse[m];
                     const int tiles_i = (ni + f.ti - 1) / f.ti, tiles_j = (nj + f.tj - 1) / f.tj;
-                    fused_kernel<tlist<Stages...>, ViewsTuple, StageIns, NS>
+                    fused_kernel<NSPL, W, tlist<Stages...>, ViewsTuple, StageIns, NS>
                         <<<dim3((unsigned)(tiles_i * tiles_j), 1, (unsigned)nk), 256, f.bytes, stream>>>(
-                            vt, si, ext, ni, nj, nk, tiles_i, f.ti, f.tj);
+                            vt, si, ext, ni, nj, g.ks, tiles_i, f.ti, f.tj);
                     check_launch("stencil engine: fused multistage launch");
                 } else if constexpr (std::is_same<Exec, parallel>::value) {
                     int s = first;
@@ -1104,8 +1155,8 @@ namespace gt355 {
                         auto v = stage_plhs<S>::views(c, args);
                         const int w = ni - e.imin + e.imax, h = nj - e.jmin + e.jmax;
                         if (w > 0 && h > 0)
-                            parallel_stage_kernel<typename S::functor, decltype(v)>
-                                <<<grid_2d(w, h, nk), 256, 0, stream>>>(v, e.imin, w, e.jmin, h, nk);
+                            parallel_stage_kernel<typename S::functor, NSPL, W, decltype(v)>
+                                <<<grid_2d(w, h, nk), 256, 0, stream>>>(v, e.imin, w, e.jmin, h, g.ks);
                         check_launch("stencil engine: parallel stage launch");
                     };
                     (one(Stages()), ...);
@@ -1126,8 +1177,8 @@ namespace gt355 {
                     init_rings(rs, c, st, first, NS, dom.d);
                     const int w = ni - u.imin + u.imax, h = nj - u.jmin + u.jmax;
                     if (w > 0 && h > 0 && nk > 0)
-                        vertical_kernel<Exec, tlist<Stages...>, ViewsTuple, RingSet, NS>
-                            <<<grid_2d(w, h, 1), 256, 0, stream>>>(vt, rs, dom, u.imin, w, u.jmin, h, nk);
+                        vertical_kernel<Exec, NSPL, W, tlist<Stages...>, ViewsTuple, RingSet, NS>
+                            <<<grid_2d(w, h, 1), 256, 0, stream>>>(vt, rs, dom, u.imin, w, u.jmin, h, g.ks);
                     check_launch("stencil engine: vertical multistage launch");
                 }
             }
@@ -1163,7 +1214,7 @@ namespace gt355 {
                     r.wb[0] = r.wb[1] = r.wb[2] = r.wb[3] = 0;
                 auto it = c.fields.find(Ring::key);
                 if (it != c.fields.end())
-                    r.mem = {static_cast<T *>(it->second.origin), it->second.s[0], it->second.s[1], it->second.s[2]};
+                    r.mem = view_of_rt<T>(it->second);
                 else
                     r.fb[0] = r.fb[1] = r.wb[0] = r.wb[1] = 0; // no memory: a purely register-held temporary
             }
@@ -1188,7 +1239,7 @@ namespace gt355 {
             void init_stage_in(stage_in<T> &s, run_ctx const &c) {
                 auto const &f = c.fields.at(P::key);
                 auto const &l = c.lds.at(P::key);
-                s.src = {static_cast<T *>(f.origin), f.s[0], f.s[1], f.s[2]};
+                s.src = view_of_rt<T>(f);
                 s.off = l.off, s.w = l.w, s.imin = l.imin, s.jmin = l.jmin;
                 s.imax = l.imax, s.jmax = l.jmax;
             }
@@ -1320,7 +1371,7 @@ namespace gt355 {
 
             // user-field bookkeeping: bounds check and host -> device synchronisation of every field argument
             template <int N, class A>
-            void bind_arg(run_ctx &c, A const &a, std::vector<stage_rt> const &st, grid const &g) {
+            void bind_arg(run_ctx &c, A const &a, std::vector<stage_rt> const &st, run_geom const &g) {
                 if constexpr (std::is_same<typename kind_of<A>::type, field_kind<typename kind_of<A>::type::data_t>>::value) {
                     bool written = false;
                     ext_rt need;
@@ -1334,20 +1385,23 @@ namespace gt355 {
                             }
                     if (!need.set)
                         return;
-                    gt355_field f = a->sid(g.origin(), written);
+                    auto const &store = sid::store_of(a);
+                    gt355_field f = store->sid(g.origin(), written);
+                    int64_t sx[3] = {0, 0, 0};
+                    sid::rename(a, f, sx);
                     auto bad = [&](int d, int lo, int hi) {
                         return f.stride[d] != 0 && (lo < f.lower[d] || hi > f.upper[d]);
                     };
                     if (bad(0, need.imin, g.i_size + need.imax) || bad(1, need.jmin, g.j_size + need.jmax))
-                        throw std::runtime_error("run: field argument " + std::to_string(N) + " (" + a->name() +
+                        throw std::runtime_error("run: field argument " + std::to_string(N) + " (" + store->name() +
                                                  ") is too small for the extents the stencil reads or writes");
                     if (written && (f.stride[0] == 0 || f.stride[1] == 0 || f.stride[2] == 0))
                         throw std::runtime_error("run: a written field cannot have masked dimensions");
-                    c.fields[N] = {f.origin, {f.stride[0], f.stride[1], f.stride[2]}};
+                    c.fields[N] = {f.origin, {f.stride[0], f.stride[1], f.stride[2]}, {sx[0], sx[1], sx[2]}};
                 }
             }
             template <class ArgsTuple, size_t... Is>
-            void bind_args(run_ctx &c, ArgsTuple const &args, std::vector<stage_rt> const &st, grid const &g,
+            void bind_args(run_ctx &c, ArgsTuple const &args, std::vector<stage_rt> const &st, run_geom const &g,
                 std::index_sequence<Is...>) {
                 (bind_arg<(int)Is>(c, std::get<Is>(args), st, g), ...);
             }
@@ -1410,13 +1464,13 @@ namespace gt355 {
                 return box;
             }
 
-            template <class... Msses, class ArgsTuple>
+            template <int NSPL, bool W, class... Msses, class ArgsTuple>
             void launch_all(tlist<Msses...>, std::vector<stage_rt> const &st, run_ctx const &c, ArgsTuple const &args,
-                grid const &g, hipStream_t stream) {
+                run_geom const &g, hipStream_t stream) {
                 int first = 0, m = 0;
                 auto one = [&](auto mss) {
                     using T = mss_traits<decltype(mss)>;
-                    launch_mss<decltype(mss)>(typename T::stages(), st, first, m++, c, args, g, stream);
+                    launch_mss<decltype(mss), NSPL, W>(typename T::stages(), st, first, m++, c, args, g, stream);
                     first += T::n;
                 };
                 (one(Msses()), ...);
@@ -1425,7 +1479,7 @@ namespace gt355 {
             // sid/allocator.hpp:44-57, made safe across streams) and go back to it after the run's launches are
             // queued on `stream`
             template <class... Msses>
-            void alloc_all(tlist<Msses...>, run_ctx &c, std::map<int, ext_rt> const &box, grid const &g, void *stream,
+            void alloc_all(tlist<Msses...>, run_ctx &c, std::map<int, ext_rt> const &box, run_geom const &g, void *stream,
                 std::vector<void *> &taken) {
                 auto one = [&](auto mss) {
                     alloc_stages(typename mss_traits<decltype(mss)>::stages(), c, box, g, stream, taken);
@@ -1433,7 +1487,7 @@ namespace gt355 {
                 (one(Msses()), ...);
             }
             template <class... Stages>
-            void alloc_stages(tlist<Stages...>, run_ctx &c, std::map<int, ext_rt> const &box, grid const &g,
+            void alloc_stages(tlist<Stages...>, run_ctx &c, std::map<int, ext_rt> const &box, run_geom const &g,
                 void *stream, std::vector<void *> &taken) {
                 (stage_plhs<Stages>::alloc_tmps(c, box, g, stream, taken), ...);
             }
@@ -1479,7 +1533,8 @@ namespace gt355 {
                 __device__ __forceinline__ decltype(auto) bind(P &&p, Acc const &a) const {
                     using A = std::decay_t<P>;
                     if constexpr (is_accessor<A>::value)
-                        return e(A(p.di + a.di + OI, p.dj + a.dj + OJ, p.dk + a.dk + OK));
+                        return e(A(p.di + a.di + OI, p.dj + a.dj + OJ, p.dk + a.dk + OK, p.x[0] + a.x[0],
+                            p.x[1] + a.x[1], p.x[2] + a.x[2]));
                     else
                         return std::forward<P>(p);
                 }
@@ -1556,14 +1611,61 @@ namespace gt355 {
             };
         } // namespace cartesian
 
+        namespace engine_ {
+            // The precision policy's temporaries: under GT355_WIDEN a temporary declared with the storage type float
+            // is held in double (memory, LDS tile or register window), so k-caches, pivots and intermediate fields
+            // keep the compute type and only the stores to user fields narrow.
+            template <class P>
+            struct promote {
+                using type = P;
+            };
+            template <int Id>
+            struct promote<tmp_plh<Id, float>> {
+                using type = tmp_plh<Id, double>;
+            };
+            template <class P, bool Fi, bool Fl>
+            struct promote<kc<P, Fi, Fl>> {
+                using type = kc<typename promote<P>::type, Fi, Fl>;
+            };
+            template <class F, class... Ps>
+            struct promote<stage_desc<F, Ps...>> {
+                using type = stage_desc<F, typename promote<Ps>::type...>;
+            };
+            template <class... Ts>
+            struct promote<tlist<Ts...>> {
+                using type = tlist<typename promote<Ts>::type...>;
+            };
+            template <class Exec, class Caches, class... Stages>
+            struct promote<mss_desc<Exec, Caches, Stages...>> {
+                using type = mss_desc<Exec, typename promote<Caches>::type, typename promote<Stages>::type...>;
+            };
+            template <class... Msses>
+            struct promote<multi_pass_desc<Msses...>> {
+                using type = multi_pass_desc<typename promote<Msses>::type...>;
+            };
+
+            template <int N>
+            run_geom geom_of(grid_t<N> const &g) {
+                run_geom r{g.i_start, g.i_size, g.j_start, g.j_size, g.k_size, {}};
+                for (int s = 0; s <= N; ++s)
+                    r.ks.pos[s] = g.splitters[s];
+                return r;
+            }
+        } // namespace engine_
+
         // run(spec, backend, grid, fields...) (stencil/frontend/run.hpp:201-250) for a spec lambda: the spec is called
         // with one placeholder per field argument and returns execute_*().stage(...) chains or a multi_pass of them.
-        template <class Spec, class Be, class... Args,
+        // The backend tag's precision selects the Deref policy: GT355_NATIVE computes in the storage type;
+        // GT355_WIDEN (float storage) reads read-only fields and global parameters as double, keeps temporaries in
+        // double and narrows on the stores to user fields.
+        template <class Spec, class Be, int N, class... Args,
             std::enable_if_t<engine_::is_spec<engine_::spec_of<Spec, Args...>>::value, int> = 0>
-        void run(Spec, Be be, grid const &g, Args const &... args) {
-            static_assert(Be::precision == GT355_NATIVE, "the generic engine computes in the storage type");
-            using spec_t = engine_::spec_of<Spec, Args...>;
+        void run(Spec, Be be, grid_t<N> const &grid, Args const &... args) {
+            constexpr bool W = Be::precision == GT355_WIDEN;
+            using raw_spec_t = engine_::spec_of<Spec, Args...>;
+            using spec_t = std::conditional_t<W, typename engine_::promote<raw_spec_t>::type, raw_spec_t>;
             using msses = typename engine_::msses_of<spec_t>::type;
+            const engine_::run_geom g = engine_::geom_of(grid);
             std::vector<engine_::stage_rt> st;
             engine_::collect(msses(), st);
             auto box = engine_::analyse(st);
@@ -1574,19 +1676,17 @@ namespace gt355 {
             if (g.i_size <= 0 || g.j_size <= 0 || g.k_size <= 0)
                 return;
             engine_::plan_all(msses(), st, box, c);
-            void *stream = be.stream;
-            if (!stream)
-                check(gt355_stream_current(&stream), "run: current stream");
+            void *stream = stream_of(be);
             std::vector<void *> taken;
             engine_::alloc_all(msses(), c, box, g, stream, taken);
-            engine_::launch_all(msses(), st, c, args_t, g, (hipStream_t)stream);
+            engine_::launch_all<N + 1, W>(msses(), st, c, args_t, g, (hipStream_t)stream);
             for (void *t : taken)
                 check(gt355_tmp_release(t, stream), "stencil engine: temporary");
         }
 
         // run_single_stage(functor, backend, grid, fields...) (stencil/frontend/run.hpp:252-258)
-        template <class F, class Be, class... Args>
-        void run_single_stage(F, Be be, grid const &g, Args const &... args) {
+        template <class F, class Be, int N, class... Args>
+        void run_single_stage(F, Be be, grid_t<N> const &g, Args const &... args) {
             run([](auto... p) { return execute_parallel().stage(F(), p...); }, be, g, args...);
         }
     } // namespace stencil

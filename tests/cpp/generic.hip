@@ -1,11 +1,11 @@
-// The generic stencil engine (include/gt355/stencil.hpp) on user specs written like the reference's cartesian
-// regression tests: functors with accessors, extents and interval overloads, multistage specs with temporaries, run
-// on the gfx950 backend tag.  Expected values come from the oracle (oracle/gt_oracle.h) on the same seeded inputs
-// (FP64 must match bit for bit: the functors evaluate the same operations in the same order) or from the reference
-// tests' known answers.
+// The generic stencil engine (include/gt355/stencil.hpp) on user specs: the reference's own regression and
+// unit-test specs (reference_specs.hpp, unmodified functors under their reference names), run on the gfx950 backend
+// tag.  Expected values come from the oracle (oracle/gt_oracle.h) on the same seeded inputs (FP64 must match bit for
+// bit: the functors evaluate the same operations in the same order) or from the reference tests' known answers.
 //
 //   gt355_generic            all cases (needs a GPU)
-//   gt355_generic --bench N  times the engine's hdiff and vadv specs at 256x256x80 FP64 (one JSON line)
+//   gt355_generic --bench N  times the engine on the hdiff, vadv and tridiagonal specs at 256x256x80 FP64 (one JSON
+//                            line)
 #include <gt355/stencil.hpp>
 
 #include <algorithm>
@@ -13,9 +13,11 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <vector>
 
 #include "../../oracle/gt_oracle.h"
+#include "reference_specs.hpp"
 
 using namespace gt355;
 using namespace gt355::stencil;
@@ -78,36 +80,15 @@ namespace {
     }
 
     // ------------------------------------------------------------------------------------- copy and laplacian
-    struct copy_functor {
-        using in = in_accessor<0>;
-        using out = inout_accessor<1>;
-        using param_list = make_param_list<in, out>;
-        template <class Eval>
-        GT_FUNCTION static void apply(Eval &&eval) {
-            eval(out()) = eval(in());
-        }
-    };
-
-    struct lap_functor {
-        using out = inout_accessor<0>;
-        using in = in_accessor<1, extent<-1, 1, -1, 1>>;
-        using param_list = make_param_list<out, in>;
-        template <class Eval>
-        GT_FUNCTION static void apply(Eval &&eval) {
-            using T = std::decay_t<decltype(eval(out()))>;
-            eval(out()) = T(4) * eval(in()) - (eval(in(1, 0)) + eval(in(0, 1)) + eval(in(-1, 0)) + eval(in(0, -1)));
-        }
-    };
-
     void copy_and_laplacian() {
         const int ni = 23, nj = 11, nk = 43, h = 1;
         auto in = seeded<double>(ni + 2, nj + 2, nk, h, 42, 1, -1, 1);
         auto b = storage::builder<storage::gpu>.type<double>().dimensions(ni + 2, nj + 2, nk).halos(h, h, 0);
         auto out = b.build(), lap = b.build();
         auto g = halo_grid(ni, nj, nk, h);
-        run_single_stage(copy_functor(), gfx950<>(), g, in, out);
+        run_single_stage(reference_specs::copy_stencil::copy_functor(), gfx950<>(), g, in, out);
         report("run_single_stage copy", same_interior(out, mirror(in), g));
-        run_single_stage(lap_functor(), gfx950<>(), g, lap, in);
+        run_single_stage(reference_specs::laplacian::lap(), gfx950<>(), g, lap, in);
         auto hin = mirror(in);
         std::vector<double> ref(hin.size(), 0.);
         auto st = in->strides();
@@ -117,52 +98,6 @@ namespace {
     }
 
     // ------------------------------------------------------------------------------------- horizontal diffusion
-    struct hd_lap {
-        using out = inout_accessor<0>;
-        using in = in_accessor<1, extent<-1, 1, -1, 1>>;
-        using param_list = make_param_list<out, in>;
-        template <class Eval>
-        GT_FUNCTION static void apply(Eval eval) {
-            using T = std::decay_t<decltype(eval(out()))>;
-            eval(out()) = T{4} * eval(in()) - (eval(in(1, 0)) + eval(in(0, 1)) + eval(in(-1, 0)) + eval(in(0, -1)));
-        }
-    };
-    struct hd_flx {
-        using out = inout_accessor<0>;
-        using in = in_accessor<1, extent<0, 1, 0, 0>>;
-        using lap = in_accessor<2, extent<0, 1, 0, 0>>;
-        using param_list = make_param_list<out, in, lap>;
-        template <class Eval>
-        GT_FUNCTION static void apply(Eval eval) {
-            auto res = eval(lap(1, 0)) - eval(lap(0, 0));
-            eval(out()) = res * (eval(in(1, 0)) - eval(in(0, 0))) > 0 ? 0 : res;
-        }
-    };
-    struct hd_fly {
-        using out = inout_accessor<0>;
-        using in = in_accessor<1, extent<0, 0, 0, 1>>;
-        using lap = in_accessor<2, extent<0, 0, 0, 1>>;
-        using param_list = make_param_list<out, in, lap>;
-        template <class Eval>
-        GT_FUNCTION static void apply(Eval eval) {
-            auto res = eval(lap(0, 1)) - eval(lap(0, 0));
-            eval(out()) = res * (eval(in(0, 1)) - eval(in(0, 0))) > 0 ? 0 : res;
-        }
-    };
-    struct hd_out {
-        using out = inout_accessor<0>;
-        using in = in_accessor<1>;
-        using flx = in_accessor<2, extent<-1, 0, 0, 0>>;
-        using fly = in_accessor<3, extent<0, 0, -1, 0>>;
-        using coeff = in_accessor<4>;
-        using param_list = make_param_list<out, in, flx, fly, coeff>;
-        template <class Eval>
-        GT_FUNCTION static void apply(Eval eval) {
-            eval(out()) =
-                eval(in()) - eval(coeff()) * (eval(flx()) - eval(flx(-1, 0)) + eval(fly()) - eval(fly(0, -1)));
-        }
-    };
-
     void horizontal_diffusion(int ni, int nj, int nk, bool cached = true) {
         const int h = 2;
         auto in = seeded<double>(ni + 4, nj + 4, nk, h, 42, 1, -1, 1);
@@ -174,20 +109,20 @@ namespace {
                 GT_DECLARE_TMP(double, lap, flx, fly);
                 return execute_parallel()
                     .ij_cached(lap, flx, fly)
-                    .stage(hd_lap(), lap, in)
-                    .stage(hd_flx(), flx, in, lap)
-                    .stage(hd_fly(), fly, in, lap)
-                    .stage(hd_out(), out, in, flx, fly, coeff);
+                    .stage(reference_specs::horizontal_diffusion::lap_function(), lap, in)
+                    .stage(reference_specs::horizontal_diffusion::flx_function(), flx, in, lap)
+                    .stage(reference_specs::horizontal_diffusion::fly_function(), fly, in, lap)
+                    .stage(reference_specs::horizontal_diffusion::out_function(), out, in, flx, fly, coeff);
             },
                 gfx950<>(), g, in, coeff, out);
         else
             run([](auto in, auto coeff, auto out) {
                 GT_DECLARE_TMP(double, lap, flx, fly);
                 return execute_parallel()
-                    .stage(hd_lap(), lap, in)
-                    .stage(hd_flx(), flx, in, lap)
-                    .stage(hd_fly(), fly, in, lap)
-                    .stage(hd_out(), out, in, flx, fly, coeff);
+                    .stage(reference_specs::horizontal_diffusion::lap_function(), lap, in)
+                    .stage(reference_specs::horizontal_diffusion::flx_function(), flx, in, lap)
+                    .stage(reference_specs::horizontal_diffusion::fly_function(), fly, in, lap)
+                    .stage(reference_specs::horizontal_diffusion::out_function(), out, in, flx, fly, coeff);
             },
                 gfx950<>(), g, in, coeff, out);
         auto hin = mirror(in), hco = mirror(coeff);
@@ -204,185 +139,10 @@ namespace {
     // ------------------------------------------------------------------------------------- stencil functions
     // horizontal_diffusion_fused.cpp:23-91: the flux and output functors call the laplacian / flux functors as
     // stencil functions at shifted points; run as one stage
-    struct sf_lap {
-        using out = inout_accessor<0>;
-        using in = in_accessor<1, extent<-1, 1, -1, 1>>;
-        using param_list = make_param_list<out, in>;
-        template <class Eval>
-        GT_FUNCTION static void apply(Eval eval) {
-            using T = std::decay_t<decltype(eval(out()))>;
-            eval(out()) = T{4} * eval(in()) - (eval(in(1, 0)) + eval(in(0, 1)) + eval(in(-1, 0)) + eval(in(0, -1)));
-        }
-    };
-    struct sf_flx {
-        using out = inout_accessor<0>;
-        using in = in_accessor<1, extent<-1, 2, -1, 1>>;
-        using param_list = make_param_list<out, in>;
-        template <class Eval>
-        GT_FUNCTION static void apply(Eval eval) {
-            auto hi = call<sf_lap>::with(eval, in(1, 0));
-            auto lo = call<sf_lap>::with(eval, in(0, 0));
-            auto flx = hi - lo;
-            eval(out()) = flx * (eval(in(1, 0)) - eval(in(0, 0))) > 0 ? 0 : flx;
-        }
-    };
-    struct sf_fly {
-        using out = inout_accessor<0>;
-        using in = in_accessor<1, extent<-1, 1, -1, 2>>;
-        using param_list = make_param_list<out, in>;
-        template <class Eval>
-        GT_FUNCTION static void apply(Eval eval) {
-            auto hi = call<sf_lap>::with(eval, in(0, 1));
-            auto lo = call<sf_lap>::with(eval, in(0, 0));
-            auto fly = hi - lo;
-            eval(out()) = fly * (eval(in(0, 1)) - eval(in(0, 0))) > 0 ? 0 : fly;
-        }
-    };
-    struct sf_out {
-        using out = inout_accessor<0>;
-        using in = in_accessor<1, extent<-2, 2, -2, 2>>;
-        using coeff = in_accessor<2>;
-        using param_list = make_param_list<out, in, coeff>;
-        template <class Eval>
-        GT_FUNCTION static void apply(Eval eval) {
-            auto flx_hi = call<sf_flx>::with(eval, in(0, 0));
-            auto flx_lo = call<sf_flx>::with(eval, in(-1, 0));
-            auto fly_hi = call<sf_fly>::with(eval, in(0, 0));
-            auto fly_lo = call<sf_fly>::with(eval, in(0, -1));
-            eval(out()) = eval(in()) - eval(coeff()) * (flx_hi - flx_lo + fly_hi - fly_lo);
-        }
-    };
     // horizontal_diffusion_functions.cpp:22-205: flx / fly evaluate the laplacian at the point and its east or
     // north neighbour in five ways (monolithic expressions, call<>::at<>, call<> with offset accessors,
     // call_proc<>::at<>, call_proc<> with offset accessors); flx and fly are ij-cached, lap is never stored
-    struct hf_lap {
-        using out = inout_accessor<0>;
-        using in = in_accessor<1, extent<-1, 1, -1, 1>>;
-        using param_list = make_param_list<out, in>;
-        template <class Eval>
-        GT_FUNCTION static void apply(Eval eval) {
-            eval(out()) = 4. * eval(in()) - (eval(in(-1, 0)) + eval(in(0, -1)) + eval(in(0, 1)) + eval(in(1, 0)));
-        }
-    };
-    template <class Eval, class Acc>
-    using hf_float = std::decay_t<decltype(std::declval<Eval>()(Acc()))>;
-    enum class hf_var { monolithic, call, call_offsets, procedures, procedures_offsets };
-    template <hf_var V, class Acc>
-    struct hf_lap_at;
-    template <class Acc>
-    struct hf_lap_at<hf_var::monolithic, Acc> {
-        template <class Eval>
-        GT_FUNCTION static auto do00(Eval &eval) {
-            return hf_float<Eval, Acc>(4) * eval(Acc()) -
-                   (eval(Acc(-1, 0)) + eval(Acc(0, -1)) + eval(Acc(0, 1)) + eval(Acc(1, 0)));
-        }
-        template <class Eval>
-        GT_FUNCTION static auto do10(Eval &eval) {
-            return hf_float<Eval, Acc>(4) * eval(Acc(1, 0)) -
-                   (eval(Acc(0, 0)) + eval(Acc(1, -1)) + eval(Acc(1, 1)) + eval(Acc(2, 0)));
-        }
-        template <class Eval>
-        GT_FUNCTION static auto do01(Eval &eval) {
-            return hf_float<Eval, Acc>(4) * eval(Acc(0, 1)) -
-                   (eval(Acc(-1, 1)) + eval(Acc(0, 0)) + eval(Acc(0, 2)) + eval(Acc(1, 1)));
-        }
-    };
-    template <class Acc>
-    struct hf_lap_at<hf_var::call, Acc> {
-        template <class Eval>
-        GT_FUNCTION static auto do00(Eval &eval) {
-            return call<hf_lap>::at<0, 0, 0>::with(eval, Acc());
-        }
-        template <class Eval>
-        GT_FUNCTION static auto do10(Eval &eval) {
-            return call<hf_lap>::at<1, 0, 0>::with(eval, Acc());
-        }
-        template <class Eval>
-        GT_FUNCTION static auto do01(Eval &eval) {
-            return call<hf_lap>::at<0, 1, 0>::with(eval, Acc());
-        }
-    };
-    template <class Acc>
-    struct hf_lap_at<hf_var::call_offsets, Acc> {
-        template <class Eval>
-        GT_FUNCTION static auto do00(Eval &eval) {
-            return call<hf_lap>::with(eval, Acc(0, 0));
-        }
-        template <class Eval>
-        GT_FUNCTION static auto do10(Eval &eval) {
-            return call<hf_lap>::with(eval, Acc(1, 0));
-        }
-        template <class Eval>
-        GT_FUNCTION static auto do01(Eval &eval) {
-            return call<hf_lap>::with(eval, Acc(0, 1));
-        }
-    };
-    template <class Acc>
-    struct hf_lap_at<hf_var::procedures, Acc> {
-        template <class Eval>
-        GT_FUNCTION static auto do00(Eval &eval) {
-            hf_float<Eval, Acc> res;
-            call_proc<hf_lap>::at<0, 0, 0>::with(eval, res, Acc());
-            return res;
-        }
-        template <class Eval>
-        GT_FUNCTION static auto do10(Eval &eval) {
-            hf_float<Eval, Acc> res;
-            call_proc<hf_lap>::at<1, 0, 0>::with(eval, res, Acc());
-            return res;
-        }
-        template <class Eval>
-        GT_FUNCTION static auto do01(Eval &eval) {
-            hf_float<Eval, Acc> res;
-            call_proc<hf_lap>::at<0, 1, 0>::with(eval, res, Acc());
-            return res;
-        }
-    };
-    template <class Acc>
-    struct hf_lap_at<hf_var::procedures_offsets, Acc> {
-        template <class Eval>
-        GT_FUNCTION static auto do00(Eval &eval) {
-            hf_float<Eval, Acc> res;
-            call_proc<hf_lap>::with(eval, res, Acc(0, 0));
-            return res;
-        }
-        template <class Eval>
-        GT_FUNCTION static auto do10(Eval &eval) {
-            hf_float<Eval, Acc> res;
-            call_proc<hf_lap>::with(eval, res, Acc(1, 0));
-            return res;
-        }
-        template <class Eval>
-        GT_FUNCTION static auto do01(Eval &eval) {
-            hf_float<Eval, Acc> res;
-            call_proc<hf_lap>::with(eval, res, Acc(0, 1));
-            return res;
-        }
-    };
-    template <hf_var V>
-    struct hf_flx {
-        using out = inout_accessor<0>;
-        using in = in_accessor<1, extent<-1, 2, -1, 1>>;
-        using param_list = make_param_list<out, in>;
-        template <class Eval>
-        GT_FUNCTION static void apply(Eval eval) {
-            eval(out()) = hf_lap_at<V, in>::do10(eval) - hf_lap_at<V, in>::do00(eval);
-            eval(out()) = eval(out()) * (eval(in(1, 0)) - eval(in(0, 0))) > 0 ? 0.0 : eval(out());
-        }
-    };
-    template <hf_var V>
-    struct hf_fly {
-        using out = inout_accessor<0>;
-        using in = in_accessor<1, extent<-1, 1, -1, 2>>;
-        using param_list = make_param_list<out, in>;
-        template <class Eval>
-        GT_FUNCTION static void apply(Eval eval) {
-            eval(out()) = hf_lap_at<V, in>::do01(eval) - hf_lap_at<V, in>::do00(eval);
-            eval(out()) = eval(out()) * (eval(in(0, 1, 0)) - eval(in(0, 0, 0))) > 0 ? 0.0 : eval(out());
-        }
-    };
-
-    template <hf_var V>
+    template <reference_specs::horizontal_diffusion_functions::variation V>
     void hdiff_functions(const char *vname, int ni, int nj, int nk) {
         const int h = 2;
         auto in = seeded<double>(ni + 4, nj + 4, nk, h, 42, 1, -1, 1);
@@ -393,9 +153,9 @@ namespace {
             GT_DECLARE_TMP(double, flx, fly);
             return execute_parallel()
                 .ij_cached(flx, fly)
-                .stage(hf_flx<V>(), flx, in)
-                .stage(hf_fly<V>(), fly, in)
-                .stage(hd_out(), out, in, flx, fly, coeff);
+                .stage(reference_specs::horizontal_diffusion_functions::flx_function<V>(), flx, in)
+                .stage(reference_specs::horizontal_diffusion_functions::fly_function<V>(), fly, in)
+                .stage(reference_specs::horizontal_diffusion_functions::out_function(), out, in, flx, fly, coeff);
         },
             gfx950<>(), g, coeff, in, out);
         // expected: the same operations in the same order on the host (the laplacian sums its neighbours as
@@ -444,37 +204,6 @@ namespace {
 
     // expandable_parameters.cpp:22-60 and expandable_parameters_single_kernel.cpp:22-100: copies through an ij-cached
     // temporary, over five members by expandable_run (expandable temporary) and as one spec of ten fields
-    struct ep_copy {
-        using parameters_out = inout_accessor<0>;
-        using parameters_in = accessor<1>;
-        using param_list = make_param_list<parameters_out, parameters_in>;
-        template <class Eval>
-        GT_FUNCTION static void apply(Eval &eval) {
-            eval(parameters_out{}) = eval(parameters_in{});
-        }
-    };
-    struct ep_single {
-        using o1 = inout_accessor<0>;
-        using o2 = inout_accessor<1>;
-        using o3 = inout_accessor<2>;
-        using o4 = inout_accessor<3>;
-        using o5 = inout_accessor<4>;
-        using i1 = in_accessor<5>;
-        using i2 = in_accessor<6>;
-        using i3 = in_accessor<7>;
-        using i4 = in_accessor<8>;
-        using i5 = in_accessor<9>;
-        using param_list = make_param_list<o1, o2, o3, o4, o5, i1, i2, i3, i4, i5>;
-        template <class Eval>
-        GT_FUNCTION static void apply(Eval eval) {
-            eval(o1()) = eval(i1());
-            eval(o2()) = eval(i2());
-            eval(o3()) = eval(i3());
-            eval(o4()) = eval(i4());
-            eval(o5()) = eval(i5());
-        }
-    };
-
     void expandable_parameters(int ni, int nj, int nk) {
         auto b = storage::builder<storage::gpu>.type<double>().dimensions(ni, nj, nk);
         auto g = make_grid(ni, nj, nk);
@@ -496,7 +225,8 @@ namespace {
             std::vector<store<double>> in = {make(-1.), make(-2.), make(-3.), make(-4.), make(-5.)};
             expandable_run<2>([](auto in, auto out) {
                 GT_DECLARE_EXPANDABLE_TMP(double, tmp);
-                return execute_parallel().ij_cached(tmp).stage(ep_copy(), tmp, in).stage(ep_copy(), out, tmp);
+                return execute_parallel().ij_cached(tmp).stage(reference_specs::expandable_parameters::copy_functor(), tmp, in)
+                    .stage(reference_specs::expandable_parameters::copy_functor(), out, tmp);
             },
                 gfx950<>(), g, in, out);
             report("expandable_parameters: expandable_run<2> over 5 members, expandable ij-cached temporary",
@@ -510,8 +240,8 @@ namespace {
                 GT_DECLARE_TMP(double, tmp0, tmp1, tmp2, tmp3, tmp4);
                 return execute_parallel()
                     .ij_cached(tmp0, tmp1, tmp2, tmp3, tmp4)
-                    .stage(ep_single(), tmp0, tmp1, tmp2, tmp3, tmp4, in0, in1, in2, in3, in4)
-                    .stage(ep_single(), out0, out1, out2, out3, out4, tmp0, tmp1, tmp2, tmp3, tmp4);
+                    .stage(reference_specs::expandable_parameters_single_kernel::functor_single_kernel(), tmp0, tmp1, tmp2, tmp3, tmp4, in0, in1, in2, in3, in4)
+                    .stage(reference_specs::expandable_parameters_single_kernel::functor_single_kernel(), out0, out1, out2, out3, out4, tmp0, tmp1, tmp2, tmp3, tmp4);
             },
                 gfx950<>(), g, out[0], out[1], out[2], out[3], out[4], in[0], in[1], in[2], in[3], in[4]);
             report("expandable_parameters_single_kernel: ten fields, five ij-cached temporaries", check_copies(in, out));
@@ -549,7 +279,7 @@ namespace {
         auto b = storage::builder<storage::gpu>.type<double>().dimensions(ni + 4, nj + 4, nk).halos(h, h, 0);
         auto out = b.build(), out2 = b.build();
         auto g = halo_grid(ni, nj, nk, h);
-        run_single_stage(sf_out(), gfx950<>(), g, out, in, coeff);
+        run_single_stage(reference_specs::horizontal_diffusion_fused::out_function(), gfx950<>(), g, out, in, coeff);
         auto hin = mirror(in), hco = mirror(coeff);
         std::vector<double> ref(hin.size(), 0.);
         auto st = in->strides();
@@ -571,90 +301,6 @@ namespace {
     }
 
     // ------------------------------------------------------------------------------------- vertical advection
-    using axis_t = axis<1, axis_config::offset_limit<3>>;
-    using full_t = axis_t::full_interval;
-
-    struct va_forward {
-        using utens_stage = in_accessor<0>;
-        using wcon = in_accessor<1, extent<0, 1, 0, 0, 0, 1>>;
-        using u_stage = in_accessor<2, extent<0, 0, 0, 0, -1, 1>>;
-        using u_pos = in_accessor<3>;
-        using utens = in_accessor<4>;
-        using dtr_stage = in_accessor<5>;
-        using ccol = inout_accessor<6, extent<0, 0, 0, 0, -1, 0>>;
-        using dcol = inout_accessor<7, extent<0, 0, 0, 0, -1, 0>>;
-        using param_list = make_param_list<utens_stage, wcon, u_stage, u_pos, utens, dtr_stage, ccol, dcol>;
-
-        template <class Eval>
-        GT_FUNCTION static auto rhs(Eval &&eval) {
-            return eval(dtr_stage()) * eval(u_pos()) + eval(utens()) + eval(utens_stage());
-        }
-        template <class Eval>
-        GT_FUNCTION static void apply(Eval &&eval, full_t::first_level) {
-            using T = std::decay_t<decltype(eval(ccol()))>;
-            auto gcv = T(.25) * (eval(wcon(1, 0, 1)) + eval(wcon(0, 0, 1)));
-            auto cs = gcv * T(0.5);
-            auto c = gcv * T(0.5);
-            auto b = eval(dtr_stage()) - c;
-            auto correction = -cs * (eval(u_stage(0, 0, 1)) - eval(u_stage()));
-            auto d = rhs(eval) + correction;
-            auto divided = T(1) / b;
-            eval(ccol()) = c * divided;
-            eval(dcol()) = d * divided;
-        }
-        template <class Eval>
-        GT_FUNCTION static void apply(Eval &&eval, full_t::modify<1, -1>) {
-            using T = std::decay_t<decltype(eval(ccol()))>;
-            auto gav = -T(.25) * (eval(wcon(1, 0, 0)) + eval(wcon(0, 0, 0)));
-            auto gcv = T(.25) * (eval(wcon(1, 0, 1)) + eval(wcon(0, 0, 1)));
-            auto as = gav * T(0.5);
-            auto cs = gcv * T(0.5);
-            auto a = gav * T(0.5);
-            auto c = gcv * T(0.5);
-            auto b = eval(dtr_stage()) - a - c;
-            auto correction =
-                -as * (eval(u_stage(0, 0, -1)) - eval(u_stage())) - cs * (eval(u_stage(0, 0, 1)) - eval(u_stage()));
-            auto d = rhs(eval) + correction;
-            auto divided = T(1) / (b - eval(ccol(0, 0, -1)) * a);
-            eval(ccol()) = c * divided;
-            eval(dcol()) = (d - eval(dcol(0, 0, -1)) * a) * divided;
-        }
-        template <class Eval>
-        GT_FUNCTION static void apply(Eval &&eval, full_t::last_level) {
-            using T = std::decay_t<decltype(eval(ccol()))>;
-            auto gav = -T(.25) * (eval(wcon(1, 0, 0)) + eval(wcon()));
-            auto as = gav * T(0.5);
-            auto a = gav * T(0.5);
-            auto b = eval(dtr_stage()) - a;
-            auto correction = -as * (eval(u_stage(0, 0, -1)) - eval(u_stage()));
-            auto d = rhs(eval) + correction;
-            auto divided = T(1) / (b - eval(ccol(0, 0, -1)) * a);
-            eval(dcol()) = (d - eval(dcol(0, 0, -1)) * a) * divided;
-        }
-    };
-
-    struct va_backward {
-        using utens_stage = inout_accessor<0>;
-        using u_pos = in_accessor<1>;
-        using dtr_stage = in_accessor<2>;
-        using ccol = in_accessor<3>;
-        using dcol = in_accessor<4>;
-        using data_col = inout_accessor<5, extent<0, 0, 0, 0, 0, 1>>;
-        using param_list = make_param_list<utens_stage, u_pos, dtr_stage, ccol, dcol, data_col>;
-
-        template <class Eval>
-        GT_FUNCTION static void apply(Eval &&eval, full_t::modify<0, -1>) {
-            auto data = eval(dcol()) - eval(ccol()) * eval(data_col(0, 0, 1));
-            eval(utens_stage()) = eval(dtr_stage()) * (data - eval(u_pos()));
-            eval(data_col()) = data;
-        }
-        template <class Eval>
-        GT_FUNCTION static void apply(Eval &&eval, full_t::last_level) {
-            eval(utens_stage()) = eval(dtr_stage()) * (eval(dcol()) - eval(u_pos()));
-            eval(data_col()) = eval(dcol());
-        }
-    };
-
     void vertical_advection(int ni, int nj, int nk) {
         const int h = 3;
         auto utens_stage = seeded<double>(ni + 6, nj + 6, nk, h, 42, 6, -1, 1);
@@ -664,16 +310,8 @@ namespace {
         auto wcon = seeded<double>(ni + 6, nj + 6, nk + 1, h, 42, 7, -0.5, 0.5);
         auto g = halo_grid(ni, nj, nk, h);
         auto uts0 = mirror(utens_stage);
-        auto spec = [](auto utens_stage, auto u_stage, auto wcon, auto u_pos, auto utens, auto dtr_stage) {
-            GT_DECLARE_TMP(double, ccol, dcol, data_col);
-            return multi_pass(execute_forward()
-                                  .k_cached(cache_io_policy::flush(), ccol, dcol)
-                                  .k_cached(cache_io_policy::fill(), u_stage)
-                                  .stage(va_forward(), utens_stage, wcon, u_stage, u_pos, utens, dtr_stage, ccol, dcol),
-                execute_backward().k_cached(data_col).stage(
-                    va_backward(), utens_stage, u_pos, dtr_stage, ccol, dcol, data_col));
-        };
-        run(spec, gfx950<>(), g, utens_stage, u_stage, wcon, u_pos, utens, global_parameter(0.15));
+        run(reference_specs::vertical_advection_dycore::spec<double>(), gfx950<>(), g, utens_stage, u_stage, wcon, u_pos,
+            utens, global_parameter(0.15));
         auto hus = mirror(u_stage), hup = mirror(u_pos), hut = mirror(utens), hw = mirror(wcon);
         auto st = u_stage->strides(), wst = wcon->strides();
         const size_t o = h + h * st[1], wo = h + h * wst[1];
@@ -685,39 +323,6 @@ namespace {
     }
 
     // ------------------------------------------------------------------------------------- tridiagonal (expressions)
-    using full1_t = axis<1>::full_interval;
-    struct thomas_forward {
-        using inf = in_accessor<0>;
-        using diag = in_accessor<1>;
-        using sup = inout_accessor<2, extent<0, 0, 0, 0, -1, 0>>;
-        using rhs = inout_accessor<3, extent<0, 0, 0, 0, -1, 0>>;
-        using param_list = make_param_list<inf, diag, sup, rhs>;
-        template <class Eval>
-        GT_FUNCTION static void apply(Eval eval, full1_t::modify<1, 0>) {
-            eval(sup{}) = eval(sup{} / (diag{} - sup{0, 0, -1} * inf{}));
-            eval(rhs{}) = eval((rhs{} - inf{} * rhs{0, 0, -1}) / (diag{} - sup{0, 0, -1} * inf{}));
-        }
-        template <class Eval>
-        GT_FUNCTION static void apply(Eval eval, full1_t::first_level) {
-            eval(sup{}) = eval(sup{}) / eval(diag{});
-            eval(rhs{}) = eval(rhs{}) / eval(diag{});
-        }
-    };
-    struct thomas_backward {
-        using out = inout_accessor<0, extent<0, 0, 0, 0, 0, 1>>;
-        using sup = in_accessor<1>;
-        using rhs = in_accessor<2>;
-        using param_list = make_param_list<out, sup, rhs>;
-        template <class Eval>
-        GT_FUNCTION static void apply(Eval eval, full1_t::modify<0, -1>) {
-            eval(out{}) = eval(rhs{} - sup{} * out{0, 0, 1});
-        }
-        template <class Eval>
-        GT_FUNCTION static void apply(Eval eval, full1_t::last_level) {
-            eval(out{}) = eval(rhs{});
-        }
-    };
-
     void tridiagonal(int ni, int nj, int nk, bool known_answer) {
         auto b = storage::builder<storage::gpu>.type<double>().dimensions(ni, nj, nk);
         store<double> inf, diag, sup, rhs;
@@ -736,8 +341,8 @@ namespace {
         auto out = b.build();
         auto hinf = mirror(inf), hdiag = mirror(diag), hsup = mirror(sup), hrhs = mirror(rhs);
         run([](auto inf, auto diag, auto sup, auto rhs, auto out) {
-            return multi_pass(execute_forward().stage(thomas_forward(), inf, diag, sup, rhs),
-                execute_backward().stage(thomas_backward(), out, sup, rhs));
+            return multi_pass(execute_forward().stage(reference_specs::tridiagonal::forward_thomas(), inf, diag, sup, rhs),
+                execute_backward().stage(reference_specs::tridiagonal::backward_thomas(), out, sup, rhs));
         },
             gfx950<>(), make_grid(ni, nj, nk), inf, diag, sup, rhs, out);
         std::vector<double> ref(hinf.size(), 0.);
@@ -757,36 +362,6 @@ namespace {
     }
 
     // ------------------------------------------------------------------------------------- simple_hori_diff (j-only)
-    struct wlap {
-        using out = inout_accessor<0>;
-        using in = in_accessor<1, extent<-1, 1, -1, 1>>;
-        using crlato = in_accessor<2>;
-        using crlatu = in_accessor<3>;
-        using param_list = make_param_list<out, in, crlato, crlatu>;
-        template <class Eval>
-        GT_FUNCTION static void apply(Eval eval) {
-            using T = std::decay_t<decltype(eval(out()))>;
-            eval(out()) = eval(in(1, 0)) + eval(in(-1, 0)) - T{2} * eval(in()) +
-                          eval(crlato()) * (eval(in(0, 1)) - eval(in())) + eval(crlatu()) * (eval(in(0, -1)) - eval(in()));
-        }
-    };
-    struct divflux {
-        using out = inout_accessor<0>;
-        using in = in_accessor<1>;
-        using lap = in_accessor<2, extent<-1, 1, -1, 1>>;
-        using crlato = in_accessor<3>;
-        using coeff = in_accessor<4>;
-        using param_list = make_param_list<out, in, lap, crlato, coeff>;
-        template <class Eval>
-        GT_FUNCTION static void apply(Eval &eval) {
-            auto fluxx = eval(lap(1, 0)) - eval(lap());
-            auto fluxx_m = eval(lap()) - eval(lap(-1, 0));
-            auto fluxy = eval(crlato()) * (eval(lap(0, 1)) - eval(lap()));
-            auto fluxy_m = eval(crlato()) * (eval(lap()) - eval(lap(0, -1)));
-            eval(out()) = eval(in()) + ((fluxx_m - fluxx) + (fluxy_m - fluxy)) * eval(coeff());
-        }
-    };
-
     void simple_hori_diff(int ni, int nj, int nk) {
         const int h = 2;
         auto in = seeded<double>(ni + 4, nj + 4, nk, h, 42, 21, -1, 1);
@@ -804,8 +379,8 @@ namespace {
             GT_DECLARE_TMP(double, lap);
             return execute_parallel()
                 .ij_cached(lap)
-                .stage(wlap(), lap, in, crlato, crlatu)
-                .stage(divflux(), out, in, lap, crlato, coeff);
+                .stage(reference_specs::simple_hori_diff::wlap_function(), lap, in, crlato, crlatu)
+                .stage(reference_specs::simple_hori_diff::divflux_function(), out, in, lap, crlato, coeff);
         },
             gfx950<>(), g, coeff, in, out, crlato, crlatu);
         auto hin = mirror(in), hco = mirror(coeff);
@@ -820,46 +395,12 @@ namespace {
     }
 
     // ------------------------------------------------------------------------------------- positional, fusion
-    struct positional_functor {
-        using out = inout_accessor<0>;
-        using i_pos = in_accessor<1>;
-        using j_pos = in_accessor<2>;
-        using k_pos = in_accessor<3>;
-        using param_list = make_param_list<out, i_pos, j_pos, k_pos>;
-        template <class Eval>
-        GT_FUNCTION static void apply(Eval &&eval) {
-            eval(out()) = eval(i_pos()) + eval(j_pos()) + eval(k_pos());
-        }
-    };
-
-    struct set_one {
-        using out = inout_accessor<0>;
-        using param_list = make_param_list<out>;
-        template <class Eval>
-        GT_FUNCTION static void apply(Eval &&eval) {
-            eval(out()) = 1;
-        }
-    };
-    struct shift_down {
-        using in = in_accessor<0, extent<0, 0, 0, 0, 0, 1>>;
-        using out = inout_accessor<1>;
-        using param_list = make_param_list<in, out>;
-        template <class Eval>
-        GT_FUNCTION static void apply(Eval &&eval, full1_t::modify<0, -1>) {
-            eval(out()) = eval(in(0, 0, 1));
-        }
-        template <class Eval>
-        GT_FUNCTION static void apply(Eval &&eval, full1_t::last_level) {
-            eval(out()) = eval(in());
-        }
-    };
-
     void positional_and_fusion() {
         const int ni = 12, nj = 33, nk = 61, h = 2;
         auto b = storage::builder<storage::gpu>.type<double>().dimensions(ni + 4, nj + 4, nk).halos(h, h, 0);
         auto out = b.value(-1.).build();
         auto g = halo_grid(ni, nj, nk, h);
-        run_single_stage(positional_functor(), gfx950<>(), g, out, positional<dim::i>(), positional<dim::j>(),
+        run_single_stage(reference_specs::positional_stencil::functor(), gfx950<>(), g, out, positional<dim::i>(), positional<dim::j>(),
             positional<dim::k>());
         bool ok = true;
         {
@@ -878,7 +419,9 @@ namespace {
         auto out2 = b.value(0.).build();
         run([](auto out) {
             GT_DECLARE_TMP(double, tmp);
-            return multi_pass(execute_parallel().stage(set_one(), tmp), execute_parallel().stage(shift_down(), tmp, out));
+            using namespace reference_specs::parallel_multistage_fusion;
+            return multi_pass(execute_parallel().stage(set_constant(), tmp),
+                execute_parallel().stage(copy_with_vertical_offset(), tmp, out));
         },
             gfx950<>(), g, out2);
         ok = true;
@@ -891,17 +434,6 @@ namespace {
     }
 
     // ------------------------------------------------------------------------------------- expandable_run, errors
-    struct scale_functor {
-        using data = inout_accessor<0>;
-        using data_nnow = in_accessor<1>;
-        using rho = in_accessor<2>;
-        using param_list = make_param_list<data, data_nnow, rho>;
-        template <class Eval>
-        GT_FUNCTION static void apply(Eval &&eval) {
-            eval(data()) = eval(rho()) * eval(data_nnow());
-        }
-    };
-
     void expandable_and_errors() {
         const int ni = 12, nj = 33, nk = 61;
         auto b = storage::builder<storage::gpu>.type<double>().dimensions(ni, nj, nk);
@@ -911,7 +443,9 @@ namespace {
             in.push_back(b.value(double(m)).build());
         }
         auto rho = b.value(1.1).build();
-        expandable_run<2>([](auto out, auto in, auto rho) { return execute_parallel().stage(scale_functor(), out, in, rho); },
+        expandable_run<2>([](auto out, auto in, auto rho) {
+            return execute_parallel().stage(reference_specs::advection_pdbott_prepare_tracers::prepare_tracers(), out, in, rho);
+        },
             gfx950<>(), make_grid(ni, nj, nk), out, in, rho);
         bool ok = true;
         for (int m = 0; m < 5 && ok; ++m) {
@@ -923,17 +457,355 @@ namespace {
         bool threw = false;
         try {
             auto small = b.build(), res = b.build();
-            run_single_stage(lap_functor(), gfx950<>(), make_grid(ni, nj, nk), res, small);
+            run_single_stage(reference_specs::laplacian::lap(), gfx950<>(), make_grid(ni, nj, nk), res, small);
         } catch (std::runtime_error const &e) {
             threw = std::strstr(e.what(), "too small") != nullptr;
         }
         report("field bounds check of the extent analysis", threw);
     }
-    // --bench: the engine's hdiff and vadv specs at 256x256x80 FP64 over 4 rotating field sets (past the 256 MB
-    // MALL), mean microseconds per run from one hipEvent pair around `reps` back-to-back runs
+    // ------------------------------------------------------------------------------------- k-caches, axis<N>
+    // the unit tests' fields (test_kcache_*.cpp: in(i, j, k) = i + j + k + 1 on a 6 x 6 grid, axis<3> intervals of 2, 6
+    // and 2 levels, no halo) and their expected columns
+    double kin(int i, int j, int k) { return i + j + k + 1; }
+
+    template <bool WithOut, class Spec>
+    bool kcache_case(const char *name, Spec spec, std::function<double(int, int, int, int)> expected) {
+        using axis_t = reference_specs::test_kcache_fill::axis_t;
+        const int d0 = 6, d1 = 6;
+        auto g = make_grid(d0, d1, axis_t(2, 6, 2));
+        const int nk = g.k_size;
+        auto b = storage::builder<storage::gpu>.type<double>().dimensions(d0, d1, nk);
+        auto in = b.initializer(kin).build();
+        auto out = b.build();
+        if constexpr (WithOut)
+            run(spec, gfx950<>(), g, in, out);
+        else
+            run(spec, gfx950<>(), g, in);
+        auto v = (WithOut ? out : in)->const_host_view();
+        bool ok = true;
+        for (int i = 0; i < d0 && ok; ++i)
+            for (int j = 0; j < d1 && ok; ++j)
+                for (int k = 0; k < nk && ok; ++k) {
+                    const double e = expected(i, j, k, nk);
+                    if (v(i, j, k) != e) {
+                        std::printf("  (%d,%d,%d): %.17g, expected %.17g\n", i, j, k, v(i, j, k), e);
+                        ok = false;
+                    }
+                }
+        report(name, ok);
+        return ok;
+    }
+
+    void kcache_tests() {
+        namespace kf = reference_specs::test_kcache_fill;
+        namespace kfl = reference_specs::test_kcache_flush;
+        namespace kff = reference_specs::test_kcache_fill_and_flush;
+        namespace kl = reference_specs::test_kcache_local;
+        // test_kcache_fill.cpp:52-125
+        kcache_case<true>("test_kcache_fill forward (axis<3>, fill policy, k offsets -1..1)",
+            [](auto in, auto out) {
+                return execute_forward().k_cached(cache_io_policy::fill(), in).stage(kf::shift_acc_forward_fill(), in, out);
+            },
+            [](int i, int j, int k, int nk) {
+                return k == 0        ? kin(i, j, 0) + kin(i, j, 1)
+                       : k == nk - 1 ? kin(i, j, nk - 1) + kin(i, j, nk - 2)
+                                     : kin(i, j, k - 1) + kin(i, j, k) + kin(i, j, k + 1);
+            });
+        kcache_case<true>("test_kcache_fill backward",
+            [](auto in, auto out) {
+                return execute_backward().k_cached(cache_io_policy::fill(), in).stage(kf::shift_acc_backward_fill(), in, out);
+            },
+            [](int i, int j, int k, int nk) {
+                return k == nk - 1 ? kin(i, j, nk - 1) + kin(i, j, nk - 2)
+                       : k == 0    ? kin(i, j, 1) + kin(i, j, 0)
+                                   : kin(i, j, k + 1) + kin(i, j, k) + kin(i, j, k - 1);
+            });
+        kcache_case<true>("test_kcache_fill fill_copy_forward",
+            [](auto in, auto out) {
+                return execute_forward().k_cached(cache_io_policy::fill(), in).stage(kf::copy_fill(), in, out);
+            },
+            [](int i, int j, int k, int) { return kin(i, j, k); });
+        // test_kcache_flush.cpp:38-99: a running sum kept in the cache and flushed to `out`
+        auto fwd_sum = [](int i, int j, int k, int) {
+            double r = 0;
+            for (int kk = 0; kk <= k; ++kk)
+                r = kk == 0 ? kin(i, j, 0) : r + kin(i, j, kk);
+            return r;
+        };
+        auto bwd_sum = [](int i, int j, int k, int nk) {
+            double r = 0;
+            for (int kk = nk - 1; kk >= k; --kk)
+                r = kk == nk - 1 ? kin(i, j, nk - 1) : r + kin(i, j, kk);
+            return r;
+        };
+        kcache_case<true>("test_kcache_flush forward (flush policy)",
+            [](auto in, auto out) {
+                return execute_forward().k_cached(cache_io_policy::flush(), out).stage(kfl::shift_acc_forward_flush(), in, out);
+            },
+            fwd_sum);
+        kcache_case<true>("test_kcache_flush backward",
+            [](auto in, auto out) {
+                return execute_backward()
+                    .k_cached(cache_io_policy::flush(), out)
+                    .stage(kfl::shift_acc_backward_flush(), in, out);
+            },
+            bwd_sum);
+        // test_kcache_fill_and_flush.cpp:38-125: the cached field is read and written in place
+        kcache_case<false>("test_kcache_fill_and_flush forward (fill + flush on one inout field)",
+            [](auto in) {
+                return execute_forward()
+                    .k_cached(cache_io_policy::fill(), cache_io_policy::flush(), in)
+                    .stage(kff::shift_acc_forward_fill_and_flush(), in);
+            },
+            [](int i, int j, int k, int) {
+                double r = 0;
+                for (int kk = 0; kk <= k; ++kk)
+                    r = kk == 0 ? kin(i, j, 0) : kin(i, j, kk) + r;
+                return r;
+            });
+        kcache_case<false>("test_kcache_fill_and_flush backward",
+            [](auto in) {
+                return execute_backward()
+                    .k_cached(cache_io_policy::fill(), cache_io_policy::flush(), in)
+                    .stage(kff::shift_acc_backward_fill_and_flush(), in);
+            },
+            [](int i, int j, int k, int nk) {
+                double r = 0;
+                for (int kk = nk - 1; kk >= k; --kk)
+                    r = kk == nk - 1 ? kin(i, j, nk - 1) : r + kin(i, j, kk);
+                return r;
+            });
+        kcache_case<false>("test_kcache_fill_and_flush copy_forward",
+            [](auto in) {
+                return execute_forward()
+                    .k_cached(cache_io_policy::fill(), cache_io_policy::flush(), in)
+                    .stage(kff::copy_fill(), in);
+            },
+            [](int i, int j, int k, int) { return kin(i, j, k); });
+        kcache_case<false>("test_kcache_fill_and_flush scale_forward",
+            [](auto in) {
+                return execute_forward()
+                    .k_cached(cache_io_policy::fill(), cache_io_policy::flush(), in)
+                    .stage(kff::scale_fill(), in);
+            },
+            [](int i, int j, int k, int) { return 2 * kin(i, j, k); });
+        // test_kcache_local.cpp:38-213: a k-cached temporary with no policy (registers only)
+        kcache_case<true>("test_kcache_local forward (policy-free k-cached temporary)",
+            [](auto in, auto out) {
+                GT_DECLARE_TMP(double, tmp);
+                return execute_forward().k_cached(tmp).stage(kl::shift_acc_forward(), in, out, tmp);
+            },
+            fwd_sum);
+        kcache_case<true>("test_kcache_local backward",
+            [](auto in, auto out) {
+                GT_DECLARE_TMP(double, tmp);
+                return execute_backward().k_cached(tmp).stage(kl::shift_acc_backward(), in, out, tmp);
+            },
+            bwd_sum);
+        kcache_case<true>("test_kcache_local biside_forward (window -2..1, writes at k+1, first_level::shift<1>)",
+            [](auto in, auto out) {
+                GT_DECLARE_TMP(double, tmp);
+                return execute_forward().k_cached(tmp).stage(kl::biside_large_kcache_forward(), in, out, tmp);
+            },
+            [](int i, int j, int k, int nk) {
+                // test_kcache_local.cpp:136-158: the reference's host loop over a buffer column
+                std::vector<double> buff(nk + 1), ref(nk);
+                buff[0] = kin(i, j, 0);
+                buff[1] = kin(i, j, 0) / 2;
+                ref[0] = kin(i, j, 0);
+                buff[2] = kin(i, j, 1) / 2;
+                ref[1] = buff[1] + buff[0] / 4;
+                for (int kk = 2; kk < nk; ++kk) {
+                    if (kk != nk - 1)
+                        buff[kk + 1] = kin(i, j, kk) / 2;
+                    ref[kk] = buff[kk] + buff[kk - 1] / 4 + .12 * buff[kk - 2];
+                }
+                return ref[k];
+            });
+        kcache_case<true>("test_kcache_local biside_backward (window -1..2, last_level::shift<-1>)",
+            [](auto in, auto out) {
+                GT_DECLARE_TMP(double, tmp);
+                return execute_backward().k_cached(tmp).stage(kl::biside_large_kcache_backward(), in, out, tmp);
+            },
+            [](int i, int j, int k, int nk) {
+                // test_kcache_local.cpp:190-212
+                std::vector<double> buff(nk + 1), ref(nk);
+                buff[nk - 1] = kin(i, j, nk - 1);
+                buff[nk - 2] = kin(i, j, nk - 1) / 2;
+                ref[nk - 1] = kin(i, j, nk - 1);
+                buff[nk - 3] = kin(i, j, nk - 2) / 2;
+                ref[nk - 2] = buff[nk - 2] + buff[nk - 1] / 4;
+                for (int kk = nk - 3; kk >= 0; --kk) {
+                    if (kk != 0)
+                        buff[kk - 1] = kin(i, j, kk) / 2;
+                    ref[kk] = buff[kk] + buff[kk + 1] / 4 + .12 * buff[kk + 2];
+                }
+                return ref[k];
+            });
+    }
+
+    // test_kparallel.cpp:38-68: parallel multistages over axis<2> / axis<3>, interval overloads per k
+    void kparallel_tests() {
+        namespace kp = reference_specs::test_kparallel;
+        auto in = [](int i, int j, int k) { return i * 1000 + j * 100 + k; };
+        {
+            auto g = make_grid(7, 8, axis<2>(14, 16));
+            auto b = storage::builder<storage::gpu>.type<double>().dimensions(7, 8, g.k_size);
+            auto out = b.value(1.5).build();
+            run([](auto in, auto out) {
+                GT_DECLARE_TMP(double, tmp);
+                return execute_parallel().stage(kp::parallel_functor(), in, tmp).stage(kp::parallel_functor(), tmp, out);
+            },
+                gfx950<>(), g, b.initializer(in).build(), out);
+            auto v = out->const_host_view();
+            bool ok = true;
+            for (int i = 0; i < 7; ++i)
+                for (int j = 0; j < 8; ++j)
+                    for (int k = 0; k < g.k_size; ++k)
+                        ok = ok && v(i, j, k) == in(i, j, k) * (k < 14 ? 1 : 4);
+            report("test_kparallel with_temporary (axis<2>: 14 + 16 levels)", ok);
+        }
+        {
+            auto g = make_grid(7, 8, axis<3>(14, 16, 18));
+            auto b = storage::builder<storage::gpu>.type<double>().dimensions(7, 8, g.k_size);
+            auto out = b.value(1.5).build();
+            run_single_stage(kp::parallel_functor_on_upper_interval(), gfx950<>(), g, b.initializer(in).build(), out);
+            auto v = out->const_host_view();
+            bool ok = true;
+            for (int i = 0; i < 7; ++i)
+                for (int j = 0; j < 8; ++j)
+                    for (int k = 0; k < g.k_size; ++k)
+                        ok = ok && v(i, j, k) == (k >= 14 && k < 14 + 16 ? in(i, j, k) : 1.5);
+            report("test_kparallel with_unused_intervals (axis<3>: only interval 1 written)", ok);
+        }
+    }
+
+    // whole_axis_access.cpp:40-60: the k dimension renamed to data dimension 3, summed over the column with a
+    // four-dimensional accessor
+    void whole_axis_access(int ni, int nj, int nk) {
+        using namespace literals;
+        auto in = [](int i, int j, int k) { return i + j + k; };
+        auto b = storage::builder<storage::gpu>.type<double>().dimensions(ni, nj, nk);
+        auto out = b.build();
+        run_single_stage(reference_specs::whole_axis_access::functor(), gfx950<>(), make_grid(ni, nj, nk), out,
+            sid::rename_dimensions<dim::k, decltype(3_c)>(b.initializer(in).build()), positional<dim::k>());
+        auto v = out->const_host_view();
+        bool ok = true;
+        for (int i = 0; i < ni; ++i)
+            for (int j = 0; j < nj; ++j)
+                for (int k = 0; k < nk; ++k) {
+                    int r = 0;
+                    for (int kk = 0; kk < k; ++kk)
+                        r += in(i, j, kk);
+                    ok = ok && v(i, j, k) == r;
+                }
+        char name[96];
+        std::snprintf(name, sizeof name, "whole_axis_access (rename_dimensions<k, 3>, 4-D accessor) %dx%dx%d", ni, nj, nk);
+        report(name, ok);
+    }
+
+    // ------------------------------------------------------------------------------------- precision policy
+    // gfx950<GT355_WIDEN>: float storages, read-only fields widened to double, temporaries (ccol, dcol, data_col, lap,
+    // flx, fly) held in double, stores narrowed -- the oracle's mixed variants bit for bit
+    void precision_policy(int ni, int nj, int nk) {
+        {
+            const int h = 2;
+            auto in = seeded<float>(ni + 4, nj + 4, nk, h, 42, 1, -1, 1);
+            auto coeff = seeded<float>(ni + 4, nj + 4, nk, h, 42, 2, 0, 0.25);
+            auto out = storage::builder<storage::gpu>.type<float>().dimensions(ni + 4, nj + 4, nk).halos(h, h, 0).build();
+            auto g = halo_grid(ni, nj, nk, h);
+            run(reference_specs::horizontal_diffusion::spec<float>(), gfx950<GT355_WIDEN>(), g, in, coeff, out);
+            auto hin = mirror(in), hco = mirror(coeff);
+            std::vector<float> ref(hin.size(), 0.f);
+            auto st = in->strides();
+            const size_t o = h + h * st[1];
+            gto_hdiff_mixed(ref.data() + o, hin.data() + o, hco.data() + o, ni, nj, nk, st[1], st[2], 4);
+            char name[112];
+            std::snprintf(name, sizeof name, "hdiff spec, precision policy (float storage, double compute) %dx%dx%d", ni,
+                nj, nk);
+            report(name, same_interior(out, ref, g));
+        }
+        {
+            const int h = 3;
+            auto utens_stage = seeded<float>(ni + 6, nj + 6, nk, h, 42, 6, -1, 1);
+            auto u_stage = seeded<float>(ni + 6, nj + 6, nk, h, 42, 3, -1, 1);
+            auto u_pos = seeded<float>(ni + 6, nj + 6, nk, h, 42, 4, -1, 1);
+            auto utens = seeded<float>(ni + 6, nj + 6, nk, h, 42, 5, -1, 1);
+            auto wcon = seeded<float>(ni + 6, nj + 6, nk + 1, h, 42, 7, -0.5, 0.5);
+            auto g = halo_grid(ni, nj, nk, h);
+            auto uts0 = mirror(utens_stage);
+            run(reference_specs::vertical_advection_dycore::spec<float>(), gfx950<GT355_WIDEN>(), g, utens_stage,
+                u_stage, wcon, u_pos, utens, global_parameter(0.15f));
+            auto hus = mirror(u_stage), hup = mirror(u_pos), hut = mirror(utens), hw = mirror(wcon);
+            auto st = u_stage->strides(), wst = wcon->strides();
+            const size_t o = h + h * st[1], wo = h + h * wst[1];
+            gto_vadv_mixed(uts0.data() + o, hus.data() + o, hw.data() + wo, hup.data() + o, hut.data() + o, 0.15f, ni,
+                nj, nk, st[1], st[2], wst[1], wst[2], 4);
+            char name[112];
+            std::snprintf(name, sizeof name, "vadv spec, precision policy (float storage, double compute) %dx%dx%d", ni,
+                nj, nk);
+            report(name, same_interior(utens_stage, uts0, g));
+        }
+    }
+
+    // ------------------------------------------------------------------------------------- temporaries on two streams
+    // the hdiff spec with HBM temporaries on two streams at once, several rounds: a temporary released on one stream
+    // is not handed to the other while the first stream's kernels may still use it
+    void two_streams(int ni, int nj, int nk) {
+        const int h = 2;
+        hipStream_t s[2];
+        for (auto &x : s)
+            check(hipStreamCreateWithFlags(&x, hipStreamNonBlocking) == hipSuccess ? 0 : -2, "stream");
+        std::vector<store<double>> in, coeff, out;
+        for (int n = 0; n < 2; ++n) {
+            in.push_back(seeded<double>(ni + 4, nj + 4, nk, h, 100 + n, 1, -1, 1));
+            coeff.push_back(seeded<double>(ni + 4, nj + 4, nk, h, 100 + n, 2, 0, 0.25));
+            out.push_back(storage::builder<storage::gpu>.type<double>().dimensions(ni + 4, nj + 4, nk).halos(h, h, 0).build());
+        }
+        auto g = halo_grid(ni, nj, nk, h);
+        auto spec = [](auto in, auto coeff, auto out) {
+            using namespace reference_specs::horizontal_diffusion;
+            GT_DECLARE_TMP(double, lap, flx, fly);
+            return execute_parallel()
+                .stage(lap_function(), lap, in)
+                .stage(flx_function(), flx, in, lap)
+                .stage(fly_function(), fly, in, lap)
+                .stage(out_function(), out, in, flx, fly, coeff);
+        };
+        // bring the inputs to the device before the streams run
+        for (int n = 0; n < 2; ++n) {
+            in[n]->get_const_target_ptr();
+            coeff[n]->get_const_target_ptr();
+            out[n]->get_target_ptr();
+        }
+        check(gt355_synchronize(), "sync");
+        for (int round = 0; round < 6; ++round)
+            for (int n = 0; n < 2; ++n) {
+                gfx950<> be;
+                be.stream = s[n];
+                run(spec, be, g, in[n], coeff[n], out[n]);
+            }
+        check(gt355_synchronize(), "sync");
+        bool ok = true;
+        for (int n = 0; n < 2; ++n) {
+            auto hin = mirror(in[n]), hco = mirror(coeff[n]);
+            std::vector<double> ref(hin.size(), 0.);
+            auto st = in[n]->strides();
+            const size_t o = h + h * st[1];
+            gto_hdiff_f64(ref.data() + o, hin.data() + o, hco.data() + o, ni, nj, nk, st[1], st[2], 4);
+            ok = ok && same_interior(out[n], ref, g);
+        }
+        for (auto &x : s)
+            (void)hipStreamDestroy(x);
+        report("hdiff spec with HBM temporaries on two streams at once", ok);
+    }
+
+    // --bench: the engine on the reference's hdiff, vadv and tridiagonal specs at 256x256x80 FP64 over 4 rotating
+    // field sets (past the 256 MB MALL), mean microseconds per run from one hipEvent pair around `reps` back-to-back
+    // runs
     int bench(int reps) {
         const int ni = 256, nj = 256, nk = 80, nsets = 4;
-        std::vector<store<double>> hin, hco, hout, uts, us, up, ut, wc;
+        std::vector<store<double>> hin, hco, hout, uts, us, up, ut, wc, inf, dia, sup, rhs, tout;
         for (int s = 0; s < nsets; ++s) {
             hin.push_back(seeded<double>(ni + 4, nj + 4, nk, 2, 42 + s, 1, -1, 1));
             hco.push_back(seeded<double>(ni + 4, nj + 4, nk, 2, 42 + s, 2, 0, 0.25));
@@ -944,32 +816,29 @@ namespace {
             up.push_back(seeded<double>(ni + 6, nj + 6, nk, 3, 42 + s, 4, -1, 1));
             ut.push_back(seeded<double>(ni + 6, nj + 6, nk, 3, 42 + s, 5, -1, 1));
             wc.push_back(seeded<double>(ni + 6, nj + 6, nk + 1, 3, 42 + s, 7, -0.5, 0.5));
+            inf.push_back(seeded<double>(ni, nj, nk, 0, 42 + s, 11, -1, 1));
+            dia.push_back(seeded<double>(ni, nj, nk, 0, 42 + s, 12, 3, 4));
+            sup.push_back(seeded<double>(ni, nj, nk, 0, 42 + s, 13, -1, 1));
+            rhs.push_back(seeded<double>(ni, nj, nk, 0, 42 + s, 14, -1, 1));
+            tout.push_back(storage::builder<storage::gpu>.type<double>().dimensions(ni, nj, nk).build());
         }
         auto gh = halo_grid(ni, nj, nk, 2), gv = halo_grid(ni, nj, nk, 3);
+        auto gt = make_grid(ni, nj, nk);
         auto hd = [&](int s) {
-            run([](auto in, auto coeff, auto out) {
-                GT_DECLARE_TMP(double, lap, flx, fly);
-                return execute_parallel()
-                    .ij_cached(lap, flx, fly)
-                    .stage(hd_lap(), lap, in)
-                    .stage(hd_flx(), flx, in, lap)
-                    .stage(hd_fly(), fly, in, lap)
-                    .stage(hd_out(), out, in, flx, fly, coeff);
-            },
-                gfx950<>(), gh, hin[s % nsets], hco[s % nsets], hout[s % nsets]);
+            run(reference_specs::horizontal_diffusion::spec<double>(), gfx950<>(), gh, hin[s % nsets], hco[s % nsets],
+                hout[s % nsets]);
         };
         auto va = [&](int s) {
-            run([](auto utens_stage, auto u_stage, auto wcon, auto u_pos, auto utens, auto dtr_stage) {
-                GT_DECLARE_TMP(double, ccol, dcol, data_col);
-                return multi_pass(execute_forward()
-                                      .k_cached(cache_io_policy::flush(), ccol, dcol)
-                                      .k_cached(cache_io_policy::fill(), u_stage)
-                                      .stage(va_forward(), utens_stage, wcon, u_stage, u_pos, utens, dtr_stage, ccol, dcol),
-                    execute_backward().k_cached(data_col).stage(
-                        va_backward(), utens_stage, u_pos, dtr_stage, ccol, dcol, data_col));
+            run(reference_specs::vertical_advection_dycore::spec<double>(), gfx950<>(), gv, uts[s % nsets], us[s % nsets],
+                wc[s % nsets], up[s % nsets], ut[s % nsets], global_parameter(0.15));
+        };
+        auto tri = [&](int s) {
+            run([](auto inf, auto diag, auto sup, auto rhs, auto out) {
+                using namespace reference_specs::tridiagonal;
+                return multi_pass(execute_forward().stage(forward_thomas(), inf, diag, sup, rhs),
+                    execute_backward().stage(backward_thomas(), out, sup, rhs));
             },
-                gfx950<>(), gv, uts[s % nsets], us[s % nsets], wc[s % nsets], up[s % nsets], ut[s % nsets],
-                global_parameter(0.15));
+                gfx950<>(), gt, inf[s % nsets], dia[s % nsets], sup[s % nsets], rhs[s % nsets], tout[s % nsets]);
         };
         auto time_us = [&](auto &&fn) {
             for (int s = 0; s < nsets; ++s)
@@ -989,11 +858,14 @@ namespace {
             (void)hipEventDestroy(e1);
             return 1e3 * ms / reps;
         };
-        const double hd_us = time_us(hd), va_us = time_us(va);
+        const double hd_us = time_us(hd), va_us = time_us(va), tri_us = time_us(tri);
         const double pts = double(ni) * nj * nk;
-        std::printf("{\"hdiff_fused_lds_us\": %.2f, \"hdiff_gbs\": %.1f, \"vadv_vertical_us\": %.2f, \"vadv_gbs\": "
-                    "%.1f, \"reps\": %d}\n",
-            hd_us, 24 * pts / (hd_us * 1e-6) / 1e9, va_us, 48 * pts / (va_us * 1e-6) / 1e9, reps);
+        // algorithmic bytes per point: hdiff 24 (in, coeff, out), vadv 48 (five reads, one write), tridiagonal 56
+        // (inf, diag, sup, rhs read; sup, rhs, out written)
+        std::printf("{\"hdiff_spec_us\": %.2f, \"hdiff_gbs\": %.1f, \"vadv_spec_us\": %.2f, \"vadv_gbs\": %.1f, "
+                    "\"tridiagonal_spec_us\": %.2f, \"tridiagonal_gbs\": %.1f, \"reps\": %d}\n",
+            hd_us, 24 * pts / (hd_us * 1e-6) / 1e9, va_us, 48 * pts / (va_us * 1e-6) / 1e9, tri_us,
+            56 * pts / (tri_us * 1e-6) / 1e9, reps);
         return 0;
     }
 } // namespace
@@ -1021,14 +893,21 @@ int main(int argc, char **argv) {
         tridiagonal(23, 11, 40, false);
         simple_hori_diff(12, 33, 61);
         stencil_functions(23, 11, 43);
-        hdiff_functions<hf_var::monolithic>("monolithic", 23, 11, 43);
-        hdiff_functions<hf_var::call>("call", 23, 11, 43);
-        hdiff_functions<hf_var::call_offsets>("call_offsets", 23, 11, 43);
-        hdiff_functions<hf_var::procedures>("procedures", 23, 11, 43);
-        hdiff_functions<hf_var::procedures_offsets>("procedures_offsets", 64, 64, 8);
+        hdiff_functions<reference_specs::horizontal_diffusion_functions::variation::monolithic>("monolithic", 23, 11, 43);
+        hdiff_functions<reference_specs::horizontal_diffusion_functions::variation::call>("call", 23, 11, 43);
+        hdiff_functions<reference_specs::horizontal_diffusion_functions::variation::call_offsets>("call_offsets", 23, 11, 43);
+        hdiff_functions<reference_specs::horizontal_diffusion_functions::variation::procedures>("procedures", 23, 11, 43);
+        hdiff_functions<reference_specs::horizontal_diffusion_functions::variation::procedures_offsets>("procedures_offsets", 64, 64, 8);
         expandable_parameters(23, 11, 43);
         positional_and_fusion();
         expandable_and_errors();
+        kcache_tests();
+        kparallel_tests();
+        whole_axis_access(12, 33, 61);
+        whole_axis_access(23, 11, 43);
+        precision_policy(23, 11, 43);
+        precision_policy(256, 256, 80);
+        two_streams(130, 70, 24);
     } catch (std::exception const &e) {
         std::printf("[FAIL] exception: %s\n", e.what());
         return 1;
