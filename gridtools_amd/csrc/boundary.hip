@@ -6,9 +6,11 @@
 // periodic process grid, and 12 unpack kernels (gcl/high_level/m_pack*.hpp, m_unpack*.hpp,
 // descriptors_manual_gpu.hpp:342-512).
 //
-// Here every call is one launch over exactly the halo points it touches: the host lists the selected regions
-// with their boxes and prefix offsets, and each thread finds its region by a scan over at most 26 entries,
-// then walks its point with i fastest.  All fields of a call share the launch.
+// Here every call is one launch over exactly the halo points it touches.  The host lists the selected regions with
+// their boxes; each region gets its own run of workgroups, and a workgroup finds its region once (a uniform scan of
+// at most 26 block offsets).  Inside a region points are numbered with i fastest in 32-bit arithmetic, and a
+// thread's (i, j, k) comes from two divisions by the region's extents done as multiply-high with host-computed
+// magic numbers, so no point pays a 64-bit division or a region search.  All fields of a call share the launch.
 #include <hip/hip_runtime.h>
 
 #include "common.hpp"
@@ -18,6 +20,25 @@ namespace gt355 {
 
         constexpr int MAX_FIELDS = 4;
 
+        // x / d for 32-bit x as ((mulhi(x, m) + x) >> l): the round-up method of Granlund and Montgomery
+        // ("Division by invariant integers using multiplication", PLDI 1994) with l = ceil(log2 d) and
+        // m = floor(2^32 (2^l - d) / d) + 1, exact for every 32-bit x.
+        struct fastdiv {
+            uint32_t d, m, l;
+        };
+        inline fastdiv make_fastdiv(uint32_t d) {
+            uint32_t l = 0;
+            while ((uint64_t(1) << l) < d)
+                ++l;
+            const uint64_t m = ((uint64_t(1) << 32) * ((uint64_t(1) << l) - d)) / d + 1;
+            return fastdiv{d, (uint32_t)m, l};
+        }
+        __device__ __forceinline__ uint32_t div_by(uint32_t x, fastdiv f) {
+            return (uint32_t)(((uint64_t)__umulhi(x, f.m) + x) >> f.l);
+        }
+
+        constexpr int BC_BLOCK = 256, BC_PER_THREAD = 4, BC_CHUNK = BC_BLOCK * BC_PER_THREAD;
+
         struct region {
             int32_t lo[3];   // first index of the region (storage coordinates)
             int32_t n[3];    // extent
@@ -25,13 +46,33 @@ namespace gt355 {
             int64_t start;   // prefix offset in points
             int64_t src;     // unpack: start of the buffer segment this region reads (start, or the segment of the
                              // opposite direction for a local periodic exchange)
+            int32_t block0;  // first workgroup of the region
+            fastdiv d0, d1;  // division by n[0] and by n[0] * n[1]
         };
 
         struct region_table {
             region r[26];
             int count;
             int64_t total;
+            int blocks; // workgroups of the launch: every region's chunks of BC_CHUNK points
         };
+
+        // the region of this workgroup (uniform) and the point's coordinates
+        __device__ __forceinline__ int block_region(const region_table &t) {
+            int q = 0;
+#pragma unroll 1
+            for (int c = 1; c < t.count; ++c)
+                if ((int)blockIdx.x >= t.r[c].block0)
+                    q = c;
+            return q;
+        }
+        __device__ __forceinline__ void point_of(const region &r, uint32_t loc, int &i, int &j, int &k) {
+            const uint32_t row = div_by(loc, r.d0);
+            const uint32_t plane = div_by(loc, r.d1);
+            i = r.lo[0] + (int)(loc - row * (uint32_t)r.n[0]);
+            j = r.lo[1] + (int)(row - plane * (uint32_t)r.n[1]);
+            k = r.lo[2] + (int)plane;
+        }
 
         struct bc_args {
             void *dst[MAX_FIELDS];
@@ -43,40 +84,38 @@ namespace gt355 {
             double value;
         };
 
-        __device__ __forceinline__ int find_region(const region_table &t, int64_t n) {
-            int lo = 0;
-#pragma unroll 1
-            for (int q = 1; q < t.count; ++q)
-                if (n >= t.r[q].start)
-                    lo = q;
-            return lo;
-        }
-
         // kind 0 zero, 1 value, 2 copy, 3 halo fill (mode per dimension)
         template <class T, int KIND>
-        __global__ void __launch_bounds__(256) bc_kernel(region_table t, bc_args a) {
-            for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < t.total;
-                 n += (int64_t)gridDim.x * blockDim.x) {
-                const region &r = t.r[find_region(t, n)];
-                int64_t loc = n - r.start;
-                int i = r.lo[0] + int(loc % r.n[0]);
-                int64_t q = loc / r.n[0];
-                int j = r.lo[1] + int(q % r.n[1]);
-                int k = r.lo[2] + int(q / r.n[1]);
-                int64_t p = i * a.s[0] + j * a.s[1] + k * a.s[2];
-                if (KIND == 3) {
-                    int c[3] = {i, j, k};
+        __global__ void __launch_bounds__(BC_BLOCK) bc_kernel(region_table t, bc_args a) {
+            const region &r = t.r[block_region(t)];
+            const uint32_t vol = (uint32_t)r.n[0] * (uint32_t)r.n[1] * (uint32_t)r.n[2];
+            const uint32_t base = (uint32_t)((int)blockIdx.x - r.block0) * BC_CHUNK + threadIdx.x;
+            // the source offset of a halo fill: clamped (mode 1) or wrapped (mode 2) along each nonzero component
+            int64_t shift = 0;
+            int clamp[3] = {-1, -1, -1};
+            if (KIND == 3) {
 #pragma unroll
-                    for (int d = 0; d < 3; ++d) {
-                        if (r.dir[d] == 0)
-                            continue;
-                        int len = a.end[d] - a.begin[d] + 1;
-                        if (a.mode[d] == 1)
-                            c[d] = r.dir[d] < 0 ? a.begin[d] : a.end[d];
-                        else
-                            c[d] = r.dir[d] < 0 ? c[d] + len : c[d] - len;
-                    }
-                    int64_t sp = c[0] * a.s[0] + c[1] * a.s[1] + c[2] * a.s[2];
+                for (int d = 0; d < 3; ++d) {
+                    if (r.dir[d] == 0)
+                        continue;
+                    const int len = a.end[d] - a.begin[d] + 1;
+                    if (a.mode[d] == 1)
+                        clamp[d] = r.dir[d] < 0 ? a.begin[d] : a.end[d];
+                    else
+                        shift += (int64_t)(r.dir[d] < 0 ? len : -len) * a.s[d];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < BC_PER_THREAD; ++u) {
+                const uint32_t loc = base + u * BC_BLOCK;
+                if (loc >= vol)
+                    break;
+                int i, j, k;
+                point_of(r, loc, i, j, k);
+                const int64_t p = i * a.s[0] + j * a.s[1] + k * a.s[2];
+                if (KIND == 3) {
+                    const int64_t sp = (clamp[0] >= 0 ? clamp[0] : i) * a.s[0] + (clamp[1] >= 0 ? clamp[1] : j) * a.s[1] +
+                                       (clamp[2] >= 0 ? clamp[2] : k) * a.s[2] + shift;
                     for (int f = 0; f < a.nfields; ++f) {
                         T *dst = (T *)a.dst[f];
                         dst[p] = dst[sp];
@@ -98,23 +137,24 @@ namespace gt355 {
         // pack (UNPACK = false): buffer <- field slab; unpack: field slab <- buffer.  Buffer order: region, field,
         // then the slab with i fastest.
         template <class T, bool UNPACK>
-        __global__ void __launch_bounds__(256) pack_kernel(region_table t, bc_args a, T *buf) {
-            for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < t.total;
-                 n += (int64_t)gridDim.x * blockDim.x) {
-                const region &r = t.r[find_region(t, n)];
-                int64_t loc = n - r.start;
-                int64_t vol = (int64_t)r.n[0] * r.n[1] * r.n[2];
-                int i = r.lo[0] + int(loc % r.n[0]);
-                int64_t q = loc / r.n[0];
-                int j = r.lo[1] + int(q % r.n[1]);
-                int k = r.lo[2] + int(q / r.n[1]);
-                int64_t p = i * a.s[0] + j * a.s[1] + k * a.s[2];
+        __global__ void __launch_bounds__(BC_BLOCK) pack_kernel(region_table t, bc_args a, T *buf) {
+            const region &r = t.r[block_region(t)];
+            const uint32_t vol = (uint32_t)r.n[0] * (uint32_t)r.n[1] * (uint32_t)r.n[2];
+            const uint32_t base = (uint32_t)((int)blockIdx.x - r.block0) * BC_CHUNK + threadIdx.x;
+#pragma unroll
+            for (int u = 0; u < BC_PER_THREAD; ++u) {
+                const uint32_t loc = base + u * BC_BLOCK;
+                if (loc >= vol)
+                    break;
+                int i, j, k;
+                point_of(r, loc, i, j, k);
+                const int64_t p = i * a.s[0] + j * a.s[1] + k * a.s[2];
                 for (int f = 0; f < a.nfields; ++f) {
                     T *fld = (T *)a.dst[f];
                     if (UNPACK)
-                        fld[p] = buf[r.src * a.nfields + f * vol + loc];
+                        fld[p] = buf[r.src * a.nfields + (int64_t)f * vol + loc];
                     else
-                        buf[r.start * a.nfields + f * vol + loc] = fld[p];
+                        buf[r.start * a.nfields + (int64_t)f * vol + loc] = fld[p];
                 }
             }
         }
@@ -178,11 +218,24 @@ namespace gt355 {
                             continue;
                         r.start = start;
                         r.src = start;
+                        r.block0 = t.blocks;
+                        r.d0 = make_fastdiv((uint32_t)r.n[0]);
+                        r.d1 = make_fastdiv((uint32_t)r.n[0] * (uint32_t)r.n[1]);
+                        t.blocks += (int)((vol + BC_CHUNK - 1) / BC_CHUNK);
                         start += vol;
                         t.r[t.count++] = r;
                     }
             t.total = start;
             return t;
+        }
+
+        // every region's points fit 32-bit numbering, and the workgroups fit one launch
+        int check_table(const region_table &t, const char *who) {
+            for (int q = 0; q < t.count; ++q)
+                GT355_CHECK((int64_t)t.r[q].n[0] * t.r[q].n[1] * t.r[q].n[2] < (int64_t(1) << 31),
+                    "%s: a halo region holds 2^31 points or more", who);
+            GT355_CHECK(t.blocks < (1 << 30), "%s: too many halo points", who);
+            return GT355_OK;
         }
 
         int fill_args(bc_args &a, int dtype, void *const *dst, void *const *src, int nfields, const int32_t halo[3][5],
@@ -208,7 +261,6 @@ namespace gt355 {
             return GT355_OK;
         }
 
-        int blocks_for(int64_t total) { return (int)std::min<int64_t>((total + 255) / 256, 16384); }
     } // namespace
 } // namespace gt355
 
@@ -233,13 +285,15 @@ extern "C" int gt355_bc_apply(int kind, int dtype, void *const *dst, void *const
     region_table t = make_table(halo, dir_mask, 0, nullptr);
     if (t.total == 0)
         return GT355_OK;
+    if ((rc = check_table(t, "gt355_bc_apply")))
+        return rc;
     hipStream_t s = (hipStream_t)stream;
-    int nb = blocks_for(t.total);
+    const int nb = t.blocks;
 #define GT355_BC_LAUNCH(T)                                                  \
     switch (kind) {                                                         \
-    case 0: bc_kernel<T, 0><<<nb, 256, 0, s>>>(t, a); break;                \
-    case 1: bc_kernel<T, 1><<<nb, 256, 0, s>>>(t, a); break;                \
-    default: bc_kernel<T, 2><<<nb, 256, 0, s>>>(t, a); break;               \
+    case 0: bc_kernel<T, 0><<<nb, BC_BLOCK, 0, s>>>(t, a); break;                \
+    case 1: bc_kernel<T, 1><<<nb, BC_BLOCK, 0, s>>>(t, a); break;                \
+    default: bc_kernel<T, 2><<<nb, BC_BLOCK, 0, s>>>(t, a); break;               \
     }
     if (dtype == GT355_F64) {
         GT355_BC_LAUNCH(double)
@@ -269,12 +323,14 @@ extern "C" int gt355_halo_fill(int dtype, void *const *fields, int nfields, cons
     region_table t = make_table(halo, 0x7FFFFFFu, 0, mode);
     if (t.total == 0)
         return GT355_OK;
+    if ((rc = check_table(t, "gt355_halo_fill")))
+        return rc;
     hipStream_t s = (hipStream_t)stream;
-    int nb = blocks_for(t.total);
+    const int nb = t.blocks;
     if (dtype == GT355_F64)
-        bc_kernel<double, 3><<<nb, 256, 0, s>>>(t, a);
+        bc_kernel<double, 3><<<nb, BC_BLOCK, 0, s>>>(t, a);
     else
-        bc_kernel<float, 3><<<nb, 256, 0, s>>>(t, a);
+        bc_kernel<float, 3><<<nb, BC_BLOCK, 0, s>>>(t, a);
     return check_launch("gt355_halo_fill");
 }
 
@@ -314,18 +370,20 @@ static int halo_pack_impl(bool unpack, int dtype, void *const *fields, int nfiel
     }
     if (t.total == 0)
         return GT355_OK;
+    if ((rc = check_table(t, who)))
+        return rc;
     hipStream_t s = (hipStream_t)stream;
-    int nb = blocks_for(t.total);
+    const int nb = t.blocks;
     if (dtype == GT355_F64) {
         if (unpack)
-            pack_kernel<double, true><<<nb, 256, 0, s>>>(t, a, (double *)buffer);
+            pack_kernel<double, true><<<nb, BC_BLOCK, 0, s>>>(t, a, (double *)buffer);
         else
-            pack_kernel<double, false><<<nb, 256, 0, s>>>(t, a, (double *)buffer);
+            pack_kernel<double, false><<<nb, BC_BLOCK, 0, s>>>(t, a, (double *)buffer);
     } else {
         if (unpack)
-            pack_kernel<float, true><<<nb, 256, 0, s>>>(t, a, (float *)buffer);
+            pack_kernel<float, true><<<nb, BC_BLOCK, 0, s>>>(t, a, (float *)buffer);
         else
-            pack_kernel<float, false><<<nb, 256, 0, s>>>(t, a, (float *)buffer);
+            pack_kernel<float, false><<<nb, BC_BLOCK, 0, s>>>(t, a, (float *)buffer);
     }
     return check_launch(who);
 }
@@ -371,13 +429,15 @@ extern "C" int gt355_halo_exchange_local(int dtype, void *const *fields, int nfi
     }
     if (tp.total == 0)
         return GT355_OK;
+    if ((rc = check_table(tp, who)) || (rc = check_table(tu, who)))
+        return rc;
     hipStream_t s = (hipStream_t)stream;
     if (dtype == GT355_F64) {
-        pack_kernel<double, false><<<blocks_for(tp.total), 256, 0, s>>>(tp, a, (double *)buffer);
-        pack_kernel<double, true><<<blocks_for(tu.total), 256, 0, s>>>(tu, a, (double *)buffer);
+        pack_kernel<double, false><<<tp.blocks, BC_BLOCK, 0, s>>>(tp, a, (double *)buffer);
+        pack_kernel<double, true><<<tu.blocks, BC_BLOCK, 0, s>>>(tu, a, (double *)buffer);
     } else {
-        pack_kernel<float, false><<<blocks_for(tp.total), 256, 0, s>>>(tp, a, (float *)buffer);
-        pack_kernel<float, true><<<blocks_for(tu.total), 256, 0, s>>>(tu, a, (float *)buffer);
+        pack_kernel<float, false><<<tp.blocks, BC_BLOCK, 0, s>>>(tp, a, (float *)buffer);
+        pack_kernel<float, true><<<tu.blocks, BC_BLOCK, 0, s>>>(tu, a, (float *)buffer);
     }
     return check_launch(who);
 }

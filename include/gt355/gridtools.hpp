@@ -71,11 +71,13 @@ namespace gt355 {
         // point 128-byte aligned, device memory with a lazily synchronised host mirror.
         struct gpu {};
 
+        // the library's element type code (gt355_dtype); storages of other arithmetic types (int for example, as
+        // boundary functors may use) get -1, which the library's stencil entry points reject
         template <class T>
         constexpr int dtype_of() {
-            static_assert(std::is_same<T, double>::value || std::is_same<T, float>::value,
-                "gt355 storage holds double or float");
-            return std::is_same<T, double>::value ? GT355_F64 : GT355_F32;
+            static_assert(std::is_arithmetic<T>::value && (sizeof(T) == 4 || sizeof(T) == 8),
+                "gt355 storage holds 4- or 8-byte arithmetic values");
+            return std::is_same<T, double>::value ? GT355_F64 : std::is_same<T, float>::value ? GT355_F32 : -1;
         }
 
         template <class T>
@@ -712,14 +714,74 @@ namespace gt355 {
         };
         struct copy_boundary {};
 
-        // precomputed_pred (boundaries/predicate.hpp:27-32) as a 27-bit direction mask: bit
-        // (di+1)*9 + (dj+1)*3 + (dk+1) selects direction (di, dj, dk)
+        // direction<I, J, K> (boundaries/direction.hpp:26-76): one of the 26 halo regions, each component minus_,
+        // zero_ or plus_
+        enum sign { any_ = -2, minus_ = -1, zero_ = 0, plus_ = 1 };
+        template <sign I, sign J, sign K>
+        struct direction {
+            static constexpr sign i = I, j = J, k = K;
+        };
+
+        // default_predicate (boundaries/predicate.hpp:27-32): every direction
+        struct default_predicate {
+            template <class Direction>
+            bool operator()(Direction) const {
+                return true;
+            }
+        };
+
+        // precomputed_pred as a 27-bit direction mask: bit (di+1)*9 + (dj+1)*3 + (dk+1) selects direction (di, dj, dk)
         constexpr uint32_t all_directions = 0x7FFFFFFu;
         constexpr uint32_t direction_bit(int di, int dj, int dk) { return 1u << ((di + 1) * 9 + (dj + 1) * 3 + (dk + 1)); }
 
-        // boundary<BC, Arch> (boundaries/boundary.hpp:31-79): apply(fields...) fills the halo points of the 26
-        // regions the halo descriptors define, for the directions the predicate selects.
-        template <class BC, class Arch = storage::gpu>
+        namespace bc_impl {
+            // a predicate evaluated once per direction on the host (the reference evaluates it per region at apply
+            // time, apply_gpu.hpp:52-83): a callable taking direction<I, J, K>, or a ready 27-bit mask
+            template <class Pred>
+            uint32_t mask_of(Pred const &pred) {
+                if constexpr (std::is_integral<Pred>::value) {
+                    return (uint32_t)pred;
+                } else {
+                    uint32_t m = 0;
+                    auto one = [&](auto d) {
+                        using D = decltype(d);
+                        if (pred(d))
+                            m |= direction_bit(D::i, D::j, D::k);
+                    };
+                    auto row = [&](auto di) {
+                        constexpr sign I = decltype(di)::value;
+                        one(direction<I, minus_, minus_>()), one(direction<I, minus_, zero_>());
+                        one(direction<I, minus_, plus_>()), one(direction<I, zero_, minus_>());
+                        if constexpr (I != zero_)
+                            one(direction<I, zero_, zero_>());
+                        one(direction<I, zero_, plus_>()), one(direction<I, plus_, minus_>());
+                        one(direction<I, plus_, zero_>()), one(direction<I, plus_, plus_>());
+                    };
+                    row(std::integral_constant<sign, minus_>());
+                    row(std::integral_constant<sign, zero_>());
+                    row(std::integral_constant<sign, plus_>());
+                    return m;
+                }
+            }
+            template <class BC>
+            constexpr bool is_predefined = std::is_same<BC, zero_boundary>::value ||
+                                           std::is_same<BC, copy_boundary>::value;
+            template <class T>
+            constexpr bool is_predefined<value_boundary<T>> = true;
+
+            // user functors run as device code: defined by gt355/boundaries.hpp, which a translation unit compiled
+            // by hipcc includes
+            template <class BC, class... Stores>
+            void apply_user(
+                std::array<halo_descriptor, 3> const &halos, BC const &bc, uint32_t mask, void *stream, Stores const &...);
+        } // namespace bc_impl
+
+        // boundary<BC, Arch, Predicate> (boundaries/boundary.hpp:31-79): apply(fields...) fills the halo points of
+        // the 26 regions the halo descriptors define, for the directions the predicate selects.  zero_boundary,
+        // value_boundary and copy_boundary run on the library's kernel; any other functor
+        // (operator()(direction<I, J, K>, fields&..., i, j, k)) is instantiated as a device kernel by
+        // gt355/boundaries.hpp.
+        template <class BC, class Arch = storage::gpu, class Predicate = default_predicate>
         class boundary {
             std::array<halo_descriptor, 3> m_halos;
             BC m_bc;
@@ -743,13 +805,15 @@ namespace gt355 {
             }
 
           public:
-            boundary(std::array<halo_descriptor, 3> const &halos, BC const &bc, uint32_t predicate = all_directions,
+            boundary(std::array<halo_descriptor, 3> const &halos, BC const &bc, Predicate predicate = Predicate(),
                 void *stream = nullptr)
-                : m_halos(halos), m_bc(bc), m_mask(predicate), m_stream(stream) {}
+                : m_halos(halos), m_bc(bc), m_mask(bc_impl::mask_of(predicate)), m_stream(stream) {}
 
             template <class T, class... Rest>
             void apply(std::shared_ptr<storage::data_store<T>> const &first, Rest const &... rest) const {
-                if constexpr (std::is_same<BC, copy_boundary>::value) {
+                if constexpr (!bc_impl::is_predefined<BC>) {
+                    bc_impl::apply_user(m_halos, m_bc, m_mask, m_stream, first, rest...);
+                } else if constexpr (std::is_same<BC, copy_boundary>::value) {
                     static_assert(sizeof...(Rest) == 1, "copy_boundary applies to (dst, src)");
                     auto src = std::get<0>(std::tie(rest...));
                     if (src->strides() != first->strides())
@@ -769,10 +833,12 @@ namespace gt355 {
             }
         };
 
-        template <class Arch = storage::gpu, class BC>
-        boundary<BC, Arch> make_boundary(std::array<halo_descriptor, 3> const &halos, BC const &bc,
-            uint32_t predicate = all_directions, void *stream = nullptr) {
-            return boundary<BC, Arch>(halos, bc, predicate, stream);
+        // make_boundary<Arch>(halos, functor[, predicate]) (boundaries/boundary.hpp:74-79); the predicate is a
+        // callable over direction<I, J, K> or a 27-bit direction mask
+        template <class Arch = storage::gpu, class BC, class Predicate = default_predicate>
+        boundary<BC, Arch, Predicate> make_boundary(std::array<halo_descriptor, 3> const &halos, BC const &bc,
+            Predicate predicate = Predicate(), void *stream = nullptr) {
+            return boundary<BC, Arch, Predicate>(halos, bc, predicate, stream);
         }
 
         // The single-rank effect of a periodic / zero-gradient halo update (gt355_halo_fill): mode per dimension

@@ -26,10 +26,13 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdlib>
+#include <cstring>
 #include <map>
 #include <mutex>
 
+#include "functors.hpp"
 #include "gridtools.hpp"
 
 // k_cached placeholders with the fill policy: 0 (default) reads them from memory at each level, where their loads
@@ -376,11 +379,11 @@ namespace gt355 {
                 T *origin;
                 int64_t s0, s1, s2;
                 int64_t sx[3]; // strides of the extra data dimensions (0 where the field has none)
-                __device__ T &at(int i, int j, int k) const { return origin[i * s0 + j * s1 + k * s2]; }
-                __device__ T &at(int i, int j, int k, const int (&x)[3]) const {
+                __host__ __device__ T &at(int i, int j, int k) const { return origin[i * s0 + j * s1 + k * s2]; }
+                __host__ __device__ T &at(int i, int j, int k, const int (&x)[3]) const {
                     return origin[i * s0 + j * s1 + k * s2 + x[0] * sx[0] + x[1] * sx[1] + x[2] * sx[2]];
                 }
-                __device__ void rebase(int, int) {}
+                __host__ __device__ void rebase(int, int) {}
             };
             template <class V>
             struct is_field_view : std::false_type {};
@@ -389,16 +392,16 @@ namespace gt355 {
             template <class T>
             struct global_view {
                 T v;
-                __device__ T const &at(int, int, int) const { return v; }
-                __device__ void rebase(int, int) {}
+                __host__ __device__ T const &at(int, int, int) const { return v; }
+                __host__ __device__ void rebase(int, int) {}
             };
             template <class D>
             struct positional_view {
                 int origin;
-                __device__ int at(int i, int j, int k) const {
+                __host__ __device__ int at(int i, int j, int k) const {
                     return origin + (std::is_same<D, dim::i>::value ? i : std::is_same<D, dim::j>::value ? j : k);
                 }
-                __device__ void rebase(int, int) {}
+                __host__ __device__ void rebase(int, int) {}
             };
 
             // the dynamic LDS of a fused parallel multistage: its ij-cached temporaries, one tile each
@@ -452,11 +455,11 @@ namespace gt355 {
             // (kring) for that key; the evaluator reads level k + dk as element dk - KMIN of the window
             template <class T, int KMIN, int KMAX, int KEY>
             struct kring_ref {
-                __device__ void rebase(int, int) {}
+                __host__ __device__ void rebase(int, int) {}
             };
 
             template <int KEY, int... Is, class... Rs>
-            __device__ __forceinline__ auto &ring_by_key(vtuple_impl<std::integer_sequence<int, Is...>, Rs...> &rs) {
+            __host__ __device__ __forceinline__ auto &ring_by_key(vtuple_impl<std::integer_sequence<int, Is...>, Rs...> &rs) {
                 constexpr int idx = ((Rs::key == KEY ? Is : 0) + ... + 0);
                 return vget<idx>(rs);
             }
@@ -466,7 +469,7 @@ namespace gt355 {
             // the value or reference passes through.  Writable accesses always see the storage type, so a store
             // narrows once.
             template <bool W, class X>
-            __device__ __forceinline__ decltype(auto) widen_read(X &&x) {
+            __host__ __device__ __forceinline__ decltype(auto) widen_read(X &&x) {
                 if constexpr (W && std::is_same<std::decay_t<X>, float>::value)
                     return double(x);
                 else if constexpr (std::is_lvalue_reference<X>::value)
@@ -481,37 +484,37 @@ namespace gt355 {
                 Rings &rs;
                 int i, j, k;
                 template <class View, class Acc>
-                __device__ __forceinline__ decltype(auto) get(View const &view, Acc const &a) const {
+                __host__ __device__ __forceinline__ decltype(auto) get(View const &view, Acc const &a) const {
                     if constexpr (is_field_view<View>::value)
                         return view.at(i + a.di, j + a.dj, k + a.dk, a.x);
                     else
                         return view.at(i + a.di, j + a.dj, k + a.dk);
                 }
                 template <class T, int KMIN, int KMAX, int KEY, class Acc>
-                __device__ __forceinline__ T &get(kring_ref<T, KMIN, KMAX, KEY> const &, Acc const &a) const {
+                __host__ __device__ __forceinline__ T &get(kring_ref<T, KMIN, KMAX, KEY> const &, Acc const &a) const {
                     return ring_by_key<KEY>(rs).r[a.dk - KMIN];
                 }
                 template <int I, class E, int N>
-                __device__ __forceinline__ decltype(auto) operator()(cartesian::in_accessor<I, E, N> const &a) const {
+                __host__ __device__ __forceinline__ decltype(auto) operator()(cartesian::in_accessor<I, E, N> const &a) const {
                     return widen_read<W>(get(vget<I>(v), a));
                 }
                 template <int I, class E, int N>
-                __device__ __forceinline__ decltype(auto) operator()(cartesian::inout_accessor<I, E, N> const &a) const {
+                __host__ __device__ __forceinline__ decltype(auto) operator()(cartesian::inout_accessor<I, E, N> const &a) const {
                     return get(vget<I>(v), a);
                 }
                 template <class X>
-                __device__ auto operand(X const &x) const {
+                __host__ __device__ auto operand(X const &x) const {
                     if constexpr (std::is_arithmetic<X>::value)
                         return x;
                     else
                         return std::decay_t<decltype((*this)(x))>((*this)(x));
                 }
                 template <class Op, class L, class R>
-                __device__ auto operator()(cartesian::expr2<Op, L, R> const &e) const {
+                __host__ __device__ auto operator()(cartesian::expr2<Op, L, R> const &e) const {
                     return Op()(operand(e.l), operand(e.r));
                 }
                 template <class Op, class A>
-                __device__ auto operator()(cartesian::expr1<Op, A> const &e) const {
+                __host__ __device__ auto operator()(cartesian::expr1<Op, A> const &e) const {
                     return Op()(operand(e.a));
                 }
             };
@@ -764,7 +767,7 @@ namespace gt355 {
             // runs F at level k: the plain apply everywhere, else the first interval overload whose interval
             // contains k (none: nothing to do at this level), like the reference's interval matching
             template <class F, class E, int M, int N>
-            __device__ __forceinline__ void try_interval(E &e, int k, ksplit const &ks, bool &done) {
+            __host__ __device__ __forceinline__ void try_interval(E &e, int k, ksplit const &ks, bool &done) {
                 if constexpr (cand<M, N>::valid) {
                     using I = typename cand<M, N>::type;
                     if constexpr (has_interval_apply<F, E, I>::value) {
@@ -776,7 +779,7 @@ namespace gt355 {
                 }
             }
             template <class F, class E, int M, int... Ns>
-            __device__ __forceinline__ void apply_at(E &e, int k, ksplit const &ks, std::integer_sequence<int, Ns...>) {
+            __host__ __device__ __forceinline__ void apply_at(E &e, int k, ksplit const &ks, std::integer_sequence<int, Ns...>) {
                 if constexpr (has_plain_apply<F, E>::value) {
                     F::apply(e);
                 } else {
@@ -785,7 +788,7 @@ namespace gt355 {
                 }
             }
             template <class F, int NSPL, bool W, class Views, class Rings>
-            __device__ __forceinline__ void run_stage_at(
+            __host__ __device__ __forceinline__ void run_stage_at(
                 Views const &v, Rings &rs, int i, int j, int k, ksplit const &ks) {
                 using E = evaluator<Views, Rings, W>;
                 E e{v, rs, i, j, k};
@@ -793,7 +796,7 @@ namespace gt355 {
                 apply_at<F, E, M>(e, k, ks, std::make_integer_sequence<int, M * M>());
             }
             template <class F, int NSPL, bool W, class Views>
-            __device__ __forceinline__ void run_stage_at(Views const &v, int i, int j, int k, ksplit const &ks) {
+            __host__ __device__ __forceinline__ void run_stage_at(Views const &v, int i, int j, int k, ksplit const &ks) {
                 vtuple<> none;
                 run_stage_at<F, NSPL, W>(v, none, i, j, k, ks);
             }
@@ -1523,14 +1526,14 @@ namespace gt355 {
                 Args const &args;
                 Res *res;
                 template <int I>
-                __device__ __forceinline__ decltype(auto) param() const {
+                __host__ __device__ __forceinline__ decltype(auto) param() const {
                     if constexpr (I == OUT)
                         return (*res);
                     else
                         return vget<(OUT >= 0 && I > OUT ? I - 1 : I)>(args);
                 }
                 template <class P, class Acc>
-                __device__ __forceinline__ decltype(auto) bind(P &&p, Acc const &a) const {
+                __host__ __device__ __forceinline__ decltype(auto) bind(P &&p, Acc const &a) const {
                     using A = std::decay_t<P>;
                     if constexpr (is_accessor<A>::value)
                         return e(A(p.di + a.di + OI, p.dj + a.dj + OJ, p.dk + a.dk + OK, p.x[0] + a.x[0],
@@ -1539,32 +1542,32 @@ namespace gt355 {
                         return std::forward<P>(p);
                 }
                 template <int I, class E, int N>
-                __device__ __forceinline__ decltype(auto) operator()(cartesian::in_accessor<I, E, N> const &a) const {
+                __host__ __device__ __forceinline__ decltype(auto) operator()(cartesian::in_accessor<I, E, N> const &a) const {
                     return bind(param<I>(), a);
                 }
                 template <int I, class E, int N>
-                __device__ __forceinline__ decltype(auto) operator()(cartesian::inout_accessor<I, E, N> const &a) const {
+                __host__ __device__ __forceinline__ decltype(auto) operator()(cartesian::inout_accessor<I, E, N> const &a) const {
                     return bind(param<I>(), a);
                 }
                 template <class X>
-                __device__ auto operand(X const &x) const {
+                __host__ __device__ auto operand(X const &x) const {
                     if constexpr (std::is_arithmetic<X>::value)
                         return x;
                     else
                         return std::decay_t<decltype((*this)(x))>((*this)(x));
                 }
                 template <class Op, class L, class R>
-                __device__ auto operator()(cartesian::expr2<Op, L, R> const &x) const {
+                __host__ __device__ auto operator()(cartesian::expr2<Op, L, R> const &x) const {
                     return Op()(operand(x.l), operand(x.r));
                 }
                 template <class Op, class A>
-                __device__ auto operator()(cartesian::expr1<Op, A> const &x) const {
+                __host__ __device__ auto operator()(cartesian::expr1<Op, A> const &x) const {
                     return Op()(operand(x.a));
                 }
             };
 
             template <class F, class Region, class E>
-            __device__ __forceinline__ void call_functor(E &ce) {
+            __host__ __device__ __forceinline__ void call_functor(E &ce) {
                 if constexpr (std::is_void<Region>::value)
                     F::apply(ce);
                 else
@@ -1581,7 +1584,7 @@ namespace gt355 {
                 using return_type = call<F, Region, R, OI, OJ, OK>;
 
                 template <class Eval, class A0, class... Args>
-                __device__ __forceinline__ static auto with(Eval &eval, A0 &&a0, Args &&... args) {
+                __host__ __device__ __forceinline__ static auto with(Eval &eval, A0 &&a0, Args &&... args) {
                     using out_t = engine_::out_index<typename F::param_list>;
                     static_assert(out_t::count == 1, "call<>: the stencil function must have exactly one inout accessor");
                     using deduced = std::conditional_t<engine_::is_accessor<std::decay_t<A0>>::value,
@@ -1602,7 +1605,7 @@ namespace gt355 {
                 using at = call_proc<F, Region, I, J, K>;
 
                 template <class Eval, class... Args>
-                __device__ __forceinline__ static void with(Eval &eval, Args &&... args) {
+                __host__ __device__ __forceinline__ static void with(Eval &eval, Args &&... args) {
                     using Tup = engine_::vtuple<Args &&...>;
                     Tup t{{std::forward<Args>(args)}...};
                     engine_::call_eval<Eval, Tup, int, -1, OI, OJ, OK> ce{eval, t, nullptr};
@@ -1653,6 +1656,403 @@ namespace gt355 {
             }
         } // namespace engine_
 
+        namespace engine_ {
+            // ------------------------------------------------------------------------------------------ host execution
+            // A spec run on host memory in the reference's semantics: a parallel multistage stage by stage over each
+            // stage's extended box, a vertical multistage column by column in k order (k-caches are plain memory).
+            // The engine uses it to check, once per spec, whether a spec computes exactly what one of the library's
+            // hand-written kernels computes.
+            constexpr int HOST_HALO = 3;
+
+            struct host_store {
+                std::vector<std::vector<unsigned char>> mem;
+                template <class T>
+                field_rt make(int ni, int nj, int nk) {
+                    const int64_t s1 = ni + 2 * HOST_HALO, s2 = s1 * (nj + 2 * HOST_HALO);
+                    mem.emplace_back((size_t)(s2 * (nk + 2 * HOST_HALO)) * sizeof(T), (unsigned char)0);
+                    T *base = reinterpret_cast<T *>(mem.back().data());
+                    return {base + HOST_HALO * (1 + s1 + s2), {1, s1, s2}, {0, 0, 0}};
+                }
+            };
+            template <class T>
+            T &host_at(field_rt const &f, int i, int j, int k) {
+                return static_cast<T *>(f.origin)[i + j * f.s[1] + k * f.s[2]];
+            }
+
+            template <class... Ps>
+            void host_tmps(tlist<Ps...>, run_ctx &c, host_store &hs, run_geom const &g) {
+                auto one = [&](auto p) {
+                    using P = decltype(p);
+                    if constexpr (P::is_tmp)
+                        if (!c.fields.count(P::key))
+                            c.fields[P::key] = hs.template make<typename P::kind::data_t>(g.i_size, g.j_size, g.k_size);
+                };
+                (one(Ps()), ...);
+            }
+            template <class... Msses>
+            void host_all_tmps(tlist<Msses...>, run_ctx &c, host_store &hs, run_geom const &g) {
+                auto one = [&](auto mss) {
+                    using St = typename mss_traits<decltype(mss)>::stages;
+                    host_stage_tmps(St(), c, hs, g);
+                };
+                (one(Msses()), ...);
+            }
+            template <class... Stages>
+            void host_stage_tmps(tlist<Stages...>, run_ctx &c, host_store &hs, run_geom const &g) {
+                (host_tmps(typename plhs_of_stages<tlist<>, Stages>::type(), c, hs, g), ...);
+            }
+
+            template <int NSPL, bool W, class S, class ArgsTuple>
+            void host_stage_box(run_ctx const &c, ArgsTuple const &args, ext_rt const &e, run_geom const &g) {
+                auto v = stage_plhs<S>::views(c, args);
+                for (int k = 0; k < g.k_size; ++k)
+                    for (int j = e.jmin; j < g.j_size + e.jmax; ++j)
+                        for (int i = e.imin; i < g.i_size + e.imax; ++i)
+                            run_stage_at<typename S::functor, NSPL, W>(v, i, j, k, g.ks);
+            }
+            template <int NSPL, bool W, class Mss, class... Stages, class ArgsTuple, size_t... Ss>
+            void host_mss(tlist<Stages...>, std::index_sequence<Ss...>, std::vector<stage_rt> const &st, int first,
+                run_ctx const &c, ArgsTuple const &args, run_geom const &g) {
+                using Exec = typename mss_traits<Mss>::exec;
+                if constexpr (std::is_same<Exec, parallel>::value) {
+                    (host_stage_box<NSPL, W, Stages>(c, args, st[first + (int)Ss].ext, g), ...);
+                } else {
+                    auto views = std::make_tuple(stage_plhs<Stages>::views(c, args)...);
+                    ext_rt u;
+                    for (size_t n = 0; n < sizeof...(Stages); ++n)
+                        u.merge(st[first + (int)n].ext);
+                    constexpr bool up = !std::is_same<Exec, backward>::value;
+                    for (int j = u.jmin; j < g.j_size + u.jmax; ++j)
+                        for (int i = u.imin; i < g.i_size + u.imax; ++i)
+                            for (int n = 0; n < g.k_size; ++n) {
+                                const int k = up ? n : g.k_size - 1 - n;
+                                auto one = [&](auto s) {
+                                    constexpr size_t S = decltype(s)::value;
+                                    ext_rt const &e = st[first + (int)S].ext;
+                                    if (i >= e.imin && i < g.i_size + e.imax && j >= e.jmin && j < g.j_size + e.jmax)
+                                        run_stage_at<typename std::tuple_element_t<S, std::tuple<Stages...>>::functor,
+                                            NSPL, W>(std::get<S>(views), i, j, k, g.ks);
+                                };
+                                (one(std::integral_constant<size_t, Ss>()), ...);
+                            }
+                }
+            }
+            template <int NSPL, bool W, class... Msses, class ArgsTuple>
+            void host_run(tlist<Msses...>, std::vector<stage_rt> const &st, run_ctx const &c, ArgsTuple const &args,
+                run_geom const &g) {
+                int first = 0;
+                auto one = [&](auto mss) {
+                    using T = mss_traits<decltype(mss)>;
+                    host_mss<NSPL, W, decltype(mss)>(
+                        typename T::stages(), std::make_index_sequence<T::n>(), st, first, c, args, g);
+                    first += T::n;
+                };
+                (one(Msses()), ...);
+            }
+
+            // ------------------------------------------------------------------------------------------ fast path
+            // A spec that computes, operation for operation, what one of the library's hand-written kernels computes
+            // runs on that kernel (the job of the reference's backend entry point for a known spec,
+            // stencil/gpu/entry_point.hpp:253-260).  The decision is made once per spec type: the roles of the
+            // fields come from the access analysis (which field is written, at which extents each is read), then
+            // the spec runs on the host on seeded random inputs and must reproduce the hand kernel's formulas
+            // (gt355/functors.hpp) bit for bit at every point.  Anything else takes the generic engine.
+            struct access_summary {
+                bool present = false, written = false;
+                ext_rt ext; // union of the accessors' extents
+            };
+            inline std::map<int, access_summary> summarize(std::vector<stage_rt> const &st) {
+                std::map<int, access_summary> m;
+                for (auto const &s : st)
+                    for (auto const &a : s.acc) {
+                        auto &x = m[a.key];
+                        x.present = true;
+                        x.written = x.written || a.write;
+                        x.ext.merge(a.ext);
+                    }
+                return m;
+            }
+            inline bool ext_is(ext_rt const &e, int i0, int i1, int j0, int j1, int k0, int k1) {
+                return e.set && e.imin == i0 && e.imax == i1 && e.jmin == j0 && e.jmax == j1 && e.kmin == k0 &&
+                       e.kmax == k1;
+            }
+            // counter-based uniform values for the host check
+            inline double host_uniform(uint64_t n, double a, double b) {
+                uint64_t z = n * 0x9E3779B97F4A7C15ULL + 0x632BE59BD9B4E019ULL;
+                z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+                z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+                z ^= z >> 31;
+                return a + (b - a) * (double)(z >> 11) * 0x1.0p-53;
+            }
+            template <class T>
+            void host_fill(field_rt const &f, run_geom const &g, int seed, double a, double b, int extra_k = 0) {
+                uint64_t n = (uint64_t)seed << 32;
+                for (int k = -HOST_HALO; k < g.k_size + extra_k; ++k)
+                    for (int j = -HOST_HALO; j < g.j_size + HOST_HALO; ++j)
+                        for (int i = -HOST_HALO; i < g.i_size + HOST_HALO; ++i)
+                            host_at<T>(f, i, j, k) = T(host_uniform(n++, a, b));
+            }
+
+            enum class fast_kind { none, hdiff, vadv };
+            struct fast_plan {
+                fast_kind kind = fast_kind::none;
+                int role[6] = {-1, -1, -1, -1, -1, -1}; // hdiff: in, coeff, out; vadv: utens_stage, u_stage, wcon,
+                                                        // u_pos, utens, dtr_stage
+            };
+
+            template <class T>
+            constexpr bool fast_type = std::is_same<T, double>::value || std::is_same<T, float>::value;
+
+            // the hand kernel's horizontal diffusion on host fields (gt355_hdiff's formulas)
+            template <class T, class CT>
+            bool hdiff_reproduced(field_rt const &in, field_rt const &co, field_rt const &out, run_geom const &g) {
+                auto I = [&](int i, int j, int k) { return CT(host_at<T>(in, i, j, k)); };
+                auto lap = [&](int i, int j, int k) {
+                    return lap_f<CT>(I(i, j, k), I(i + 1, j, k), I(i, j + 1, k), I(i - 1, j, k), I(i, j - 1, k));
+                };
+                auto flx = [&](int i, int j, int k) {
+                    return flux_f<CT>(lap(i + 1, j, k), lap(i, j, k), I(i + 1, j, k), I(i, j, k));
+                };
+                auto fly = [&](int i, int j, int k) {
+                    return flux_f<CT>(lap(i, j + 1, k), lap(i, j, k), I(i, j + 1, k), I(i, j, k));
+                };
+                for (int k = 0; k < g.k_size; ++k)
+                    for (int j = 0; j < g.j_size; ++j)
+                        for (int i = 0; i < g.i_size; ++i) {
+                            const T e = T(hdiff_out_f<CT>(I(i, j, k), CT(host_at<T>(co, i, j, k)), flx(i, j, k),
+                                flx(i - 1, j, k), fly(i, j, k), fly(i, j - 1, k)));
+                            const T got = host_at<T>(out, i, j, k);
+                            if (std::memcmp(&e, &got, sizeof(T)) != 0)
+                                return false;
+                        }
+                return true;
+            }
+
+            // the hand kernel's vertical advection on host fields (gt355_vadv's formulas); uts0 holds the input
+            template <class T, class CT>
+            bool vadv_reproduced(field_rt const &uts0, field_rt const &uts, field_rt const &us, field_rt const &wc,
+                field_rt const &up, field_rt const &ut, T dtr_s, run_geom const &g) {
+                const int nk = g.k_size, top = nk - 1;
+                const CT dtr = CT(dtr_s);
+                std::vector<CT> c(nk), d(nk);
+                for (int j = 0; j < g.j_size; ++j)
+                    for (int i = 0; i < g.i_size; ++i) {
+                        auto V = [&](field_rt const &f, int ii, int k) { return CT(host_at<T>(f, ii, j, k)); };
+                        for (int k = 0; k < nk; ++k) {
+                            if (k == 0) {
+                                auto r = forward_first<CT>(dtr, V(wc, i, 1), V(wc, i + 1, 1), V(us, i, 0), V(us, i, 1),
+                                    V(up, i, 0), V(ut, i, 0), V(uts0, i, 0));
+                                c[0] = r.c, d[0] = r.d;
+                            } else if (k < top) {
+                                auto r = forward_body<CT>(dtr, V(wc, i, k), V(wc, i + 1, k), V(wc, i, k + 1),
+                                    V(wc, i + 1, k + 1), V(us, i, k - 1), V(us, i, k), V(us, i, k + 1), V(up, i, k),
+                                    V(ut, i, k), V(uts0, i, k), c[k - 1], d[k - 1], false);
+                                c[k] = r.c, d[k] = r.d;
+                            } else {
+                                d[k] = forward_last<CT>(dtr, V(wc, i, k), V(wc, i + 1, k), V(us, i, k - 1),
+                                    V(us, i, k), V(up, i, k), V(ut, i, k), V(uts0, i, k), c[k - 1], d[k - 1]);
+                            }
+                        }
+                        CT data = CT(0);
+                        for (int k = top; k >= 0; --k) {
+                            const CT nd = backward_data<CT>(c[k], d[k], data, k == top);
+                            const T e = T(dtr * (nd - V(up, i, k)));
+                            data = nd;
+                            const T got = host_at<T>(uts, i, j, k);
+                            if (std::memcmp(&e, &got, sizeof(T)) != 0)
+                                return false;
+                        }
+                    }
+                return true;
+            }
+
+            template <class A>
+            struct arg_value {
+                using type = void;
+            };
+            template <class T>
+            struct arg_value<std::shared_ptr<storage::data_store<T>>> {
+                using type = T;
+            };
+            template <class T>
+            struct arg_value<global_parameter_t<T>> {
+                using type = T;
+            };
+            template <class A>
+            constexpr bool is_store = std::is_same<typename kind_of<A>::type,
+                field_kind<typename arg_value<A>::type>>::value && !std::is_void<typename arg_value<A>::type>::value;
+
+            template <bool W, class Msses, class... Args>
+            fast_plan plan_fast(std::vector<stage_rt> const &st, Args const &... args) {
+                fast_plan none;
+                using T0 = typename arg_value<std::decay_t<std::tuple_element_t<0, std::tuple<Args...>>>>::type;
+                constexpr bool same_t =
+                    (std::is_same<typename arg_value<std::decay_t<Args>>::type, T0>::value && ...);
+                if constexpr (!same_t || !fast_type<T0> || (W && !std::is_same<T0, float>::value)) {
+                    return none;
+                } else {
+                    using CT = std::conditional_t<W, double, T0>;
+                    constexpr int nargs = sizeof...(Args);
+                    constexpr bool stores[] = {is_store<std::decay_t<Args>>...};
+                    const auto acc = summarize(st);
+                    auto args_t = std::forward_as_tuple(args...);
+                    run_geom g{0, 9, 0, 7, 5, {}};
+                    g.ks.pos[0] = 0, g.ks.pos[1] = g.k_size;
+                    auto run_host = [&](run_ctx &c, host_store &hs) {
+                        host_all_tmps(Msses(), c, hs, g);
+                        host_run<2, W>(Msses(), st, c, args_t, g);
+                    };
+                    int written = -1, nwritten = 0;
+                    for (int a = 0; a < nargs; ++a)
+                        if (stores[a] && acc.count(a) && acc.at(a).written)
+                            written = a, ++nwritten;
+                    if (nwritten != 1)
+                        return none;
+                    if (nargs == 3 && stores[0] && stores[1] && stores[2]) {
+                        // horizontal diffusion: out written at no offset; in read over <-2, 2, -2, 2> (through the
+                        // stages' extents: the accessors read it at up to one point); coeff read at no offset
+                        int in = -1, co = -1;
+                        for (int a = 0; a < 3; ++a) {
+                            if (a == written || !acc.count(a))
+                                continue;
+                            if (ext_is(acc.at(a).ext, 0, 0, 0, 0, 0, 0))
+                                co = a;
+                            else
+                                in = a;
+                        }
+                        if (in < 0 || co < 0 || !ext_is(acc.at(written).ext, 0, 0, 0, 0, 0, 0))
+                            return none;
+                        run_ctx c;
+                        host_store hs;
+                        c.origin[0] = c.origin[1] = c.origin[2] = 0;
+                        for (int a = 0; a < 3; ++a)
+                            c.fields[a] = hs.template make<T0>(g.i_size, g.j_size, g.k_size);
+                        host_fill<T0>(c.fields[in], g, 1, -1, 1);
+                        host_fill<T0>(c.fields[co], g, 2, 0, 0.25);
+                        run_host(c, hs);
+                        if (!hdiff_reproduced<T0, CT>(c.fields[in], c.fields[co], c.fields[written], g))
+                            return none;
+                        fast_plan p;
+                        p.kind = fast_kind::hdiff;
+                        p.role[0] = in, p.role[1] = co, p.role[2] = written;
+                        return p;
+                    }
+                    if (nargs == 6) {
+                        // vertical advection: utens_stage read and written; u_stage read at k-1..k+1; wcon at i..i+1
+                        // and k..k+1; u_pos and utens read at no offset (told apart by the check); dtr_stage a global
+                        int us = -1, wc = -1, dtr = -1, zero[2] = {-1, -1}, nz = 0;
+                        for (int a = 0; a < 6; ++a) {
+                            if (!stores[a]) {
+                                dtr = a;
+                                continue;
+                            }
+                            if (a == written || !acc.count(a))
+                                continue;
+                            ext_rt const &e = acc.at(a).ext;
+                            if (ext_is(e, 0, 0, 0, 0, -1, 1))
+                                us = a;
+                            else if (ext_is(e, 0, 1, 0, 0, 0, 1))
+                                wc = a;
+                            else if (ext_is(e, 0, 0, 0, 0, 0, 0) && nz < 2)
+                                zero[nz++] = a;
+                        }
+                        if (us < 0 || wc < 0 || dtr < 0 || nz != 2 || stores[dtr] ||
+                            !ext_is(acc.at(written).ext, 0, 0, 0, 0, 0, 0))
+                            return none;
+                        T0 dtr_value = 0;
+                        {
+                            int a = 0;
+                            auto pick = [&](auto const &x) {
+                                if constexpr (!is_store<std::decay_t<decltype(x)>>)
+                                    if (a == dtr)
+                                        dtr_value = x.value;
+                                ++a;
+                            };
+                            (pick(args), ...);
+                        }
+                        for (int perm = 0; perm < 2; ++perm) {
+                            const int up = zero[perm], ut = zero[1 - perm];
+                            run_ctx c;
+                            host_store hs;
+                            c.origin[0] = c.origin[1] = c.origin[2] = 0;
+                            for (int a = 0; a < 6; ++a)
+                                if (stores[a])
+                                    c.fields[a] = hs.template make<T0>(g.i_size, g.j_size, g.k_size + 1);
+                            host_fill<T0>(c.fields[written], g, 6, -1, 1);
+                            host_fill<T0>(c.fields[us], g, 3, -1, 1);
+                            host_fill<T0>(c.fields[wc], g, 7, -0.5, 0.5, 1);
+                            host_fill<T0>(c.fields[up], g, 4, -1, 1);
+                            host_fill<T0>(c.fields[ut], g, 5, -1, 1);
+                            field_rt uts0 = hs.template make<T0>(g.i_size, g.j_size, g.k_size + 1);
+                            host_fill<T0>(uts0, g, 6, -1, 1);
+                            run_host(c, hs);
+                            if (vadv_reproduced<T0, CT>(uts0, c.fields[written], c.fields[us], c.fields[wc], c.fields[up],
+                                    c.fields[ut], dtr_value, g)) {
+                                fast_plan p;
+                                p.kind = fast_kind::vadv;
+                                const int r[6] = {written, us, wc, up, ut, dtr};
+                                for (int q = 0; q < 6; ++q)
+                                    p.role[q] = r[q];
+                                return p;
+                            }
+                        }
+                    }
+                    return none;
+                }
+            }
+
+            // runs handed to a hand-written kernel so far (for tests and diagnostics)
+            inline std::atomic<long> &fast_runs() {
+                static std::atomic<long> n{0};
+                return n;
+            }
+
+            // GT355_ENGINE_FAST=0 keeps every spec on the generic engine (read once per process)
+            inline bool fast_path_enabled() {
+                static const bool on = [] {
+                    const char *e = std::getenv("GT355_ENGINE_FAST");
+                    return !(e && std::atoi(e) == 0);
+                }();
+                return on;
+            }
+
+            template <class... Args>
+            bool run_fast(fast_plan const &p, bool W, run_geom const &g, void *stream, Args const &... args) {
+                constexpr int nargs = sizeof...(Args);
+                gt355_field f[nargs > 0 ? nargs : 1] = {};
+                double gval = 0;
+                int a = 0;
+                auto one = [&](auto const &x) {
+                    using X = std::decay_t<decltype(x)>;
+                    if constexpr (is_store<X>) {
+                        const bool w = (p.kind == fast_kind::hdiff && a == p.role[2]) ||
+                                       (p.kind == fast_kind::vadv && a == p.role[0]);
+                        f[a] = x->sid(g.origin(), w);
+                    } else if constexpr (!std::is_void<typename arg_value<X>::type>::value) {
+                        gval = (double)x.value;
+                    }
+                    ++a;
+                };
+                (one(args), ...);
+                const int prec = W ? GT355_WIDEN : GT355_NATIVE;
+                if (p.kind == fast_kind::hdiff) {
+                    check(gt355_hdiff(prec, &f[p.role[2]], &f[p.role[0]], &f[p.role[1]], g.i_size, g.j_size, g.k_size,
+                              stream),
+                        "run (horizontal diffusion spec)");
+                    ++fast_runs();
+                    return true;
+                }
+                if (p.kind == fast_kind::vadv && g.k_size >= 2) {
+                    check(gt355_vadv(prec, &f[p.role[0]], &f[p.role[1]], &f[p.role[2]], &f[p.role[3]], &f[p.role[4]],
+                              gval, g.i_size, g.j_size, g.k_size, stream),
+                        "run (vertical advection spec)");
+                    ++fast_runs();
+                    return true;
+                }
+                return false;
+            }
+        } // namespace engine_
+
         // run(spec, backend, grid, fields...) (stencil/frontend/run.hpp:201-250) for a spec lambda: the spec is called
         // with one placeholder per field argument and returns execute_*().stage(...) chains or a multi_pass of them.
         // The backend tag's precision selects the Deref policy: GT355_NATIVE computes in the storage type;
@@ -1675,8 +2075,15 @@ namespace gt355 {
             engine_::bind_args(c, args_t, st, g, std::index_sequence_for<Args...>());
             if (g.i_size <= 0 || g.j_size <= 0 || g.k_size <= 0)
                 return;
-            engine_::plan_all(msses(), st, box, c);
             void *stream = stream_of(be);
+            if constexpr (N == 1) {
+                // a spec that reproduces a hand-written kernel runs on it (decided once per spec type)
+                static const engine_::fast_plan plan =
+                    engine_::fast_path_enabled() ? engine_::plan_fast<W, msses>(st, args...) : engine_::fast_plan();
+                if (plan.kind != engine_::fast_kind::none && engine_::run_fast(plan, W, g, stream, args...))
+                    return;
+            }
+            engine_::plan_all(msses(), st, box, c);
             std::vector<void *> taken;
             engine_::alloc_all(msses(), c, box, g, stream, taken);
             engine_::launch_all<N + 1, W>(msses(), st, c, args_t, g, (hipStream_t)stream);

@@ -800,6 +800,271 @@ namespace {
         report("hdiff spec with HBM temporaries on two streams at once", ok);
     }
 
+    // ------------------------------------------------------------------------------------- fast path
+    // the reference's hdiff and vadv specs are recognised and run on the hand-written kernels (FP64, native FP32 and
+    // the precision policy); a spec that differs in one operation is not, and runs on the generic engine
+    struct flx_strict { // flx_function with the limiter's comparison >= instead of > (this project's variant)
+        using out = inout_accessor<0>;
+        using in = in_accessor<1, extent<0, 1, 0, 0>>;
+        using lap = in_accessor<2, extent<0, 1, 0, 0>>;
+        using param_list = make_param_list<out, in, lap>;
+        template <typename Evaluation>
+        GT_FUNCTION static void apply(Evaluation eval) {
+            auto res = eval(lap(1, 0)) - eval(lap(0, 0));
+            eval(out()) = res * (eval(in(1, 0)) - eval(in(0, 0))) >= 0 ? 0 : res;
+        }
+    };
+    struct backward_shifted { // u_backward_function with u_pos added instead of subtracted (this project's variant)
+        using utens_stage = inout_accessor<0>;
+        using u_pos = in_accessor<1>;
+        using dtr_stage = in_accessor<2>;
+        using ccol = in_accessor<3>;
+        using dcol = in_accessor<4>;
+        using data_col = inout_accessor<5, extent<0, 0, 0, 0, 0, 1>>;
+        using param_list = make_param_list<utens_stage, u_pos, dtr_stage, ccol, dcol, data_col>;
+        using full_t = reference_specs::vertical_advection_dycore::full_t;
+        template <class Eval>
+        GT_FUNCTION static void apply(Eval &&eval, full_t::modify<0, -1>) {
+            auto data = eval(dcol()) - eval(ccol()) * eval(data_col(0, 0, 1));
+            eval(utens_stage()) = eval(dtr_stage()) * (data + eval(u_pos()));
+            eval(data_col()) = data;
+        }
+        template <class Eval>
+        GT_FUNCTION static void apply(Eval &&eval, full_t::last_level) {
+            eval(utens_stage()) = eval(dtr_stage()) * (eval(dcol()) + eval(u_pos()));
+            eval(data_col()) = eval(dcol());
+        }
+    };
+
+    void fast_path() {
+        if (const char *e = std::getenv("GT355_ENGINE_FAST"); e && std::atoi(e) == 0) {
+            report("fast path switched off (GT355_ENGINE_FAST=0): every spec on the generic engine",
+                engine_::fast_runs() == 0);
+            return;
+        }
+        const int ni = 70, nj = 33, nk = 21;
+        // the reference specs: taken by the hand-written kernels, results equal the oracle
+        {
+            const long before = engine_::fast_runs();
+            horizontal_diffusion(ni, nj, nk);
+            vertical_advection(ni, nj, nk);
+            precision_policy(ni, nj, nk);
+            report("reference hdiff / vadv specs (FP64 and precision policy) ran on the hand-written kernels",
+                engine_::fast_runs() == before + 4);
+        }
+        // one changed operation: the generic engine
+        {
+            const int h = 2;
+            auto in = seeded<double>(ni + 4, nj + 4, nk, h, 42, 1, -1, 1);
+            auto coeff = seeded<double>(ni + 4, nj + 4, nk, h, 42, 2, 0, 0.25);
+            auto b = storage::builder<storage::gpu>.type<double>().dimensions(ni + 4, nj + 4, nk).halos(h, h, 0);
+            auto out = b.build();
+            auto g = halo_grid(ni, nj, nk, h);
+            const long before = engine_::fast_runs();
+            run([](auto in, auto coeff, auto out) {
+                using namespace reference_specs::horizontal_diffusion;
+                GT_DECLARE_TMP(double, lap, flx, fly);
+                return execute_parallel()
+                    .ij_cached(lap, flx, fly)
+                    .stage(lap_function(), lap, in)
+                    .stage(flx_strict(), flx, in, lap)
+                    .stage(fly_function(), fly, in, lap)
+                    .stage(out_function(), out, in, flx, fly, coeff);
+            },
+                gfx950<>(), g, in, coeff, out);
+            const bool generic = engine_::fast_runs() == before;
+            report("hdiff spec with a changed limiter comparison runs on the generic engine", generic);
+        }
+        {
+            const int h = 3;
+            auto utens_stage = seeded<double>(ni + 6, nj + 6, nk, h, 42, 6, -1, 1);
+            auto u_stage = seeded<double>(ni + 6, nj + 6, nk, h, 42, 3, -1, 1);
+            auto u_pos = seeded<double>(ni + 6, nj + 6, nk, h, 42, 4, -1, 1);
+            auto utens = seeded<double>(ni + 6, nj + 6, nk, h, 42, 5, -1, 1);
+            auto wcon = seeded<double>(ni + 6, nj + 6, nk + 1, h, 42, 7, -0.5, 0.5);
+            auto g = halo_grid(ni, nj, nk, h);
+            const long before = engine_::fast_runs();
+            run([](auto utens_stage, auto u_stage, auto wcon, auto u_pos, auto utens, auto dtr_stage) {
+                using namespace reference_specs::vertical_advection_dycore;
+                GT_DECLARE_TMP(double, ccol, dcol, data_col);
+                return multi_pass(execute_forward()
+                                      .k_cached(cache_io_policy::flush(), ccol, dcol)
+                                      .k_cached(cache_io_policy::fill(), u_stage)
+                                      .stage(u_forward_function(), utens_stage, wcon, u_stage, u_pos, utens, dtr_stage,
+                                          ccol, dcol),
+                    execute_backward().k_cached(data_col).stage(
+                        backward_shifted(), utens_stage, u_pos, dtr_stage, ccol, dcol, data_col));
+            },
+                gfx950<>(), g, utens_stage, u_stage, wcon, u_pos, utens, global_parameter(0.15));
+            report("vadv spec with a changed backward stage runs on the generic engine", engine_::fast_runs() == before);
+        }
+    }
+
+    // ------------------------------------------------------------------------------------- boundary functors
+    // the host restatement of the reference's CPU boundary_apply (boundaries/apply.hpp:45-135): every selected
+    // region's halo points, the functor called with host views -- the expected values of the device kernel
+    template <class T>
+    struct host_field {
+        storage::host_view<T> v;
+        T &operator()(unsigned i, unsigned j, unsigned k) const { return v((int)i, (int)j, (int)k); }
+    };
+    template <class BC, class... F>
+    void host_boundary_apply(
+        std::array<halo_descriptor, 3> const &h, BC const &bc, uint32_t mask, F const &... f) {
+        auto lo_n = [&](int d, int dir, int &lo, int &n) {
+            if (dir == 0)
+                lo = h[d].begin(), n = h[d].end() - h[d].begin() + 1;
+            else if (dir < 0)
+                lo = h[d].begin() - h[d].minus(), n = h[d].minus();
+            else
+                lo = h[d].end() + 1, n = h[d].plus();
+        };
+        auto region = [&](auto dir) {
+            using D = decltype(dir);
+            if (!(mask & boundaries::direction_bit(D::i, D::j, D::k)))
+                return;
+            int lo[3], n[3];
+            lo_n(0, D::i, lo[0], n[0]), lo_n(1, D::j, lo[1], n[1]), lo_n(2, D::k, lo[2], n[2]);
+            for (int k = lo[2]; k < lo[2] + n[2]; ++k)
+                for (int j = lo[1]; j < lo[1] + n[1]; ++j)
+                    for (int i = lo[0]; i < lo[0] + n[0]; ++i)
+                        bc(dir, f..., (unsigned)i, (unsigned)j, (unsigned)k);
+        };
+        using namespace boundaries;
+        auto row = [&](auto di) {
+            constexpr sign I = decltype(di)::value;
+            region(direction<I, minus_, minus_>()), region(direction<I, minus_, zero_>());
+            region(direction<I, minus_, plus_>()), region(direction<I, zero_, minus_>());
+            if constexpr (I != zero_)
+                region(direction<I, zero_, zero_>());
+            region(direction<I, zero_, plus_>()), region(direction<I, plus_, minus_>());
+            region(direction<I, plus_, zero_>()), region(direction<I, plus_, plus_>());
+        };
+        row(std::integral_constant<sign, minus_>());
+        row(std::integral_constant<sign, zero_>());
+        row(std::integral_constant<sign, plus_>());
+    }
+
+    template <class T>
+    bool same_everywhere(store<T> const &a, store<T> const &b) {
+        auto va = a->const_host_view(), vb = b->const_host_view();
+        auto len = a->lengths();
+        for (int k = 0; k < len[2]; ++k)
+            for (int j = 0; j < len[1]; ++j)
+                for (int i = 0; i < len[0]; ++i)
+                    if (std::memcmp(&va(i, j, k), &vb(i, j, k), sizeof(T)) != 0) {
+                        std::printf("  mismatch at (%d,%d,%d)\n", i, j, k);
+                        return false;
+                    }
+        return true;
+    }
+
+    // a predicate over directions (boundaries/predicate.hpp): the halo of the j-minus side only
+    struct minus_j_only {
+        template <class Direction>
+        bool operator()(Direction) const {
+            return Direction::j == boundaries::minus_;
+        }
+    };
+
+    void boundary_functors() {
+        using namespace boundaries;
+        // examples/boundaries/boundaries.cpp:121-190: int fields with a 1-point halo in every dimension
+        {
+            const int d1 = 13, d2 = 11, d3 = 9;
+            auto b = storage::builder<storage::gpu>.type<int>().dimensions(d1, d2, d3).halos(1, 1, 1);
+            auto in_s = b.name("in").initializer([](int i, int j, int k) { return i + j + k; }).build();
+            auto out_s = b.name("out").value(0).build();
+            std::array<halo_descriptor, 3> halos = {halo_descriptor(1, 1, 1, d1 - 2, d1),
+                halo_descriptor(1, 1, 1, d2 - 2, d2), halo_descriptor(1, 1, 1, d3 - 2, d3)};
+            using bc_t = reference_specs::examples_boundaries::direction_bc_input<unsigned>;
+            make_boundary<storage::gpu>(halos, bc_t(42)).apply(out_s, in_s);
+            auto in = in_s->const_host_view();
+            auto out = out_s->const_host_view();
+            // the example's own checks
+            bool ok = out(0, 0, 0) == in(1, 1, 1);
+            for (int k = 1; k < d3; ++k)
+                ok = ok && out(0, 0, k) == in(1, 1, k);
+            for (int i = 1; i < d1; ++i)
+                for (int k = 0; k < d3; ++k)
+                    ok = ok && out(i, 0, k) == in(i, 1, k);
+            for (int i = 0; i < d1; ++i)
+                for (int j = 0; j < d2; ++j)
+                    for (int k = 0; k < d3; ++k)
+                        if (((i == 0 || i == d1 - 1) && j > 0) || (j > 0 && (k == 0 || k == d3 - 1)))
+                            ok = ok && out(i, j, k) == 42;
+            // every point against the host apply of the same functor
+            auto ref = b.value(0).build();
+            host_boundary_apply(halos, bc_t(42), all_directions, host_field<int>{ref->host_view()},
+                host_field<int>{in_s->host_view()});
+            ok = ok && same_everywhere(out_s, ref);
+            report("boundary functor of examples/boundaries (int fields, direction overloads)", ok);
+        }
+        // tests/regression/boundary_conditions.cpp:61-143 with halo 3 in every dimension
+        auto bc_case = [&](auto float_tag, int n0, int n1, int n2) {
+            using F = decltype(float_tag);
+            using namespace reference_specs::boundary_conditions;
+            const int h = 3;
+            auto b = storage::builder<storage::gpu>.type<F>().dimensions(n0 + 2 * h, n1 + 2 * h, n2 + 2 * h).halos(h, h, h);
+            auto src = b.initializer([](int i, int j, int k) { return F(i + j + k); }).build();
+            auto dst = b.value(F(0)).build();
+            auto lengths = src->lengths();
+            std::array<halo_descriptor, 3> halos;
+            for (int d = 0; d < 3; ++d)
+                halos[d] = halo_descriptor(h, h, h, lengths[d] - h - 1, lengths[d]);
+            make_boundary<storage::gpu>(halos, direction_bc_input<F>()).apply(src, dst);
+            // verify_result (boundary_conditions.cpp:61-118), as known answers
+            auto sv = src->const_host_view();
+            auto dv = dst->const_host_view();
+            bool ok = true;
+            for (int i = 0; i < lengths[0]; ++i)
+                for (int j = 0; j < lengths[1]; ++j)
+                    for (int k = 0; k < lengths[2]; ++k) {
+                        const bool ii = i >= h && i <= lengths[0] - h - 1, ij = j >= h && j <= lengths[1] - h - 1,
+                                   ik = k >= h && k <= lengths[2] - h - 1;
+                        F e;
+                        if (ii && ij && ik)
+                            e = 0;
+                        else if (i < h && j < h && k < h)
+                            e = factor<F> * corner_value<F>;
+                        else if (i < h && j < h)
+                            e = factor<F> * edge_value<F>;
+                        else if (j < h)
+                            e = factor<F> * face_value<F>;
+                        else
+                            e = factor<F> * sv(i, j, k);
+                        ok = ok && dv(i, j, k) == e;
+                    }
+            auto ref = b.value(F(0)).build();
+            host_boundary_apply(halos, direction_bc_input<F>(), all_directions, host_field<F>{src->host_view()},
+                host_field<F>{ref->host_view()});
+            ok = ok && same_everywhere(dst, ref);
+            char name[112];
+            std::snprintf(name, sizeof name, "boundary_conditions functor (%s, halo 3 in i, j, k) %dx%dx%d",
+                sizeof(F) == 8 ? "double" : "float", n0, n1, n2);
+            report(name, ok);
+        };
+        bc_case(double(), 12, 33, 61);
+        bc_case(float(), 23, 11, 43);
+        // a callable predicate: only the j-minus side's regions are written
+        {
+            const int d1 = 40, d2 = 20, d3 = 10;
+            auto b = storage::builder<storage::gpu>.type<double>().dimensions(d1, d2, d3).halos(2, 2, 1);
+            auto in_s = b.initializer([](int i, int j, int k) { return 0.5 * i + j - 0.25 * k; }).build();
+            auto out_s = b.value(-3.).build();
+            std::array<halo_descriptor, 3> halos = {halo_descriptor(2, 2, 2, d1 - 3, d1),
+                halo_descriptor(2, 2, 2, d2 - 3, d2), halo_descriptor(1, 1, 1, d3 - 2, d3)};
+            using bc_t = reference_specs::examples_boundaries::direction_bc_input<double>;
+            make_boundary<storage::gpu>(halos, bc_t(7.5), minus_j_only()).apply(out_s, in_s);
+            auto ref = b.value(-3.).build();
+            host_boundary_apply(halos, bc_t(7.5), bc_impl::mask_of(minus_j_only()), host_field<double>{ref->host_view()},
+                host_field<double>{in_s->host_view()});
+            bool ok = same_everywhere(out_s, ref) && out_s->const_host_view()(20, 10, 5) == -3. &&
+                      out_s->const_host_view()(20, d2 - 1, 5) == -3.;
+            report("boundary functor with a callable predicate (j-minus regions only)", ok);
+        }
+    }
+
     // --bench: the engine on the reference's hdiff, vadv and tridiagonal specs at 256x256x80 FP64 over 4 rotating
     // field sets (past the 256 MB MALL), mean microseconds per run from one hipEvent pair around `reps` back-to-back
     // runs
@@ -908,6 +1173,8 @@ int main(int argc, char **argv) {
         precision_policy(23, 11, 43);
         precision_policy(256, 256, 80);
         two_streams(130, 70, 24);
+        boundary_functors();
+        fast_path();
     } catch (std::exception const &e) {
         std::printf("[FAIL] exception: %s\n", e.what());
         return 1;

@@ -1015,3 +1015,113 @@ namespace reference_specs {
         };
     } // namespace whole_axis_access
 } // namespace reference_specs
+
+#include <gt355/boundaries.hpp>
+
+namespace reference_specs {
+    // examples/boundaries/boundaries.cpp:63-110 (uint_t spelled unsigned): writes `value` into the halo of the first
+    // field, except the directions (*, -1, *) and (-1, -1, *), which copy the second field from a shifted position
+    namespace examples_boundaries {
+        namespace bd = gt355::boundaries;
+
+        template <typename T>
+        struct direction_bc_input {
+            T value;
+
+            GT_FUNCTION
+            direction_bc_input() : value(1) {}
+
+            GT_FUNCTION
+            direction_bc_input(T v) : value(v) {}
+
+            // This is the primary template and it will be picked when all the other specializations below fail to be
+            // selected.  It is important for debugging to note that, if a needed specialization i missing, this
+            // version will be selected.
+            template <typename Direction, typename DataField0, typename DataField1>
+            GT_FUNCTION void operator()(
+                Direction, DataField0 &data_field0, DataField1 const &, unsigned i, unsigned j, unsigned k) const {
+                data_field0(i, j, k) = value;
+            }
+
+            template <bd::sign I, bd::sign K, typename DataField0, typename DataField1>
+            GT_FUNCTION void operator()(bd::direction<I, bd::minus_, K>,
+                DataField0 &data_field0,
+                DataField1 const &data_field1,
+                unsigned i,
+                unsigned j,
+                unsigned k) const {
+                data_field0(i, j, k) = data_field1(i, j + 1, k);
+            }
+
+            template <bd::sign K, typename DataField0, typename DataField1>
+            GT_FUNCTION void operator()(bd::direction<bd::minus_, bd::minus_, K>,
+                DataField0 &data_field0,
+                DataField1 const &data_field1,
+                unsigned i,
+                unsigned j,
+                unsigned k) const {
+                data_field0(i, j, k) = data_field1(i + 1, j + 1, k);
+            }
+
+            template <typename DataField0, typename DataField1>
+            GT_FUNCTION void operator()(bd::direction<bd::minus_, bd::minus_, bd::minus_>,
+                DataField0 &data_field0,
+                DataField1 const &data_field1,
+                unsigned i,
+                unsigned j,
+                unsigned k) const {
+                data_field0(i, j, k) = data_field1(i + 1, j + 1, k + 1);
+            }
+        };
+    } // namespace examples_boundaries
+
+    // tests/regression/boundary_conditions.cpp:20-59: writes the second field -- a factor times the first field's
+    // value by default, fixed values on the (*, -1, *) faces, the (-1, -1, *) edges and the (-1, -1, -1) corner
+    namespace boundary_conditions {
+        using namespace gt355::boundaries;
+
+        template <class Float>
+        constexpr Float face_value = 88;
+        template <class Float>
+        constexpr Float edge_value = 77777;
+        template <class Float>
+        constexpr Float corner_value = 55555;
+        template <class Float>
+        constexpr Float factor = 2;
+
+        template <class Float>
+        struct direction_bc_input {
+            template <typename Direction, typename DataField0, typename DataField1>
+            GT_FUNCTION void operator()(
+                Direction, DataField0 &data_field0, DataField1 const &data_field1, unsigned i, unsigned j, unsigned k) const {
+                data_field1(i, j, k) = data_field0(i, j, k) * factor<Float>;
+            }
+
+            template <sign I, sign K, typename DataField0, typename DataField1>
+            GT_FUNCTION void operator()(
+                direction<I, minus_, K>, DataField0 &, DataField1 const &data_field1, unsigned i, unsigned j, unsigned k) const {
+                data_field1(i, j, k) = face_value<Float> * factor<Float>;
+            }
+
+            template <sign K, typename DataField0, typename DataField1>
+            GT_FUNCTION void operator()(direction<minus_, minus_, K>,
+                DataField0 &,
+                DataField1 const &data_field1,
+                unsigned i,
+                unsigned j,
+                unsigned k) const {
+                data_field1(i, j, k) = edge_value<Float> * factor<Float>;
+            }
+
+            template <typename DataField0, typename DataField1>
+            GT_FUNCTION void operator()(direction<minus_, minus_, minus_>,
+                DataField0 &,
+                DataField1 const &data_field1,
+                unsigned i,
+                unsigned j,
+                unsigned k) const {
+                data_field1(i, j, k) = corner_value<Float> * factor<Float>;
+            }
+        };
+    } // namespace boundary_conditions
+} // namespace reference_specs
