@@ -26,7 +26,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -1784,13 +1786,21 @@ namespace gt355 {
                 z ^= z >> 31;
                 return a + (b - a) * (double)(z >> 11) * 0x1.0p-53;
             }
+            // quant = 0: continuous U(a, b); quant = q > 1: one of q equally spaced values a .. b, so that equal
+            // neighbours, zero differences and zero products occur (a comparison that differs only at a tie, such as
+            // > against >=, then changes the result)
             template <class T>
-            void host_fill(field_rt const &f, run_geom const &g, int seed, double a, double b, int extra_k = 0) {
+            void host_fill(
+                field_rt const &f, run_geom const &g, int seed, double a, double b, int extra_k = 0, int quant = 0) {
                 uint64_t n = (uint64_t)seed << 32;
                 for (int k = -HOST_HALO; k < g.k_size + extra_k; ++k)
                     for (int j = -HOST_HALO; j < g.j_size + HOST_HALO; ++j)
-                        for (int i = -HOST_HALO; i < g.i_size + HOST_HALO; ++i)
-                            host_at<T>(f, i, j, k) = T(host_uniform(n++, a, b));
+                        for (int i = -HOST_HALO; i < g.i_size + HOST_HALO; ++i) {
+                            double v = host_uniform(n++, a, b);
+                            if (quant > 1)
+                                v = a + (b - a) * std::floor((v - a) / (b - a) * quant) / (quant - 1);
+                            host_at<T>(f, i, j, k) = T(std::min(v, b));
+                        }
             }
 
             enum class fast_kind { none, hdiff, vadv };
@@ -1922,16 +1932,19 @@ namespace gt355 {
                         }
                         if (in < 0 || co < 0 || !ext_is(acc.at(written).ext, 0, 0, 0, 0, 0, 0))
                             return none;
-                        run_ctx c;
-                        host_store hs;
-                        c.origin[0] = c.origin[1] = c.origin[2] = 0;
-                        for (int a = 0; a < 3; ++a)
-                            c.fields[a] = hs.template make<T0>(g.i_size, g.j_size, g.k_size);
-                        host_fill<T0>(c.fields[in], g, 1, -1, 1);
-                        host_fill<T0>(c.fields[co], g, 2, 0, 0.25);
-                        run_host(c, hs);
-                        if (!hdiff_reproduced<T0, CT>(c.fields[in], c.fields[co], c.fields[written], g))
-                            return none;
+                        // continuous inputs, then inputs on a few levels (ties in the limiter's comparison)
+                        for (int quant : {0, 5}) {
+                            run_ctx c;
+                            host_store hs;
+                            c.origin[0] = c.origin[1] = c.origin[2] = 0;
+                            for (int a = 0; a < 3; ++a)
+                                c.fields[a] = hs.template make<T0>(g.i_size, g.j_size, g.k_size);
+                            host_fill<T0>(c.fields[in], g, 1, -1, 1, 0, quant);
+                            host_fill<T0>(c.fields[co], g, 2, 0, 0.25, 0, quant);
+                            run_host(c, hs);
+                            if (!hdiff_reproduced<T0, CT>(c.fields[in], c.fields[co], c.fields[written], g))
+                                return none;
+                        }
                         fast_plan p;
                         p.kind = fast_kind::hdiff;
                         p.role[0] = in, p.role[1] = co, p.role[2] = written;
@@ -1970,24 +1983,28 @@ namespace gt355 {
                             };
                             (pick(args), ...);
                         }
-                        for (int perm = 0; perm < 2; ++perm) {
-                            const int up = zero[perm], ut = zero[1 - perm];
+                        // continuous inputs, then inputs on a few levels (zero differences and zero winds)
+                        auto reproduced = [&](int up, int ut, int quant) {
                             run_ctx c;
                             host_store hs;
                             c.origin[0] = c.origin[1] = c.origin[2] = 0;
                             for (int a = 0; a < 6; ++a)
                                 if (stores[a])
                                     c.fields[a] = hs.template make<T0>(g.i_size, g.j_size, g.k_size + 1);
-                            host_fill<T0>(c.fields[written], g, 6, -1, 1);
-                            host_fill<T0>(c.fields[us], g, 3, -1, 1);
-                            host_fill<T0>(c.fields[wc], g, 7, -0.5, 0.5, 1);
-                            host_fill<T0>(c.fields[up], g, 4, -1, 1);
-                            host_fill<T0>(c.fields[ut], g, 5, -1, 1);
+                            host_fill<T0>(c.fields[written], g, 6, -1, 1, 0, quant);
+                            host_fill<T0>(c.fields[us], g, 3, -1, 1, 0, quant);
+                            host_fill<T0>(c.fields[wc], g, 7, -0.5, 0.5, 1, quant);
+                            host_fill<T0>(c.fields[up], g, 4, -1, 1, 0, quant);
+                            host_fill<T0>(c.fields[ut], g, 5, -1, 1, 0, quant);
                             field_rt uts0 = hs.template make<T0>(g.i_size, g.j_size, g.k_size + 1);
-                            host_fill<T0>(uts0, g, 6, -1, 1);
+                            host_fill<T0>(uts0, g, 6, -1, 1, 0, quant);
                             run_host(c, hs);
-                            if (vadv_reproduced<T0, CT>(uts0, c.fields[written], c.fields[us], c.fields[wc], c.fields[up],
-                                    c.fields[ut], dtr_value, g)) {
+                            return vadv_reproduced<T0, CT>(uts0, c.fields[written], c.fields[us], c.fields[wc],
+                                c.fields[up], c.fields[ut], dtr_value, g);
+                        };
+                        for (int perm = 0; perm < 2; ++perm) {
+                            const int up = zero[perm], ut = zero[1 - perm];
+                            if (reproduced(up, ut, 0) && reproduced(up, ut, 5)) {
                                 fast_plan p;
                                 p.kind = fast_kind::vadv;
                                 const int r[6] = {written, us, wc, up, ut, dtr};
