@@ -98,6 +98,7 @@ SIGNATURES = {
     "gt355_transform_layout": [_I, _P, _P, _I, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
                                ctypes.POINTER(ctypes.c_int64), _P],
     "gt355_vadv": [_I, _FP, _FP, _FP, _FP, _FP, ctypes.c_double, _I, _I, _I, _P],
+    "gt355_vadv_noalloc": [_I, _FP, _FP, _FP, _FP, _FP, ctypes.c_double, _I, _I, _I, _P],
     "gt355_simple_hori_diff": [_I, _FP, _FP, _FP, _FP, _FP, _I, _I, _I, _P],
     "gt355_prepare_tracers": [_I, _FP, _FP, _I, _FP, _I, _I, _I, _P],
     "gt355_copy_fields": [_FP, _FP, _I, _I, _I, _I, _P],

@@ -57,6 +57,8 @@ namespace gt355 {
     int scratch_for(hipStream_t stream, size_t bytes, void **out);
     // Frees the pools and retired buffers (gt355_finalize).
     void release_pools();
+    // Resolves, on the current device, the device address of gt355_vadv_noalloc's static scratch (vadv.hip).
+    int noalloc_setup();
 
     // After a launch: report a launch-configuration error as a status; in the debug launch mode also wait for the
     // kernel and report its failure here.

@@ -132,9 +132,12 @@ int gt355_init(int device) {
     GT355_HIP(hipGetDeviceCount(&n));
     GT355_CHECK(device >= 0 && device < n, "gt355_init: device %d out of range (%d devices)", device, n);
     GT355_HIP(hipSetDevice(device));
-    std::lock_guard<std::mutex> lock(g_mutex);
-    g_device = device;
-    return GT355_OK;
+    {
+        std::lock_guard<std::mutex> lock(g_mutex);
+        g_device = device;
+    }
+    // the no-allocation entry point's static scratch: its address on this device, resolved once
+    return noalloc_setup();
 }
 
 int gt355_finalize(void) {

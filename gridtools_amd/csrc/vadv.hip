@@ -14,6 +14,7 @@
 // Arithmetic follows the functors operation by operation (built with -ffp-contract=off).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <mutex>
 
 #include "common.hpp"
@@ -28,7 +29,7 @@ namespace gt355 {
         __global__ void __launch_bounds__(256) vadv_scratch_kernel(ST *__restrict__ uts_f,
             const ST *__restrict__ us_f, const ST *__restrict__ wc_f, const ST *__restrict__ up_f,
             const ST *__restrict__ ut_f, CT dtr, int ni, int nj, int nk, int64_t sj, int64_t sk, int64_t wsj,
-            int64_t wsk, CT *__restrict__ scratch, int64_t plane) {
+            int64_t wsk, CT *__restrict__ cd_buf, int64_t plane) {
             const int i = blockIdx.x * 64 + (threadIdx.x & 63);
             const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
             if (i >= ni || j >= nj)
@@ -36,8 +37,8 @@ namespace gt355 {
             const int64_t col = (int64_t)j * sj + i;
             const int64_t wcol = (int64_t)j * wsj + i;
             const int64_t scol = (int64_t)j * ni + i; // scratch column index, plane = ni*nj
-            CT *__restrict__ sc = scratch + scol;
-            CT *__restrict__ sd = scratch + (int64_t)(nk - 1) * plane + scol;
+            CT *__restrict__ sc = cd_buf + scol;
+            CT *__restrict__ sd = cd_buf + (int64_t)(nk - 1) * plane + scol;
 
             // k = 0
             CT us_c = CT(us_f[col]);
@@ -355,9 +356,13 @@ namespace gt355 {
             return 80;
         }
 
+        // ext_buf == nullptr: the deep-column path takes its ccol/dcol buffer from the per-stream scratch pool.
+        // Otherwise ext_buf (ext_bytes long) is used and nothing is allocated: the rows are processed in chunks whose
+        // ccol/dcol fit it (columns are independent, so a chunked launch computes the same bits).
         template <class ST, class CT>
         int launch_vadv(const gt355_field *uts, const gt355_field *us, const gt355_field *wc, const gt355_field *up,
-            const gt355_field *ut, double dtr_stage, int ni, int nj, int nk, hipStream_t stream) {
+            const gt355_field *ut, double dtr_stage, int ni, int nj, int nk, hipStream_t stream,
+            void *ext_buf = nullptr, size_t ext_bytes = 0) {
             int64_t sj = uts->stride[1], sk = uts->stride[2];
             GT355_CHECK(us->stride[1] == sj && us->stride[2] == sk && up->stride[1] == sj && up->stride[2] == sk &&
                             ut->stride[1] == sj && ut->stride[2] == sk,
@@ -444,28 +449,77 @@ namespace gt355 {
                 GT355_VADV_ONCHIP(80)
 #undef GT355_VADV_ONCHIP
             }
-            int64_t plane = (int64_t)ni * nj;
-            size_t bytes = sizeof(CT) * (size_t)plane * (size_t)(2 * nk - 1);
-            void *scratch = nullptr;
-            int rc = scratch_for(stream, bytes, &scratch);
-            if (rc)
-                return rc;
-            CT dtr = dtr_c;
-            dim3 grid((ni + 63) / 64, (nj + 3) / 4);
-            GT355_CHECK(grid.y <= 65535, "gt355_vadv: nj too large");
-            vadv_scratch_kernel<ST, CT><<<grid, 256, 0, stream>>>((ST *)uts->origin, (const ST *)us->origin,
-                (const ST *)wc->origin, (const ST *)up->origin, (const ST *)ut->origin, dtr, ni, nj, nk, sj, sk,
-                wc->stride[1], wc->stride[2], (CT *)scratch, plane);
-            return check_launch("gt355_vadv");
+            // ccol/dcol of levels 0..nk-2 for every column of a row
+            const size_t row_bytes = sizeof(CT) * (size_t)ni * (size_t)(2 * nk - 1);
+            int rows = nj;
+            void *buf = ext_buf;
+            if (ext_buf) {
+                GT355_CHECK(row_bytes <= ext_bytes,
+                    "gt355_vadv_noalloc: ni = %d, nk = %d needs %zu bytes of ccol/dcol for one row of columns, more "
+                    "than the %zu-byte static scratch holds",
+                    ni, nk, row_bytes, ext_bytes);
+                rows = (int)std::min<size_t>((size_t)nj, ext_bytes / row_bytes);
+            } else {
+                int rc = scratch_for(stream, row_bytes * (size_t)nj, &buf);
+                if (rc)
+                    return rc;
+            }
+            GT355_CHECK((std::min(rows, nj) + 3) / 4 <= 65535, "gt355_vadv: nj too large");
+            for (int j0 = 0; j0 < nj; j0 += rows) {
+                const int nr = std::min(rows, nj - j0);
+                dim3 grid((ni + 63) / 64, (nr + 3) / 4);
+                vadv_scratch_kernel<ST, CT><<<grid, 256, 0, stream>>>((ST *)uts->origin + j0 * sj,
+                    (const ST *)us->origin + j0 * sj, (const ST *)wc->origin + j0 * wc->stride[1],
+                    (const ST *)up->origin + j0 * sj, (const ST *)ut->origin + j0 * sj, dtr_c, ni, nr, nk, sj, sk,
+                    wc->stride[1], wc->stride[2], (CT *)buf, (int64_t)ni * nr);
+                int rc = check_launch("gt355_vadv");
+                if (rc)
+                    return rc;
+            }
+            return GT355_OK;
+        }
+        // The no-allocation entry point's ccol/dcol buffer: a file-scope device array, one copy per device for the
+        // life of the process.  Host code never uses its name as an address (on the host it names a stand-in): the
+        // device address comes from hipGetSymbolAddress, once per device, with that device current.
+        constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(64) << 20;
+        static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+        constexpr int NOALLOC_MAX_DEVICES = 64;
+        std::mutex g_noalloc_mutex;
+        void *g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+        // The current device's address of `scratch`.
+        int noalloc_scratch(void **out) {
+            int dev = 0;
+            GT355_HIP(hipGetDevice(&dev));
+            GT355_CHECK(dev >= 0 && dev < NOALLOC_MAX_DEVICES, "gt355_vadv_noalloc: device %d", dev);
+            std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+            if (!g_noalloc_ptr[dev]) {
+                void *p = nullptr;
+                hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+                if (e != hipSuccess)
+                    return hip_fail(e, "gt355_vadv_noalloc: hipGetSymbolAddress(scratch)");
+                GT355_CHECK(p != nullptr, "gt355_vadv_noalloc: no device address for the static scratch");
+                g_noalloc_ptr[dev] = p;
+            }
+            *out = g_noalloc_ptr[dev];
+            return GT355_OK;
         }
     } // namespace
+
+    // Called by gt355_init with the selected device current: resolves the static scratch's address there.
+    int noalloc_setup() {
+        void *p = nullptr;
+        return noalloc_scratch(&p);
+    }
 } // namespace gt355
 
 using namespace gt355;
 
-extern "C" int gt355_vadv(int precision, const gt355_field *utens_stage, const gt355_field *u_stage,
-    const gt355_field *wcon, const gt355_field *u_pos, const gt355_field *utens, double dtr_stage, int ni, int nj,
-    int nk, void *stream) {
+namespace {
+    int vadv_entry(int precision, const gt355_field *utens_stage, const gt355_field *u_stage, const gt355_field *wcon,
+        const gt355_field *u_pos, const gt355_field *utens, double dtr_stage, int ni, int nj, int nk, void *stream,
+        bool noalloc) {
     GT355_CHECK(ni >= 0 && nj >= 0, "gt355_vadv: negative size");
     GT355_CHECK(nk >= 2, "gt355_vadv: nk = %d, the spec needs at least 2 levels (first and last level)", nk);
     GT355_CHECK(utens_stage && u_stage && wcon && u_pos && utens, "gt355_vadv: null field");
@@ -487,9 +541,27 @@ extern "C" int gt355_vadv(int precision, const gt355_field *utens_stage, const g
     if ((rc = check_field(utens, "utens", dtype, 0, ni, 0, nj, 0, nk)))
         return rc;
     hipStream_t s = (hipStream_t)stream;
+    void *buf = nullptr;
+    const size_t bytes = noalloc ? NOALLOC_SCRATCH_BYTES : 0;
+    if (noalloc && (rc = noalloc_scratch(&buf)))
+        return rc;
     if (dtype == GT355_F64)
-        return launch_vadv<double, double>(utens_stage, u_stage, wcon, u_pos, utens, dtr_stage, ni, nj, nk, s);
+        return launch_vadv<double, double>(
+            utens_stage, u_stage, wcon, u_pos, utens, dtr_stage, ni, nj, nk, s, buf, bytes);
     if (precision == GT355_WIDEN)
-        return launch_vadv<float, double>(utens_stage, u_stage, wcon, u_pos, utens, dtr_stage, ni, nj, nk, s);
-    return launch_vadv<float, float>(utens_stage, u_stage, wcon, u_pos, utens, dtr_stage, ni, nj, nk, s);
+        return launch_vadv<float, double>(utens_stage, u_stage, wcon, u_pos, utens, dtr_stage, ni, nj, nk, s, buf, bytes);
+    return launch_vadv<float, float>(utens_stage, u_stage, wcon, u_pos, utens, dtr_stage, ni, nj, nk, s, buf, bytes);
+    }
+} // namespace
+
+extern "C" int gt355_vadv(int precision, const gt355_field *utens_stage, const gt355_field *u_stage,
+    const gt355_field *wcon, const gt355_field *u_pos, const gt355_field *utens, double dtr_stage, int ni, int nj,
+    int nk, void *stream) {
+    return vadv_entry(precision, utens_stage, u_stage, wcon, u_pos, utens, dtr_stage, ni, nj, nk, stream, false);
+}
+
+extern "C" int gt355_vadv_noalloc(int precision, const gt355_field *utens_stage, const gt355_field *u_stage,
+    const gt355_field *wcon, const gt355_field *u_pos, const gt355_field *utens, double dtr_stage, int ni, int nj,
+    int nk, void *stream) {
+    return vadv_entry(precision, utens_stage, u_stage, wcon, u_pos, utens, dtr_stage, ni, nj, nk, stream, true);
 }

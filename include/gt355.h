@@ -143,6 +143,18 @@ int gt355_laplacian(int precision, const gt355_field *out, const gt355_field *in
 int gt355_vadv(int precision, const gt355_field *utens_stage, const gt355_field *u_stage, const gt355_field *wcon,
     const gt355_field *u_pos, const gt355_field *utens, double dtr_stage, int ni, int nj, int nk, void *stream);
 
+/* gt355_vadv without any allocation.  No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels and results as gt355_vadv.  Where gt355_vadv takes ccol/dcol scratch from its per-stream pool
+ * (nk > 80, or a field span beyond 4 GiB), this entry point uses a static device array of 64 MiB that each device
+ * holds for the life of the process, processing the rows in chunks whose ccol/dcol fit it; a single row needing more
+ * than 64 MiB (sizeof(compute type) * ni * (2 nk - 1) bytes) is rejected with GT355_EINVAL before any launch.  It
+ * calls no hipMalloc / hipFree.  Every call on one device shares that device's array: calls that overlap in time on
+ * one device (two streams, two threads) race and must be ordered by the caller. */
+int gt355_vadv_noalloc(int precision, const gt355_field *utens_stage, const gt355_field *u_stage,
+    const gt355_field *wcon, const gt355_field *u_pos, const gt355_field *utens, double dtr_stage, int ni, int nj,
+    int nk, void *stream);
+
 /* hdiff -> vadv composition (one run() with multi_pass(hdiff MSS, forward MSS, backward MSS); the reference gpu
  * backend splits it into the same two launches, stencil/gpu/entry_point.hpp:96-124): u_stage = hdiff(in, coeff),
  * then vadv(utens_stage, u_stage, wcon, u_pos, utens).  Every field is validated before the first launch. */
