@@ -88,6 +88,14 @@ namespace gt355 {
     // offsets below 4 GiB (launchers check the field span).  Data format 32 (word 3 = 0x00020000) enables
     // untyped dword access; num_records 0xFFFFFFFF disables the range clamp.
     using rsrc_t = __amdgpu_buffer_rsrc_t;
+    // An address the caller knows to be the same on every lane, moved to scalar registers (a descriptor built from
+    // an address the compiler cannot prove uniform would be used through a loop over the lanes' values).
+    __device__ __forceinline__ const void *wave_uniform(const void *p) {
+        const uint64_t v = (uint64_t)(uintptr_t)p;
+        const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+        const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+        return (const void *)(uintptr_t)(((uint64_t)hi << 32) | lo);
+    }
     __device__ __forceinline__ rsrc_t make_rsrc(const void *base) {
         return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)0xFFFFFFFF, 0x00020000);
     }
@@ -96,6 +104,10 @@ namespace gt355 {
     // lines that are re-read.  Measured on vadv FP64 256x256x80 (five input streams and one output stream):
     // 54 us with the default policy, 41 us with non-temporal loads and stores.
     constexpr int CPOL_NT = 2;
+    // Cache policy of the output stores: non-temporal, like the input streams.  System-scope stores (sc0 sc1,
+    // written through the L2) shorten a plain 16-byte-per-lane FP32 copy of 42 MB from 7.95 to 7.27 us, but were
+    // measured slower on the stencils (hdiff FP64 256x256x80 28.5 us against 22.6 us, vadv FP64 41.7 against 40.2 us).
+    constexpr int CPOL_OUT = CPOL_NT;
     template <class T, int CPOL = 0>
     __device__ __forceinline__ T bload(rsrc_t r, uint32_t voff, uint32_t soff) {
         if constexpr (sizeof(T) == 4)
@@ -114,6 +126,29 @@ namespace gt355 {
             return __builtin_bit_cast(two<T>, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, CPOL));
         else
             return __builtin_bit_cast(two<T>, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, CPOL));
+    }
+    // two consecutive elements (columns i and i+1) in one store
+    template <class T, int CPOL = 0>
+    __device__ __forceinline__ void bstore_pair(rsrc_t r, T a, T b, uint32_t voff, uint32_t soff) {
+        const two<T> v{a, b};
+        if constexpr (sizeof(T) == 4)
+            __builtin_amdgcn_raw_buffer_store_b64(
+                __builtin_bit_cast(decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0)), v), r, voff, soff, CPOL);
+        else
+            __builtin_amdgcn_raw_buffer_store_b128(
+                __builtin_bit_cast(decltype(__builtin_amdgcn_raw_buffer_load_b128(r, 0, 0, 0)), v), r, voff, soff,
+                CPOL);
+    }
+    // 16 bytes per lane (four FP32 elements, i to i+3) in one request
+    typedef float f32x4 __attribute__((ext_vector_type(4)));
+    template <int CPOL = 0>
+    __device__ __forceinline__ f32x4 bload4(rsrc_t r, uint32_t voff, uint32_t soff) {
+        return __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, CPOL));
+    }
+    template <int CPOL = 0>
+    __device__ __forceinline__ void bstore4(rsrc_t r, f32x4 v, uint32_t voff, uint32_t soff) {
+        __builtin_amdgcn_raw_buffer_store_b128(
+            __builtin_bit_cast(decltype(__builtin_amdgcn_raw_buffer_load_b128(r, 0, 0, 0)), v), r, voff, soff, CPOL);
     }
     template <class T, int CPOL = 0>
     __device__ __forceinline__ void bstore(rsrc_t r, T v, uint32_t voff, uint32_t soff) {

@@ -64,7 +64,12 @@ namespace gt355 {
         CT lap1 = 0, lap0 = 0;           // lap of rows R-1 and R-2 (own column)
         CT sl1 = 0, sl0 = 0;             // side laplacian of rows R-1 and R-2 (i0-1 on lane 0, i0+64 on lane 63)
         CT fly_m = 0;                    // fly of row R-3
-        const PO orow = out + (int64_t)k * osk + (i0 + lane);
+        // output rows through buffer descriptors based at the strip's first point (wave-uniform): the row offset is
+        // a scalar, the lane's column a vector offset, and the stores take the output cache policy
+        const int64_t o0 = (int64_t)k * osk + (int64_t)j0 * osj + i0;
+        const rsrc_t r_o = make_rsrc(wave_uniform((const void *)(out + o0)));
+        const rsrc_t r_l = make_rsrc(wave_uniform((const void *)(LAP ? lap + o0 : out + o0)));
+        const uint32_t ob = (uint32_t)(lane * sizeof(ST));
         const bool col_ok = i0 + lane < ni;
 #pragma clang loop unroll(full)
         for (int t = 0; t < NR; ++t) {
@@ -99,9 +104,10 @@ namespace gt355 {
                 fly_m = fly_c;
                 const int r_out = j0 + t - 4;
                 if (col_ok && r_out < nj) {
-                    __builtin_nontemporal_store(static_cast<ST>(res), orow + (int64_t)r_out * osj);
+                    const uint32_t rb = (uint32_t)((t - 4) * osj * (int64_t)sizeof(ST));
+                    bstore<ST, CPOL_OUT>(r_o, static_cast<ST>(res), ob, rb);
                     if constexpr (LAP)
-                        __builtin_nontemporal_store(static_cast<ST>(lap0), lap + (orow - out) + (int64_t)r_out * osj);
+                        bstore<ST, CPOL_OUT>(r_l, static_cast<ST>(lap0), ob, rb);
                 }
             } else if (t == 3) {
                 // fly of row j0-1, the fly[j-1] of the first output row
@@ -137,7 +143,6 @@ namespace gt355 {
         int64_t csk, int i0, int j0, int k) {
         typedef ST vec2 __attribute__((ext_vector_type(2)));
         // vector pointers in the address space of the field pointers
-        using VP = decltype((vec2 *)(PO)nullptr);
         using VCP = decltype((const vec2 *)(PI)nullptr);
         constexpr int NR = JB + 4; // in rows j0-2 .. j0+JB+1
         const int lane = threadIdx.x & 63;
@@ -175,7 +180,10 @@ namespace gt355 {
         CT la1 = 0, la0 = 0, lb1 = 0, lb0 = 0; // lap of rows R-1, R-2 at a and b
         CT sl1 = 0, sl0 = 0;                  // side laplacian (i0-1 on lane 0, i0+128 on lane 63)
         CT flya_m = 0, flyb_m = 0;            // fly of row R-3
-        const PO orow = out + (int64_t)k * osk + ca;
+        const int64_t o0 = (int64_t)k * osk + (int64_t)j0 * osj + i0;
+        const rsrc_t r_o = make_rsrc(wave_uniform((const void *)(out + o0)));
+        const rsrc_t r_l = make_rsrc(wave_uniform((const void *)(LAP ? lap + o0 : out + o0)));
+        const uint32_t ob = (uint32_t)(2 * lane * sizeof(ST));
         const bool col_ok = ca < ni; // ni even: b = a+1 is inside too
 #pragma clang loop unroll(full)
         for (int t = 0; t < NR; ++t) {
@@ -215,16 +223,10 @@ namespace gt355 {
                 flyb_m = fly_b;
                 const int r_out = j0 + t - 4;
                 if (col_ok && r_out < nj) {
-                    vec2 o;
-                    o.x = static_cast<ST>(ra);
-                    o.y = static_cast<ST>(rb);
-                    __builtin_nontemporal_store(o, (VP)(orow + (int64_t)r_out * osj));
-                    if constexpr (LAP) {
-                        vec2 l;
-                        l.x = static_cast<ST>(la0);
-                        l.y = static_cast<ST>(lb0);
-                        __builtin_nontemporal_store(l, (VP)(lap + (orow - out) + (int64_t)r_out * osj));
-                    }
+                    const uint32_t rbo = (uint32_t)((t - 4) * osj * (int64_t)sizeof(ST));
+                    bstore_pair<ST, CPOL_OUT>(r_o, static_cast<ST>(ra), static_cast<ST>(rb), ob, rbo);
+                    if constexpr (LAP)
+                        bstore_pair<ST, CPOL_OUT>(r_l, static_cast<ST>(la0), static_cast<ST>(lb0), ob, rbo);
                 }
             } else if (t == 3) {
                 // fly of row j0-1, the fly[j-1] of the first output row

@@ -203,12 +203,23 @@ namespace gt355 {
                 for (int s = 0; s < D; ++s)
                     r_up[(NK - 1 - s) % D] = bload<ST, CPOL_NT>(rs_up, colb, lvl(NK - 1 - s) * sk_o);
             }
+            // UPOS_LDS: the parked u_pos is read UA levels ahead of its use, so the backward chain does not wait for
+            // each LDS read (with one wavefront per SIMD nothing else would hide its latency)
+            constexpr int UA = NK < 4 ? NK : 4;
+            ST l_up[UA];
+            if constexpr (UPOS_LDS) {
+#pragma clang loop unroll(full)
+                for (int s = 0; s < UA; ++s)
+                    l_up[(NK - 1 - s) % UA] = s_up[NK - 1 - s][threadIdx.x];
+            }
             CT data = CT(0);
 #pragma clang loop unroll(full)
             for (int k = NK - 1; k >= 0; --k) {
                 CT v_up;
                 if constexpr (UPOS_LDS) {
-                    v_up = CT(s_up[k][threadIdx.x]);
+                    v_up = CT(l_up[k % UA]);
+                    if (k - UA >= 0)
+                        l_up[k % UA] = s_up[k - UA][threadIdx.x];
                 } else {
                     v_up = CT(r_up[k % D]);
                     pin();
@@ -223,7 +234,7 @@ namespace gt355 {
                 // Unconditional store to the clamped level: a step above the top writes a value that the step at
                 // the top overwrites later in program order, and lanes past ni compute column ni-1 and write its
                 // bits again.  No branch, so the load counter stays exact.
-                bstore<ST, CPOL_NT>(r_uts, ST(out), colb, lvl(k) * sk_o);
+                bstore<ST, CPOL_OUT>(r_uts, ST(out), colb, lvl(k) * sk_o);
                 __builtin_amdgcn_sched_barrier(SBM);
             }
         }
@@ -333,20 +344,270 @@ namespace gt355 {
                     pin();
                     prologue(ncolb, nwcolb);
                 }
+                // the parked u_pos read four levels ahead of its use (see vadv_onchip_kernel)
+                ST l_up[4];
+#pragma clang loop unroll(full)
+                for (int s = 0; s < 4; ++s)
+                    l_up[(NK - 1 - s) % 4] = s_up[NK - 1 - s][threadIdx.x];
                 CT data = CT(0);
 #pragma clang loop unroll(full)
                 for (int k = NK - 1; k >= 0; --k) {
-                    const CT v_up = CT(s_up[k][threadIdx.x]);
+                    const CT v_up = CT(l_up[k % 4]);
+                    if (k - 4 >= 0)
+                        l_up[k % 4] = s_up[k - 4][threadIdx.x];
                     // u_backward_function (vertical_advection_dycore.cpp:110-121)
                     const CT dt = d[k] - c[k] * data;
                     const CT nd = k == top ? d[k] : dt;
                     data = nd;
-                    bstore<ST, CPOL_NT>(r_uts, ST(dtr * (nd - v_up)), colb, lvl(k) * sk_o);
+                    bstore<ST, CPOL_OUT>(r_uts, ST(dtr * (nd - v_up)), colb, lvl(k) * sk_o);
                     __builtin_amdgcn_sched_barrier(0);
                 }
                 colb = ncolb;
                 wcolb = nwcolb;
             }
+        }
+
+        // FP32-storage form of the persistent sweep with 16-byte requests.  A 64-column row of one FP32 field is
+        // 256 bytes, so one dwordx4 load instruction of a wavefront fetches four rows: lanes 16g..16g+15 read row g,
+        // four columns each.  Per group of four levels a wavefront issues six loads (utens_stage, u_pos, utens at
+        // levels 4G..4G+3; u_stage and wcon at 4G+1..4G+4, one level ahead as the forward step needs them; wcon's
+        // column 64, the i+1 neighbour of the strip's last column, as one dword per row), against 20 single-dword
+        // loads with one row per instruction.  The rows pass through LDS (a per-wavefront staging area; the u_pos
+        // rows land in the column's u_pos park, which the backward sweep reads) and each lane reads its own column
+        // back.  Outputs go the same way in reverse: four levels of a lane's column into LDS, then one dwordx4
+        // store of four rows.  Fewer, larger requests keep more bytes in flight under the wavefront's limit of
+        // outstanding memory instructions.  Only one wavefront uses each LDS area, whose operations it executes in
+        // order, so no barrier is needed; an empty asm with a memory clobber keeps the compiler from reordering the
+        // LDS writes and reads of one group.  The arithmetic is vadv_onchip_kernel's, operation by operation.
+        // Requirements checked by the launcher: ni % 4 == 0 (a lane's four output columns are all inside the domain
+        // or, past ni, clamped onto the last four, which it writes with the same bits as their owner); 16-byte
+        // aligned rows and level planes; every row read whole (columns up to the strip's end, and column 64 of wcon)
+        // inside each field's allocation.
+        template <class CT, int NK, int DG>
+        __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
+        vadv_wide_kernel(float *__restrict__ uts_f, const float *__restrict__ us_f, const float *__restrict__ wc_f,
+            const float *__restrict__ up_f, const float *__restrict__ ut_f, CT dtr, int ni, int nk, int64_t sj,
+            int64_t sk, int64_t wsj, int64_t wsk, int strips_i, int nitems, int nxcd) {
+            constexpr int NG = NK / 4;
+            static_assert(NK % 4 == 0 && DG >= 1 && DG <= NG, "level groups");
+            constexpr int RS = 68; // staging row: 64 columns + wcon's column 64, padded to 16 bytes
+            const int lane = threadIdx.x, g = lane >> 4, q = lane & 15;
+            const int top = nk - 1;
+            const rsrc_t r_us = make_rsrc(us_f), r_wc = make_rsrc(wc_f), rs_up = make_rsrc(up_f),
+                         r_ut = make_rsrc(ut_f), r_uts = make_rsrc(uts_f);
+            uint32_t skb = (uint32_t)(sk * 4), wskb = (uint32_t)(wsk * 4);
+            auto pin = [&]() {
+                asm volatile("" : "+s"(skb));
+                asm volatile("" : "+s"(wskb));
+            };
+            auto lds_order = []() { asm volatile("" ::: "memory"); };
+            __shared__ __align__(16) float s_up[NK * 64];
+            __shared__ __align__(16) float s_us[4 * RS], s_wc[4 * RS], s_uts[4 * RS], s_ut[4 * RS];
+            __shared__ __align__(16) float s_out[4 * 64];
+
+            struct group {
+                f32x4 uts, up, ut, us, wc;
+                float wx;
+            };
+            group ring[DG];
+            // per strip: row-load offsets of this lane's four columns (lb, wlb), wcon's column 64 (wxb), the lane's
+            // own column (colb; lanes past ni take column ni-1 through p) and its output chunk (sb, sc)
+            uint32_t lb = 0, wlb = 0, wxb = 0, colb = 0, sb = 0;
+            int p = 0, sc = 0;
+            auto offsets = [&](int it) {
+                const int si = it % strips_i, j = it / strips_i, i0 = si * 64;
+                lb = (uint32_t)(((int64_t)j * sj + i0 + 4 * q) * 4);
+                wlb = (uint32_t)(((int64_t)j * wsj + i0 + 4 * q) * 4);
+                wxb = (uint32_t)(((int64_t)j * wsj + i0 + 64) * 4);
+                p = min(lane, ni - 1 - i0);
+                colb = (uint32_t)(((int64_t)j * sj + i0 + p) * 4);
+                sc = min(4 * q, ni - 4 - i0);
+                sb = (uint32_t)(((int64_t)j * sj + i0 + sc) * 4);
+            };
+            auto issue = [&](group &r, int G) {
+                const uint32_t lB = (uint32_t)min(4 * G + g, top), lA = (uint32_t)min(4 * G + 1 + g, top);
+                r.uts = bload4<CPOL_NT>(r_uts, lb + lB * skb, 0);
+                r.up = bload4<CPOL_NT>(rs_up, lb + lB * skb, 0);
+                r.ut = bload4<CPOL_NT>(r_ut, lb + lB * skb, 0);
+                r.us = bload4<CPOL_NT>(r_us, lb + lA * skb, 0);
+                r.wc = bload4<CPOL_NT>(r_wc, wlb + lA * wskb, 0);
+                r.wx = bload<float, CPOL_NT>(r_wc, wxb + lA * wskb, 0);
+            };
+            auto stage = [&](group const &r, int G) {
+                *(f32x4 *)&s_uts[g * RS + 4 * q] = r.uts;
+                *(f32x4 *)&s_ut[g * RS + 4 * q] = r.ut;
+                *(f32x4 *)&s_up[G * 256 + 4 * lane] = r.up; // rows 4G..4G+3 of the park are contiguous
+                *(f32x4 *)&s_us[g * RS + 4 * q] = r.us;
+                *(f32x4 *)&s_wc[g * RS + 4 * q] = r.wc;
+                s_wc[g * RS + 64] = r.wx; // the 16 lanes of a row write the same value
+            };
+            float us0 = 0.f; // u_stage of level 0 at the lane's column
+            auto prologue = [&]() {
+                us0 = bload<float, CPOL_NT>(r_us, colb, 0);
+#pragma clang loop unroll(full)
+                for (int G = 0; G < DG; ++G)
+                    issue(ring[G], G);
+            };
+
+            // a group's values of the lane's column, read from LDS one group ahead of their use so that the LDS
+            // latency is hidden behind the previous group's arithmetic
+            struct vals {
+                float uts[4], up[4], ut[4], us[4], wi[4], wip[4];
+            };
+            auto lds_vals = [&](vals &v, int G) {
+#pragma clang loop unroll(full)
+                for (int r = 0; r < 4; ++r) {
+                    v.uts[r] = s_uts[r * RS + p];
+                    v.up[r] = s_up[(4 * G + r) * 64 + p];
+                    v.ut[r] = s_ut[r * RS + p];
+                    v.us[r] = s_us[r * RS + p];
+                    v.wi[r] = s_wc[r * RS + p];
+                    v.wip[r] = s_wc[r * RS + p + 1];
+                }
+            };
+
+            int it = xcd_remap(blockIdx.x, gridDim.x, nxcd);
+            if (it >= nitems)
+                return; // a whole wavefront leaves; there is no barrier in this kernel
+            offsets(it);
+            prologue();
+            for (; it < nitems; it += gridDim.x) {
+                CT c[NK], d[NK];
+                CT us_m = CT(0), us_c = CT(us0), w_c_i = CT(0), w_c_ip = CT(0);
+                vals a, b;
+                // group 0: staged and read before the sweep; its ring slot refilled with group DG
+                lds_order();
+                stage(ring[0], 0);
+                pin();
+                if (DG < NG)
+                    issue(ring[0], DG);
+                lds_order();
+                lds_vals(a, 0);
+#pragma clang loop unroll(full)
+                for (int G = 0; G < NG; ++G) {
+                    // stage group G+1 (the staging rows of group G were read into `a` before, in program order),
+                    // refill its ring slot with group G+1+DG and read its values, all in flight while group G computes
+                    if (G + 1 < NG) {
+                        lds_order();
+                        stage(ring[(G + 1) % DG], G + 1);
+                        pin();
+                        if (G + 1 + DG < NG)
+                            issue(ring[(G + 1) % DG], G + 1 + DG);
+                        lds_order();
+                        lds_vals(b, G + 1);
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma clang loop unroll(full)
+                    for (int r = 0; r < 4; ++r) {
+                        const int k = 4 * G + r;
+                        const CT v_uts = CT(a.uts[r]), v_up = CT(a.up[r]), v_ut = CT(a.ut[r]);
+                        // top level: the k+1 wind taken as zero reproduces last_level exactly (see
+                        // vadv_onchip_kernel)
+                        const bool at_top = k == top;
+                        const CT us_p = at_top ? us_c : CT(a.us[r]);
+                        const CT w_p_i = at_top ? CT(0) : CT(a.wi[r]);
+                        const CT w_p_ip = at_top ? CT(0) : CT(a.wip[r]);
+                        if (k == 0) {
+                            fwd_out<CT> o = forward_first<CT>(dtr, w_p_i, w_p_ip, us_c, us_p, v_up, v_ut, v_uts);
+                            asm volatile("" : "+v"(o.c), "+v"(o.d));
+                            c[0] = o.c;
+                            d[0] = o.d;
+                        } else {
+                            const CT cm = c[k > 0 ? k - 1 : 0], dm = d[k > 0 ? k - 1 : 0];
+                            fwd_out<CT> o = forward_body<CT>(dtr, w_c_i, w_c_ip, w_p_i, w_p_ip, us_m, us_c, us_p,
+                                v_up, v_ut, v_uts, cm, dm, at_top);
+                            asm volatile("" : "+v"(o.c), "+v"(o.d));
+                            c[k] = o.c;
+                            d[k] = o.d;
+                        }
+                        us_m = us_c;
+                        us_c = us_p;
+                        w_c_i = w_p_i;
+                        w_c_ip = w_p_ip;
+                    }
+                    a = b;
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+                // the next strip's first groups, in flight during this strip's backward sweep
+                const int nx = it + (int)gridDim.x;
+                const uint32_t csb = sb;
+                const int cp = p, csc = sc;
+                if (nx < nitems) {
+                    offsets(nx);
+                    pin();
+                    prologue();
+                }
+                // backward: u_pos of the next group read while this group computes; the four output rows of a group
+                // read from LDS at its end and stored at the end of the next group (after the read has completed)
+                float upa[4], upb[4];
+#pragma clang loop unroll(full)
+                for (int r = 0; r < 4; ++r)
+                    upa[r] = s_up[(NK - 4 + r) * 64 + cp];
+                f32x4 o_prev = {0.f, 0.f, 0.f, 0.f};
+                uint32_t so_prev = 0;
+                CT data = CT(0);
+#pragma clang loop unroll(full)
+                for (int G = NG - 1; G >= 0; --G) {
+                    if (G > 0) {
+#pragma clang loop unroll(full)
+                        for (int r = 0; r < 4; ++r)
+                            upb[r] = s_up[(4 * G - 4 + r) * 64 + cp];
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma clang loop unroll(full)
+                    for (int r = 3; r >= 0; --r) {
+                        const int k = 4 * G + r;
+                        // u_backward_function (vertical_advection_dycore.cpp:110-121)
+                        const CT dt = d[k] - c[k] * data;
+                        const CT nd = k == top ? d[k] : dt;
+                        data = nd;
+                        s_out[r * 64 + lane] = float(dtr * (nd - CT(upa[r])));
+                    }
+                    // levels 4G..4G+3: row g holds level 4G+g; a row above the top is stored as the top row (the same
+                    // bits its own lanes store) to the top level, which the group holding the top, stored later in
+                    // program order, overwrites with the same bits or its final value
+                    lds_order();
+                    const int L = min(4 * G + g, top), rg = max(L - 4 * G, 0);
+                    const f32x4 o = *(const f32x4 *)&s_out[rg * 64 + csc];
+                    lds_order();
+                    if (G < NG - 1) {
+                        pin();
+                        bstore4<CPOL_OUT>(r_uts, o_prev, so_prev, 0);
+                    }
+                    o_prev = o;
+                    so_prev = csb + (uint32_t)L * skb;
+#pragma clang loop unroll(full)
+                    for (int r = 0; r < 4; ++r)
+                        upa[r] = upb[r];
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+                pin();
+                bstore4<CPOL_OUT>(r_uts, o_prev, so_prev, 0);
+            }
+        }
+
+        // Largest linear element offset of a field's allocation box (lower/upper are the box around the origin).
+        inline int64_t alloc_last(const gt355_field *f) {
+            return (int64_t)(f->upper[0] - 1) + (int64_t)(f->upper[1] - 1) * f->stride[1] +
+                   (int64_t)(f->upper[2] - 1) * f->stride[2];
+        }
+        // Whether vadv_wide_kernel may run on these FP32 fields (its requirements, see the kernel).
+        inline bool wide_fits(const gt355_field *uts, const gt355_field *us, const gt355_field *wc,
+            const gt355_field *up, const gt355_field *ut, int ni, int nj, int nk) {
+            if (ni % 4 != 0)
+                return false;
+            const int64_t sj = uts->stride[1], sk = uts->stride[2], wsj = wc->stride[1], wsk = wc->stride[2];
+            if (sj % 4 || sk % 4 || wsj % 4 || wsk % 4)
+                return false;
+            const int64_t end_i = (int64_t)((ni + 63) / 64) * 64; // rows are read to the end of the last strip
+            const int64_t last = (int64_t)(nk - 1) * sk + (int64_t)(nj - 1) * sj + end_i - 1;
+            const int64_t wlast = (int64_t)(nk - 1) * wsk + (int64_t)(nj - 1) * wsj + end_i; // + column 64
+            for (const gt355_field *f : {uts, us, up, ut})
+                if (((uintptr_t)f->origin & 15) || last > alloc_last(f))
+                    return false;
+            if (((uintptr_t)wc->origin & 15) || wlast > alloc_last(wc))
+                return false;
+            return std::max(last, wlast) * 4 < (int64_t(1) << 32);
         }
 
         // Largest k-depth whose ccol/dcol fit the register file next to the prefetch rings without private-memory
@@ -388,6 +649,30 @@ namespace gt355 {
                 (const ST *)ut->origin, dtr_c, ni, nk, sj, sk, wc->stride[1], wc->stride[2]);                    \
         return check_launch("gt355_vadv");                                                                       \
     }
+                // FP32 storage, 65..80 levels: the 16-byte-request form (GT355_TUNE_vadv_wide=0 leaves it; _dg sets
+                // the groups of four levels in flight, 3 to 5)
+                if constexpr (sizeof(ST) == 4) {
+                    if (nk > 64 && tune_param("vadv_wide", 1) != 0 && wide_fits(uts, us, wc, up, ut, ni, nj, nk)) {
+                        const int strips_i = (ni + 63) / 64;
+                        const int64_t items = (int64_t)strips_i * nj;
+                        GT355_CHECK(items < (int64_t(1) << 31), "gt355_vadv: too many columns");
+                        const int waves = (int)std::min<int64_t>(items, (int64_t)cu_count() * 4);
+                        // FP64 compute fits three groups in flight without spills, FP32 compute four
+                        const int dg = tune_param("vadv_wide_dg", sizeof(CT) == 8 ? 3 : 4);
+#define GT355_VADV_WIDE(DGV)                                                                                     \
+    if (dg == DGV) {                                                                                             \
+        vadv_wide_kernel<CT, 80, DGV><<<waves, 64, 0, stream>>>((float *)uts->origin, (const float *)us->origin, \
+            (const float *)wc->origin, (const float *)up->origin, (const float *)ut->origin, dtr_c, ni, nk, sj, sk, \
+            wc->stride[1], wc->stride[2], strips_i, (int)items, xcd_count());                                    \
+        return check_launch("gt355_vadv");                                                                       \
+    }
+                        GT355_VADV_WIDE(3)
+                        GT355_VADV_WIDE(5)
+                        GT355_VADV_WIDE(4)
+#undef GT355_VADV_WIDE
+                        GT355_CHECK(false, "gt355_vadv: GT355_TUNE_vadv_wide_dg = %d (3, 4 or 5)", dg);
+                    }
+                }
                 // more strips than two rounds of one wavefront per SIMD: the persistent form (GT355_TUNE_vadv_persist=0
                 // selects the one-strip-per-wavefront grid)
                 if (nk > 64 && tune_param("vadv_upos_lds", 1) != 0 && tune_param("vadv_persist", 1) != 0) {

@@ -222,6 +222,7 @@ def test_vadv_persistent_strips(gpu, oracle, monkeypatch, dtype, precision, ni, 
     """Domains with at least two rounds of 64-column strips per SIMD take the persistent kernel (next strip's first
     levels prefetched during the backward sweep): bit-identical to the oracle, with a partial last strip (130), a
     one-column domain (1) and a depth below the bucket (67, 72)."""
+    monkeypatch.setenv("GT355_TUNE_vadv_wide", "0")  # FP32 storage with ni % 4 == 0 would take the wide form
     print(f"[case] vadv persistent {np.dtype(dtype).name} prec={precision} {ni}x{nj}x{nk}", flush=True)
     f, grid = vadv_case(gpu, ni, nj, nk, dtype)
     dtr = 0.15 if dtype == np.float64 else float(np.float32(0.15))
@@ -235,11 +236,34 @@ def test_vadv_persistent_strips(gpu, oracle, monkeypatch, dtype, precision, ni, 
 def test_vadv_persistent_strips_depth16(gpu, oracle, monkeypatch, precision, ni, nj, nk):
     """The persistent kernel with 16 levels of load-ahead (FP32 storage, GT355_TUNE_vadv_persist_d=16)."""
     monkeypatch.setenv("GT355_TUNE_vadv_persist_d", "16")
+    monkeypatch.setenv("GT355_TUNE_vadv_wide", "0")
     print(f"[case] vadv persistent D=16 prec={precision} {ni}x{nj}x{nk}", flush=True)
     f, grid = vadv_case(gpu, ni, nj, nk, np.float32)
     ref = oracle_vadv(oracle, f, float(np.float32(0.15)), 3, ni, nj, nk, "native" if precision == 0 else "mixed")
     got = run_vadv(f, grid, 0.15, precision)[:nk, 3:3 + nj, 3:3 + ni]
     assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("dg", [3, 4, 5])
+@pytest.mark.parametrize("precision", [1, 0])
+@pytest.mark.parametrize("ni,nj,nk", [(4, 3, 65), (60, 7, 80), (64, 5, 66), (100, 9, 77), (256, 256, 80),
+                                      (132, 700, 71), (512, 40, 79)])
+def test_vadv_wide_requests(gpu, oracle, monkeypatch, dg, precision, ni, nj, nk):
+    """The FP32-storage form with 16-byte requests (four rows per load, staged in LDS; four levels per store) for
+    65..80 levels and ni % 4 == 0: bit-identical to the oracle with 3, 4 and 5 groups of four levels in flight, a
+    partial last strip (4, 60, 100, 132: lanes past ni clamp their output chunk onto the last four columns), depths
+    below the bucket (65, 66, 71, 77, 79: rows above the top level), one and several strips per wavefront."""
+    monkeypatch.setenv("GT355_TUNE_vadv_wide_dg", str(dg))
+    print(f"[case] vadv wide dg={dg} prec={precision} {ni}x{nj}x{nk}", flush=True)
+    f, grid = vadv_case(gpu, ni, nj, nk, np.float32)
+    ref = oracle_vadv(oracle, f, float(np.float32(0.15)), 3, ni, nj, nk, "native" if precision == 0 else "mixed")
+    got = run_vadv(f, grid, 0.15, precision)
+    assert np.array_equal(got[:nk, 3:3 + nj, 3:3 + ni], ref)
+    # nothing written outside the interior
+    host0 = vadv_case(gpu, ni, nj, nk, np.float32)[0]["utens_stage"].padded_numpy()
+    mask = np.ones(got.shape, bool)
+    mask[:nk, 3:3 + nj, 3:3 + ni] = False
+    assert np.array_equal(got[mask], host0[mask])
 
 
 @pytest.mark.parametrize("ni,nj,nk", [(23, 11, 43), (256, 256, 80)])
