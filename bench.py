@@ -467,6 +467,7 @@ def batch_leg(gt, nsteps=10, reps=3, seed=2024):
         "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
         "items": items,
         "tuned_wg_per_cu": [w0, w1],
+        "tuned_config": [bt.batches[0].config(), bt.batches[1].config()],
         "tune_us": [round(us0, 1), round(us1, 1)],
         "workgroups": grid,
     }

@@ -135,6 +135,8 @@ SIGNATURES = {
     "gt355_batch_info": [_P, ctypes.POINTER(_I), ctypes.POINTER(_I)],
     "gt355_batch_destroy": [_P],
     "gt355_batch_tune": [_P, _I, _P, ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_double)],
+    "gt355_batch_get_config": [_P, ctypes.POINTER(ctypes.c_int32)],
+    "gt355_batch_set_config": [_P, ctypes.POINTER(ctypes.c_int32)],
 }
 _RESTYPES = {"gt355_last_error": ctypes.c_char_p, "gt355_version": ctypes.c_char_p}
 

@@ -220,9 +220,10 @@ namespace gt355 {
         }
 
 
-        // Work item of the strip form: 64 columns (from i0) of row j of one domain.
+        // Work item of the strip form: 64 columns (from i0) of row j of one domain; with `flat` set (the two-phase
+        // form's vertical phase), 64 entries of the domain's column list from entry i0 (j unused).
         struct sitem {
-            int32_t dom, i0, j, pad;
+            int32_t dom, i0, j, flat;
         };
         constexpr int SD = 4; // load-ahead depth of the strip sweep
         // tail levels a wavefront's scratch holds, for a register part of NK levels
@@ -255,24 +256,26 @@ namespace gt355 {
                 const strip_fields<ST> f{(const ST *)dm.in, (const ST *)dm.coeff, (const ST *)dm.wcon,
                     (const ST *)dm.u_pos, (const ST *)dm.utens, (ST *)dm.u_stage, (ST *)dm.utens_stage,
                     (ST *)dm.lap, dm.sj, dm.sk, dm.sj, dm.sk, dm.sj, dm.sk, dm.sj, dm.sk, dm.wsj, dm.wsk};
+                // the vertical phase takes items of the column list (flagged in the item) or of one row's strip
+                const bool flat = item.flat != 0;
                 if constexpr (!HD && NK == 80) {
                     // the vertical phase: columns of at most 48 or 64 levels run a register part of that depth
                     // (a uniform branch per item), not 80 steps with the levels past their top clamped
                     if (dm.nk <= 48)
-                        hv_strip<ST, CT, 48, SD, true, true, false, HD>(f, dtr, dm.ni, dm.nj, dm.nk, item.i0, item.j,
-                            s_up, tail);
+                        hv_strip<ST, CT, 48, SD, true, true, false, HD, !HD>(
+                            f, dtr, dm.ni, dm.nj, dm.nk, item.i0, item.j, s_up, tail, flat);
                     else if (dm.nk <= 64)
-                        hv_strip<ST, CT, 64, SD, true, true, false, HD>(f, dtr, dm.ni, dm.nj, dm.nk, item.i0, item.j,
-                            s_up, tail);
+                        hv_strip<ST, CT, 64, SD, true, true, false, HD, !HD>(
+                            f, dtr, dm.ni, dm.nj, dm.nk, item.i0, item.j, s_up, tail, flat);
                     else if (DEEP && dm.nk >= deep_nk)
-                        hv_strip<ST, CT, DEEP ? 96 : NK, SD, true, true, true, HD>(f, dtr, dm.ni, dm.nj, dm.nk,
-                            item.i0, item.j, s_up, tail);
+                        hv_strip<ST, CT, DEEP ? 96 : NK, SD, true, true, true, HD, !HD>(
+                            f, dtr, dm.ni, dm.nj, dm.nk, item.i0, item.j, s_up, tail, flat);
                     else
-                        hv_strip<ST, CT, NK, SD, true, true, true, HD>(f, dtr, dm.ni, dm.nj, dm.nk, item.i0, item.j,
-                            s_up, tail);
+                        hv_strip<ST, CT, NK, SD, true, true, true, HD, !HD>(
+                            f, dtr, dm.ni, dm.nj, dm.nk, item.i0, item.j, s_up, tail, flat);
                 } else {
-                    hv_strip<ST, CT, NK, SD, true, true, true, HD>(f, dtr, dm.ni, dm.nj, dm.nk, item.i0, item.j, s_up,
-                        tail);
+                    hv_strip<ST, CT, NK, SD, true, true, true, HD, !HD>(
+                        f, dtr, dm.ni, dm.nj, dm.nk, item.i0, item.j, s_up, tail, flat);
                 }
             }
         }
@@ -308,14 +311,17 @@ using namespace gt355;
 struct gt355_batch {
     bdom *doms;    // device table
     void *items;   // device table: bitem (tile form) or sitem (strip and two-phase forms)
-    void *hitems;  // device table: hitem (the two-phase form's stencil phase)
+    void *hitems[3]; // device tables: hitem (the two-phase form's stencil phase) for 8, 16 and 32 rows per item
     void *scratch; // ccol/dcol of every workgroup the grid may have, in the compute type
     int impl;      // 1: 64x4 tiles, ccol/dcol in scratch; 2: 64-column strips, ccol/dcol in registers;
-                   // 3: two phases (stencil sweep, then the strips' vertical advection)
+                   // 3: two phases (stencil sweep, then the vertical advection over column-list items)
     int nk_reg;    // strip form: levels of ccol/dcol held in registers (80: one wavefront per SIMD; 32: two)
     int deep_nk;   // vertical phase, FP32 storage: columns of at least this many levels keep 96 in registers
     int ndoms, nitems, dtype, precision, grid, gridmax, cus, rows;
-    int nhitems, hgrid, hjb; // the two-phase form's stencil phase: items, workgroups, rows per item
+    int w;           // the persistent grid's setting (batch_grid)
+    int nhitems[3];  // the stencil phase: items per table
+    int hjb, hw;     // the stencil phase: rows per item, workgroups per CU
+    uint64_t sig;    // signature of the domain list (key of the tuner's table)
 };
 
 namespace gt355 {
@@ -327,6 +333,33 @@ namespace gt355 {
         inline int batch_grid(const gt355_batch *bt, int w) {
             return std::min<int>(bt->nitems, bt->cus * (bt->impl >= 2 ? (bt->nk_reg == 32 ? 4 : 2) * w : w));
         }
+        // the stencil phase's tables: index of 8, 16 or 32 rows per item
+        inline int hjb_index(int hjb) { return hjb == 8 ? 0 : hjb == 32 ? 2 : 1; }
+        constexpr int HJB_OF[3] = {8, 16, 32};
+        // the stencil phase's grid: four wavefronts per workgroup, hw workgroups per CU (8: one round at 8
+        // wavefronts per SIMD), each wavefront striding through the items
+        inline int stencil_grid(const gt355_batch *bt) {
+            const int64_t n = bt->nhitems[hjb_index(bt->hjb)];
+            return (int)std::max<int64_t>(1, std::min<int64_t>((n + 3) / 4, (int64_t)bt->cus * bt->hw));
+        }
+        // FNV-1a over the batch's form, types and domain sizes: the key of its tuned configuration
+        inline uint64_t batch_signature(int impl, int dt, int precision, const gt355_batch_domain *d, int n) {
+            uint64_t h = 1469598103934665603ULL;
+            auto mix = [&](int64_t v) {
+                for (int b = 0; b < 8; ++b) {
+                    h ^= (uint64_t)((v >> (8 * b)) & 0xff);
+                    h *= 1099511628211ULL;
+                }
+            };
+            mix(impl), mix(dt), mix(precision), mix(n);
+            for (int k = 0; k < n; ++k)
+                mix(d[k].ni), mix(d[k].nj), mix(d[k].nk);
+            return h;
+        }
+        inline int batch_ct(const gt355_batch *bt) {
+            return (bt->dtype == GT355_F64 || bt->precision == GT355_WIDEN) ? 8 : 4;
+        }
+        inline int batch_st(const gt355_batch *bt) { return bt->dtype == GT355_F64 ? 8 : 4; }
     } // namespace
 } // namespace gt355
 
@@ -345,15 +378,12 @@ extern "C" int gt355_batch_create(int precision, const gt355_batch_domain *domai
     // FP32 storage / FP64 compute, 2.41 ms per timestep against 2.73-2.87 ms for the single-launch strip form)
     const int impl_p = tune_param("batch_impl", 3);
     const int impl = impl_p == 1 ? 1 : impl_p == 3 ? 3 : 2;
-    // rows per stencil-phase item of the two-phase form (GT355_TUNE_batch_hjb: 8, 16 or 32)
-    const int hjb_p = tune_param("batch_hjb", 16);
-    const int hjb = hjb_p == 8 ? 8 : hjb_p == 32 ? 32 : 16;
     // register depth of the strip form (GT355_TUNE_batch_nk: 80 or 32)
     const int nk_reg = tune_param("batch_nk", 80) == 32 ? 32 : 80;
     std::vector<bdom> doms(ndomains);
     std::vector<bitem> items;
     std::vector<sitem> sitems;
-    std::vector<hitem> hitems;
+    std::vector<hitem> hitems[3];
     for (int d = 0; d < ndomains; ++d) {
         const gt355_batch_domain &b = domains[d];
         const int ni = b.ni, nj = b.nj, nk = b.nk;
@@ -392,21 +422,36 @@ extern "C" int gt355_batch_create(int precision, const gt355_batch_domain *domai
         m.nk = nk;
         m.tiles_i = (ni + BTI - 1) / BTI;
         if (impl == 3) {
-            const int strips = (ni + 63) / 64, rowblocks = (nj + hjb - 1) / hjb;
-            for (int k = 0; k < nk; ++k)
-                for (int rb = 0; rb < rowblocks; ++rb)
-                    for (int si = 0; si < strips; ++si)
-                        hitems.push_back(hitem{d, si * 64, rb * hjb, k});
+            // the stencil phase's item tables for every row-block height the tuner may choose
+            for (int h = 0; h < 3; ++h) {
+                const int jb = HJB_OF[h];
+                const int strips = (ni + 63) / 64, rowblocks = (nj + jb - 1) / jb;
+                for (int k = 0; k < nk; ++k)
+                    for (int rb = 0; rb < rowblocks; ++rb)
+                        for (int si = 0; si < strips; ++si)
+                            hitems[h].push_back(hitem{d, si * 64, rb * jb, k});
+            }
         }
         if (impl >= 2) {
             const int64_t lim = int64_t(1) << 32, es = dt == GT355_F64 ? 8 : 4;
             GT355_CHECK(((int64_t)(nk - 1) * m.sk + (int64_t)(nj + 3) * m.sj + ni + 4) * es < lim &&
                             ((int64_t)(nk - 1) * m.wsk + (int64_t)(nj + 1) * m.wsj + ni + 1) * es < lim,
                 "gt355_batch_create: domain %d spans 4 GiB or more", d);
-            const int strips = (ni + 63) / 64;
-            for (int j = 0; j < nj; ++j)
-                for (int si = 0; si < strips; ++si)
-                    sitems.push_back(sitem{d, si * 64, j, 0}); // disjoint strips: each column has one owner
+            if (impl == 3 && tune_param("batch_flat", 0) != 0) {
+                // the vertical phase on 64 consecutive entries of the domain's column list per item (no partial
+                // strips at row ends; disjoint, so each column has one owner).  Measured slower than row strips on
+                // 64 domains of seed 2024 (2.47 against 2.14 ms per timestep): an item's rows start at any column
+                // and may span two rows, so its requests touch more, unaligned lines
+                GT355_CHECK((int64_t)ni * nj < (int64_t(1) << 31), "gt355_batch_create: domain %d has too many columns",
+                    d);
+                for (int c0 = 0; c0 < ni * nj; c0 += 64)
+                    sitems.push_back(sitem{d, c0, 0, 1});
+            } else {
+                const int strips = (ni + 63) / 64;
+                for (int j = 0; j < nj; ++j)
+                    for (int si = 0; si < strips; ++si)
+                        sitems.push_back(sitem{d, si * 64, j, 0}); // disjoint strips: each column has one owner
+            }
         } else {
             const int tj_n = (nj + BTJ - 1) / BTJ;
             for (int tj = 0; tj < tj_n; ++tj)
@@ -424,18 +469,23 @@ extern "C" int gt355_batch_create(int precision, const gt355_batch_domain *domai
     gt355_batch *bt = new gt355_batch{};
     bt->impl = impl;
     bt->nk_reg = nk_reg;
-    bt->deep_nk = tune_param("batch_deep_nk", 89);
-    bt->hjb = hjb;
-    bt->nhitems = (int)hitems.size();
-    // the stencil phase: four wavefronts per workgroup, at most 8 workgroups per CU (one round at 8 wavefronts
-    // per SIMD), each wavefront striding through the items
-    bt->hgrid = (int)std::max<int64_t>(1, std::min<int64_t>(((int64_t)hitems.size() + 3) / 4, (int64_t)p.multiProcessorCount * 8));
     bt->ndoms = ndomains;
     bt->nitems = impl >= 2 ? (int)sitems.size() : (int)items.size();
     bt->dtype = dt;
     bt->precision = precision;
     bt->cus = p.multiProcessorCount;
-    bt->grid = batch_grid(bt, wg_per_cu);
+    for (int h = 0; h < 3; ++h)
+        bt->nhitems[h] = (int)hitems[h].size();
+    // the launch configuration: the tuner's entry for this list of domains if one was recorded (or loaded), else
+    // the measured defaults and the caller's grid setting; environment knobs override both
+    bt->sig = batch_signature(impl, dt, precision, domains, ndomains);
+    batch_cfg cfg{16, 8, wg_per_cu, 89};
+    (void)find_batch_cfg(bt->sig, dt == GT355_F64 ? 8 : 4, batch_ct(bt), impl, &cfg);
+    bt->hjb = HJB_OF[hjb_index(tune_param("batch_hjb", cfg.hjb))];
+    bt->hw = std::min(8, std::max(1, tune_param("batch_hw", cfg.hw)));
+    bt->w = std::min(BWG_MAX, std::max(1, cfg.w));
+    bt->deep_nk = tune_param("batch_deep_nk", cfg.deep_nk);
+    bt->grid = batch_grid(bt, bt->w);
     bt->gridmax = batch_grid(bt, BWG_MAX); // scratch for any grid the tuner may pick
     const size_t item_bytes = impl >= 2 ? sizeof(sitem) * sitems.size() : sizeof(bitem) * items.size();
     const void *item_src = impl >= 2 ? (const void *)sitems.data() : (const void *)items.data();
@@ -446,10 +496,12 @@ extern "C" int gt355_batch_create(int precision, const gt355_batch_domain *domai
         e = hipMemcpy(bt->doms, doms.data(), sizeof(bdom) * doms.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(bt->items, item_src, item_bytes, hipMemcpyHostToDevice);
-    if (e == hipSuccess && !hitems.empty())
-        e = hipMalloc(&bt->hitems, sizeof(hitem) * hitems.size());
-    if (e == hipSuccess && !hitems.empty())
-        e = hipMemcpy(bt->hitems, hitems.data(), sizeof(hitem) * hitems.size(), hipMemcpyHostToDevice);
+    for (int h = 0; h < 3; ++h) {
+        if (e == hipSuccess && !hitems[h].empty())
+            e = hipMalloc(&bt->hitems[h], sizeof(hitem) * hitems[h].size());
+        if (e == hipSuccess && !hitems[h].empty())
+            e = hipMemcpy(bt->hitems[h], hitems[h].data(), sizeof(hitem) * hitems[h].size(), hipMemcpyHostToDevice);
+    }
     // each batch owns its scratch, so batches on different streams never share it
     int maxnk = 0;
     for (const bdom &m : doms)
@@ -462,7 +514,8 @@ extern "C" int gt355_batch_create(int precision, const gt355_batch_domain *domai
     if (e != hipSuccess) {
         (void)hipFree(bt->doms);
         (void)hipFree(bt->items);
-        (void)hipFree(bt->hitems);
+        for (void *h : bt->hitems)
+            (void)hipFree(h);
         (void)hipFree(bt->scratch);
         delete bt;
         return hip_fail(e, "gt355_batch_create");
@@ -477,15 +530,17 @@ extern "C" int gt355_batch_run(gt355_batch *bt, double dtr_stage, void *stream) 
     hipStream_t s = (hipStream_t)stream;
     if (bt->impl >= 2) {
         const sitem *it = (const sitem *)bt->items;
-        if (bt->impl == 3 && bt->nhitems > 0) {
-            const hitem *hi = (const hitem *)bt->hitems;
+        const int hx = hjb_index(bt->hjb);
+        if (bt->impl == 3 && bt->nhitems[hx] > 0) {
+            const hitem *hi = (const hitem *)bt->hitems[hx];
+            const int nh = bt->nhitems[hx], hg = stencil_grid(bt);
 #define GT355_BATCH_HDIFF(JB)                                                                                   \
     if (bt->dtype == GT355_F64)                                                                                 \
-        batch_hdiff_kernel<double, double, JB, 4><<<bt->hgrid, 256, 0, s>>>(bt->doms, hi, bt->nhitems, xcd_count()); \
+        batch_hdiff_kernel<double, double, JB, 4><<<hg, 256, 0, s>>>(bt->doms, hi, nh, xcd_count());            \
     else if (bt->precision == GT355_WIDEN)                                                                      \
-        batch_hdiff_kernel<float, double, JB, 4><<<bt->hgrid, 256, 0, s>>>(bt->doms, hi, bt->nhitems, xcd_count());  \
+        batch_hdiff_kernel<float, double, JB, 4><<<hg, 256, 0, s>>>(bt->doms, hi, nh, xcd_count());             \
     else                                                                                                        \
-        batch_hdiff_kernel<float, float, JB, 4><<<bt->hgrid, 256, 0, s>>>(bt->doms, hi, bt->nhitems, xcd_count());
+        batch_hdiff_kernel<float, float, JB, 4><<<hg, 256, 0, s>>>(bt->doms, hi, nh, xcd_count());
             if (bt->hjb == 8) {
                 GT355_BATCH_HDIFF(8)
             } else if (bt->hjb == 32) {
@@ -545,7 +600,8 @@ extern "C" int gt355_batch_destroy(gt355_batch *bt) {
         return GT355_OK;
     (void)hipFree(bt->doms);
     (void)hipFree(bt->items);
-    (void)hipFree(bt->hitems);
+    for (void *h : bt->hitems)
+        (void)hipFree(h);
     (void)hipFree(bt->scratch);
     delete bt;
     return GT355_OK;
@@ -560,6 +616,26 @@ extern "C" int gt355_batch_info(const gt355_batch *bt, int *nitems, int *grid) {
     return GT355_OK;
 }
 
+namespace {
+    // mean microseconds of `reps` runs (after one warm-up run) in the batch's current configuration
+    int time_runs(gt355_batch *bt, int reps, hipStream_t s, hipEvent_t e0, hipEvent_t e1, double *us) {
+        int rc = gt355_batch_run(bt, 0.15, s);
+        if (rc)
+            return rc;
+        GT355_HIP(hipEventRecord(e0, s));
+        for (int r = 0; r < reps && rc == GT355_OK; ++r)
+            rc = gt355_batch_run(bt, 0.15, s);
+        if (rc)
+            return rc;
+        GT355_HIP(hipEventRecord(e1, s));
+        GT355_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        GT355_HIP(hipEventElapsedTime(&ms, e0, e1));
+        *us = 1e3 * ms / reps;
+        return GT355_OK;
+    }
+} // namespace
+
 extern "C" int gt355_batch_tune(gt355_batch *bt, int reps, void *stream, int *wg_per_cu, double *best_us) {
     GT355_CHECK(bt != nullptr, "gt355_batch_tune: null batch");
     GT355_CHECK(reps >= 1, "gt355_batch_tune: reps %d", reps);
@@ -567,40 +643,78 @@ extern "C" int gt355_batch_tune(gt355_batch *bt, int reps, void *stream, int *wg
     hipEvent_t e0, e1;
     GT355_HIP(hipEventCreate(&e0));
     GT355_HIP(hipEventCreate(&e1));
-    int best = 0, rc = GT355_OK;
-    double best_t = 1e300;
+    int rc = GT355_OK;
+    double best_t = 1e300, t = 0;
+    // the persistent grid of the vertical phase (or of the single-launch forms), then, for FP32 storage, the depth
+    // from which the vertical phase keeps 96 levels in registers
+    const int deep_opts[4] = {81, 89, 97, 1 << 20};
+    const bool deep = bt->impl == 3 && bt->dtype == GT355_F32 && bt->nk_reg == 80;
+    int best_w = bt->w, best_deep = bt->deep_nk;
     for (int w = 1; w <= BWG_MAX && rc == GT355_OK; ++w) {
-        const int g = batch_grid(bt, w);
-        if (w > 1 && g == batch_grid(bt, w - 1))
+        if (w > 1 && batch_grid(bt, w) == batch_grid(bt, w - 1))
             break; // fewer items than workgroups: larger grids are the same grid
-        bt->grid = g;
-        if ((rc = gt355_batch_run(bt, 0.15, stream)) != GT355_OK) // warm-up
-            break;
-        hipError_t e = hipEventRecord(e0, s);
-        for (int r = 0; r < reps && rc == GT355_OK && e == hipSuccess; ++r)
-            rc = gt355_batch_run(bt, 0.15, stream);
-        if (e == hipSuccess)
-            e = hipEventRecord(e1, s);
-        if (e == hipSuccess)
-            e = hipEventSynchronize(e1);
-        float ms = 0;
-        if (e == hipSuccess)
-            e = hipEventElapsedTime(&ms, e0, e1);
-        if (e != hipSuccess && rc == GT355_OK)
-            rc = hip_fail(e, "gt355_batch_tune");
-        if (rc == GT355_OK && ms / reps < best_t) {
-            best_t = ms / reps;
-            best = w;
+        bt->grid = batch_grid(bt, w);
+        for (int o = 0; o < (deep ? 4 : 1) && rc == GT355_OK; ++o) {
+            if (deep)
+                bt->deep_nk = deep_opts[o];
+            rc = time_runs(bt, reps, s, e0, e1, &t);
+            if (rc == GT355_OK && t < best_t)
+                best_t = t, best_w = w, best_deep = bt->deep_nk;
         }
+    }
+    bt->w = best_w;
+    bt->grid = batch_grid(bt, best_w);
+    bt->deep_nk = best_deep;
+    // the two-phase form's stencil phase: rows per item and workgroups per CU
+    if (bt->impl == 3 && rc == GT355_OK) {
+        int best_hjb = bt->hjb, best_hw = bt->hw;
+        const int hws[3] = {2, 4, 8};
+        for (int h = 0; h < 3 && rc == GT355_OK; ++h)
+            for (int q = 0; q < 3 && rc == GT355_OK; ++q) {
+                bt->hjb = HJB_OF[h];
+                bt->hw = hws[q];
+                rc = time_runs(bt, reps, s, e0, e1, &t);
+                if (rc == GT355_OK && t < best_t)
+                    best_t = t, best_hjb = bt->hjb, best_hw = bt->hw;
+            }
+        bt->hjb = best_hjb;
+        bt->hw = best_hw;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc != GT355_OK)
         return rc;
-    bt->grid = batch_grid(bt, best);
+    record_batch_cfg(bt->sig, batch_st(bt), batch_ct(bt), bt->impl, batch_cfg{bt->hjb, bt->hw, bt->w, bt->deep_nk},
+        best_t);
     if (wg_per_cu)
-        *wg_per_cu = best;
+        *wg_per_cu = bt->w;
     if (best_us)
-        *best_us = best_t * 1e3;
+        *best_us = best_t;
+    return GT355_OK;
+}
+
+extern "C" int gt355_batch_get_config(const gt355_batch *bt, int32_t cfg[5]) {
+    GT355_CHECK(bt != nullptr && cfg != nullptr, "gt355_batch_get_config: null argument");
+    cfg[0] = bt->impl;
+    cfg[1] = bt->hjb;
+    cfg[2] = bt->hw;
+    cfg[3] = bt->w;
+    cfg[4] = bt->deep_nk;
+    return GT355_OK;
+}
+
+extern "C" int gt355_batch_set_config(gt355_batch *bt, const int32_t cfg[5]) {
+    GT355_CHECK(bt != nullptr && cfg != nullptr, "gt355_batch_set_config: null argument");
+    GT355_CHECK(cfg[0] == bt->impl, "gt355_batch_set_config: the form (%d) is fixed at create time (%d)", cfg[0],
+        bt->impl);
+    GT355_CHECK(cfg[1] == 8 || cfg[1] == 16 || cfg[1] == 32, "gt355_batch_set_config: rows per item %d", cfg[1]);
+    GT355_CHECK(cfg[2] >= 1 && cfg[2] <= 8, "gt355_batch_set_config: stencil workgroups per CU %d", cfg[2]);
+    GT355_CHECK(cfg[3] >= 1 && cfg[3] <= BWG_MAX, "gt355_batch_set_config: grid setting %d", cfg[3]);
+    GT355_CHECK(cfg[4] >= 2, "gt355_batch_set_config: deep_nk %d", cfg[4]);
+    bt->hjb = cfg[1];
+    bt->hw = cfg[2];
+    bt->w = cfg[3];
+    bt->grid = batch_grid(bt, bt->w);
+    bt->deep_nk = cfg[4];
     return GT355_OK;
 }

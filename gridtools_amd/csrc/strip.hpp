@@ -53,10 +53,15 @@ namespace gt355 {
     // NARROW = false only when every strip is full (ni a multiple of 64).
     // HD = false: the vertical advection alone, with u_stage read from memory (written beforehand by the batching
     // form's stencil phase) instead of evaluated; nothing but utens_stage is stored.
-    template <class ST, class CT, int NK, int D, bool NARROW, bool LAP, bool TAIL, bool HD = true>
+    // FLAT (with HD = false only): when `flat` is set, the wavefront's columns are 64 consecutive entries of the
+    // domain's column list (column n is i = n % ni of row j = n / ni), starting at i0 (j is not used); lanes past the
+    // last column repeat it, computing and storing the same bits as its own lane.  Columns are independent in the
+    // vertical advection, so no lane idles at the end of a row.
+    template <class ST, class CT, int NK, int D, bool NARROW, bool LAP, bool TAIL, bool HD = true, bool FLAT = false>
     __device__ __forceinline__ void hv_strip(const strip_fields<ST> &f, CT dtr, int ni, int nj, int nk, int i0,
-        int j, ST (*s_up)[64], CT *tail) {
+        int j, ST (*s_up)[64], CT *tail, bool flat = false) {
         static_assert(NK % D == 0, "tail blocks start on a ring-slot boundary");
+        static_assert(!(FLAT && HD), "the column list form has no horizontal neighbours");
         constexpr int64_t S = sizeof(ST);
         const int lane = threadIdx.x & 63;
         const bool left = lane < 32;
@@ -66,7 +71,14 @@ namespace gt355 {
         const int ic = min(i0 + lane, ni + 1);
         const int e1c = left ? i0 - 1 : min(i0 + 64, ni + 1);
         const int e2c = left ? i0 - 2 : min(i0 + 65, ni + 1);
-        const int iv = min(i0 + lane, ni - 1);
+        int iv = min(i0 + lane, ni - 1);
+        if constexpr (FLAT) {
+            if (flat) {
+                const int n = min(i0 + lane, ni * nj - 1);
+                j = n / ni;
+                iv = n - j * ni;
+            }
+        }
         const int top = nk - 1;
         auto lvl = [&](int L) { return (uint32_t)min(L, top); };
 
@@ -159,11 +171,11 @@ namespace gt355 {
             if constexpr (NARROW)
                 res = lane_value(res, src_lane);
             const uint32_t o = lvl(L) * osk_b;
-            bstore<ST, CPOL_NT>(r_ust, ST(res), v_o, o);
+            bstore<ST, CPOL_OUT>(r_ust, ST(res), v_o, o);
             if constexpr (LAP) {
                 if constexpr (NARROW)
                     lap_c = lane_value(lap_c, src_lane);
-                bstore<ST, CPOL_NT>(r_lap, ST(lap_c), v_o, o);
+                bstore<ST, CPOL_OUT>(r_lap, ST(lap_c), v_o, o);
             }
             return res;
         };
@@ -249,17 +261,26 @@ namespace gt355 {
                     const CT dt = rd[s] - rc[s] * data;
                     const CT nd = k == top ? rd[s] : dt;
                     data = nd;
-                    bstore<ST, CPOL_NT>(r_uts, ST(dtr * (nd - rp[s])), v_col, lvl(k) * sk_b);
+                    bstore<ST, CPOL_OUT>(r_uts, ST(dtr * (nd - rp[s])), v_col, lvl(k) * sk_b);
                 }
             }
         }
+        // the parked u_pos read four levels ahead of its use: with one wavefront per SIMD nothing else would hide
+        // the latency of each LDS read in the backward chain
+        constexpr int UA = NK < 4 ? NK : 4;
+        ST l_up[UA];
+#pragma clang loop unroll(full)
+        for (int s = 0; s < UA; ++s)
+            l_up[(NK - 1 - s) % UA] = s_up[NK - 1 - s][lane];
 #pragma clang loop unroll(full)
         for (int k = NK - 1; k >= 0; --k) {
-            const CT v_up = CT(s_up[k][lane]);
+            const CT v_up = CT(l_up[k % UA]);
+            if (k - UA >= 0)
+                l_up[k % UA] = s_up[k - UA][lane];
             const CT dt = d[k] - c[k] * data;
             const CT nd = k == top ? d[k] : dt;
             data = nd;
-            bstore<ST, CPOL_NT>(r_uts, ST(dtr * (nd - v_up)), v_col, lvl(k) * sk_b);
+            bstore<ST, CPOL_OUT>(r_uts, ST(dtr * (nd - v_up)), v_col, lvl(k) * sk_b);
             __builtin_amdgcn_sched_barrier(6);
         }
     }

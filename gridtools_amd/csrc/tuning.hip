@@ -1,9 +1,11 @@
 // Launch-configuration selection and the autotuner's table.
 //
 // The defaults were measured on MI355X at 256x256x80.  gt355_tune_hdiff (hdiff.hip) times every instantiated
-// configuration for one problem shape and records the fastest here; gt355_tune_save / gt355_tune_load persist the
-// table as text, one entry per line:
+// configuration for one problem shape and records the fastest here, and gt355_batch_tune (batch.hip) the fastest
+// launch configuration of a batch, keyed by the signature of its domain list; gt355_tune_save / gt355_tune_load
+// persist the table as text, one entry per line:
 //   hdiff <storage bytes> <compute bytes> <ni> <nj> <nk> <impl> <a> <b> <c> <microseconds>
+//   batch <signature, hex> <storage bytes> <compute bytes> <form> <hjb> <hw> <w> <deep_nk> <microseconds>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -25,7 +27,27 @@ namespace gt355 {
         };
         std::mutex g_tune_mutex;
         std::map<tune_key, entry> g_hdiff;
+        using batch_key = std::tuple<uint64_t, int, int, int>;
+        struct batch_entry {
+            batch_cfg cfg;
+            double us;
+        };
+        std::map<batch_key, batch_entry> g_batch;
     } // namespace
+
+    bool find_batch_cfg(uint64_t sig, int st_size, int ct_size, int impl, batch_cfg *out) {
+        std::lock_guard<std::mutex> lock(g_tune_mutex);
+        auto it = g_batch.find(batch_key{sig, st_size, ct_size, impl});
+        if (it == g_batch.end())
+            return false;
+        *out = it->second.cfg;
+        return true;
+    }
+
+    void record_batch_cfg(uint64_t sig, int st_size, int ct_size, int impl, batch_cfg cfg, double us) {
+        std::lock_guard<std::mutex> lock(g_tune_mutex);
+        g_batch[batch_key{sig, st_size, ct_size, impl}] = batch_entry{cfg, us};
+    }
 
     // The GT355_TUNE_* knobs, read from the environment once (at the first lookup) and again only on
     // gt355_tune_env_refresh: a launch looks its knobs up in this snapshot, with no getenv and no allocation.
@@ -123,6 +145,12 @@ extern "C" int gt355_tune_save(const char *path) {
         std::fprintf(f, "hdiff %d %d %d %d %d %d %d %d %d %.3f\n", std::get<0>(k), std::get<1>(k), std::get<2>(k),
             std::get<3>(k), std::get<4>(k), e.cfg.impl, e.cfg.a, e.cfg.b, e.cfg.c, e.us);
     }
+    for (const auto &kv : g_batch) {
+        const auto &k = kv.first;
+        const auto &e = kv.second;
+        std::fprintf(f, "batch %016llx %d %d %d %d %d %d %d %.3f\n", (unsigned long long)std::get<0>(k),
+            std::get<1>(k), std::get<2>(k), std::get<3>(k), e.cfg.hjb, e.cfg.hw, e.cfg.w, e.cfg.deep_nk, e.us);
+    }
     std::fclose(f);
     return GT355_OK;
 }
@@ -131,15 +159,20 @@ extern "C" int gt355_tune_load(const char *path, int *entries) {
     GT355_CHECK(path != nullptr, "gt355_tune_load: null path");
     FILE *f = std::fopen(path, "r");
     GT355_CHECK(f != nullptr, "gt355_tune_load: cannot open %s", path);
-    char kind[32];
+    char line[256];
     int st, ct, ni, nj, nk, impl, a, b, c;
+    unsigned long long sig;
     double us;
     int n = 0;
     std::lock_guard<std::mutex> lock(g_tune_mutex);
-    while (std::fscanf(f, "%31s %d %d %d %d %d %d %d %d %d %lf", kind, &st, &ct, &ni, &nj, &nk, &impl, &a, &b, &c,
-               &us) == 11) {
-        if (std::string(kind) == "hdiff") {
+    while (std::fgets(line, sizeof line, f)) {
+        if (std::sscanf(line, "hdiff %d %d %d %d %d %d %d %d %d %lf", &st, &ct, &ni, &nj, &nk, &impl, &a, &b, &c,
+                &us) == 10) {
             g_hdiff[tune_key{st, ct, ni, nj, nk}] = entry{hdiff_cfg{impl, a, b, c}, us};
+            ++n;
+        } else if (std::sscanf(line, "batch %llx %d %d %d %d %d %d %d %lf", &sig, &st, &ct, &impl, &a, &b, &c, &nk,
+                       &us) == 9) {
+            g_batch[batch_key{(uint64_t)sig, st, ct, impl}] = batch_entry{batch_cfg{a, b, c, nk}, us};
             ++n;
         }
     }
@@ -157,5 +190,6 @@ extern "C" int gt355_tune_env_refresh(void) {
 extern "C" int gt355_tune_clear(void) {
     std::lock_guard<std::mutex> lock(g_tune_mutex);
     g_hdiff.clear();
+    g_batch.clear();
     return GT355_OK;
 }

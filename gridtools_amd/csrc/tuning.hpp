@@ -31,6 +31,16 @@ namespace gt355 {
     hdiff_cfg hdiff_cfg_for(int st_size, int ct_size, int ni, int nj, int nk);
     void record_hdiff_cfg(int st_size, int ct_size, int ni, int nj, int nk, hdiff_cfg cfg, double us);
 
+    // A batch's launch configuration, as the batch tuner records it for one list of domains (its signature).
+    struct batch_cfg {
+        int hjb;     // stencil phase: rows per item (8, 16, 32)
+        int hw;      // stencil phase: workgroups per CU (2, 4, 8)
+        int w;       // vertical phase (or the single-launch forms): the persistent grid's setting (1..4)
+        int deep_nk; // vertical phase, FP32 storage: columns of at least this many levels keep 96 in registers
+    };
+    bool find_batch_cfg(uint64_t sig, int st_size, int ct_size, int impl, batch_cfg *out);
+    void record_batch_cfg(uint64_t sig, int st_size, int ct_size, int impl, batch_cfg cfg, double us);
+
     // A named integer tuning knob: the environment variable GT355_TUNE_<name> if set, otherwise dflt.  The
     // environment is read once per process; tune_env_refresh (gt355_tune_env_refresh) reads it again.
     int tune_param(const char *name, int dflt);

@@ -307,7 +307,8 @@ int gt355_graph_destroy(void *graph_exec);
  * domains have the dtype of domains[0].in.  Forms (chosen at create time, GT355_TUNE_batch_impl):
  *   3 (default), two phases: one persistent launch of the laplacian and hdiff register sweep over every domain's
  *     (64-column strip, row block, level) items, then one of the vertical advection over every domain's 64-column
- *     strips, u_stage read back from memory; results equal gt355_laplacian + gt355_hdiff_vadv's two-launch
+ *     strips of one row (GT355_TUNE_batch_flat=1: 64 consecutive entries of its column list per item instead),
+ *     u_stage read back from memory; results equal gt355_laplacian + gt355_hdiff_vadv's two-launch
  *     composition bit for bit (in the precision form u_stage reaches the vertical phase rounded to FP32);
  *   2, one launch: strips evaluate hdiff and the vertical advection together, u_stage passed on in the compute
  *     type; results equal gt355_laplacian + the fused gt355_hdiff_vadv bit for bit;
@@ -325,10 +326,19 @@ int gt355_batch_create(int precision, const gt355_batch_domain *domains, int ndo
     gt355_batch **batch);
 int gt355_batch_run(gt355_batch *batch, double dtr_stage, void *stream);
 int gt355_batch_info(const gt355_batch *batch, int *nitems, int *grid);
-/* The batch's part of the autotuner: times `reps` runs for each persistent grid size (1..4 workgroups per CU),
- * keeps the fastest for later runs of this batch and returns it with its mean time.  The batch's outputs (and
- * utens_stage, read and rewritten by every run) are overwritten as by that many gt355_batch_run calls. */
+/* The batch's part of the autotuner: times `reps` runs of each launch configuration (coordinate by coordinate): the
+ * persistent grid setting (1..4), for FP32 storage in the two-phase form the depth from which the vertical phase
+ * keeps 96 levels in registers (81, 89, 97 or never), then the stencil phase's rows per item (8, 16, 32) and
+ * workgroups per CU (2, 4, 8).  It keeps the fastest for later runs of this batch, records it in the tuning table
+ * under the signature of the domain list (gt355_tune_save persists it; a batch created later for the same list of
+ * domain sizes, form and types starts from it), and returns the grid setting with the mean time.  The batch's
+ * outputs (and utens_stage, read and rewritten by every run) are overwritten as by that many gt355_batch_run
+ * calls. */
 int gt355_batch_tune(gt355_batch *batch, int reps, void *stream, int *wg_per_cu, double *best_us);
+/* The launch configuration of a batch: cfg = {form, stencil-phase rows per item, stencil-phase workgroups per CU,
+ * persistent grid setting, vertical-phase 96-level depth}.  set_config applies one (the form must be the batch's). */
+int gt355_batch_get_config(const gt355_batch *batch, int32_t cfg[5]);
+int gt355_batch_set_config(gt355_batch *batch, const int32_t cfg[5]);
 int gt355_batch_destroy(gt355_batch *batch);
 
 /* ------------------------------------------------------------------ measurement */

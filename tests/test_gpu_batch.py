@@ -5,6 +5,8 @@ The work table has more items than workgroups in the wide-domain cases (wg_per_c
 several items and reuse their LDS tiles and scratch slot; k depths cover the first/last-level special cases
 (nk = 2), partial ring blocks (nk not a multiple of the ring depth) and the deepest columns (137, 144).
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -135,6 +137,17 @@ def test_batch_two_phase_bit_exact(gpu, oracle, monkeypatch, dtype, precision, h
     check_batch(gpu, oracle, WIDE + SIZES[:3], dtype, precision, wg_per_cu=1, rounded_ustage=True)
 
 
+@pytest.mark.parametrize("dtype,precision", [(np.float64, 0), (np.float32, 1), (np.float32, 0)])
+def test_batch_vertical_phase_column_list_items_bit_exact(gpu, oracle, monkeypatch, dtype, precision):
+    """GT355_TUNE_batch_flat=1: the vertical phase's items are 64 consecutive entries of a domain's column list
+    (spanning row ends; the last item of a domain repeats its last column), every register depth."""
+    monkeypatch.setenv("GT355_TUNE_batch_impl", "3")
+    monkeypatch.setenv("GT355_TUNE_batch_flat", "1")
+    print(f"[case] batch column-list items {np.dtype(dtype).name} prec={precision}", flush=True)
+    check_batch(gpu, oracle, SIZES + NARROW + DEEP96, dtype, precision, wg_per_cu=2, rounded_ustage=True)
+    check_batch(gpu, oracle, WIDE + SIZES[:3], dtype, precision, wg_per_cu=1, rounded_ustage=True)
+
+
 @pytest.mark.parametrize("deep_nk", [89, 65, 145])
 @pytest.mark.parametrize("dtype,precision", [(np.float32, 1), (np.float32, 0)])
 def test_batch_vertical_96_register_levels_bit_exact(gpu, oracle, monkeypatch, dtype, precision, deep_nk):
@@ -247,3 +260,103 @@ def test_batch_tune_picks_a_grid_and_keeps_results(gpu):
     for ma, mb in zip(a, b):
         for x, y in zip(gpu_domain(ma), gpu_domain(mb)):
             assert np.array_equal(x, y)
+
+
+@pytest.mark.timeout(600)
+def test_batch_c4_workload_step_and_graph(gpu, oracle):
+    """The batching-form workload as stated: 64 domains with ni, nj ~ U{48..384} and nk ~ U{40..137} drawn from seed
+    2024, FP32 storage / FP64 compute, the default two-phase form.  One eager timestep (on the single-stencil
+    distributions, wcon ~ U(-0.5, 0.5)) equals the oracle chain on every domain bit for bit (lap, u_stage and
+    utens_stage); then a captured 10-timestep graph equals 10 eager timesteps of an identical twin on every domain,
+    bit for bit, with the timestep loop's bounded wind (the state stays finite)."""
+    from gridtools_amd.batch import BatchedTimesteps, draw_sizes
+    from gridtools_amd.timing import Stream
+
+    sizes = draw_sizes(2024)
+    assert len(sizes) == 64
+    print(f"[case] batch C4: 64 domains, {sum(a * b * c for a, b, c in sizes) / 1e6:.1f} M points", flush=True)
+    eager = BatchedTimesteps(sizes, seed=2024, halo=H, wcon_max=0.5)
+    host = []
+    for f in eager.domains:
+        h = {n: f[n].padded_numpy() for n in ("coeff", "utens_stage", "u_pos", "utens", "wcon")}
+        h["in_"] = f["u"].padded_numpy()
+        host.append(h)
+    eager.step()
+    gpu.synchronize()
+    dtr = float(np.float32(0.15))
+    for d, ((ni, nj, nk), h) in enumerate(zip(sizes, host)):
+        h64 = {k: v.astype(np.float64) for k, v in h.items()}
+        ref_l, ref_u, ref_t = oracle_domain_rounded_ustage(oracle, h64, ni, nj, nk, dtr)
+        box = np.s_[:nk, H:H + nj, H:H + ni]
+        f = eager.domains[d]
+        assert np.array_equal(f["lap"].padded_numpy()[box], ref_l), ("lap", d)
+        assert np.array_equal(eager.current(d).padded_numpy()[box], ref_u), ("u_stage", d)
+        assert np.array_equal(f["utens_stage"].padded_numpy()[box], ref_t), ("utens_stage", d)
+        host[d] = None
+    del eager
+    print("[case] batch C4: one step equals the oracle; 10-step graph against 10 eager steps", flush=True)
+    twin = BatchedTimesteps(sizes, seed=2024, halo=H)
+    graphed = BatchedTimesteps(sizes, seed=2024, halo=H)
+    st = Stream()
+    g = graphed.capture(10, st)
+    g.launch(st)
+    for _ in range(10):
+        twin.step()
+    st.synchronize()
+    gpu.synchronize()
+    for d in range(len(sizes)):
+        for a, b in ((graphed.current(d), twin.current(d)),
+                     (graphed.domains[d]["utens_stage"], twin.domains[d]["utens_stage"])):
+            x, y = a.padded_numpy(), b.padded_numpy()
+            assert np.all(np.isfinite(x)), d
+            assert np.array_equal(x.view(np.uint32), y.view(np.uint32)), d
+
+
+def test_batch_tuner_configuration_persists(gpu, tmp_path):
+    """The batch tuner's choice (grid, the vertical phase's 96-level depth, the stencil phase's rows per item and
+    workgroups per CU) is valid, recorded under the domain list's signature (a new batch of the same sizes starts
+    from it, also after gt355_tune_save / clear / load), and every configuration computes the same bits."""
+    from gridtools_amd import _abi
+    from gridtools_amd.batch import Batch
+
+    sizes = [(300, 200, 60), (64, 300, 120), (100, 100, 90), (70, 50, 137)]
+    _abi.call("gt355_tune_clear")
+    tuner = Batch(make_members(gpu, sizes, np.float32), precision=1, wg_per_cu=1)
+    default = tuner.config()
+    assert default["form"] == 3 and default["hjb"] == 16 and default["hw"] == 8
+    w, us = tuner.tune(reps=2)
+    cfg = tuner.config()
+    print(f"[tune] batch {cfg} ({us:.1f} us)", flush=True)
+    assert cfg["w"] == w and 1 <= w <= 4 and us > 0
+    assert cfg["hjb"] in (8, 16, 32) and cfg["hw"] in (2, 4, 8) and cfg["deep_nk"] in (81, 89, 97, 1 << 20)
+    again = Batch(make_members(gpu, sizes, np.float32), precision=1, wg_per_cu=1)
+    assert again.config() == cfg
+    other = Batch(make_members(gpu, sizes[:3], np.float32), precision=1, wg_per_cu=1)
+    assert other.config() == default  # another domain list: no entry
+    path = str(tmp_path / "tune.txt")
+    _abi.call("gt355_tune_save", path.encode())
+    assert any(ln.startswith("batch ") for ln in open(path))
+    _abi.call("gt355_tune_clear")
+    assert Batch(make_members(gpu, sizes, np.float32), precision=1, wg_per_cu=1).config() == default
+    n = ctypes.c_int()
+    _abi.call("gt355_tune_load", path.encode(), ctypes.byref(n))
+    assert Batch(make_members(gpu, sizes, np.float32), precision=1, wg_per_cu=1).config() == cfg
+    _abi.call("gt355_tune_clear")
+    # the same bits in every configuration
+    ref = make_members(gpu, sizes, np.float32)
+    Batch(ref, precision=1, wg_per_cu=1).run(0.15)
+    gpu.synchronize()
+    want = [gpu_domain(m) for m in ref]
+    for c in ({"hjb": 8, "hw": 2, "w": 4, "deep_nk": 81}, {"hjb": 32, "hw": 4, "w": 2, "deep_nk": 1 << 20},
+              {"hjb": 16, "hw": 8, "w": 3, "deep_nk": 97}):
+        m = make_members(gpu, sizes, np.float32)
+        b = Batch(m, precision=1, wg_per_cu=1)
+        b.set_config(**c)
+        assert b.config() == dict(c, form=3)
+        b.run(0.15)
+        gpu.synchronize()
+        for x, y in zip([gpu_domain(mm) for mm in m], want):
+            for p, q in zip(x, y):
+                assert np.array_equal(p, q), c
+    with pytest.raises(RuntimeError, match="rows per item"):
+        tuner.set_config(hjb=12)
