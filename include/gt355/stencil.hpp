@@ -37,12 +37,10 @@
 #include "functors.hpp"
 #include "gridtools.hpp"
 
-// k_cached placeholders with the fill policy: 0 (default) reads them from memory at each level, where their loads
-// issue with the stage's other loads; 1 keeps them in the register window too (one load per level at the slide, which
-// measured slower on the vadv spec: 218 vs 159 us at 256x256x80).  Flush and policy-free k-caches always live in the
-// register window.
-#ifndef GT355_ENGINE_RING_FILL
-#define GT355_ENGINE_RING_FILL 0
+// Load-ahead depth of the register windows of vertical multistages: a filled window takes the level it uncovers
+// from a load issued this many levels earlier, and the k sweep runs in unrolled blocks of this many levels.
+#ifndef GT355_ENGINE_PREFETCH
+#define GT355_ENGINE_PREFETCH 16
 #endif
 
 #ifndef GT_FUNCTION
@@ -521,26 +519,46 @@ namespace gt355 {
                 }
             };
 
-            // the register window of one k-cached placeholder: levels kcur + KMIN .. kcur + KMAX of the thread's column
-            template <class T, int KMIN, int KMAX, int KEY, bool FILL, bool FLUSH>
+            // the register window of one k-cached placeholder: levels kcur + KMIN .. kcur + KMAX of the thread's column.
+            // With FILL and PD > 0, the sweep runs in blocks of PD levels and each block starts by loading, for every
+            // filled window, the PD levels its slides will uncover (pf[], one slot per step of the block, so every slot
+            // index is static in the unrolled block): the block's loads are in flight together and the block waits for
+            // memory once instead of once per level.  No load is pending across the loop's back edge.
+            template <class T, int KMIN, int KMAX, int KEY, bool FILL, bool FLUSH, int PD = 0>
             struct kring {
                 static constexpr int key = KEY;
+                static constexpr int pd = PD;
                 T r[KMAX - KMIN + 1];
+                T pf[(FILL && PD > 0) ? PD : 1];
                 int kcur;
+                int fi, fj;        // the thread's column clamped into the fill box: every fill load is unconditional
                 field_view<T> mem; // the placeholder's memory (fill / flush)
                 int fb[4], wb[4];  // column boxes {i0, i1, j0, j1}: read (fill) and written (flush) by the multistage
                 __device__ bool in(const int *b, int i, int j) const { return i >= b[0] && i < b[1] && j >= b[2] && j < b[3]; }
-                // the whole window at the sweep's first level
+                // the level uncovered by the slide after step n of the sweep (the sweep starts at level k0)
+                template <bool Up>
+                __device__ static int uncovered(int n, int nk) {
+                    return Up ? n + 1 + KMAX : nk - 2 - n + KMIN;
+                }
+                __device__ T load(int k, int nk) const { return mem.at(fi, fj, min(max(k, 0), nk - 1)); }
+                // the whole window at the sweep's first level.  Loads are unconditional (column clamped into the fill
+                // box, level into [0, nk)): values outside the box or the column are never read.
+                template <bool Up>
                 __device__ void start(int i, int j, int k, int nk) {
                     kcur = k;
                     if constexpr (FILL) {
-                        if (in(fb, i, j)) {
+                        fi = min(max(i, fb[0]), fb[1] - 1);
+                        fj = min(max(j, fb[2]), fb[3] - 1);
 #pragma unroll
-                            for (int o = KMIN; o <= KMAX; ++o)
-                                if (k + o >= 0 && k + o < nk)
-                                    r[o - KMIN] = mem.at(i, j, k + o);
-                        }
+                        for (int o = KMIN; o <= KMAX; ++o)
+                            r[o - KMIN] = load(k + o, nk);
                     }
+                }
+                // step S of the block starting at step nb: the load of the level its slide uncovers
+                template <bool Up, int S>
+                __device__ void block_load(int nb, int nk) {
+                    if constexpr (FILL && PD > 0)
+                        pf[S] = load(uncovered<Up>(nb + S, nk), nk);
                 }
                 __device__ void flush(int i, int j) {
                     if constexpr (FLUSH) {
@@ -548,27 +566,27 @@ namespace gt355 {
                             mem.at(i, j, kcur) = r[-KMIN];
                     }
                 }
-                // move the window one level in the sweep direction and fill the level it uncovers
-                template <bool Up>
-                __device__ void slide(int i, int j, int nk) {
+                // move the window one level in the sweep direction and fill the level it uncovers (from slot S of the
+                // block's loads)
+                template <bool Up, int S = 0>
+                __device__ void slide(int nk) {
                     if constexpr (Up) {
 #pragma unroll
                         for (int o = KMIN; o < KMAX; ++o)
                             r[o - KMIN] = r[o + 1 - KMIN];
                         ++kcur;
-                        if constexpr (FILL) {
-                            if (in(fb, i, j) && kcur + KMAX < nk)
-                                r[KMAX - KMIN] = mem.at(i, j, kcur + KMAX);
-                        }
                     } else {
 #pragma unroll
                         for (int o = KMAX; o > KMIN; --o)
                             r[o - KMIN] = r[o - 1 - KMIN];
                         --kcur;
-                        if constexpr (FILL) {
-                            if (in(fb, i, j) && kcur + KMIN >= 0)
-                                r[0] = mem.at(i, j, kcur + KMIN);
-                        }
+                    }
+                    if constexpr (FILL) {
+                        T &dst = r[Up ? KMAX - KMIN : 0];
+                        if constexpr (PD > 0)
+                            dst = pf[S];
+                        else
+                            dst = load(Up ? kcur + KMAX : kcur + KMIN, nk);
                     }
                 }
             };
@@ -621,7 +639,7 @@ namespace gt355 {
             };
             template <class P, class Caches, class... Stages>
             struct view_for_t<P, vctx<Caches, Stages...>> {
-                using type = std::conditional_t<find_kc<P, Caches>::found && !(GT355_ENGINE_RING_FILL == 0 && find_kc<P, Caches>::fill),
+                using type = std::conditional_t<find_kc<P, Caches>::found,
                     kring_ref<typename P::kind::data_t, mss_kext<P, Stages...>::kmin, mss_kext<P, Stages...>::kmax, P::key>,
                     typename view_of<typename P::kind>::type>;
             };
@@ -638,6 +656,8 @@ namespace gt355 {
                                             ...);
                 static constexpr bool kofs =
                     ((std::is_same<P, Ps>::value && (As::extent_t::kminus || As::extent_t::kplus)) || ...);
+                // accessed with more than three dimensions (a data dimension or a renamed axis)
+                static constexpr bool nd = ((std::is_same<P, Ps>::value && As::ndims > 3) || ...);
             };
             template <class P, class Stage>
             struct stage_props;
@@ -694,6 +714,37 @@ namespace gt355 {
             };
             template <class... Stages>
             using staged_inputs = typename staged_of<typename plhs_of_stages<tlist<>, Stages...>::type, tlist<Stages...>>::type;
+
+            // The register windows a vertical multistage gets besides the spec's own k-caches: every field placeholder
+            // its stages access only at zero horizontal offsets, in three dimensions, that they only read, or that they
+            // write and also access at a k offset (a recurrence).  Such a window is the reference's k_cache with the
+            // fill policy (and flush, if written): the same values in registers, so its reads are served by the
+            // load-ahead slots instead of one memory round trip per level.  A placeholder written at its own level only
+            // keeps its direct stores.
+            template <class P, class... Stages>
+            struct mss_acc {
+                static constexpr bool written = (stage_props<P, Stages>::written || ...);
+                static constexpr bool ij = (stage_props<P, Stages>::ij || ...);
+                static constexpr bool kofs = (stage_props<P, Stages>::kofs || ...);
+                static constexpr bool nd = (stage_props<P, Stages>::nd || ...);
+                static constexpr bool field =
+                    std::is_same<typename view_of<typename P::kind>::type, field_view<typename P::kind::data_t>>::value;
+            };
+            template <class Caches, class Plhs, class... Stages>
+            struct auto_vc {
+                using type = Caches;
+            };
+            template <class Caches, class P, class... Ps, class... Stages>
+            struct auto_vc<Caches, tlist<P, Ps...>, Stages...> {
+                using A = mss_acc<P, Stages...>;
+                static constexpr bool take =
+                    A::field && !A::ij && !A::nd && !find_kc<P, Caches>::found && (!A::written || A::kofs);
+                using next = std::conditional_t<take, typename lcat<Caches, tlist<kc<P, true, A::written>>>::type, Caches>;
+                using type = typename auto_vc<next, tlist<Ps...>, Stages...>::type;
+            };
+            template <class Caches, class... Stages>
+            using auto_vcaches = typename auto_vc<Caches, typename plhs_of_stages<tlist<>, Stages...>::type, Stages...>::type;
+
 
             // one staged input: its memory and its LDS tile (first element, row length, tile box relative to the tile)
             template <class T>
@@ -839,7 +890,7 @@ namespace gt355 {
             template <class P, bool Fi, bool Fl, class... Stages>
             struct ring_of<kc<P, Fi, Fl>, Stages...> {
                 using type = kring<typename P::kind::data_t, mss_kext<P, Stages...>::kmin, mss_kext<P, Stages...>::kmax,
-                    P::key, Fi, Fl>;
+                    P::key, Fi, Fl, Fi ? GT355_ENGINE_PREFETCH : 0>;
             };
             template <class Caches, class StageList>
             struct ring_list {
@@ -853,8 +904,7 @@ namespace gt355 {
             template <class P, bool Fi, bool Fl, class... Cs, class... Stages>
             struct ring_list<tlist<kc<P, Fi, Fl>, Cs...>, tlist<Stages...>> {
                 using rest = typename ring_list<tlist<Cs...>, tlist<Stages...>>::type;
-                using type = std::conditional_t<GT355_ENGINE_RING_FILL == 0 && Fi, rest,
-                    typename lcat<tlist<typename ring_of<kc<P, Fi, Fl>, Stages...>::type>, rest>::type>;
+                using type = typename lcat<tlist<typename ring_of<kc<P, Fi, Fl>, Stages...>::type>, rest>::type;
             };
             template <class L>
             struct to_vtuple;
@@ -866,6 +916,12 @@ namespace gt355 {
             template <class F, int... Is, class... Rs>
             __device__ void for_rings(vtuple_impl<std::integer_sequence<int, Is...>, Rs...> &rs, F &&f) {
                 (f(vget<Is>(rs)), ...);
+            }
+
+            // the steps of one block of the sweep, each with its static slot
+            template <bool Up, int PD, int... Ss, class F>
+            __device__ __forceinline__ void step_block(std::integer_sequence<int, Ss...>, F &&f) {
+                (f(std::integral_constant<int, Ss>()), ...);
             }
 
             template <class Exec, int NSPL, bool W, class StageList, class ViewsTuple, class RingSet, int NS>
@@ -880,13 +936,25 @@ namespace gt355 {
                 const int nk = ks.pos[NSPL - 1];
                 RingSet rs = rs0;
                 const int k0 = up ? 0 : nk - 1;
-                for_rings(rs, [&](auto &r) { r.start(i, j, k0, nk); });
-                for (int n = 0; n < nk; ++n) {
-                    const int k = up ? n : nk - 1 - n;
-                    R::level(v0, rs, dom.d, i, j, k, ks, std::make_integer_sequence<int, NS>());
-                    for_rings(rs, [&](auto &r) { r.flush(i, j); });
-                    if (n + 1 < nk)
-                        for_rings(rs, [&](auto &r) { r.template slide<up>(i, j, nk); });
+                for_rings(rs, [&](auto &r) { r.template start<up>(i, j, k0, nk); });
+                // blocks of PD levels, unrolled, so that step S of a block uses the windows' load slot S
+                constexpr int PD = GT355_ENGINE_PREFETCH > 0 ? GT355_ENGINE_PREFETCH : 1;
+                for (int nb = 0; nb < nk; nb += PD) {
+                    step_block<up, PD>(std::make_integer_sequence<int, PD>(), [&](auto sc) {
+                        for_rings(rs, [&](auto &r) { r.template block_load<up, decltype(sc)::value>(nb, nk); });
+                    });
+                    // the block's loads stay together at its start
+                    __builtin_amdgcn_sched_barrier(0);
+                    step_block<up, PD>(std::make_integer_sequence<int, PD>(), [&](auto sc) {
+                        constexpr int S = decltype(sc)::value;
+                        const int n = nb + S;
+                        if (n < nk) {
+                            const int k = up ? n : nk - 1 - n;
+                            R::level(v0, rs, dom.d, i, j, k, ks, std::make_integer_sequence<int, NS>());
+                            for_rings(rs, [&](auto &r) { r.flush(i, j); });
+                            for_rings(rs, [&](auto &r) { r.template slide<up, S>(nk); });
+                        }
+                    });
                 }
             }
 
@@ -1166,9 +1234,10 @@ namespace gt355 {
                     };
                     (one(Stages()), ...);
                 } else {
-                    using Ctx = vctx<typename T::caches, Stages...>;
+                    using Caches = auto_vcaches<typename T::caches, Stages...>;
+                    using Ctx = vctx<Caches, Stages...>;
                     using ViewsTuple = vtuple<typename stage_views<Stages, Ctx>::type...>;
-                    using RingSet = typename to_vtuple<typename ring_list<typename T::caches, tlist<Stages...>>::type>::type;
+                    using RingSet = typename to_vtuple<typename ring_list<Caches, tlist<Stages...>>::type>::type;
                     ViewsTuple vt{};
                     RingSet rs{};
                     domains<NS> dom{};
