@@ -453,7 +453,7 @@ namespace gt355 {
 
             // a k-cached placeholder of a vertical multistage: its values live in the vertical kernel's register window
             // (kring) for that key; the evaluator reads level k + dk as element dk - KMIN of the window
-            template <class T, int KMIN, int KMAX, int KEY>
+            template <class T, int KMIN, int KMAX, int KEY, int IMIN = 0, int IMAX = 0, int JMIN = 0, int JMAX = 0>
             struct kring_ref {
                 __host__ __device__ void rebase(int, int) {}
             };
@@ -490,9 +490,10 @@ namespace gt355 {
                     else
                         return view.at(i + a.di, j + a.dj, k + a.dk);
                 }
-                template <class T, int KMIN, int KMAX, int KEY, class Acc>
-                __host__ __device__ __forceinline__ T &get(kring_ref<T, KMIN, KMAX, KEY> const &, Acc const &a) const {
-                    return ring_by_key<KEY>(rs).r[a.dk - KMIN];
+                template <class T, int KMIN, int KMAX, int KEY, int IMIN, int IMAX, int JMIN, int JMAX, class Acc>
+                __host__ __device__ __forceinline__ T &get(
+                    kring_ref<T, KMIN, KMAX, KEY, IMIN, IMAX, JMIN, JMAX> const &, Acc const &a) const {
+                    return ring_by_key<KEY>(rs).r[(a.di - IMIN) * (JMAX - JMIN + 1) + (a.dj - JMIN)][a.dk - KMIN];
                 }
                 template <int I, class E, int N>
                 __host__ __device__ __forceinline__ decltype(auto) operator()(cartesian::in_accessor<I, E, N> const &a) const {
@@ -524,12 +525,18 @@ namespace gt355 {
             // filled window, the PD levels its slides will uncover (pf[], one slot per step of the block, so every slot
             // index is static in the unrolled block): the block's loads are in flight together and the block waits for
             // memory once instead of once per level.  No load is pending across the loop's back edge.
-            template <class T, int KMIN, int KMAX, int KEY, bool FILL, bool FLUSH, int PD = 0>
+            template <class T, int KMIN, int KMAX, int KEY, bool FILL, bool FLUSH, int PD = 0, int IMIN = 0, int IMAX = 0,
+                int JMIN = 0, int JMAX = 0>
             struct kring {
                 static constexpr int key = KEY;
                 static constexpr int pd = PD;
-                T r[KMAX - KMIN + 1];
-                T pf[(FILL && PD > 0) ? PD : 1];
+                // columns of the window: (i + di, j + dj) for di in [IMIN, IMAX], dj in [JMIN, JMAX] (a read-only field
+                // read at horizontal offsets); a written window has the thread's own column only
+                static constexpr int nc = (IMAX - IMIN + 1) * (JMAX - JMIN + 1);
+                static_assert(!FLUSH || nc == 1, "a written window holds the thread's own column");
+                static constexpr int ncol = nc;
+                T r[nc][KMAX - KMIN + 1];
+                T pf[nc][(FILL && PD > 0) ? PD : 1];
                 int kcur;
                 int fi, fj;        // the thread's column clamped into the fill box: every fill load is unconditional
                 field_view<T> mem; // the placeholder's memory (fill / flush)
@@ -540,7 +547,10 @@ namespace gt355 {
                 __device__ static int uncovered(int n, int nk) {
                     return Up ? n + 1 + KMAX : nk - 2 - n + KMIN;
                 }
-                __device__ T load(int k, int nk) const { return mem.at(fi, fj, min(max(k, 0), nk - 1)); }
+                __device__ T load(int c, int k, int nk) const {
+                    const int di = IMIN + c / (JMAX - JMIN + 1), dj = JMIN + c % (JMAX - JMIN + 1);
+                    return mem.at(fi + di, fj + dj, min(max(k, 0), nk - 1));
+                }
                 // the whole window at the sweep's first level.  Loads are unconditional (column clamped into the fill
                 // box, level into [0, nk)): values outside the box or the column are never read.
                 template <bool Up>
@@ -550,43 +560,53 @@ namespace gt355 {
                         fi = min(max(i, fb[0]), fb[1] - 1);
                         fj = min(max(j, fb[2]), fb[3] - 1);
 #pragma unroll
-                        for (int o = KMIN; o <= KMAX; ++o)
-                            r[o - KMIN] = load(k + o, nk);
+                        for (int c = 0; c < nc; ++c)
+#pragma unroll
+                            for (int o = KMIN; o <= KMAX; ++o)
+                                r[c][o - KMIN] = load(c, k + o, nk);
                     }
                 }
                 // step S of the block starting at step nb: the load of the level its slide uncovers
                 template <bool Up, int S>
                 __device__ void block_load(int nb, int nk) {
-                    if constexpr (FILL && PD > 0)
-                        pf[S] = load(uncovered<Up>(nb + S, nk), nk);
+                    if constexpr (FILL && PD > 0) {
+#pragma unroll
+                        for (int c = 0; c < nc; ++c)
+                            pf[c][S] = load(c, uncovered<Up>(nb + S, nk), nk);
+                    }
                 }
                 __device__ void flush(int i, int j) {
                     if constexpr (FLUSH) {
                         if (in(wb, i, j))
-                            mem.at(i, j, kcur) = r[-KMIN];
+                            mem.at(i, j, kcur) = r[0][-KMIN];
                     }
                 }
                 // move the window one level in the sweep direction and fill the level it uncovers (from slot S of the
                 // block's loads)
                 template <bool Up, int S = 0>
                 __device__ void slide(int nk) {
-                    if constexpr (Up) {
 #pragma unroll
-                        for (int o = KMIN; o < KMAX; ++o)
-                            r[o - KMIN] = r[o + 1 - KMIN];
-                        ++kcur;
-                    } else {
+                    for (int c = 0; c < nc; ++c) {
+                        if constexpr (Up) {
 #pragma unroll
-                        for (int o = KMAX; o > KMIN; --o)
-                            r[o - KMIN] = r[o - 1 - KMIN];
-                        --kcur;
+                            for (int o = KMIN; o < KMAX; ++o)
+                                r[c][o - KMIN] = r[c][o + 1 - KMIN];
+                        } else {
+#pragma unroll
+                            for (int o = KMAX; o > KMIN; --o)
+                                r[c][o - KMIN] = r[c][o - 1 - KMIN];
+                        }
                     }
+                    kcur += Up ? 1 : -1;
                     if constexpr (FILL) {
-                        T &dst = r[Up ? KMAX - KMIN : 0];
-                        if constexpr (PD > 0)
-                            dst = pf[S];
-                        else
-                            dst = load(Up ? kcur + KMAX : kcur + KMIN, nk);
+#pragma unroll
+                        for (int c = 0; c < nc; ++c) {
+                            T &dst = r[c][Up ? KMAX - KMIN : 0];
+                            if constexpr (PD > 0)
+                                dst = pf[c][S];
+                            else
+                                dst = load(c, Up ? kcur + KMAX : kcur + KMIN, nk);
+                        }
                     }
                 }
             };
@@ -614,6 +634,29 @@ namespace gt355 {
                 static constexpr int kmax = std::max({0, stage_kext<P, Stages>::kmax...});
             };
 
+            // the horizontal extent box of placeholder P over the stages of a multistage
+            template <class P, class PL, class AL>
+            struct ijext_pair;
+            template <class P, class... Ps, class... As>
+            struct ijext_pair<P, tlist<Ps...>, cartesian::param_list_t<As...>> {
+                static constexpr int imin = std::min({0, (std::is_same<P, Ps>::value ? As::extent_t::iminus : 0)...});
+                static constexpr int imax = std::max({0, (std::is_same<P, Ps>::value ? As::extent_t::iplus : 0)...});
+                static constexpr int jmin = std::min({0, (std::is_same<P, Ps>::value ? As::extent_t::jminus : 0)...});
+                static constexpr int jmax = std::max({0, (std::is_same<P, Ps>::value ? As::extent_t::jplus : 0)...});
+            };
+            template <class P, class Stage>
+            struct stage_ijext;
+            template <class P, class F, class... Ps>
+            struct stage_ijext<P, stage_desc<F, Ps...>> : ijext_pair<P, tlist<Ps...>, typename F::param_list> {};
+            template <class P, class... Stages>
+            struct mss_ijext {
+                static constexpr int imin = std::min({0, stage_ijext<P, Stages>::imin...});
+                static constexpr int imax = std::max({0, stage_ijext<P, Stages>::imax...});
+                static constexpr int jmin = std::min({0, stage_ijext<P, Stages>::jmin...});
+                static constexpr int jmax = std::max({0, stage_ijext<P, Stages>::jmax...});
+                static constexpr int area = (imax - imin + 1) * (jmax - jmin + 1);
+            };
+
             // the kc<> entry of P in a cache list, if any
             template <class P, class L>
             struct find_kc {
@@ -639,8 +682,10 @@ namespace gt355 {
             };
             template <class P, class Caches, class... Stages>
             struct view_for_t<P, vctx<Caches, Stages...>> {
+                using X = mss_ijext<P, Stages...>;
                 using type = std::conditional_t<find_kc<P, Caches>::found,
-                    kring_ref<typename P::kind::data_t, mss_kext<P, Stages...>::kmin, mss_kext<P, Stages...>::kmax, P::key>,
+                    kring_ref<typename P::kind::data_t, mss_kext<P, Stages...>::kmin, mss_kext<P, Stages...>::kmax, P::key,
+                        X::imin, X::imax, X::jmin, X::jmax>,
                     typename view_of<typename P::kind>::type>;
             };
             // fused parallel multistages stage every field they only read, at a horizontal offset and without k offsets,
@@ -716,8 +761,9 @@ namespace gt355 {
             using staged_inputs = typename staged_of<typename plhs_of_stages<tlist<>, Stages...>::type, tlist<Stages...>>::type;
 
             // The register windows a vertical multistage gets besides the spec's own k-caches: every field placeholder
-            // its stages access only at zero horizontal offsets, in three dimensions, that they only read, or that they
-            // write and also access at a k offset (a recurrence).  Such a window is the reference's k_cache with the
+            // its stages access in three dimensions and either only read (at zero horizontal offset, or at up to four
+            // horizontal offsets, with one window per column) or, at zero horizontal offset, write and also access at
+            // a k offset (a recurrence).  Such a window is the reference's k_cache with the
             // fill policy (and flush, if written): the same values in registers, so its reads are served by the
             // load-ahead slots instead of one memory round trip per level.  A placeholder written at its own level only
             // keeps its direct stores.
@@ -737,8 +783,10 @@ namespace gt355 {
             template <class Caches, class P, class... Ps, class... Stages>
             struct auto_vc<Caches, tlist<P, Ps...>, Stages...> {
                 using A = mss_acc<P, Stages...>;
-                static constexpr bool take =
-                    A::field && !A::ij && !A::nd && !find_kc<P, Caches>::found && (!A::written || A::kofs);
+                // read-only fields may also be read at up to four horizontal offsets (one window per column)
+                static constexpr bool take = A::field && !A::nd && !find_kc<P, Caches>::found &&
+                                             (A::ij ? (!A::written && mss_ijext<P, Stages...>::area <= 4)
+                                                    : (!A::written || A::kofs));
                 using next = std::conditional_t<take, typename lcat<Caches, tlist<kc<P, true, A::written>>>::type, Caches>;
                 using type = typename auto_vc<next, tlist<Ps...>, Stages...>::type;
             };
@@ -889,8 +937,9 @@ namespace gt355 {
             struct ring_of;
             template <class P, bool Fi, bool Fl, class... Stages>
             struct ring_of<kc<P, Fi, Fl>, Stages...> {
+                using X = mss_ijext<P, Stages...>;
                 using type = kring<typename P::kind::data_t, mss_kext<P, Stages...>::kmin, mss_kext<P, Stages...>::kmax,
-                    P::key, Fi, Fl, Fi ? GT355_ENGINE_PREFETCH : 0>;
+                    P::key, Fi, Fl, Fi ? GT355_ENGINE_PREFETCH : 0, X::imin, X::imax, X::jmin, X::jmax>;
             };
             template <class Caches, class StageList>
             struct ring_list {
@@ -1262,13 +1311,13 @@ namespace gt355 {
             template <class Ring>
             void init_ring(Ring &r, run_ctx const &c, std::vector<stage_rt> const &st, int first, int ns,
                 const int (*dom)[4]) {
-                using T = std::decay_t<decltype(r.r[0])>;
+                using T = std::decay_t<decltype(r.r[0][0])>;
                 bool any_r = false, any_w = false;
                 for (int n = 0; n < ns; ++n)
                     for (auto const &a : st[first + n].acc) {
                         if (a.key != Ring::key)
                             continue;
-                        if (a.ext.imin || a.ext.imax || a.ext.jmin || a.ext.jmax)
+                        if (Ring::ncol == 1 && (a.ext.imin || a.ext.imax || a.ext.jmin || a.ext.jmax))
                             throw std::runtime_error("run: a k_cached placeholder is accessed at a horizontal offset");
                         auto grow = [&](int *b, bool &any) {
                             if (!any) {
