@@ -496,6 +496,7 @@ def large_leg(gt, nsteps=20, n=1536, nk=80):
     ms, pts = run_graph("fill")
     ms_x, _ = run_graph("exchange")
     gbs = 8 * 4 * pts * nsteps / (ms * 1e-3) / 1e9  # fused launch traffic only (halo fill excluded)
+    bc = boundary_conditions(n, nk)
     return {
         "workload": f"{n}x{n}x{nk} FP32 storage / FP64 compute, {nsteps} timesteps (compose + halo BCs) in a graph",
         "ms_total": round(ms, 3),
@@ -504,6 +505,49 @@ def large_leg(gt, nsteps=20, n=1536, nk=80):
         "achieved_gbs_compose": round(gbs, 1),
         "hbm_frac_compose": round(gbs / HBM_PEAK_GBS, 4),
         "ms_per_timestep_bc_by_halo_exchange": round(ms_x / nsteps, 4),
+        "boundary_conditions": bc,
+    }
+
+
+def boundary_conditions(n, nk, h=3, reps=KERNEL_WINDOW):
+    """The timestep's boundary conditions alone on its two n x n x nk FP32 fields with an h-point halo (periodic i,
+    zero-gradient j): one halo-fill launch, and the halo-pack route (zero-gradient j by halo fill, then the periodic
+    i faces by the single-rank pack / exchange / unpack).  Mean microseconds over `reps` back-to-back calls on the
+    launch stream; algorithmic bytes = every halo point of both fields written once and its source read once."""
+    from gridtools_amd import boundaries
+    from gridtools_amd.grid import halo_descriptor
+    from gridtools_amd.timestep import HdiffVadvTimestepper
+    from gridtools_amd.timing import Event
+
+    ts = HdiffVadvTimestepper(n, n, nk, seed=SEED, bc="exchange", halo=h)
+    st = launch_stream()
+    fields = [ts.u_stage, ts.utens_stage]
+    P, Z, N = boundaries.PERIODIC, boundaries.ZERO_GRADIENT, boundaries.NONE
+
+    def timed(fn):
+        fn()
+        st.synchronize()
+        start, stop = Event(), Event()
+        start.record(st)
+        for _ in range(reps):
+            fn()
+        stop.record(st)
+        stop.synchronize()
+        return 1e3 * start.elapsed_ms(stop) / reps
+
+    fill_us = timed(lambda: boundaries.halo_fill(fields, ts.halos, (P, Z, N), stream=st.handle))
+    xchg_us = timed(lambda: (boundaries.halo_fill(fields, ts.halos, (N, Z, N), stream=st.handle),
+                             boundaries.halo_exchange_local(fields, ts._xhalos, ts._xbuf.value, ts._xmask,
+                                                            stream=st.handle)))
+    halo_pts = ((n + 2 * h) ** 2 - n * n) * nk
+    algo = 2 * 2 * 4 * halo_pts
+    return {
+        "fields": f"2 x {n}x{n}x{nk} FP32, halo {h}, periodic i / zero-gradient j",
+        "halo_points_per_field": halo_pts,
+        "algo_bytes": algo,
+        "halo_fill_us": round(fill_us, 2),
+        "halo_fill_gbs": round(algo / (fill_us * 1e-6) / 1e9, 1),
+        "pack_exchange_route_us": round(xchg_us, 2),
     }
 
 
