@@ -649,10 +649,14 @@ namespace gt355 {
                 (const ST *)ut->origin, dtr_c, ni, nk, sj, sk, wc->stride[1], wc->stride[2]);                    \
         return check_launch("gt355_vadv");                                                                       \
     }
-                // FP32 storage, 65..80 levels: the 16-byte-request form (GT355_TUNE_vadv_wide=0 leaves it; _dg sets
-                // the groups of four levels in flight, 3 to 5)
+                // FP32 storage, 65..80 levels, fewer strips than two rounds of one wavefront per SIMD: the
+                // 16-byte-request form (GT355_TUNE_vadv_wide=0 leaves it; _dg sets the groups of four levels in flight,
+                // 3 to 5).  Measured against the kernels below: 256x256x80 (one round) 22.9 against 23.4 us; with more
+                // rounds the persistent form is faster (1536x1536x80 timestep 1.43 against 1.49 ms)
+                const bool persistent_shape = (int64_t)((ni + 63) / 64) * nj >= 2 * (int64_t)cu_count() * 4;
                 if constexpr (sizeof(ST) == 4) {
-                    if (nk > 64 && tune_param("vadv_wide", 1) != 0 && wide_fits(uts, us, wc, up, ut, ni, nj, nk)) {
+                    if (nk > 64 && !persistent_shape && tune_param("vadv_wide", 1) != 0 &&
+                        wide_fits(uts, us, wc, up, ut, ni, nj, nk)) {
                         const int strips_i = (ni + 63) / 64;
                         const int64_t items = (int64_t)strips_i * nj;
                         GT355_CHECK(items < (int64_t(1) << 31), "gt355_vadv: too many columns");

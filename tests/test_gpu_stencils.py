@@ -247,7 +247,7 @@ def test_vadv_persistent_strips_depth16(gpu, oracle, monkeypatch, precision, ni,
 @pytest.mark.parametrize("dg", [3, 4, 5])
 @pytest.mark.parametrize("precision", [1, 0])
 @pytest.mark.parametrize("ni,nj,nk", [(4, 3, 65), (60, 7, 80), (64, 5, 66), (100, 9, 77), (256, 256, 80),
-                                      (132, 700, 71), (512, 40, 79)])
+                                      (132, 500, 71), (512, 40, 79)])
 def test_vadv_wide_requests(gpu, oracle, monkeypatch, dg, precision, ni, nj, nk):
     """The FP32-storage form with 16-byte requests (four rows per load, staged in LDS; four levels per store) for
     65..80 levels and ni % 4 == 0: bit-identical to the oracle with 3, 4 and 5 groups of four levels in flight, a
