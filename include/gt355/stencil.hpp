@@ -18,8 +18,10 @@
  *     (kring) that slides with k, flushed to memory under the flush policy;
  *   * apply overloads are dispatched per k-level by interval (level<splitter, offset>, full_interval::modify<>,
  *     first_level, last_level), like the reference's interval matching.
- * Stages of a vertical multistage may not read each other's outputs at a horizontal offset (the reference needs
- * block synchronisation for that; the engine reports it as an error).  Fields are the data_store stores of
+ * A vertical multistage whose stages read earlier stages' outputs at a horizontal offset (the reference's need_sync)
+ * runs tile by tile instead: a workgroup walks the levels of its tile and runs each stage over the tile's extended
+ * box with a barrier between stages; specs in which workgroups could see each other's writes in an order the sweep
+ * does not fix are rejected with an error.  Fields are the data_store stores of
  * gt355/gridtools.hpp; the engine is header-only and must be compiled by hipcc (it launches kernels).
  */
 #pragma once
@@ -33,6 +35,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <set>
 
 #include "functors.hpp"
 #include "gridtools.hpp"
@@ -760,6 +763,33 @@ namespace gt355 {
             template <class... Stages>
             using staged_inputs = typename staged_of<typename plhs_of_stages<tlist<>, Stages...>::type, tlist<Stages...>>::type;
 
+            // A vertical multistage needs barriers between its stages when a stage reads, at a horizontal offset, a
+            // placeholder that an earlier stage of the multistage writes (the reference's need_sync,
+            // stencil/core/need_sync.hpp:21-56): the values at the neighbouring columns come from other threads.
+            // Such a multistage runs tile by tile, every workgroup walking the levels of its tile in the sweep's order.
+            template <class Written, class... Stages>
+            struct hsync : std::false_type {};
+            template <class Written, class F, class... Ps, class... Stages>
+            struct hsync<Written, stage_desc<F, Ps...>, Stages...> {
+                using S = stage_desc<F, Ps...>;
+                static constexpr bool here = ((in_list<Ps, Written>::value && stage_props<Ps, S>::ij) || ...);
+                // the placeholders S writes join the written set (void stands for the ones it only reads)
+                using next = typename tset_add_all<Written,
+                    std::conditional_t<stage_props<Ps, S>::written, Ps, void>...>::type;
+                static constexpr bool value = here || hsync<next, Stages...>::value;
+            };
+            // the ij_cached entries of a cache list (its kc<> entries dropped)
+            template <class L>
+            struct ij_entries {
+                using type = tlist<>;
+            };
+            template <class C, class... Cs>
+            struct ij_entries<tlist<C, Cs...>> {
+                using type = typename lcat<tlist<C>, typename ij_entries<tlist<Cs...>>::type>::type;
+            };
+            template <class P, bool Fi, bool Fl, class... Cs>
+            struct ij_entries<tlist<kc<P, Fi, Fl>, Cs...>> : ij_entries<tlist<Cs...>> {};
+
             // The register windows a vertical multistage gets besides the spec's own k-caches: every field placeholder
             // its stages access in three dimensions and either only read (at zero horizontal offset, or at up to four
             // horizontal offsets, with one window per column) or, at zero horizontal offset, write and also access at
@@ -1045,6 +1075,29 @@ namespace gt355 {
                 fused_runner<NSPL, W, StageList, ViewsTuple>::run(
                     v, ext.d, ti0, tj0, TI, TJ, ni, nj, (int)blockIdx.z, ks, std::make_integer_sequence<int, NS>());
             }
+            // one vertical multistage with horizontal dependencies between its stages: a workgroup owns a TI x TJ tile
+            // and walks the levels in the sweep's order; at each level it runs every stage over the tile extended by
+            // the stage's extent, with a barrier between stages, as the fused parallel form does for its one level.
+            // Stages that run on extended boxes compute the points of the neighbouring tiles' rims again (the same
+            // values); the host checks that no read can see another workgroup's write in an order the sweep does not
+            // fix.
+            template <class Exec, int NSPL, bool W, class StageList, class ViewsTuple, class StageIns, int NS>
+            __global__ void __launch_bounds__(256) vfused_kernel(
+                ViewsTuple v0, StageIns si, domains<NS> ext, int ni, int nj, ksplit ks, int tiles_i, int TI, int TJ) {
+                const int tile = (int)blockIdx.x;
+                const int ti0 = (tile % tiles_i) * TI, tj0 = (tile / tiles_i) * TJ;
+                constexpr bool up = !std::is_same<Exec, backward>::value;
+                const int nk = ks.pos[NSPL - 1];
+                for (int n = 0; n < nk; ++n) {
+                    const int k = up ? n : nk - 1 - n;
+                    // the previous level's last stage ended with a barrier, so the staging tiles are free
+                    load_stage_ins(si, ti0, tj0, TI, TJ, ni, nj, k);
+                    __syncthreads();
+                    ViewsTuple v = v0;
+                    fused_runner<NSPL, W, StageList, ViewsTuple>::run(
+                        v, ext.d, ti0, tj0, TI, TJ, ni, nj, k, ks, std::make_integer_sequence<int, NS>());
+                }
+            }
 
             // ------------------------------------------------------------------------------------------ host side
             struct ext_rt {
@@ -1105,6 +1158,9 @@ namespace gt355 {
                 // a parallel multistage of several stages, or with ij-caches, runs as one fused launch
                 static constexpr bool fused =
                     std::is_same<Exec, parallel>::value && (sizeof...(Stages) > 1 || !std::is_same<Caches, tlist<>>::value);
+                // a vertical multistage with horizontal dependencies between its stages runs tile by tile
+                static constexpr bool vsync = !std::is_same<Exec, parallel>::value && hsync<tlist<>, Stages...>::value;
+                using ij_caches = typename ij_entries<Caches>::type;
             };
             template <class Spec>
             struct msses_of {
@@ -1249,8 +1305,8 @@ namespace gt355 {
                 using Exec = typename T::exec;
                 constexpr int NS = sizeof...(Stages);
                 const int ni = g.i_size, nj = g.j_size, nk = g.k_size;
-                if constexpr (T::fused) {
-                    using Ctx = fctx<typename T::caches, Stages...>;
+                if constexpr (T::fused || T::vsync) {
+                    using Ctx = fctx<typename T::ij_caches, Stages...>;
                     using ViewsTuple = vtuple<typename stage_views<Stages, Ctx>::type...>;
                     using Staged = staged_inputs<Stages...>;
                     using StageIns = typename stage_in_tuple<Staged>::type;
@@ -1265,10 +1321,17 @@ namespace gt355 {
                     assign_all<Ctx>(vt, c, args, tlist<Stages...>(), std::make_integer_sequence<int, NS>());
                     fuse_rt const &f = c.fuse[m];
                     const int tiles_i = (ni + f.ti - 1) / f.ti, tiles_j = (nj + f.tj - 1) / f.tj;
-                    fused_kernel<NSPL, W, tlist<Stages...>, ViewsTuple, StageIns, NS>
-                        <<<dim3((unsigned)(tiles_i * tiles_j), 1, (unsigned)nk), 256, f.bytes, stream>>>(
-                            vt, si, ext, ni, nj, g.ks, tiles_i, f.ti, f.tj);
-                    check_launch("stencil engine: fused multistage launch");
+                    if constexpr (T::fused) {
+                        fused_kernel<NSPL, W, tlist<Stages...>, ViewsTuple, StageIns, NS>
+                            <<<dim3((unsigned)(tiles_i * tiles_j), 1, (unsigned)nk), 256, f.bytes, stream>>>(
+                                vt, si, ext, ni, nj, g.ks, tiles_i, f.ti, f.tj);
+                        check_launch("stencil engine: fused multistage launch");
+                    } else {
+                        vfused_kernel<Exec, NSPL, W, tlist<Stages...>, ViewsTuple, StageIns, NS>
+                            <<<dim3((unsigned)(tiles_i * tiles_j), 1, 1), 256, f.bytes, stream>>>(
+                                vt, si, ext, ni, nj, g.ks, tiles_i, f.ti, f.tj);
+                        check_launch("stencil engine: vertical multistage launch (tiles)");
+                    }
                 } else if constexpr (std::is_same<Exec, parallel>::value) {
                     int s = first;
                     auto one = [&](auto stage) {
@@ -1371,10 +1434,76 @@ namespace gt355 {
                 (init_stage_in<Ps>(vget<Is>(si), c), ...);
             }
 
+            // The tiled vertical form (vfused_kernel) is exact when no read can see another workgroup's write in an
+            // order that the sweep does not fix.  Workgroups share the points a stage on an extended box computes
+            // again for the neighbouring tiles, and they run at different levels.  For a placeholder in memory that
+            // the multistage writes:
+            //  * if several stages write it, each writes only its own columns (extent zero);
+            //  * a read at a level where a stage at or after the reader writes it (a later stage at the same level,
+            //    or any writer at a level ahead of the sweep) stays on the tile's own columns, and those writers
+            //    write only their own columns;
+            //  * a read of a level already written (an earlier stage at the same level, or a level behind the sweep)
+            //    is covered by the box on which the last such writer runs, so the workgroup wrote every value it
+            //    reads itself.
+            // Placeholders in LDS (ij_cached) are private to the workgroup.  An inout accessor writes at its own
+            // point and reads at the other levels of its k extent; a read of the point it writes (eval(x) += ...)
+            // counts as part of the write, so a stage that does so on an extended box is not caught here.
+            inline void check_vsync(int m, int dir, std::vector<stage_rt> const &st, std::set<int> const &lds_keys) {
+                auto zero_h = [](ext_rt const &e) {
+                    return !e.set || (e.imin == 0 && e.imax == 0 && e.jmin == 0 && e.jmax == 0);
+                };
+                auto covers = [](ext_rt const &o, ext_rt const &r) {
+                    return o.imin <= r.imin && o.imax >= r.imax && o.jmin <= r.jmin && o.jmax >= r.jmax;
+                };
+                std::vector<int> ids;
+                for (int q = 0; q < (int)st.size(); ++q)
+                    if (st[q].mss == m)
+                        ids.push_back(q);
+                std::map<int, std::vector<int>> writers;
+                for (int q : ids)
+                    for (auto const &a : st[q].acc)
+                        if (a.write && !lds_keys.count(a.key) &&
+                            (writers[a.key].empty() || writers[a.key].back() != q))
+                            writers[a.key].push_back(q);
+                for (auto const &w : writers)
+                    if (w.second.size() > 1)
+                        for (int q : w.second)
+                            if (!zero_h(st[q].ext))
+                                throw std::runtime_error("run: a vertical multistage with horizontal dependencies "
+                                                         "writes a field from several stages, one of them on an "
+                                                         "extended box");
+                for (int s : ids)
+                    for (auto const &a : st[s].acc) {
+                        if ((a.write && a.ext.kmin == 0 && a.ext.kmax == 0) || !writers.count(a.key))
+                            continue;
+                        auto const &ws = writers.at(a.key);
+                        ext_rt r{st[s].ext.imin + a.ext.imin, st[s].ext.imax + a.ext.imax, st[s].ext.jmin + a.ext.jmin,
+                            st[s].ext.jmax + a.ext.jmax, 0, 0, true};
+                        const bool same = !a.write && a.ext.kmin <= 0 && a.ext.kmax >= 0;
+                        const bool ahead = dir > 0 ? a.ext.kmax > 0 : a.ext.kmin < 0;
+                        const bool behind = dir > 0 ? a.ext.kmin < 0 : a.ext.kmax > 0;
+                        int last_before = -1, last_all = ws.back();
+                        for (int t : ws) {
+                            const bool after = ahead || (same && t >= s);
+                            if (after && (!zero_h(st[t].ext) || !zero_h(r)))
+                                throw std::runtime_error("run: a vertical multistage with horizontal dependencies "
+                                                         "reads a field that the same or a later stage writes, "
+                                                         "outside the workgroup's own columns");
+                            if (same && t < s)
+                                last_before = t;
+                        }
+                        if ((same && last_before >= 0 && !covers(st[last_before].ext, r)) ||
+                            (behind && !covers(st[last_all].ext, r)))
+                            throw std::runtime_error("run: a vertical multistage with horizontal dependencies reads, "
+                                                     "at columns its writer does not compute there, a field written "
+                                                     "at another level");
+                    }
+            }
+
             // LDS plan of a fused multistage: each ij-cached temporary gets a tile of the workgroup's TI x TJ block
             // extended by the temporary's extent; TJ shrinks until the tiles fit 64 KB of LDS
             template <class... Cs, class... Ins>
-            void plan_lds(tlist<Cs...>, tlist<Ins...>, int m, std::vector<stage_rt> const &st,
+            void plan_lds(tlist<Cs...>, tlist<Ins...>, int m, int dir, std::vector<stage_rt> const &st,
                 std::map<int, ext_rt> const &box, run_ctx &c) {
                 const int keys[] = {0, Cs::key...};
                 const int elems[] = {0, (int)sizeof(typename Cs::kind::data_t)...};
@@ -1406,6 +1535,12 @@ namespace gt355 {
                                         throw std::runtime_error("run: a stage reads a field at a horizontal offset "
                                                                  "that it or a later stage of the multistage writes");
                     }
+                }
+                if (dir != 0) {
+                    std::set<int> lds_keys;
+                    for (int q = 1; q <= (int)sizeof...(Cs); ++q)
+                        lds_keys.insert(keys[q]);
+                    check_vsync(m, dir, st, lds_keys);
                 }
                 fuse_rt f;
                 // tile columns: 64 less the widest rim of the multistage's LDS tiles and stage extents, so that every row
@@ -1459,8 +1594,9 @@ namespace gt355 {
                 int m = 0;
                 auto one = [&](auto mss) {
                     using T = mss_traits<decltype(mss)>;
-                    if constexpr (T::fused)
-                        plan_lds(typename T::caches(), typename staged_list<decltype(mss)>::type(), m, st, box, c);
+                    if constexpr (T::fused || T::vsync)
+                        plan_lds(typename T::ij_caches(), typename staged_list<decltype(mss)>::type(), m,
+                            T::vsync ? (std::is_same<typename T::exec, backward>::value ? -1 : 1) : 0, st, box, c);
                     ++m;
                 };
                 (one(Msses()), ...);
@@ -1567,8 +1703,8 @@ namespace gt355 {
                         box[a.key].merge(r);
                     }
                 }
-                // vertical multistages: no stage may read, at a horizontal offset, a field some stage of the same
-                // multistage writes (that would need synchronisation between columns)
+                // vertical multistages: a stage may read, at a horizontal offset, a field of the multistage only
+                // after an earlier stage wrote it (the multistage then runs tile by tile with barriers)
                 for (auto const &s : st) {
                     if (!s.vertical)
                         continue;
@@ -1576,12 +1712,12 @@ namespace gt355 {
                         if (!(a.ext.imin || a.ext.imax || a.ext.jmin || a.ext.jmax))
                             continue;
                         for (auto const &t : st)
-                            if (t.mss == s.mss)
+                            if (t.mss == s.mss && &t >= &s)
                                 for (auto const &b : t.acc)
                                     if (b.write && b.key == a.key)
                                         throw std::runtime_error(
-                                            "run: a vertical multistage reads, at a horizontal offset, a field it "
-                                            "writes; split it into a parallel multistage");
+                                            "run: a vertical multistage reads, at a horizontal offset, a field that "
+                                            "the same or a later stage writes");
                     }
                 }
                 return box;
@@ -1837,24 +1973,22 @@ namespace gt355 {
                 if constexpr (std::is_same<Exec, parallel>::value) {
                     (host_stage_box<NSPL, W, Stages>(c, args, st[first + (int)Ss].ext, g), ...);
                 } else {
+                    // level by level in the sweep's order, each stage over its whole box (the reference's naive
+                    // backend): the semantics of both vertical forms
                     auto views = std::make_tuple(stage_plhs<Stages>::views(c, args)...);
-                    ext_rt u;
-                    for (size_t n = 0; n < sizeof...(Stages); ++n)
-                        u.merge(st[first + (int)n].ext);
                     constexpr bool up = !std::is_same<Exec, backward>::value;
-                    for (int j = u.jmin; j < g.j_size + u.jmax; ++j)
-                        for (int i = u.imin; i < g.i_size + u.imax; ++i)
-                            for (int n = 0; n < g.k_size; ++n) {
-                                const int k = up ? n : g.k_size - 1 - n;
-                                auto one = [&](auto s) {
-                                    constexpr size_t S = decltype(s)::value;
-                                    ext_rt const &e = st[first + (int)S].ext;
-                                    if (i >= e.imin && i < g.i_size + e.imax && j >= e.jmin && j < g.j_size + e.jmax)
-                                        run_stage_at<typename std::tuple_element_t<S, std::tuple<Stages...>>::functor,
-                                            NSPL, W>(std::get<S>(views), i, j, k, g.ks);
-                                };
-                                (one(std::integral_constant<size_t, Ss>()), ...);
-                            }
+                    for (int n = 0; n < g.k_size; ++n) {
+                        const int k = up ? n : g.k_size - 1 - n;
+                        auto one = [&](auto s) {
+                            constexpr size_t S = decltype(s)::value;
+                            ext_rt const &e = st[first + (int)S].ext;
+                            for (int j = e.jmin; j < g.j_size + e.jmax; ++j)
+                                for (int i = e.imin; i < g.i_size + e.imax; ++i)
+                                    run_stage_at<typename std::tuple_element_t<S, std::tuple<Stages...>>::functor, NSPL,
+                                        W>(std::get<S>(views), i, j, k, g.ks);
+                        };
+                        (one(std::integral_constant<size_t, Ss>()), ...);
+                    }
                 }
             }
             template <int NSPL, bool W, class... Msses, class ArgsTuple>

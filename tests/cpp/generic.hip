@@ -463,6 +463,256 @@ namespace {
         }
         report("field bounds check of the extent analysis", threw);
     }
+    // ------------------------------------------------------------------------------------- horizontal dependencies
+    // Vertical multistages whose stages read an earlier stage's output at a horizontal offset (the reference marks
+    // such stages need_sync, stencil/core/need_sync.hpp:21-56, and its gpu backend synchronises the block there,
+    // stencil/gpu/make_kernel_fun.hpp:32-40).  Expected values: plain host loops in the reference's order (level by
+    // level in the sweep's direction, each stage over its whole extended box), the same operations in the same
+    // order, so FP64 matches bit for bit.
+    namespace hdep {
+        using full_t = axis<1>::full_interval;
+        // tmp = in + tmp(k - 1) / 2, from in at the first level
+        struct accumulate {
+            using tmp = inout_accessor<0, extent<0, 0, 0, 0, -1, 0>>;
+            using in = in_accessor<1>;
+            using param_list = make_param_list<tmp, in>;
+            template <class Evaluation>
+            GT_FUNCTION static void apply(Evaluation eval, full_t::modify<1, 0>) {
+                eval(tmp()) = eval(in()) + 0.5 * eval(tmp{0, 0, -1});
+            }
+            template <class Evaluation>
+            GT_FUNCTION static void apply(Evaluation eval, full_t::first_level) {
+                eval(tmp()) = eval(in());
+            }
+        };
+        // out = the mean of tmp's four neighbours, less half of tmp at (i + 1, k - 1)
+        struct spread {
+            using out = inout_accessor<0>;
+            using tmp = in_accessor<1, extent<-1, 1, -1, 1, -1, 0>>;
+            using param_list = make_param_list<out, tmp>;
+            template <class Evaluation>
+            GT_FUNCTION static void apply(Evaluation eval, full_t::modify<1, 0>) {
+                eval(out()) = 0.25 * (eval(tmp{-1, 0, 0}) + eval(tmp{1, 0, 0}) + eval(tmp{0, -1, 0}) + eval(tmp{0, 1, 0})) -
+                              0.5 * eval(tmp{1, 0, -1});
+            }
+            template <class Evaluation>
+            GT_FUNCTION static void apply(Evaluation eval, full_t::first_level) {
+                eval(out()) = 0.25 * (eval(tmp{-1, 0, 0}) + eval(tmp{1, 0, 0}) + eval(tmp{0, -1, 0}) + eval(tmp{0, 1, 0}));
+            }
+        };
+        // backward: mid = in + mid(k + 1) / 2, from in at the top
+        struct descend {
+            using mid = inout_accessor<0, extent<0, 0, 0, 0, 0, 1>>;
+            using in = in_accessor<1>;
+            using param_list = make_param_list<mid, in>;
+            template <class Evaluation>
+            GT_FUNCTION static void apply(Evaluation eval, full_t::modify<0, -1>) {
+                eval(mid()) = eval(in()) + 0.5 * eval(mid{0, 0, 1});
+            }
+            template <class Evaluation>
+            GT_FUNCTION static void apply(Evaluation eval, full_t::last_level) {
+                eval(mid()) = eval(in());
+            }
+        };
+        // out = mid(i + 1) - mid(i - 1) + mid(j + 1, k + 1) / 4
+        struct gradient {
+            using out = inout_accessor<0>;
+            using mid = in_accessor<1, extent<-1, 1, 0, 1, 0, 1>>;
+            using param_list = make_param_list<out, mid>;
+            template <class Evaluation>
+            GT_FUNCTION static void apply(Evaluation eval, full_t::modify<0, -1>) {
+                eval(out()) = eval(mid{1, 0, 0}) - eval(mid{-1, 0, 0}) + 0.25 * eval(mid{0, 1, 1});
+            }
+            template <class Evaluation>
+            GT_FUNCTION static void apply(Evaluation eval, full_t::last_level) {
+                eval(out()) = eval(mid{1, 0, 0}) - eval(mid{-1, 0, 0});
+            }
+        };
+        struct twice {
+            using t = inout_accessor<0>;
+            using in = in_accessor<1>;
+            using param_list = make_param_list<t, in>;
+            template <class Evaluation>
+            GT_FUNCTION static void apply(Evaluation eval) {
+                eval(t()) = 2.0 * eval(in());
+            }
+        };
+        // acc = the sum of t's four neighbours + acc(k - 1) / 2 + w(i + 1, j + 1)
+        struct gather {
+            using acc = inout_accessor<0, extent<0, 0, 0, 0, -1, 0>>;
+            using t = in_accessor<1, extent<-1, 1, -1, 1>>;
+            using w = in_accessor<2, extent<0, 1, 0, 1>>;
+            using param_list = make_param_list<acc, t, w>;
+            template <class Evaluation>
+            GT_FUNCTION static void apply(Evaluation eval, full_t::modify<1, 0>) {
+                eval(acc()) = eval(t{1, 0, 0}) + eval(t{-1, 0, 0}) + eval(t{0, 1, 0}) + eval(t{0, -1, 0}) +
+                              0.5 * eval(acc{0, 0, -1}) + eval(w{1, 1, 0});
+            }
+            template <class Evaluation>
+            GT_FUNCTION static void apply(Evaluation eval, full_t::first_level) {
+                eval(acc()) = eval(t{1, 0, 0}) + eval(t{-1, 0, 0}) + eval(t{0, 1, 0}) + eval(t{0, -1, 0}) + eval(w{1, 1, 0});
+            }
+        };
+        // reads tmp one level ahead of the sweep at a neighbour: rejected
+        struct peek_ahead {
+            using out = inout_accessor<0>;
+            using tmp = in_accessor<1, extent<0, 1, 0, 0, 0, 1>>;
+            using param_list = make_param_list<out, tmp>;
+            template <class Evaluation>
+            GT_FUNCTION static void apply(Evaluation eval, full_t::modify<0, -1>) {
+                eval(out()) = eval(tmp{1, 0, 1});
+            }
+            template <class Evaluation>
+            GT_FUNCTION static void apply(Evaluation eval, full_t::last_level) {
+                eval(out()) = eval(tmp{1, 0, 0});
+            }
+        };
+        // on the extended box the next stage needs, reads out at k - 1, which the next stage writes on its own
+        // columns only: rejected
+        struct lagged {
+            using tmp = inout_accessor<0>;
+            using out = in_accessor<1, extent<0, 0, 0, 0, -1, 0>>;
+            using param_list = make_param_list<tmp, out>;
+            template <class Evaluation>
+            GT_FUNCTION static void apply(Evaluation eval, full_t::modify<1, 0>) {
+                eval(tmp()) = eval(out{0, 0, -1});
+            }
+            template <class Evaluation>
+            GT_FUNCTION static void apply(Evaluation eval, full_t::first_level) {
+                eval(tmp()) = 0.0;
+            }
+        };
+    } // namespace hdep
+
+    void horizontal_dependencies(int ni, int nj, int nk) {
+        const int h = 2, n0 = ni + 2 * h, n1 = nj + 2 * h;
+        auto g = halo_grid(ni, nj, nk, h);
+        auto in = seeded<double>(n0, n1, nk, h, 91, 1, -1, 1);
+        auto w = seeded<double>(n0, n1, nk, h, 91, 2, -1, 1);
+        auto b = storage::builder<storage::gpu>.type<double>().dimensions(n0, n1, nk).halos(h, h, 0);
+        const std::vector<double> vin = mirror(in), vw = mirror(w);
+        const auto lay = in->const_host_view();
+        const int64_t s1 = lay.stride[1], s2 = lay.stride[2];
+        // grid coordinates (origin at the first interior point) to the mirror's index
+        auto at = [&](int i, int j, int k) { return (size_t)((i + h) + (j + h) * s1 + k * s2); };
+        auto same = [&](const char *name, store<double> const &ds, std::vector<double> const &e) {
+            const std::vector<double> got = mirror(ds);
+            size_t bad = 0, first = 0;
+            for (int k = 0; k < nk; ++k)
+                for (int j = 0; j < n1; ++j)
+                    for (int i = 0; i < n0; ++i) {
+                        const size_t n = (size_t)(i + j * s1 + k * s2); // row padding excluded
+                        if (std::memcmp(&got[n], &e[n], sizeof(double)) != 0 && bad++ == 0)
+                            first = n;
+                    }
+            if (bad)
+                std::printf("  %s: %zu values differ, first at %zu: %.17g, expected %.17g\n", name, bad, first, got[first],
+                    e[first]);
+            return bad == 0;
+        };
+        char label[160];
+        // forward: a recurrence in a temporary on the box the next stage reads at +-1 in i and j
+        {
+            auto out = b.value(-7.).build();
+            run([](auto out, auto in) {
+                GT_DECLARE_TMP(double, tmp);
+                return execute_forward().stage(hdep::accumulate(), tmp, in).stage(hdep::spread(), out, tmp);
+            },
+                gfx950<>(), g, out, in);
+            std::vector<double> tmp(vin.size(), 0.0), e(vin.size(), -7.);
+            for (int k = 0; k < nk; ++k) {
+                for (int j = -1; j < nj + 1; ++j)
+                    for (int i = -1; i < ni + 1; ++i)
+                        tmp[at(i, j, k)] = k == 0 ? vin[at(i, j, k)] : vin[at(i, j, k)] + 0.5 * tmp[at(i, j, k - 1)];
+                for (int j = 0; j < nj; ++j)
+                    for (int i = 0; i < ni; ++i) {
+                        const double m = 0.25 * (tmp[at(i - 1, j, k)] + tmp[at(i + 1, j, k)] + tmp[at(i, j - 1, k)] +
+                                                    tmp[at(i, j + 1, k)]);
+                        e[at(i, j, k)] = k == 0 ? m : m - 0.5 * tmp[at(i + 1, j, k - 1)];
+                    }
+            }
+            std::snprintf(label, sizeof label,
+                "vertical multistage, forward: temporary read at +-1 in i and j by the next stage %dx%dx%d", ni, nj, nk);
+            report(label, same("out", out, e));
+        }
+        // backward: a user field written on the extended box (its halo) and read at offsets and at k + 1
+        {
+            auto mid = b.value(3.).build(), out = b.value(-7.).build();
+            run([](auto out, auto mid, auto in) {
+                return execute_backward().stage(hdep::descend(), mid, in).stage(hdep::gradient(), out, mid);
+            },
+                gfx950<>(), g, out, mid, in);
+            std::vector<double> em(vin.size(), 3.), e(vin.size(), -7.);
+            for (int k = nk - 1; k >= 0; --k) {
+                for (int j = 0; j < nj + 1; ++j)
+                    for (int i = -1; i < ni + 1; ++i)
+                        em[at(i, j, k)] = k == nk - 1 ? vin[at(i, j, k)] : vin[at(i, j, k)] + 0.5 * em[at(i, j, k + 1)];
+                for (int j = 0; j < nj; ++j)
+                    for (int i = 0; i < ni; ++i) {
+                        const double d = em[at(i + 1, j, k)] - em[at(i - 1, j, k)];
+                        e[at(i, j, k)] = k == nk - 1 ? d : d + 0.25 * em[at(i, j + 1, k + 1)];
+                    }
+            }
+            std::snprintf(label, sizeof label,
+                "vertical multistage, backward: field written on its halo, read at offsets and k + 1 %dx%dx%d", ni, nj,
+                nk);
+            report(label, same("mid", mid, em) && same("out", out, e));
+        }
+        // forward: an ij_cached temporary (LDS) and a staged read-only field, with a recurrence in the output
+        {
+            auto acc = b.value(-7.).build();
+            run([](auto acc, auto in, auto w) {
+                GT_DECLARE_TMP(double, t);
+                return execute_forward().ij_cached(t).stage(hdep::twice(), t, in).stage(hdep::gather(), acc, t, w);
+            },
+                gfx950<>(), g, acc, in, w);
+            std::vector<double> t(vin.size(), 0.0), e(vin.size(), -7.);
+            for (int k = 0; k < nk; ++k) {
+                for (int j = -1; j < nj + 1; ++j)
+                    for (int i = -1; i < ni + 1; ++i)
+                        t[at(i, j, k)] = 2.0 * vin[at(i, j, k)];
+                for (int j = 0; j < nj; ++j)
+                    for (int i = 0; i < ni; ++i) {
+                        double v = t[at(i + 1, j, k)] + t[at(i - 1, j, k)] + t[at(i, j + 1, k)] + t[at(i, j - 1, k)];
+                        if (k > 0)
+                            v = v + 0.5 * e[at(i, j, k - 1)];
+                        e[at(i, j, k)] = v + vw[at(i + 1, j + 1, k)];
+                    }
+            }
+            std::snprintf(label, sizeof label,
+                "vertical multistage, forward: ij_cached temporary and staged field at offsets %dx%dx%d", ni, nj, nk);
+            report(label, same("acc", acc, e));
+        }
+        // what the tiles cannot run exactly is rejected before any launch
+        auto rejects = [&](const char *name, auto spec, auto const &... fields) {
+            bool threw = false;
+            try {
+                run(spec, gfx950<>(), g, fields...);
+            } catch (std::runtime_error const &x) {
+                threw = std::strstr(x.what(), "horizontal dependencies") != nullptr;
+                if (!threw)
+                    std::printf("  unexpected message: %s\n", x.what());
+            }
+            report(name, threw);
+        };
+        if (ni == 70) {
+            auto out = b.build();
+            rejects("vertical multistage: a neighbour's value one level ahead of the sweep is rejected",
+                [](auto out, auto in) {
+                    GT_DECLARE_TMP(double, tmp);
+                    return execute_forward().stage(hdep::accumulate(), tmp, in).stage(hdep::peek_ahead(), out, tmp);
+                },
+                out, in);
+            rejects("vertical multistage: an extended box reading a level behind that only its own columns wrote is "
+                    "rejected",
+                [](auto out) {
+                    GT_DECLARE_TMP(double, tmp);
+                    return execute_forward().stage(hdep::lagged(), tmp, out).stage(hdep::spread(), out, tmp);
+                },
+                out);
+        }
+    }
+
     // ------------------------------------------------------------------------------------- k-caches, axis<N>
     // the unit tests' fields (test_kcache_*.cpp: in(i, j, k) = i + j + k + 1 on a 6 x 6 grid, axis<3> intervals of 2, 6
     // and 2 levels, no halo) and their expected columns
@@ -1168,6 +1418,8 @@ int main(int argc, char **argv) {
         expandable_and_errors();
         kcache_tests();
         kparallel_tests();
+        horizontal_dependencies(70, 37, 19);
+        horizontal_dependencies(130, 70, 24);
         whole_axis_access(12, 33, 61);
         whole_axis_access(23, 11, 43);
         precision_policy(23, 11, 43);

@@ -3,7 +3,7 @@ copy, laplacian, the four-stage hdiff with ij-cached temporaries, the forward / 
 and k-cached temporaries, the tridiagonal solve with expressions and inout fields, simple_hori_diff with j-only metric
 fields, stencil functions (horizontal_diffusion_fused and the five variations of horizontal_diffusion_functions),
 positional, multistage fusion through a temporary, expandable_run, the k-cache unit tests over axis<3>, the k-parallel
-tests over axis<2> / axis<3>, whole_axis_access (a renamed dimension read by a 4-D accessor), the precision policy,
+tests over axis<2> / axis<3>, vertical multistages with horizontal dependencies between stages, whole_axis_access (a renamed dimension read by a 4-D accessor), the precision policy,
 temporaries on two streams and user boundary functors -- run on the GPU and checked against the oracle bit for bit
 or the reference's known answers (tests/cpp/generic.hip, built by the package build).
 
