@@ -137,6 +137,9 @@ SIGNATURES = {
     "gt355_batch_tune": [_P, _I, _P, ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_double)],
     "gt355_batch_get_config": [_P, ctypes.POINTER(ctypes.c_int32)],
     "gt355_batch_set_config": [_P, ctypes.POINTER(ctypes.c_int32)],
+    "gt355_batch_begin": [_P],
+    "gt355_batch_flush": [_P],
+    "gt355_batch_end": [_P],
 }
 _RESTYPES = {"gt355_last_error": ctypes.c_char_p, "gt355_version": ctypes.c_char_p}
 

@@ -33,11 +33,14 @@ class BatchDomain(ctypes.Structure):
     """gt355_batch_domain."""
 
     _fields_ = [(n, _abi.Field) for n in FIELDS] + [
-        ("ni", ctypes.c_int32), ("nj", ctypes.c_int32), ("nk", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+        ("ni", ctypes.c_int32), ("nj", ctypes.c_int32), ("nk", ctypes.c_int32), ("phases", ctypes.c_int32)]
 
 
 class Batch:
-    """A work table over ``members`` (dicts of the eight DataStores of FIELDS plus ``grid``) and its launch."""
+    """A work table over ``members`` (dicts of the eight DataStores of FIELDS plus ``grid``) and its launch.
+
+    A member may carry ``phases``: 0 (both phases), 1 (the stencil phase alone: u_stage = hdiff(in, coeff)) or 2 (the
+    vertical phase alone: vadv reading u_stage); fields its phases do not use, and ``lap``, may then be None."""
 
     def __init__(self, members, precision=stencils.WIDEN, wg_per_cu=2):
         self.members = list(members)  # keeps the stores alive while the table points at them
@@ -45,8 +48,10 @@ class Batch:
         for d, m in enumerate(self.members):
             g = m["grid"]
             for n in FIELDS:
-                setattr(arr[d], n, m[n].sid(g.origin))
+                if m.get(n) is not None:
+                    setattr(arr[d], n, m[n].sid(g.origin))
             arr[d].ni, arr[d].nj, arr[d].nk = g.i_size, g.j_size, g.k_size
+            arr[d].phases = m.get("phases", 0)
         h = ctypes.c_void_p()
         _abi.call("gt355_batch_create", precision, arr, len(self.members), wg_per_cu, ctypes.byref(h))
         self.handle = h
@@ -96,6 +101,36 @@ class Batch:
 
     def points(self):
         return sum(m["grid"].i_size * m["grid"].j_size * m["grid"].k_size for m in self.members)
+
+
+class deferred:
+    """A deferred-run scope (gt355_batch_begin / gt355_batch_end) on ``stream`` (a torch stream, a raw handle or
+    None): the hdiff, vadv and hdiff_vadv calls made on that stream inside the ``with`` block are recorded and run
+    at its end as persistent batches, with the results of running them one after another.
+
+        with deferred(stream):
+            for d in domains:
+                stencils.hdiff(...); stencils.vadv(...)
+    """
+
+    def __init__(self, stream=None):
+        # the library's Stream (.handle), a torch stream (.cuda_stream) or a raw handle
+        self.handle = getattr(stream, "handle", getattr(stream, "cuda_stream", stream))
+
+    def __enter__(self):
+        _abi.call("gt355_batch_begin", self.handle)
+        return self
+
+    def flush(self):
+        """Runs the calls recorded so far; the scope stays open."""
+        _abi.call("gt355_batch_flush", self.handle)
+
+    def __exit__(self, exc_type, exc, tb):
+        if exc_type is None:
+            _abi.call("gt355_batch_end", self.handle)
+        else:
+            _abi.lib().gt355_batch_end(self.handle)  # already unwinding: the scope closes, its status is dropped
+        return False
 
 
 def draw_sizes(seed, n=64, ij=(48, 384), k=(40, 137)):

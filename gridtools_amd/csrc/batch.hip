@@ -285,7 +285,8 @@ namespace gt355 {
             int32_t dom, i0, j0, k;
         };
 
-        // The stencil phase: laplacian (stored as lap) and hdiff (stored as u_stage) of every domain, one register
+        // The stencil phase: laplacian (stored as lap where the domain has one) and hdiff (stored as u_stage) of every
+        // domain in the phase, one register
         // sweep per item, items taken by wavefronts with a static stride; the item order (domain, level, row block,
         // strip) keeps the wavefronts in flight on neighbouring strips, whose rim rows they share through L2.
         template <class ST, class CT, int JB, int P>
@@ -299,8 +300,13 @@ namespace gt355 {
                 // global-address-space views of the table's pointers (global_* rather than flat_* accesses)
                 const gptr<ST> ust = (gptr<ST>)dm.u_stage, lap = (gptr<ST>)dm.lap;
                 const gptr<const ST> in = (gptr<const ST>)dm.in, co = (gptr<const ST>)dm.coeff;
-                hdiff_sweep_wave<ST, CT, JB, P, true, gptr<ST>, gptr<const ST>>(ust, in, co, lap, dm.ni, dm.nj, dm.sj,
-                    dm.sk, dm.sj, dm.sk, dm.sj, dm.sk, item.i0, item.j0, item.k);
+                // the laplacian is stored where the domain has a lap field (a uniform branch per item)
+                if (dm.lap)
+                    hdiff_sweep_wave<ST, CT, JB, P, true, gptr<ST>, gptr<const ST>>(ust, in, co, lap, dm.ni, dm.nj,
+                        dm.sj, dm.sk, dm.sj, dm.sk, dm.sj, dm.sk, item.i0, item.j0, item.k);
+                else
+                    hdiff_sweep_wave<ST, CT, JB, P, false, gptr<ST>, gptr<const ST>>(ust, in, co, ust, dm.ni, dm.nj,
+                        dm.sj, dm.sk, dm.sj, dm.sk, dm.sj, dm.sk, item.i0, item.j0, item.k);
             }
         }
     } // namespace
@@ -353,7 +359,7 @@ namespace gt355 {
             };
             mix(impl), mix(dt), mix(precision), mix(n);
             for (int k = 0; k < n; ++k)
-                mix(d[k].ni), mix(d[k].nj), mix(d[k].nk);
+                mix(d[k].ni), mix(d[k].nj), mix(d[k].nk), mix(d[k].phases);
             return h;
         }
         inline int batch_ct(const gt355_batch *bt) {
@@ -363,11 +369,24 @@ namespace gt355 {
     } // namespace
 } // namespace gt355
 
+namespace gt355 {
+    // gt355_batch_create with the form given (0: the GT355_TUNE_batch_impl knob); the deferred-run scope builds
+    // its batches in the two-phase form whatever the knob says
+    int batch_create_form(int precision, const gt355_batch_domain *domains, int ndomains, int wg_per_cu,
+        gt355_batch **out, int form);
+}
+
 extern "C" int gt355_batch_create(int precision, const gt355_batch_domain *domains, int ndomains, int wg_per_cu,
     gt355_batch **out) {
+    return batch_create_form(precision, domains, ndomains, wg_per_cu, out, 0);
+}
+
+int gt355::batch_create_form(int precision, const gt355_batch_domain *domains, int ndomains, int wg_per_cu,
+    gt355_batch **out, int form) {
     GT355_CHECK(out && domains && ndomains > 0, "gt355_batch_create: bad arguments");
     *out = nullptr;
-    const int dt = domains[0].in.dtype;
+    // the dtype: of the first field the first domain's phases use
+    const int dt = domains[0].phases == 2 ? domains[0].u_stage.dtype : domains[0].in.dtype;
     GT355_CHECK(dt == GT355_F64 || dt == GT355_F32, "gt355_batch_create: dtype %d", dt);
     GT355_CHECK(precision == GT355_NATIVE || (precision == GT355_WIDEN && dt == GT355_F32),
         "gt355_batch_create: precision %d with dtype %d", precision, dt);
@@ -376,7 +395,7 @@ extern "C" int gt355_batch_create(int precision, const gt355_batch_domain *domai
     // the strip form (default) or the tile form (GT355_TUNE_batch_impl=1)
     // the form (GT355_TUNE_batch_impl): two-phase by default (measured fastest on MI355X: 64 domains of seed 2024,
     // FP32 storage / FP64 compute, 2.41 ms per timestep against 2.73-2.87 ms for the single-launch strip form)
-    const int impl_p = tune_param("batch_impl", 3);
+    const int impl_p = form ? form : tune_param("batch_impl", 3);
     const int impl = impl_p == 1 ? 1 : impl_p == 3 ? 3 : 2;
     // register depth of the strip form (GT355_TUNE_batch_nk: 80 or 32)
     const int nk_reg = tune_param("batch_nk", 80) == 32 ? 32 : 80;
@@ -386,42 +405,50 @@ extern "C" int gt355_batch_create(int precision, const gt355_batch_domain *domai
     std::vector<hitem> hitems[3];
     for (int d = 0; d < ndomains; ++d) {
         const gt355_batch_domain &b = domains[d];
-        const int ni = b.ni, nj = b.nj, nk = b.nk;
-        GT355_CHECK(ni > 0 && nj > 0 && nk >= 2 && nk <= BMAXK, "gt355_batch_create: domain %d size %dx%dx%d", d, ni,
-            nj, nk);
+        const int ni = b.ni, nj = b.nj, nk = b.nk, ph = b.phases;
+        GT355_CHECK(ph >= 0 && ph <= 2, "gt355_batch_create: domain %d: phases %d", d, ph);
+        GT355_CHECK(ph == 0 || impl == 3, "gt355_batch_create: domain %d: one-phase domains need the two-phase form",
+            d);
+        const bool sten = ph != 2, vert = ph != 1;
+        GT355_CHECK(ni > 0 && nj > 0 && nk >= (vert ? 2 : 1) && (nk <= BMAXK || !vert),
+            "gt355_batch_create: domain %d size %dx%dx%d", d, ni, nj, nk);
+        const bool has_lap = sten && b.lap.origin != nullptr;
+        GT355_CHECK(has_lap || !sten || impl == 3, "gt355_batch_create: domain %d: lap is required by this form", d);
         int rc;
-        if ((rc = check_field(&b.in, "in", dt, -2, ni + 2, -2, nj + 2, 0, nk)) ||
-            (rc = check_field(&b.coeff, "coeff", dt, 0, ni, 0, nj, 0, nk)) ||
-            (rc = check_field(&b.lap, "lap", dt, 0, ni, 0, nj, 0, nk)) ||
+        if ((sten && (rc = check_field(&b.in, "in", dt, -2, ni + 2, -2, nj + 2, 0, nk))) ||
+            (sten && (rc = check_field(&b.coeff, "coeff", dt, 0, ni, 0, nj, 0, nk))) ||
+            (has_lap && (rc = check_field(&b.lap, "lap", dt, 0, ni, 0, nj, 0, nk))) ||
             (rc = check_field(&b.u_stage, "u_stage", dt, 0, ni, 0, nj, 0, nk)) ||
-            (rc = check_field(&b.utens_stage, "utens_stage", dt, 0, ni, 0, nj, 0, nk)) ||
-            (rc = check_field(&b.wcon, "wcon", dt, 0, ni + 1, 0, nj, 0, nk)) ||
-            (rc = check_field(&b.u_pos, "u_pos", dt, 0, ni, 0, nj, 0, nk)) ||
-            (rc = check_field(&b.utens, "utens", dt, 0, ni, 0, nj, 0, nk)))
+            (vert && (rc = check_field(&b.utens_stage, "utens_stage", dt, 0, ni, 0, nj, 0, nk))) ||
+            (vert && (rc = check_field(&b.wcon, "wcon", dt, 0, ni + 1, 0, nj, 0, nk))) ||
+            (vert && (rc = check_field(&b.u_pos, "u_pos", dt, 0, ni, 0, nj, 0, nk))) ||
+            (vert && (rc = check_field(&b.utens, "utens", dt, 0, ni, 0, nj, 0, nk))))
             return rc;
-        const gt355_field *same[] = {&b.coeff, &b.lap, &b.u_stage, &b.utens_stage, &b.u_pos, &b.utens};
+        // every used field but wcon shares one layout (the layout of u_stage, which both phases use)
+        const gt355_field *same[] = {sten ? &b.in : nullptr, sten ? &b.coeff : nullptr, has_lap ? &b.lap : nullptr,
+            vert ? &b.utens_stage : nullptr, vert ? &b.u_pos : nullptr, vert ? &b.utens : nullptr};
         for (const gt355_field *f : same)
-            GT355_CHECK(f->stride[1] == b.in.stride[1] && f->stride[2] == b.in.stride[2],
-                "gt355_batch_create: domain %d: every field but wcon must share the layout of in", d);
-        GT355_CHECK(b.u_stage.origin != b.in.origin, "gt355_batch_create: domain %d: u_stage aliases in", d);
+            GT355_CHECK(!f || (f->stride[1] == b.u_stage.stride[1] && f->stride[2] == b.u_stage.stride[2]),
+                "gt355_batch_create: domain %d: every field but wcon must share one layout", d);
+        GT355_CHECK(!sten || b.u_stage.origin != b.in.origin, "gt355_batch_create: domain %d: u_stage aliases in", d);
         bdom &m = doms[d];
-        m.in = b.in.origin;
-        m.coeff = b.coeff.origin;
-        m.wcon = b.wcon.origin;
-        m.u_pos = b.u_pos.origin;
-        m.utens = b.utens.origin;
-        m.lap = b.lap.origin;
+        m.in = sten ? b.in.origin : nullptr;
+        m.coeff = sten ? b.coeff.origin : nullptr;
+        m.wcon = vert ? b.wcon.origin : nullptr;
+        m.u_pos = vert ? b.u_pos.origin : nullptr;
+        m.utens = vert ? b.utens.origin : nullptr;
+        m.lap = has_lap ? b.lap.origin : nullptr;
         m.u_stage = b.u_stage.origin;
-        m.utens_stage = b.utens_stage.origin;
-        m.sj = b.in.stride[1];
-        m.sk = b.in.stride[2];
-        m.wsj = b.wcon.stride[1];
-        m.wsk = b.wcon.stride[2];
+        m.utens_stage = vert ? b.utens_stage.origin : nullptr;
+        m.sj = b.u_stage.stride[1];
+        m.sk = b.u_stage.stride[2];
+        m.wsj = vert ? b.wcon.stride[1] : 0;
+        m.wsk = vert ? b.wcon.stride[2] : 0;
         m.ni = ni;
         m.nj = nj;
         m.nk = nk;
         m.tiles_i = (ni + BTI - 1) / BTI;
-        if (impl == 3) {
+        if (impl == 3 && sten) {
             // the stencil phase's item tables for every row-block height the tuner may choose
             for (int h = 0; h < 3; ++h) {
                 const int jb = HJB_OF[h];
@@ -437,7 +464,9 @@ extern "C" int gt355_batch_create(int precision, const gt355_batch_domain *domai
             GT355_CHECK(((int64_t)(nk - 1) * m.sk + (int64_t)(nj + 3) * m.sj + ni + 4) * es < lim &&
                             ((int64_t)(nk - 1) * m.wsk + (int64_t)(nj + 1) * m.wsj + ni + 1) * es < lim,
                 "gt355_batch_create: domain %d spans 4 GiB or more", d);
-            if (impl == 3 && tune_param("batch_flat", 0) != 0) {
+            if (!vert) {
+                // a stencil-phase-only domain has no vertical items
+            } else if (impl == 3 && tune_param("batch_flat", 0) != 0) {
                 // the vertical phase on 64 consecutive entries of the domain's column list per item (no partial
                 // strips at row ends; disjoint, so each column has one owner).  Measured slower than row strips on
                 // 64 domains of seed 2024 (2.47 against 2.14 ms per timestep): an item's rows start at any column
@@ -490,11 +519,11 @@ extern "C" int gt355_batch_create(int precision, const gt355_batch_domain *domai
     const size_t item_bytes = impl >= 2 ? sizeof(sitem) * sitems.size() : sizeof(bitem) * items.size();
     const void *item_src = impl >= 2 ? (const void *)sitems.data() : (const void *)items.data();
     hipError_t e = hipMalloc(&bt->doms, sizeof(bdom) * doms.size());
-    if (e == hipSuccess)
+    if (e == hipSuccess && item_bytes)
         e = hipMalloc(&bt->items, item_bytes);
     if (e == hipSuccess)
         e = hipMemcpy(bt->doms, doms.data(), sizeof(bdom) * doms.size(), hipMemcpyHostToDevice);
-    if (e == hipSuccess)
+    if (e == hipSuccess && item_bytes)
         e = hipMemcpy(bt->items, item_src, item_bytes, hipMemcpyHostToDevice);
     for (int h = 0; h < 3; ++h) {
         if (e == hipSuccess && !hitems[h].empty())
@@ -509,7 +538,7 @@ extern "C" int gt355_batch_create(int precision, const gt355_batch_domain *domai
     bt->rows = maxnk + BPAST;
     const size_t ct = (dt == GT355_F64 || precision == GT355_WIDEN) ? 8 : 4;
     const size_t per_wg = impl >= 2 ? (size_t)stail_rows(nk_reg) * 128 : 2 * (size_t)bt->rows * 256;
-    if (e == hipSuccess)
+    if (e == hipSuccess && bt->gridmax > 0)
         e = hipMalloc(&bt->scratch, ct * per_wg * (size_t)bt->gridmax);
     if (e != hipSuccess) {
         (void)hipFree(bt->doms);
@@ -525,6 +554,7 @@ extern "C" int gt355_batch_create(int precision, const gt355_batch_domain *domai
 }
 
 extern "C" int gt355_batch_run(gt355_batch *bt, double dtr_stage, void *stream) {
+    GT355_DEFER_FLUSH(stream);
     GT355_CHECK(bt != nullptr, "gt355_batch_run: null batch");
     void *scr = bt->scratch;
     hipStream_t s = (hipStream_t)stream;
@@ -553,6 +583,8 @@ extern "C" int gt355_batch_run(gt355_batch *bt, double dtr_stage, void *stream) 
             if (rc)
                 return rc;
         }
+        if (bt->nitems == 0)
+            return GT355_OK; // stencil-phase domains only
         const int deep_nk = bt->deep_nk;
 #define GT355_BATCH_STRIP(NK, WPE, HD)                                                                          \
     if (bt->dtype == GT355_F64)                                                                                 \
@@ -637,6 +669,7 @@ namespace {
 } // namespace
 
 extern "C" int gt355_batch_tune(gt355_batch *bt, int reps, void *stream, int *wg_per_cu, double *best_us) {
+    GT355_DEFER_FLUSH(stream);
     GT355_CHECK(bt != nullptr, "gt355_batch_tune: null batch");
     GT355_CHECK(reps >= 1, "gt355_batch_tune: reps %d", reps);
     hipStream_t s = (hipStream_t)stream;

@@ -268,6 +268,7 @@ using namespace gt355;
 
 extern "C" int gt355_bc_apply(int kind, int dtype, void *const *dst, void *const *src, int nfields, double value,
     const int32_t halo[3][5], const int64_t stride[3], uint32_t dir_mask, void *stream) {
+    GT355_DEFER_FLUSH(stream);
     GT355_CHECK(halo != nullptr, "gt355_bc_apply: null halo");
     int rc;
     if ((rc = check_halo(halo, "gt355_bc_apply")))
@@ -306,6 +307,7 @@ extern "C" int gt355_bc_apply(int kind, int dtype, void *const *dst, void *const
 
 extern "C" int gt355_halo_fill(int dtype, void *const *fields, int nfields, const int32_t halo[3][5],
     const int64_t stride[3], const int32_t mode[3], void *stream) {
+    GT355_DEFER_FLUSH(stream);
     GT355_CHECK(halo != nullptr && mode != nullptr, "gt355_halo_fill: null argument");
     int rc;
     if ((rc = check_halo(halo, "gt355_halo_fill")))
@@ -390,16 +392,19 @@ static int halo_pack_impl(bool unpack, int dtype, void *const *fields, int nfiel
 
 extern "C" int gt355_halo_pack(int dtype, void *const *fields, int nfields, const int32_t halo[3][5],
     const int64_t stride[3], uint32_t dir_mask, void *buffer, void *stream) {
+    GT355_DEFER_FLUSH(stream);
     return halo_pack_impl(false, dtype, fields, nfields, halo, stride, dir_mask, buffer, stream);
 }
 
 extern "C" int gt355_halo_unpack(int dtype, void *const *fields, int nfields, const int32_t halo[3][5],
     const int64_t stride[3], uint32_t dir_mask, const void *buffer, void *stream) {
+    GT355_DEFER_FLUSH(stream);
     return halo_pack_impl(true, dtype, fields, nfields, halo, stride, dir_mask, const_cast<void *>(buffer), stream);
 }
 
 extern "C" int gt355_halo_exchange_local(int dtype, void *const *fields, int nfields, const int32_t halo[3][5],
     const int64_t stride[3], uint32_t dir_mask, void *buffer, void *stream) {
+    GT355_DEFER_FLUSH(stream);
     const char *who = "gt355_halo_exchange_local";
     GT355_CHECK(halo && buffer, "%s: null argument", who);
     int rc;

@@ -352,6 +352,7 @@ using namespace gt355;
 extern "C" int gt355_simple_hori_diff(int precision, const gt355_field *out, const gt355_field *in,
     const gt355_field *crlato, const gt355_field *crlatu, const gt355_field *coeff, int ni, int nj, int nk,
     void *stream) {
+    GT355_DEFER_FLUSH(stream);
     GT355_CHECK(ni >= 0 && nj >= 0 && nk >= 0, "gt355_simple_hori_diff: negative size");
     GT355_CHECK(out && in && crlato && crlatu && coeff, "gt355_simple_hori_diff: null field");
     const int dtype = in->dtype;
@@ -379,6 +380,7 @@ extern "C" int gt355_simple_hori_diff(int precision, const gt355_field *out, con
 
 extern "C" int gt355_prepare_tracers(int precision, const gt355_field *data, const gt355_field *data_nnow,
     int nmembers, const gt355_field *rho, int ni, int nj, int nk, void *stream) {
+    GT355_DEFER_FLUSH(stream);
     GT355_CHECK(ni >= 0 && nj >= 0 && nk >= 0 && nmembers >= 0, "gt355_prepare_tracers: negative size");
     GT355_CHECK(rho && (nmembers == 0 || (data && data_nnow)), "gt355_prepare_tracers: null field");
     const int dtype = rho->dtype;
@@ -431,6 +433,7 @@ extern "C" int gt355_prepare_tracers(int precision, const gt355_field *data, con
 
 extern "C" int gt355_copy_fields(const gt355_field *out, const gt355_field *in, int nfields, int ni, int nj, int nk,
     void *stream) {
+    GT355_DEFER_FLUSH(stream);
     GT355_CHECK(ni >= 0 && nj >= 0 && nk >= 0 && nfields >= 0, "gt355_copy_fields: negative size");
     GT355_CHECK(nfields == 0 || (out && in), "gt355_copy_fields: null field");
     if (ni == 0 || nj == 0 || nk == 0 || nfields == 0)
@@ -472,6 +475,7 @@ extern "C" int gt355_copy_fields(const gt355_field *out, const gt355_field *in, 
 extern "C" int gt355_extended_4d(const double phi[16], const double psi[16], const gt355_field *result,
     const int64_t result_extra[3], const gt355_field *f, const int64_t f_extra[3], const gt355_field *jac,
     int64_t jac_extra, int ni, int nj, int nk, void *stream) {
+    GT355_DEFER_FLUSH(stream);
     GT355_CHECK(ni >= 0 && nj >= 0 && nk >= 0, "gt355_extended_4d: negative size");
     GT355_CHECK(phi && psi && result && result_extra && f && f_extra && jac, "gt355_extended_4d: null argument");
     const int dtype = result->dtype;

@@ -60,6 +60,25 @@ namespace gt355 {
     // Resolves, on the current device, the device address of gt355_vadv_noalloc's static scratch (vadv.hip).
     int noalloc_setup();
 
+    // The deferred-run scope (defer.hip).  defer_active: a scope records the calling thread's calls on `stream`;
+    // defer_hdiff / defer_vadv record a checked call; defer_flush runs what the scope on `stream` recorded (every
+    // other entry point that launches, copies or waits on a stream calls it first, keeping the recorded order);
+    // defer_flush_all does so for every scope of the thread (before device-wide waits and blocking copies);
+    // release_deferred frees the kept batches (gt355_finalize).
+    bool defer_active(void *stream);
+    int defer_hdiff(int precision, const gt355_field *out, const gt355_field *in, const gt355_field *coeff, int ni,
+        int nj, int nk, void *stream);
+    int defer_vadv(int precision, const gt355_field *utens_stage, const gt355_field *u_stage, const gt355_field *wcon,
+        const gt355_field *u_pos, const gt355_field *utens, double dtr, int ni, int nj, int nk, void *stream);
+    int defer_flush(void *stream);
+    int defer_flush_all();
+    void release_deferred();
+#define GT355_DEFER_FLUSH(stream)                                                                                \
+    do {                                                                                                         \
+        if (int rc_defer_ = ::gt355::defer_flush(stream))                                                        \
+            return rc_defer_;                                                                                    \
+    } while (0)
+
     // After a launch: report a launch-configuration error as a status; in the debug launch mode also wait for the
     // kernel and report its failure here.
     inline int check_launch(const char *what) {

@@ -336,6 +336,12 @@ extern "C" int gt355_hdiff_vadv(int precision, const gt355_field *in, const gt35
     GT355_CHECK(utens_stage->stride[1] == u_pos->stride[1] && utens_stage->stride[2] == u_pos->stride[2] &&
                     utens_stage->stride[1] == utens->stride[1] && utens_stage->stride[2] == utens->stride[2],
         "gt355_hdiff_vadv: utens_stage, u_pos and utens must share one layout");
+    if (defer_active(stream)) {
+        // inside a deferred-run scope: recorded as its hdiff and vadv parts (the two-launch composition)
+        if ((rc = defer_hdiff(precision, u_stage, in, coeff, ni, nj, nk, stream)))
+            return rc;
+        return defer_vadv(precision, utens_stage, u_stage, wcon, u_pos, utens, dtr_stage, ni, nj, nk, stream);
+    }
     hipStream_t s = (hipStream_t)stream;
     // the two-launch sequence of the reference backend (default: on MI355X at 512x512x80 it takes 300 us in FP64
     // and 180 us in the precision form, the fused launch 337 us and 268 us, because the fused kernel holds

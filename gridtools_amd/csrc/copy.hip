@@ -50,6 +50,7 @@ using namespace gt355;
 
 extern "C" int gt355_convert(int dst_dtype, void *dst, const int64_t dst_stride[3], int src_dtype, const void *src,
     const int64_t src_stride[3], const int64_t lengths[3], void *stream) {
+    GT355_DEFER_FLUSH(stream);
     GT355_CHECK(dst && src && dst_stride && src_stride && lengths, "gt355_convert: null argument");
     GT355_CHECK((dst_dtype == GT355_F64 || dst_dtype == GT355_F32) && (src_dtype == GT355_F64 || src_dtype == GT355_F32),
         "gt355_convert: dtypes %d <- %d", dst_dtype, src_dtype);
@@ -80,6 +81,7 @@ extern "C" int gt355_convert(int dst_dtype, void *dst, const int64_t dst_stride[
 }
 
 extern "C" int gt355_copy(const gt355_field *out, const gt355_field *in, int ni, int nj, int nk, void *stream) {
+    GT355_DEFER_FLUSH(stream);
     GT355_CHECK(out && in, "gt355_copy: null field");
     GT355_CHECK(ni >= 0 && nj >= 0 && nk >= 0, "gt355_copy: negative size");
     if (ni == 0 || nj == 0 || nk == 0)

@@ -314,6 +314,8 @@ extern "C" int gt355_hdiff(int precision, const gt355_field *out, const gt355_fi
     int rc = check_hdiff_args(precision, out, in, coeff, ni, nj, nk);
     if (rc || ni == 0 || nj == 0 || nk == 0)
         return rc;
+    if (defer_active(stream)) // inside a deferred-run scope: recorded, run at its flush
+        return defer_hdiff(precision, out, in, coeff, ni, nj, nk, stream);
     hipStream_t s = (hipStream_t)stream;
     if (in->dtype == GT355_F64)
         return dispatch_hdiff<double, double>(out, in, coeff, ni, nj, nk, s);
@@ -324,6 +326,7 @@ extern "C" int gt355_hdiff(int precision, const gt355_field *out, const gt355_fi
 
 extern "C" int gt355_tune_hdiff(int precision, const gt355_field *out, const gt355_field *in, const gt355_field *coeff,
     int ni, int nj, int nk, int reps, void *stream, int32_t best[4], double *best_us) {
+    GT355_DEFER_FLUSH(stream);
     int rc = check_hdiff_args(precision, out, in, coeff, ni, nj, nk);
     if (rc)
         return rc;
@@ -436,6 +439,7 @@ extern "C" int gt355_tune_hdiff(int precision, const gt355_field *out, const gt3
 
 extern "C" int gt355_laplacian(int precision, const gt355_field *out, const gt355_field *in, int ni, int nj, int nk,
     void *stream) {
+    GT355_DEFER_FLUSH(stream);
     GT355_CHECK(ni >= 0 && nj >= 0 && nk >= 0, "gt355_laplacian: negative size");
     GT355_CHECK(out && in, "gt355_laplacian: null field");
     if (ni == 0 || nj == 0 || nk == 0)

@@ -148,6 +148,7 @@ using namespace gt355;
 
 extern "C" int gt355_transform_layout(int elem_size, void *dst, const void *src, int ndims, const int64_t *dims,
     const int64_t *dst_strides, const int64_t *src_strides, void *stream) {
+    GT355_DEFER_FLUSH(stream);
     GT355_CHECK(dst && src && dims && dst_strides && src_strides, "gt355_transform_layout: null argument");
     GT355_CHECK(ndims >= 1 && ndims <= LT_MAXD, "gt355_transform_layout: %d dimensions (1..%d)", ndims, LT_MAXD);
     GT355_CHECK(elem_size == 1 || elem_size == 2 || elem_size == 4 || elem_size == 8 || elem_size == 16,

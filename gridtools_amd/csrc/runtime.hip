@@ -141,7 +141,10 @@ int gt355_init(int device) {
 }
 
 int gt355_finalize(void) {
+    if (int rc = defer_flush_all())
+        return rc;
     GT355_HIP(hipDeviceSynchronize());
+    release_deferred();
     release_pools();
     std::lock_guard<std::mutex> lock(g_mutex);
     for (auto &f : g_flush) {
@@ -235,37 +238,47 @@ int gt355_free(void *ptr) {
 }
 
 int gt355_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+    if (int rc = defer_flush_all())
+        return rc;
     GT355_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
     return GT355_OK;
 }
 
 int gt355_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+    if (int rc = defer_flush_all())
+        return rc;
     GT355_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
     return GT355_OK;
 }
 
 int gt355_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream) {
+    GT355_DEFER_FLUSH(stream);
     GT355_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
     return GT355_OK;
 }
 
 int gt355_memset(void *dst, int value, size_t bytes, void *stream) {
+    GT355_DEFER_FLUSH(stream);
     GT355_HIP(hipMemsetAsync(dst, value, bytes, (hipStream_t)stream));
     return GT355_OK;
 }
 
 int gt355_synchronize(void) {
+    if (int rc = defer_flush_all())
+        return rc;
     GT355_HIP(hipDeviceSynchronize());
     return GT355_OK;
 }
 
 int gt355_stream_synchronize(void *stream) {
+    GT355_DEFER_FLUSH(stream);
     GT355_HIP(hipStreamSynchronize((hipStream_t)stream));
     return GT355_OK;
 }
 
 int gt355_fill_uniform(int dtype, void *base, const int64_t lengths[3], const int64_t stride[3], uint64_t seed,
     uint64_t field_id, double a, double b, void *stream) {
+    GT355_DEFER_FLUSH(stream);
     GT355_CHECK(base && lengths && stride, "gt355_fill_uniform: null argument");
     GT355_CHECK(dtype == GT355_F64 || dtype == GT355_F32, "gt355_fill_uniform: dtype %d", dtype);
     for (int d = 0; d < 3; ++d)
@@ -312,6 +325,7 @@ int gt355_event_destroy(void *event) {
 
 int gt355_event_record(void *event, void *stream) {
     GT355_CHECK(event != nullptr, "gt355_event_record: null event");
+    GT355_DEFER_FLUSH(stream);
     GT355_HIP(hipEventRecord((hipEvent_t)event, (hipStream_t)stream));
     return GT355_OK;
 }
@@ -330,6 +344,7 @@ int gt355_event_elapsed_ms(void *start, void *stop, float *ms) {
 
 int gt355_graph_begin(void *stream) {
     GT355_CHECK(stream != nullptr, "gt355_graph_begin: capture needs a non-null stream");
+    GT355_DEFER_FLUSH(stream); // calls recorded before the capture run before it
     GT355_HIP(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal));
     set_capturing(true);
     return GT355_OK;
@@ -337,6 +352,7 @@ int gt355_graph_begin(void *stream) {
 
 int gt355_graph_end(void *stream, void **graph_exec) {
     GT355_CHECK(stream != nullptr && graph_exec != nullptr, "gt355_graph_end: null argument");
+    GT355_DEFER_FLUSH(stream); // calls recorded during the capture belong to the graph
     hipGraph_t g = nullptr;
     set_capturing(false);
     GT355_HIP(hipStreamEndCapture((hipStream_t)stream, &g));
@@ -351,6 +367,7 @@ int gt355_graph_end(void *stream, void **graph_exec) {
 
 int gt355_graph_launch(void *graph_exec, void *stream) {
     GT355_CHECK(graph_exec != nullptr, "gt355_graph_launch: null graph");
+    GT355_DEFER_FLUSH(stream);
     GT355_HIP(hipGraphLaunch((hipGraphExec_t)graph_exec, (hipStream_t)stream));
     return GT355_OK;
 }
@@ -373,6 +390,7 @@ int gt355_range_pop(void) {
 }
 
 int gt355_mall_flush(void *stream) {
+    GT355_DEFER_FLUSH(stream);
     // 1 GiB: four times the 256 MB Infinity Cache of MI355X.
     const size_t bytes = size_t(1) << 30;
     int dev = current_device();

@@ -829,6 +829,12 @@ namespace {
         return rc;
     if ((rc = check_field(utens, "utens", dtype, 0, ni, 0, nj, 0, nk)))
         return rc;
+    // inside a deferred-run scope gt355_vadv is recorded and runs at the scope's flush; the no-allocation entry
+    // point always runs at once (the flush may build a batch), after what the scope recorded before it
+    if (!noalloc && defer_active(stream))
+        return defer_vadv(precision, utens_stage, u_stage, wcon, u_pos, utens, dtr_stage, ni, nj, nk, stream);
+    if (noalloc && (rc = defer_flush(stream)))
+        return rc;
     hipStream_t s = (hipStream_t)stream;
     void *buf = nullptr;
     const size_t bytes = noalloc ? NOALLOC_SCRATCH_BYTES : 0;

@@ -316,10 +316,14 @@ int gt355_graph_destroy(void *graph_exec);
  * gt355_batch_create builds the work tables (longest columns first) and the per-workgroup scratch; gt355_batch_run
  * launches the persistent grids (no counters or memsets, so a run may be captured into a graph once the batch
  * exists).  Each batch owns its ccol/dcol scratch: two runs of ONE batch must not overlap in time, runs of
- * different batches may. */
+ * different batches may.
+ * In the two-phase form a domain may take part in one phase only (`phases`: 0 both, 1 the stencil phase alone,
+ * computing u_stage = hdiff(in, coeff) like gt355_hdiff; 2 the vertical phase alone, reading u_stage like
+ * gt355_vadv); fields a domain's phases do not use may be null (origin NULL), and `lap` may be null in the
+ * two-phase form (the laplacian is then not stored).  A stencil-phase-only domain may have any nk >= 1. */
 typedef struct gt355_batch_domain {
     gt355_field in, coeff, lap, u_stage, utens_stage, wcon, u_pos, utens;
-    int32_t ni, nj, nk, reserved;
+    int32_t ni, nj, nk, phases;
 } gt355_batch_domain;
 typedef struct gt355_batch gt355_batch;
 int gt355_batch_create(int precision, const gt355_batch_domain *domains, int ndomains, int wg_per_cu,
@@ -339,6 +343,26 @@ int gt355_batch_tune(gt355_batch *batch, int reps, void *stream, int *wg_per_cu,
  * persistent grid setting, vertical-phase 96-level depth}.  set_config applies one (the form must be the batch's). */
 int gt355_batch_get_config(const gt355_batch *batch, int32_t cfg[5]);
 int gt355_batch_set_config(gt355_batch *batch, const int32_t cfg[5]);
+
+/* A deferred-run scope (no counterpart in the reference, whose run() launches at once, stencil/frontend/run.hpp:
+ * 34-60; it generalises expandable_run, stencil/frontend/expandable_run.hpp:80-183, to domains of different sizes).
+ * Between gt355_batch_begin(stream) and gt355_batch_end(stream), calls of gt355_hdiff, gt355_vadv and
+ * gt355_hdiff_vadv on that stream (the same pointer, NULL included) from the calling thread are checked and
+ * recorded, not launched.  gt355_batch_end runs them as if they ran one after another in the recorded order: each
+ * call is placed in the first launch after every recorded call it conflicts with (one reads or writes bytes the
+ * other writes, by the fields' declared bounds), hdiff parts in stencil phases and vadv parts in vertical phases,
+ * and every phase with its calls' domains runs as one persistent launch of a two-phase batch (gt355_batch_create,
+ * grouped by dtype, precision and dtr_stage).  gt355_hdiff_vadv records as its hdiff and vadv parts (the
+ * two-launch composition's results).  A call whose fields do not share one layout, or whose depth the vertical
+ * phase does not hold (2..144 levels), runs on its own entry point at its place in the order.  The batches are
+ * built at the first end with a given list of calls and kept (up to 16 lists per thread, released by
+ * gt355_finalize), so a repeated scope launches only: run it once before capturing it into a graph.  Any other
+ * library call that launches on the scope's stream (the generic engine's runs included) first runs the calls
+ * recorded so far, keeping the order; gt355_batch_flush does the same explicitly and leaves the scope open.
+ * Fields of recorded calls must not be changed by other work before the calls run. */
+int gt355_batch_begin(void *stream);
+int gt355_batch_flush(void *stream);
+int gt355_batch_end(void *stream);
 int gt355_batch_destroy(gt355_batch *batch);
 
 /* ------------------------------------------------------------------ measurement */

@@ -155,6 +155,9 @@ namespace gt355 {
                     return;
                 const std::array<int, 3> need = {
                     halos[0].total_length(), halos[1].total_length(), halos[2].total_length()};
+                // calls a deferred-run scope recorded on this stream run first
+                if (gt355_batch_flush(stream) != GT355_OK)
+                    throw std::runtime_error(std::string("boundary::apply: ") + gt355_last_error());
                 user_bc_kernel<<<t.blocks, BLOCK, 0, (hipStream_t)stream>>>(bc, t, view_of(stores, need)...);
                 hipError_t e = hipGetLastError();
                 if (e != hipSuccess)

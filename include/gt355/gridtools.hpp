@@ -25,6 +25,7 @@
 #include <tuple>
 #include <type_traits>
 #include <utility>
+#include <utility>
 #include <vector>
 
 #include "../gt355.h"
@@ -653,8 +654,9 @@ namespace gt355 {
 
         // expandable_run<Factor>(spec, backend, grid, fields...) (stencil/frontend/expandable_run.hpp:150-183):
         // std::vector arguments are expanded member by member, scalar arguments are shared; every vector must
-        // have the same size.  Factor is kept for source compatibility; the members are launched back to back
-        // on the backend's stream (the batching form, gt355_batch_*, is the one-launch path).
+        // have the same size.  Factor is kept for source compatibility; the members run on the backend's stream
+        // inside a deferred-run scope, so members that run the hdiff or vertical advection kernels go out as one
+        // persistent launch per phase.
         // expandable_run of prepare_tracers: every member in one library call (one launch per 32 members) instead of
         // one run per member
         template <size_t Factor, class Be, class T>
@@ -677,6 +679,35 @@ namespace gt355 {
                 "expandable_run(prepare_tracers)");
         }
 
+        // A deferred-run scope (gt355_batch_begin / gt355_batch_end) on a stream: the hdiff and vertical advection
+        // runs on that stream between construction and end() -- tag specs, and spec lambdas the engine recognises
+        // as those kernels -- are recorded and then run together as persistent batches, with the results of running
+        // them one after another.  end() reports errors; the destructor ends an open scope and drops them.
+        class deferred {
+            void *m_stream;
+            bool m_open = true;
+
+          public:
+            explicit deferred(void *stream = nullptr) : m_stream(stream) {
+                check(gt355_batch_begin(stream), "deferred");
+            }
+            template <class Be, class = decltype(std::declval<Be>().stream)>
+            explicit deferred(Be const &be) : deferred(stream_of(be)) {}
+            deferred(deferred const &) = delete;
+            deferred &operator=(deferred const &) = delete;
+            void flush() { check(gt355_batch_flush(m_stream), "deferred: flush"); }
+            void end() {
+                if (m_open) {
+                    m_open = false;
+                    check(gt355_batch_end(m_stream), "deferred: end");
+                }
+            }
+            ~deferred() {
+                if (m_open)
+                    (void)gt355_batch_end(m_stream);
+            }
+        };
+
         template <size_t Factor, class Spec, class Be, int N, class... Args>
         void expandable_run(Spec spec, Be be, grid_t<N> const &g, Args const &... args) {
             static_assert(Factor > 0, "expandable_run: Factor must be positive");
@@ -686,8 +717,23 @@ namespace gt355 {
                     throw std::invalid_argument("expandable_run: vector arguments differ in size");
                 n = std::max(n, c);
             }
+            // the members' hdiff / vertical advection runs go out as persistent batches (a scope of its own unless
+            // the caller has one open on the stream)
+            struct scope_guard {
+                void *s;
+                bool own;
+                ~scope_guard() {
+                    if (own)
+                        (void)gt355_batch_end(s);
+                }
+            } guard{stream_of(be), false};
+            guard.own = gt355_batch_begin(guard.s) == GT355_OK;
             for (size_t m = 0; m < n; ++m)
                 run(spec, be, g, expandable_impl::member(args, m)...);
+            if (guard.own) {
+                guard.own = false;
+                check(gt355_batch_end(guard.s), "expandable_run: deferred runs");
+            }
         }
     } // namespace stencil
 

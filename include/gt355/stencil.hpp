@@ -2352,6 +2352,8 @@ namespace gt355 {
                 if (plan.kind != engine_::fast_kind::none && engine_::run_fast(plan, W, g, stream, args...))
                     return;
             }
+            // calls a deferred-run scope recorded on this stream run before the engine's own launches
+            check(gt355_batch_flush(stream), "run: deferred calls");
             engine_::plan_all(msses(), st, box, c);
             std::vector<void *> taken;
             engine_::alloc_all(msses(), c, box, g, stream, taken);

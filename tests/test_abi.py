@@ -247,3 +247,20 @@ def test_transform_layout_validates_without_a_device():
     # an empty box is a no-op
     assert lib.gt355_transform_layout(8, ctypes.c_void_p(0x1000), ctypes.c_void_p(0x2000), 2, arr(0, 5), arr(1, 4),
                                       arr(1, 4), None) == 0
+
+
+def test_deferred_scope_bookkeeping_without_a_device():
+    """gt355_batch_begin / flush / end: one scope per stream and thread; an empty scope launches nothing."""
+    lib = _abi.lib()
+    assert lib.gt355_batch_flush(None) == 0  # no scope open: nothing to run
+    assert lib.gt355_batch_end(None) == -1
+    assert "no scope" in _abi.last_error()
+    assert lib.gt355_batch_begin(None) == 0
+    assert lib.gt355_batch_begin(None) == -1
+    assert "already open" in _abi.last_error()
+    other = ctypes.c_void_p(0x1234)  # a scope is kept per stream handle; this one is never launched on
+    assert lib.gt355_batch_begin(other) == 0
+    assert lib.gt355_batch_flush(None) == 0
+    assert lib.gt355_batch_end(other) == 0
+    assert lib.gt355_batch_end(None) == 0
+    assert lib.gt355_batch_end(None) == -1
