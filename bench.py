@@ -12,6 +12,7 @@ dominant kernel (vadv FP64, algorithmic bytes / mean launch time from hipEvents 
 CPU baseline (the oracle port, OpenMP, on this host) on a bounded sample of the same workload.
 """
 import argparse
+import contextlib
 import json
 import os
 import statistics
@@ -457,6 +458,8 @@ def batch_leg(gt, nsteps=10, reps=3, seed=2024):
     # algorithmic bytes per point and timestep (FP32 storage): in, coeff, utens_stage, wcon, u_pos, utens read;
     # lap, u_stage, utens_stage written
     gbs = 9 * 4 * pts * nsteps / (ms * 1e-3) / 1e9
+    scoped = deferred_calls(bt, nsteps, reps, True)
+    direct = deferred_calls(bt, nsteps, reps, False)
     return {
         "workload": f"64 domains (seed {seed}), {pts / 1e6:.1f} M points, {nsteps}-timestep graph, two-phase form "
                     "(stencil-phase + vertical-phase launch per timestep, each over every domain)",
@@ -470,7 +473,44 @@ def batch_leg(gt, nsteps=10, reps=3, seed=2024):
         "tuned_config": [bt.batches[0].config(), bt.batches[1].config()],
         "tune_us": [round(us0, 1), round(us1, 1)],
         "workgroups": grid,
+        "per_domain_calls": {
+            "workload": "the same domains as one gt355_hdiff_vadv call per domain and timestep (no laplacian), "
+                        f"{nsteps} timesteps in a graph",
+            "ms_per_timestep_in_deferred_scope": round(scoped / nsteps, 4),
+            "ms_per_timestep_direct": round(direct / nsteps, 4),
+        },
     }
+
+
+def deferred_calls(bt, nsteps, reps, scoped):
+    """Milliseconds per graph of nsteps timesteps that call gt355_hdiff_vadv once per domain (u and u_stage swap
+    roles every step), inside a deferred-run scope per timestep (two persistent launches) or directly (two launches
+    per domain).  The graph is captured after one eager run, which builds the scope's batches."""
+    from gridtools_amd import stencils
+    from gridtools_amd.batch import deferred
+    from gridtools_amd.timestep import capture
+    from gridtools_amd.timing import Event, Stream
+
+    def steps(s):
+        for n in range(nsteps):
+            with deferred(s) if scoped else contextlib.nullcontext():
+                for f in bt.domains:
+                    a, b = (f["u"], f["v"]) if n % 2 == 0 else (f["v"], f["u"])
+                    stencils.hdiff_vadv(a, f["coeff"], b, f["utens_stage"], f["wcon"], f["u_pos"], f["utens"], bt.dtr,
+                                        f["grid"], precision=bt.batches[0].precision, stream=s.handle)
+
+    st = Stream()
+    steps(st)
+    g = capture(st, steps)
+    g.launch(st)
+    st.synchronize()
+    start, stop = Event(), Event()
+    start.record(st)
+    for _ in range(reps):
+        g.launch(st)
+    stop.record(st)
+    stop.synchronize()
+    return start.elapsed_ms(stop) / reps
 
 
 def large_leg(gt, nsteps=20, n=1536, nk=80):
