@@ -137,6 +137,10 @@ SIGNATURES = {
     "gt355_batch_tune": [_P, _I, _P, ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_double)],
     "gt355_batch_get_config": [_P, ctypes.POINTER(ctypes.c_int32)],
     "gt355_batch_set_config": [_P, ctypes.POINTER(ctypes.c_int32)],
+    "gt355_tune_lookup": [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64), _I, ctypes.POINTER(ctypes.c_int32),
+                          ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I)],
+    "gt355_tune_store": [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64), _I, ctypes.POINTER(ctypes.c_int32),
+                         ctypes.c_double],
     "gt355_batch_begin": [_P],
     "gt355_batch_flush": [_P],
     "gt355_batch_end": [_P],

@@ -284,6 +284,16 @@ namespace gt355 {
             return v;
         }
 
+    } // namespace
+
+    bool hdiff_cfg_instantiated(hdiff_cfg cfg) {
+        for (const hdiff_cfg &c : hdiff_candidates())
+            if (c.impl == cfg.impl && c.a == cfg.a && c.b == cfg.b && (cfg.impl != 0 || c.c == cfg.c))
+                return true;
+        return false;
+    }
+
+    namespace {
         int check_hdiff_args(int precision, const gt355_field *out, const gt355_field *in, const gt355_field *coeff,
             int ni, int nj, int nk) {
             GT355_CHECK(ni >= 0 && nj >= 0 && nk >= 0, "gt355_hdiff: negative size");

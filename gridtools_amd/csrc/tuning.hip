@@ -182,6 +182,79 @@ extern "C" int gt355_tune_load(const char *path, int *entries) {
     return GT355_OK;
 }
 
+// The table by kernel key.  "hdiff": shape {storage bytes, compute bytes, ni, nj, nk}, cfg {impl, a, b, c};
+// "batch": shape {signature, storage bytes, compute bytes, form}, cfg {hjb, hw, w, deep_nk}.
+namespace {
+    int tune_key_of(const char *key, const int64_t *shape, int n, const char *what, int *kind) {
+        GT355_CHECK(key != nullptr && shape != nullptr, "%s: null argument", what);
+        if (std::strcmp(key, "hdiff") == 0) {
+            GT355_CHECK(n == 5, "%s: an hdiff shape has 5 values (storage bytes, compute bytes, ni, nj, nk), not %d",
+                what, n);
+            *kind = 0;
+        } else if (std::strcmp(key, "batch") == 0) {
+            GT355_CHECK(n == 4, "%s: a batch shape has 4 values (signature, storage bytes, compute bytes, form), not %d",
+                what, n);
+            *kind = 1;
+        } else {
+            GT355_CHECK(false, "%s: unknown kernel key '%s' (hdiff, batch)", what, key);
+        }
+        return GT355_OK;
+    }
+} // namespace
+
+extern "C" int gt355_tune_lookup(const char *kernel_key, const int64_t *shape, int n, int32_t cfg[4], double *us,
+    int *found) {
+    int kind = 0;
+    if (int rc = tune_key_of(kernel_key, shape, n, "gt355_tune_lookup", &kind))
+        return rc;
+    GT355_CHECK(cfg != nullptr && found != nullptr, "gt355_tune_lookup: null output");
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    *found = 0;
+    if (kind == 0) {
+        auto it = g_hdiff.find(tune_key{(int)shape[0], (int)shape[1], (int)shape[2], (int)shape[3], (int)shape[4]});
+        if (it != g_hdiff.end()) {
+            const hdiff_cfg &c = it->second.cfg;
+            cfg[0] = c.impl, cfg[1] = c.a, cfg[2] = c.b, cfg[3] = c.c;
+            if (us)
+                *us = it->second.us;
+            *found = 1;
+        }
+    } else {
+        auto it = g_batch.find(batch_key{(uint64_t)shape[0], (int)shape[1], (int)shape[2], (int)shape[3]});
+        if (it != g_batch.end()) {
+            const batch_cfg &c = it->second.cfg;
+            cfg[0] = c.hjb, cfg[1] = c.hw, cfg[2] = c.w, cfg[3] = c.deep_nk;
+            if (us)
+                *us = it->second.us;
+            *found = 1;
+        }
+    }
+    return GT355_OK;
+}
+
+extern "C" int gt355_tune_store(const char *kernel_key, const int64_t *shape, int n, const int32_t cfg[4], double us) {
+    int kind = 0;
+    if (int rc = tune_key_of(kernel_key, shape, n, "gt355_tune_store", &kind))
+        return rc;
+    GT355_CHECK(cfg != nullptr, "gt355_tune_store: null configuration");
+    if (kind == 0) {
+        const hdiff_cfg c{cfg[0], cfg[1], cfg[2], cfg[3]};
+        GT355_CHECK(hdiff_cfg_instantiated(c), "gt355_tune_store: hdiff configuration impl=%d (%d,%d,%d) is not "
+                                                "instantiated", c.impl, c.a, c.b, c.c);
+        GT355_CHECK((shape[0] == 4 || shape[0] == 8) && (shape[1] == 4 || shape[1] == 8) && shape[2] > 0 &&
+                        shape[3] > 0 && shape[4] > 0,
+            "gt355_tune_store: bad hdiff shape");
+        record_hdiff_cfg((int)shape[0], (int)shape[1], (int)shape[2], (int)shape[3], (int)shape[4], c, us);
+    } else {
+        GT355_CHECK((cfg[0] == 8 || cfg[0] == 16 || cfg[0] == 32) && cfg[1] >= 1 && cfg[1] <= 8 && cfg[2] >= 1 &&
+                        cfg[2] <= 4 && cfg[3] >= 2,
+            "gt355_tune_store: bad batch configuration {%d, %d, %d, %d}", cfg[0], cfg[1], cfg[2], cfg[3]);
+        record_batch_cfg((uint64_t)shape[0], (int)shape[1], (int)shape[2], (int)shape[3],
+            batch_cfg{cfg[0], cfg[1], cfg[2], cfg[3]}, us);
+    }
+    return GT355_OK;
+}
+
 extern "C" int gt355_tune_env_refresh(void) {
     gt355::tune_env_refresh();
     return GT355_OK;

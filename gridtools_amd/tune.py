@@ -36,3 +36,21 @@ def load(path):
 
 def clear():
     _abi.call("gt355_tune_clear")
+
+
+def lookup(kernel_key, shape):
+    """The table's entry for ``kernel_key`` ("hdiff" or "batch") and ``shape`` (hdiff: storage bytes, compute
+    bytes, ni, nj, nk; batch: signature, storage bytes, compute bytes, form): (cfg list of 4, microseconds), or
+    None if there is none."""
+    sh = (ctypes.c_int64 * len(shape))(*shape)
+    cfg = (ctypes.c_int32 * 4)()
+    us, found = ctypes.c_double(), ctypes.c_int()
+    _abi.call("gt355_tune_lookup", kernel_key.encode(), sh, len(shape), cfg, ctypes.byref(us), ctypes.byref(found))
+    return (list(cfg), us.value) if found.value else None
+
+
+def store(kernel_key, shape, cfg, us=0.0):
+    """Records an entry (hdiff: cfg = [impl, a, b, c], an instantiated configuration; batch: [hjb, hw, w, deep_nk])
+    for the next launch of that shape."""
+    sh = (ctypes.c_int64 * len(shape))(*shape)
+    _abi.call("gt355_tune_store", kernel_key.encode(), sh, len(shape), (ctypes.c_int32 * 4)(*cfg), us)

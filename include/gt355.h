@@ -280,6 +280,14 @@ int gt355_event_elapsed_ms(void *start, void *stop, float *ms);
  * gt355_tune_save / gt355_tune_load (text, one entry per line). */
 int gt355_tune_hdiff(int precision, const gt355_field *out, const gt355_field *in, const gt355_field *coeff, int ni,
     int nj, int nk, int reps, void *stream, int32_t best[4], double *best_us);
+/* The tuning table by kernel key (the autotune cache lookup of the reference's backend template parameters,
+ * stencil/gpu/entry_point.hpp:147-150, made a run-time table).  "hdiff": shape = {storage bytes, compute bytes, ni,
+ * nj, nk}, cfg = {impl, a, b, c} as gt355_tune_hdiff returns it; "batch": shape = {domain-list signature, storage
+ * bytes, compute bytes, form}, cfg = {rows per stencil item, stencil workgroups per CU, grid setting, 96-level
+ * depth}.  lookup sets *found (and cfg, *us when found); store records an entry as the tuners do (an hdiff
+ * configuration must be an instantiated one), for the next launch of that shape. */
+int gt355_tune_lookup(const char *kernel_key, const int64_t *shape, int n, int32_t cfg[4], double *us, int *found);
+int gt355_tune_store(const char *kernel_key, const int64_t *shape, int n, const int32_t cfg[4], double us);
 int gt355_tune_save(const char *path);
 int gt355_tune_load(const char *path, int *entries);
 int gt355_tune_clear(void);

@@ -264,3 +264,29 @@ def test_deferred_scope_bookkeeping_without_a_device():
     assert lib.gt355_batch_end(other) == 0
     assert lib.gt355_batch_end(None) == 0
     assert lib.gt355_batch_end(None) == -1
+
+
+def test_tune_table_lookup_and_store_by_key_without_a_device(tmp_path):
+    """gt355_tune_lookup / gt355_tune_store: the tuning table by kernel key; a stored hdiff entry must be an
+    instantiated configuration and survives save / load."""
+    from gridtools_amd import tune
+
+    tune.clear()
+    shape = (4, 8, 256, 256, 80)
+    assert tune.lookup("hdiff", shape) is None
+    tune.store("hdiff", shape, [2, 8, 4, 0], 13.5)
+    assert tune.lookup("hdiff", shape) == ([2, 8, 4, 0], 13.5)
+    with pytest.raises(RuntimeError, match="not instantiated"):
+        tune.store("hdiff", shape, [2, 7, 4, 0])
+    with pytest.raises(RuntimeError, match="unknown kernel key"):
+        tune.lookup("vadv", shape)
+    bshape = (0x1234abcd5678, 4, 8, 3)
+    tune.store("batch", bshape, [16, 4, 2, 89], 2130.0)
+    out = tmp_path / "t.txt"
+    tune.save(out)
+    tune.clear()
+    assert tune.lookup("batch", bshape) is None
+    assert tune.load(out) == 2
+    assert tune.lookup("batch", bshape) == ([16, 4, 2, 89], 2130.0)
+    assert tune.lookup("hdiff", shape) == ([2, 8, 4, 0], 13.5)
+    tune.clear()
