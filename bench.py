@@ -121,6 +121,38 @@ def timed_steps(gt, sets, precision, steps, warmup):
     return total, hd_ms, va_ms
 
 
+def graphed_steps(gt, sets, precision, nsteps=40, reps=10):
+    """Milliseconds per step of a graph of nsteps steps (cycling over the field sets, the two launches of a step one
+    after the other on one stream), replayed reps times after one warm-up replay: the step without host launch
+    gaps."""
+    from gridtools_amd import stencils
+    from gridtools_amd.timestep import capture
+    from gridtools_amd.timing import Event, Stream
+
+    gh, gv = grids(gt)
+
+    def steps(st):
+        for s in range(nsteps):
+            hd, va = sets[s % len(sets)]
+            stencils.hdiff(hd["out"], hd["in"], hd["coeff"], gh, precision=precision, stream=st.handle)
+            stencils.vadv(va["utens_stage"], va["u_stage"], va["wcon"], va["u_pos"], va["utens"], 0.15, gv,
+                          precision=precision, stream=st.handle)
+
+    st = Stream()
+    gt_sync()
+    steps(st)
+    g = capture(st, steps)
+    g.launch(st)
+    st.synchronize()
+    start, stop = Event(), Event()
+    start.record(st)
+    for _ in range(reps):
+        g.launch(st)
+    stop.record(st)
+    stop.synchronize()
+    return start.elapsed_ms(stop) / (reps * nsteps)
+
+
 def autotune(gt, dtype, precision, shapes):
     """Runs the project's autotuner for hdiff on each (n, nk) shape (outside every timed region) on scratch fields
     of that shape; later launches of that shape use the recorded winner.  Returns {shape: (config, us)}."""
@@ -673,7 +705,7 @@ def main():
     # 100 untimed steps: after 20 the first timed steps still ran 2-3% slower than steady state
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--sets", type=int, default=4)
-    ap.add_argument("--legs", default="tune,fp64,fp32,copy,cartesian,generic,compose,batch,large,cpu",
+    ap.add_argument("--legs", default="tune,fp64,fp32,graph,copy,cartesian,generic,compose,batch,large,cpu",
                     help="comma list of legs: tune (hdiff autotuner first, untimed), fp64 (always run), fp32 "
                          "(precision form per kernel), copy (bandwidth "
                          "yardstick), cartesian (simple_hori_diff, prepare_tracers x11, copy_stencil_tuple x5, extended_4D, "
@@ -715,6 +747,9 @@ def main():
     ms_per_step = total / args.steps
     hd64 = kernel_stats("hdiff", hd_ms, 8)
     va64 = kernel_stats("vadv", va_ms, 8)
+    graphed = None
+    if "graph" in legs:
+        graphed = {"fp64": graphed_steps(gt, sets64, gt.NATIVE)}
     del sets64
 
     fp32 = None
@@ -730,7 +765,16 @@ def main():
             "hdiff": kernel_stats("hdiff", hd_ms32, 4),
             "vadv": kernel_stats("vadv", va_ms32, 4),
         }
+        if graphed is not None:
+            graphed["fp32_form"] = graphed_steps(gt, sets32, gt.WIDEN)
         del sets32
+    in_graph = None
+    if graphed is not None:
+        in_graph = {"workload": "the same steps captured in a graph (40 steps, replayed 10 times)"}
+        for form, v in graphed.items():
+            in_graph[form] = {"ms_per_step": round(v, 5), "gpts_per_s": round(P / (v * 1e-3) / 1e9, 3)}
+        if "fp32_form" in graphed:
+            in_graph["fp32_form_speedup_vs_fp64"] = round(graphed["fp64"] / graphed["fp32_form"], 3)
 
     copy = copy_leg(gt, args.steps, args.warmup, args.sets) if "copy" in legs else None
     cartesian = None
@@ -788,6 +832,7 @@ def main():
                       "hbm_frac": round(72 * P / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "autotuned_hdiff": tuned,
         "fp32_form": fp32,
+        "graphed_steps": in_graph,
         "copy_fp64": copy,
         "cartesian_stencils": cartesian,
         "generic_engine": generic,

@@ -9,8 +9,12 @@
 // where two parts conflict when one writes bytes the other reads or writes (the spans of their access boxes).
 // Running the slots in order then gives the results of the calls in recorded order.  The parts of a slot with
 // one dtype, precision and dtr run as one two-phase batch (batch.hip) whose domains take that phase only; a part
-// that the batch kernels cannot take (fields of different layouts, a depth outside 2..144 for the vertical phase)
-// runs on its own entry point in its slot.  Batches are kept per thread by their exact domain list, so a scope
+// alone in its group, or one the batch kernels cannot take (fields of different layouts, a depth outside 2..144
+// for the vertical phase), runs on its own entry point (the single-domain kernels, tuned per shape) in its slot.
+// The slots run one after another on the scope's stream.  (Running an independent vertical slot concurrently on a
+// forked side stream was measured slower: a vertical-phase wavefront holds a SIMD's whole register file, so
+// stencil wavefronts placed first delay its single round; hdiff + vadv at 256x256x80 FP64 took 72.6 us per step
+// overlapped against 63.3 us in sequence.)  Batches are kept per thread by their exact domain list, so a scope
 // that records the same calls again (a timestep loop) only launches.
 #include <hip/hip_runtime.h>
 
@@ -46,13 +50,26 @@ namespace gt355 {
             gt355_batch *batch;
         };
         constexpr size_t CACHE_MAX = 16;
+        // a thread's kept batches
+        struct thread_state {
+            std::list<cached> cache;
+        };
 
         thread_local std::vector<scope> t_scopes;
         thread_local bool t_running = false; // flushing: the entry points launch instead of recording
-        thread_local std::list<cached> *t_cache = nullptr;
-        // every thread's cache, so that gt355_finalize can release the batches
+        thread_local thread_state *t_state = nullptr;
+        // every thread's state, so that gt355_finalize can release the batches
         std::mutex g_mu;
-        std::vector<std::list<cached> *> g_caches;
+        std::vector<thread_state *> g_states;
+
+        thread_state &state() {
+            if (!t_state) {
+                t_state = new thread_state();
+                std::lock_guard<std::mutex> g(g_mu);
+                g_states.push_back(t_state);
+            }
+            return *t_state;
+        }
 
         scope *find_scope(void *stream) {
             for (scope &s : t_scopes)
@@ -145,11 +162,7 @@ namespace gt355 {
 
         // the batch for a domain list: from the calling thread's cache, else built and cached
         int batch_for(std::vector<gt355_batch_domain> const &doms, int precision, gt355_batch **out) {
-            if (!t_cache) {
-                t_cache = new std::list<cached>();
-                std::lock_guard<std::mutex> g(g_mu);
-                g_caches.push_back(t_cache);
-            }
+            std::list<cached> *t_cache = &state().cache;
             const int dev = current_device();
             std::vector<unsigned char> key(sizeof(int) * 2 + sizeof(gt355_batch_domain) * doms.size());
             std::memcpy(key.data(), &dev, sizeof(int));
@@ -183,6 +196,40 @@ namespace gt355 {
                 p.precision, &p.f[0], &p.f[1], &p.f[2], &p.f[3], &p.f[4], p.dtr, p.ni, p.nj, p.nk, stream);
         }
 
+        // the parts of slot s on `stream`: groups of one dtype, precision and dtr (the vertical phase takes dtr as
+        // a launch argument) as batches, a group of one part and a part the batch kernels cannot take directly
+        int run_slot(std::vector<part> const &parts, int s, void *stream) {
+            std::vector<int> done(parts.size(), 0);
+            int rc = GT355_OK;
+            for (size_t p = 0; p < parts.size() && rc == GT355_OK; ++p) {
+                if (parts[p].slot != s || done[p])
+                    continue;
+                std::vector<size_t> group{p};
+                done[p] = 1;
+                if (batchable(parts[p]))
+                    for (size_t q = p + 1; q < parts.size(); ++q) {
+                        const part &a = parts[p], &b = parts[q];
+                        if (b.slot != s || done[q] || !batchable(b) || b.precision != a.precision ||
+                            b.f[0].dtype != a.f[0].dtype || (a.kind == PART_VADV && b.dtr != a.dtr))
+                            continue;
+                        done[q] = 1;
+                        group.push_back(q);
+                    }
+                if (group.size() == 1) {
+                    rc = run_direct(parts[p], stream);
+                    continue;
+                }
+                std::vector<gt355_batch_domain> doms;
+                for (size_t q : group)
+                    doms.push_back(domain_of(parts[q]));
+                gt355_batch *b = nullptr;
+                rc = batch_for(doms, parts[p].precision, &b);
+                if (rc == GT355_OK)
+                    rc = gt355_batch_run(b, parts[p].kind == PART_VADV ? parts[p].dtr : 0.0, stream);
+            }
+            return rc;
+        }
+
         // runs and clears what the scope recorded
         int run_scope(scope &sc) {
             std::vector<part> parts;
@@ -200,32 +247,8 @@ namespace gt355 {
             }
             t_running = true;
             int rc = GT355_OK;
-            for (int s = 0; s < nslots && rc == GT355_OK; ++s) {
-                // groups of one dtype, precision and dtr (the vertical phase takes dtr as a launch argument)
-                std::vector<int> done(parts.size(), 0);
-                for (size_t p = 0; p < parts.size() && rc == GT355_OK; ++p) {
-                    if (parts[p].slot != s || done[p])
-                        continue;
-                    if (!batchable(parts[p])) {
-                        done[p] = 1;
-                        rc = run_direct(parts[p], sc.stream);
-                        continue;
-                    }
-                    std::vector<gt355_batch_domain> doms;
-                    for (size_t q = p; q < parts.size(); ++q) {
-                        const part &a = parts[p], &b = parts[q];
-                        if (b.slot != s || done[q] || !batchable(b) || b.precision != a.precision ||
-                            b.f[0].dtype != a.f[0].dtype || (a.kind == PART_VADV && b.dtr != a.dtr))
-                            continue;
-                        done[q] = 1;
-                        doms.push_back(domain_of(b));
-                    }
-                    gt355_batch *b = nullptr;
-                    rc = batch_for(doms, parts[p].precision, &b);
-                    if (rc == GT355_OK)
-                        rc = gt355_batch_run(b, parts[p].kind == PART_VADV ? parts[p].dtr : 0.0, sc.stream);
-                }
-            }
+            for (int s = 0; s < nslots && rc == GT355_OK; ++s)
+                rc = run_slot(parts, s, sc.stream);
             t_running = false;
             return rc;
         }
@@ -267,10 +290,10 @@ namespace gt355 {
 
     void release_deferred() {
         std::lock_guard<std::mutex> g(g_mu);
-        for (std::list<cached> *c : g_caches) {
-            for (cached &e : *c)
+        for (thread_state *t : g_states) {
+            for (cached &e : t->cache)
                 gt355_batch_destroy(e.batch);
-            c->clear();
+            t->cache.clear();
         }
     }
 } // namespace gt355

@@ -13,7 +13,7 @@ from test_gpu_batch import make_members
 pytestmark = pytest.mark.gpu
 
 SIZES = [(23, 11, 43), (70, 9, 81), (130, 37, 2), (64, 4, 137), (96, 33, 150), (1, 1, 3)]
-FIELDS = ("in_", "u_stage", "utens_stage")
+FIELDS = ("in_", "u_stage", "utens_stage", "lap")
 
 
 def program(stencils, members, precision, stream=None):
@@ -109,6 +109,61 @@ def test_deferred_scope_in_a_graph(gpu):
     for r in range(3):
         g.launch(st)
         program(stencils, b, 1)
+        st.synchronize()
+        gpu.synchronize()
+        same_state(a, b, f"replay {r}")
+
+
+@pytest.mark.parametrize("dtype,precision", [(np.float64, 0), (np.float32, 1)])
+def test_deferred_single_calls_and_flushes_in_order(gpu, dtype, precision):
+    """Scopes flushed after every independent hdiff / vadv pair (each slot holds one call, which runs on its own
+    single-domain entry point) and a dependent pair after them, eager and captured: the results equal the calls in
+    order, and so do three replays of the captured scope."""
+    from gridtools_amd import stencils
+    from gridtools_amd.batch import deferred
+    from gridtools_amd.timestep import capture
+    from gridtools_amd.timing import Stream
+
+    sizes = [(256, 256, 80), (100, 60, 96), (77, 130, 33)]
+    a = make_members(gpu, sizes, dtype)
+    b = make_members(gpu, sizes, dtype)
+    print(f"[case] independent phases {np.dtype(dtype).name} precision={precision}", flush=True)
+
+    def prog(ms, stream=None):
+        # hdiff of domain d with vadv of domain d+1 (independent), one pair per scope flush
+        n = len(ms)
+        for d in range(n):
+            h, v = ms[d], ms[(d + 1) % n]
+            stencils.hdiff(h["u_stage"], h["in_"], h["coeff"], h["grid"], precision=precision, stream=stream)
+            stencils.vadv(v["utens_stage"], v["in_"], v["wcon"], v["u_pos"], v["utens"], 0.15, v["grid"],
+                          precision=precision, stream=stream)
+            yield
+        # dependent: vadv reads the u_stage the hdiff writes
+        m = ms[0]
+        stencils.hdiff(m["lap"], m["in_"], m["coeff"], m["grid"], precision=precision, stream=stream)
+        stencils.vadv(m["utens_stage"], m["lap"], m["wcon"], m["u_pos"], m["utens"], 0.15, m["grid"],
+                      precision=precision, stream=stream)
+
+    def scoped(ms, st=None):
+        handle = st.handle if st is not None else None
+        with deferred(st) as sc:
+            for _ in prog(ms, handle):
+                sc.flush()
+
+    def direct(ms):
+        for _ in prog(ms):
+            pass
+
+    st = Stream()
+    scoped(a, st)
+    direct(b)
+    st.synchronize()
+    gpu.synchronize()
+    same_state(a, b, "eager")
+    g = capture(st, lambda s: scoped(a, s))
+    for r in range(3):
+        g.launch(st)
+        direct(b)
         st.synchronize()
         gpu.synchronize()
         same_state(a, b, f"replay {r}")
