@@ -225,9 +225,13 @@ namespace gt355 {
         struct sitem {
             int32_t dom, i0, j, flat;
         };
-        constexpr int SD = 4; // load-ahead depth of the strip sweep
+        // load-ahead depths of the strip sweep: 4 levels in the single-launch strip form and with 32 register
+        // levels (deeper rings spill there), 8 in the two-phase form's vertical phase, whose kernel then takes 476 of
+        // 512 registers without spills (its one wavefront per SIMD is latency-bound at 4: five loads per level,
+        // 5 MB in flight over the GPU)
+        constexpr int SD = 4, SDV = 8;
         // tail levels a wavefront's scratch holds, for a register part of NK levels
-        constexpr int stail_rows(int nk_reg) { return BMAXK - nk_reg + SD; }
+        constexpr int stail_rows(int nk_reg) { return BMAXK - nk_reg + SDV; }
 
         // The batching form on the strip sweep (strip.hpp): one wavefront per workgroup and per item, no LDS
         // tile, no barrier, ccol/dcol in registers for the first NK levels and in the wavefront's tail scratch
@@ -247,6 +251,7 @@ namespace gt355 {
         batch_strip_kernel(const bdom *__restrict__ doms, const sitem *__restrict__ items, int nitems, CT dtr,
             CT *__restrict__ scratch, int nxcd, int deep_nk) {
             constexpr bool DEEP = !HD && NK == 80 && sizeof(ST) == 4;
+            constexpr int KD = !HD && NK == 80 ? SDV : SD;
             __shared__ ST s_up[DEEP ? 96 : NK][64];
             CT *tail = scratch + (int64_t)blockIdx.x * (stail_rows(NK) * 128);
             const int wid = xcd_remap(blockIdx.x, gridDim.x, nxcd);
@@ -262,19 +267,19 @@ namespace gt355 {
                     // the vertical phase: columns of at most 48 or 64 levels run a register part of that depth
                     // (a uniform branch per item), not 80 steps with the levels past their top clamped
                     if (dm.nk <= 48)
-                        hv_strip<ST, CT, 48, SD, true, true, false, HD, !HD>(
+                        hv_strip<ST, CT, 48, KD, true, true, false, HD, !HD>(
                             f, dtr, dm.ni, dm.nj, dm.nk, item.i0, item.j, s_up, tail, flat);
                     else if (dm.nk <= 64)
-                        hv_strip<ST, CT, 64, SD, true, true, false, HD, !HD>(
+                        hv_strip<ST, CT, 64, KD, true, true, false, HD, !HD>(
                             f, dtr, dm.ni, dm.nj, dm.nk, item.i0, item.j, s_up, tail, flat);
                     else if (DEEP && dm.nk >= deep_nk)
-                        hv_strip<ST, CT, DEEP ? 96 : NK, SD, true, true, true, HD, !HD>(
+                        hv_strip<ST, CT, DEEP ? 96 : NK, KD, true, true, true, HD, !HD>(
                             f, dtr, dm.ni, dm.nj, dm.nk, item.i0, item.j, s_up, tail, flat);
                     else
-                        hv_strip<ST, CT, NK, SD, true, true, true, HD, !HD>(
+                        hv_strip<ST, CT, NK, KD, true, true, true, HD, !HD>(
                             f, dtr, dm.ni, dm.nj, dm.nk, item.i0, item.j, s_up, tail, flat);
                 } else {
-                    hv_strip<ST, CT, NK, SD, true, true, true, HD, !HD>(
+                    hv_strip<ST, CT, NK, KD, true, true, true, HD, !HD>(
                         f, dtr, dm.ni, dm.nj, dm.nk, item.i0, item.j, s_up, tail, flat);
                 }
             }
