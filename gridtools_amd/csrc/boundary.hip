@@ -11,9 +11,17 @@
 // at most 26 block offsets).  Inside a region points are numbered with i fastest in 32-bit arithmetic, and a
 // thread's (i, j, k) comes from two divisions by the region's extents done as multiply-high with host-computed
 // magic numbers, so no point pays a 64-bit division or a region search.  All fields of a call share the launch.
+//
+// The i faces of a halo fill write 3 values per row: alone they are partial 32-byte sectors, which the memory turns
+// into read-modify-writes.  When the fields are laid out by gt355_storage_layout (so the bytes between a row's right
+// halo and the next row's left halo are the layout's own row padding, zero since allocation and outside every
+// field), the row's right halo, that padding and the next row's left halo form one aligned run of whole sectors
+// (for ni * element size a multiple of 32 bytes), and extra workgroups of the same launch write each such run with
+// 16-byte stores: the halo values and zeros.
 #include <hip/hip_runtime.h>
 
 #include "common.hpp"
+#include "tuning.hpp"
 
 namespace gt355 {
     namespace {
@@ -84,9 +92,65 @@ namespace gt355 {
             double value;
         };
 
+        // The row-gap runs of a halo fill: rows [row0, row0 + nrows) of levels [k0, k0 + nk); run p of a level
+        // starts at column end_i + 1 of row row0 + p and holds the right halo (h1 values) of that row, the row
+        // padding and the left halo (h0 values) of the next row: len = stride_j - ni elements, in chunks of 16
+        // bytes.  Its workgroups follow the region table's.
+        struct gap_args {
+            int32_t row0, nrows, k0, h0, h1, len, cpr; // cpr: 16-byte chunks per run
+            uint32_t total;                            // chunks of every run (0: no gap workgroups)
+            fastdiv dcpr, druns;                       // division by cpr and by nrows - 1
+        };
+
+        template <class T>
+        __device__ __forceinline__ void gap_fill(const gap_args &g, const bc_args &a, int blk) {
+            constexpr int V = 16 / sizeof(T);
+            typedef T vec __attribute__((ext_vector_type(V)));
+            const uint32_t cid = (uint32_t)blk * BC_BLOCK + threadIdx.x;
+            if (cid >= g.total)
+                return;
+            const uint32_t run = div_by(cid, g.dcpr), c = cid - run * (uint32_t)g.cpr;
+            const uint32_t lev = div_by(run, g.druns), p = run - lev * (uint32_t)(g.nrows - 1);
+            const int r = g.row0 + (int)p, k = g.k0 + (int)lev;
+            const int len_i = a.end[0] - a.begin[0] + 1, len_j = a.end[1] - a.begin[1] + 1;
+            // the source row of a row: itself inside, clamped (mode 1) or wrapped (mode 2) in the j halo
+            auto srow = [&](int j) {
+                if (j < a.begin[1])
+                    return a.mode[1] == 1 ? a.begin[1] : j + len_j;
+                if (j > a.end[1])
+                    return a.mode[1] == 1 ? a.end[1] : j - len_j;
+                return j;
+            };
+            const int64_t base = (int64_t)(a.end[0] + 1) + (int64_t)r * a.s[1] + (int64_t)k * a.s[2] + c * V;
+            const int64_t right_src = (int64_t)srow(r) * a.s[1] + (int64_t)k * a.s[2];
+            const int64_t left_src = (int64_t)srow(r + 1) * a.s[1] + (int64_t)k * a.s[2];
+            for (int f = 0; f < a.nfields; ++f) {
+                T *fld = (T *)a.dst[f];
+                vec v;
+#pragma unroll
+                for (int e = 0; e < V; ++e) {
+                    const int el = (int)c * V + e;
+                    T x = T(0);
+                    if (el < g.h1) {
+                        const int col = a.end[0] + 1 + el;
+                        x = fld[right_src + (a.mode[0] == 1 ? a.end[0] : col - len_i)];
+                    } else if (el >= g.len - g.h0) {
+                        const int col = a.begin[0] - (g.len - el);
+                        x = fld[left_src + (a.mode[0] == 1 ? a.begin[0] : col + len_i)];
+                    }
+                    v[e] = x;
+                }
+                *(vec *)(fld + base) = v;
+            }
+        }
+
         // kind 0 zero, 1 value, 2 copy, 3 halo fill (mode per dimension)
         template <class T, int KIND>
-        __global__ void __launch_bounds__(BC_BLOCK) bc_kernel(region_table t, bc_args a) {
+        __global__ void __launch_bounds__(BC_BLOCK) bc_kernel(region_table t, bc_args a, gap_args g) {
+            if (KIND == 3 && (int)blockIdx.x >= t.blocks) {
+                gap_fill<T>(g, a, (int)blockIdx.x - t.blocks);
+                return;
+            }
             const region &r = t.r[block_region(t)];
             const uint32_t vol = (uint32_t)r.n[0] * (uint32_t)r.n[1] * (uint32_t)r.n[2];
             const uint32_t base = (uint32_t)((int)blockIdx.x - r.block0) * BC_CHUNK + threadIdx.x;
@@ -238,6 +302,84 @@ namespace gt355 {
             return GT355_OK;
         }
 
+        void add_region(region_table &t, const int32_t lo[3], const int32_t n[3], const int32_t dir[3]) {
+            region r{};
+            int64_t vol = 1;
+            for (int d = 0; d < 3; ++d) {
+                r.lo[d] = lo[d], r.n[d] = n[d], r.dir[d] = dir[d];
+                vol *= n[d];
+            }
+            if (vol == 0)
+                return;
+            r.start = r.src = t.total;
+            r.block0 = t.blocks;
+            r.d0 = make_fastdiv((uint32_t)r.n[0]);
+            r.d1 = make_fastdiv((uint32_t)r.n[0] * (uint32_t)r.n[1]);
+            t.blocks += (int)((vol + BC_CHUNK - 1) / BC_CHUNK);
+            t.total += vol;
+            t.r[t.count++] = r;
+        }
+
+        // The halo fill with the i faces as row-gap runs, if the call allows it: the i halo filled, no k halo
+        // regions, the logical i box exactly halo + interior + halo, strides of gt355_storage_layout (row padding
+        // owned by the layout), 32-byte alignment of every run (first interior column of each field, ni, the
+        // strides), at least two rows.  Then the table holds the j faces and the two end rows' i halos of each
+        // level, and g the runs between consecutive rows.
+        bool plan_gap_fill(const bc_args &a, int64_t es, const int32_t halo[3][5], const int32_t mode[3],
+            region_table &t, gap_args &g) {
+            const int32_t *hi = halo[0], *hj = halo[1], *hk = halo[2];
+            if (mode[0] == 0 || hi[0] == 0 || hi[1] == 0 || (mode[2] != 0 && (hk[0] || hk[1])))
+                return false;
+            if (hi[2] != hi[0] || hi[4] != hi[3] + hi[1] + 1)
+                return false;
+            const int64_t align = 128 / es, row = (hi[4] + align - 1) / align * align;
+            if (a.s[1] != row || a.s[2] != row * hj[4])
+                return false;
+            const int64_t ni = hi[3] - hi[2] + 1;
+            if ((ni * es) % 32 || (a.s[1] * es) % 32 || (a.s[2] * es) % 32)
+                return false;
+            for (int f = 0; f < a.nfields; ++f)
+                if (((uintptr_t)a.dst[f] + (uintptr_t)(hi[2] * es)) % 32)
+                    return false;
+            const int row0 = mode[1] ? hj[2] - hj[0] : hj[2], row1 = mode[1] ? hj[3] + hj[1] : hj[3];
+            const int nrows = row1 - row0 + 1, nk = hk[3] - hk[2] + 1;
+            if (nrows < 2)
+                return false;
+            const int64_t len = a.s[1] - ni, cpr = len * es / 16;
+            const int64_t total = cpr * (nrows - 1) * nk;
+            if (total >= (int64_t(1) << 31))
+                return false;
+            t = region_table{};
+            // the j faces (all interior columns of the j halo rows)
+            if (mode[1]) {
+                for (int dj = -1; dj <= 1; dj += 2) {
+                    int32_t lo[3], n[3];
+                    const int32_t dir[3] = {0, dj, 0};
+                    dim_range(hi, 0, 0, lo[0], n[0]);
+                    dim_range(hj, dj, 0, lo[1], n[1]);
+                    dim_range(hk, 0, 0, lo[2], n[2]);
+                    add_region(t, lo, n, dir);
+                }
+            }
+            // the left halo of the first row and the right halo of the last row of each level
+            {
+                const int32_t lo[3] = {hi[2] - hi[0], row0, hk[2]}, n[3] = {hi[0], 1, nk};
+                const int32_t dir[3] = {-1, row0 < hj[2] ? -1 : 0, 0};
+                add_region(t, lo, n, dir);
+            }
+            {
+                const int32_t lo[3] = {hi[3] + 1, row1, hk[2]}, n[3] = {hi[1], 1, nk};
+                const int32_t dir[3] = {1, row1 > hj[3] ? 1 : 0, 0};
+                add_region(t, lo, n, dir);
+            }
+            g = gap_args{};
+            g.row0 = row0, g.nrows = nrows, g.k0 = hk[2], g.h0 = hi[0], g.h1 = hi[1];
+            g.len = (int32_t)len, g.cpr = (int32_t)cpr, g.total = (uint32_t)total;
+            g.dcpr = make_fastdiv((uint32_t)cpr);
+            g.druns = make_fastdiv((uint32_t)(nrows - 1));
+            return true;
+        }
+
         int fill_args(bc_args &a, int dtype, void *const *dst, void *const *src, int nfields, const int32_t halo[3][5],
             const int64_t stride[3], const char *who) {
             GT355_CHECK(dtype == GT355_F64 || dtype == GT355_F32, "%s: dtype %d", who, dtype);
@@ -292,9 +434,9 @@ extern "C" int gt355_bc_apply(int kind, int dtype, void *const *dst, void *const
     const int nb = t.blocks;
 #define GT355_BC_LAUNCH(T)                                                  \
     switch (kind) {                                                         \
-    case 0: bc_kernel<T, 0><<<nb, BC_BLOCK, 0, s>>>(t, a); break;                \
-    case 1: bc_kernel<T, 1><<<nb, BC_BLOCK, 0, s>>>(t, a); break;                \
-    default: bc_kernel<T, 2><<<nb, BC_BLOCK, 0, s>>>(t, a); break;               \
+    case 0: bc_kernel<T, 0><<<nb, BC_BLOCK, 0, s>>>(t, a, gap_args{}); break;    \
+    case 1: bc_kernel<T, 1><<<nb, BC_BLOCK, 0, s>>>(t, a, gap_args{}); break;    \
+    default: bc_kernel<T, 2><<<nb, BC_BLOCK, 0, s>>>(t, a, gap_args{}); break;   \
     }
     if (dtype == GT355_F64) {
         GT355_BC_LAUNCH(double)
@@ -322,17 +464,23 @@ extern "C" int gt355_halo_fill(int dtype, void *const *fields, int nfields, cons
             "gt355_halo_fill: periodic halo wider than the interior in dim %d", d);
         a.mode[d] = mode[d];
     }
-    region_table t = make_table(halo, 0x7FFFFFFu, 0, mode);
-    if (t.total == 0)
+    region_table t{};
+    gap_args g{};
+    // the i faces as whole-sector row-gap runs where the layout allows (GT355_TUNE_halo_gap=0: per point)
+    if (!(tune_param("halo_gap", 1) && plan_gap_fill(a, dtype == GT355_F64 ? 8 : 4, halo, mode, t, g))) {
+        t = make_table(halo, 0x7FFFFFFu, 0, mode);
+        g = gap_args{};
+    }
+    if (t.total == 0 && g.total == 0)
         return GT355_OK;
     if ((rc = check_table(t, "gt355_halo_fill")))
         return rc;
     hipStream_t s = (hipStream_t)stream;
-    const int nb = t.blocks;
+    const int nb = t.blocks + (int)((g.total + BC_BLOCK - 1) / BC_BLOCK);
     if (dtype == GT355_F64)
-        bc_kernel<double, 3><<<nb, BC_BLOCK, 0, s>>>(t, a);
+        bc_kernel<double, 3><<<nb, BC_BLOCK, 0, s>>>(t, a, g);
     else
-        bc_kernel<float, 3><<<nb, BC_BLOCK, 0, s>>>(t, a);
+        bc_kernel<float, 3><<<nb, BC_BLOCK, 0, s>>>(t, a, g);
     return check_launch("gt355_halo_fill");
 }
 

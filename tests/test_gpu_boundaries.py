@@ -106,3 +106,33 @@ def test_pack_self_exchange_unpack_is_periodic_fill(gpu, oracle):
     finally:
         _abi.lib().gt355_free(send)
         _abi.lib().gt355_free(recv)
+
+
+@pytest.mark.parametrize("dtype,ni", [(np.float32, 40), (np.float64, 36), (np.float32, 41)])
+@pytest.mark.parametrize("gap", ["1", "0"])
+def test_halo_fill_row_gap_runs(gpu, oracle, monkeypatch, dtype, ni, gap):
+    """The halo fill with the i faces written as whole-sector row-gap runs (fields of the library's layout, ni times
+    the element size a multiple of 32 bytes: 40 FP32, 36 FP64) equals the per-point fill (41 FP32, and
+    GT355_TUNE_halo_gap=0) and the oracle; the row padding stays zero."""
+    from gridtools_amd import _abi, boundaries
+
+    monkeypatch.setenv("GT355_TUNE_halo_gap", gap)
+    _abi.call("gt355_tune_env_refresh")
+    nj, nk = 13, 7
+    for modes in [(2, 1, 0), (1, 2, 0), (2, 0, 0)]:
+        print(f"[case] halo fill {np.dtype(dtype).name} ni={ni} modes={modes} gap={gap}", flush=True)
+        hal = halos3(ni, nj, nk, 3, 3, 0)
+        lengths = (ni + 6, nj + 6, nk)
+        # the library's layout for a halo of 3: the first interior point aligned
+        fs = [gpu.builder().type(dtype).dimensions(*lengths).halos(3, 3, 0).uniform(3, 70 + f, -1.0, 1.0).build()
+              for f in range(2)]
+        exp = [f.padded_numpy() for f in fs]
+        for e in exp:
+            oracle.halo_fill(e, hal, modes)
+        boundaries.halo_fill(fs, hal, modes)
+        for f, e in zip(fs, exp):
+            got = f.padded_numpy()
+            assert np.array_equal(box(f, got), box(f, e)), modes
+            assert not got[:, :, lengths[0]:].any(), "row padding changed"
+    monkeypatch.delenv("GT355_TUNE_halo_gap")
+    _abi.call("gt355_tune_env_refresh")
