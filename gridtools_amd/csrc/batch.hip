@@ -253,6 +253,10 @@ namespace gt355 {
             constexpr bool DEEP = !HD && NK == 80 && sizeof(ST) == 4;
             constexpr int KD = !HD && NK == 80 ? SDV : SD;
             __shared__ ST s_up[DEEP ? 96 : NK][64];
+            // with FP32 storage the 96-level part keeps the first LTAIL tail levels of ccol/dcol in LDS: 24 KiB of
+            // u_pos + 16 KiB of tail per wavefront, the 160 KiB of a CU for its four wavefronts
+            constexpr int LTAIL = DEEP && sizeof(CT) == 8 ? 16 : 0;
+            __shared__ CT s_tail[LTAIL > 0 ? LTAIL * 128 : 1];
             CT *tail = scratch + (int64_t)blockIdx.x * (stail_rows(NK) * 128);
             const int wid = xcd_remap(blockIdx.x, gridDim.x, nxcd);
             for (int it = wid; it < nitems; it += gridDim.x) {
@@ -273,8 +277,8 @@ namespace gt355 {
                         hv_strip<ST, CT, 64, KD, true, true, false, HD, !HD>(
                             f, dtr, dm.ni, dm.nj, dm.nk, item.i0, item.j, s_up, tail, flat);
                     else if (DEEP && dm.nk >= deep_nk)
-                        hv_strip<ST, CT, DEEP ? 96 : NK, KD, true, true, true, HD, !HD>(
-                            f, dtr, dm.ni, dm.nj, dm.nk, item.i0, item.j, s_up, tail, flat);
+                        hv_strip<ST, CT, DEEP ? 96 : NK, KD, true, true, true, HD, !HD, LTAIL>(
+                            f, dtr, dm.ni, dm.nj, dm.nk, item.i0, item.j, s_up, tail, flat, s_tail);
                     else
                         hv_strip<ST, CT, NK, KD, true, true, true, HD, !HD>(
                             f, dtr, dm.ni, dm.nj, dm.nk, item.i0, item.j, s_up, tail, flat);

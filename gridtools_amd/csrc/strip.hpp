@@ -11,7 +11,8 @@
 // forward Thomas step in the compute type.  u_pos is parked in LDS for the backward sweep.
 // TAIL: columns deeper than NK continue in a runtime loop over blocks of D levels whose ccol/dcol go to the
 // wavefront's `tail` scratch (64 lanes x (c, d) per level); the backward sweep reads them back, then finishes the
-// register part.
+// register part.  With TL > 0 the first TL tail levels use the LDS array `s_tail` instead (TL a multiple of D), so
+// that only columns deeper than NK + TL move ccol/dcol through memory.
 // Every load and store is unconditional, with levels clamped to the top (a step above the top recomputes or
 // overwrites the top with the same bits, or with a value the top's own step overwrites later in program order),
 // which keeps the code straight-line and the load counter exact.  Addresses are buffer descriptors plus 32-bit
@@ -57,10 +58,12 @@ namespace gt355 {
     // domain's column list (column n is i = n % ni of row j = n / ni), starting at i0 (j is not used); lanes past the
     // last column repeat it, computing and storing the same bits as its own lane.  Columns are independent in the
     // vertical advection, so no lane idles at the end of a row.
-    template <class ST, class CT, int NK, int D, bool NARROW, bool LAP, bool TAIL, bool HD = true, bool FLAT = false>
+    template <class ST, class CT, int NK, int D, bool NARROW, bool LAP, bool TAIL, bool HD = true, bool FLAT = false,
+        int TL = 0>
     __device__ __forceinline__ void hv_strip(const strip_fields<ST> &f, CT dtr, int ni, int nj, int nk, int i0,
-        int j, ST (*s_up)[64], CT *tail, bool flat = false) {
+        int j, ST (*s_up)[64], CT *tail, bool flat = false, CT *s_tail = nullptr) {
         static_assert(NK % D == 0, "tail blocks start on a ring-slot boundary");
+        static_assert(TL % D == 0, "LDS tail blocks are whole ring blocks");
         static_assert(!(FLAT && HD), "the column list form has no horizontal neighbours");
         constexpr int64_t S = sizeof(ST);
         const int lane = threadIdx.x & 63;
@@ -229,12 +232,20 @@ namespace gt355 {
             // memory operations stay at their step; arithmetic may move across steps
             __builtin_amdgcn_sched_barrier(6);
         }
+        // the tail level n (0-based past NK): the LDS tail for n < TL, the scratch tail beyond (a block of D levels
+        // is wholly on one side; a generic pointer serves both)
+        auto tail_at = [&](int n) -> CT * {
+            if constexpr (TL > 0)
+                if (n < TL)
+                    return s_tail + (int64_t)n * 128;
+            return tail + (int64_t)(n - TL) * 128;
+        };
         if constexpr (TAIL) {
             for (int kb = NK; kb < nk; kb += D) {
 #pragma clang loop unroll(full)
                 for (int s = 0; s < D; ++s) {
                     const fwd_out<CT> r = fstep(kb + s, s, (s + 1) % D, false, true);
-                    CT *t = tail + (int64_t)(kb + s - NK) * 128;
+                    CT *t = tail_at(kb + s - NK);
                     t[lane] = r.c;
                     t[64 + lane] = r.d;
                 }
@@ -250,7 +261,7 @@ namespace gt355 {
                 CT rc[D], rd[D], rp[D];
 #pragma clang loop unroll(full)
                 for (int s = 0; s < D; ++s) {
-                    const CT *t = tail + (int64_t)(kb + s - NK) * 128;
+                    const CT *t = tail_at(kb + s - NK);
                     rc[s] = t[lane];
                     rd[s] = t[64 + lane];
                     rp[s] = CT(bload<ST, CPOL_NT>(r_up, v_col, lvl(kb + s) * sk_b));
