@@ -90,8 +90,17 @@ class DataStore:
     def sid(self, origin=None, extra=()):
         """gt355_field whose origin is storage index ``origin`` (default: the first interior point) of the spatial
         slice ``extra``, with the allocation box as bounds -- the SID after core::shift_origin
-        (stencil/core/backend.hpp:28-34)."""
+        (stencil/core/backend.hpp:28-34).  The structure is built once per (origin, extra) and shared: treat it as
+        read-only (a launch's host cost is mostly building these)."""
         o = self.halos if origin is None else tuple(origin)
+        key = (o, tuple(extra))
+        cache = self.__dict__.setdefault("_sids", {})
+        f = cache.get(key)
+        if f is None:
+            f = cache[key] = self._make_sid(o, extra)
+        return f
+
+    def _make_sid(self, o, extra):
         f = _abi.Field()
         f.origin = self.address(*o, *extra)
         f.stride = (ctypes.c_int64 * 3)(*self.strides)
