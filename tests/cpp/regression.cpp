@@ -429,6 +429,55 @@ namespace {
     }
 
     // boundaries: zero / value / copy on the 26 regions against the oracle's per-region loop
+    // stencil::deferred: hdiff and vertical advection runs of uneven domains recorded in a scope and run as
+    // persistent batches at end() equal the same runs one after another (two dependent rounds: the second hdiff
+    // reads the first round's output)
+    void deferred_scope() {
+        const int h = 3;
+        const int sizes[3][3] = {{70, 9, 81}, {23, 11, 43}, {130, 5, 2}};
+        struct dom {
+            std::shared_ptr<storage::data_store<double>> in, coeff, ust, uts, wcon, up, ut;
+            grid g;
+        };
+        auto make = [&](int m) {
+            const int ni = sizes[m][0], nj = sizes[m][1], nk = sizes[m][2];
+            const uint64_t seed = 400 + m;
+            auto g = make_grid(halo_descriptor(h, h, h, ni + h - 1, ni + 2 * h),
+                halo_descriptor(h, h, h, nj + h - 1, nj + 2 * h), nk);
+            return dom{seeded<double>(ni, nj, nk, h, seed, 1, -1, 1), seeded<double>(ni, nj, nk, h, seed, 2, 0, 0.25),
+                seeded<double>(ni, nj, nk, h, seed, 3, -1, 1), seeded<double>(ni, nj, nk, h, seed, 6, -1, 1),
+                seeded<double>(ni, nj, nk, h, seed, 7, -0.5, 0.5, 1), seeded<double>(ni, nj, nk, h, seed, 4, -1, 1),
+                seeded<double>(ni, nj, nk, h, seed, 5, -1, 1), g};
+        };
+        std::vector<dom> a, b;
+        for (int m = 0; m < 3; ++m) {
+            a.push_back(make(m));
+            b.push_back(make(m));
+        }
+        auto program = [](std::vector<dom> &ds) {
+            for (auto &d : ds) {
+                run(specs::horizontal_diffusion(), gfx950<>(), d.g, d.in, d.coeff, d.ust);
+                run(specs::vertical_advection(), gfx950<>(), d.g, d.uts, d.ust, d.wcon, d.up, d.ut,
+                    global_parameter(0.15));
+            }
+            for (auto &d : ds)
+                run(specs::horizontal_diffusion(), gfx950<>(), d.g, d.ust, d.coeff, d.in);
+        };
+        {
+            deferred scope(gfx950<>{});
+            program(a);
+            scope.end();
+        }
+        program(b);
+        bool ok = true;
+        for (int m = 0; m < 3; ++m) {
+            ok &= same_interior(a[m].in, mirror(b[m].in), a[m].g);
+            ok &= same_interior(a[m].ust, mirror(b[m].ust), a[m].g);
+            ok &= same_interior(a[m].uts, mirror(b[m].uts), a[m].g);
+        }
+        report("deferred scope of 3 uneven domains == the runs in order", ok);
+    }
+
     void boundary_conditions() {
         const int n = 20, h = 3, nk = 6;
         std::array<halo_descriptor, 3> halos{halo_descriptor(h, h, h, n + h - 1, n + 2 * h),
@@ -506,6 +555,7 @@ int main(int argc, char **argv) {
             hdiff_vadv_precision_form(40, 30, 50);
             expandable();
             batching();
+            deferred_scope();
             boundary_conditions();
             simple_hori_diff(12, 33, 61);
             simple_hori_diff(23, 11, 43);

@@ -29,4 +29,4 @@ def test_cpp_frontend_host_cases():
 @pytest.mark.gpu
 def test_cpp_frontend_regression_on_device():
     out = run(timeout=240)
-    assert out.count("[ok]") >= 26
+    assert out.count("[ok]") >= 27
