@@ -1148,6 +1148,50 @@ namespace {
                 gfx950<>(), g, utens_stage, u_stage, wcon, u_pos, utens, global_parameter(0.15));
             report("vadv spec with a changed backward stage runs on the generic engine", engine_::fast_runs() == before);
         }
+        // inside a deferred-run scope the recognised hdiff spec is recorded; a generic-engine run (a copy spec) that
+        // reads its output first runs the recorded call, so the copy sees the diffusion, as without the scope
+        {
+            const int h = 2;
+            auto in = seeded<double>(ni + 4, nj + 4, nk, h, 42, 1, -1, 1);
+            auto coeff = seeded<double>(ni + 4, nj + 4, nk, h, 42, 2, 0, 0.25);
+            auto b = storage::builder<storage::gpu>.type<double>().dimensions(ni + 4, nj + 4, nk).halos(h, h, 0);
+            auto out_a = b.build(), copy_a = b.build(), out_b = b.build(), copy_b = b.build();
+            auto g = halo_grid(ni, nj, nk, h);
+            auto hdiff_spec = [](auto in, auto coeff, auto out) {
+                using namespace reference_specs::horizontal_diffusion;
+                GT_DECLARE_TMP(double, lap, flx, fly);
+                return execute_parallel()
+                    .ij_cached(lap, flx, fly)
+                    .stage(lap_function(), lap, in)
+                    .stage(flx_function(), flx, in, lap)
+                    .stage(fly_function(), fly, in, lap)
+                    .stage(out_function(), out, in, flx, fly, coeff);
+            };
+            auto copy_spec = [](auto out, auto in) {
+                return execute_parallel().stage(reference_specs::copy_stencil::copy_functor(), in, out);
+            };
+            const long before = engine_::fast_runs();
+            {
+                deferred scope(gfx950<>{});
+                run(hdiff_spec, gfx950<>(), g, in, coeff, out_a);
+                run(copy_spec, gfx950<>(), g, copy_a, out_a);
+                scope.end();
+            }
+            run(hdiff_spec, gfx950<>(), g, in, coeff, out_b);
+            run(copy_spec, gfx950<>(), g, copy_b, out_b);
+            auto va = copy_a->const_host_view(), vb = copy_b->const_host_view();
+            bool ok = engine_::fast_runs() == before + 2;
+            if (!ok)
+                std::printf("  hand-kernel runs: %ld, expected %ld\n", engine_::fast_runs() - before, 2L);
+            for (int k = 0; k < nk && ok; ++k)
+                for (int j = h; j < h + nj && ok; ++j)
+                    for (int i = h; i < h + ni && ok; ++i) {
+                        ok = std::memcmp(&va(i, j, k), &vb(i, j, k), sizeof(double)) == 0 && va(i, j, k) != 0.0;
+                        if (!ok)
+                            std::printf("  (%d,%d,%d): %.17g, expected %.17g\n", i, j, k, va(i, j, k), vb(i, j, k));
+                    }
+            report("deferred scope: a recorded hdiff spec runs before a generic-engine run that reads its output", ok);
+        }
     }
 
     // ------------------------------------------------------------------------------------- boundary functors
