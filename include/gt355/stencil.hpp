@@ -853,6 +853,56 @@ namespace gt355 {
                 using type = vtuple<stage_in<typename Ps::kind::data_t>...>;
             };
 
+            // One level of a staged input in registers: a thread's points of the tile are its lane's column in rows
+            // wave, wave + 4, ... (at most SROWS of them; the host checks that the tile fits one pass of the 64 lanes
+            // and 4 * SROWS rows).  Loads are unconditional, clamped into the tile, so that none waits on a branch.
+            constexpr int SROWS = 4;
+            template <class T>
+            struct stage_buf {
+                T v[SROWS];
+            };
+            template <class L>
+            struct stage_buf_tuple;
+            template <class... Ps>
+            struct stage_buf_tuple<tlist<Ps...>> {
+                using type = vtuple<stage_buf<typename Ps::kind::data_t>...>;
+            };
+            template <class T>
+            __device__ __forceinline__ void fetch_stage_in(
+                stage_in<T> const &s, stage_buf<T> &b, int ti0, int tj0, int TI, int TJ, int ni, int nj, int k) {
+                const int i_lo = ti0 + s.imin, i_hi = min(ti0 + TI + s.imax, ni + s.imax);
+                const int j_lo = tj0 + s.jmin, j_hi = min(tj0 + TJ + s.jmax, nj + s.jmax);
+                const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+                const int i = min(i_lo + lane, i_hi - 1);
+#pragma unroll
+                for (int r = 0; r < SROWS; ++r)
+                    b.v[r] = s.src.at(i, min(j_lo + wave + 4 * r, j_hi - 1), k);
+            }
+            template <class T>
+            __device__ __forceinline__ void put_stage_in(
+                stage_in<T> const &s, stage_buf<T> const &b, int ti0, int tj0, int TI, int TJ, int ni, int nj) {
+                const int i_lo = ti0 + s.imin, i_hi = min(ti0 + TI + s.imax, ni + s.imax);
+                const int j_lo = tj0 + s.jmin, j_hi = min(tj0 + TJ + s.jmax, nj + s.jmax);
+                const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+                T *lds = reinterpret_cast<T *>(engine_lds) + s.off;
+#pragma unroll
+                for (int r = 0; r < SROWS; ++r) {
+                    const int j = j_lo + wave + 4 * r, i = i_lo + lane;
+                    if (j < j_hi && i < i_hi)
+                        lds[(j - j_lo) * s.w + (i - i_lo)] = b.v[r];
+                }
+            }
+            template <int... Is, class... Ss, class Bs>
+            __device__ void fetch_stage_ins(vtuple_impl<std::integer_sequence<int, Is...>, Ss...> const &si, Bs &bs,
+                int ti0, int tj0, int TI, int TJ, int ni, int nj, int k) {
+                (fetch_stage_in(vget<Is>(si), vget<Is>(bs), ti0, tj0, TI, TJ, ni, nj, k), ...);
+            }
+            template <int... Is, class... Ss, class Bs>
+            __device__ void put_stage_ins(vtuple_impl<std::integer_sequence<int, Is...>, Ss...> const &si,
+                Bs const &bs, int ti0, int tj0, int TI, int TJ, int ni, int nj) {
+                (put_stage_in(vget<Is>(si), vget<Is>(bs), ti0, tj0, TI, TJ, ni, nj), ...);
+            }
+
             template <class Stage, class Caches = tlist<>>
             struct stage_views;
             template <class F, class... Plhs, class Caches>
@@ -1075,6 +1125,34 @@ namespace gt355 {
                 fused_runner<NSPL, W, StageList, ViewsTuple>::run(
                     v, ext.d, ti0, tj0, TI, TJ, ni, nj, (int)blockIdx.z, ks, std::make_integer_sequence<int, NS>());
             }
+            // The same over a chunk of KC levels per workgroup: the staged inputs of level k + 1 are fetched into
+            // registers before level k's stages run and written to their LDS tiles after (the last stage of a level
+            // ends with a barrier), so their loads overlap the level's compute, and a workgroup's setup is paid once
+            // per chunk.
+            template <int NSPL, bool W, class StageList, class ViewsTuple, class StageIns, class StageBufs, int NS>
+            __global__ void __launch_bounds__(256) fused_kc_kernel(ViewsTuple v0, StageIns si, domains<NS> ext, int ni,
+                int nj, ksplit ks, int tiles_i, int TI, int TJ, int KC, int nk) {
+                const int tile = (int)blockIdx.x;
+                const int ti0 = (tile % tiles_i) * TI, tj0 = (tile / tiles_i) * TJ;
+                const int k0 = (int)blockIdx.z * KC, k1 = min(k0 + KC, nk);
+                StageBufs buf;
+                fetch_stage_ins(si, buf, ti0, tj0, TI, TJ, ni, nj, k0);
+                put_stage_ins(si, buf, ti0, tj0, TI, TJ, ni, nj);
+                __syncthreads();
+                for (int k = k0; k < k1; ++k) {
+                    const bool more = k + 1 < k1;
+                    if (more)
+                        fetch_stage_ins(si, buf, ti0, tj0, TI, TJ, ni, nj, k + 1);
+                    ViewsTuple v = v0;
+                    fused_runner<NSPL, W, StageList, ViewsTuple>::run(
+                        v, ext.d, ti0, tj0, TI, TJ, ni, nj, k, ks, std::make_integer_sequence<int, NS>());
+                    if (more) {
+                        put_stage_ins(si, buf, ti0, tj0, TI, TJ, ni, nj);
+                        __syncthreads();
+                    }
+                }
+            }
+
             // one vertical multistage with horizontal dependencies between its stages: a workgroup owns a TI x TJ tile
             // and walks the levels in the sweep's order; at each level it runs every stage over the tile extended by
             // the stage's extent, with a barrier between stages, as the fused parallel form does for its one level.
@@ -1212,6 +1290,7 @@ namespace gt355 {
             };
             struct fuse_rt {
                 int ti = 64, tj = 8;
+                int kc = 1; // levels per workgroup (fused_kc_kernel when > 1)
                 size_t bytes = 0;
             };
             struct run_ctx {
@@ -1322,9 +1401,16 @@ namespace gt355 {
                     fuse_rt const &f = c.fuse[m];
                     const int tiles_i = (ni + f.ti - 1) / f.ti, tiles_j = (nj + f.tj - 1) / f.tj;
                     if constexpr (T::fused) {
-                        fused_kernel<NSPL, W, tlist<Stages...>, ViewsTuple, StageIns, NS>
-                            <<<dim3((unsigned)(tiles_i * tiles_j), 1, (unsigned)nk), 256, f.bytes, stream>>>(
-                                vt, si, ext, ni, nj, g.ks, tiles_i, f.ti, f.tj);
+                        if (f.kc > 1) {
+                            using StageBufs = typename stage_buf_tuple<Staged>::type;
+                            fused_kc_kernel<NSPL, W, tlist<Stages...>, ViewsTuple, StageIns, StageBufs, NS>
+                                <<<dim3((unsigned)(tiles_i * tiles_j), 1, (unsigned)((nk + f.kc - 1) / f.kc)), 256,
+                                    f.bytes, stream>>>(vt, si, ext, ni, nj, g.ks, tiles_i, f.ti, f.tj, f.kc, nk);
+                        } else {
+                            fused_kernel<NSPL, W, tlist<Stages...>, ViewsTuple, StageIns, NS>
+                                <<<dim3((unsigned)(tiles_i * tiles_j), 1, (unsigned)nk), 256, f.bytes, stream>>>(
+                                    vt, si, ext, ni, nj, g.ks, tiles_i, f.ti, f.tj);
+                        }
                         check_launch("stencil engine: fused multistage launch");
                     } else {
                         vfused_kernel<Exec, NSPL, W, tlist<Stages...>, ViewsTuple, StageIns, NS>
@@ -1583,6 +1669,19 @@ namespace gt355 {
                         f.bytes = bytes;
                         for (auto &l : lds)
                             c.lds[l.first] = l.second;
+                        // a parallel multistage walks chunks of levels (GT355_ENGINE_KC, default 2) when its staged
+                        // inputs fit the registers of one level: one pass of the 64 lanes, 4 * SROWS rows.  Measured
+                        // on the hdiff spec at 256x256x80 FP64: 54.3 us one level per workgroup, 47.5 at 2, 48.5 at
+                        // 4, 53.9 at 8
+                        bool fits = true;
+                        for (int q = 1; q <= (int)sizeof...(Ins); ++q)
+                            if (box.count(in_keys[q])) {
+                                ext_rt const &b = box.at(in_keys[q]);
+                                fits = fits && f.ti + b.imax - b.imin <= 64 && f.tj + b.jmax - b.jmin <= 4 * SROWS;
+                            }
+                        const char *kenv = std::getenv("GT355_ENGINE_KC");
+                        const int kc = kenv ? std::max(1, std::atoi(kenv)) : 2;
+                        f.kc = dir == 0 && fits ? kc : 1;
                         break;
                     }
                 }
