@@ -462,7 +462,7 @@ def gt_sync():
     synchronize()
 
 
-def batch_leg(gt, nsteps=10, reps=3, seed=2024):
+def batch_leg(gt, nsteps=10, reps=3, seed=2024, per_domain_calls=True):
     """The batching form: 64 domains, ni, nj ~ U{48..384}, nk ~ U{40..137} (seeded), laplacian + hdiff + vadv in
     FP32 storage / FP64 compute, in the batch's default two-phase form (per timestep one persistent launch of the
     stencil phase and one of the vertical phase, each over every domain); the grid size from the batch tuner; a
@@ -490,8 +490,16 @@ def batch_leg(gt, nsteps=10, reps=3, seed=2024):
     # algorithmic bytes per point and timestep (FP32 storage): in, coeff, utens_stage, wcon, u_pos, utens read;
     # lap, u_stage, utens_stage written
     gbs = 9 * 4 * pts * nsteps / (ms * 1e-3) / 1e9
-    scoped = deferred_calls(bt, nsteps, reps, True)
-    direct = deferred_calls(bt, nsteps, reps, False)
+    calls = None
+    if per_domain_calls:
+        scoped = deferred_calls(bt, nsteps, reps, True)
+        direct = deferred_calls(bt, nsteps, reps, False)
+        calls = {
+            "workload": "the same domains as one gt355_hdiff_vadv call per domain and timestep (no laplacian), "
+                        f"{nsteps} timesteps in a graph",
+            "ms_per_timestep_in_deferred_scope": round(scoped / nsteps, 4),
+            "ms_per_timestep_direct": round(direct / nsteps, 4),
+        }
     return {
         "workload": f"64 domains (seed {seed}), {pts / 1e6:.1f} M points, {nsteps}-timestep graph, two-phase form "
                     "(stencil-phase + vertical-phase launch per timestep, each over every domain)",
@@ -505,12 +513,7 @@ def batch_leg(gt, nsteps=10, reps=3, seed=2024):
         "tuned_config": [bt.batches[0].config(), bt.batches[1].config()],
         "tune_us": [round(us0, 1), round(us1, 1)],
         "workgroups": grid,
-        "per_domain_calls": {
-            "workload": "the same domains as one gt355_hdiff_vadv call per domain and timestep (no laplacian), "
-                        f"{nsteps} timesteps in a graph",
-            "ms_per_timestep_in_deferred_scope": round(scoped / nsteps, 4),
-            "ms_per_timestep_direct": round(direct / nsteps, 4),
-        },
+        "per_domain_calls": calls,
     }
 
 
@@ -705,13 +708,14 @@ def main():
     # 100 untimed steps: after 20 the first timed steps still ran 2-3% slower than steady state
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--sets", type=int, default=4)
-    ap.add_argument("--legs", default="tune,fp64,fp32,graph,copy,cartesian,generic,compose,batch,large,cpu",
+    ap.add_argument("--legs", default="tune,fp64,fp32,graph,copy,cartesian,generic,compose,batch,deferred,large,cpu",
                     help="comma list of legs: tune (hdiff autotuner first, untimed), fp64 (always run), fp32 "
-                         "(precision form per kernel), copy (bandwidth "
+                         "(precision form per kernel), graph (the steps captured in a graph), copy (bandwidth "
                          "yardstick), cartesian (simple_hori_diff, prepare_tracers x11, copy_stencil_tuple x5, extended_4D, "
                          "transform_layout), generic (the generic engine on user hdiff / vadv specs), "
                          "compose (precision form on the fused composition, 512x512x80), batch (64 "
-                         "uneven domains, persistent launch, graph of 10 timesteps), large (1536x1536x80, 20 "
+                         "uneven domains, persistent launch, graph of 10 timesteps), deferred (with batch: the same "
+                         "domains as per-domain calls in a deferred-run scope and directly), large (1536x1536x80, 20 "
                          "timesteps with boundary conditions), cpu (CPU baseline)")
     args = ap.parse_args()
     legs = set(args.legs.split(","))
@@ -796,7 +800,7 @@ def main():
     if "batch" in legs:
         print("[bench] batch leg", file=sys.stderr, flush=True)
         with Range("bench: batching form, 64 domains"):
-            batch = batch_leg(gt)
+            batch = batch_leg(gt, per_domain_calls="deferred" in legs)
     large = None
     if "large" in legs:
         print("[bench] large-domain leg", file=sys.stderr, flush=True)
