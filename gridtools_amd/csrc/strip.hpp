@@ -143,7 +143,12 @@ namespace gt355 {
             b_ut[s] = bload<ST, CPOL_NT>(r_ut, v_col, o);
         };
         auto load_a = [&](int L, int s) {
-            const two<ST> w = bload2<ST, CPOL_NT>(r_wc, v_wc, lvl(L) * wsk_b);
+            // wcon at i and i + 1: column i0 + 64 of a strip is the first column of the next strip's line, so the
+            // batch's vertical phase (HD = false) keeps wcon lines in L2 (default policy) for the neighbouring item
+            // on the same XCD; non-temporal loads there fetched those lines twice (vertical-phase reads 5.29 ->
+            // 5.02 GB per timestep, 2.042 -> 1.998 ms)
+            const two<ST> w = HD ? bload2<ST, CPOL_NT>(r_wc, v_wc, lvl(L) * wsk_b)
+                                 : bload2<ST>(r_wc, v_wc, lvl(L) * wsk_b);
             a_wi[s] = w.a;
             a_wip[s] = w.b;
         };
