@@ -275,7 +275,11 @@ namespace gt355 {
             auto load_a = [&](int L, uint32_t cb, uint32_t wb) {
                 const uint32_t o = lvl(L) * sk_o, w = lvl(L) * wsk_o;
                 a_us[L % D] = bload<ST, CPOL_NT>(r_us, cb, o);
-                const two<ST> ww = bload2<ST, CPOL_NT>(r_wc, wb, w);
+                // wcon at i and i + 1: with many strips per wavefront (the 16-level variant) the neighbouring
+                // strip reads the column-64 line soon after, from L2 with the default policy (1536x1536x80 timestep
+                // 1.40 -> 1.36 ms); with few (8 levels ahead) non-temporal loads measured faster (512x512x80
+                // composition 156 against 160 us)
+                const two<ST> ww = bload2<ST, D == 16 ? 0 : CPOL_NT>(r_wc, wb, w);
                 a_wi[L % D] = ww.a;
                 a_wip[L % D] = ww.b;
             };
