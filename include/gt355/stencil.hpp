@@ -45,6 +45,11 @@
 #ifndef GT355_ENGINE_PREFETCH
 #define GT355_ENGINE_PREFETCH 16
 #endif
+// Non-temporal loads for the register windows of fields no multistage of the spec writes, read at one column: each
+// value is read once, so it need not stay in the caches (0: every load with the default policy).
+#ifndef GT355_ENGINE_NT
+#define GT355_ENGINE_NT 1
+#endif
 
 #ifndef GT_FUNCTION
 #define GT_FUNCTION __host__ __device__ __forceinline__
@@ -529,7 +534,7 @@ namespace gt355 {
             // index is static in the unrolled block): the block's loads are in flight together and the block waits for
             // memory once instead of once per level.  No load is pending across the loop's back edge.
             template <class T, int KMIN, int KMAX, int KEY, bool FILL, bool FLUSH, int PD = 0, int IMIN = 0, int IMAX = 0,
-                int JMIN = 0, int JMAX = 0>
+                int JMIN = 0, int JMAX = 0, bool NT = false>
             struct kring {
                 static constexpr int key = KEY;
                 static constexpr int pd = PD;
@@ -552,7 +557,11 @@ namespace gt355 {
                 }
                 __device__ T load(int c, int k, int nk) const {
                     const int di = IMIN + c / (JMAX - JMIN + 1), dj = JMIN + c % (JMAX - JMIN + 1);
-                    return mem.at(fi + di, fj + dj, min(max(k, 0), nk - 1));
+                    T *x = &mem.at(fi + di, fj + dj, min(max(k, 0), nk - 1));
+                    if constexpr (NT)
+                        return __builtin_nontemporal_load(x);
+                    else
+                        return *x;
                 }
                 // the whole window at the sweep's first level.  Loads are unconditional (column clamped into the fill
                 // box, level into [0, nk)): values outside the box or the column are never read.
@@ -1013,28 +1022,53 @@ namespace gt355 {
                 int d[NS][4];
             };
             // the register windows of a vertical multistage: one kring per kc<> entry of its cache list
-            template <class Entry, class... Stages>
+            // Written: the placeholders some multistage of the spec writes
+            template <class Written, class Entry, class... Stages>
             struct ring_of;
-            template <class P, bool Fi, bool Fl, class... Stages>
-            struct ring_of<kc<P, Fi, Fl>, Stages...> {
+            template <class Written, class P, bool Fi, bool Fl, class... Stages>
+            struct ring_of<Written, kc<P, Fi, Fl>, Stages...> {
                 using X = mss_ijext<P, Stages...>;
+                static constexpr bool nt = GT355_ENGINE_NT && Fi && X::area == 1 && !in_list<P, Written>::value;
                 using type = kring<typename P::kind::data_t, mss_kext<P, Stages...>::kmin, mss_kext<P, Stages...>::kmax,
-                    P::key, Fi, Fl, Fi ? GT355_ENGINE_PREFETCH : 0, X::imin, X::imax, X::jmin, X::jmax>;
+                    P::key, Fi, Fl, Fi ? GT355_ENGINE_PREFETCH : 0, X::imin, X::imax, X::jmin, X::jmax, nt>;
             };
-            template <class Caches, class StageList>
+            template <class Written, class Caches, class StageList>
             struct ring_list {
                 using type = tlist<>;
             };
-            template <class C0, class... Cs, class... Stages>
-            struct ring_list<tlist<C0, Cs...>, tlist<Stages...>> {
-                using rest = typename ring_list<tlist<Cs...>, tlist<Stages...>>::type;
+            template <class Written, class C0, class... Cs, class... Stages>
+            struct ring_list<Written, tlist<C0, Cs...>, tlist<Stages...>> {
+                using rest = typename ring_list<Written, tlist<Cs...>, tlist<Stages...>>::type;
                 using type = rest;
             };
-            template <class P, bool Fi, bool Fl, class... Cs, class... Stages>
-            struct ring_list<tlist<kc<P, Fi, Fl>, Cs...>, tlist<Stages...>> {
-                using rest = typename ring_list<tlist<Cs...>, tlist<Stages...>>::type;
-                using type = typename lcat<tlist<typename ring_of<kc<P, Fi, Fl>, Stages...>::type>, rest>::type;
+            template <class Written, class P, bool Fi, bool Fl, class... Cs, class... Stages>
+            struct ring_list<Written, tlist<kc<P, Fi, Fl>, Cs...>, tlist<Stages...>> {
+                using rest = typename ring_list<Written, tlist<Cs...>, tlist<Stages...>>::type;
+                using type =
+                    typename lcat<tlist<typename ring_of<Written, kc<P, Fi, Fl>, Stages...>::type>, rest>::type;
             };
+            // the placeholders written by some stage of a list of multistages
+            template <class L, class... Stages>
+            struct written_by {
+                using type = L;
+            };
+            template <class L, class F, class... Ps, class... Stages>
+            struct written_by<L, stage_desc<F, Ps...>, Stages...>
+                : written_by<typename tset_add_all<L,
+                                 std::conditional_t<stage_props<Ps, stage_desc<F, Ps...>>::written, Ps, void>...>::type,
+                      Stages...> {};
+            template <class L, class Mss>
+            struct mss_written;
+            template <class L, class E, class C, class... Stages>
+            struct mss_written<L, mss_desc<E, C, Stages...>> : written_by<L, Stages...> {};
+            template <class L, class Msses>
+            struct spec_written;
+            template <class L>
+            struct spec_written<L, tlist<>> {
+                using type = L;
+            };
+            template <class L, class M0, class... Ms>
+            struct spec_written<L, tlist<M0, Ms...>> : spec_written<typename mss_written<L, M0>::type, tlist<Ms...>> {};
             template <class L>
             struct to_vtuple;
             template <class... Ts>
@@ -1377,7 +1411,7 @@ namespace gt355 {
                 return dim3((unsigned)((ni + 63) / 64), (unsigned)((nj + 3) / 4), (unsigned)nz);
             }
 
-            template <class Mss, int NSPL, bool W, class... Stages, class ArgsTuple>
+            template <class Mss, int NSPL, bool W, class Written, class... Stages, class ArgsTuple>
             void launch_mss(tlist<Stages...>, std::vector<stage_rt> const &st, int first, int m, run_ctx const &c,
                 ArgsTuple const &args, run_geom const &g, hipStream_t stream) {
                 using T = mss_traits<Mss>;
@@ -1435,7 +1469,8 @@ namespace gt355 {
                     using Caches = auto_vcaches<typename T::caches, Stages...>;
                     using Ctx = vctx<Caches, Stages...>;
                     using ViewsTuple = vtuple<typename stage_views<Stages, Ctx>::type...>;
-                    using RingSet = typename to_vtuple<typename ring_list<Caches, tlist<Stages...>>::type>::type;
+                    using RingSet =
+                        typename to_vtuple<typename ring_list<Written, Caches, tlist<Stages...>>::type>::type;
                     ViewsTuple vt{};
                     RingSet rs{};
                     domains<NS> dom{};
@@ -1828,7 +1863,8 @@ namespace gt355 {
                 int first = 0, m = 0;
                 auto one = [&](auto mss) {
                     using T = mss_traits<decltype(mss)>;
-                    launch_mss<decltype(mss), NSPL, W>(typename T::stages(), st, first, m++, c, args, g, stream);
+                    using Written = typename spec_written<tlist<>, tlist<Msses...>>::type;
+                    launch_mss<decltype(mss), NSPL, W, Written>(typename T::stages(), st, first, m++, c, args, g, stream);
                     first += T::n;
                 };
                 (one(Msses()), ...);
