@@ -400,33 +400,45 @@ extern "C" int gt355_tune_hdiff(int precision, const gt355_field *out, const gt3
             return launch_cfg<float, double>(c, &f[0], &f[1], &f[2], ni, nj, nk, s);
         return launch_cfg<float, float>(c, &f[0], &f[1], &f[2], ni, nj, nk, s);
     };
-    for (const hdiff_cfg &c : hdiff_candidates()) {
-        if (rc)
-            break;
-        for (int k = 0; k < nsets && !rc; ++k) // warm-up (and code load) on every set
-            rc = launch_set(c, k);
-        if (rc)
-            break;
-        (void)hipEventRecord(e0, s);
-        for (int r = 0; r < reps && !rc; ++r)
-            rc = launch_set(c, r);
-        (void)hipEventRecord(e1, s);
-        if (rc)
-            break;
-        hipError_t err = hipEventSynchronize(e1);
-        float ms = 0.f;
-        if (err == hipSuccess)
-            err = hipEventElapsedTime(&ms, e0, e1);
-        if (err != hipSuccess) {
-            rc = hip_fail(err, "gt355_tune_hdiff: timing");
-            break;
+    // two passes over the candidates, each candidate's time the faster of its two; the measured default stays
+    // unless a candidate beats it by more than 2% (single timings of candidates within noise of each other made the
+    // winner a matter of chance, and a chance winner then ran slower than the default)
+    const std::vector<hdiff_cfg> cands = hdiff_candidates();
+    std::vector<double> best_of(cands.size(), 1e30);
+    for (int pass = 0; pass < 2 && !rc; ++pass)
+        for (size_t ci = 0; ci < cands.size() && !rc; ++ci) {
+            const hdiff_cfg &c = cands[ci];
+            for (int k = 0; k < nsets && !rc; ++k) // warm-up (and code load) on every set
+                rc = launch_set(c, k);
+            if (rc)
+                break;
+            (void)hipEventRecord(e0, s);
+            for (int r = 0; r < reps && !rc; ++r)
+                rc = launch_set(c, r);
+            (void)hipEventRecord(e1, s);
+            if (rc)
+                break;
+            hipError_t err = hipEventSynchronize(e1);
+            float ms = 0.f;
+            if (err == hipSuccess)
+                err = hipEventElapsedTime(&ms, e0, e1);
+            if (err != hipSuccess) {
+                rc = hip_fail(err, "gt355_tune_hdiff: timing");
+                break;
+            }
+            best_of[ci] = std::min(best_of[ci], 1e3 * ms / reps);
         }
-        const double us = 1e3 * ms / reps;
-        if (us < win_us) {
-            win_us = us;
-            win = c;
-        }
+    const hdiff_cfg dflt = hdiff_default_cfg(st, ni);
+    double dflt_us = 1e30;
+    for (size_t ci = 0; ci < cands.size(); ++ci) {
+        const hdiff_cfg &c = cands[ci];
+        if (best_of[ci] < win_us)
+            win_us = best_of[ci], win = c;
+        if (c.impl == dflt.impl && c.a == dflt.a && c.b == dflt.b && (c.impl != 0 || c.c == dflt.c))
+            dflt_us = best_of[ci];
     }
+    if (dflt_us <= win_us * 1.02)
+        win = dflt, win_us = dflt_us;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (!copies.empty())

@@ -100,11 +100,15 @@ namespace gt355 {
         read_knobs_locked();
     }
 
-    hdiff_cfg hdiff_cfg_for(int st_size, int ct_size, int ni, int nj, int nk) {
+    hdiff_cfg hdiff_default_cfg(int st_size, int ni) {
         // measured defaults (MI355X, x80 levels, autotuner winners): FP32 storage takes the column-pair sweep with
         // 8-row strips (256x256: 13.5 us, 512x512: 41 us); FP64 storage the one-column sweep with 8-row strips at
         // 256x256 (22.2 us) and the column-pair sweep from 512 columns on (88 us against 97 us)
-        hdiff_cfg c = st_size == 4 ? hdiff_cfg{2, 8, 4, 0} : ni >= 512 ? hdiff_cfg{2, 8, 4, 0} : hdiff_cfg{1, 8, 4, 0};
+        return st_size == 4 ? hdiff_cfg{2, 8, 4, 0} : ni >= 512 ? hdiff_cfg{2, 8, 4, 0} : hdiff_cfg{1, 8, 4, 0};
+    }
+
+    hdiff_cfg hdiff_cfg_for(int st_size, int ct_size, int ni, int nj, int nk) {
+        hdiff_cfg c = hdiff_default_cfg(st_size, ni);
         {
             std::lock_guard<std::mutex> lock(g_tune_mutex);
             auto it = g_hdiff.find(tune_key{st_size, ct_size, ni, nj, nk});

@@ -29,6 +29,8 @@ namespace gt355 {
     // The configuration for an hdiff launch, in this order: environment overrides (GT355_TUNE_hdiff_*), the entry
     // the autotuner recorded for (storage size, compute size, ni, nj, nk), the measured default for the storage type.
     hdiff_cfg hdiff_cfg_for(int st_size, int ct_size, int ni, int nj, int nk);
+    // the measured default configuration for a storage size and domain width (no tuned entry, no knob)
+    hdiff_cfg hdiff_default_cfg(int st_size, int ni);
     // whether a configuration is one of the instantiated ones (hdiff.hip)
     bool hdiff_cfg_instantiated(hdiff_cfg cfg);
     void record_hdiff_cfg(int st_size, int ct_size, int ni, int nj, int nk, hdiff_cfg cfg, double us);

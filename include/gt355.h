@@ -273,9 +273,10 @@ int gt355_event_elapsed_ms(void *start, void *stop, float *ms);
  * configuration (impl 0: LDS tiles TI x TJ x KC; impl 1: one-column register sweeps JB/P; impl 2: column-pair
  * register sweeps JB/P, for an even ni and 2-element-aligned rows) on the given fields, `reps` launches each on
  * `stream` back to back, cycling through temporary copies of the three fields so that the data exceeds the MALL
- * (as a caller streaming several field sets sees it), records the fastest for (storage type, compute type, ni, nj,
- * nk) so later gt355_hdiff calls with that
- * shape use it, and returns it in best[4] = {impl, a, b, c} with its mean time.  The fields' contents are
+ * (as a caller streaming several field sets sees it), in two passes (a candidate's time is the faster of its two),
+ * records the fastest for (storage type, compute type, ni, nj, nk) so later gt355_hdiff calls with that shape use
+ * it -- the measured default configuration unless a candidate beats it by more than 2% -- and returns it in
+ * best[4] = {impl, a, b, c} with its mean time.  The fields' contents are
  * overwritten (out) and read (in, coeff) as by gt355_hdiff.  The table persists across processes through
  * gt355_tune_save / gt355_tune_load (text, one entry per line). */
 int gt355_tune_hdiff(int precision, const gt355_field *out, const gt355_field *in, const gt355_field *coeff, int ni,
