@@ -590,6 +590,283 @@ namespace gt355 {
             }
         }
 
+        // vadv_wide_kernel with its rows loaded straight into LDS: each dwordx4 request goes to LDS by the buffer
+        // unit (buffer_load ... lds; lane l's 16 bytes land at the instruction's LDS base + 16 l, so the 16 lanes of
+        // row g fill row g of a 64-column staging block), and no VGPR holds a row in flight.  The 21 registers per
+        // group of the register ring are what limited vadv_wide_kernel to three groups in flight at FP64 compute
+        // (four spill); here NS LDS slots hold the groups (each 4 fields x 4 rows x 64 columns), NS - 1 of them in
+        // flight while a group computes.  The u_pos rows of groups loaded inside a sweep go straight to the park;
+        // those of the next strip's first NS groups, loaded during the backward sweep while the park is still read,
+        // take registers and are written to the park as the sweep reaches them.  wcon's column 64 (the i+1
+        // neighbour of the strip's last column) comes as one dword per lane into a register, and lane 63 takes row
+        // r's value from lane 16 r.  The backward sweep stages a group's output rows in the park rows of the same
+        // group, whose u_pos values it has read by then.  LDS: NK x 64 x 4 bytes of park plus NS x 4 KiB of slots
+        // (40 KiB at NK = 80, NS = 5: four wavefronts per CU).
+        // Ordering is the kernel's own.  The compiler's waits for these LDS writes cannot be relied on: with every
+        // slot in one array it waits for the newest request before any read (all groups drained), with more than
+        // eight target arrays it emits no wait for the later ones, and it never waits before a refill overwrites a
+        // slot whose reads may be pending.  So the six requests of a group are issued between two fences (asm
+        // memory clobber + sched_barrier; their order inside the group is free); before a group's slot is read,
+        // `s_waitcnt vmcnt(n)` waits until at most n requests are outstanding, n = 6 x the groups issued after it
+        // (a load request's count drops only after its data is in LDS); before a slot is refilled,
+        // `s_waitcnt lgkmcnt(0)` waits for the reads of its previous group.  Requests or stores issued later only
+        // make a wait stricter.  One array per slot and per park group keeps the compiler's own waits no stricter
+        // than these (a read then depends only on the requests into its array).  Requirements and arithmetic as
+        // vadv_wide_kernel.  Measured at 256x256x80, FP64 compute: 22.7 against 22.9 us for the register ring.
+        template <class CT, int NK, int NS>
+        __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
+        vadv_lds_kernel(float *__restrict__ uts_f, const float *__restrict__ us_f, const float *__restrict__ wc_f,
+            const float *__restrict__ up_f, const float *__restrict__ ut_f, CT dtr, int ni, int nk, int64_t sj,
+            int64_t sk, int64_t wsj, int64_t wsk, int strips_i, int nitems, int nxcd) {
+            constexpr int NG = NK / 4;
+            static_assert(NK % 4 == 0 && NS >= 2 && NS <= NG, "level groups");
+            constexpr int PER_GROUP = 6; // requests per group: wcon column 64, u_pos, utens_stage, utens, u_stage, wcon
+            const int lane = threadIdx.x, g = lane >> 4, q = lane & 15;
+            const int top = nk - 1;
+            const rsrc_t r_us = make_rsrc(us_f), r_wc = make_rsrc(wc_f), rs_up = make_rsrc(up_f),
+                         r_ut = make_rsrc(ut_f), r_uts = make_rsrc(uts_f);
+            uint32_t skb = (uint32_t)(sk * 4), wskb = (uint32_t)(wsk * 4);
+            auto pin = [&]() {
+                asm volatile("" : "+s"(skb));
+                asm volatile("" : "+s"(wskb));
+            };
+            auto fence = []() {
+                asm volatile("" ::: "memory");
+                __builtin_amdgcn_sched_barrier(0);
+            };
+            // LDS: one array per u_pos park group (rows 4G..4G+3, row-major; a group's output rows during the
+            // backward sweep) and one per slot ([utens_stage, utens, u_stage, wcon][row * 64 + column]).  Separate
+            // arrays keep the compiler's own LDS-DMA tracking from ordering a read after every request in flight:
+            // a read of one array can only depend on the requests into that array.
+            static_assert(NG == 20 && NS == 5, "the park and slot arrays below are written out for NK = 80, NS = 5");
+#define GT355_LDS_ARRAY(name, n) __shared__ __align__(16) float name##n[256];
+            GT355_LDS_ARRAY(pk, 0) GT355_LDS_ARRAY(pk, 1) GT355_LDS_ARRAY(pk, 2) GT355_LDS_ARRAY(pk, 3)
+            GT355_LDS_ARRAY(pk, 4) GT355_LDS_ARRAY(pk, 5) GT355_LDS_ARRAY(pk, 6) GT355_LDS_ARRAY(pk, 7)
+            GT355_LDS_ARRAY(pk, 8) GT355_LDS_ARRAY(pk, 9) GT355_LDS_ARRAY(pk, 10) GT355_LDS_ARRAY(pk, 11)
+            GT355_LDS_ARRAY(pk, 12) GT355_LDS_ARRAY(pk, 13) GT355_LDS_ARRAY(pk, 14) GT355_LDS_ARRAY(pk, 15)
+            GT355_LDS_ARRAY(pk, 16) GT355_LDS_ARRAY(pk, 17) GT355_LDS_ARRAY(pk, 18) GT355_LDS_ARRAY(pk, 19)
+#undef GT355_LDS_ARRAY
+            __shared__ __align__(16) float sl0[4 * 256], sl1[4 * 256], sl2[4 * 256], sl3[4 * 256], sl4[4 * 256];
+            // LDS-space pointers (a generic pointer would be converted with a null check at every request)
+            typedef __attribute__((address_space(3))) float lfloat;
+            typedef __attribute__((address_space(3))) f32x4 lf32x4;
+#define GT355_L(a) ((lfloat *)(a))
+            lfloat *const park[NG] = {GT355_L(pk0), GT355_L(pk1), GT355_L(pk2), GT355_L(pk3), GT355_L(pk4),
+                GT355_L(pk5), GT355_L(pk6), GT355_L(pk7), GT355_L(pk8), GT355_L(pk9), GT355_L(pk10), GT355_L(pk11),
+                GT355_L(pk12), GT355_L(pk13), GT355_L(pk14), GT355_L(pk15), GT355_L(pk16), GT355_L(pk17),
+                GT355_L(pk18), GT355_L(pk19)};
+            lfloat *const slot[NS] = {GT355_L(sl0), GT355_L(sl1), GT355_L(sl2), GT355_L(sl3), GT355_L(sl4)};
+#undef GT355_L
+
+            f32x4 up_r[NS]; // u_pos rows of the first NS groups of a strip (register loads)
+            float wx_r[NS]; // wcon column 64 of row g = lane >> 4
+            uint32_t lb = 0, wlb = 0, wxb = 0, colb = 0, sb = 0;
+            int p = 0, sc = 0;
+            auto offsets = [&](int it) {
+                const int si = it % strips_i, j = it / strips_i, i0 = si * 64;
+                lb = (uint32_t)(((int64_t)j * sj + i0 + 4 * q) * 4);
+                wlb = (uint32_t)(((int64_t)j * wsj + i0 + 4 * q) * 4);
+                wxb = (uint32_t)(((int64_t)j * wsj + i0 + 64) * 4);
+                p = min(lane, ni - 1 - i0);
+                colb = (uint32_t)(((int64_t)j * sj + i0 + p) * 4);
+                sc = min(4 * q, ni - 4 - i0);
+                sb = (uint32_t)(((int64_t)j * sj + i0 + sc) * 4);
+            };
+            auto to_lds = [](rsrc_t r, lfloat *lds, uint32_t voff) {
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (__attribute__((address_space(3))) void *)lds, 16, voff, 0,
+                    0, CPOL_NT);
+            };
+            // group G's requests into slot G % NS; u_pos to the park (in_sweep) or to up_r
+            auto issue = [&](int G, bool in_sweep) {
+                const int s = G % NS;
+                const uint32_t lB = (uint32_t)min(4 * G + g, top), lA = (uint32_t)min(4 * G + 1 + g, top);
+                fence();
+                wx_r[s] = bload<float, CPOL_NT>(r_wc, wxb + lA * wskb, 0);
+                if (in_sweep)
+                    to_lds(rs_up, park[G], lb + lB * skb);
+                else
+                    up_r[s] = bload4<CPOL_NT>(rs_up, lb + lB * skb, 0);
+                to_lds(r_uts, slot[s], lb + lB * skb);
+                to_lds(r_ut, slot[s] + 256, lb + lB * skb);
+                to_lds(r_us, slot[s] + 512, lb + lA * skb);
+                to_lds(r_wc, slot[s] + 768, wlb + lA * wskb);
+                fence();
+            };
+            // waits until group G's requests are complete, given that groups G+1..last were issued after it
+            auto wait_group = [&](int G, int last) {
+                const int n = PER_GROUP * (last - G);
+                fence();
+                // vmcnt(n): bits 3:0 and 15:14; expcnt 7 and lgkmcnt 15 leave those counters alone
+                switch (n) {
+#define GT355_VMWAIT(N)                                                                                          \
+    case N:                                                                                                      \
+        __builtin_amdgcn_s_waitcnt(((N) & 15) | (7 << 4) | (15 << 8) | (((N) >> 4) << 14));                     \
+        break;
+                    GT355_VMWAIT(0)
+                    GT355_VMWAIT(6)
+                    GT355_VMWAIT(12)
+                    GT355_VMWAIT(18)
+                    GT355_VMWAIT(24)
+                    GT355_VMWAIT(30)
+                    GT355_VMWAIT(36)
+                    GT355_VMWAIT(42)
+#undef GT355_VMWAIT
+                default:
+                    __builtin_amdgcn_s_waitcnt(0 | (7 << 4) | (15 << 8));
+                }
+                fence();
+            };
+            auto lgkm_wait = [&]() {
+                fence();
+                __builtin_amdgcn_s_waitcnt(15 | (7 << 4) | (0 << 8) | (3 << 14)); // lgkmcnt(0)
+                fence();
+            };
+            float us0 = 0.f;
+            auto prologue = [&]() {
+                us0 = bload<float, CPOL_NT>(r_us, colb, 0);
+#pragma clang loop unroll(full)
+                for (int G = 0; G < NS; ++G)
+                    issue(G, false);
+            };
+
+            struct vals {
+                float uts[4], up[4], ut[4], us[4], wi[4], wip[4];
+            };
+            auto lds_vals = [&](vals &v, int G) {
+                const int s = G % NS;
+                const int pn = min(p + 1, 63);
+#pragma clang loop unroll(full)
+                for (int r = 0; r < 4; ++r) {
+                    v.uts[r] = slot[s][r * 64 + p];
+                    v.up[r] = park[G][r * 64 + p];
+                    v.ut[r] = slot[s][256 + r * 64 + p];
+                    v.us[r] = slot[s][512 + r * 64 + p];
+                    v.wi[r] = slot[s][768 + r * 64 + p];
+                    const float wn = slot[s][768 + r * 64 + pn];
+                    const float wx = __builtin_bit_cast(float,
+                        __builtin_amdgcn_readlane(__builtin_bit_cast(int, wx_r[s]), 16 * r));
+                    v.wip[r] = p == 63 ? wx : wn;
+                }
+            };
+
+            int it = xcd_remap(blockIdx.x, gridDim.x, nxcd);
+            if (it >= nitems)
+                return; // a whole wavefront leaves; there is no barrier in this kernel
+            offsets(it);
+            prologue();
+            for (; it < nitems; it += gridDim.x) {
+                CT c[NK], d[NK];
+                CT us_m = CT(0), us_c = CT(us0), w_c_i = CT(0), w_c_ip = CT(0);
+                vals a, b;
+                wait_group(0, NS - 1);
+                *(lf32x4 *)&park[0][4 * lane] = up_r[0];
+                fence();
+                lds_vals(a, 0);
+#pragma clang loop unroll(full)
+                for (int G = 0; G < NG; ++G) {
+                    if (G + 1 < NG) {
+                        // slot G % NS: group G's values are in `a`; refill it with group G+NS
+                        if (G + NS < NG) {
+                            lgkm_wait();
+                            pin();
+                            issue(G + NS, true);
+                        }
+                        wait_group(G + 1, min(G + NS, NG - 1));
+                        if (G + 1 < NS)
+                            *(lf32x4 *)&park[G + 1][4 * lane] = up_r[G + 1];
+                        fence();
+                        lds_vals(b, G + 1);
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma clang loop unroll(full)
+                    for (int r = 0; r < 4; ++r) {
+                        const int k = 4 * G + r;
+                        const CT v_uts = CT(a.uts[r]), v_up = CT(a.up[r]), v_ut = CT(a.ut[r]);
+                        // top level: the k+1 wind taken as zero reproduces last_level exactly (see
+                        // vadv_onchip_kernel)
+                        const bool at_top = k == top;
+                        const CT us_p = at_top ? us_c : CT(a.us[r]);
+                        const CT w_p_i = at_top ? CT(0) : CT(a.wi[r]);
+                        const CT w_p_ip = at_top ? CT(0) : CT(a.wip[r]);
+                        if (k == 0) {
+                            fwd_out<CT> o = forward_first<CT>(dtr, w_p_i, w_p_ip, us_c, us_p, v_up, v_ut, v_uts);
+                            asm volatile("" : "+v"(o.c), "+v"(o.d));
+                            c[0] = o.c;
+                            d[0] = o.d;
+                        } else {
+                            const CT cm = c[k > 0 ? k - 1 : 0], dm = d[k > 0 ? k - 1 : 0];
+                            fwd_out<CT> o = forward_body<CT>(dtr, w_c_i, w_c_ip, w_p_i, w_p_ip, us_m, us_c, us_p,
+                                v_up, v_ut, v_uts, cm, dm, at_top);
+                            asm volatile("" : "+v"(o.c), "+v"(o.d));
+                            c[k] = o.c;
+                            d[k] = o.d;
+                        }
+                        us_m = us_c;
+                        us_c = us_p;
+                        w_c_i = w_p_i;
+                        w_c_ip = w_p_ip;
+                    }
+                    a = b;
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+                // the next strip's first groups, in flight during this strip's backward sweep (every slot's values
+                // have been read and used by now)
+                const int nx = it + (int)gridDim.x;
+                const uint32_t csb = sb;
+                const int cp = p, csc = sc;
+                if (nx < nitems) {
+                    lgkm_wait();
+                    offsets(nx);
+                    pin();
+                    prologue();
+                }
+                float upa[4], upb[4];
+#pragma clang loop unroll(full)
+                for (int r = 0; r < 4; ++r)
+                    upa[r] = park[NG - 1][r * 64 + cp];
+                f32x4 o_prev = {0.f, 0.f, 0.f, 0.f};
+                uint32_t so_prev = 0;
+                CT data = CT(0);
+#pragma clang loop unroll(full)
+                for (int G = NG - 1; G >= 0; --G) {
+                    if (G > 0) {
+#pragma clang loop unroll(full)
+                        for (int r = 0; r < 4; ++r)
+                            upb[r] = park[G > 0 ? G - 1 : 0][r * 64 + cp];
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+                    // the group's output rows go to its own park rows: every lane's reads of them (upa) precede
+                    // these writes in program order
+                    lfloat *const s_out = park[G];
+#pragma clang loop unroll(full)
+                    for (int r = 3; r >= 0; --r) {
+                        const int k = 4 * G + r;
+                        // u_backward_function (vertical_advection_dycore.cpp:110-121)
+                        const CT dt = d[k] - c[k] * data;
+                        const CT nd = k == top ? d[k] : dt;
+                        data = nd;
+                        s_out[r * 64 + lane] = float(dtr * (nd - CT(upa[r])));
+                    }
+                    asm volatile("" ::: "memory");
+                    const int L = min(4 * G + g, top), rg = max(L - 4 * G, 0);
+                    const f32x4 o = *(const lf32x4 *)&s_out[rg * 64 + csc];
+                    asm volatile("" ::: "memory");
+                    if (G < NG - 1) {
+                        pin();
+                        bstore4<CPOL_OUT>(r_uts, o_prev, so_prev, 0);
+                    }
+                    o_prev = o;
+                    so_prev = csb + (uint32_t)L * skb;
+#pragma clang loop unroll(full)
+                    for (int r = 0; r < 4; ++r)
+                        upa[r] = upb[r];
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+                pin();
+                bstore4<CPOL_OUT>(r_uts, o_prev, so_prev, 0);
+            }
+        }
+
         // Largest linear element offset of a field's allocation box (lower/upper are the box around the origin).
         inline int64_t alloc_last(const gt355_field *f) {
             return (int64_t)(f->upper[0] - 1) + (int64_t)(f->upper[1] - 1) * f->stride[1] +
@@ -654,9 +931,11 @@ namespace gt355 {
         return check_launch("gt355_vadv");                                                                       \
     }
                 // FP32 storage, 65..80 levels, fewer strips than two rounds of one wavefront per SIMD: the
-                // 16-byte-request form (GT355_TUNE_vadv_wide=0 leaves it; _dg sets the groups of four levels in flight,
-                // 3 to 5).  Measured against the kernels below: 256x256x80 (one round) 22.9 against 23.4 us; with more
-                // rounds the persistent form is faster (1536x1536x80 timestep 1.43 against 1.49 ms)
+                // 16-byte-request form, by default with its rows loaded straight into LDS (GT355_TUNE_vadv_wide=0
+                // leaves the form; vadv_wide_lds=0 takes the register-ring kernel, whose groups of four levels in flight
+                // vadv_wide_dg sets, 3 to 5).  Measured at 256x256x80 (one round), FP64 compute: LDS slots 22.7 us,
+                // register ring 22.9 us, the one-row kernels below 23.4 us; with more rounds the persistent form is
+                // faster (1536x1536x80 timestep 1.43 against 1.49 ms)
                 const bool persistent_shape = (int64_t)((ni + 63) / 64) * nj >= 2 * (int64_t)cu_count() * 4;
                 if constexpr (sizeof(ST) == 4) {
                     if (nk > 64 && !persistent_shape && tune_param("vadv_wide", 1) != 0 &&
@@ -674,6 +953,15 @@ namespace gt355 {
             wc->stride[1], wc->stride[2], strips_i, (int)items, xcd_count());                                    \
         return check_launch("gt355_vadv");                                                                       \
     }
+                        if (tune_param("vadv_wide_lds", 1) != 0) {
+                            // rows loaded straight into LDS: five slots, four groups in flight (write-back instead of
+                            // non-temporal output stores measured 24.8-25.0 against 22.7 us)
+                            vadv_lds_kernel<CT, 80, 5><<<waves, 64, 0, stream>>>((float *)uts->origin,
+                                (const float *)us->origin, (const float *)wc->origin, (const float *)up->origin,
+                                (const float *)ut->origin, dtr_c, ni, nk, sj, sk, wc->stride[1], wc->stride[2], strips_i,
+                                (int)items, xcd_count());
+                            return check_launch("gt355_vadv");
+                        }
                         GT355_VADV_WIDE(3)
                         GT355_VADV_WIDE(5)
                         GT355_VADV_WIDE(4)

@@ -244,17 +244,23 @@ def test_vadv_persistent_strips_depth16(gpu, oracle, monkeypatch, precision, ni,
     assert np.array_equal(got, ref)
 
 
-@pytest.mark.parametrize("dg", [3, 4, 5])
+@pytest.mark.parametrize("form", ["lds", 3, 4, 5])
 @pytest.mark.parametrize("precision", [1, 0])
 @pytest.mark.parametrize("ni,nj,nk", [(4, 3, 65), (60, 7, 80), (64, 5, 66), (100, 9, 77), (256, 256, 80),
                                       (132, 500, 71), (512, 40, 79)])
-def test_vadv_wide_requests(gpu, oracle, monkeypatch, dg, precision, ni, nj, nk):
+def test_vadv_wide_requests(gpu, oracle, monkeypatch, form, precision, ni, nj, nk):
     """The FP32-storage form with 16-byte requests (four rows per load, staged in LDS; four levels per store) for
-    65..80 levels and ni % 4 == 0: bit-identical to the oracle with 3, 4 and 5 groups of four levels in flight, a
-    partial last strip (4, 60, 100, 132: lanes past ni clamp their output chunk onto the last four columns), depths
-    below the bucket (65, 66, 71, 77, 79: rows above the top level), one and several strips per wavefront."""
-    monkeypatch.setenv("GT355_TUNE_vadv_wide_dg", str(dg))
-    print(f"[case] vadv wide dg={dg} prec={precision} {ni}x{nj}x{nk}", flush=True)
+    65..80 levels and ni % 4 == 0, both kernels: rows loaded straight into LDS slots ("lds", the default) and rows
+    through a register ring of 3, 4 and 5 groups of four levels.  Bit-identical to the oracle with a partial last
+    strip (4, 60, 100, 132: lanes past ni clamp their output chunk onto the last four columns), depths below the
+    bucket (65, 66, 71, 77, 79: rows above the top level), one and several strips per wavefront (132x500: 1500
+    strips)."""
+    if form == "lds":
+        monkeypatch.setenv("GT355_TUNE_vadv_wide_lds", "1")
+    else:
+        monkeypatch.setenv("GT355_TUNE_vadv_wide_lds", "0")
+        monkeypatch.setenv("GT355_TUNE_vadv_wide_dg", str(form))
+    print(f"[case] vadv wide form={form} prec={precision} {ni}x{nj}x{nk}", flush=True)
     f, grid = vadv_case(gpu, ni, nj, nk, np.float32)
     ref = oracle_vadv(oracle, f, float(np.float32(0.15)), 3, ni, nj, nk, "native" if precision == 0 else "mixed")
     got = run_vadv(f, grid, 0.15, precision)
