@@ -33,6 +33,9 @@ BYTES_PER_PT = {"hdiff": 3, "vadv": 6}
 SEED = 42
 # launches per per-kernel timing window (hdiff FP64 256x256x80: 250 x 22 us = 5.5 ms)
 KERNEL_WINDOW = 250
+# steps of the long step window: the comparison of the two forms uses it whatever --steps is (a window of 200
+# steps, 8 ms of FP32-form steps, measured 39-42 us per step where 2000 steps give 36.8 us, the sum of its kernels)
+STEP_WINDOW = 2000
 
 
 def make_set(gt, dtype, s):
@@ -75,7 +78,8 @@ def launch_stream():
 
 def timed_steps(gt, sets, precision, steps, warmup):
     """Runs warmup + steps steps; returns (total ms of the timed steps, mean ms per hdiff launch, mean ms per vadv
-    launch).  The step loop is bracketed by one event pair (synchronised on both sides).  Per-kernel times come
+    launch, mean ms per step over a window of at least STEP_WINDOW steps).  The step loop is bracketed by one event
+    pair (synchronised on both sides).  Per-kernel times come
     from separate runs of `steps` back-to-back launches of one kernel, also bracketed by one event pair on the
     launch stream, so that they measure launch-to-launch kernel time without the dispatch gap an event pair
     around every single launch would add."""
@@ -118,7 +122,9 @@ def timed_steps(gt, sets, precision, steps, warmup):
     n = max(steps, KERNEL_WINDOW)
     hd_ms = timed([hdiff], n, warmup + steps) / n
     va_ms = timed([vadv], n, warmup + steps + n) / n
-    return total, hd_ms, va_ms
+    nw = max(steps, STEP_WINDOW)
+    win_ms = timed([hdiff, vadv], nw, warmup + steps + 2 * n) / nw
+    return total, hd_ms, va_ms, win_ms
 
 
 def graphed_steps(gt, sets, precision, nsteps=40, reps=10):
@@ -746,7 +752,7 @@ def main():
                  "fp32_form": autotune(gt, np.float32, gt.WIDEN, shapes)}
     sets64 = [make_set(gt, np.float64, s) for s in range(args.sets)]
     with Range("bench: hdiff + vadv FP64 256x256x80"):
-        total, hd_ms, va_ms = timed_steps(gt, sets64, gt.NATIVE, args.steps, args.warmup)
+        total, hd_ms, va_ms, win64 = timed_steps(gt, sets64, gt.NATIVE, args.steps, args.warmup)
     print(f"[bench] fp64 {args.steps} steps: {total:.3f} ms", file=sys.stderr, flush=True)
     ms_per_step = total / args.steps
     hd64 = kernel_stats("hdiff", hd_ms, 8)
@@ -759,13 +765,16 @@ def main():
     fp32 = None
     if not args.no_fp32:
         sets32 = [make_set(gt, np.float32, s) for s in range(args.sets)]
-        total32, hd_ms32, va_ms32 = timed_steps(gt, sets32, gt.WIDEN, args.steps, args.warmup)
+        total32, hd_ms32, va_ms32, win32 = timed_steps(gt, sets32, gt.WIDEN, args.steps, args.warmup)
         print(f"[bench] fp32 form {args.steps} steps: {total32:.3f} ms", file=sys.stderr, flush=True)
         fp32 = {
             "form": "FP32 storage, FP64 compute (pivots and divisions in FP64)",
             "ms_per_step": round(total32 / args.steps, 5),
             "gpts_per_s": round(P / (total32 / args.steps * 1e-3) / 1e9, 3),
-            "speedup_vs_fp64": round(total / total32, 3),
+            "speedup_vs_fp64_k_steps": round(total / total32, 3),
+            # both forms over the same long window of steps (step_window below)
+            "ms_per_step_long_window": round(win32, 5),
+            "speedup_vs_fp64": round(win64 / win32, 3),
             "hdiff": kernel_stats("hdiff", hd_ms32, 4),
             "vadv": kernel_stats("vadv", va_ms32, 4),
         }
@@ -795,7 +804,7 @@ def main():
     if "compose" in legs:
         print("[bench] compose leg", file=sys.stderr, flush=True)
         with Range("bench: precision-form composition 512x512x80"):
-            compose = compose_leg(gt, max(10, args.steps // 4), min(args.warmup, 10))
+            compose = compose_leg(gt, max(200, args.steps // 4), min(args.warmup, 20))
     batch = None
     if "batch" in legs:
         print("[bench] batch leg", file=sys.stderr, flush=True)
@@ -834,6 +843,8 @@ def main():
         # the whole step: hdiff's 24 B/point and vadv's 48 B/point of algorithmic traffic over the step time
         "step_fp64": {"algo_bytes": 72 * P, "achieved_gbs": round(72 * P / (ms_per_step * 1e-3) / 1e9, 1),
                       "hbm_frac": round(72 * P / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        # the FP64 step again over a window of max(steps, STEP_WINDOW) steps, the FP32 form's comparison basis
+        "step_window": {"steps": max(args.steps, STEP_WINDOW), "fp64_ms_per_step": round(win64, 5)},
         "autotuned_hdiff": tuned,
         "fp32_form": fp32,
         "graphed_steps": in_graph,
