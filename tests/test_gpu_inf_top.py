@@ -61,9 +61,12 @@ def test_vadv_inf_top_fixture(gpu):
     np.testing.assert_array_equal(got, z["expected"][:, H:H + nj, H:H + ni])
 
 
-# (ni, nj, nk, env): onchip buckets, the persistent strips (>= 2 rounds of strips per SIMD) and the deep scratch path
-CASES = [(67, 5, 17, {}), (67, 5, 80, {}), (130, 700, 67, {}), (70, 9, 81, {}), (256, 512, 80,
-                                                                                 {"GT355_TUNE_vadv_persist_d": "16"})]
+# (ni, nj, nk, env): onchip buckets, the persistent strips (>= 2 rounds of strips per SIMD), the deep scratch path
+# and, for FP32 storage with ni % 4 == 0 and one round of strips, the 16-byte-request kernels (LDS slots and the
+# register ring; 128 columns: both strips whole, every row read inside the halo)
+CASES = [(67, 5, 17, {}), (67, 5, 80, {}), (130, 700, 67, {}), (70, 9, 81, {}),
+         (256, 512, 80, {"GT355_TUNE_vadv_persist_d": "16"}), (128, 40, 77, {}),
+         (128, 40, 77, {"GT355_TUNE_vadv_wide_lds": "0"})]
 
 
 @pytest.mark.parametrize("ni,nj,nk,env", CASES)
