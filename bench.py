@@ -711,8 +711,10 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
-    # 100 untimed steps: after 20 the first timed steps still ran 2-3% slower than steady state
-    ap.add_argument("--warmup", type=int, default=100)
+    # 1000 untimed steps (63 ms FP64, 37 ms FP32 form): after 100 the 200 timed FP32-form steps still read 40.3-40.4
+    # us per step where 2000 steps give 36.7 us, and the FP64 step 63.4-63.7 against 62.6-62.8 us; after 1000 the
+    # timed steps read the steady state
+    ap.add_argument("--warmup", type=int, default=1000)
     ap.add_argument("--sets", type=int, default=4)
     ap.add_argument("--legs", default="tune,fp64,fp32,graph,copy,cartesian,generic,compose,batch,deferred,large,cpu",
                     help="comma list of legs: tune (hdiff autotuner first, untimed), fp64 (always run), fp32 "
