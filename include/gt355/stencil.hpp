@@ -1087,13 +1087,10 @@ namespace gt355 {
                 (f(std::integral_constant<int, Ss>()), ...);
             }
 
+            // one column's sweep of a vertical multistage over every level
             template <class Exec, int NSPL, bool W, class StageList, class ViewsTuple, class RingSet, int NS>
-            __global__ void __launch_bounds__(256) vertical_kernel(
-                ViewsTuple v0, RingSet rs0, domains<NS> dom, int i0, int ni, int j0, int nj, ksplit ks) {
-                const int i = i0 + (int)(blockIdx.x * 64 + (threadIdx.x & 63));
-                const int j = j0 + (int)(blockIdx.y * 4 + (threadIdx.x >> 6));
-                if (i >= i0 + ni || j >= j0 + nj)
-                    return;
+            __device__ __forceinline__ void vertical_sweep(
+                ViewsTuple const &v0, RingSet const &rs0, domains<NS> const &dom, int i, int j, ksplit const &ks) {
                 using R = vertical_runner<Exec, NSPL, W, StageList, ViewsTuple>;
                 constexpr bool up = !std::is_same<Exec, backward>::value;
                 const int nk = ks.pos[NSPL - 1];
@@ -1119,6 +1116,41 @@ namespace gt355 {
                         }
                     });
                 }
+            }
+
+            // one vertical multistage: a thread per column of the box [i0, i0 + ni) x [j0, j0 + nj)
+            template <class Exec, int NSPL, bool W, class StageList, class ViewsTuple, class RingSet, int NS>
+            __global__ void __launch_bounds__(256) vertical_kernel(
+                ViewsTuple v0, RingSet rs0, domains<NS> dom, int i0, int ni, int j0, int nj, ksplit ks) {
+                const int i = i0 + (int)(blockIdx.x * 64 + (threadIdx.x & 63));
+                const int j = j0 + (int)(blockIdx.y * 4 + (threadIdx.x >> 6));
+                if (i >= i0 + ni || j >= j0 + nj)
+                    return;
+                vertical_sweep<Exec, NSPL, W, StageList, ViewsTuple, RingSet, NS>(v0, rs0, dom, i, j, ks);
+            }
+
+            // a column box [i0, i1) x [j0, j1)
+            struct cbox {
+                int i0, i1, j0, j1;
+                __device__ bool in(int i, int j) const { return i >= i0 && i < i1 && j >= j0 && j < j1; }
+            };
+            // two consecutive vertical multistages in one launch, as the reference fuses multistages whose data pass
+            // at zero horizontal offset (stencil/gpu/entry_point.hpp:107-124): a thread per column of the union of
+            // their boxes runs the first multistage's sweep where its box holds the column, then the second's.  The
+            // launcher fuses only when every placeholder one of the two writes and the other accesses is accessed at
+            // the column itself, so a column's second sweep depends on nothing but its own first sweep.
+            template <class EA, class SA, class VA, class RA, int NA, class EB, class SB, class VB, class RB, int NB,
+                int NSPL, bool W>
+            __global__ void __launch_bounds__(256) vertical_pair_kernel(VA va, RA ra, domains<NA> da, cbox ba, VB vb,
+                RB rb, domains<NB> db, cbox bb, int i0, int ni, int j0, int nj, ksplit ks) {
+                const int i = i0 + (int)(blockIdx.x * 64 + (threadIdx.x & 63));
+                const int j = j0 + (int)(blockIdx.y * 4 + (threadIdx.x >> 6));
+                if (i >= i0 + ni || j >= j0 + nj)
+                    return;
+                if (ba.in(i, j))
+                    vertical_sweep<EA, NSPL, W, SA, VA, RA, NA>(va, ra, da, i, j, ks);
+                if (bb.in(i, j))
+                    vertical_sweep<EB, NSPL, W, SB, VB, RB, NB>(vb, rb, db, i, j, ks);
             }
 
             // one fused parallel multistage: a workgroup owns a TI x TJ tile of one k-level and runs every stage over the
@@ -1411,6 +1443,104 @@ namespace gt355 {
                 return dim3((unsigned)((ni + 63) / 64), (unsigned)((nj + 3) / 4), (unsigned)nz);
             }
 
+            // the kernel arguments of a vertical multistage: its views, register windows, stage domains and the box
+            // of columns its stages cover
+            template <class Mss, class Written>
+            struct vprep {
+                template <class... Stages>
+                struct of {
+                    using T = mss_traits<Mss>;
+                    using Caches = auto_vcaches<typename T::caches, Stages...>;
+                    using Ctx = vctx<Caches, Stages...>;
+                    using views = vtuple<typename stage_views<Stages, Ctx>::type...>;
+                    using rings = typename to_vtuple<typename ring_list<Written, Caches, tlist<Stages...>>::type>::type;
+                    static constexpr int NS = sizeof...(Stages);
+                    views vt{};
+                    rings rs{};
+                    domains<NS> dom{};
+                    cbox box{};
+                };
+                template <class... Stages, class ArgsTuple>
+                static of<Stages...> make(tlist<Stages...>, std::vector<stage_rt> const &st, int first,
+                    run_ctx const &c, ArgsTuple const &args, run_geom const &g) {
+                    using O = of<Stages...>;
+                    constexpr int NS = O::NS;
+                    const int ni = g.i_size, nj = g.j_size;
+                    O p;
+                    ext_rt u;
+                    for (int n = 0; n < NS; ++n) {
+                        ext_rt const &e = st[first + n].ext;
+                        p.dom.d[n][0] = e.imin, p.dom.d[n][1] = ni + e.imax, p.dom.d[n][2] = e.jmin,
+                        p.dom.d[n][3] = nj + e.jmax;
+                        u.merge(e);
+                    }
+                    assign_all<typename O::Ctx>(p.vt, c, args, tlist<Stages...>(), std::make_integer_sequence<int, NS>());
+                    init_rings(p.rs, c, st, first, NS, p.dom.d);
+                    p.box = cbox{u.imin, ni + u.imax, u.jmin, nj + u.jmax};
+                    return p;
+                }
+            };
+
+            // whether two consecutive vertical multistages (stages [fa, fa + na) and [fb, fb + nb)) may share a
+            // launch: every placeholder that one of them writes and the other accesses is accessed at the column
+            // itself (no horizontal offset) by both
+            inline bool vpair_fusable(std::vector<stage_rt> const &st, int fa, int na, int fb, int nb) {
+                struct use {
+                    bool wa = false, aa = false, wb = false, ab = false, off = false;
+                };
+                std::map<int, use> u;
+                for (int s = fa; s < fb + nb; ++s) {
+                    const bool first = s < fa + na;
+                    for (access_rt const &a : st[s].acc) {
+                        use &x = u[a.key];
+                        (first ? x.aa : x.ab) = true;
+                        if (a.write)
+                            (first ? x.wa : x.wb) = true;
+                        x.off = x.off || a.ext.imin || a.ext.imax || a.ext.jmin || a.ext.jmax;
+                    }
+                }
+                for (auto const &e : u) {
+                    use const &x = e.second;
+                    if (((x.wa && x.ab) || (x.wb && x.aa)) && x.off)
+                        return false;
+                }
+                return true;
+            }
+            // GT355_ENGINE_MSS_FUSION=0 launches every multistage on its own
+            inline bool mss_fusion_on() {
+                static const bool on = [] {
+                    const char *e = std::getenv("GT355_ENGINE_MSS_FUSION");
+                    return !(e && std::atoi(e) == 0);
+                }();
+                return on;
+            }
+
+            template <class MA, class MB, int NSPL, bool W, class Written, class ArgsTuple>
+            void launch_vpair(std::vector<stage_rt> const &st, int fa, int fb, run_ctx const &c, ArgsTuple const &args,
+                run_geom const &g, hipStream_t stream) {
+                auto a = vprep<MA, Written>::make(typename mss_traits<MA>::stages(), st, fa, c, args, g);
+                auto b = vprep<MB, Written>::make(typename mss_traits<MB>::stages(), st, fb, c, args, g);
+                using A = decltype(a);
+                using B = decltype(b);
+                auto empty = [](cbox const &x) { return x.i1 <= x.i0 || x.j1 <= x.j0; };
+                if (empty(a.box))
+                    a.box = cbox{0, 0, 0, 0};
+                if (empty(b.box))
+                    b.box = cbox{0, 0, 0, 0};
+                cbox un = empty(a.box) ? b.box : a.box;
+                if (!empty(b.box))
+                    un = cbox{std::min(un.i0, b.box.i0), std::max(un.i1, b.box.i1), std::min(un.j0, b.box.j0),
+                        std::max(un.j1, b.box.j1)};
+                const int w = un.i1 - un.i0, h = un.j1 - un.j0;
+                if (w > 0 && h > 0 && g.k_size > 0)
+                    vertical_pair_kernel<typename mss_traits<MA>::exec, typename mss_traits<MA>::stages,
+                        typename A::views, typename A::rings, A::NS, typename mss_traits<MB>::exec,
+                        typename mss_traits<MB>::stages, typename B::views, typename B::rings, B::NS, NSPL, W>
+                        <<<grid_2d(w, h, 1), 256, 0, stream>>>(a.vt, a.rs, a.dom, a.box, b.vt, b.rs, b.dom, b.box,
+                            un.i0, w, un.j0, h, g.ks);
+                check_launch("stencil engine: fused vertical multistages launch");
+            }
+
             template <class Mss, int NSPL, bool W, class Written, class... Stages, class ArgsTuple>
             void launch_mss(tlist<Stages...>, std::vector<stage_rt> const &st, int first, int m, run_ctx const &c,
                 ArgsTuple const &args, run_geom const &g, hipStream_t stream) {
@@ -1466,29 +1596,16 @@ namespace gt355 {
                     };
                     (one(Stages()), ...);
                 } else {
-                    using Caches = auto_vcaches<typename T::caches, Stages...>;
-                    using Ctx = vctx<Caches, Stages...>;
-                    using ViewsTuple = vtuple<typename stage_views<Stages, Ctx>::type...>;
-                    using RingSet =
-                        typename to_vtuple<typename ring_list<Written, Caches, tlist<Stages...>>::type>::type;
-                    ViewsTuple vt{};
-                    RingSet rs{};
-                    domains<NS> dom{};
-                    ext_rt u;
-                    for (int n = 0; n < NS; ++n) {
-                        ext_rt const &e = st[first + n].ext;
-                        dom.d[n][0] = e.imin, dom.d[n][1] = ni + e.imax, dom.d[n][2] = e.jmin, dom.d[n][3] = nj + e.jmax;
-                        u.merge(e);
-                    }
-                    assign_all<Ctx>(vt, c, args, tlist<Stages...>(), std::make_integer_sequence<int, NS>());
-                    init_rings(rs, c, st, first, NS, dom.d);
-                    const int w = ni - u.imin + u.imax, h = nj - u.jmin + u.jmax;
+                    auto p = vprep<Mss, Written>::make(tlist<Stages...>(), st, first, c, args, g);
+                    using P = decltype(p);
+                    const int w = p.box.i1 - p.box.i0, h = p.box.j1 - p.box.j0;
                     if (w > 0 && h > 0 && nk > 0)
-                        vertical_kernel<Exec, NSPL, W, tlist<Stages...>, ViewsTuple, RingSet, NS>
-                            <<<grid_2d(w, h, 1), 256, 0, stream>>>(vt, rs, dom, u.imin, w, u.jmin, h, g.ks);
+                        vertical_kernel<Exec, NSPL, W, tlist<Stages...>, typename P::views, typename P::rings, NS>
+                            <<<grid_2d(w, h, 1), 256, 0, stream>>>(p.vt, p.rs, p.dom, p.box.i0, w, p.box.j0, h, g.ks);
                     check_launch("stencil engine: vertical multistage launch");
                 }
             }
+
 
             // a register window's memory and column boxes: it fills where its multistage reads the placeholder and
             // flushes where it writes it
@@ -1857,17 +1974,53 @@ namespace gt355 {
                 return box;
             }
 
+            template <int I, class... Ts>
+            struct type_at;
+            template <class T0, class... Ts>
+            struct type_at<0, T0, Ts...> {
+                using type = T0;
+            };
+            template <int I, class T0, class... Ts>
+            struct type_at<I, T0, Ts...> : type_at<I - 1, Ts...> {};
+            // a multistage that launch_mss runs as vertical_kernel (a sweep per column, no tiles)
+            template <class Mss>
+            constexpr bool plain_vertical() {
+                using T = mss_traits<Mss>;
+                return !T::fused && !T::vsync && !std::is_same<typename T::exec, parallel>::value;
+            }
+            // multistages I.. of the spec (first[m]: index of multistage m's first stage): two consecutive plain
+            // vertical multistages that pass data only at the column itself share one launch (vertical_pair_kernel);
+            // every other multistage launches on its own
+            template <int NSPL, bool W, class Written, int I, class... Msses, class ArgsTuple>
+            void launch_from(tlist<Msses...> l, const int *first, std::vector<stage_rt> const &st, run_ctx const &c,
+                ArgsTuple const &args, run_geom const &g, hipStream_t stream) {
+                constexpr int N = sizeof...(Msses);
+                if constexpr (I < N) {
+                    using MA = typename type_at<I, Msses...>::type;
+                    if constexpr (I + 1 < N) {
+                        using MB = typename type_at<I + 1, Msses...>::type;
+                        if constexpr (plain_vertical<MA>() && plain_vertical<MB>()) {
+                            if (mss_fusion_on() &&
+                                vpair_fusable(st, first[I], mss_traits<MA>::n, first[I + 1], mss_traits<MB>::n)) {
+                                launch_vpair<MA, MB, NSPL, W, Written>(st, first[I], first[I + 1], c, args, g, stream);
+                                launch_from<NSPL, W, Written, I + 2>(l, first, st, c, args, g, stream);
+                                return;
+                            }
+                        }
+                    }
+                    launch_mss<MA, NSPL, W, Written>(
+                        typename mss_traits<MA>::stages(), st, first[I], I, c, args, g, stream);
+                    launch_from<NSPL, W, Written, I + 1>(l, first, st, c, args, g, stream);
+                }
+            }
             template <int NSPL, bool W, class... Msses, class ArgsTuple>
-            void launch_all(tlist<Msses...>, std::vector<stage_rt> const &st, run_ctx const &c, ArgsTuple const &args,
+            void launch_all(tlist<Msses...> l, std::vector<stage_rt> const &st, run_ctx const &c, ArgsTuple const &args,
                 run_geom const &g, hipStream_t stream) {
-                int first = 0, m = 0;
-                auto one = [&](auto mss) {
-                    using T = mss_traits<decltype(mss)>;
-                    using Written = typename spec_written<tlist<>, tlist<Msses...>>::type;
-                    launch_mss<decltype(mss), NSPL, W, Written>(typename T::stages(), st, first, m++, c, args, g, stream);
-                    first += T::n;
-                };
-                (one(Msses()), ...);
+                using Written = typename spec_written<tlist<>, tlist<Msses...>>::type;
+                int first[sizeof...(Msses) + 1];
+                int n = 0, q = 0;
+                ((first[q++] = n, n += mss_traits<Msses>::n), ...);
+                launch_from<NSPL, W, Written, 0>(l, first, st, c, args, g, stream);
             }
             // temporaries come from the library's pool (gt355_tmp_acquire: the cached allocator of
             // sid/allocator.hpp:44-57, made safe across streams) and go back to it after the run's launches are
