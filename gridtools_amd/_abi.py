@@ -71,6 +71,7 @@ SIGNATURES = {
     "gt355_finalize": [],
     "gt355_tmp_acquire": [ctypes.c_size_t, _P, ctypes.POINTER(_P)],
     "gt355_tmp_release": [_P, _P],
+    "gt355_tmp_release_n": [_P, _I, _P],
     "gt355_stream_set_current": [_P],
     "gt355_stream_current": [ctypes.POINTER(_P)],
     "gt355_debug_sync": [_I],

@@ -106,6 +106,10 @@ int gt355_malloc(size_t bytes, void **ptr);   /* storage_allocate (storage/gpu.h
  * are reused on that stream only.  Blocks are freed by gt355_finalize. */
 int gt355_tmp_acquire(size_t bytes, void *stream, void **ptr);
 int gt355_tmp_release(void *ptr, void *stream);
+/* Releases n blocks (null entries skipped) with one event for all of them, as the reference's run drops every
+ * temporary of a spec together once its kernels are queued (stencil/frontend/run.hpp:34-60 with the allocator of
+ * sid/allocator.hpp:44-57).  Every pointer is checked first: on an error no block is released. */
+int gt355_tmp_release_n(void *const *ptrs, int n, void *stream);
 int gt355_free(void *ptr);
 int gt355_memcpy_h2d(void *dst, const void *src, size_t bytes); /* storage_update_target (gpu.hpp:83-90) */
 int gt355_memcpy_d2h(void *dst, const void *src, size_t bytes); /* storage_update_host (gpu.hpp:92-99) */

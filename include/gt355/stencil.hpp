@@ -2646,8 +2646,9 @@ namespace gt355 {
             std::vector<void *> taken;
             engine_::alloc_all(msses(), c, box, g, stream, taken);
             engine_::launch_all<N + 1, W>(msses(), st, c, args_t, g, (hipStream_t)stream);
-            for (void *t : taken)
-                check(gt355_tmp_release(t, stream), "stencil engine: temporary");
+            // every temporary of the run back to the pool with one event (one record per run, not per temporary)
+            if (!taken.empty())
+                check(gt355_tmp_release_n(taken.data(), (int)taken.size(), stream), "stencil engine: temporaries");
         }
 
         // run_single_stage(functor, backend, grid, fields...) (stencil/frontend/run.hpp:252-258)

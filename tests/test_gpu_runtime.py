@@ -46,6 +46,36 @@ def test_tmp_pool_reuse_rules(gpu):
     _abi.call("gt355_synchronize")
 
 
+def test_tmp_pool_release_n(gpu):
+    """gt355_tmp_release_n: several blocks back to the pool with one event; a list holding an unknown or repeated
+    pointer releases nothing; null entries are skipped."""
+    from gridtools_amd import _abi
+    from gridtools_amd.timing import Stream
+
+    a, b = Stream(), Stream()
+    nb = (3 << 20) + 12288  # a size no other test asks for
+    p, q = _acq(nb, a.handle), _acq(nb, a.handle)
+    assert p and q and p != q
+    arr = (ctypes.c_void_p * 3)(p, None, q)
+    # an unknown pointer in the list: an error, and p and q stay in use
+    bad = (ctypes.c_void_p * 2)(p, 0x1234)
+    assert _abi.lib().gt355_tmp_release_n(bad, 2, a.handle) != 0
+    assert "not acquired" in _abi.last_error()
+    twice = (ctypes.c_void_p * 2)(p, p)
+    assert _abi.lib().gt355_tmp_release_n(twice, 2, a.handle) != 0
+    r = _acq(nb, a.handle)  # p and q still in use: a third block
+    assert r not in (p, q)
+    _abi.call("gt355_tmp_release_n", arr, 3, a.handle)
+    # both back: the same stream takes them at once, another only after the release's work completed
+    a.synchronize()
+    got = {_acq(nb, b.handle), _acq(nb, b.handle)}
+    assert got == {p, q}
+    arr2 = (ctypes.c_void_p * 3)(p, q, r)
+    _abi.call("gt355_tmp_release_n", arr2, 3, b.handle)
+    _abi.call("gt355_tmp_release_n", arr2, 0, b.handle)  # an empty list
+    _abi.call("gt355_synchronize")
+
+
 def test_deep_scratch_growth_refused_during_capture(gpu, oracle):
     """The nk > 80 vadv keeps ccol/dcol in a per-stream scratch buffer.  Growing it inside a capture is refused with
     a status (a graph must not reference an allocation made for it); after a run of the shape outside capture, the
