@@ -938,7 +938,8 @@ namespace gt355 {
                 // faster (1536x1536x80 timestep 1.43 against 1.49 ms)
                 const bool persistent_shape = (int64_t)((ni + 63) / 64) * nj >= 2 * (int64_t)cu_count() * 4;
                 if constexpr (sizeof(ST) == 4) {
-                    if (nk > 64 && !persistent_shape && tune_param("vadv_wide", 1) != 0 &&
+                    const int wide_mode = tune_param("vadv_wide", 1); // 2: the wide form for any number of rounds
+                    if (nk > 64 && (!persistent_shape || wide_mode == 2) && wide_mode != 0 &&
                         wide_fits(uts, us, wc, up, ut, ni, nj, nk)) {
                         const int strips_i = (ni + 63) / 64;
                         const int64_t items = (int64_t)strips_i * nj;
