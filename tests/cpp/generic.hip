@@ -585,6 +585,28 @@ namespace {
         };
     } // namespace hdep
 
+    // the rule that lets two consecutive vertical multistages share a launch, on hand-made access lists
+    void mss_fusion_rule() {
+        using engine_::access_rt;
+        using engine_::ext_rt;
+        using engine_::stage_rt;
+        const ext_rt at0{0, 0, 0, 0, 0, 0, true}, around{-1, 1, -1, 1, 0, 0, true}, below{0, 0, 0, 0, -1, 0, true};
+        auto st = [](std::vector<access_rt> a, std::vector<access_rt> b) {
+            return std::vector<stage_rt>{stage_rt{0, true, std::move(a), {}}, stage_rt{1, true, std::move(b), {}}};
+        };
+        auto fusable = [](std::vector<stage_rt> const &v) { return engine_::vpair_fusable(v, 0, 1, 1, 1); };
+        // key 1 written by the first, read by the second at the column (and at k - 1): fused
+        const bool col = fusable(st({{1, true, at0, 8}, {2, false, around, 8}}, {{1, false, below, 8}, {3, true, at0, 8}}));
+        // the second reads key 1 at +-1: not fused
+        const bool off = fusable(st({{1, true, at0, 8}}, {{1, false, around, 8}, {3, true, at0, 8}}));
+        // the first reads key 3 at +-1, which the second writes: not fused
+        const bool war = fusable(st({{3, false, around, 8}, {1, true, at0, 8}}, {{3, true, at0, 8}}));
+        // a field both only read, at offsets: fused
+        const bool ro = fusable(st({{2, false, around, 8}, {1, true, at0, 8}}, {{2, false, around, 8}, {3, true, at0, 8}}));
+        report("multistage fusion rule (column data fused; offsets on passed data, either way, refused)",
+            col && !off && !war && ro);
+    }
+
     void horizontal_dependencies(int ni, int nj, int nk) {
         const int h = 2, n0 = ni + 2 * h, n1 = nj + 2 * h;
         auto g = halo_grid(ni, nj, nk, h);
@@ -634,6 +656,34 @@ namespace {
             }
             std::snprintf(label, sizeof label,
                 "vertical multistage, forward: temporary read at +-1 in i and j by the next stage %dx%dx%d", ni, nj, nk);
+            report(label, same("out", out, e));
+        }
+        // the same recurrence and spread in two vertical multistages: the second reads the first's temporary at +-1
+        // in i and j, so the two must not share a launch (engine_::vpair_fusable refuses the pair)
+        {
+            auto out = b.value(-7.).build();
+            run([](auto out, auto in) {
+                GT_DECLARE_TMP(double, tmp);
+                return multi_pass(execute_forward().stage(hdep::accumulate(), tmp, in),
+                    execute_backward().stage(hdep::spread(), out, tmp));
+            },
+                gfx950<>(), g, out, in);
+            std::vector<double> tmp(vin.size(), 0.0), e(vin.size(), -7.);
+            for (int k = 0; k < nk; ++k) {
+                for (int j = -1; j < nj + 1; ++j)
+                    for (int i = -1; i < ni + 1; ++i)
+                        tmp[at(i, j, k)] = k == 0 ? vin[at(i, j, k)] : vin[at(i, j, k)] + 0.5 * tmp[at(i, j, k - 1)];
+            }
+            for (int k = 0; k < nk; ++k)
+                for (int j = 0; j < nj; ++j)
+                    for (int i = 0; i < ni; ++i) {
+                        const double m = 0.25 * (tmp[at(i - 1, j, k)] + tmp[at(i + 1, j, k)] + tmp[at(i, j - 1, k)] +
+                                                    tmp[at(i, j + 1, k)]);
+                        e[at(i, j, k)] = k == 0 ? m : m - 0.5 * tmp[at(i + 1, j, k - 1)];
+                    }
+            std::snprintf(label, sizeof label,
+                "two vertical multistages, the second reading the first's temporary at +-1 (not fused) %dx%dx%d", ni, nj,
+                nk);
             report(label, same("out", out, e));
         }
         // backward: a user field written on the extended box (its halo) and read at offsets and at k + 1
@@ -1470,6 +1520,7 @@ int main(int argc, char **argv) {
         expandable_and_errors();
         kcache_tests();
         kparallel_tests();
+        mss_fusion_rule();
         horizontal_dependencies(70, 37, 19);
         horizontal_dependencies(130, 70, 24);
         whole_axis_access(12, 33, 61);
