@@ -29,6 +29,24 @@ namespace gt355 {
         struct word16 {
             uint64_t lo, hi;
         };
+        // Every element is read once and written once: both streams are non-temporal, so that neither the L2 nor the
+        // Infinity Cache keeps lines at the expense of others.
+        template <class W>
+        __device__ __forceinline__ W ld_nt(const W *p) {
+            if constexpr (std::is_same<W, word16>::value)
+                return word16{__builtin_nontemporal_load(&p->lo), __builtin_nontemporal_load(&p->hi)};
+            else
+                return __builtin_nontemporal_load(p);
+        }
+        template <class W>
+        __device__ __forceinline__ void st_nt(W v, W *p) {
+            if constexpr (std::is_same<W, word16>::value) {
+                __builtin_nontemporal_store(v.lo, &p->lo);
+                __builtin_nontemporal_store(v.hi, &p->hi);
+            } else {
+                __builtin_nontemporal_store(v, p);
+            }
+        }
 
         // offsets of outer index o over every dimension except a and b (a == b: all but a), first dimension fastest
         template <class IT>
@@ -68,7 +86,7 @@ namespace gt355 {
                     const int aa = e % TA, bb = e / TA;
                     const int64_t ia = a0 + aa, ib = b0 + bb;
                     if (ia < na && ib < nb)
-                        tile[bb][aa] = src[so + ia * ssa + ib * ssb];
+                        tile[bb][aa] = ld_nt(src + so + ia * ssa + ib * ssb);
                 }
                 __syncthreads();
 #pragma unroll
@@ -77,7 +95,7 @@ namespace gt355 {
                     const int bb = e % TB, aa = e / TB;
                     const int64_t ia = a0 + aa, ib = b0 + bb;
                     if (ia < na && ib < nb)
-                        dst[dof + ia * dsa + ib * dsb] = tile[bb][aa];
+                        st_nt(tile[bb][aa], dst + dof + ia * dsa + ib * dsb);
                 }
                 __syncthreads();
             }
@@ -91,7 +109,7 @@ namespace gt355 {
                 int64_t so, dof;
                 outer_offsets<IT>(d, (IT)row, so, dof);
                 for (int64_t ia = blockIdx.x * 256 + threadIdx.x; ia < na; ia += (int64_t)gridDim.x * 256)
-                    dst[dof + ia * dsa] = src[so + ia * ssa];
+                    st_nt(ld_nt(src + so + ia * ssa), dst + dof + ia * dsa);
             }
         }
 
