@@ -8,14 +8,17 @@
 //  * a != b (a transpose in the (a, b) plane): LDS tiles of 64 (a) x TB (b) elements, TB = 64, 32 or 16 to fit the b
 //    extent, read along a and written along b, so both HBM streams are coalesced; one workgroup walks tiles in a
 //    grid-stride loop;
+//  * with elements of 4 or 8 bytes, unit strides along a (source) and b (destination) and aligned vectors, the
+//    tiles move 16 bytes per access on both sides (lt_tile_vec_kernel);
 //  * a == b: a flat copy along a, 256 elements per workgroup row, the outer index decomposed once per row.
-// Elements are moved as raw words of 1, 2, 4, 8 or 16 bytes.
+// Elements are moved as raw words of 1, 2, 4, 8 or 16 bytes, with non-temporal loads and stores.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <type_traits>
 
 #include "common.hpp"
+#include "tuning.hpp"
 
 namespace gt355 {
     namespace {
@@ -101,6 +104,66 @@ namespace gt355 {
             }
         }
 
+        // The same transpose with 16-byte accesses on both sides (V elements of 4 or 8 bytes): the source is
+        // contiguous along a and the destination along b, and every vector is 16-byte aligned (both checked by the
+        // launcher).  Loads: TA / V lanes per tile row, consecutive along a; stores: TB / V lanes per tile column,
+        // consecutive along b.  A tile that reaches past the domain edge moves element by element.
+        template <class W, int V, int TA, int TB, class IT>
+        __global__ void __launch_bounds__(256) lt_tile_vec_kernel(W *__restrict__ dst, const W *__restrict__ src,
+            lt_desc d, int64_t tiles_a, int64_t tiles_b, int64_t ntiles) {
+            typedef W vec __attribute__((ext_vector_type(V)));
+            constexpr int LPR = TA / V, LROWS = 256 / LPR, SPC = TB / V, SCOLS = 256 / SPC;
+            static_assert(V * sizeof(W) == 16 && TA % V == 0 && TB % V == 0 && 256 % LPR == 0 && 256 % SPC == 0 &&
+                              TB % LROWS == 0 && TA % SCOLS == 0,
+                "tile shape");
+            __shared__ W tile[TB][TA + 1];
+            const int64_t na = d.dims[d.a], nb = d.dims[d.b];
+            const int64_t ssb = d.ss[d.b], dsa = d.ds[d.a];
+            const int la = (threadIdx.x % LPR) * V, lr = threadIdx.x / LPR;
+            const int sb = (threadIdx.x % SPC) * V, sc = threadIdx.x / SPC;
+            for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+                const int64_t ta = t % tiles_a, r = t / tiles_a;
+                const int64_t tb = r % tiles_b, o = r / tiles_b;
+                int64_t so, dof;
+                outer_offsets<IT>(d, (IT)o, so, dof);
+                const int64_t a0 = ta * TA, b0 = tb * TB;
+                const bool full = a0 + TA <= na && b0 + TB <= nb;
+#pragma unroll
+                for (int rr = lr; rr < TB; rr += LROWS) {
+                    const int64_t ia = a0 + la, ib = b0 + rr;
+                    if (full) {
+                        const vec v = ld_nt((const vec *)(src + so + ia + ib * ssb));
+#pragma unroll
+                        for (int q = 0; q < V; ++q)
+                            tile[rr][la + q] = v[q];
+                    } else {
+#pragma unroll
+                        for (int q = 0; q < V; ++q)
+                            if (ia + q < na && ib < nb)
+                                tile[rr][la + q] = ld_nt(src + so + ia + q + ib * ssb);
+                    }
+                }
+                __syncthreads();
+#pragma unroll
+                for (int cc = sc; cc < TA; cc += SCOLS) {
+                    const int64_t ia = a0 + cc, ib = b0 + sb;
+                    if (full) {
+                        vec v;
+#pragma unroll
+                        for (int q = 0; q < V; ++q)
+                            v[q] = tile[sb + q][cc];
+                        st_nt(v, (vec *)(dst + dof + ia * dsa + ib));
+                    } else {
+#pragma unroll
+                        for (int q = 0; q < V; ++q)
+                            if (ia < na && ib + q < nb)
+                                st_nt(tile[sb + q][cc], dst + dof + ia * dsa + ib + q);
+                    }
+                }
+                __syncthreads();
+            }
+        }
+
         template <class W, class IT>
         __global__ void __launch_bounds__(256) lt_flat_kernel(W *__restrict__ dst, const W *__restrict__ src, lt_desc d,
             int64_t outer) {
@@ -126,11 +189,33 @@ namespace gt355 {
                 const int64_t nb = d.dims[d.b];
                 auto waste_ok = [&](int64_t tb) { return ((nb + tb - 1) / tb * tb - nb) * 8 <= nb; };
                 const int tbsel = sizeof(W) > 8 ? 32 : waste_ok(64) ? 64 : waste_ok(32) ? 32 : 16;
+                // 16-byte accesses: 4- and 8-byte elements, unit stride along a (source) and b (destination), and
+                // every vector aligned: the bases and all strides but those two in multiples of 16 bytes
+                bool vec_ok = (sizeof(W) == 4 || sizeof(W) == 8) && d.ss[d.a] == 1 && d.ds[d.b] == 1 &&
+                              (uintptr_t)src % 16 == 0 && (uintptr_t)dst % 16 == 0 && tune_param("layout_vec", 1) != 0;
+                for (int q = 0; q < d.nd && vec_ok; ++q) {
+                    if (q != d.a)
+                        vec_ok = (d.ss[q] * (int64_t)sizeof(W)) % 16 == 0;
+                    if (q != d.b && vec_ok)
+                        vec_ok = (d.ds[q] * (int64_t)sizeof(W)) % 16 == 0;
+                }
                 auto go = [&](auto ta_c, auto tb_c) {
                     constexpr int TA = decltype(ta_c)::value, TB = decltype(tb_c)::value;
                     const int64_t tiles_a = (d.dims[d.a] + TA - 1) / TA, tiles_b = (nb + TB - 1) / TB;
                     const int64_t ntiles = tiles_a * tiles_b * outer;
                     const int blocks = (int)std::min<int64_t>(ntiles, 65536);
+                    if constexpr (sizeof(W) == 4 || sizeof(W) == 8) {
+                        constexpr int V = 16 / sizeof(W);
+                        if (vec_ok) {
+                            if (narrow)
+                                lt_tile_vec_kernel<W, V, TA, TB, uint32_t><<<blocks, 256, 0, s>>>((W *)dst,
+                                    (const W *)src, d, tiles_a, tiles_b, ntiles);
+                            else
+                                lt_tile_vec_kernel<W, V, TA, TB, uint64_t><<<blocks, 256, 0, s>>>((W *)dst,
+                                    (const W *)src, d, tiles_a, tiles_b, ntiles);
+                            return;
+                        }
+                    }
                     if (narrow)
                         lt_tile_kernel<W, TA, TB, uint32_t><<<blocks, 256, 0, s>>>((W *)dst, (const W *)src, d, tiles_a,
                             tiles_b, ntiles);

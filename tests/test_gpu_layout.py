@@ -2,7 +2,8 @@
 reference's known answers (tests/unit_tests/layout_transformation/test_layout_transformation.cpp:52-120,
 tests/regression/layout_transformation.cpp:30-40): reversed layouts in 2-5 dimensions (the LDS-tiled transpose
 path), copies whose fastest dimension agrees (the flat path), a 1-D copy into stride 2 that leaves the gaps
-untouched, every element size, and a storage <-> dense exchange with halo and padding at 256x256x80."""
+untouched, every element size, a storage <-> dense exchange with halo and padding at 256x256x80, and the 16-byte
+access kernel for 4- and 8-byte elements with partial tiles."""
 import ctypes
 import numpy as np
 import pytest
@@ -57,6 +58,25 @@ def test_reverse_layout(gpu, oracle, shape):
     got, want = run_both(oracle, dst0, src, shape, ds, ss)
     assert np.array_equal(got, want)
     assert np.array_equal(got, src.transpose(*range(n - 1, -1, -1)))
+
+
+@pytest.mark.parametrize("vec", ["1", "0"])
+@pytest.mark.parametrize("dtype", [np.float32, np.int32, np.float64, np.int64])
+def test_reverse_layout_16_byte_accesses(gpu, oracle, monkeypatch, dtype, vec):
+    """A reversed 132 x 4 x 264 layout: unit stride along the source's and the destination's fastest dimensions and
+    every other stride a multiple of 16 bytes, so the transpose moves 16 bytes per access (vec "1"; "0" forces one
+    element per access).  Partial tiles along both: 264 = 4 x 64 + 8 along a, 132 = 8 x 16 + 4 along b."""
+    monkeypatch.setenv("GT355_TUNE_layout_vec", vec)
+    shape = (132, 4, 264)
+    print(f"[case] transform_layout 16-byte accesses {np.dtype(dtype).name} vec={vec} {shape}", flush=True)
+    n = len(shape)
+    src = (np.arange(int(np.prod(shape))) % 1000003).astype(dtype).reshape(shape)
+    dst0 = np.full(shape[::-1], -1, dtype=dtype)
+    ss = [int(np.prod(shape[d + 1:])) for d in range(n)]
+    ds = [int(np.prod(shape[:d])) for d in range(n)]
+    got, want = run_both(oracle, dst0, src, shape, ds, ss)
+    assert np.array_equal(got, want)
+    assert np.array_equal(got, src.transpose(2, 1, 0))
 
 
 def test_1d_into_stride_2(gpu, oracle):
