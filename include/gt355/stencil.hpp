@@ -42,8 +42,11 @@
 
 // Load-ahead depth of the register windows of vertical multistages: a filled window takes the level it uncovers
 // from a load issued this many levels earlier, and the k sweep runs in unrolled blocks of this many levels.
+// Measured with consecutive vertical multistages sharing a launch (256x256x80 FP64, the vadv and tridiagonal specs
+// on the engine's kernels): 4 -> 88.8 / 87.6 us, 8 -> 82.5 / 72.7 us, 12 -> 86.3 / 72.5 us, 16 -> 88.9 / 74.5 us,
+// 32 -> 99.7 / 83.7 us.
 #ifndef GT355_ENGINE_PREFETCH
-#define GT355_ENGINE_PREFETCH 16
+#define GT355_ENGINE_PREFETCH 8
 #endif
 // Non-temporal loads for the register windows of fields no multistage of the spec writes, read at one column: each
 // value is read once, so it need not stay in the caches (0: every load with the default policy).
